@@ -1,0 +1,364 @@
+/*
+ * rainmd.h - the C ABI of the MI355X (gfx950) molecular-dynamics step.
+ *
+ * This is the drop-in boundary for RainMD's GPU-resident MD step. Every entry
+ * point below replaces one method (or one host-side helper) that the
+ * reference's JavaScript surface exposes; the citation next to each names the
+ * reference function it stands in for. A Node.js N-API addon, a Python ctypes
+ * binding or a C program can call it: the signatures hold plain pointers,
+ * sizes and PODs only.
+ *
+ * Conventions
+ *   - Units are the reference's: nm, ps, kJ/mol, amu, elementary charge, K.
+ *   - Every function returns an int status (RMD_OK or a negative RMD_E_*).
+ *     A failing call leaves a message retrievable with rmd_last_error() (per
+ *     context) or rmd_last_global_error() (for calls without a context).
+ *   - Input arrays are copied; output arrays are caller-allocated.
+ *   - One context is bound to one HIP device and one stream and is not
+ *     re-entrant. Two contexts may live on one device.
+ */
+#ifndef RAINMD_H
+#define RAINMD_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define RMD_ABI_VERSION 1
+
+/* ---- status codes: each maps to one of the reference's error classes ---- */
+enum {
+  RMD_OK = 0,
+  RMD_E_INVALID = -1,          /* plain Error: bad argument / validation     */
+  RMD_E_PARSE = -2,            /* parsePrmtop / parseInpcrd refusal           */
+  RMD_E_SYSTEM = -3,           /* buildSystem refusal (cutoff > half box ...) */
+  RMD_E_DEVICE = -4,           /* HIP error, or no device                     */
+  RMD_E_NONFINITE = -5,        /* NonFiniteStateError (shared/finite.js:63)   */
+  RMD_E_NEIGHBOUR_OVERFLOW = -6,/* NeighbourOverflowError (neighbours.js:172) */
+  RMD_E_STOPPED = -7,          /* SimulationStopped (webgpu/runtime/yield.js:44) */
+  RMD_E_BUDGET = -8,           /* GpuBudgetError (webgpu/runtime/allocator.js:141) */
+  RMD_E_CONSTRAINT = -9        /* ConstraintFailure (cpu/integrate/langevin.js:164) */
+};
+
+/* ======================================================================== */
+/* Host side: topology, coordinates and the flattened system                */
+/* ======================================================================== */
+
+typedef struct rmd_topology rmd_topology;
+typedef struct rmd_coordinates rmd_coordinates;
+typedef struct rmd_system rmd_system;
+
+/* Constraint modes of buildSystem (shared/topology/system.js:40-50). */
+enum { RMD_CONSTRAINTS_NONE = 0, RMD_CONSTRAINTS_HBONDS = 1, RMD_CONSTRAINTS_ALLBONDS = 2 };
+
+/* Replaces parsePrmtop (shared/amber/prmtop.js:111-385): text of a parm7 file. */
+int rmd_topology_parse_prmtop(const char *text, size_t length, rmd_topology **out);
+void rmd_topology_free(rmd_topology *topology);
+
+/* Replaces parseInpcrd (shared/amber/inpcrd.js:34-132). expected_atom_count
+ * <= 0 skips the NATOM cross-check. */
+int rmd_coordinates_parse_inpcrd(const char *text, size_t length, int32_t expected_atom_count,
+                                 rmd_coordinates **out);
+void rmd_coordinates_free(rmd_coordinates *coordinates);
+
+typedef struct rmd_coordinates_view {
+  int32_t atom_count;
+  const double *positions;   /* 3N, nm */
+  const double *velocities;  /* 3N nm/ps, or NULL */
+  int32_t has_box;
+  double box_lengths[3];     /* nm */
+  double box_angles[3];      /* degrees */
+  double time;               /* ps, NaN when absent */
+} rmd_coordinates_view;
+int rmd_coordinates_view_get(const rmd_coordinates *coordinates, rmd_coordinates_view *view);
+
+/* Replaces boxVectorsFromLengthsAndAngles (shared/amber/inpcrd.js:154-196). */
+int rmd_box_vectors_from_lengths_and_angles(const double lengths[3], const double angles_deg[3],
+                                            double box_out[9]);
+
+typedef struct rmd_build_options {
+  int32_t constraints;       /* RMD_CONSTRAINTS_*; HBonds by default        */
+  int32_t rigid_water;       /* 1 by default                                */
+  double cutoff;             /* nm; 1.0 by default; 0 = non-periodic use    */
+  int32_t has_box_vectors;   /* 1: box_vectors overrides the box below      */
+  double box_vectors[9];     /* reduced, row-major                          */
+  int32_t has_box;           /* 1: lengths/angles (an inpcrd's box)         */
+  double box_lengths[3];
+  double box_angles[3];
+} rmd_build_options;
+void rmd_build_options_default(rmd_build_options *options);
+
+/* Replaces buildSystem (shared/topology/system.js:40-236). The positions of
+ * the coordinates (may be NULL) are carried along so a system is
+ * self-describing. */
+int rmd_system_build(const rmd_topology *topology, const rmd_coordinates *coordinates,
+                     const rmd_build_options *options, rmd_system **out);
+void rmd_system_free(rmd_system *system);
+
+/* The seeded solvated-peptide builder used by the synthetic workloads: the
+ * ACE-ALA-NME solute of ala-wat.prmtop at the box centre and `water_count`
+ * TIP3P waters (the same file's WAT residue) on a jittered cubic lattice with
+ * uniformly random orientations. No counterpart in the reference (its systems
+ * come from tleap); it reuses the reference's own force-field file. */
+typedef struct rmd_solvated_options {
+  int32_t water_count;
+  double box_edge;           /* nm, cubic                                     */
+  uint32_t seed;
+  int32_t include_solute;    /* 0: pure water box                             */
+  double solute_clearance;   /* nm, lattice sites closer to the solute drop   */
+  double jitter;             /* nm, uniform +-jitter per axis on the oxygen   */
+  double cutoff;             /* nm                                            */
+  int32_t constraints;       /* RMD_CONSTRAINTS_*                              */
+  int32_t rigid_water;
+} rmd_solvated_options;
+void rmd_solvated_options_default(rmd_solvated_options *options);
+int rmd_system_build_solvated(const char *template_prmtop, size_t template_length,
+                              const char *solute_inpcrd, size_t solute_length,
+                              const rmd_solvated_options *options, rmd_system **out);
+
+/* A flat, read-only view of a built system: every array the kernels and the
+ * oracle read. Pointers stay valid until rmd_system_free. */
+typedef struct rmd_system_view {
+  int32_t atom_count, type_count, molecule_count, residue_count;
+  const double *charges, *masses;              /* N                           */
+  const int32_t *atom_types, *atomic_numbers;  /* N                           */
+  const int32_t *residue_of_atom;              /* N                           */
+  const double *lj_a, *lj_b;                   /* T*T, kJ/mol nm^12 / nm^6    */
+  int32_t bond_count;                          /* harmonic bonds (unconstrained) */
+  const int32_t *bond_a, *bond_b;
+  const double *bond_k, *bond_r0;
+  int32_t angle_count;
+  const int32_t *angle_a, *angle_b, *angle_c;
+  const double *angle_k, *angle_theta0;
+  int32_t torsion_count;
+  const int32_t *torsion_a, *torsion_b, *torsion_c, *torsion_d;
+  const double *torsion_k, *torsion_periodicity, *torsion_phase;
+  int32_t constraint_count;
+  const int32_t *constraint_a, *constraint_b;
+  const double *constraint_distance;
+  int32_t exception_count;
+  const int32_t *exception_a, *exception_b;    /* a < b                        */
+  const double *exception_charge_product, *exception_lj_a, *exception_lj_b;
+  int32_t exclusion_total;                     /* symmetric, sorted per atom   */
+  const int32_t *exclusion_start, *exclusion_partners; /* N+1, exclusion_total */
+  const int32_t *molecule_start, *molecule_atoms, *molecule_of_atom; /* M+1, N, N */
+  int32_t is_periodic;
+  double box[9];
+  double cutoff;
+  int32_t degrees_of_freedom;
+  int32_t constraint_mode, rigid_water;
+  const double *positions;                     /* 3N nm (the build's coordinates) or NULL */
+  const double *velocities;                    /* 3N nm/ps or NULL              */
+  int32_t water_count;                         /* rigid 3-site waters (SETTLE units) */
+} rmd_system_view;
+int rmd_system_view_get(const rmd_system *system, rmd_system_view *view);
+
+/* Replaces the derived-setup helpers a simulation calls at construction:
+ * pmeParameters (shared/pme/parameters.js:227-252), skinFor
+ * (shared/topology/cells.js:77-94), planCellList (cells.js:111-181),
+ * dispersionCoefficients (shared/dispersion.js:37-66) and ewaldSelfEnergy
+ * (cpu/forces/pme.js:270-286). grid[] entries <= 0 are derived; alpha <= 0 is
+ * derived. Unlike the reference, radix-7 grids (2^a 3^b 5^c 7^d) are allowed. */
+typedef struct rmd_pme_parameters {
+  double alpha;
+  int32_t grid[3];
+  int32_t order;
+  double tolerance;
+} rmd_pme_parameters;
+int rmd_pme_parameters_derive(const double box[9], double cutoff, double tolerance,
+                              double forced_alpha, const int32_t forced_grid[3],
+                              rmd_pme_parameters *out);
+/* B-spline moduli |b(m)|^2 with the Nyquist patch (parameters.js:164-215). */
+int rmd_pme_moduli(int32_t grid_size, int32_t order, double *moduli_out);
+double rmd_skin_for(const rmd_system *system, double timestep, double temperature);
+double rmd_dispersion_energy(const rmd_system *system, double volume);
+
+/* ======================================================================== */
+/* Device side: the simulation context (GpuSimulation, webgpu/simulation.js) */
+/* ======================================================================== */
+
+typedef struct rmd_ctx rmd_ctx;
+
+/* Constructor options (webgpu/simulation.js:93-188). Fields <= 0 (or < 0 where
+ * 0 is meaningful) mean "derive / default", mirroring the `??` defaults. */
+typedef struct rmd_options {
+  double timestep;               /* 0.002 ps                                   */
+  double temperature;            /* 298 K                                      */
+  double friction;               /* 1 /ps (negative -> default; 0 allowed)     */
+  uint32_t seed;
+  double tolerance;              /* 1e-5 relative                              */
+  int32_t max_constraint_iterations; /* 150                                    */
+  int32_t rebuild_interval;      /* 20; accepted for surface parity (the list
+                                    is rebuilt on a device-side skin check)   */
+  int32_t adaptive_rebuild;      /* 1                                          */
+  int32_t has_pressure;          /* 0: NVT                                     */
+  double pressure;               /* bar                                        */
+  int32_t barostat_interval;     /* 25 steps                                   */
+  int32_t dispersion_correction; /* 1                                          */
+  double restraint_force;        /* kJ/mol/nm^2, k*d^2 (no half)               */
+  int32_t restraint_count;
+  const int32_t *restraint_atoms;       /* restraint_count                     */
+  const double *restraint_reference;    /* 3N (only restrained entries read)   */
+  double skin;                   /* nm; <= 0 derives skinFor                   */
+  double neighbour_slack;        /* 2.0                                        */
+  int32_t max_neighbours;        /* <= 0 estimates                             */
+  double ewald_tolerance;        /* 5e-4                                       */
+  double alpha;                  /* <= 0 derives                               */
+  int32_t grid[3];               /* <= 0 derives                               */
+  int32_t device;                /* HIP device ordinal                         */
+  int32_t lanes_per_atom;        /* nonbonded lanes per atom (1..64), 0 = tuned */
+} rmd_options;
+void rmd_options_default(rmd_options *options);
+
+/* new GpuSimulation(device, system, options) (webgpu/simulation.js:83-190). */
+int rmd_create(const rmd_system *system, const rmd_options *options, rmd_ctx **out);
+/* release() (webgpu/simulation.js:1336-1340). */
+void rmd_destroy(rmd_ctx *ctx);
+const char *rmd_last_error(const rmd_ctx *ctx);
+const char *rmd_last_global_error(void);
+
+/* setState(positions, velocities, {step, suppliedNoise}) (simulation.js:356-369).
+ * velocities may be NULL (zero). */
+int rmd_set_state(rmd_ctx *ctx, const double *positions, const double *velocities, int64_t step,
+                  int32_t supplied_noise);
+/* setNoise(noise) (simulation.js:371): 3N normals used when supplied_noise. */
+int rmd_set_noise(rmd_ctx *ctx, const float *noise);
+/* initialise() (simulation.js:412-422): one force evaluation, fresh list. */
+int rmd_initialise(rmd_ctx *ctx);
+/* run(count, {blockSize, stopSignal}) (simulation.js:456-527). The barostat
+ * move happens between blocks, as in the reference. */
+int rmd_run(rmd_ctx *ctx, int64_t steps, int32_t block_size, int64_t *step_out);
+/* StopSignal (webgpu/runtime/yield.js:40-51): checked between blocks. */
+int rmd_request_stop(rmd_ctx *ctx);
+
+/* energy() (simulation.js:1073-1230). All FP64 host-visible sums. */
+typedef struct rmd_energy_terms {
+  int64_t step;
+  double time;
+  double bonded, harmonic_bond, harmonic_angle, periodic_torsion;
+  double lennard_jones;      /* direct + exception LJ                         */
+  double dispersion;
+  double coulomb_direct;     /* direct + exception Coulomb                    */
+  double coulomb_reciprocal;
+  double coulomb_self;       /* self + net-charge                             */
+  double coulomb;
+  double position_restraint;
+  int32_t restrained_atoms;
+  double potential, kinetic, total;
+  double temperature, kinetic_for_temperature;
+  int32_t kinetic_sampled_in_step; /* 1: "before the closing half kick"       */
+  double volume;
+  double box[9];
+} rmd_energy_terms;
+int rmd_energy(rmd_ctx *ctx, rmd_energy_terms *out);
+
+/* readPositions() / readState() (simulation.js:1239-1261). Positions,
+ * velocities and forces are 3N float (f32), like the reference's readback. */
+int rmd_read_positions(rmd_ctx *ctx, float *positions);
+int rmd_read_state(rmd_ctx *ctx, float *positions, float *velocities, float *forces,
+                   double box[9], double *volume);
+
+/* scaleVolume(s) (simulation.js:801-811): centroid scaling + box + potential. */
+int rmd_scale_volume(rmd_ctx *ctx, double length_scale, double *potential);
+
+/* attemptVolumeChange() (simulation.js:995-1070). */
+typedef struct rmd_move {
+  int32_t attempted;         /* 0: no barostat (the reference returns null)    */
+  int32_t accepted;
+  int32_t refused;           /* 1: cutoff would not fit the proposed box       */
+  double weight, delta_volume, length_scale;
+  double old_potential, new_potential, volume, volume_scale, acceptance_rate;
+} rmd_move;
+int rmd_attempt_volume_change(rmd_ctx *ctx, rmd_move *out);
+
+/* minimise(options) (simulation.js:880-979). */
+typedef struct rmd_min_options {
+  int32_t max_iterations;    /* 200 */
+  double force_tolerance;    /* 500 kJ/mol/nm */
+  double max_step;           /* 0.002 nm */
+  double max_displacement;   /* 0.05 nm */
+} rmd_min_options;
+void rmd_min_options_default(rmd_min_options *options);
+typedef struct rmd_min_result {
+  int32_t iterations, rejected, stalled, converged;
+  double potential, max_force, max_step, energy_resolution;
+} rmd_min_result;
+int rmd_minimise(rmd_ctx *ctx, const rmd_min_options *options, rmd_min_result *out);
+
+/* diagnostics() (simulation.js:1264-1334). */
+typedef struct rmd_diagnostics {
+  int32_t max_shake_iterations, shake_hit_iteration_cap, rattle_hit_iteration_cap;
+  int32_t neighbour_required, neighbour_capacity, worst_neighbour_required, neighbour_growths;
+  double skin, worst_displacement;
+  int64_t rebuilds, steps;
+  int32_t cell_counts[3];
+  double volume;
+  int32_t barostat_attempts, barostat_accepted, barostat_refused;
+  double volume_scale;
+  double constraint_tolerance;
+  int32_t pme_grid[3];
+  double pme_alpha;
+  int32_t lanes_per_atom;
+  int32_t water_units, cluster_units, free_units;
+} rmd_diagnostics;
+int rmd_diagnostics_get(rmd_ctx *ctx, rmd_diagnostics *out);
+
+/* ---- evaluation entry points used by the parity tests and the bench ---- */
+
+/* Term masks for rmd_evaluate. */
+enum {
+  RMD_TERM_BONDED = 1, RMD_TERM_DIRECT = 2, RMD_TERM_EXCEPTIONS = 4,
+  RMD_TERM_RECIPROCAL = 8, RMD_TERM_RESTRAINT = 16, RMD_TERM_ALL = 31
+};
+/* One force evaluation at the current positions with a fresh neighbour list,
+ * restricted to `terms`. Fills energies (FP64 sums; may be NULL) and forces
+ * (3N doubles converted from the fixed-point accumulators; may be NULL). The
+ * reference's counterpart is #evaluatePotential (simulation.js:734-789). */
+int rmd_evaluate(rmd_ctx *ctx, int32_t terms, rmd_energy_terms *energies, double *forces);
+
+/* The cell list and neighbour list the device built at the last rebuild.
+ * cell_of_atom: N entries. neighbour_count: N. neighbours: N*capacity (the
+ * capacity is rmd_diagnostics.neighbour_capacity). Counterparts: the buffers
+ * encodeCellList / encodeNeighbours fill (cell-list.js:176-226,
+ * neighbours.js:149-169). */
+int rmd_read_cells(rmd_ctx *ctx, int32_t *cell_of_atom, int32_t *cell_start, int32_t *cell_atoms);
+int rmd_read_neighbours(rmd_ctx *ctx, int32_t *neighbour_count, int32_t *neighbours);
+
+/* Runs exactly one BAOAB step with the current forces and returns, without
+ * the closing half kick being folded into anything else. Used to compare one
+ * step against the host integrator with supplied noise and forces
+ * (tools/gpu/check-integrator.mjs:68-127). */
+int rmd_step_once(rmd_ctx *ctx);
+
+/* Verification entry: the forward real-to-complex 3D FFT of a real grid
+ * (K0*K1*K2 floats, z fastest) with the context's PME plan and sign
+ * convention (exp(+2 pi i m k / K), pme-fft.js:81-144); out receives
+ * K0*K1*(K2/2+1) complex values as interleaved floats. */
+int rmd_debug_fft_forward(rmd_ctx *ctx, const float *real, float *out);
+
+/* Tuning and plumbing: the nonbonded lanes per atom (a power of two <= 64),
+ * hipGraph capture of step blocks on/off, and the context's HIP stream (for
+ * timing on that stream). */
+int rmd_set_lanes(rmd_ctx *ctx, int32_t lanes);
+int rmd_set_graphs(rmd_ctx *ctx, int32_t enabled);
+int rmd_get_stream(rmd_ctx *ctx, void **stream);
+
+/* Device and build identity. */
+typedef struct rmd_device_info {
+  int32_t device_count;
+  char name[128];
+  char arch[64];
+  int32_t compute_units;
+  int64_t global_memory;
+} rmd_device_info;
+int rmd_device_info_get(int32_t device, rmd_device_info *out);
+int32_t rmd_abi_version(void);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* RAINMD_H */

@@ -1,0 +1,224 @@
+// Bonded terms, exception pairs and positional restraints in one launch.
+//
+// Replaces bondedShader (webgpu/kernels/bonded.js:30-166), exceptionShader
+// (webgpu/kernels/nonbonded.js:128-200) and restraintShader
+// (webgpu/kernels/restraints.js:29-63). The reference inverts the term
+// lists so each thread owns one atom and needs no atomics; here each thread
+// owns one TERM and adds its forces to int64 fixed-point accumulators, which
+// commute, so the result is identical on every run and there is no
+// recomputation per role.
+//
+// The harmonic/torsion terms and the restraint are evaluated in FP64 (a
+// solute's few hundred terms; the stretch r - r0 of a stiff bond cancels
+// badly in FP32). Exceptions are pairs, about one per atom, and use the same
+// FP32 arithmetic as the direct-space kernel.
+#include "kernels.h"
+
+namespace rmd {
+
+__device__ __forceinline__ void addForce(const Dev &d, int atom, double fx, double fy, double fz) {
+  atomicAddFixed(&d.force[atom], __double2ll_rn(fx * FORCE_SCALE));
+  atomicAddFixed(&d.force[d.nPad + atom], __double2ll_rn(fy * FORCE_SCALE));
+  atomicAddFixed(&d.force[2 * d.nPad + atom], __double2ll_rn(fz * FORCE_SCALE));
+}
+
+__device__ __forceinline__ void loadPos(const Dev &d, int a, double &x, double &y, double &z) {
+  const float4 p = d.pos[a];
+  x = p.x;
+  y = p.y;
+  z = p.z;
+}
+
+// Harmonic bond, (1/2) k (r - r0)^2 (cpu/forces/bonded.js:22-46).
+__device__ double bondTerm(const Dev &d, int t) {
+  const int2 ab = d.bondAtoms[t];
+  const double2 kp = d.bondParams[t];
+  double ax, ay, az, bx, by, bz;
+  loadPos(d, ab.x, ax, ay, az);
+  loadPos(d, ab.y, bx, by, bz);
+  const double dx = ax - bx, dy = ay - by, dz = az - bz;
+  const double r = sqrt(dx * dx + dy * dy + dz * dz);
+  const double stretch = r - (double)kp.y;
+  const double scale = r > 0 ? (double)kp.x * stretch / r : 0.0;
+  addForce(d, ab.x, -scale * dx, -scale * dy, -scale * dz);
+  addForce(d, ab.y, scale * dx, scale * dy, scale * dz);
+  return 0.5 * (double)kp.x * stretch * stretch;
+}
+
+// Harmonic angle via atan2 with the clamped cross product
+// (cpu/forces/bonded.js:58-114).
+__device__ double angleTerm(const Dev &d, int t) {
+  const int4 abc = d.angleAtoms[t];
+  const double2 kp = d.angleParams[t];
+  double ax, ay, az, bx, by, bz, cx, cy, cz;
+  loadPos(d, abc.x, ax, ay, az);
+  loadPos(d, abc.y, bx, by, bz);
+  loadPos(d, abc.z, cx, cy, cz);
+  const double v1x = ax - bx, v1y = ay - by, v1z = az - bz;
+  const double v2x = cx - bx, v2y = cy - by, v2z = cz - bz;
+  const double n1 = v1x * v1x + v1y * v1y + v1z * v1z, n2 = v2x * v2x + v2y * v2y + v2z * v2z;
+  const double nx = v1y * v2z - v1z * v2y, ny = v1z * v2x - v1x * v2z, nz = v1x * v2y - v1y * v2x;
+  const double cross = sqrt(nx * nx + ny * ny + nz * nz);
+  const double dot = v1x * v2x + v1y * v2y + v1z * v2z;
+  const double theta = atan2(cross, dot);
+  const double dev = theta - (double)kp.y;
+  const double dE = (double)kp.x * dev;
+  const double safe = cross > 1e-12 ? cross : 1e-12;
+  const double s1 = -dE / (n1 * safe), s2 = dE / (n2 * safe);
+  const double fax = s1 * (v1y * nz - v1z * ny), fay = s1 * (v1z * nx - v1x * nz), faz = s1 * (v1x * ny - v1y * nx);
+  const double fcx = s2 * (v2y * nz - v2z * ny), fcy = s2 * (v2z * nx - v2x * nz), fcz = s2 * (v2x * ny - v2y * nx);
+  addForce(d, abc.x, fax, fay, faz);
+  addForce(d, abc.z, fcx, fcy, fcz);
+  addForce(d, abc.y, -(fax + fcx), -(fay + fcy), -(faz + fcz));
+  return 0.5 * (double)kp.x * dev * dev;
+}
+
+// Periodic torsion k (1 + cos(n phi - phase)) with the reference's middle-atom
+// coefficients (cpu/forces/bonded.js:126-207).
+__device__ double torsionTerm(const Dev &d, int t) {
+  const int4 q = d.torsionAtoms[t];
+  const double4 kp = d.torsionParams[t];
+  double ax, ay, az, bx, by, bz, cx, cy, cz, dx_, dy_, dz_;
+  loadPos(d, q.x, ax, ay, az);
+  loadPos(d, q.y, bx, by, bz);
+  loadPos(d, q.z, cx, cy, cz);
+  loadPos(d, q.w, dx_, dy_, dz_);
+  const double b1x = bx - ax, b1y = by - ay, b1z = bz - az;
+  const double b2x = cx - bx, b2y = cy - by, b2z = cz - bz;
+  const double b3x = dx_ - cx, b3y = dy_ - cy, b3z = dz_ - cz;
+  const double n1x = b1y * b2z - b1z * b2y, n1y = b1z * b2x - b1x * b2z, n1z = b1x * b2y - b1y * b2x;
+  const double n2x = b2y * b3z - b2z * b3y, n2y = b2z * b3x - b2x * b3z, n2z = b2x * b3y - b2y * b3x;
+  const double n1s = n1x * n1x + n1y * n1y + n1z * n1z, n2s = n2x * n2x + n2y * n2y + n2z * n2z;
+  const double b2l = sqrt(b2x * b2x + b2y * b2y + b2z * b2z);
+  if (n1s < 1e-24 || n2s < 1e-24 || b2l < 1e-12) return 0.0;
+  const double dn = n1x * n2x + n1y * n2y + n1z * n2z;
+  const double cr = (n1y * n2z - n1z * n2y) * b2x + (n1z * n2x - n1x * n2z) * b2y + (n1x * n2y - n1y * n2x) * b2z;
+  const double phi = atan2(cr / b2l, dn);
+  const double n = (double)kp.y;
+  const double arg = n * phi - (double)kp.z;
+  const double k = (double)kp.x;
+  const double dE = -k * n * sin(arg);
+  const double fA = dE * b2l / n1s, fD = -dE * b2l / n2s;
+  const double fax = fA * n1x, fay = fA * n1y, faz = fA * n1z;
+  const double fdx = fD * n2x, fdy = fD * n2y, fdz = fD * n2z;
+  const double inv = 1.0 / (b2l * b2l);
+  const double tt = (b1x * b2x + b1y * b2y + b1z * b2z) * inv;
+  const double u = (b3x * b2x + b3y * b2y + b3z * b2z) * inv;
+  addForce(d, q.x, fax, fay, faz);
+  addForce(d, q.w, fdx, fdy, fdz);
+  addForce(d, q.y, -(1 + tt) * fax + u * fdx, -(1 + tt) * fay + u * fdy, -(1 + tt) * faz + u * fdz);
+  addForce(d, q.z, tt * fax - (1 + u) * fdx, tt * fay - (1 + u) * fdy, tt * faz - (1 + u) * fdz);
+  return k * (1.0 + cos(arg));
+}
+
+// Exception pair: scaled 1-4 LJ and Coulomb plus the Ewald erf correction
+// with the atoms' full charges, no cutoff (webgpu/kernels/nonbonded.js:128-200).
+__device__ void exceptionTerm(const Dev &d, const BoxF &box, float alpha, int t, float &elj, float &ecoul) {
+  const int2 ij = d.excAtoms[t];
+  const float4 prm = d.excParams[t];  // qq scaled, A, B, qq full (both with 1/4 pi eps0)
+  const float4 pi = d.pos[ij.x], pj = d.pos[ij.y];
+  float dx = pi.x - pj.x, dy = pi.y - pj.y, dz = pi.z - pj.z;
+  const float r2 = minimumImageR2(dx, dy, dz, box);
+  const float invR2 = __builtin_amdgcn_rcpf(r2);
+  const float invR = __builtin_amdgcn_rsqf(r2);
+  const float r = r2 * invR;
+  const float invR6 = invR2 * invR2 * invR2;
+  float scalar = 0.f;
+  if (prm.y != 0.f || prm.z != 0.f) {
+    elj += (prm.y * invR6 - prm.z) * invR6;
+    scalar += (12.0f * prm.y * invR6 - 6.0f * prm.z) * invR6 * invR2;
+  }
+  if (prm.x != 0.f) {
+    ecoul += prm.x * invR;
+    scalar += prm.x * invR * invR2;
+  }
+  if (alpha > 0.f && prm.w != 0.f) {
+    float gaussian;
+    const float unscreened = erfWithGaussian(alpha * r, gaussian);
+    ecoul -= prm.w * unscreened * invR;
+    scalar -= prm.w * (unscreened * invR - TWO_OVER_SQRT_PI_F * alpha * gaussian) * invR2;
+  }
+  addForce(d, ij.x, (double)(scalar * dx), (double)(scalar * dy), (double)(scalar * dz));
+  addForce(d, ij.y, -(double)(scalar * dx), -(double)(scalar * dy), -(double)(scalar * dz));
+}
+
+// Positional restraint k |minImage(r - r0)|^2, no half
+// (shared/topology/restraints.js:98-126), in FP64 against an FP64 reference.
+__device__ double restraintTerm(const Dev &d, const BoxF &box, int t) {
+  const int a = d.restraintAtoms[t];
+  const double4 ref = d.restraintRef[t];
+  double px, py, pz;
+  loadPos(d, a, px, py, pz);
+  double dx = px - ref.x, dy = py - ref.y, dz = pz - ref.z;
+  if (box.periodic) {
+    const double *bx = box.box64;
+    // JavaScript Math.round semantics (halves towards +infinity), c -> b -> a.
+    double s2 = floor(dz / bx[8] + 0.5);
+    dx -= s2 * bx[6];
+    dy -= s2 * bx[7];
+    dz -= s2 * bx[8];
+    double s1 = floor(dy / bx[4] + 0.5);
+    dx -= s1 * bx[3];
+    dy -= s1 * bx[4];
+    double s0 = floor(dx / bx[0] + 0.5);
+    dx -= s0 * bx[0];
+  }
+  const double k = ref.w;
+  addForce(d, a, -2 * k * dx, -2 * k * dy, -2 * k * dz);
+  return k * (dx * dx + dy * dy + dz * dz);
+}
+
+enum { BT_BOND = 0, BT_ANGLE, BT_TORSION, BT_LJX, BT_COULX, BT_RESTRAINT, BT_COUNT };
+
+__global__ void __launch_bounds__(256) bondedKernel(Dev d, float alpha, int terms, int energies) {
+  const BoxF &box = *d.box;
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  double e[BT_COUNT] = {0, 0, 0, 0, 0, 0};
+  int base = 0;
+  const bool doBonded = terms & RMD_TERM_BONDED;
+  if (t < base + d.nBond) {
+    if (doBonded) e[BT_BOND] = bondTerm(d, t - base);
+  } else if ((base += d.nBond, t < base + d.nAngle)) {
+    if (doBonded) e[BT_ANGLE] = angleTerm(d, t - base);
+  } else if ((base += d.nAngle, t < base + d.nTorsion)) {
+    if (doBonded) e[BT_TORSION] = torsionTerm(d, t - base);
+  } else if ((base += d.nTorsion, t < base + d.nException)) {
+    if (terms & RMD_TERM_EXCEPTIONS) {
+      float elj = 0.f, ecoul = 0.f;
+      exceptionTerm(d, box, alpha, t - base, elj, ecoul);
+      e[BT_LJX] = elj;
+      e[BT_COULX] = ecoul;
+    }
+  } else if ((base += d.nException, t < base + d.nRestraint)) {
+    if (terms & RMD_TERM_RESTRAINT) e[BT_RESTRAINT] = restraintTerm(d, box, t - base);
+  }
+  if (!energies) return;
+  __shared__ double partial[BT_COUNT][4];
+  const int wave = threadIdx.x >> 6;
+  for (int k = 0; k < BT_COUNT; ++k) {
+    const double v = waveSum(e[k]);
+    if ((threadIdx.x & 63) == 0) partial[k][wave] = v;
+  }
+  __syncthreads();
+  if (threadIdx.x < BT_COUNT) {
+    double s = 0;
+    for (int w = 0; w < (int)(blockDim.x >> 6); ++w) s += partial[threadIdx.x][w];
+    const int term = threadIdx.x == BT_BOND ? E_BOND
+                     : threadIdx.x == BT_ANGLE ? E_ANGLE
+                     : threadIdx.x == BT_TORSION ? E_TORSION
+                     : threadIdx.x == BT_LJX ? E_LJ_EXCEPTION
+                     : threadIdx.x == BT_COULX ? E_COUL_EXCEPTION
+                                               : E_RESTRAINT;
+    d.slots[term * MAX_SLOTS + blockIdx.x] = s;
+  }
+}
+
+int launchBonded(const Dev &d, float alpha, int terms, int energies, hipStream_t s, int *slotsUsed) {
+  const int total = d.nBond + d.nAngle + d.nTorsion + d.nException + d.nRestraint;
+  const int blocks = (total + 255) / 256;
+  if (blocks > 0) bondedKernel<<<blocks, 256, 0, s>>>(d, alpha, terms, energies);
+  if (slotsUsed) *slotsUsed = blocks;
+  return blocks;
+}
+
+}  // namespace rmd

@@ -1,0 +1,1240 @@
+// The simulation context behind the C ABI: device allocation, the block
+// loop, force evaluations, the Monte Carlo barostat, the minimiser and state
+// readback. Mirrors GpuSimulation (webgpu/simulation.js:82-1341) method for
+// method; the machinery is HIP streams, device-side rebuild decisions and a
+// hipGraph per block of steps.
+#include <algorithm>
+#include <atomic>
+#include <cmath>
+#include <cstring>
+#include <functional>
+#include <map>
+#include <numeric>
+#include <string>
+#include <vector>
+
+#include "host.h"
+#include "kernels.h"
+
+namespace rmd {
+void setGlobalError(const std::string &m);
+}
+
+using namespace rmd;
+
+#define HIP_CHECK(call)                                                                              \
+  do {                                                                                               \
+    hipError_t e_ = (call);                                                                          \
+    if (e_ != hipSuccess)                                                                            \
+      throw rmd::Error(RMD_E_DEVICE, std::string(#call) + " failed: " + hipGetErrorString(e_));     \
+  } while (0)
+
+namespace {
+
+template <typename T>
+T *devAlloc(size_t count, std::vector<void *> &held) {
+  void *p = nullptr;
+  if (count == 0) count = 1;
+  hipError_t e = hipMalloc(&p, count * sizeof(T));
+  if (e != hipSuccess)
+    throw rmd::Error(e == hipErrorOutOfMemory ? RMD_E_BUDGET : RMD_E_DEVICE,
+                     std::string("hipMalloc of ") + std::to_string(count * sizeof(T)) + " bytes failed: " +
+                         hipGetErrorString(e));
+  (void)hipMemset(p, 0, count * sizeof(T));
+  held.push_back(p);
+  return static_cast<T *>(p);
+}
+
+template <typename T>
+void upload(T *dst, const std::vector<T> &src, hipStream_t s) {
+  if (src.empty()) return;
+  HIP_CHECK(hipMemcpyAsync(dst, src.data(), src.size() * sizeof(T), hipMemcpyHostToDevice, s));
+}
+
+std::vector<int> factorRadices(int K) {
+  std::vector<int> r;
+  int rem = K;
+  while (rem % 4 == 0) {
+    r.push_back(4);
+    rem /= 4;
+  }
+  for (int p : {2, 3, 5, 7})
+    while (rem % p == 0) {
+      r.push_back(p);
+      rem /= p;
+    }
+  if (rem != 1 || r.size() > 8) throw rmd::Error(RMD_E_INVALID, "PME grid size " + std::to_string(K) + " cannot be transformed");
+  return r;
+}
+
+}  // namespace
+
+struct rmd_ctx {
+  System sys;
+  rmd_options opt{};
+  std::string lastError;
+  // derived parameters
+  double timestep = 0.002, temperature = 298, friction = 1, tolerance = 1e-5;
+  uint32_t seed = 0;
+  int maxIterations = 150;
+  bool hasPressure = false;
+  double pressure = 0;
+  int barostatInterval = 25;
+  bool dispersionOn = true;
+  double skin = 0, slack = 2.0;
+  int lanes = 16;
+  CellPlan plan;
+  rmd_pme_parameters pme{};
+  std::vector<double> moduli[3];
+  DispersionCoefficients disp;
+  double box[9] = {0};
+  double selfEnergy = 0, netEnergy = 0;
+  double restraintK = 0;
+  std::vector<int> restraintAtoms;
+  // device
+  int device = 0;
+  hipStream_t stream = nullptr;
+  std::vector<void *> held;
+  Dev d{};
+  PmeDev p{};
+  BoxF *boxDev = nullptr;
+  double *dModuli[3] = {nullptr, nullptr, nullptr};
+  float4 *posSaved = nullptr, *velSaved = nullptr, *centroids = nullptr;
+  long long *forceSaved = nullptr;
+  Scalars *scSaved = nullptr;
+  int *molStart = nullptr, *molAtoms = nullptr, *molOfAtom = nullptr;
+  int *dCounts = nullptr;
+  double *dTerms = nullptr;
+  unsigned int *dMaxForce = nullptr;
+  int nUnits = 0, waterUnits = 0, clusterUnits = 0, freeUnits = 0;
+  // graphs per block length
+  std::map<int, hipGraphExec_t> graphs;
+  bool useGraphs = true;
+  // host state
+  int64_t step = 0;
+  std::atomic<int> stop{0};
+  int barostatAttempts = 0, barostatAccepted = 0, barostatRefusals = 0;
+  int windowAttempts = 0, windowAccepted = 0;
+  double volumeScale = -1;
+  int64_t stepsSinceBarostat = 0;
+  int neighbourGrowths = 0;
+  unsigned int worstRequired = 0;
+  int64_t kineticSampledAtStep = -1;
+  int64_t stepsRun = 0;
+  bool initialised = false;
+  int slotCounts[E_TERM_COUNT] = {0};
+  IntegratorParams ip{};
+};
+
+namespace {
+
+double volumeOf(const double *b) { return std::fabs(b[0] * b[4] * b[8]); }
+
+void invalidateGraphs(rmd_ctx *c) {
+  for (auto &g : c->graphs) (void)hipGraphExecDestroy(g.second);
+  c->graphs.clear();
+}
+
+// Write the device box: f32 rows and inverse widths for the pair kernels,
+// FP64 rows for PME and restraints, the cell grid shape.
+void writeBox(rmd_ctx *c) {
+  BoxF b{};
+  double recip[9];
+  reciprocalBoxVectors(c->box, recip);
+  for (int k = 0; k < 3; ++k) {
+    b.a[k] = (float)c->box[k];
+    b.b[k] = (float)c->box[3 + k];
+    b.c[k] = (float)c->box[6 + k];
+  }
+  b.inv[0] = (float)(1.0 / c->box[0]);
+  b.inv[1] = (float)(1.0 / c->box[4]);
+  b.inv[2] = (float)(1.0 / c->box[8]);
+  for (int k = 0; k < 9; ++k) {
+    b.recip[k] = (float)recip[k];
+    b.recip64[k] = recip[k];
+    b.box64[k] = c->box[k];
+  }
+  b.cutoff = (float)c->sys.cutoff;
+  b.rc2 = (float)(c->sys.cutoff * c->sys.cutoff);
+  b.rl2 = (float)(c->plan.listCutoff * c->plan.listCutoff);
+  b.periodic = 1;
+  for (int k = 0; k < 3; ++k) b.cells[k] = c->plan.counts[k];
+  b.cellTotal = c->plan.cellCount;
+  HIP_CHECK(hipMemcpyAsync(c->boxDev, &b, sizeof b, hipMemcpyHostToDevice, c->stream));
+  ewaldSelfEnergy(c->sys, c->pme.alpha, volumeOf(c->box), &c->selfEnergy, &c->netEnergy);
+  const double *mods[3] = {c->dModuli[0], c->dModuli[1], c->dModuli[2]};
+  pmeComputeInfluence(c->p, c->boxDev, mods, c->pme.alpha, c->stream);
+  HIP_CHECK(hipGetLastError());
+}
+
+void allocateCells(rmd_ctx *c, int cellCount) {
+  if (cellCount <= c->d.cellCapacity) return;
+  const int capacity = std::max(cellCount, c->d.cellCapacity * 2);
+  c->d.cellCount = devAlloc<int>(capacity + 1, c->held);
+  c->d.cellStart = devAlloc<int>(capacity + 1, c->held);
+  c->d.cellCursor = devAlloc<int>(capacity + 1, c->held);
+  c->d.cellCapacity = capacity;
+  invalidateGraphs(c);
+}
+
+void allocateNeighbours(rmd_ctx *c, int cap) {
+  c->d.cap = cap;
+  c->d.nbrList = devAlloc<int>((size_t)c->d.nPad * cap, c->held);
+  invalidateGraphs(c);
+}
+
+void buildUnits(rmd_ctx *c, std::vector<int4> &units, std::vector<double4> &geom, std::vector<int> &conStart,
+                std::vector<int> &atomStart, std::vector<int> &atoms, std::vector<int4> &cons,
+                std::vector<double> &conD) {
+  const System &s = c->sys;
+  const Topology &t = s.topology;
+  const int n = t.atomCount;
+  const int nc = (int)s.constraints.a.size();
+  std::vector<int> parent(n);
+  std::iota(parent.begin(), parent.end(), 0);
+  std::function<int(int)> find = [&](int x) { return parent[x] == x ? x : parent[x] = find(parent[x]); };
+  for (int k = 0; k < nc; ++k) {
+    int a = find(s.constraints.a[k]), b = find(s.constraints.b[k]);
+    if (a != b) parent[std::max(a, b)] = std::min(a, b);
+  }
+  std::map<int, std::vector<int>> consOf, atomsOf;
+  for (int k = 0; k < nc; ++k) consOf[find(s.constraints.a[k])].push_back(k);
+  std::vector<char> constrained(n, 0);
+  for (int k = 0; k < nc; ++k) constrained[s.constraints.a[k]] = constrained[s.constraints.b[k]] = 1;
+  for (int a = 0; a < n; ++a)
+    if (constrained[a]) atomsOf[find(a)].push_back(a);
+  conStart.assign(1, 0);
+  atomStart.assign(1, 0);
+  std::vector<char> emitted(n, 0);
+  for (int a = 0; a < n; ++a) {
+    if (!constrained[a]) {
+      units.push_back(make_int4(0, a, 0, 0));
+      geom.push_back(make_double4(0, 0, 0, 0));
+      ++c->freeUnits;
+      continue;
+    }
+    const int root = find(a);
+    if (emitted[root]) continue;
+    emitted[root] = 1;
+    const std::vector<int> &ca = atomsOf[root];
+    const std::vector<int> &ck = consOf[root];
+    // A rigid water: three atoms, three constraints, heavy centre, equal
+    // partner masses and equal centre-partner lengths (SETTLE's shape).
+    bool water = false;
+    if (ca.size() == 3 && ck.size() == 3) {
+      int centre = ca[0];
+      for (int x : ca)
+        if (t.masses[x] > t.masses[centre]) centre = x;
+      std::vector<int> partners;
+      for (int x : ca)
+        if (x != centre) partners.push_back(x);
+      auto len = [&](int p, int q) {
+        for (int k : ck)
+          if ((s.constraints.a[k] == p && s.constraints.b[k] == q) || (s.constraints.a[k] == q && s.constraints.b[k] == p))
+            return s.constraints.distance[k];
+        return -1.0;
+      };
+      const double d1 = len(centre, partners[0]), d2 = len(centre, partners[1]), d3 = len(partners[0], partners[1]);
+      if (d1 > 0 && d2 > 0 && d3 > 0 && d1 == d2 && t.masses[partners[0]] == t.masses[partners[1]] &&
+          t.masses[partners[0]] > 0 && t.masses[centre] > 0) {
+        water = true;
+        units.push_back(make_int4(1, centre, partners[0], partners[1]));
+        geom.push_back(make_double4(d1, d3, t.masses[centre], t.masses[partners[0]]));
+        ++c->waterUnits;
+      }
+    }
+    if (water) continue;
+    const int cl = (int)conStart.size() - 1;
+    const int base = (int)atoms.size();
+    for (int x : ca) atoms.push_back(x);
+    for (int k : ck) {
+      int sa = base + (int)(std::find(ca.begin(), ca.end(), s.constraints.a[k]) - ca.begin());
+      int sb = base + (int)(std::find(ca.begin(), ca.end(), s.constraints.b[k]) - ca.begin());
+      cons.push_back(make_int4(sa, sb, k, 0));
+      conD.push_back(s.constraints.distance[k]);
+    }
+    conStart.push_back((int)cons.size());
+    atomStart.push_back((int)atoms.size());
+    units.push_back(make_int4(2, cl, 0, 0));
+    geom.push_back(make_double4(0, 0, 0, 0));
+    ++c->clusterUnits;
+  }
+}
+
+void setupIntegrator(rmd_ctx *c, int suppliedNoise) {
+  const double decay = std::exp(-c->friction * c->timestep);
+  c->ip.halfStep = 0.5 * c->timestep;
+  c->ip.decay = decay;
+  c->ip.noiseScale = std::sqrt(std::max(0.0, 1.0 - decay * decay));
+  c->ip.kT = BOLTZMANN * c->temperature;
+  c->ip.tolerance = c->tolerance;
+  c->ip.seed = c->seed;
+  c->ip.maxIterations = c->maxIterations;
+  c->ip.suppliedNoise = suppliedNoise;
+  const double half = std::max(0.0, 0.5 * c->plan.skin - 1e-4);
+  c->ip.dispThreshold2 = half * half;
+}
+
+// ---- force evaluation ---------------------------------------------------
+
+__global__ void decideRebuildKernel(Scalars *sc, float threshold2, int force) {
+  const float maxDisp2 = __uint_as_float(sc->maxDisp2Bits);
+  const bool rebuild = force || sc->forceRebuild || !(maxDisp2 <= threshold2);
+  sc->rebuild = rebuild ? 1u : 0u;
+  if (rebuild) {
+    sc->maxDisp2Bits = 0u;
+    sc->required = 0u;
+    sc->rebuilds += 1u;
+    sc->forceRebuild = 0u;
+  }
+}
+
+__global__ void setForceRebuildKernel(Scalars *sc) { sc->forceRebuild = 1u; }
+__global__ void setStepKernel(Scalars *sc, long long step) { sc->step = step; }
+
+void enqueueForces(rmd_ctx *c, int terms, int energies, int slots[E_TERM_COUNT]) {
+  Dev &d = c->d;
+  launchRebuild(d, c->stream);
+  for (int k = 0; k < E_TERM_COUNT; ++k) slots[k] = 0;
+  const float alpha = (float)c->pme.alpha;
+  if (terms & RMD_TERM_DIRECT) {
+    const int b = launchNonbonded(d, alpha, c->lanes, energies, c->stream);
+    slots[E_LJ_DIRECT] = slots[E_COUL_DIRECT] = b;
+  }
+  if (terms & (RMD_TERM_BONDED | RMD_TERM_EXCEPTIONS | RMD_TERM_RESTRAINT)) {
+    int b = 0;
+    launchBonded(d, alpha, terms, energies, c->stream, &b);
+    slots[E_BOND] = slots[E_ANGLE] = slots[E_TORSION] = slots[E_LJ_EXCEPTION] = slots[E_COUL_EXCEPTION] =
+        slots[E_RESTRAINT] = b;
+  }
+  if (terms & RMD_TERM_RECIPROCAL) {
+    int b = 0;
+    launchPme(d, c->p, energies, c->stream, &b);
+    slots[E_RECIPROCAL] = b;
+  }
+}
+
+void checkScalars(rmd_ctx *c, Scalars &h) {
+  HIP_CHECK(hipMemcpyAsync(&h, c->d.sc, sizeof h, hipMemcpyDeviceToHost, c->stream));
+  HIP_CHECK(hipStreamSynchronize(c->stream));
+  if (h.cellOverflow)
+    throw Error(RMD_E_INVALID, "a cell held more atoms than the cell sort supports (4096); the box is too dense "
+                               "or too small for this cutoff");
+}
+
+// One evaluation at the current positions. force_rebuild mirrors the
+// reference's #evaluatePotential, which always rebuilds.
+void evaluate(rmd_ctx *c, int terms, bool forceRebuild, double terms_out[E_TERM_COUNT]) {
+  for (int attempt = 0; attempt < 4; ++attempt) {
+    HIP_CHECK(hipMemsetAsync(&c->d.sc->requiredMax, 0, sizeof(unsigned int), c->stream));
+    decideRebuildKernel<<<1, 1, 0, c->stream>>>(c->d.sc, (float)c->ip.dispThreshold2, forceRebuild ? 1 : 0);
+    launchZeroForces(c->d, c->stream);
+    int slots[E_TERM_COUNT];
+    enqueueForces(c, terms, terms_out ? 1 : 0, slots);
+    HIP_CHECK(hipGetLastError());
+    Scalars h;
+    checkScalars(c, h);
+    c->worstRequired = std::max(c->worstRequired, h.requiredMax);
+    if (h.requiredMax > (unsigned int)c->d.cap) {
+      const int grown = std::max(c->d.cap + 32, (int)std::ceil(h.requiredMax * 1.25));
+      allocateNeighbours(c, grown);
+      ++c->neighbourGrowths;
+      forceRebuild = true;
+      continue;
+    }
+    if (terms_out) {
+      std::memcpy(c->slotCounts, slots, sizeof slots);
+      HIP_CHECK(hipMemcpyAsync(c->dCounts, slots, sizeof slots, hipMemcpyHostToDevice, c->stream));
+      launchReduceEnergies(c->d, c->dCounts, c->dTerms, c->stream);
+      HIP_CHECK(hipMemcpyAsync(terms_out, c->dTerms, sizeof(double) * E_TERM_COUNT, hipMemcpyDeviceToHost,
+                               c->stream));
+      HIP_CHECK(hipStreamSynchronize(c->stream));
+    }
+    return;
+  }
+  throw Error(RMD_E_NEIGHBOUR_OVERFLOW, "the neighbour list kept overflowing after growth");
+}
+
+double dispersionNow(rmd_ctx *c) {
+  return c->dispersionOn ? dispersionEnergy(c->disp, volumeOf(c->box), c->sys.cutoff) : 0.0;
+}
+
+double potentialFromTerms(rmd_ctx *c, const double *t) {
+  return t[E_BOND] + t[E_ANGLE] + t[E_TORSION] + t[E_LJ_DIRECT] + t[E_LJ_EXCEPTION] + t[E_COUL_DIRECT] +
+         t[E_COUL_EXCEPTION] + t[E_RECIPROCAL] + c->selfEnergy + c->netEnergy + dispersionNow(c) + t[E_RESTRAINT];
+}
+
+double evaluatePotential(rmd_ctx *c) {
+  double t[E_TERM_COUNT];
+  evaluate(c, RMD_TERM_ALL, true, t);
+  return potentialFromTerms(c, t);
+}
+
+// ---- the block loop ---------------------------------------------------------
+
+void enqueueBlock(rmd_ctx *c, int steps) {
+  for (int s = 0; s < steps; ++s) {
+    launchIntegrateOpen(c->d, c->ip, s > 0 ? 1 : 0, s == steps - 1 ? 1 : 0, c->stream);
+    int slots[E_TERM_COUNT];
+    enqueueForces(c, RMD_TERM_ALL, 0, slots);
+  }
+  launchIntegrateClose(c->d, c->ip, 0, c->stream);
+}
+
+void launchBlock(rmd_ctx *c, int steps) {
+  if (!c->useGraphs) {
+    enqueueBlock(c, steps);
+    HIP_CHECK(hipGetLastError());
+    return;
+  }
+  auto it = c->graphs.find(steps);
+  if (it == c->graphs.end()) {
+    hipGraph_t graph;
+    HIP_CHECK(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+    enqueueBlock(c, steps);
+    HIP_CHECK(hipStreamEndCapture(c->stream, &graph));
+    hipGraphExec_t exec;
+    HIP_CHECK(hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0));
+    HIP_CHECK(hipGraphDestroy(graph));
+    it = c->graphs.emplace(steps, exec).first;
+  }
+  HIP_CHECK(hipGraphLaunch(it->second, c->stream));
+}
+
+void snapshot(rmd_ctx *c) {
+  const size_t n = c->d.nPad;
+  HIP_CHECK(hipMemcpyAsync(c->posSaved, c->d.pos, n * sizeof(float4), hipMemcpyDeviceToDevice, c->stream));
+  HIP_CHECK(hipMemcpyAsync(c->velSaved, c->d.vel, n * sizeof(float4), hipMemcpyDeviceToDevice, c->stream));
+  HIP_CHECK(hipMemcpyAsync(c->forceSaved, c->d.force, 3 * n * sizeof(long long), hipMemcpyDeviceToDevice, c->stream));
+  HIP_CHECK(hipMemcpyAsync(c->scSaved, c->d.sc, sizeof(Scalars), hipMemcpyDeviceToDevice, c->stream));
+}
+
+void restore(rmd_ctx *c) {
+  const size_t n = c->d.nPad;
+  HIP_CHECK(hipMemcpyAsync(c->d.pos, c->posSaved, n * sizeof(float4), hipMemcpyDeviceToDevice, c->stream));
+  HIP_CHECK(hipMemcpyAsync(c->d.vel, c->velSaved, n * sizeof(float4), hipMemcpyDeviceToDevice, c->stream));
+  HIP_CHECK(hipMemcpyAsync(c->d.force, c->forceSaved, 3 * n * sizeof(long long), hipMemcpyDeviceToDevice, c->stream));
+  HIP_CHECK(hipMemcpyAsync(c->d.sc, c->scSaved, sizeof(Scalars), hipMemcpyDeviceToDevice, c->stream));
+}
+
+void runBlock(rmd_ctx *c, int steps) {
+  for (int attempt = 0; attempt < 4; ++attempt) {
+    HIP_CHECK(hipMemsetAsync(&c->d.sc->requiredMax, 0, sizeof(unsigned int), c->stream));
+    snapshot(c);
+    launchBlock(c, steps);
+    Scalars h;
+    checkScalars(c, h);
+    if (h.nonFinite || std::isnan(__builtin_bit_cast(float, h.maxDisp2Bits)))
+      throw Error(RMD_E_NONFINITE, "a coordinate became non-finite in the block of " + std::to_string(steps) +
+                                       " steps that ended at step " + std::to_string(c->step + steps) +
+                                       "; everything written during it should be discarded too.");
+    if (h.shakeFailed) throw Error(RMD_E_CONSTRAINT, "constraints did not converge in the iteration limit");
+    c->worstRequired = std::max(c->worstRequired, h.requiredMax);
+    if (h.requiredMax > (unsigned int)c->d.cap) {
+      // The list overflowed somewhere in the block: grow it and run the
+      // block again from its start, so no step ever used a truncated list.
+      restore(c);
+      const int grown = std::max(c->d.cap + 32, (int)std::ceil(h.requiredMax * 1.25));
+      allocateNeighbours(c, grown);
+      ++c->neighbourGrowths;
+      setForceRebuildKernel<<<1, 1, 0, c->stream>>>(c->d.sc);
+      continue;
+    }
+    c->step += steps;
+    c->stepsRun += steps;
+    c->kineticSampledAtStep = c->step;
+    return;
+  }
+  throw Error(RMD_E_NEIGHBOUR_OVERFLOW, "the neighbour list kept overflowing after growth");
+}
+
+// #updateBox (webgpu/simulation.js:681-731): adopt a new box, re-plan the
+// cells and the capacity, update the self energy and the PME influence
+// function; the PME grid and alpha stay fixed.
+void updateBox(rmd_ctx *c, const double *nb) {
+  for (int k = 0; k < 9; ++k) c->box[k] = nb[k];
+  CellPlan estimate = planCellList(c->sys, c->plan.skin, c->slack, 0, c->box);
+  const int capacity = std::max(c->d.cap, estimate.maxNeighbours);
+  CellPlan plan = planCellList(c->sys, c->plan.skin, c->slack, capacity, c->box);
+  c->plan = plan;
+  allocateCells(c, plan.cellCount);
+  if (capacity > c->d.cap) {
+    allocateNeighbours(c, capacity);
+    ++c->neighbourGrowths;
+  }
+  writeBox(c);
+  setForceRebuildKernel<<<1, 1, 0, c->stream>>>(c->d.sc);
+}
+
+double scaleVolume(rmd_ctx *c, double s) {
+  launchScaleMolecules(c->d, c->molStart, c->molAtoms, c->sys.topology.moleculeCount, c->centroids, c->molOfAtom,
+                       (float)s, c->stream);
+  double nb[9];
+  for (int k = 0; k < 9; ++k) nb[k] = c->box[k] * s;
+  updateBox(c, nb);
+  return evaluatePotential(c);
+}
+
+void toDouble(rmd_ctx *c, double *forces) {
+  const int n = c->sys.topology.atomCount;
+  std::vector<long long> f(3 * (size_t)c->d.nPad);
+  HIP_CHECK(hipMemcpyAsync(f.data(), c->d.force, f.size() * sizeof(long long), hipMemcpyDeviceToHost, c->stream));
+  HIP_CHECK(hipStreamSynchronize(c->stream));
+  for (int a = 0; a < n; ++a)
+    for (int k = 0; k < 3; ++k) forces[3 * a + k] = (double)f[(size_t)k * c->d.nPad + a] * FORCE_UNSCALE;
+}
+
+void fillEnergy(rmd_ctx *c, const double *t, rmd_energy_terms *e, bool withKinetic, double kinetic, double early,
+                bool earlyValid) {
+  std::memset(e, 0, sizeof *e);
+  e->step = c->step;
+  e->time = c->step * c->timestep;
+  e->harmonic_bond = t[E_BOND];
+  e->harmonic_angle = t[E_ANGLE];
+  e->periodic_torsion = t[E_TORSION];
+  e->bonded = t[E_BOND] + t[E_ANGLE] + t[E_TORSION];
+  e->lennard_jones = t[E_LJ_DIRECT] + t[E_LJ_EXCEPTION];
+  e->dispersion = dispersionNow(c);
+  e->coulomb_direct = t[E_COUL_DIRECT] + t[E_COUL_EXCEPTION];
+  e->coulomb_reciprocal = t[E_RECIPROCAL];
+  e->coulomb_self = c->selfEnergy + c->netEnergy;
+  e->coulomb = e->coulomb_direct + e->coulomb_reciprocal + e->coulomb_self;
+  e->position_restraint = t[E_RESTRAINT];
+  e->restrained_atoms = (int32_t)c->restraintAtoms.size();
+  e->potential = e->bonded + e->lennard_jones + e->dispersion + e->coulomb + e->position_restraint;
+  if (withKinetic) {
+    e->kinetic = kinetic;
+    e->total = e->potential + kinetic;
+    e->kinetic_for_temperature = earlyValid && (early > 0 || kinetic <= 0) ? early : kinetic;
+    e->kinetic_sampled_in_step = earlyValid && (early > 0 || kinetic <= 0) ? 1 : 0;
+    e->temperature = 2 * e->kinetic_for_temperature / (c->sys.degreesOfFreedom * BOLTZMANN);
+  } else {
+    e->total = e->potential;
+  }
+  e->volume = volumeOf(c->box);
+  for (int k = 0; k < 9; ++k) e->box[k] = c->box[k];
+}
+
+void create(rmd_ctx *c, const System &sys, const rmd_options &o) {
+  c->sys = sys;
+  c->opt = o;
+  const Topology &t = sys.topology;
+  if (!sys.isPeriodic) throw Error(RMD_E_INVALID, "this engine runs periodic PME systems; the system has no box");
+  c->timestep = o.timestep > 0 ? o.timestep : 0.002;
+  c->temperature = o.temperature > 0 ? o.temperature : 298;
+  c->friction = o.friction >= 0 ? o.friction : 1.0;
+  c->seed = o.seed;
+  c->tolerance = o.tolerance > 0 ? o.tolerance : 1e-5;
+  c->maxIterations = o.max_constraint_iterations > 0 ? o.max_constraint_iterations : 150;
+  c->hasPressure = o.has_pressure != 0;
+  c->pressure = o.pressure;
+  c->barostatInterval = o.barostat_interval > 0 ? o.barostat_interval : 25;
+  c->dispersionOn = o.dispersion_correction != 0;
+  c->slack = o.neighbour_slack > 0 ? o.neighbour_slack : 2.0;
+  c->lanes = o.lanes_per_atom > 0 ? o.lanes_per_atom : 16;
+  if (c->lanes & (c->lanes - 1) || c->lanes > 64) throw Error(RMD_E_INVALID, "lanes_per_atom must be a power of two <= 64");
+  for (int k = 0; k < 9; ++k) c->box[k] = sys.box[k];
+  c->skin = o.skin > 0 ? o.skin : skinFor(sys, c->timestep, c->temperature, nullptr);
+  c->plan = planCellList(sys, c->skin, c->slack, o.max_neighbours, c->box);
+  {
+    rmd_pme_parameters pp;
+    const int32_t forced[3] = {o.grid[0], o.grid[1], o.grid[2]};
+    if (rmd_pme_parameters_derive(c->box, sys.cutoff, o.ewald_tolerance > 0 ? o.ewald_tolerance : 0.0005,
+                                  o.alpha > 0 ? o.alpha : 0, forced, &pp) != RMD_OK)
+      throw Error(RMD_E_INVALID, rmd_last_global_error());
+    c->pme = pp;
+  }
+  c->disp = dispersionCoefficients(sys);
+  c->device = o.device;
+  int count = 0;
+  if (hipGetDeviceCount(&count) != hipSuccess || count <= 0)
+    throw Error(RMD_E_DEVICE, "no HIP device is available; this library runs on an MI355X (gfx950)");
+  HIP_CHECK(hipSetDevice(c->device));
+  HIP_CHECK(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
+
+  const int n = t.atomCount;
+  Dev &d = c->d;
+  d.n = n;
+  d.nPad = (n + 63) / 64 * 64;
+  d.pos = devAlloc<float4>(d.nPad, c->held);
+  d.vel = devAlloc<float4>(d.nPad, c->held);
+  d.force = devAlloc<long long>(3 * (size_t)d.nPad, c->held);
+  d.posRebuild = devAlloc<float4>(d.nPad, c->held);
+  c->posSaved = devAlloc<float4>(d.nPad, c->held);
+  c->velSaved = devAlloc<float4>(d.nPad, c->held);
+  c->forceSaved = devAlloc<long long>(3 * (size_t)d.nPad, c->held);
+  c->boxDev = devAlloc<BoxF>(1, c->held);
+  d.box = c->boxDev;
+  d.type = devAlloc<int>(d.nPad, c->held);
+  d.types = t.typeCount;
+  {
+    std::vector<float2> lj(t.ljA.size());
+    for (size_t k = 0; k < lj.size(); ++k) lj[k] = make_float2((float)t.ljA[k], (float)t.ljB[k]);
+    d.lj = devAlloc<float2>(lj.size(), c->held);
+    upload(d.lj, lj, c->stream);
+    upload(d.type, t.atomTypes, c->stream);
+  }
+  d.cellOfAtom = devAlloc<int>(d.nPad, c->held);
+  d.cellAtoms = devAlloc<int>(d.nPad, c->held);
+  d.cellCapacity = 0;
+  allocateCells(c, c->plan.cellCount);
+  d.nbrCount = devAlloc<int>(d.nPad, c->held);
+  allocateNeighbours(c, c->plan.maxNeighbours);
+  d.exclStart = devAlloc<int>(sys.exclusionStart.size(), c->held);
+  d.exclPartners = devAlloc<int>(sys.exclusionPartners.size(), c->held);
+  upload(d.exclStart, sys.exclusionStart, c->stream);
+  upload(d.exclPartners, sys.exclusionPartners, c->stream);
+
+  // Bonded terms.
+  d.nBond = (int)sys.harmonicBonds.a.size();
+  d.nAngle = (int)t.angles.a.size();
+  d.nTorsion = (int)t.dihedrals.a.size();
+  d.nException = (int)sys.exceptions.a.size();
+  {
+    std::vector<int2> ba(d.nBond);
+    std::vector<double2> bp(d.nBond);
+    for (int k = 0; k < d.nBond; ++k) {
+      ba[k] = make_int2(sys.harmonicBonds.a[k], sys.harmonicBonds.b[k]);
+      bp[k] = make_double2(sys.harmonicBonds.k[k], sys.harmonicBonds.r0[k]);
+    }
+    d.bondAtoms = devAlloc<int2>(ba.size(), c->held);
+    d.bondParams = devAlloc<double2>(bp.size(), c->held);
+    upload(d.bondAtoms, ba, c->stream);
+    upload(d.bondParams, bp, c->stream);
+    std::vector<int4> aa(d.nAngle);
+    std::vector<double2> ap(d.nAngle);
+    for (int k = 0; k < d.nAngle; ++k) {
+      aa[k] = make_int4(t.angles.a[k], t.angles.b[k], t.angles.c[k], 0);
+      ap[k] = make_double2(t.angles.k[k], t.angles.theta0[k]);
+    }
+    d.angleAtoms = devAlloc<int4>(aa.size(), c->held);
+    d.angleParams = devAlloc<double2>(ap.size(), c->held);
+    upload(d.angleAtoms, aa, c->stream);
+    upload(d.angleParams, ap, c->stream);
+    std::vector<int4> ta(d.nTorsion);
+    std::vector<double4> tp(d.nTorsion);
+    for (int k = 0; k < d.nTorsion; ++k) {
+      ta[k] = make_int4(t.dihedrals.a[k], t.dihedrals.b[k], t.dihedrals.c[k], t.dihedrals.d[k]);
+      tp[k] = make_double4(t.dihedrals.k[k], t.dihedrals.periodicity[k], t.dihedrals.phase[k], 0);
+    }
+    d.torsionAtoms = devAlloc<int4>(ta.size(), c->held);
+    d.torsionParams = devAlloc<double4>(tp.size(), c->held);
+    upload(d.torsionAtoms, ta, c->stream);
+    upload(d.torsionParams, tp, c->stream);
+    std::vector<int2> ea(d.nException);
+    std::vector<float4> ep(d.nException);
+    for (int k = 0; k < d.nException; ++k) {
+      const int i = sys.exceptions.a[k], j = sys.exceptions.b[k];
+      ea[k] = make_int2(i, j);
+      ep[k] = make_float4((float)(ONE_4PI_EPS0 * sys.exceptions.chargeProduct[k]), (float)sys.exceptions.ljA[k],
+                          (float)sys.exceptions.ljB[k], (float)(ONE_4PI_EPS0 * t.charges[i] * t.charges[j]));
+    }
+    d.excAtoms = devAlloc<int2>(ea.size(), c->held);
+    d.excParams = devAlloc<float4>(ep.size(), c->held);
+    upload(d.excAtoms, ea, c->stream);
+    upload(d.excParams, ep, c->stream);
+  }
+  // Restraints (k*d^2, reference positions in FP64).
+  c->restraintK = o.restraint_force > 0 ? o.restraint_force : 0;
+  if (c->restraintK > 0 && o.restraint_count > 0) {
+    if (!o.restraint_atoms || !o.restraint_reference)
+      throw Error(RMD_E_INVALID, "restraint atoms and reference are required when restraint_force > 0");
+    std::vector<double4> ref;
+    for (int k = 0; k < o.restraint_count; ++k) {
+      const int a = o.restraint_atoms[k];
+      if (a < 0 || a >= n) throw Error(RMD_E_INVALID, "restraint atom out of range");
+      c->restraintAtoms.push_back(a);
+      ref.push_back(make_double4(o.restraint_reference[3 * a], o.restraint_reference[3 * a + 1],
+                                 o.restraint_reference[3 * a + 2], c->restraintK));
+    }
+    d.nRestraint = (int)c->restraintAtoms.size();
+    d.restraintAtoms = devAlloc<int>(d.nRestraint, c->held);
+    d.restraintRef = devAlloc<double4>(d.nRestraint, c->held);
+    upload(d.restraintAtoms, c->restraintAtoms, c->stream);
+    upload(d.restraintRef, ref, c->stream);
+  }
+  // Integrator units.
+  {
+    std::vector<int4> units, cons;
+    std::vector<double4> geom;
+    std::vector<int> conStart, atomStart, atoms;
+    std::vector<double> conD;
+    buildUnits(c, units, geom, conStart, atomStart, atoms, cons, conD);
+    d.nUnits = (int)units.size();
+    c->nUnits = d.nUnits;
+    d.units = devAlloc<int4>(units.size(), c->held);
+    d.waterGeom = devAlloc<double4>(geom.size(), c->held);
+    d.clusterConStart = devAlloc<int>(conStart.size(), c->held);
+    d.clusterAtomStart = devAlloc<int>(atomStart.size(), c->held);
+    d.clusterAtoms = devAlloc<int>(atoms.size(), c->held);
+    d.clusterCons = devAlloc<int4>(cons.size(), c->held);
+    d.clusterConD = devAlloc<double>(conD.size(), c->held);
+    d.clusterScratch = devAlloc<double>(12 * std::max<size_t>(1, atoms.size()), c->held);
+    upload(d.units, units, c->stream);
+    upload(d.waterGeom, geom, c->stream);
+    upload(d.clusterConStart, conStart, c->stream);
+    upload(d.clusterAtomStart, atomStart, c->stream);
+    upload(d.clusterAtoms, atoms, c->stream);
+    upload(d.clusterCons, cons, c->stream);
+    upload(d.clusterConD, conD, c->stream);
+  }
+  d.noise = devAlloc<float>(3 * (size_t)d.nPad, c->held);
+  d.slots = devAlloc<double>((size_t)E_TERM_COUNT * MAX_SLOTS, c->held);
+  d.sc = devAlloc<Scalars>(1, c->held);
+  c->scSaved = devAlloc<Scalars>(1, c->held);
+  c->dCounts = devAlloc<int>(E_TERM_COUNT, c->held);
+  c->dTerms = devAlloc<double>(E_TERM_COUNT, c->held);
+  c->dMaxForce = devAlloc<unsigned int>(1, c->held);
+  // Molecules for the barostat.
+  c->molStart = devAlloc<int>(sys.moleculeStart.size(), c->held);
+  c->molAtoms = devAlloc<int>(sys.moleculeAtoms.size(), c->held);
+  c->molOfAtom = devAlloc<int>(t.moleculeOfAtom.size(), c->held);
+  c->centroids = devAlloc<float4>(std::max(1, t.moleculeCount), c->held);
+  upload(c->molStart, sys.moleculeStart, c->stream);
+  upload(c->molAtoms, sys.moleculeAtoms, c->stream);
+  upload(c->molOfAtom, t.moleculeOfAtom, c->stream);
+  // PME.
+  PmeDev &p = c->p;
+  for (int k = 0; k < 3; ++k) p.K[k] = c->pme.grid[k];
+  p.K2h = p.K[2] / 2 + 1;
+  for (int k = 0; k < 3; ++k) {
+    if (p.K[k] > 1024) throw Error(RMD_E_INVALID, "PME grid sizes above 1024 are not supported");
+    std::vector<int> r = factorRadices(p.K[k]);
+    for (int q = 0; q < 8; ++q) p.radices[k][q] = q < (int)r.size() ? r[q] : 0;
+    std::vector<float2> tw(p.K[k]);
+    for (int q = 0; q < p.K[k]; ++q) {
+      const double ang = 2.0 * M_PI * q / p.K[k];
+      tw[q] = make_float2((float)std::cos(ang), (float)std::sin(ang));
+    }
+    p.twiddle[k] = devAlloc<float2>(tw.size(), c->held);
+    upload(p.twiddle[k], tw, c->stream);
+    c->moduli[k] = bsplineModuli(p.K[k], PME_ORDER);
+    c->dModuli[k] = devAlloc<double>(p.K[k], c->held);
+    upload(c->dModuli[k], c->moduli[k], c->stream);
+  }
+  const size_t grid = (size_t)p.K[0] * p.K[1] * p.K[2];
+  const size_t half = (size_t)p.K[0] * p.K[1] * p.K2h;
+  p.qgrid = devAlloc<long long>(grid, c->held);
+  p.rgrid = devAlloc<float>(grid, c->held);
+  p.cgrid = devAlloc<float2>(half, c->held);
+  p.influence = devAlloc<float>(half, c->held);
+  writeBox(c);
+  setupIntegrator(c, 0);
+  // Initial state from the system's own coordinates, if it has them.
+  HIP_CHECK(hipStreamSynchronize(c->stream));
+}
+
+void setState(rmd_ctx *c, const double *pos, const double *vel, int64_t step, int supplied) {
+  const System &s = c->sys;
+  const int n = s.topology.atomCount;
+  std::vector<float4> p(c->d.nPad, make_float4(0, 0, 0, 0)), v(c->d.nPad, make_float4(0, 0, 0, 0));
+  for (int a = 0; a < n; ++a) {
+    p[a] = make_float4((float)pos[3 * a], (float)pos[3 * a + 1], (float)pos[3 * a + 2], (float)s.topology.charges[a]);
+    const double m = s.topology.masses[a];
+    const float im = m > 0 ? (float)(1.0 / m) : 0.f;
+    if (vel)
+      v[a] = make_float4((float)vel[3 * a], (float)vel[3 * a + 1], (float)vel[3 * a + 2], im);
+    else
+      v[a] = make_float4(0, 0, 0, im);
+  }
+  upload(c->d.pos, p, c->stream);
+  upload(c->d.vel, v, c->stream);
+  HIP_CHECK(hipMemsetAsync(c->d.sc, 0, sizeof(Scalars), c->stream));
+  setStepKernel<<<1, 1, 0, c->stream>>>(c->d.sc, step);
+  setForceRebuildKernel<<<1, 1, 0, c->stream>>>(c->d.sc);
+  c->step = step;
+  c->kineticSampledAtStep = -1;
+  c->initialised = false;
+  if (supplied != c->ip.suppliedNoise) {
+    setupIntegrator(c, supplied);
+    invalidateGraphs(c);
+  }
+  HIP_CHECK(hipStreamSynchronize(c->stream));
+}
+
+}  // namespace
+
+// ---- C ABI ----------------------------------------------------------------
+
+#define CTX_GUARD(ctx, ...)                                   \
+  do {                                                         \
+    if (!(ctx)) {                                              \
+      rmd::setGlobalError("null context");                     \
+      return RMD_E_INVALID;                                    \
+    }                                                          \
+    try {                                                      \
+      HIP_CHECK(hipSetDevice((ctx)->device));                  \
+      __VA_ARGS__;                                             \
+      return RMD_OK;                                           \
+    } catch (const rmd::Error &e) {                            \
+      (ctx)->lastError = e.what();                             \
+      rmd::setGlobalError(e.what());                           \
+      return e.code;                                           \
+    } catch (const std::exception &e) {                        \
+      (ctx)->lastError = e.what();                             \
+      rmd::setGlobalError(e.what());                           \
+      return RMD_E_INVALID;                                    \
+    }                                                          \
+  } while (0)
+
+extern "C" {
+
+void rmd_options_default(rmd_options *o) {
+  std::memset(o, 0, sizeof *o);
+  o->timestep = 0.002;
+  o->temperature = 298;
+  o->friction = 1.0;
+  o->seed = 0;
+  o->tolerance = 1e-5;
+  o->max_constraint_iterations = 150;
+  o->rebuild_interval = 20;
+  o->adaptive_rebuild = 1;
+  o->barostat_interval = 25;
+  o->dispersion_correction = 1;
+  o->neighbour_slack = 2.0;
+  o->ewald_tolerance = 0.0005;
+}
+
+void rmd_min_options_default(rmd_min_options *o) {
+  o->max_iterations = 200;
+  o->force_tolerance = 500;
+  o->max_step = 0.002;
+  o->max_displacement = 0.05;
+}
+
+int rmd_create(const rmd_system *system, const rmd_options *options, rmd_ctx **out) {
+  if (!system || !out) {
+    setGlobalError("null argument");
+    return RMD_E_INVALID;
+  }
+  rmd_ctx *c = new rmd_ctx();
+  try {
+    rmd_options d;
+    rmd_options_default(&d);
+    create(c, system->s, options ? *options : d);
+    if (!system->s.positions.empty())
+      setState(c, system->s.positions.data(), system->s.velocities.empty() ? nullptr : system->s.velocities.data(), 0, 0);
+    *out = c;
+    return RMD_OK;
+  } catch (const rmd::Error &e) {
+    setGlobalError(e.what());
+    rmd_destroy(c);
+    return e.code;
+  } catch (const std::exception &e) {
+    setGlobalError(e.what());
+    rmd_destroy(c);
+    return RMD_E_INVALID;
+  }
+}
+
+void rmd_destroy(rmd_ctx *c) {
+  if (!c) return;
+  (void)hipSetDevice(c->device);
+  if (c->stream) (void)hipStreamSynchronize(c->stream);
+  invalidateGraphs(c);
+  for (void *p : c->held) (void)hipFree(p);
+  if (c->stream) (void)hipStreamDestroy(c->stream);
+  delete c;
+}
+
+const char *rmd_last_error(const rmd_ctx *c) { return c ? c->lastError.c_str() : rmd_last_global_error(); }
+
+int rmd_set_state(rmd_ctx *c, const double *pos, const double *vel, int64_t step, int32_t supplied) {
+  CTX_GUARD(c, {
+    if (!pos) throw Error(RMD_E_INVALID, "positions are required");
+    setState(c, pos, vel, step, supplied);
+  });
+}
+
+int rmd_set_noise(rmd_ctx *c, const float *noise) {
+  CTX_GUARD(c, {
+    if (!noise) throw Error(RMD_E_INVALID, "noise is required");
+    HIP_CHECK(hipMemcpy(c->d.noise, noise, 3 * (size_t)c->d.n * sizeof(float), hipMemcpyHostToDevice));
+  });
+}
+
+int rmd_initialise(rmd_ctx *c) {
+  CTX_GUARD(c, {
+    evaluate(c, RMD_TERM_ALL, true, nullptr);
+    c->initialised = true;
+  });
+}
+
+int rmd_request_stop(rmd_ctx *c) {
+  if (!c) return RMD_E_INVALID;
+  c->stop.store(1);
+  return RMD_OK;
+}
+
+int rmd_attempt_volume_change(rmd_ctx *c, rmd_move *m);
+
+int rmd_run(rmd_ctx *c, int64_t steps, int32_t block, int64_t *stepOut) {
+  CTX_GUARD(c, {
+    if (!c->initialised) {
+      evaluate(c, RMD_TERM_ALL, true, nullptr);
+      c->initialised = true;
+    }
+    const int blockSize = block > 0 ? block : (c->hasPressure ? c->barostatInterval : 25);
+    int64_t remaining = steps;
+    while (remaining > 0) {
+      if (c->stop.exchange(0)) throw Error(RMD_E_STOPPED, "the simulation was stopped between blocks");
+      if (c->hasPressure && c->stepsSinceBarostat >= c->barostatInterval) {
+        rmd_move mv;
+        const int rc = rmd_attempt_volume_change(c, &mv);
+        if (rc != RMD_OK) throw Error(rc, c->lastError);
+        c->stepsSinceBarostat = 0;
+      }
+      const int n = (int)std::min<int64_t>(blockSize, remaining);
+      runBlock(c, n);
+      remaining -= n;
+      c->stepsSinceBarostat += n;
+    }
+    if (stepOut) *stepOut = c->step;
+  });
+}
+
+int rmd_step_once(rmd_ctx *c) {
+  CTX_GUARD(c, {
+    if (!c->initialised) {
+      evaluate(c, RMD_TERM_ALL, true, nullptr);
+      c->initialised = true;
+    }
+    const bool graphs = c->useGraphs;
+    c->useGraphs = false;
+    try {
+      runBlock(c, 1);
+    } catch (...) {
+      c->useGraphs = graphs;
+      throw;
+    }
+    c->useGraphs = graphs;
+  });
+}
+
+int rmd_energy(rmd_ctx *c, rmd_energy_terms *out) {
+  CTX_GUARD(c, {
+    if (!out) throw Error(RMD_E_INVALID, "null output");
+    double t[E_TERM_COUNT];
+    evaluate(c, RMD_TERM_ALL, false, t);
+    int slots[E_TERM_COUNT];
+    std::memcpy(slots, c->slotCounts, sizeof slots);
+    slots[E_KINETIC] = launchKinetic(c->d, E_KINETIC, c->stream);
+    const bool earlyValid = c->kineticSampledAtStep == c->step;
+    slots[E_KINETIC_EARLY] = earlyValid ? (c->nUnits + 255) / 256 : 0;
+    HIP_CHECK(hipMemcpyAsync(c->dCounts, slots, sizeof slots, hipMemcpyHostToDevice, c->stream));
+    launchReduceEnergies(c->d, c->dCounts, c->dTerms, c->stream);
+    double all[E_TERM_COUNT];
+    HIP_CHECK(hipMemcpyAsync(all, c->dTerms, sizeof all, hipMemcpyDeviceToHost, c->stream));
+    HIP_CHECK(hipStreamSynchronize(c->stream));
+    fillEnergy(c, t, out, true, all[E_KINETIC], all[E_KINETIC_EARLY], earlyValid);
+    if (!std::isfinite(out->potential) || !std::isfinite(out->kinetic))
+      throw Error(RMD_E_NONFINITE, c->step == 0 ? "the first force evaluation is non-finite: the input itself "
+                                                  "cannot be evaluated"
+                                                : "the energy is non-finite at a reporting interval");
+  });
+}
+
+int rmd_evaluate(rmd_ctx *c, int32_t terms, rmd_energy_terms *energies, double *forces) {
+  CTX_GUARD(c, {
+    double t[E_TERM_COUNT];
+    evaluate(c, terms ? terms : RMD_TERM_ALL, true, t);
+    if (energies) fillEnergy(c, t, energies, false, 0, 0, false);
+    if (forces) toDouble(c, forces);
+  });
+}
+
+int rmd_read_positions(rmd_ctx *c, float *out) {
+  CTX_GUARD(c, {
+    std::vector<float4> p(c->d.nPad);
+    HIP_CHECK(hipMemcpyAsync(p.data(), c->d.pos, p.size() * sizeof(float4), hipMemcpyDeviceToHost, c->stream));
+    HIP_CHECK(hipStreamSynchronize(c->stream));
+    for (int a = 0; a < c->d.n; ++a) {
+      out[3 * a] = p[a].x;
+      out[3 * a + 1] = p[a].y;
+      out[3 * a + 2] = p[a].z;
+    }
+  });
+}
+
+int rmd_read_state(rmd_ctx *c, float *pos, float *vel, float *forces, double box[9], double *volume) {
+  CTX_GUARD(c, {
+    std::vector<float4> p(c->d.nPad), v(c->d.nPad);
+    HIP_CHECK(hipMemcpyAsync(p.data(), c->d.pos, p.size() * sizeof(float4), hipMemcpyDeviceToHost, c->stream));
+    HIP_CHECK(hipMemcpyAsync(v.data(), c->d.vel, v.size() * sizeof(float4), hipMemcpyDeviceToHost, c->stream));
+    HIP_CHECK(hipStreamSynchronize(c->stream));
+    for (int a = 0; a < c->d.n; ++a) {
+      if (pos) {
+        pos[3 * a] = p[a].x;
+        pos[3 * a + 1] = p[a].y;
+        pos[3 * a + 2] = p[a].z;
+      }
+      if (vel) {
+        vel[3 * a] = v[a].x;
+        vel[3 * a + 1] = v[a].y;
+        vel[3 * a + 2] = v[a].z;
+      }
+    }
+    if (forces) {
+      std::vector<double> f(3 * (size_t)c->d.n);
+      toDouble(c, f.data());
+      for (size_t k = 0; k < f.size(); ++k) forces[k] = (float)f[k];
+    }
+    if (box)
+      for (int k = 0; k < 9; ++k) box[k] = c->box[k];
+    if (volume) *volume = volumeOf(c->box);
+  });
+}
+
+int rmd_scale_volume(rmd_ctx *c, double s, double *potential) {
+  CTX_GUARD(c, {
+    const double u = scaleVolume(c, s);
+    if (potential) *potential = u;
+  });
+}
+
+int rmd_attempt_volume_change(rmd_ctx *c, rmd_move *m) {
+  CTX_GUARD(c, {
+    if (!m) throw Error(RMD_E_INVALID, "null output");
+    std::memset(m, 0, sizeof *m);
+    if (!c->hasPressure) return RMD_OK;
+    const double volume = volumeOf(c->box);
+    if (c->volumeScale < 0) c->volumeScale = 0.01 * volume;
+    const double kT = BOLTZMANN * c->temperature;
+    const double pressure = c->pressure * BAR_TO_KJ_PER_MOL_NM3;
+    const double oldPotential = evaluatePotential(c);
+    const size_t np = c->d.nPad;
+    HIP_CHECK(hipMemcpyAsync(c->posSaved, c->d.pos, np * sizeof(float4), hipMemcpyDeviceToDevice, c->stream));
+    HIP_CHECK(hipMemcpyAsync(c->forceSaved, c->d.force, 3 * np * sizeof(long long), hipMemcpyDeviceToDevice, c->stream));
+    c->barostatAttempts += 1;
+    c->windowAttempts += 1;
+    const uint32_t bseed = c->seed ^ 0x5a17u;
+    const double draw = uniformAt(bseed, (uint32_t)c->barostatAttempts, 0, 0);
+    const double deltaVolume = c->volumeScale * 2 * (draw - 0.5);
+    const double newVolume = volume + deltaVolume;
+    m->attempted = 1;
+    if (!(newVolume > 0)) return RMD_OK;
+    const double s = std::cbrt(newVolume / volume);
+    const double smallest = s * std::min({std::fabs(c->box[0]), std::fabs(c->box[4]), std::fabs(c->box[8])});
+    m->delta_volume = deltaVolume;
+    m->length_scale = s;
+    m->volume = volume;
+    if (c->sys.cutoff > 0.5 * smallest) {
+      c->barostatRefusals += 1;
+      m->refused = 1;
+      m->volume_scale = c->volumeScale;
+      m->acceptance_rate = (double)c->barostatAccepted / c->barostatAttempts;
+      return RMD_OK;
+    }
+    const double newPotential = scaleVolume(c, s);
+    const double weight = (newPotential - oldPotential) + pressure * deltaVolume -
+                          c->sys.topology.moleculeCount * kT * std::log(newVolume / volume);
+    const bool accept = weight <= 0 || uniformAt(bseed, (uint32_t)c->barostatAttempts, 1, 0) < std::exp(-weight / kT);
+    if (accept) {
+      c->barostatAccepted += 1;
+      c->windowAccepted += 1;
+    } else {
+      HIP_CHECK(hipMemcpyAsync(c->d.pos, c->posSaved, np * sizeof(float4), hipMemcpyDeviceToDevice, c->stream));
+      double nb[9];
+      for (int k = 0; k < 9; ++k) nb[k] = c->box[k] / s;
+      updateBox(c, nb);
+      HIP_CHECK(hipMemcpyAsync(c->d.force, c->forceSaved, 3 * np * sizeof(long long), hipMemcpyDeviceToDevice, c->stream));
+      HIP_CHECK(hipStreamSynchronize(c->stream));
+    }
+    if (c->windowAttempts >= 10) {
+      const double rate = (double)c->windowAccepted / c->windowAttempts;
+      if (rate < 0.25)
+        c->volumeScale /= 1.1;
+      else if (rate > 0.75)
+        c->volumeScale = std::min(c->volumeScale * 1.1, 0.3 * volume);
+      c->windowAttempts = c->windowAccepted = 0;
+    }
+    m->accepted = accept;
+    m->weight = weight;
+    m->old_potential = oldPotential;
+    m->new_potential = newPotential;
+    m->volume = volumeOf(c->box);
+    m->volume_scale = c->volumeScale;
+    m->acceptance_rate = (double)c->barostatAccepted / c->barostatAttempts;
+  });
+}
+
+int rmd_minimise(rmd_ctx *c, const rmd_min_options *options, rmd_min_result *out) {
+  CTX_GUARD(c, {
+    rmd_min_options o;
+    rmd_min_options_default(&o);
+    if (options) o = *options;
+    double maxStep = o.max_step;
+    const size_t np = c->d.nPad;
+    auto maxForce = [&]() {
+      HIP_CHECK(hipMemsetAsync(c->dMaxForce, 0, sizeof(unsigned int), c->stream));
+      launchMaxForce(c->d, c->dMaxForce, c->stream);
+      unsigned int bits = 0;
+      HIP_CHECK(hipMemcpyAsync(&bits, c->dMaxForce, sizeof bits, hipMemcpyDeviceToHost, c->stream));
+      HIP_CHECK(hipStreamSynchronize(c->stream));
+      return (double)__builtin_bit_cast(float, bits);
+    };
+    double potential = evaluatePotential(c);
+    int iterations = 0, rejected = 0;
+    bool stalled = false;
+    double worst = 0;
+    for (int it = 0; it < o.max_iterations; ++it) {
+      iterations = it + 1;
+      worst = maxForce();
+      if (worst <= o.force_tolerance) break;
+      HIP_CHECK(hipMemcpyAsync(c->posSaved, c->d.pos, np * sizeof(float4), hipMemcpyDeviceToDevice, c->stream));
+      HIP_CHECK(hipMemcpyAsync(c->forceSaved, c->d.force, 3 * np * sizeof(long long), hipMemcpyDeviceToDevice, c->stream));
+      bool accepted = false;
+      for (int attempt = 0; attempt < 14; ++attempt) {
+        launchDisplace(c->d, (float)(maxStep / std::max(worst, 1e-6)), (float)o.max_displacement, c->stream);
+        launchConstrainPositions(c->d, c->posSaved, c->ip, c->stream);
+        const double trial = evaluatePotential(c);
+        if (trial < potential) {
+          potential = trial;
+          maxStep *= 1.3;
+          accepted = true;
+          break;
+        }
+        rejected += 1;
+        // Back to the pre-move positions AND forces: the next, smaller step
+        // goes along the gradient at the base point.
+        HIP_CHECK(hipMemcpyAsync(c->d.pos, c->posSaved, np * sizeof(float4), hipMemcpyDeviceToDevice, c->stream));
+        HIP_CHECK(hipMemcpyAsync(c->d.force, c->forceSaved, 3 * np * sizeof(long long), hipMemcpyDeviceToDevice, c->stream));
+        maxStep *= 0.5;
+        if (maxStep < 1e-9) break;
+      }
+      if (!accepted) {
+        stalled = true;
+        break;
+      }
+    }
+    worst = maxForce();
+    // Velocities are zeroed, as the reference does after minimising.
+    std::vector<float4> v(np);
+    HIP_CHECK(hipMemcpyAsync(v.data(), c->d.vel, np * sizeof(float4), hipMemcpyDeviceToHost, c->stream));
+    HIP_CHECK(hipStreamSynchronize(c->stream));
+    for (auto &x : v) x.x = x.y = x.z = 0.f;
+    HIP_CHECK(hipMemcpyAsync(c->d.vel, v.data(), np * sizeof(float4), hipMemcpyHostToDevice, c->stream));
+    HIP_CHECK(hipStreamSynchronize(c->stream));
+    c->initialised = true;
+    if (out) {
+      out->iterations = iterations;
+      out->rejected = rejected;
+      out->stalled = stalled;
+      out->converged = worst <= o.force_tolerance;
+      out->potential = potential;
+      out->max_force = worst;
+      out->max_step = maxStep;
+      out->energy_resolution = std::fabs(potential) * std::pow(2.0, -24) * std::sqrt((double)c->d.n);
+    }
+  });
+}
+
+int rmd_diagnostics_get(rmd_ctx *c, rmd_diagnostics *o) {
+  CTX_GUARD(c, {
+    std::memset(o, 0, sizeof *o);
+    Scalars h;
+    HIP_CHECK(hipMemcpy(&h, c->d.sc, sizeof h, hipMemcpyDeviceToHost));
+    o->max_shake_iterations = (int32_t)h.shakeMaxIterations;
+    o->shake_hit_iteration_cap = h.shakeFailed ? 1 : 0;
+    o->rattle_hit_iteration_cap = h.rattleFailed ? 1 : 0;
+    o->neighbour_required = (int32_t)h.required;
+    o->neighbour_capacity = c->d.cap;
+    o->worst_neighbour_required = (int32_t)c->worstRequired;
+    o->neighbour_growths = c->neighbourGrowths;
+    o->skin = c->plan.skin;
+    o->worst_displacement = std::sqrt(std::max(0.0f, __builtin_bit_cast(float, h.maxDisp2Bits)));
+    o->rebuilds = h.rebuilds;
+    o->steps = c->stepsRun;
+    for (int k = 0; k < 3; ++k) {
+      o->cell_counts[k] = c->plan.counts[k];
+      o->pme_grid[k] = c->pme.grid[k];
+    }
+    o->volume = volumeOf(c->box);
+    o->barostat_attempts = c->barostatAttempts;
+    o->barostat_accepted = c->barostatAccepted;
+    o->barostat_refused = c->barostatRefusals;
+    o->volume_scale = c->volumeScale;
+    o->constraint_tolerance = c->tolerance;
+    o->pme_alpha = c->pme.alpha;
+    o->lanes_per_atom = c->lanes;
+    o->water_units = c->waterUnits;
+    o->cluster_units = c->clusterUnits;
+    o->free_units = c->freeUnits;
+  });
+}
+
+int rmd_read_cells(rmd_ctx *c, int32_t *cellOfAtom, int32_t *cellStart, int32_t *cellAtoms) {
+  CTX_GUARD(c, {
+    HIP_CHECK(hipStreamSynchronize(c->stream));
+    if (cellOfAtom) HIP_CHECK(hipMemcpy(cellOfAtom, c->d.cellOfAtom, c->d.n * sizeof(int), hipMemcpyDeviceToHost));
+    if (cellStart)
+      HIP_CHECK(hipMemcpy(cellStart, c->d.cellStart, (c->plan.cellCount + 1) * sizeof(int), hipMemcpyDeviceToHost));
+    if (cellAtoms) HIP_CHECK(hipMemcpy(cellAtoms, c->d.cellAtoms, c->d.n * sizeof(int), hipMemcpyDeviceToHost));
+  });
+}
+
+int rmd_read_neighbours(rmd_ctx *c, int32_t *counts, int32_t *lists) {
+  CTX_GUARD(c, {
+    HIP_CHECK(hipStreamSynchronize(c->stream));
+    if (counts) HIP_CHECK(hipMemcpy(counts, c->d.nbrCount, c->d.n * sizeof(int), hipMemcpyDeviceToHost));
+    if (lists)
+      HIP_CHECK(hipMemcpy(lists, c->d.nbrList, (size_t)c->d.n * c->d.cap * sizeof(int), hipMemcpyDeviceToHost));
+  });
+}
+
+int rmd_device_info_get(int32_t device, rmd_device_info *out) {
+  if (!out) return RMD_E_INVALID;
+  std::memset(out, 0, sizeof *out);
+  int count = 0;
+  if (hipGetDeviceCount(&count) != hipSuccess) count = 0;
+  out->device_count = count;
+  if (count <= 0 || device >= count) {
+    setGlobalError("no HIP device is available");
+    return RMD_E_DEVICE;
+  }
+  hipDeviceProp_t prop;
+  if (hipGetDeviceProperties(&prop, device) != hipSuccess) return RMD_E_DEVICE;
+  std::strncpy(out->name, prop.name, sizeof out->name - 1);
+  std::strncpy(out->arch, prop.gcnArchName, sizeof out->arch - 1);
+  out->compute_units = prop.multiProcessorCount;
+  out->global_memory = (int64_t)prop.totalGlobalMem;
+  return RMD_OK;
+}
+
+// Verification entry: the forward real-to-complex 3D transform of a real grid
+// with the context's PME plan. out holds K0*K1*(K2/2+1) complex values.
+int rmd_debug_fft_forward(rmd_ctx *c, const float *real, float *out) {
+  CTX_GUARD(c, {
+    const size_t n = (size_t)c->p.K[0] * c->p.K[1] * c->p.K[2];
+    const size_t nc = (size_t)c->p.K[0] * c->p.K[1] * c->p.K2h;
+    std::vector<void *> tmp;
+    float *dr = devAlloc<float>(n, tmp);
+    float2 *dc = devAlloc<float2>(nc, tmp);
+    HIP_CHECK(hipMemcpy(dr, real, n * sizeof(float), hipMemcpyHostToDevice));
+    launchFftForward(c->p, dr, dc, c->stream);
+    HIP_CHECK(hipStreamSynchronize(c->stream));
+    HIP_CHECK(hipMemcpy(out, dc, nc * sizeof(float2), hipMemcpyDeviceToHost));
+    for (void *p : tmp) (void)hipFree(p);
+  });
+}
+
+int rmd_set_lanes(rmd_ctx *c, int32_t lanes) {
+  CTX_GUARD(c, {
+    if (lanes <= 0 || lanes > 64 || (lanes & (lanes - 1))) throw Error(RMD_E_INVALID, "lanes must be a power of two <= 64");
+    c->lanes = lanes;
+    invalidateGraphs(c);
+  });
+}
+
+int rmd_set_graphs(rmd_ctx *c, int32_t enabled) {
+  CTX_GUARD(c, {
+    c->useGraphs = enabled != 0;
+    invalidateGraphs(c);
+  });
+}
+
+int rmd_get_stream(rmd_ctx *c, void **stream) {
+  CTX_GUARD(c, { *stream = (void *)c->stream; });
+}
+
+}  // extern "C"

@@ -1,0 +1,126 @@
+// Device-side helpers shared by every kernel file: fixed-point force
+// accumulation, the FP32 minimum-image distance test (in a fixed operation
+// order, without contraction, so a CPU emulation reproduces it bit for bit),
+// an FP32 erfc/erf, and wave64 reductions.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace rmd {
+
+// Forces are accumulated as int64 at 2^32 units per kJ/mol/nm: order
+// independent, so every run gives the same bits.
+constexpr double FORCE_SCALE = 4294967296.0;
+constexpr double FORCE_UNSCALE = 1.0 / 4294967296.0;
+// PME charge grid: int64 at 2^32 units per e.
+constexpr double GRID_SCALE = 4294967296.0;
+constexpr double GRID_UNSCALE = 1.0 / 4294967296.0;
+
+constexpr float ONE_4PI_EPS0_F = 138.935456f;
+constexpr float TWO_OVER_SQRT_PI_F = 1.1283791670955126f;
+
+// Device geometry for one box, kept in device memory so a captured block of
+// steps reads the current box: f32 rows, per-axis inverse widths (the image
+// test multiplies by them), the f32 reciprocal rows for fractional
+// coordinates, the squared cutoffs, the FP64 box and reciprocal rows, and the
+// cell-grid shape.
+struct BoxF {
+  float a[3], b[3], c[3];
+  float inv[3];
+  float recip[9];
+  float rc2, rl2;
+  float cutoff;
+  int periodic;
+  double box64[9];
+  double recip64[9];
+  int cells[3];
+  int cellTotal;
+};
+
+__device__ __forceinline__ long long toFixed(double v) { return static_cast<long long>(__double2ll_rn(v * FORCE_SCALE)); }
+
+__device__ __forceinline__ void atomicAddFixed(long long *address, long long value) {
+  atomicAdd(reinterpret_cast<unsigned long long *>(address), static_cast<unsigned long long>(value));
+}
+
+// Minimum image and squared distance. The order of operations is fixed and
+// contraction is off, so oracle_r2_f32 reproduces every bit: image along c,
+// then b, then a, with s = floor(d * inv + 0.5) (the reference's kernel
+// order, webgpu/kernels/wgsl-common.js:81-88, with the division replaced by
+// a multiply by the f32 inverse width).
+__device__ __forceinline__ float minimumImageR2(float &dx, float &dy, float &dz, const BoxF &b) {
+#pragma clang fp contract(off)
+  float s2 = floorf(dz * b.inv[2] + 0.5f);
+  dx = dx - s2 * b.c[0];
+  dy = dy - s2 * b.c[1];
+  dz = dz - s2 * b.c[2];
+  float s1 = floorf(dy * b.inv[1] + 0.5f);
+  dx = dx - s1 * b.b[0];
+  dy = dy - s1 * b.b[1];
+  float s0 = floorf(dx * b.inv[0] + 0.5f);
+  dx = dx - s0 * b.a[0];
+  float xx = dx * dx;
+  float yy = dy * dy;
+  float zz = dz * dz;
+  float sum = xx + yy;
+  return sum + zz;
+}
+
+// erfc(x) for x >= 0 as exp(-x^2) * t * P(t), t = 1/(1 + x/2). P is a
+// degree-8 relative-error minimax fit of erfcx(x)/t over x in [0, 4] made for
+// this file (max fit error 2e-10; the f32 evaluation is ~1e-7 rms). x^2 is
+// split into hi + lo with an FMA so the exponential's argument is exact.
+// Also returns exp(-x^2), which the force needs.
+__device__ __forceinline__ float erfcWithGaussian(float x, float &gaussian) {
+  const float t = 1.0f / fmaf(0.5f, x, 1.0f);
+  float p = -4.261296242e-02f;
+  p = fmaf(p, t, 1.989050657e-01f);
+  p = fmaf(p, t, -2.959875166e-01f);
+  p = fmaf(p, t, 3.936168179e-02f);
+  p = fmaf(p, t, 1.554754525e-01f);
+  p = fmaf(p, t, 1.158383936e-01f);
+  p = fmaf(p, t, 2.683742046e-01f);
+  p = fmaf(p, t, 2.782751918e-01f);
+  p = fmaf(p, t, 2.823704481e-01f);
+  const float hi = x * x;
+  const float lo = fmaf(x, x, -hi);
+  const float e = expf(-hi) * (1.0f - lo);
+  gaussian = e;
+  return (e * t) * p;
+}
+
+// erf(x) for x >= 0: an odd polynomial x*Q(x^2) below 0.8 (a degree-5
+// relative-error fit in x^2 made for this file, max fit error 3e-9), and
+// 1 - erfc above.
+__device__ __forceinline__ float erfWithGaussian(float x, float &gaussian) {
+  if (x < 0.8f) {
+    const float x2 = x * x;
+    float s = -6.535415887e-04f;
+    s = fmaf(s, x2, 5.085162353e-03f);
+    s = fmaf(s, x2, -2.682082728e-02f);
+    s = fmaf(s, x2, 1.128311306e-01f);
+    s = fmaf(s, x2, -3.761260211e-01f);
+    s = fmaf(s, x2, 1.128379107e+00f);
+    const float lo = fmaf(x, x, -x2);
+    gaussian = expf(-x2) * (1.0f - lo);
+    return x * s;
+  }
+  float g;
+  const float c = erfcWithGaussian(x, g);
+  gaussian = g;
+  return 1.0f - c;
+}
+
+// Wave64 reductions.
+template <typename T>
+__device__ __forceinline__ T waveSum(T v) {
+  for (int offset = 32; offset > 0; offset >>= 1) v += __shfl_xor(v, offset, 64);
+  return v;
+}
+__device__ __forceinline__ float waveMax(float v) {
+  for (int offset = 32; offset > 0; offset >>= 1) v = fmaxf(v, __shfl_xor(v, offset, 64));
+  return v;
+}
+
+}  // namespace rmd

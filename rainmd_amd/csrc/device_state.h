@@ -1,0 +1,94 @@
+// The device-resident state every kernel reads, passed by value as a kernel
+// argument (a few hundred bytes of pointers and sizes).
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "../../include/rainmd.h"
+#include "device_common.h"
+
+namespace rmd {
+
+// Energy accumulation: each kernel block that computes energies writes one
+// FP64 partial into its own slot of a term's row; a reduction kernel sums
+// each row in a fixed order, so energies are bit-reproducible.
+enum EnergyTerm {
+  E_LJ_DIRECT = 0,
+  E_COUL_DIRECT,
+  E_LJ_EXCEPTION,
+  E_COUL_EXCEPTION,
+  E_BOND,
+  E_ANGLE,
+  E_TORSION,
+  E_RESTRAINT,
+  E_RECIPROCAL,
+  E_KINETIC,
+  E_KINETIC_EARLY,
+  E_TERM_COUNT
+};
+constexpr int MAX_SLOTS = 65536;
+
+// Scalars that live on the device so a captured block never needs the host.
+struct Scalars {
+  unsigned int maxDisp2Bits;  // max |x - x_rebuild|^2 since the last rebuild (float bits)
+  unsigned int rebuild;       // this step rebuilds the cell and neighbour lists
+  unsigned int forceRebuild;  // set by the host: rebuild at the next decision
+  unsigned int required;      // neighbour requirement seen at the last rebuild
+  unsigned int ticket;        // last-block ticket of the integrator
+  unsigned int rebuilds;      // rebuild count
+  unsigned int shakeMaxIterations;
+  unsigned int shakeFailed;
+  unsigned int rattleFailed;
+  unsigned int cellOverflow;  // a cell held more atoms than the sort can take
+  unsigned int nonFinite;     // a non-finite coordinate was seen
+  unsigned int requiredMax;   // largest requirement since the host last cleared it
+  long long step;             // the noise counter (completed steps)
+};
+
+struct Dev {
+  int n;          // atoms
+  int nPad;       // stride of the three force planes
+  float4 *pos;    // x, y, z, charge
+  float4 *vel;    // vx, vy, vz, inverse mass
+  long long *force;  // 3 planes of nPad fixed-point components
+  float4 *posRebuild;
+  int *type;
+  float2 *lj;     // (A, B) per type pair
+  int types;
+  const BoxF *box;  // device-resident geometry (see BoxF)
+  // cells (arrays sized for cellCapacity cells)
+  int *cellOfAtom, *cellCount, *cellStart, *cellCursor, *cellAtoms;
+  int cellCapacity;
+  // neighbours: full list, `cap` slots per atom
+  int *nbrCount, *nbrList;
+  int cap;
+  int *exclStart, *exclPartners;
+  // bonded terms (f32 parameters)
+  int nBond, nAngle, nTorsion, nException, nRestraint;
+  int2 *bondAtoms;
+  double2 *bondParams;      // k, r0 (FP64: r - r0 cancels)
+  int4 *angleAtoms;         // a, b, c
+  double2 *angleParams;     // k, theta0
+  int4 *torsionAtoms;
+  double4 *torsionParams;   // k, n, phase
+  int2 *excAtoms;
+  float4 *excParams;        // qq scaled (with 1/4pi eps0), A, B, full qq (with 1/4pi eps0)
+  int *restraintAtoms;
+  double4 *restraintRef;    // reference xyz and k, FP64
+  // integrator units
+  int nUnits;
+  int4 *units;              // kind, a, b, c (see integrate.hip)
+  double4 *waterGeom;       // dOH, dHH, mO, mH per unit (used by water units)
+  int *clusterConStart, *clusterAtomStart, *clusterAtoms;
+  int4 *clusterCons;        // local slot a, local slot b, constraint index, unused
+  double *clusterConD;      // constrained length per cluster constraint
+  double *clusterScratch;   // 12 doubles per cluster atom slot
+  // noise
+  float *noise;             // 3N supplied normals (optional)
+  // energies and scalars
+  double *slots;            // E_TERM_COUNT * MAX_SLOTS
+  Scalars *sc;
+};
+
+}  // namespace rmd

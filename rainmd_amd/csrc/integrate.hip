@@ -1,0 +1,731 @@
+// The BAOAB (LangevinMiddle) integrator with constraints, fused per
+// constraint unit, plus the kinetic-energy and energy reductions, the
+// barostat's centroid scaling and the minimiser's helpers.
+//
+// Replaces encodeLangevinStep and its kernels (webgpu/kernels/integrate.js:
+// kick 126-147, drift 158-173, shake 188-334, rattle 337-398, thermostat
+// 406-437, kinetic 440-453, step counter 456-464) and the barostat kernels
+// (webgpu/kernels/barostat.js:25-107). The step order is the reference's
+// (integrate.js:523-557; cpu/integrate/langevin.js:30-114):
+//   B R A S O R A S [forces] B R
+// Every constraint cluster is independent, so one thread owns one unit - a
+// free atom, a rigid water (SETTLE, closed form, FP64) or a general cluster
+// (iterative SHAKE/RATTLE in FP64) - and runs the whole B R A S O R A S
+// sequence for it in registers with no synchronisation. The closing B R of
+// step n is folded into the start of step n+1's launch, so a step is this
+// kernel plus the force kernels.
+#include "kernels.h"
+
+namespace rmd {
+
+enum { UNIT_FREE = 0, UNIT_WATER = 1, UNIT_CLUSTER = 2 };
+
+struct Atom {
+  double x[3], v[3], f[3];
+  double im;  // inverse mass (0 for massless)
+};
+
+__device__ __forceinline__ void loadAtom(const Dev &d, int a, Atom &s) {
+  const float4 p = d.pos[a], v = d.vel[a];
+  s.x[0] = p.x;
+  s.x[1] = p.y;
+  s.x[2] = p.z;
+  s.v[0] = v.x;
+  s.v[1] = v.y;
+  s.v[2] = v.z;
+  s.im = v.w;
+  s.f[0] = (double)d.force[a] * FORCE_UNSCALE;
+  s.f[1] = (double)d.force[d.nPad + a] * FORCE_UNSCALE;
+  s.f[2] = (double)d.force[2 * d.nPad + a] * FORCE_UNSCALE;
+}
+
+// Stores position and velocity, clears the force accumulators for the coming
+// evaluation and returns |x - x_rebuild|^2 in f32.
+__device__ __forceinline__ float storeAtom(const Dev &d, int a, const Atom &s, bool clearForce) {
+  const float4 old = d.pos[a];
+  const float4 p = make_float4((float)s.x[0], (float)s.x[1], (float)s.x[2], old.w);
+  d.pos[a] = p;
+  d.vel[a] = make_float4((float)s.v[0], (float)s.v[1], (float)s.v[2], (float)s.im);
+  if (clearForce) {
+    d.force[a] = 0;
+    d.force[d.nPad + a] = 0;
+    d.force[2 * d.nPad + a] = 0;
+  }
+  const float4 r = d.posRebuild[a];
+  const float dx = p.x - r.x, dy = p.y - r.y, dz = p.z - r.z;
+  return dx * dx + dy * dy + dz * dz;
+}
+
+__device__ __forceinline__ void kick(Atom &s, double h) {
+  if (s.im <= 0) return;
+  for (int c = 0; c < 3; ++c) s.v[c] += h * s.im * s.f[c];
+}
+
+__device__ __forceinline__ double kinetic(const Atom &s) {
+  if (s.im <= 0) return 0.0;
+  return 0.5 / s.im * (s.v[0] * s.v[0] + s.v[1] * s.v[1] + s.v[2] * s.v[2]);
+}
+
+// shared/random.js:24-87, in integers (bit-identical to the host) and FP64.
+__device__ __forceinline__ unsigned int mix(unsigned int v) {
+  v ^= v >> 16;
+  v *= 0x7feb352du;
+  v ^= v >> 15;
+  v *= 0x846ca68bu;
+  v ^= v >> 16;
+  return v;
+}
+__device__ __forceinline__ double uniformAt(unsigned int seed, unsigned int a, unsigned int b, unsigned int c) {
+  unsigned int st = mix(seed ^ (a * 0x9e3779b1u));
+  st = mix(st ^ (b * 0x85ebca6bu));
+  st = mix(st ^ (c * 0xc2b2ae35u));
+  return ((double)(st >> 9) + 0.5) * (1.0 / 8388608.0);
+}
+
+// O: v <- a v + sqrt(1 - a^2) sqrt(kT/m) xi (integrate.js:406-437).
+__device__ __forceinline__ void thermostat(const Dev &d, const IntegratorParams &ip, int a, Atom &s,
+                                           unsigned int step) {
+  if (s.im <= 0) return;
+  double g[3];
+  if (ip.suppliedNoise) {
+    for (int c = 0; c < 3; ++c) g[c] = d.noise[3 * a + c];
+  } else {
+    const double u1 = uniformAt(ip.seed, a, step, 0), u2 = uniformAt(ip.seed, a, step, 1);
+    const double u3 = uniformAt(ip.seed, a, step, 2), u4 = uniformAt(ip.seed, a, step, 3);
+    const double r1 = sqrt(-2.0 * log(u1)), r2 = sqrt(-2.0 * log(u3));
+    g[0] = r1 * cos(2.0 * M_PI * u2);
+    g[1] = r1 * sin(2.0 * M_PI * u2);
+    g[2] = r2 * cos(2.0 * M_PI * u4);
+  }
+  const double amp = ip.noiseScale * sqrt(ip.kT * s.im);
+  for (int c = 0; c < 3; ++c) s.v[c] = ip.decay * s.v[c] + amp * g[c];
+}
+
+// ---- rigid water ----------------------------------------------------------
+
+// SETTLE (Miyamoto & Kollman, J. Comput. Chem. 13, 952 (1992)): the rigid
+// water's new positions in closed form, with the constraint displacements
+// along the old bond vectors - the limit SHAKE converges to
+// (cpu/integrate/constraints.js:93-171).
+__device__ void settle(const double o[9], double n[9], double mO, double mH, double dOH, double dHH) {
+  const double wohh = mO + 2.0 * mH;
+  const double rc = 0.5 * dHH;
+  const double hgt = sqrt(dOH * dOH - rc * rc);
+  const double ra = 2.0 * mH * hgt / wohh, rb = hgt - ra;
+  double b0[3], c0[3], com[3], a1[3], b1[3], c1[3];
+  for (int k = 0; k < 3; ++k) {
+    b0[k] = o[3 + k] - o[k];
+    c0[k] = o[6 + k] - o[k];
+    com[k] = (mO * n[k] + mH * (n[3 + k] + n[6 + k])) / wohh;
+    a1[k] = n[k] - com[k];
+    b1[k] = n[3 + k] - com[k];
+    c1[k] = n[6 + k] - com[k];
+  }
+  double ez[3] = {b0[1] * c0[2] - b0[2] * c0[1], b0[2] * c0[0] - b0[0] * c0[2], b0[0] * c0[1] - b0[1] * c0[0]};
+  double ex[3] = {a1[1] * ez[2] - a1[2] * ez[1], a1[2] * ez[0] - a1[0] * ez[2], a1[0] * ez[1] - a1[1] * ez[0]};
+  double ey[3] = {ez[1] * ex[2] - ez[2] * ex[1], ez[2] * ex[0] - ez[0] * ex[2], ez[0] * ex[1] - ez[1] * ex[0]};
+  const double lx = rsqrt(ex[0] * ex[0] + ex[1] * ex[1] + ex[2] * ex[2]);
+  const double ly = rsqrt(ey[0] * ey[0] + ey[1] * ey[1] + ey[2] * ey[2]);
+  const double lz = rsqrt(ez[0] * ez[0] + ez[1] * ez[1] + ez[2] * ez[2]);
+  for (int k = 0; k < 3; ++k) {
+    ex[k] *= lx;
+    ey[k] *= ly;
+    ez[k] *= lz;
+  }
+  auto dot = [](const double *u, const double *v) { return u[0] * v[0] + u[1] * v[1] + u[2] * v[2]; };
+  const double xb0 = dot(b0, ex), yb0 = dot(b0, ey), xc0 = dot(c0, ex), yc0 = dot(c0, ey);
+  const double za1 = dot(a1, ez);
+  const double xb1 = dot(b1, ex), yb1 = dot(b1, ey), zb1 = dot(b1, ez);
+  const double xc1 = dot(c1, ex), yc1 = dot(c1, ey), zc1 = dot(c1, ez);
+  const double sinphi = za1 / ra;
+  const double cosphi = sqrt(fmax(0.0, 1.0 - sinphi * sinphi));
+  const double sinpsi = (zb1 - zc1) / (2.0 * rc * cosphi);
+  const double cospsi = sqrt(fmax(0.0, 1.0 - sinpsi * sinpsi));
+  const double ya2 = ra * cosphi, xb2 = -rc * cospsi;
+  const double t1 = -rb * cosphi, t2 = rc * sinpsi * sinphi;
+  const double yb2 = t1 - t2, yc2 = t1 + t2;
+  const double alpa = xb2 * (xb0 - xc0) + yb0 * yb2 + yc0 * yc2;
+  const double beta = xb2 * (yc0 - yb0) + xb0 * yb2 + xc0 * yc2;
+  const double gama = xb0 * yb1 - xb1 * yb0 + xc0 * yc1 - xc1 * yc0;
+  const double al2be2 = alpa * alpa + beta * beta;
+  const double sinthe = (alpa * gama - beta * sqrt(fmax(0.0, al2be2 - gama * gama))) / al2be2;
+  const double costhe = sqrt(fmax(0.0, 1.0 - sinthe * sinthe));
+  const double a3[3] = {-ya2 * sinthe, ya2 * costhe, za1};
+  const double b3[3] = {xb2 * costhe - yb2 * sinthe, xb2 * sinthe + yb2 * costhe, zb1};
+  const double c3[3] = {-xb2 * costhe - yc2 * sinthe, -xb2 * sinthe + yc2 * costhe, zc1};
+  for (int k = 0; k < 3; ++k) {
+    n[k] = com[k] + a3[0] * ex[k] + a3[1] * ey[k] + a3[2] * ez[k];
+    n[3 + k] = com[k] + b3[0] * ex[k] + b3[1] * ey[k] + b3[2] * ez[k];
+    n[6 + k] = com[k] + c3[0] * ex[k] + c3[1] * ey[k] + c3[2] * ez[k];
+  }
+}
+
+// RATTLE for a rigid water, exactly: the three bond multipliers that remove
+// every relative velocity along a bond solve a 3x3 system.
+__device__ void rattleWater(Atom *w) {
+  double r01[3], r02[3], r12[3], v01[3], v02[3], v12[3];
+  for (int c = 0; c < 3; ++c) {
+    r01[c] = w[0].x[c] - w[1].x[c];
+    r02[c] = w[0].x[c] - w[2].x[c];
+    r12[c] = w[1].x[c] - w[2].x[c];
+    v01[c] = w[0].v[c] - w[1].v[c];
+    v02[c] = w[0].v[c] - w[2].v[c];
+    v12[c] = w[1].v[c] - w[2].v[c];
+  }
+  auto dot = [](const double *u, const double *v) { return u[0] * v[0] + u[1] * v[1] + u[2] * v[2]; };
+  const double iO = w[0].im, i1 = w[1].im, i2 = w[2].im;
+  // dv0 = (l0 r01 + l1 r02) iO; dv1 = (-l0 r01 + l2 r12) i1; dv2 = (-l1 r02 - l2 r12) i2
+  const double A[3][3] = {
+      {(iO + i1) * dot(r01, r01), iO * dot(r01, r02), -i1 * dot(r01, r12)},
+      {iO * dot(r02, r01), (iO + i2) * dot(r02, r02), i2 * dot(r02, r12)},
+      {-i1 * dot(r12, r01), i2 * dot(r12, r02), (i1 + i2) * dot(r12, r12)}};
+  const double bvec[3] = {-dot(r01, v01), -dot(r02, v02), -dot(r12, v12)};
+  const double det = A[0][0] * (A[1][1] * A[2][2] - A[1][2] * A[2][1]) -
+                     A[0][1] * (A[1][0] * A[2][2] - A[1][2] * A[2][0]) +
+                     A[0][2] * (A[1][0] * A[2][1] - A[1][1] * A[2][0]);
+  if (!(fabs(det) > 0)) return;
+  double l[3];
+  for (int k = 0; k < 3; ++k) {
+    double M[3][3];
+    for (int r = 0; r < 3; ++r)
+      for (int c = 0; c < 3; ++c) M[r][c] = c == k ? bvec[r] : A[r][c];
+    l[k] = (M[0][0] * (M[1][1] * M[2][2] - M[1][2] * M[2][1]) - M[0][1] * (M[1][0] * M[2][2] - M[1][2] * M[2][0]) +
+            M[0][2] * (M[1][0] * M[2][1] - M[1][1] * M[2][0])) /
+           det;
+  }
+  for (int c = 0; c < 3; ++c) {
+    w[0].v[c] += (l[0] * r01[c] + l[1] * r02[c]) * iO;
+    w[1].v[c] += (-l[0] * r01[c] + l[2] * r12[c]) * i1;
+    w[2].v[c] += (-l[1] * r02[c] - l[2] * r12[c]) * i2;
+  }
+}
+
+// A: half drift, SETTLE, then the velocity takes the constraint displacement
+// (v += correction / h, integrate.js:237-253).
+__device__ void driftWater(Atom *w, double h, const double4 geom) {
+  double o[9], n[9], u[9];
+  for (int a = 0; a < 3; ++a)
+    for (int c = 0; c < 3; ++c) {
+      o[3 * a + c] = w[a].x[c];
+      n[3 * a + c] = w[a].x[c] + h * w[a].v[c];
+      u[3 * a + c] = n[3 * a + c];
+    }
+  settle(o, n, geom.z, geom.w, geom.x, geom.y);
+  const double ih = 1.0 / h;
+  for (int a = 0; a < 3; ++a)
+    for (int c = 0; c < 3; ++c) {
+      w[a].x[c] = n[3 * a + c];
+      w[a].v[c] += (n[3 * a + c] - u[3 * a + c]) * ih;
+    }
+}
+
+// ---- general clusters (iterative, FP64, in per-slot scratch) -------------
+// Scratch per cluster atom slot: x[3], ref[3], v[3], corr[3].
+
+__device__ void clusterShake(const Dev &d, const IntegratorParams &ip, int cluster, double h) {
+  const int cb = d.clusterConStart[cluster], ce = d.clusterConStart[cluster + 1];
+  const int ab = d.clusterAtomStart[cluster], ae = d.clusterAtomStart[cluster + 1];
+  double *S = d.clusterScratch;
+  for (int s = ab; s < ae; ++s)
+    for (int c = 0; c < 3; ++c) {
+      S[12 * s + 3 + c] = S[12 * s + c];
+      S[12 * s + c] += h * S[12 * s + 6 + c];
+      S[12 * s + 9 + c] = 0.0;
+    }
+  int iterations = 0;
+  bool converged = false, stalled = false;
+  double previous = 1e300;
+  for (int it = 0; it < ip.maxIterations; ++it) {
+    double worst = 0;
+    for (int k = cb; k < ce; ++k) {
+      const int4 con = d.clusterCons[k];
+      const double target = d.clusterConD[k];
+      double *pa = S + 12 * con.x, *pb = S + 12 * con.y;
+      const double ia = d.vel[d.clusterAtoms[con.x]].w, ib = d.vel[d.clusterAtoms[con.y]].w;
+      const double dx = pa[0] - pb[0], dy = pa[1] - pb[1], dz = pa[2] - pb[2];
+      const double err = dx * dx + dy * dy + dz * dz - target * target;
+      const double rel = fabs(err) / (target * target);
+      worst = fmax(worst, rel);
+      if (rel <= ip.tolerance) continue;
+      const double rx = pa[3] - pb[3], ry = pa[4] - pb[4], rz = pa[5] - pb[5];
+      const double den = 2.0 * (ia + ib) * (dx * rx + dy * ry + dz * rz);
+      if (fabs(den) < 1e-20) continue;
+      const double lam = -err / den;
+      const double r[3] = {rx, ry, rz};
+      for (int c = 0; c < 3; ++c) {
+        pa[c] += lam * ia * r[c];
+        pb[c] -= lam * ib * r[c];
+        pa[9 + c] += lam * ia * r[c];
+        pb[9 + c] -= lam * ib * r[c];
+      }
+    }
+    iterations = it + 1;
+    if (worst <= ip.tolerance) {
+      converged = true;
+      break;
+    }
+    if (it >= 2 && worst >= previous * 0.99) {
+      stalled = true;
+      break;
+    }
+    previous = worst;
+  }
+  const double ih = 1.0 / h;
+  for (int s = ab; s < ae; ++s)
+    for (int c = 0; c < 3; ++c) S[12 * s + 6 + c] += S[12 * s + 9 + c] * ih;
+  atomicMax(&d.sc->shakeMaxIterations, (unsigned int)iterations);
+  if (!converged && !stalled) atomicMax(&d.sc->shakeFailed, 1u);
+}
+
+__device__ void clusterRattle(const Dev &d, const IntegratorParams &ip, int cluster) {
+  const int cb = d.clusterConStart[cluster], ce = d.clusterConStart[cluster + 1];
+  double *S = d.clusterScratch;
+  bool converged = false, stalled = false;
+  double previous = 1e300;
+  for (int it = 0; it < ip.maxIterations; ++it) {
+    double worst = 0;
+    for (int k = cb; k < ce; ++k) {
+      const int4 con = d.clusterCons[k];
+      const double target = d.clusterConD[k];
+      double *pa = S + 12 * con.x, *pb = S + 12 * con.y;
+      const double ia = d.vel[d.clusterAtoms[con.x]].w, ib = d.vel[d.clusterAtoms[con.y]].w;
+      const double r[3] = {pa[0] - pb[0], pa[1] - pb[1], pa[2] - pb[2]};
+      const double vr[3] = {pa[6] - pb[6], pa[7] - pb[7], pa[8] - pb[8]};
+      const double proj = r[0] * vr[0] + r[1] * vr[1] + r[2] * vr[2];
+      const double rel = fabs(proj) / (target * target);
+      worst = fmax(worst, rel);
+      if (rel <= ip.tolerance) continue;
+      const double den = (ia + ib) * (r[0] * r[0] + r[1] * r[1] + r[2] * r[2]);
+      if (den < 1e-20) continue;
+      const double lam = -proj / den;
+      for (int c = 0; c < 3; ++c) {
+        pa[6 + c] += lam * ia * r[c];
+        pb[6 + c] -= lam * ib * r[c];
+      }
+    }
+    if (worst <= ip.tolerance) {
+      converged = true;
+      break;
+    }
+    if (it >= 2 && worst >= previous * 0.99) {
+      stalled = true;
+      break;
+    }
+    previous = worst;
+  }
+  if (!converged && !stalled) atomicMax(&d.sc->rattleFailed, 1u);
+}
+
+__device__ void clusterKick(const Dev &d, int cluster, double h) {
+  const int ab = d.clusterAtomStart[cluster], ae = d.clusterAtomStart[cluster + 1];
+  for (int s = ab; s < ae; ++s) {
+    const int a = d.clusterAtoms[s];
+    const double im = d.vel[a].w;
+    if (im <= 0) continue;
+    const double f[3] = {(double)d.force[a] * FORCE_UNSCALE, (double)d.force[d.nPad + a] * FORCE_UNSCALE,
+                         (double)d.force[2 * d.nPad + a] * FORCE_UNSCALE};
+    for (int c = 0; c < 3; ++c) d.clusterScratch[12 * s + 6 + c] += h * im * f[c];
+  }
+}
+
+// ---- the kernels ------------------------------------------------------------
+
+// Block reductions: FP64 sum into a slot, float max into an atomic word.
+__device__ void blockSumToSlot(double v, double *slot) {
+  __shared__ double part[4];
+  v = waveSum(v);
+  if ((threadIdx.x & 63) == 0) part[threadIdx.x >> 6] = v;
+  __syncthreads();
+  if (threadIdx.x == 0) *slot = part[0] + part[1] + part[2] + part[3];
+  __syncthreads();
+}
+
+__device__ void blockMaxToWord(float v, unsigned int *word) {
+  __shared__ float part[4];
+  // NaN must win: compare as unsigned bits (non-negative floats order like
+  // their bit patterns and every NaN sorts above every finite value).
+  unsigned int bits = __float_as_uint(v);
+  for (int offset = 32; offset > 0; offset >>= 1) {
+    const unsigned int other = __shfl_xor(bits, offset, 64);
+    bits = other > bits ? other : bits;
+  }
+  if ((threadIdx.x & 63) == 0) part[threadIdx.x >> 6] = __uint_as_float(bits);
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    unsigned int m = 0;
+    for (int w = 0; w < 4; ++w) m = __float_as_uint(part[w]) > m ? __float_as_uint(part[w]) : m;
+    atomicMax(word, m);
+  }
+  __syncthreads();
+}
+
+__global__ void __launch_bounds__(256) integrateOpenKernel(Dev d, IntegratorParams ip, int closing, int early) {
+  const int u = blockIdx.x * blockDim.x + threadIdx.x;
+  const double h = ip.halfStep;
+  const unsigned int step = (unsigned int)d.sc->step;
+  double ke = 0.0;
+  float disp = 0.f;
+  if (u < d.nUnits) {
+    const int4 desc = d.units[u];
+    if (desc.x == UNIT_FREE) {
+      Atom s;
+      loadAtom(d, desc.y, s);
+      if (closing) kick(s, h);
+      kick(s, h);
+      for (int c = 0; c < 3; ++c) s.x[c] += h * s.v[c];
+      thermostat(d, ip, desc.y, s, step);
+      for (int c = 0; c < 3; ++c) s.x[c] += h * s.v[c];
+      ke = kinetic(s);
+      disp = storeAtom(d, desc.y, s, true);
+    } else if (desc.x == UNIT_WATER) {
+      Atom w[3];
+      const int atoms[3] = {desc.y, desc.z, desc.w};
+      for (int a = 0; a < 3; ++a) loadAtom(d, atoms[a], w[a]);
+      const double4 geom = d.waterGeom[u];
+      if (closing) {
+        for (int a = 0; a < 3; ++a) kick(w[a], h);
+        rattleWater(w);
+      }
+      for (int a = 0; a < 3; ++a) kick(w[a], h);
+      rattleWater(w);
+      driftWater(w, h, geom);
+      for (int a = 0; a < 3; ++a) thermostat(d, ip, atoms[a], w[a], step);
+      rattleWater(w);
+      driftWater(w, h, geom);
+      for (int a = 0; a < 3; ++a) {
+        ke += kinetic(w[a]);
+        disp = fmaxf(disp, storeAtom(d, atoms[a], w[a], true));
+      }
+    } else {
+      const int cl = desc.y;
+      const int ab = d.clusterAtomStart[cl], ae = d.clusterAtomStart[cl + 1];
+      double *S = d.clusterScratch;
+      for (int s = ab; s < ae; ++s) {
+        const int a = d.clusterAtoms[s];
+        const float4 p = d.pos[a], v = d.vel[a];
+        S[12 * s + 0] = p.x;
+        S[12 * s + 1] = p.y;
+        S[12 * s + 2] = p.z;
+        S[12 * s + 6] = v.x;
+        S[12 * s + 7] = v.y;
+        S[12 * s + 8] = v.z;
+      }
+      if (closing) {
+        clusterKick(d, cl, h);
+        clusterRattle(d, ip, cl);
+      }
+      clusterKick(d, cl, h);
+      clusterRattle(d, ip, cl);
+      clusterShake(d, ip, cl, h);
+      for (int s = ab; s < ae; ++s) {
+        const int a = d.clusterAtoms[s];
+        Atom t;
+        t.im = d.vel[a].w;
+        for (int c = 0; c < 3; ++c) t.v[c] = S[12 * s + 6 + c];
+        thermostat(d, ip, a, t, step);
+        for (int c = 0; c < 3; ++c) S[12 * s + 6 + c] = t.v[c];
+      }
+      clusterRattle(d, ip, cl);
+      clusterShake(d, ip, cl, h);
+      for (int s = ab; s < ae; ++s) {
+        const int a = d.clusterAtoms[s];
+        Atom t;
+        t.im = d.vel[a].w;
+        for (int c = 0; c < 3; ++c) {
+          t.x[c] = S[12 * s + c];
+          t.v[c] = S[12 * s + 6 + c];
+        }
+        ke += kinetic(t);
+        disp = fmaxf(disp, storeAtom(d, a, t, true));
+      }
+    }
+  }
+  blockMaxToWord(disp, &d.sc->maxDisp2Bits);
+  if (early) blockSumToSlot(ke, &d.slots[E_KINETIC_EARLY * MAX_SLOTS + blockIdx.x]);
+  // Last block: the displacement maximum is final; decide the rebuild and
+  // advance the noise counter (integrate.js:456-464).
+  __shared__ bool last;
+  __threadfence();
+  if (threadIdx.x == 0) last = atomicAdd(&d.sc->ticket, 1u) == gridDim.x - 1;
+  __syncthreads();
+  if (last && threadIdx.x == 0) {
+    __threadfence();
+    const unsigned int bits = atomicAdd(&d.sc->maxDisp2Bits, 0u);
+    const float maxDisp2 = __uint_as_float(bits);
+    const bool rebuild = d.sc->forceRebuild || !(maxDisp2 <= (float)ip.dispThreshold2);
+    d.sc->rebuild = rebuild ? 1u : 0u;
+    if (rebuild) {
+      d.sc->maxDisp2Bits = 0u;
+      d.sc->required = 0u;
+      d.sc->rebuilds += 1u;
+      d.sc->forceRebuild = 0u;
+    }
+    d.sc->ticket = 0u;
+    d.sc->step += 1;
+  }
+}
+
+__global__ void __launch_bounds__(256) integrateCloseKernel(Dev d, IntegratorParams ip, int withKinetic) {
+  const int u = blockIdx.x * blockDim.x + threadIdx.x;
+  const double h = ip.halfStep;
+  double ke = 0.0;
+  if (u < d.nUnits) {
+    const int4 desc = d.units[u];
+    if (desc.x == UNIT_FREE) {
+      Atom s;
+      loadAtom(d, desc.y, s);
+      kick(s, h);
+      ke = kinetic(s);
+      d.vel[desc.y] = make_float4((float)s.v[0], (float)s.v[1], (float)s.v[2], (float)s.im);
+    } else if (desc.x == UNIT_WATER) {
+      Atom w[3];
+      const int atoms[3] = {desc.y, desc.z, desc.w};
+      for (int a = 0; a < 3; ++a) loadAtom(d, atoms[a], w[a]);
+      for (int a = 0; a < 3; ++a) kick(w[a], h);
+      rattleWater(w);
+      for (int a = 0; a < 3; ++a) {
+        ke += kinetic(w[a]);
+        d.vel[atoms[a]] = make_float4((float)w[a].v[0], (float)w[a].v[1], (float)w[a].v[2], (float)w[a].im);
+      }
+    } else {
+      const int cl = desc.y;
+      const int ab = d.clusterAtomStart[cl], ae = d.clusterAtomStart[cl + 1];
+      double *S = d.clusterScratch;
+      for (int s = ab; s < ae; ++s) {
+        const int a = d.clusterAtoms[s];
+        const float4 p = d.pos[a], v = d.vel[a];
+        S[12 * s + 0] = p.x;
+        S[12 * s + 1] = p.y;
+        S[12 * s + 2] = p.z;
+        S[12 * s + 6] = v.x;
+        S[12 * s + 7] = v.y;
+        S[12 * s + 8] = v.z;
+      }
+      clusterKick(d, cl, h);
+      clusterRattle(d, ip, cl);
+      for (int s = ab; s < ae; ++s) {
+        const int a = d.clusterAtoms[s];
+        const float im = d.vel[a].w;
+        Atom t;
+        t.im = im;
+        for (int c = 0; c < 3; ++c) t.v[c] = S[12 * s + 6 + c];
+        ke += kinetic(t);
+        d.vel[a] = make_float4((float)t.v[0], (float)t.v[1], (float)t.v[2], im);
+      }
+    }
+  }
+  if (withKinetic) blockSumToSlot(ke, &d.slots[E_KINETIC * MAX_SLOTS + blockIdx.x]);
+}
+
+__global__ void __launch_bounds__(256) kineticKernel(Dev d, int term) {
+  const int a = blockIdx.x * blockDim.x + threadIdx.x;
+  double ke = 0.0;
+  if (a < d.n) {
+    const float4 v = d.vel[a];
+    if (v.w > 0.f) ke = 0.5 / (double)v.w * ((double)v.x * v.x + (double)v.y * v.y + (double)v.z * v.z);
+  }
+  blockSumToSlot(ke, &d.slots[term * MAX_SLOTS + blockIdx.x]);
+}
+
+// Sum each term's slot row in a fixed order: one block per term.
+__global__ void __launch_bounds__(256) reduceEnergiesKernel(const double *slots, const int *counts, double *out) {
+  const int term = blockIdx.x;
+  const int count = counts[term];
+  double s = 0.0;
+  for (int k = threadIdx.x; k < count; k += blockDim.x) s += slots[term * MAX_SLOTS + k];
+  __shared__ double part[256];
+  part[threadIdx.x] = s;
+  __syncthreads();
+  for (int stride = 128; stride > 0; stride >>= 1) {
+    if (threadIdx.x < stride) part[threadIdx.x] += part[threadIdx.x + stride];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) out[term] = part[0];
+}
+
+__global__ void zeroForcesKernel(Dev d) {
+  const int a = blockIdx.x * blockDim.x + threadIdx.x;
+  if (a >= d.nPad) return;
+  d.force[a] = 0;
+  d.force[d.nPad + a] = 0;
+  d.force[2 * d.nPad + a] = 0;
+}
+
+// Barostat (webgpu/kernels/barostat.js:25-107): mass-weighted molecule
+// centroids (geometric when massless), then x += (s - 1) * centroid.
+__global__ void centroidKernel(Dev d, const int *molStart, const int *molAtoms, int nMol, float4 *centroids) {
+  const int m = blockIdx.x * blockDim.x + threadIdx.x;
+  if (m >= nMol) return;
+  double w[3] = {0, 0, 0}, total = 0;
+  for (int k = molStart[m]; k < molStart[m + 1]; ++k) {
+    const int a = molAtoms[k];
+    const float4 p = d.pos[a];
+    const double im = d.vel[a].w;
+    const double mass = im > 0 ? 1.0 / im : 0.0;
+    w[0] += mass * p.x;
+    w[1] += mass * p.y;
+    w[2] += mass * p.z;
+    total += mass;
+  }
+  if (total > 0) {
+    for (int c = 0; c < 3; ++c) w[c] /= total;
+  } else {
+    const int count = molStart[m + 1] - molStart[m];
+    for (int k = molStart[m]; k < molStart[m + 1]; ++k) {
+      const float4 p = d.pos[molAtoms[k]];
+      w[0] += p.x;
+      w[1] += p.y;
+      w[2] += p.z;
+    }
+    for (int c = 0; c < 3; ++c) w[c] /= (count > 0 ? count : 1);
+  }
+  centroids[m] = make_float4((float)w[0], (float)w[1], (float)w[2], 0.f);
+}
+
+__global__ void scaleKernel(Dev d, const float4 *centroids, const int *molOfAtom, float factor) {
+  const int a = blockIdx.x * blockDim.x + threadIdx.x;
+  if (a >= d.n) return;
+  const float4 c = centroids[molOfAtom[a]];
+  float4 p = d.pos[a];
+  p.x += (factor - 1.0f) * c.x;
+  p.y += (factor - 1.0f) * c.y;
+  p.z += (factor - 1.0f) * c.z;
+  d.pos[a] = p;
+}
+
+// Minimiser: the largest per-atom force magnitude, as float bits.
+__global__ void maxForceKernel(Dev d, unsigned int *out) {
+  const int a = blockIdx.x * blockDim.x + threadIdx.x;
+  float m = 0.f;
+  if (a < d.n) {
+    const double fx = (double)d.force[a] * FORCE_UNSCALE, fy = (double)d.force[d.nPad + a] * FORCE_UNSCALE,
+                 fz = (double)d.force[2 * d.nPad + a] * FORCE_UNSCALE;
+    m = (float)sqrt(fx * fx + fy * fy + fz * fz);
+  }
+  blockMaxToWord(m, out);
+}
+
+// x += clamp(scale * F, cap) per atom (webgpu/kernels/integrate.js:642-665).
+__global__ void displaceKernel(Dev d, float scale, float cap) {
+  const int a = blockIdx.x * blockDim.x + threadIdx.x;
+  if (a >= d.n) return;
+  const float fx = (float)((double)d.force[a] * FORCE_UNSCALE), fy = (float)((double)d.force[d.nPad + a] * FORCE_UNSCALE),
+              fz = (float)((double)d.force[2 * d.nPad + a] * FORCE_UNSCALE);
+  float sx = scale * fx, sy = scale * fy, sz = scale * fz;
+  const float dist = sqrtf(sx * sx + sy * sy + sz * sz);
+  if (dist > cap) {
+    const float k = cap / dist;
+    sx *= k;
+    sy *= k;
+    sz *= k;
+  }
+  float4 p = d.pos[a];
+  p.x += sx;
+  p.y += sy;
+  p.z += sz;
+  d.pos[a] = p;
+}
+
+// Constraint projection of positions against saved references (the
+// minimiser's SHAKE, integrate.js:668-688): the displacement since the
+// reference is treated as a drift of unit length.
+__global__ void constrainPositionsKernel(Dev d, const float4 *reference, IntegratorParams ip) {
+  const int u = blockIdx.x * blockDim.x + threadIdx.x;
+  if (u >= d.nUnits) return;
+  const int4 desc = d.units[u];
+  if (desc.x == UNIT_FREE) return;
+  if (desc.x == UNIT_WATER) {
+    const int atoms[3] = {desc.y, desc.z, desc.w};
+    double o[9], n[9];
+    for (int a = 0; a < 3; ++a) {
+      const float4 r = reference[atoms[a]], p = d.pos[atoms[a]];
+      o[3 * a] = r.x;
+      o[3 * a + 1] = r.y;
+      o[3 * a + 2] = r.z;
+      n[3 * a] = p.x;
+      n[3 * a + 1] = p.y;
+      n[3 * a + 2] = p.z;
+    }
+    const double4 geom = d.waterGeom[u];
+    settle(o, n, geom.z, geom.w, geom.x, geom.y);
+    for (int a = 0; a < 3; ++a) {
+      float4 p = d.pos[atoms[a]];
+      p.x = (float)n[3 * a];
+      p.y = (float)n[3 * a + 1];
+      p.z = (float)n[3 * a + 2];
+      d.pos[atoms[a]] = p;
+    }
+    return;
+  }
+  const int cl = desc.y;
+  const int ab = d.clusterAtomStart[cl], ae = d.clusterAtomStart[cl + 1];
+  double *S = d.clusterScratch;
+  for (int s = ab; s < ae; ++s) {
+    const int a = d.clusterAtoms[s];
+    const float4 r = reference[a], p = d.pos[a];
+    // drift of length one from the reference to the trial position
+    S[12 * s + 0] = r.x;
+    S[12 * s + 1] = r.y;
+    S[12 * s + 2] = r.z;
+    S[12 * s + 6] = (double)p.x - r.x;
+    S[12 * s + 7] = (double)p.y - r.y;
+    S[12 * s + 8] = (double)p.z - r.z;
+  }
+  clusterShake(d, ip, cl, 1.0);
+  for (int s = ab; s < ae; ++s) {
+    const int a = d.clusterAtoms[s];
+    float4 p = d.pos[a];
+    p.x = (float)S[12 * s + 0];
+    p.y = (float)S[12 * s + 1];
+    p.z = (float)S[12 * s + 2];
+    d.pos[a] = p;
+  }
+}
+
+// ---- launchers ----------------------------------------------------------------
+
+static int unitBlocks(const Dev &d) { return (d.nUnits + 255) / 256; }
+
+int launchIntegrateOpen(const Dev &d, const IntegratorParams &ip, int closing, int early, hipStream_t s) {
+  const int blocks = unitBlocks(d);
+  integrateOpenKernel<<<blocks, 256, 0, s>>>(d, ip, closing, early);
+  return blocks;
+}
+
+int launchIntegrateClose(const Dev &d, const IntegratorParams &ip, int withKinetic, hipStream_t s) {
+  const int blocks = unitBlocks(d);
+  integrateCloseKernel<<<blocks, 256, 0, s>>>(d, ip, withKinetic);
+  return blocks;
+}
+
+int launchKinetic(const Dev &d, int term, hipStream_t s) {
+  const int blocks = (d.n + 255) / 256;
+  kineticKernel<<<blocks, 256, 0, s>>>(d, term);
+  return blocks;
+}
+
+void launchReduceEnergies(const Dev &d, const int *counts, double *out, hipStream_t s) {
+  reduceEnergiesKernel<<<E_TERM_COUNT, 256, 0, s>>>(d.slots, counts, out);
+}
+
+void launchZeroForces(const Dev &d, hipStream_t s) { zeroForcesKernel<<<(d.nPad + 255) / 256, 256, 0, s>>>(d); }
+
+void launchScaleMolecules(const Dev &d, const int *molStart, const int *molAtoms, int nMol, float4 *centroids,
+                          const int *molOfAtom, float scale, hipStream_t s) {
+  centroidKernel<<<(nMol + 255) / 256, 256, 0, s>>>(d, molStart, molAtoms, nMol, centroids);
+  scaleKernel<<<(d.n + 255) / 256, 256, 0, s>>>(d, centroids, molOfAtom, scale);
+}
+
+void launchMaxForce(const Dev &d, unsigned int *out, hipStream_t s) {
+  maxForceKernel<<<(d.n + 255) / 256, 256, 0, s>>>(d, out);
+}
+
+void launchDisplace(const Dev &d, float scale, float cap, hipStream_t s) {
+  displaceKernel<<<(d.n + 255) / 256, 256, 0, s>>>(d, scale, cap);
+}
+
+void launchConstrainPositions(const Dev &d, const float4 *reference, const IntegratorParams &ip, hipStream_t s) {
+  constrainPositionsKernel<<<unitBlocks(d), 256, 0, s>>>(d, reference, ip);
+}
+
+}  // namespace rmd

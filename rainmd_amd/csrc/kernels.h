@@ -1,0 +1,64 @@
+// Host-side launchers for every kernel. Each kernel file defines its own
+// launchers; the context calls only these.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include "device_state.h"
+
+namespace rmd {
+
+// ---- cell list and neighbour list (neighbours.hip) ----
+// The five rebuild stages are launched every step and return at once unless
+// the integrator's last block (or the host) asked for a rebuild.
+void launchRebuild(const Dev &d, hipStream_t s);
+void launchForceRebuildFlag(const Dev &d, hipStream_t s);
+
+// ---- nonbonded direct space (nonbonded.hip) ----
+// lanes: lanes per atom (1, 2, 4, ..., 64). energies != 0 fills the
+// E_LJ_DIRECT / E_COUL_DIRECT slot rows and returns the slot count used.
+int launchNonbonded(const Dev &d, float alpha, int lanes, int energies, hipStream_t s);
+
+// ---- bonded, exceptions, restraints (bonded.hip) ----
+int launchBonded(const Dev &d, float alpha, int terms, int energies, hipStream_t s, int *slotsUsed);
+
+// ---- PME (pme.hip) ----
+struct PmeDev {
+  int K[3];
+  int K2h;                  // K[2]/2 + 1
+  long long *qgrid;         // K0*K1*K2 fixed-point charges
+  float2 *cgrid;            // K0*K1*K2h complex half spectrum
+  float *rgrid;             // K0*K1*K2 real potential
+  float *influence;         // K0*K1*K2h: pref * kernel(m) (0 at m = 0)
+  float2 *twiddle[3];       // per axis, exp(+2 pi i k / K)
+  int radices[3][8];        // factorisation per axis (0-terminated)
+};
+void pmeComputeInfluence(const PmeDev &p, const BoxF *box, const double *moduli[3], double alpha, hipStream_t s);
+int launchPme(const Dev &d, const PmeDev &p, int energies, hipStream_t s, int *slotsUsed);
+// Debug entry: forward R2C 3D FFT of a real grid into the half spectrum.
+void launchFftForward(const PmeDev &p, const float *real, float2 *out, hipStream_t s);
+
+// ---- integrator and reductions (integrate.hip) ----
+struct IntegratorParams {
+  double halfStep, decay, noiseScale, kT, tolerance;
+  double dispThreshold2;  // rebuild when max |x - x_rebuild|^2 exceeds this
+  unsigned int seed;
+  int maxIterations;
+  int suppliedNoise;
+};
+// closing: apply the previous step's closing kick + RATTLE first.
+// early: record the in-step kinetic energy (E_KINETIC_EARLY).
+int launchIntegrateOpen(const Dev &d, const IntegratorParams &ip, int closing, int early, hipStream_t s);
+int launchIntegrateClose(const Dev &d, const IntegratorParams &ip, int kinetic, hipStream_t s);
+int launchKinetic(const Dev &d, int term, hipStream_t s);
+void launchReduceEnergies(const Dev &d, const int *slotsUsed, double *out, hipStream_t s);
+void launchZeroForces(const Dev &d, hipStream_t s);
+// Barostat: mass-weighted molecule centroids and the scaling about them.
+void launchScaleMolecules(const Dev &d, const int *molStart, const int *molAtoms, int nMol, float4 *centroids,
+                          const int *molOfAtom, float scale, hipStream_t s);
+// Minimiser helpers.
+void launchMaxForce(const Dev &d, unsigned int *out, hipStream_t s);
+void launchDisplace(const Dev &d, float scale, float cap, hipStream_t s);
+void launchConstrainPositions(const Dev &d, const float4 *reference, const IntegratorParams &ip, hipStream_t s);
+
+}  // namespace rmd

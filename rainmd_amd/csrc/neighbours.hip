@@ -1,0 +1,239 @@
+// Cell-list binning by counting sort, and the Verlet-skin neighbour list.
+//
+// Replaces encodeCellList (webgpu/kernels/cell-list.js:176-226) and
+// encodeNeighbours (webgpu/kernels/neighbours.js:46-169). The rebuild is
+// decided on the device: the integrator's last block compares the largest
+// displacement since the last rebuild with half the skin and sets
+// Scalars::rebuild, so these kernels run every step and return at once when
+// the list is still valid. That keeps a captured block free of host
+// decisions and never steps on a stale list.
+//
+// The binning arithmetic is FP32 with a fixed operation order and no
+// contraction, so the FP32 CPU emulation in the oracle reproduces every cell
+// assignment and every pair decision bit for bit.
+#include "kernels.h"
+
+namespace rmd {
+
+constexpr int SORT_CAPACITY = 4096;  // atoms per cell the LDS sort takes
+
+// cellOfPosition (webgpu/kernels/wgsl-common.js:112-123): fractional
+// coordinate from the reciprocal rows, wrapped with f - floor(f), scaled,
+// truncated and clamped.
+__device__ __forceinline__ int cellIndex(float x, float y, float z, const BoxF &b, int c[3]) {
+#pragma clang fp contract(off)
+  const int *cells = b.cells;
+  for (int d = 0; d < 3; ++d) {
+    float a0 = x * b.recip[d];
+    float a1 = y * b.recip[3 + d];
+    float a2 = z * b.recip[6 + d];
+    float f = (a0 + a1) + a2;
+    float w = f - floorf(f);
+    float scaled = w * (float)cells[d];
+    int ci = (int)scaled;
+    ci = ci > cells[d] - 1 ? cells[d] - 1 : ci;
+    ci = ci < 0 ? 0 : ci;
+    c[d] = ci;
+  }
+  return (c[0] * cells[1] + c[1]) * cells[2] + c[2];
+}
+
+__global__ void setRebuildKernel(Scalars *sc) {
+  sc->rebuild = 1u;
+  sc->maxDisp2Bits = 0u;
+  sc->required = 0u;
+  sc->rebuilds += 1u;
+}
+
+__global__ void cellsAssignKernel(Dev d) {
+  if (!d.sc->rebuild) return;
+  const BoxF &box = *d.box;
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= d.n) return;
+  const float4 p = d.pos[i];
+  int c3[3];
+  const int c = cellIndex(p.x, p.y, p.z, box, c3);
+  d.cellOfAtom[i] = c;
+  atomicAdd(&d.cellCount[c], 1);
+  if (!(isfinite(p.x) && isfinite(p.y) && isfinite(p.z))) atomicMax(&d.sc->nonFinite, 1u);
+}
+
+// Exclusive scan of the per-cell counts in one workgroup; leaves the counts
+// zeroed for the next rebuild and the scatter cursors at the cell starts.
+__global__ void __launch_bounds__(1024) cellsScanKernel(Dev d) {
+  if (!d.sc->rebuild) return;
+  __shared__ int partial[1024];
+  __shared__ int carry;
+  const int total = d.box->cellTotal;
+  if (threadIdx.x == 0) carry = 0;
+  __syncthreads();
+  for (int base = 0; base < total; base += 1024) {
+    const int c = base + threadIdx.x;
+    const int v = c < total ? d.cellCount[c] : 0;
+    partial[threadIdx.x] = v;
+    __syncthreads();
+    for (int offset = 1; offset < 1024; offset <<= 1) {
+      int add = threadIdx.x >= offset ? partial[threadIdx.x - offset] : 0;
+      __syncthreads();
+      partial[threadIdx.x] += add;
+      __syncthreads();
+    }
+    const int exclusive = carry + partial[threadIdx.x] - v;
+    if (c < total) {
+      d.cellStart[c] = exclusive;
+      d.cellCursor[c] = exclusive;
+      d.cellCount[c] = 0;
+    }
+    __syncthreads();
+    if (threadIdx.x == 1023) carry += partial[1023];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) d.cellStart[total] = carry;
+}
+
+__global__ void cellsScatterKernel(Dev d) {
+  if (!d.sc->rebuild) return;
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= d.n) return;
+  const int slot = atomicAdd(&d.cellCursor[d.cellOfAtom[i]], 1);
+  d.cellAtoms[slot] = i;
+}
+
+// The scatter leaves each cell's atoms in arrival order; a bitonic sort in
+// LDS puts them in index order so the list, and every sum over it, is the
+// same on every run.
+__global__ void __launch_bounds__(256) cellsSortKernel(Dev d) {
+  if (!d.sc->rebuild) return;
+  __shared__ int keys[SORT_CAPACITY];
+  const int c = blockIdx.x;
+  if (c >= d.box->cellTotal) return;
+  const int begin = d.cellStart[c], end = d.cellStart[c + 1];
+  const int len = end - begin;
+  if (len <= 1) return;
+  if (len > SORT_CAPACITY) {
+    if (threadIdx.x == 0) atomicMax(&d.sc->cellOverflow, 1u);
+    return;
+  }
+  int size = 1;
+  while (size < len) size <<= 1;
+  for (int k = threadIdx.x; k < size; k += blockDim.x) keys[k] = k < len ? d.cellAtoms[begin + k] : 0x7fffffff;
+  __syncthreads();
+  for (int k = 2; k <= size; k <<= 1) {
+    for (int j = k >> 1; j > 0; j >>= 1) {
+      for (int t = threadIdx.x; t < size; t += blockDim.x) {
+        const int partner = t ^ j;
+        if (partner > t) {
+          const bool ascending = (t & k) == 0;
+          const int a = keys[t], b = keys[partner];
+          if ((a > b) == ascending) {
+            keys[t] = b;
+            keys[partner] = a;
+          }
+        }
+      }
+      __syncthreads();
+    }
+  }
+  for (int k = threadIdx.x; k < len; k += blockDim.x) d.cellAtoms[begin + k] = keys[k];
+}
+
+// Per-axis stencil offsets, deduplicated (shared/topology/cells.js:25-29):
+// three or more cells visit {-1, 0, 1}, two visit {0, 1}, one visits {0}.
+__device__ __forceinline__ void axisRange(int count, int &base, int &len) {
+  if (count >= 3) {
+    base = -1;
+    len = 3;
+  } else if (count == 2) {
+    base = 0;
+    len = 2;
+  } else {
+    base = 0;
+    len = 1;
+  }
+}
+
+__device__ __forceinline__ int wrapCell(int v, int count) {
+  v %= count;
+  return v < 0 ? v + count : v;
+}
+
+// One wave per atom. Candidates are taken 64 at a time from each stencil
+// cell; a ballot compacts the accepted ones into the atom's slice in
+// candidate order. Every pair found is counted; only `cap` are stored, and
+// the true requirement reaches the host through Scalars::required
+// (the reference's overflow contract, webgpu/kernels/neighbours.js:128-146).
+__global__ void __launch_bounds__(256) neighboursKernel(Dev d) {
+  if (!d.sc->rebuild) return;
+  const BoxF box = *d.box;
+  const int *cells = box.cells;
+  const int lane = threadIdx.x & 63;
+  const int atom = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+  if (atom >= d.n) return;
+  const float4 pi = d.pos[atom];
+  if (lane == 0) d.posRebuild[atom] = pi;
+  const int own = d.cellOfAtom[atom];
+  const int c3[3] = {own / (cells[1] * cells[2]), (own / cells[2]) % cells[1], own % cells[2]};
+  const int exBegin = d.exclStart[atom], exEnd = d.exclStart[atom + 1];
+  int bx, nx, by, ny, bz, nz;
+  axisRange(cells[0], bx, nx);
+  axisRange(cells[1], by, ny);
+  axisRange(cells[2], bz, nz);
+  int found = 0;
+  int *out = d.nbrList + (size_t)atom * d.cap;
+  for (int ox = 0; ox < nx; ++ox) {
+    const int cx = wrapCell(c3[0] + bx + ox, cells[0]);
+    for (int oy = 0; oy < ny; ++oy) {
+      const int cy = wrapCell(c3[1] + by + oy, cells[1]);
+      for (int oz = 0; oz < nz; ++oz) {
+        const int cz = wrapCell(c3[2] + bz + oz, cells[2]);
+        const int cell = (cx * cells[1] + cy) * cells[2] + cz;
+        const int begin = d.cellStart[cell], end = d.cellStart[cell + 1];
+        for (int base = begin; base < end; base += 64) {
+          const int slot = base + lane;
+          bool accept = false;
+          int j = -1;
+          if (slot < end) {
+            j = d.cellAtoms[slot];
+            if (j != atom) {
+              const float4 pj = d.pos[j];
+              float dx = pi.x - pj.x, dy = pi.y - pj.y, dz = pi.z - pj.z;
+              const float r2 = minimumImageR2(dx, dy, dz, box);
+              accept = r2 <= box.rl2;
+              if (accept)
+                for (int e = exBegin; e < exEnd; ++e)
+                  if (d.exclPartners[e] == j) {
+                    accept = false;
+                    break;
+                  }
+            }
+          }
+          const unsigned long long mask = __ballot(accept);
+          if (accept) {
+            const int rank = __popcll(mask & ((1ull << lane) - 1ull));
+            const int where = found + rank;
+            if (where < d.cap) out[where] = j;
+          }
+          found += __popcll(mask);
+        }
+      }
+    }
+  }
+  if (lane == 0) {
+    d.nbrCount[atom] = found < d.cap ? found : d.cap;
+    atomicMax(&d.sc->required, (unsigned int)found);
+    atomicMax(&d.sc->requiredMax, (unsigned int)found);
+  }
+}
+
+void launchForceRebuildFlag(const Dev &d, hipStream_t s) { setRebuildKernel<<<1, 1, 0, s>>>(d.sc); }
+
+void launchRebuild(const Dev &d, hipStream_t s) {
+  const int blocks = (d.n + 255) / 256;
+  cellsAssignKernel<<<blocks, 256, 0, s>>>(d);
+  cellsScanKernel<<<1, 1024, 0, s>>>(d);
+  cellsScatterKernel<<<blocks, 256, 0, s>>>(d);
+  cellsSortKernel<<<d.cellCapacity, 256, 0, s>>>(d);
+  neighboursKernel<<<(d.n + 3) / 4, 256, 0, s>>>(d);
+}
+
+}  // namespace rmd

@@ -1,0 +1,108 @@
+// LJ + erfc-screened real-space Ewald over the Verlet list.
+//
+// Replaces directSpaceShader (webgpu/kernels/nonbonded.js:30-109). The pair
+// arithmetic is the reference's (A/r^12 - B/r^6, qq*erfc(ar)/r, and the
+// Gaussian term of the screened-Coulomb derivative, nonbonded.js:68-98) in
+// FP32. The machinery is MI355X's: a group of `LANES` lanes of a wave64
+// shares one atom's list, each lane's partial force is converted to int64
+// fixed point and the group reduces it exactly, so the sum does not depend
+// on the lane count or the order. Energies are halved (each pair appears in
+// both atoms' lists) and reduced in FP64 into per-block slots.
+#include "kernels.h"
+
+namespace rmd {
+
+template <int LANES>
+__global__ void __launch_bounds__(256) nonbondedKernel(Dev d, float alpha, int energies) {
+  const BoxF &box = *d.box;
+  const int tid = blockIdx.x * blockDim.x + threadIdx.x;
+  const int atom = tid / LANES;
+  const int sub = tid % LANES;
+  const bool valid = atom < d.n;
+  float fx = 0.f, fy = 0.f, fz = 0.f, elj = 0.f, ecoul = 0.f;
+  if (valid) {
+    const float4 pi = d.pos[atom];
+    const int ti = d.type[atom] * d.types;
+    const float qi = ONE_4PI_EPS0_F * pi.w;
+    const float gaussScale = TWO_OVER_SQRT_PI_F * alpha;
+    const int count = d.nbrCount[atom];
+    const int *list = d.nbrList + (size_t)atom * d.cap;
+    for (int k = sub; k < count; k += LANES) {
+      const int j = list[k];
+      const float4 pj = d.pos[j];
+      float dx = pi.x - pj.x, dy = pi.y - pj.y, dz = pi.z - pj.z;
+      const float r2 = minimumImageR2(dx, dy, dz, box);
+      if (r2 > box.rc2) continue;
+      const float2 ab = d.lj[ti + d.type[j]];
+      const float invR2 = __builtin_amdgcn_rcpf(r2);
+      const float invR = __builtin_amdgcn_rsqf(r2);
+      const float r = r2 * invR;
+      const float invR6 = invR2 * invR2 * invR2;
+      const float lj12 = ab.x * invR6;
+      float scalar = (12.0f * lj12 - 6.0f * ab.y) * invR6 * invR2;
+      const float qq = qi * pj.w;
+      float gaussian;
+      const float screened = erfcWithGaussian(alpha * r, gaussian);
+      scalar += qq * (screened * invR + gaussScale * gaussian) * invR2;
+      if (energies) {
+        elj += (lj12 - ab.y) * invR6;
+        ecoul += qq * screened * invR;
+      }
+      fx += scalar * dx;
+      fy += scalar * dy;
+      fz += scalar * dz;
+    }
+  }
+  // Exact group reduction in fixed point.
+  long long ix = __double2ll_rn((double)fx * FORCE_SCALE);
+  long long iy = __double2ll_rn((double)fy * FORCE_SCALE);
+  long long iz = __double2ll_rn((double)fz * FORCE_SCALE);
+#pragma unroll
+  for (int offset = LANES / 2; offset > 0; offset >>= 1) {
+    ix += __shfl_xor(ix, offset, LANES);
+    iy += __shfl_xor(iy, offset, LANES);
+    iz += __shfl_xor(iz, offset, LANES);
+  }
+  if (valid && sub == 0) {
+    atomicAddFixed(&d.force[atom], ix);
+    atomicAddFixed(&d.force[d.nPad + atom], iy);
+    atomicAddFixed(&d.force[2 * d.nPad + atom], iz);
+  }
+  if (energies) {
+    __shared__ double partial[2][4];
+    double a = waveSum(0.5 * (double)elj);
+    double b = waveSum(0.5 * (double)ecoul);
+    const int wave = threadIdx.x >> 6;
+    if ((threadIdx.x & 63) == 0) {
+      partial[0][wave] = a;
+      partial[1][wave] = b;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      double sa = 0, sb = 0;
+      for (int w = 0; w < (int)(blockDim.x >> 6); ++w) {
+        sa += partial[0][w];
+        sb += partial[1][w];
+      }
+      d.slots[E_LJ_DIRECT * MAX_SLOTS + blockIdx.x] = sa;
+      d.slots[E_COUL_DIRECT * MAX_SLOTS + blockIdx.x] = sb;
+    }
+  }
+}
+
+int launchNonbonded(const Dev &d, float alpha, int lanes, int energies, hipStream_t s) {
+  const long long threads = (long long)d.n * lanes;
+  const int blocks = (int)((threads + 255) / 256);
+  switch (lanes) {
+    case 1: nonbondedKernel<1><<<blocks, 256, 0, s>>>(d, alpha, energies); break;
+    case 2: nonbondedKernel<2><<<blocks, 256, 0, s>>>(d, alpha, energies); break;
+    case 4: nonbondedKernel<4><<<blocks, 256, 0, s>>>(d, alpha, energies); break;
+    case 8: nonbondedKernel<8><<<blocks, 256, 0, s>>>(d, alpha, energies); break;
+    case 16: nonbondedKernel<16><<<blocks, 256, 0, s>>>(d, alpha, energies); break;
+    case 32: nonbondedKernel<32><<<blocks, 256, 0, s>>>(d, alpha, energies); break;
+    default: nonbondedKernel<64><<<blocks, 256, 0, s>>>(d, alpha, energies); break;
+  }
+  return blocks;
+}
+
+}  // namespace rmd

@@ -1,0 +1,488 @@
+"""The Python host surface over the C ABI.
+
+Mirrors the reference's JavaScript host: parsePrmtop / parseInpcrd /
+buildSystem (shared/amber/*.js, shared/topology/system.js) and the
+GpuSimulation class (webgpu/simulation.js:82-1341), with the same methods,
+option names (snake_case), return fields and error classes. Every call goes
+through librainmd_amd.so; nothing here computes physics.
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+from . import _lib as L
+
+DATA = os.path.join(os.path.dirname(os.path.abspath(__file__)), "data")
+
+
+# ---- errors (the reference's named Error subclasses) ----------------------
+
+class RainmdError(RuntimeError):
+    """A plain Error: validation or device failure (execution.js:48-55)."""
+
+
+class ParseError(RainmdError):
+    """A refusal from parsePrmtop / parseInpcrd."""
+
+
+class SystemError_(RainmdError):
+    """A refusal from buildSystem (system.js:80-87)."""
+
+
+class NonFiniteStateError(RainmdError):
+    """shared/finite.js:63-95."""
+
+
+class NeighbourOverflowError(RainmdError):
+    """webgpu/kernels/neighbours.js:172-183."""
+
+
+class SimulationStopped(RainmdError):
+    """webgpu/runtime/yield.js:44-50."""
+
+
+class GpuBudgetError(RainmdError):
+    """webgpu/runtime/allocator.js:141-154."""
+
+
+class ConstraintFailure(RainmdError):
+    """cpu/integrate/langevin.js:160-166."""
+
+
+class DeviceError(RainmdError):
+    """A HIP failure or no device."""
+
+
+_ERRORS = {
+    L.RMD_E_PARSE: ParseError,
+    L.RMD_E_SYSTEM: SystemError_,
+    L.RMD_E_DEVICE: DeviceError,
+    L.RMD_E_NONFINITE: NonFiniteStateError,
+    L.RMD_E_NEIGHBOUR_OVERFLOW: NeighbourOverflowError,
+    L.RMD_E_STOPPED: SimulationStopped,
+    L.RMD_E_BUDGET: GpuBudgetError,
+    L.RMD_E_CONSTRAINT: ConstraintFailure,
+}
+
+
+def _check(rc, ctx=None):
+    if rc == L.RMD_OK:
+        return
+    lib = L.load()
+    msg = (lib.rmd_last_error(ctx) if ctx else lib.rmd_last_global_error()) or b""
+    raise _ERRORS.get(rc, RainmdError)(msg.decode(errors="replace"))
+
+
+def _arr(ptr, n, dtype):
+    if not ptr or n == 0:
+        return np.zeros(0, dtype=dtype)
+    return np.ctypeslib.as_array(ptr, shape=(n,)).astype(dtype, copy=True)
+
+
+def read_data(name):
+    with open(os.path.join(DATA, name), "rb") as f:
+        return f.read()
+
+
+# ---- topology, coordinates, system ----------------------------------------
+
+class Topology:
+    """parsePrmtop (shared/amber/prmtop.js:111-385)."""
+
+    def __init__(self, text):
+        lib = L.load()
+        raw = text.encode() if isinstance(text, str) else text
+        h = C.c_void_p()
+        _check(lib.rmd_topology_parse_prmtop(raw, len(raw), C.byref(h)))
+        self._h = h
+
+    @classmethod
+    def load(cls, name_or_path):
+        path = name_or_path if os.path.exists(name_or_path) else os.path.join(DATA, name_or_path)
+        with open(path, "rb") as f:
+            return cls(f.read())
+
+    def __del__(self):
+        if getattr(self, "_h", None):
+            L.load().rmd_topology_free(self._h)
+            self._h = None
+
+
+class Coordinates:
+    """parseInpcrd (shared/amber/inpcrd.js:34-132)."""
+
+    def __init__(self, text, expected_atom_count=0):
+        lib = L.load()
+        raw = text.encode() if isinstance(text, str) else text
+        h = C.c_void_p()
+        _check(lib.rmd_coordinates_parse_inpcrd(raw, len(raw), int(expected_atom_count), C.byref(h)))
+        self._h = h
+        v = L.CoordinatesView()
+        _check(lib.rmd_coordinates_view_get(h, C.byref(v)))
+        self.atom_count = v.atom_count
+        self.positions = _arr(v.positions, 3 * v.atom_count, np.float64)
+        self.velocities = _arr(v.velocities, 3 * v.atom_count, np.float64) if v.velocities else None
+        self.box = {"lengths": list(v.box_lengths), "angles": list(v.box_angles)} if v.has_box else None
+        self.time = v.time
+
+    @classmethod
+    def load(cls, name_or_path, expected_atom_count=0):
+        path = name_or_path if os.path.exists(name_or_path) else os.path.join(DATA, name_or_path)
+        with open(path, "rb") as f:
+            return cls(f.read(), expected_atom_count)
+
+    def __del__(self):
+        if getattr(self, "_h", None):
+            L.load().rmd_coordinates_free(self._h)
+            self._h = None
+
+
+def box_vectors_from_lengths_and_angles(lengths, angles):
+    out = (C.c_double * 9)()
+    _check(L.load().rmd_box_vectors_from_lengths_and_angles((C.c_double * 3)(*lengths), (C.c_double * 3)(*angles), out))
+    return np.array(out[:])
+
+
+class System:
+    """buildSystem (shared/topology/system.js:40-236), flattened for the kernels.
+
+    Every array of the built system is exposed as a numpy copy (see
+    rmd_system_view in include/rainmd.h).
+    """
+
+    def __init__(self, handle):
+        self._h = handle
+        lib = L.load()
+        v = L.SystemView()
+        _check(lib.rmd_system_view_get(handle, C.byref(v)))
+        n, t = v.atom_count, v.type_count
+        self.atom_count = n
+        self.type_count = t
+        self.molecule_count = v.molecule_count
+        self.residue_count = v.residue_count
+        self.charges = _arr(v.charges, n, np.float64)
+        self.masses = _arr(v.masses, n, np.float64)
+        self.atom_types = _arr(v.atom_types, n, np.int32)
+        self.atomic_numbers = _arr(v.atomic_numbers, n, np.int32)
+        self.residue_of_atom = _arr(v.residue_of_atom, n, np.int32)
+        self.lj_a = _arr(v.lj_a, t * t, np.float64)
+        self.lj_b = _arr(v.lj_b, t * t, np.float64)
+        self.bond_a = _arr(v.bond_a, v.bond_count, np.int32)
+        self.bond_b = _arr(v.bond_b, v.bond_count, np.int32)
+        self.bond_k = _arr(v.bond_k, v.bond_count, np.float64)
+        self.bond_r0 = _arr(v.bond_r0, v.bond_count, np.float64)
+        self.angle_a = _arr(v.angle_a, v.angle_count, np.int32)
+        self.angle_b = _arr(v.angle_b, v.angle_count, np.int32)
+        self.angle_c = _arr(v.angle_c, v.angle_count, np.int32)
+        self.angle_k = _arr(v.angle_k, v.angle_count, np.float64)
+        self.angle_theta0 = _arr(v.angle_theta0, v.angle_count, np.float64)
+        self.torsion_a = _arr(v.torsion_a, v.torsion_count, np.int32)
+        self.torsion_b = _arr(v.torsion_b, v.torsion_count, np.int32)
+        self.torsion_c = _arr(v.torsion_c, v.torsion_count, np.int32)
+        self.torsion_d = _arr(v.torsion_d, v.torsion_count, np.int32)
+        self.torsion_k = _arr(v.torsion_k, v.torsion_count, np.float64)
+        self.torsion_periodicity = _arr(v.torsion_periodicity, v.torsion_count, np.float64)
+        self.torsion_phase = _arr(v.torsion_phase, v.torsion_count, np.float64)
+        self.constraint_a = _arr(v.constraint_a, v.constraint_count, np.int32)
+        self.constraint_b = _arr(v.constraint_b, v.constraint_count, np.int32)
+        self.constraint_distance = _arr(v.constraint_distance, v.constraint_count, np.float64)
+        self.exception_a = _arr(v.exception_a, v.exception_count, np.int32)
+        self.exception_b = _arr(v.exception_b, v.exception_count, np.int32)
+        self.exception_charge_product = _arr(v.exception_charge_product, v.exception_count, np.float64)
+        self.exception_lj_a = _arr(v.exception_lj_a, v.exception_count, np.float64)
+        self.exception_lj_b = _arr(v.exception_lj_b, v.exception_count, np.float64)
+        self.exclusion_start = _arr(v.exclusion_start, n + 1, np.int32)
+        self.exclusion_partners = _arr(v.exclusion_partners, v.exclusion_total, np.int32)
+        self.molecule_start = _arr(v.molecule_start, v.molecule_count + 1, np.int32)
+        self.molecule_atoms = _arr(v.molecule_atoms, n, np.int32)
+        self.molecule_of_atom = _arr(v.molecule_of_atom, n, np.int32)
+        self.is_periodic = bool(v.is_periodic)
+        self.box = np.array(v.box[:])
+        self.cutoff = v.cutoff
+        self.degrees_of_freedom = v.degrees_of_freedom
+        self.positions = _arr(v.positions, 3 * n, np.float64) if v.positions else None
+        self.velocities = _arr(v.velocities, 3 * n, np.float64) if v.velocities else None
+        self.water_count = v.water_count
+
+    @classmethod
+    def build(cls, topology, coordinates=None, constraints="HBonds", rigid_water=True, cutoff=1.0,
+              box_vectors=None, box=None):
+        lib = L.load()
+        o = L.BuildOptions()
+        lib.rmd_build_options_default(C.byref(o))
+        if constraints not in L.CONSTRAINTS:
+            raise SystemError_(
+                f"unsupported constraints mode {constraints!r}; this port implements none, HBonds and AllBonds "
+                "(HAngles is refused rather than approximated)")
+        o.constraints = L.CONSTRAINTS[constraints]
+        o.rigid_water = 1 if rigid_water else 0
+        o.cutoff = cutoff
+        if box_vectors is not None:
+            o.has_box_vectors = 1
+            o.box_vectors[:] = [float(x) for x in box_vectors]
+        elif box is not None:
+            o.has_box = 1
+            o.box_lengths[:] = box["lengths"]
+            o.box_angles[:] = box["angles"]
+        h = C.c_void_p()
+        _check(lib.rmd_system_build(topology._h, coordinates._h if coordinates is not None else None,
+                                    C.byref(o), C.byref(h)))
+        return cls(h)
+
+    @classmethod
+    def solvated(cls, water_count, box_edge, seed, include_solute=True, cutoff=0.9, constraints="HBonds",
+                 rigid_water=True, clearance=0.25, jitter=0.02):
+        """The seeded solvated-peptide builder (include/rainmd.h)."""
+        lib = L.load()
+        o = L.SolvatedOptions()
+        lib.rmd_solvated_options_default(C.byref(o))
+        o.water_count = water_count
+        o.box_edge = box_edge
+        o.seed = seed
+        o.include_solute = 1 if include_solute else 0
+        o.solute_clearance = clearance
+        o.jitter = jitter
+        o.cutoff = cutoff
+        o.constraints = L.CONSTRAINTS[constraints]
+        o.rigid_water = 1 if rigid_water else 0
+        templ = read_data("ala-wat.prmtop")
+        solute = read_data("ala-vac.inpcrd")
+        h = C.c_void_p()
+        _check(lib.rmd_system_build_solvated(templ, len(templ), solute, len(solute), C.byref(o), C.byref(h)))
+        return cls(h)
+
+    def skin_for(self, timestep=0.002, temperature=298.0):
+        return L.load().rmd_skin_for(self._h, timestep, temperature)
+
+    def dispersion_energy(self, volume):
+        return L.load().rmd_dispersion_energy(self._h, volume)
+
+    def __del__(self):
+        if getattr(self, "_h", None):
+            L.load().rmd_system_free(self._h)
+            self._h = None
+
+
+def pme_parameters(box, cutoff, tolerance=5e-4, alpha=None, grid=None):
+    """pmeParameters (shared/pme/parameters.js:227-252)."""
+    out = L.PmeParameters()
+    g = (C.c_int32 * 3)(*(grid or (0, 0, 0)))
+    _check(L.load().rmd_pme_parameters_derive((C.c_double * 9)(*box), cutoff, tolerance, alpha or 0.0, g,
+                                               C.byref(out)))
+    return {"alpha": out.alpha, "grid": list(out.grid), "order": out.order, "tolerance": out.tolerance}
+
+
+def pme_moduli(size, order=5):
+    out = (C.c_double * size)()
+    _check(L.load().rmd_pme_moduli(size, order, out))
+    return np.array(out[:])
+
+
+def device_info(device=0):
+    info = L.DeviceInfo()
+    rc = L.load().rmd_device_info_get(device, C.byref(info))
+    return {"device_count": info.device_count, "name": info.name.decode(), "arch": info.arch.decode(),
+            "compute_units": info.compute_units, "global_memory": info.global_memory, "ok": rc == L.RMD_OK}
+
+
+def _struct_dict(s):
+    out = {}
+    for name, _ in s._fields_:
+        v = getattr(s, name)
+        if isinstance(v, C.Array):
+            v = list(v)
+        out[name] = v
+    return out
+
+
+# ---- the simulation -------------------------------------------------------
+
+class GpuSimulation:
+    """new GpuSimulation(device, system, options) (webgpu/simulation.js:83-190).
+
+    Options mirror the reference's constructor options (simulation.js:93-188):
+    timestep, temperature, friction, seed, tolerance, max_constraint_iterations,
+    rebuild_interval, adaptive_rebuild, pressure, barostat_interval,
+    dispersion_correction, restraint_force / restraint_atoms /
+    restraint_reference, skin, neighbour_slack, max_neighbours,
+    ewald_tolerance, alpha, grid; plus device and lanes_per_atom.
+    """
+
+    def __init__(self, system, **options):
+        lib = L.load()
+        self.system = system
+        o = L.Options()
+        lib.rmd_options_default(C.byref(o))
+        simple = ["timestep", "temperature", "friction", "seed", "tolerance", "max_constraint_iterations",
+                  "rebuild_interval", "barostat_interval", "skin", "neighbour_slack", "max_neighbours",
+                  "ewald_tolerance", "alpha", "device", "lanes_per_atom"]
+        for key in simple:
+            if options.get(key) is not None:
+                setattr(o, key, options[key])
+        if options.get("adaptive_rebuild") is not None:
+            o.adaptive_rebuild = 1 if options["adaptive_rebuild"] else 0
+        if options.get("dispersion_correction") is not None:
+            o.dispersion_correction = 1 if options["dispersion_correction"] else 0
+        if options.get("pressure") is not None:
+            o.has_pressure = 1
+            o.pressure = options["pressure"]
+        if options.get("grid") is not None:
+            o.grid[:] = list(options["grid"])
+        self._keep = []
+        if options.get("restraint_force"):
+            atoms = np.ascontiguousarray(options["restraint_atoms"], dtype=np.int32)
+            ref = np.ascontiguousarray(options["restraint_reference"], dtype=np.float64)
+            self._keep += [atoms, ref]
+            o.restraint_force = options["restraint_force"]
+            o.restraint_count = len(atoms)
+            o.restraint_atoms = atoms.ctypes.data_as(L._ip)
+            o.restraint_reference = ref.ctypes.data_as(L._dp)
+        h = C.c_void_p()
+        _check(lib.rmd_create(system._h, C.byref(o), C.byref(h)))
+        self._h = h
+        self.atom_count = system.atom_count
+
+    # -- state --
+    def set_state(self, positions, velocities=None, step=0, supplied_noise=False):
+        p = np.ascontiguousarray(positions, dtype=np.float64).reshape(-1)
+        v = None if velocities is None else np.ascontiguousarray(velocities, dtype=np.float64).reshape(-1)
+        _check(L.load().rmd_set_state(self._h, p.ctypes.data_as(L._dp), None if v is None else v.ctypes.data_as(L._dp),
+                                      int(step), 1 if supplied_noise else 0), self._h)
+
+    def set_noise(self, noise):
+        n = np.ascontiguousarray(noise, dtype=np.float32).reshape(-1)
+        _check(L.load().rmd_set_noise(self._h, n.ctypes.data_as(L._fp)), self._h)
+
+    def initialise(self):
+        _check(L.load().rmd_initialise(self._h), self._h)
+
+    def run(self, count, block_size=0):
+        step = C.c_int64()
+        _check(L.load().rmd_run(self._h, int(count), int(block_size), C.byref(step)), self._h)
+        return step.value
+
+    def request_stop(self):
+        _check(L.load().rmd_request_stop(self._h), self._h)
+
+    def step_once(self):
+        _check(L.load().rmd_step_once(self._h), self._h)
+
+    # -- energies and state readback --
+    def energy(self):
+        e = L.EnergyTerms()
+        _check(L.load().rmd_energy(self._h, C.byref(e)), self._h)
+        d = _struct_dict(e)
+        d["kinetic_sampled_at"] = "before the closing half kick" if e.kinetic_sampled_in_step else "after the step"
+        return d
+
+    def evaluate(self, terms=L.TERM_ALL, forces=True):
+        e = L.EnergyTerms()
+        f = np.zeros(3 * self.atom_count) if forces else None
+        _check(L.load().rmd_evaluate(self._h, terms, C.byref(e), f.ctypes.data_as(L._dp) if forces else None),
+               self._h)
+        return _struct_dict(e), (f.reshape(-1, 3) if forces else None)
+
+    def read_positions(self):
+        out = np.zeros(3 * self.atom_count, dtype=np.float32)
+        _check(L.load().rmd_read_positions(self._h, out.ctypes.data_as(L._fp)), self._h)
+        return out
+
+    def read_state(self):
+        n3 = 3 * self.atom_count
+        p = np.zeros(n3, dtype=np.float32)
+        v = np.zeros(n3, dtype=np.float32)
+        f = np.zeros(n3, dtype=np.float32)
+        box = (C.c_double * 9)()
+        vol = C.c_double()
+        _check(L.load().rmd_read_state(self._h, p.ctypes.data_as(L._fp), v.ctypes.data_as(L._fp),
+                                       f.ctypes.data_as(L._fp), box, C.byref(vol)), self._h)
+        return {"positions": p, "velocities": v, "forces": f, "box_vectors": np.array(box[:]), "volume": vol.value}
+
+    # -- barostat --
+    def scale_volume(self, length_scale):
+        u = C.c_double()
+        _check(L.load().rmd_scale_volume(self._h, float(length_scale), C.byref(u)), self._h)
+        return u.value
+
+    def attempt_volume_change(self):
+        m = L.Move()
+        _check(L.load().rmd_attempt_volume_change(self._h, C.byref(m)), self._h)
+        if not m.attempted:
+            return None
+        return _struct_dict(m)
+
+    # -- minimiser --
+    def minimise(self, max_iterations=200, force_tolerance=500.0, max_step=0.002, max_displacement=0.05):
+        o = L.MinOptions(max_iterations, force_tolerance, max_step, max_displacement)
+        r = L.MinResult()
+        _check(L.load().rmd_minimise(self._h, C.byref(o), C.byref(r)), self._h)
+        return _struct_dict(r)
+
+    def diagnostics(self):
+        d = L.Diagnostics()
+        _check(L.load().rmd_diagnostics_get(self._h, C.byref(d)), self._h)
+        return _struct_dict(d)
+
+    # -- inspection used by the parity tests --
+    def read_cells(self):
+        diag = self.diagnostics()
+        n = self.atom_count
+        cells = int(np.prod(diag["cell_counts"]))
+        coa = np.zeros(n, dtype=np.int32)
+        start = np.zeros(cells + 1, dtype=np.int32)
+        atoms = np.zeros(n, dtype=np.int32)
+        _check(L.load().rmd_read_cells(self._h, coa.ctypes.data_as(L._ip), start.ctypes.data_as(L._ip),
+                                       atoms.ctypes.data_as(L._ip)), self._h)
+        return coa, start, atoms
+
+    def read_neighbours(self):
+        cap = self.diagnostics()["neighbour_capacity"]
+        counts = np.zeros(self.atom_count, dtype=np.int32)
+        lists = np.zeros(self.atom_count * cap, dtype=np.int32)
+        _check(L.load().rmd_read_neighbours(self._h, counts.ctypes.data_as(L._ip), lists.ctypes.data_as(L._ip)),
+               self._h)
+        return counts, lists.reshape(self.atom_count, cap)
+
+    def debug_fft_forward(self, real):
+        grid = self.diagnostics()["pme_grid"]
+        r = np.ascontiguousarray(real, dtype=np.float32).reshape(-1)
+        half = grid[2] // 2 + 1
+        out = np.zeros(grid[0] * grid[1] * half * 2, dtype=np.float32)
+        _check(L.load().rmd_debug_fft_forward(self._h, r.ctypes.data_as(L._fp), out.ctypes.data_as(L._fp)), self._h)
+        c = out.reshape(grid[0], grid[1], half, 2)
+        return c[..., 0] + 1j * c[..., 1]
+
+    def set_lanes(self, lanes):
+        _check(L.load().rmd_set_lanes(self._h, int(lanes)), self._h)
+
+    def set_graphs(self, enabled):
+        _check(L.load().rmd_set_graphs(self._h, 1 if enabled else 0), self._h)
+
+    def stream(self):
+        s = C.c_void_p()
+        _check(L.load().rmd_get_stream(self._h, C.byref(s)), self._h)
+        return s.value
+
+    def release(self):
+        if getattr(self, "_h", None):
+            L.load().rmd_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        self.release()
+
+
+# ---- the synthetic workloads ------------------------------------------------
+
+# Solvated alanine dipeptide (ala-wat.prmtop's ACE-ALA-NME + TIP3P) on a
+# seeded jittered lattice, and the pure water box.
+WORKLOADS = {
+    "ala-389w": dict(water_count=389, box_edge=2.2816, seed=101, include_solute=True, cutoff=0.9),
+    "ala-944w": dict(water_count=944, box_edge=3.0539, seed=202, include_solute=True, cutoff=0.9),
+    "water-33400": dict(water_count=33400, box_edge=10.0, seed=303, include_solute=False, cutoff=1.0),
+}
+
+
+def workload_system(name):
+    return System.solvated(**WORKLOADS[name])
