@@ -1,0 +1,211 @@
+#!/usr/bin/env python3
+"""Throughput of the MI355X MD step on the 2854-atom NPT workload.
+
+Workload: ACE-ALA-NME + 944 TIP3P waters (2854 atoms) on a seeded jittered
+lattice in a 3.0539 nm cube, rc 0.9 nm, PME 30^3 order 5 (Ewald tolerance
+5e-4), Langevin middle integrator at 300 K / 1 ps^-1, 2 fs with SETTLE + H-bond
+SHAKE, NPT at 1 bar with the Monte Carlo barostat every 25 steps. The system
+is minimised, given Maxwell-Boltzmann velocities and run W warm-up steps;
+then exactly K steps are timed (barostat moves included) with device
+synchronisation on both sides.
+
+One JSON line on stdout: ns/day, the nonbonded kernel's FP32 roofline (its
+launch time measured with HIP events on the simulation's stream, 78 FLOP per
+unique in-cutoff pair), and the CPU baseline: the FP64 restatement of the
+reference's cpu/ path timed on a bounded sample on one host core, whose first
+evaluation is also the energy gate (every term of the final GPU state against
+FP64 at the same f32 positions).
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "ns/day (single system & batched aggregate) at energy rel.err ≤1e-6 vs FP64"
+TIMESTEP_PS = 0.002
+FLOP_PER_PAIR = 78
+FP32_PEAK_TFLOPS = 157.3
+REFERENCE_NS_PER_DAY = 74.0  # ala-wat-big, Apple M-series WebGPU (docs/STATUS.md:199-203)
+
+
+def log(msg):
+    print(msg, file=sys.stderr, flush=True)
+
+
+def in_cutoff_pairs(positions, box, cutoff, excl_start, excl_partners):
+    """Unique non-excluded pairs with r < rc at these positions (numpy, FP64)."""
+    x = positions.reshape(-1, 3).astype(np.float64)
+    n = len(x)
+    L = np.array([box[0], box[4], box[8]])
+    total = 0
+    rc2 = cutoff * cutoff
+    for i in range(n - 1):
+        d = x[i + 1:] - x[i]
+        d -= L * np.floor(d / L + 0.5)
+        r2 = np.einsum("ij,ij->i", d, d)
+        hit = np.nonzero(r2 <= rc2)[0] + i + 1
+        if len(hit):
+            ex = excl_partners[excl_start[i]:excl_start[i + 1]]
+            total += len(hit) - np.count_nonzero(np.isin(hit, ex))
+    return total
+
+
+def cpu_baseline(system, pme, state, target_seconds):
+    """The FP64 CPU port of the reference's host path (oracle/), timed on one
+    core: BAOAB steps with an all-pairs direct space and a naive-DFT PME. Its
+    first evaluation also checks the GPU's energies term by term."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle as O
+
+    o = O.Oracle(system, box=state["box"])
+    pos = state["positions"].astype(np.float64)
+    vel = state["velocities"].astype(np.float64)
+    t0 = time.perf_counter()
+    ref, forces = o.potential(pos, alpha=pme["alpha"], grid=pme["grid"], f32_membership=True)
+    first = time.perf_counter() - t0
+    steps = max(2, int(target_seconds / max(first, 1e-3)))
+    t0 = time.perf_counter()
+    for k in range(steps):
+        pos, vel, _ = o.langevin_step(pos, vel, forces, temperature=300.0, seed=202, step=k)
+        _, forces = o.potential(pos, alpha=pme["alpha"], grid=pme["grid"])
+        vel, _ = o.langevin_finish(pos, vel, forces)
+    elapsed = time.perf_counter() - t0
+    ms = 1e3 * elapsed / steps
+    return ref, {
+        "value": TIMESTEP_PS * 1e-3 * 86400.0 / (ms * 1e-3),
+        "unit": "ns/day",
+        "cores": 1,
+        "kind": "port",
+        "ms_per_step": ms,
+        "sample": f"{steps} NVT BAOAB steps of the same 2854-atom system from the final GPU state "
+                  "(FP64 all-pairs direct space, naive-DFT PME, global SHAKE/RATTLE), single-threaded C",
+    }
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=5000)
+    ap.add_argument("--warmup", type=int, default=1000)
+    ap.add_argument("--workload", default="ala-944w")
+    ap.add_argument("--skin", type=float, default=0.0, help="nm; 0 derives the reference's skinFor")
+    ap.add_argument("--lanes", type=int, default=0)
+    ap.add_argument("--block", type=int, default=25)
+    ap.add_argument("--cpu-seconds", type=float, default=15.0)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    args = ap.parse_args()
+
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU: this build runs one system per MI355X and has no multi-GPU "
+              f"path (asked for --gpus {args.gpus}).", file=sys.stderr)
+        sys.exit(2)
+
+    import rainmd_amd as R
+
+    system = R.workload_system(args.workload)
+    opts = dict(temperature=300.0, pressure=1.0, barostat_interval=25, seed=202)
+    if args.skin > 0:
+        opts["skin"] = args.skin
+    if args.lanes > 0:
+        opts["lanes_per_atom"] = args.lanes
+    sim = R.GpuSimulation(system, **opts)
+    diag = sim.diagnostics()
+    pme = {"alpha": diag["pme_alpha"], "grid": diag["pme_grid"]}
+    log(f"workload {args.workload}: {system.atom_count} atoms, grid {pme['grid']}, skin {diag['skin']:.3f}")
+    mini = sim.minimise(max_iterations=100, force_tolerance=1000.0)
+    log(f"minimised: {mini['iterations']} iterations, potential {mini['potential']:.1f}, "
+        f"max force {mini['max_force']:.1f}")
+    positions = sim.read_positions().astype(np.float64)
+    sim.set_state(positions, R.maxwell_boltzmann(system, 300.0, 202), step=0)
+    t0 = time.perf_counter()
+    sim.run(args.warmup, args.block)
+    log(f"warm-up {args.warmup} steps in {time.perf_counter() - t0:.2f} s")
+
+    t0 = time.perf_counter()
+    sim.run(args.steps, args.block)
+    elapsed = time.perf_counter() - t0
+    ms_per_step = 1e3 * elapsed / args.steps
+    ns_day = TIMESTEP_PS * 1e-3 * 86400.0 / (ms_per_step * 1e-3)
+    energy = sim.energy()
+    diag = sim.diagnostics()
+    log(f"timed {args.steps} steps: {ms_per_step * 1e3:.1f} us/step, {ns_day:.1f} ns/day; "
+        f"T {energy['temperature']:.1f} K, volume {energy['volume']:.4f} nm^3, "
+        f"barostat {diag['barostat_accepted']}/{diag['barostat_attempts']}, rebuilds {diag['rebuilds']}")
+
+    # Roofline of the dominant kernel at the final state.
+    state = sim.read_state()
+    state["box"] = np.array(energy["box"])
+    nb_ms = sim.profile_kernel("nonbonded", reps=200)
+    pairs = in_cutoff_pairs(state["positions"], state["box"], system.cutoff, system.exclusion_start,
+                            system.exclusion_partners)
+    achieved = FLOP_PER_PAIR * pairs / (nb_ms * 1e-3) / 1e12
+    kernels = {k: sim.profile_kernel(k, reps=100) for k in ("pme", "integrate", "bonded", "rebuild")}
+    log(f"nonbonded {nb_ms * 1e3:.2f} us/launch over {pairs} pairs -> {achieved:.3f} TFLOP/s; "
+        + ", ".join(f"{k} {v * 1e3:.2f} us" for k, v in kernels.items()))
+
+    result = {
+        "metric": METRIC,
+        "value": ns_day,
+        "unit": "ns/day",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms_per_step,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": ns_day / REFERENCE_NS_PER_DAY,
+        "baseline_reference": "74 ns/day, same composition (ala-wat-big), Apple M-series WebGPU, docs/STATUS.md:199-203",
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {
+            "workload": f"{args.workload}: ACE-ALA-NME + 944 TIP3P, {system.atom_count} atoms, "
+                        f"{system.box[0]:.4f} nm cube, NPT 1 bar 300 K, MC barostat every 25 steps, 2 fs, "
+                        f"rc 0.9 nm, PME {pme['grid'][0]}^3 order 5",
+            "atoms": system.atom_count,
+            "skin_nm": diag["skin"],
+            "lanes_per_atom": diag["lanes_per_atom"],
+            "block": args.block,
+        },
+        "roofline": {
+            "kernel": "nonbonded (LJ + erfc real space)",
+            "bound": "mfma",
+            "achieved": achieved,
+            "peak": FP32_PEAK_TFLOPS,
+            "unit": "TFLOP/s",
+            "frac": achieved / FP32_PEAK_TFLOPS,
+            "traffic": None,
+            "launch_us": nb_ms * 1e3,
+            "pairs": pairs,
+            "flop_per_pair": FLOP_PER_PAIR,
+            "note": "FP32 vector ALU bound; peak is MI355X's published FP32 figure",
+        },
+        "kernels_us": {k: v * 1e3 for k, v in kernels.items()},
+    }
+    if not args.no_cpu_baseline:
+        gpu_terms, _ = sim.evaluate(forces=False)
+        ref, baseline = cpu_baseline(system, pme, state, args.cpu_seconds)
+        pairs_ = {
+            "bonded": (gpu_terms["bonded"], ref["bonded"]),
+            "lennard_jones": (gpu_terms["lennard_jones"], ref["lennard_jones"]),
+            "coulomb_direct": (gpu_terms["coulomb_direct"], ref["coulomb_direct_total"]),
+            "coulomb_reciprocal": (gpu_terms["coulomb_reciprocal"], ref["reciprocal"]),
+            "total": (gpu_terms["potential"] - gpu_terms["dispersion"], ref["total"]),
+        }
+        check = {k: abs(a - b) / max(abs(b), 1.0) for k, (a, b) in pairs_.items()}
+        result["energy_check"] = {"max_rel_err": max(check.values()), "terms": check}
+        baseline["cores"] = 1
+        result["cpu_baseline"] = baseline
+        log(f"energy check vs FP64: max rel {result['energy_check']['max_rel_err']:.2e}; "
+            f"cpu baseline {baseline['ms_per_step']:.1f} ms/step")
+    sim.release()
+    print(json.dumps(result), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -345,6 +345,10 @@ int rmd_debug_fft_forward(rmd_ctx *ctx, const float *real, float *out);
 int rmd_set_lanes(rmd_ctx *ctx, int32_t lanes);
 int rmd_set_graphs(rmd_ctx *ctx, int32_t enabled);
 int rmd_get_stream(rmd_ctx *ctx, void **stream);
+/* Average launch duration (ms) of one kernel family at the current state,
+ * timed with HIP events on the context's stream over `reps` launches:
+ * 0 nonbonded, 1 PME chain, 2 integrator, 3 bonded/exceptions, 4 rebuild. */
+int rmd_profile_kernel(rmd_ctx *ctx, int32_t kernel, int32_t reps, double *avg_ms);
 
 /* Device and build identity. */
 typedef struct rmd_device_info {

@@ -245,6 +245,7 @@ SIGNATURES = {
     "rmd_set_lanes": (C.c_int, [C.c_void_p, C.c_int32]),
     "rmd_set_graphs": (C.c_int, [C.c_void_p, C.c_int32]),
     "rmd_get_stream": (C.c_int, [C.c_void_p, C.POINTER(C.c_void_p)]),
+    "rmd_profile_kernel": (C.c_int, [C.c_void_p, C.c_int32, C.c_int32, C.POINTER(C.c_double)]),
     "rmd_device_info_get": (C.c_int, [C.c_int32, C.POINTER(DeviceInfo)]),
 }
 

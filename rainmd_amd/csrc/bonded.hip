@@ -8,10 +8,9 @@
 // commute, so the result is identical on every run and there is no
 // recomputation per role.
 //
-// The harmonic/torsion terms and the restraint are evaluated in FP64 (a
-// solute's few hundred terms; the stretch r - r0 of a stiff bond cancels
-// badly in FP32). Exceptions are pairs, about one per atom, and use the same
-// FP32 arithmetic as the direct-space kernel.
+// Everything here is evaluated in FP64: a solute's few hundred bonded terms
+// (the stretch r - r0 of a stiff bond cancels badly in FP32) and about one
+// exception pair per atom; the cost is negligible next to the pair list.
 #include "kernels.h"
 
 namespace rmd {
@@ -113,33 +112,48 @@ __device__ double torsionTerm(const Dev &d, int t) {
 
 // Exception pair: scaled 1-4 LJ and Coulomb plus the Ewald erf correction
 // with the atoms' full charges, no cutoff (webgpu/kernels/nonbonded.js:128-200).
-__device__ void exceptionTerm(const Dev &d, const BoxF &box, float alpha, int t, float &elj, float &ecoul) {
+// FP64: a solvated system has one such pair per atom, and a rigid water's
+// corrections are the same large number (~150 kJ/mol) for every water, so an
+// FP32 rounding bias would add up coherently across the box.
+__device__ void exceptionTerm(const Dev &d, const BoxF &box, double alpha, int t, double &elj, double &ecoul) {
   const int2 ij = d.excAtoms[t];
-  const float4 prm = d.excParams[t];  // qq scaled, A, B, qq full (both with 1/4 pi eps0)
-  const float4 pi = d.pos[ij.x], pj = d.pos[ij.y];
-  float dx = pi.x - pj.x, dy = pi.y - pj.y, dz = pi.z - pj.z;
-  const float r2 = minimumImageR2(dx, dy, dz, box);
-  const float invR2 = __builtin_amdgcn_rcpf(r2);
-  const float invR = __builtin_amdgcn_rsqf(r2);
-  const float r = r2 * invR;
-  const float invR6 = invR2 * invR2 * invR2;
-  float scalar = 0.f;
-  if (prm.y != 0.f || prm.z != 0.f) {
-    elj += (prm.y * invR6 - prm.z) * invR6;
-    scalar += (12.0f * prm.y * invR6 - 6.0f * prm.z) * invR6 * invR2;
+  const double4 prm = d.excParams[t];  // qq scaled, A, B, qq full (both with 1/4 pi eps0)
+  double xi, yi, zi, xj, yj, zj;
+  loadPos(d, ij.x, xi, yi, zi);
+  loadPos(d, ij.y, xj, yj, zj);
+  double dx = xi - xj, dy = yi - yj, dz = zi - zj;
+  if (box.periodic) {
+    const double *bx = box.box64;
+    const double s2 = floor(dz / bx[8] + 0.5);
+    dx -= s2 * bx[6];
+    dy -= s2 * bx[7];
+    dz -= s2 * bx[8];
+    const double s1 = floor(dy / bx[4] + 0.5);
+    dx -= s1 * bx[3];
+    dy -= s1 * bx[4];
+    const double s0 = floor(dx / bx[0] + 0.5);
+    dx -= s0 * bx[0];
   }
-  if (prm.x != 0.f) {
+  const double r2 = dx * dx + dy * dy + dz * dz;
+  const double r = sqrt(r2), invR = 1.0 / r, invR2 = invR * invR, invR6 = invR2 * invR2 * invR2;
+  double scalar = 0.0;
+  if (prm.y != 0.0 || prm.z != 0.0) {
+    elj += (prm.y * invR6 - prm.z) * invR6;
+    scalar += (12.0 * prm.y * invR6 - 6.0 * prm.z) * invR6 * invR2;
+  }
+  if (prm.x != 0.0) {
     ecoul += prm.x * invR;
     scalar += prm.x * invR * invR2;
   }
-  if (alpha > 0.f && prm.w != 0.f) {
-    float gaussian;
-    const float unscreened = erfWithGaussian(alpha * r, gaussian);
+  if (alpha > 0.0 && prm.w != 0.0) {
+    const double ar = alpha * r;
+    const double unscreened = erf(ar);
+    const double gaussian = 1.1283791670955126 * alpha * exp(-ar * ar);
     ecoul -= prm.w * unscreened * invR;
-    scalar -= prm.w * (unscreened * invR - TWO_OVER_SQRT_PI_F * alpha * gaussian) * invR2;
+    scalar -= prm.w * (unscreened * invR - gaussian) * invR2;
   }
-  addForce(d, ij.x, (double)(scalar * dx), (double)(scalar * dy), (double)(scalar * dz));
-  addForce(d, ij.y, -(double)(scalar * dx), -(double)(scalar * dy), -(double)(scalar * dz));
+  addForce(d, ij.x, scalar * dx, scalar * dy, scalar * dz);
+  addForce(d, ij.y, -scalar * dx, -scalar * dy, -scalar * dz);
 }
 
 // Positional restraint k |minImage(r - r0)|^2, no half
@@ -170,7 +184,7 @@ __device__ double restraintTerm(const Dev &d, const BoxF &box, int t) {
 
 enum { BT_BOND = 0, BT_ANGLE, BT_TORSION, BT_LJX, BT_COULX, BT_RESTRAINT, BT_COUNT };
 
-__global__ void __launch_bounds__(256) bondedKernel(Dev d, float alpha, int terms, int energies) {
+__global__ void __launch_bounds__(256) bondedKernel(Dev d, double alphaD, int terms, int energies) {
   const BoxF &box = *d.box;
   const int t = blockIdx.x * blockDim.x + threadIdx.x;
   double e[BT_COUNT] = {0, 0, 0, 0, 0, 0};
@@ -184,8 +198,8 @@ __global__ void __launch_bounds__(256) bondedKernel(Dev d, float alpha, int term
     if (doBonded) e[BT_TORSION] = torsionTerm(d, t - base);
   } else if ((base += d.nTorsion, t < base + d.nException)) {
     if (terms & RMD_TERM_EXCEPTIONS) {
-      float elj = 0.f, ecoul = 0.f;
-      exceptionTerm(d, box, alpha, t - base, elj, ecoul);
+      double elj = 0.0, ecoul = 0.0;
+      exceptionTerm(d, box, alphaD, t - base, elj, ecoul);
       e[BT_LJX] = elj;
       e[BT_COULX] = ecoul;
     }
@@ -213,7 +227,7 @@ __global__ void __launch_bounds__(256) bondedKernel(Dev d, float alpha, int term
   }
 }
 
-int launchBonded(const Dev &d, float alpha, int terms, int energies, hipStream_t s, int *slotsUsed) {
+int launchBonded(const Dev &d, double alpha, int terms, int energies, hipStream_t s, int *slotsUsed) {
   const int total = d.nBond + d.nAngle + d.nTorsion + d.nException + d.nRestraint;
   const int blocks = (total + 255) / 256;
   if (blocks > 0) bondedKernel<<<blocks, 256, 0, s>>>(d, alpha, terms, energies);

@@ -303,7 +303,7 @@ void enqueueForces(rmd_ctx *c, int terms, int energies, int slots[E_TERM_COUNT])
   }
   if (terms & (RMD_TERM_BONDED | RMD_TERM_EXCEPTIONS | RMD_TERM_RESTRAINT)) {
     int b = 0;
-    launchBonded(d, alpha, terms, energies, c->stream, &b);
+    launchBonded(d, c->pme.alpha, terms, energies, c->stream, &b);
     slots[E_BOND] = slots[E_ANGLE] = slots[E_TORSION] = slots[E_LJ_EXCEPTION] = slots[E_COUL_EXCEPTION] =
         slots[E_RESTRAINT] = b;
   }
@@ -622,15 +622,15 @@ void create(rmd_ctx *c, const System &sys, const rmd_options &o) {
     upload(d.torsionAtoms, ta, c->stream);
     upload(d.torsionParams, tp, c->stream);
     std::vector<int2> ea(d.nException);
-    std::vector<float4> ep(d.nException);
+    std::vector<double4> ep(d.nException);
     for (int k = 0; k < d.nException; ++k) {
       const int i = sys.exceptions.a[k], j = sys.exceptions.b[k];
       ea[k] = make_int2(i, j);
-      ep[k] = make_float4((float)(ONE_4PI_EPS0 * sys.exceptions.chargeProduct[k]), (float)sys.exceptions.ljA[k],
-                          (float)sys.exceptions.ljB[k], (float)(ONE_4PI_EPS0 * t.charges[i] * t.charges[j]));
+      ep[k] = make_double4(ONE_4PI_EPS0 * sys.exceptions.chargeProduct[k], sys.exceptions.ljA[k],
+                           sys.exceptions.ljB[k], ONE_4PI_EPS0 * t.charges[i] * t.charges[j]);
     }
     d.excAtoms = devAlloc<int2>(ea.size(), c->held);
-    d.excParams = devAlloc<float4>(ep.size(), c->held);
+    d.excParams = devAlloc<double4>(ep.size(), c->held);
     upload(d.excAtoms, ea, c->stream);
     upload(d.excParams, ep, c->stream);
   }
@@ -727,6 +727,10 @@ void create(rmd_ctx *c, const System &sys, const rmd_options &o) {
 void setState(rmd_ctx *c, const double *pos, const double *vel, int64_t step, int supplied) {
   const System &s = c->sys;
   const int n = s.topology.atomCount;
+  for (int k = 0; k < 3 * n; ++k)
+    if (!std::isfinite(pos[k]) || (vel && !std::isfinite(vel[k])))
+      throw Error(RMD_E_NONFINITE, "setState was given a non-finite coordinate or velocity for atom " +
+                                       std::to_string(k / 3));
   std::vector<float4> p(c->d.nPad, make_float4(0, 0, 0, 0)), v(c->d.nPad, make_float4(0, 0, 0, 0));
   for (int a = 0; a < n; ++a) {
     p[a] = make_float4((float)pos[3 * a], (float)pos[3 * a + 1], (float)pos[3 * a + 2], (float)s.topology.charges[a]);
@@ -1230,6 +1234,49 @@ int rmd_set_graphs(rmd_ctx *c, int32_t enabled) {
   CTX_GUARD(c, {
     c->useGraphs = enabled != 0;
     invalidateGraphs(c);
+  });
+}
+
+// Average duration of one launch of a kernel family, timed with HIP events
+// on the context's stream at the current state (the list is rebuilt first so
+// the nonbonded kernel sees a fresh list). kernel: 0 nonbonded, 1 PME chain,
+// 2 integrator (open), 3 bonded/exceptions, 4 neighbour rebuild chain.
+int rmd_profile_kernel(rmd_ctx *c, int32_t kernel, int32_t reps, double *avg_ms) {
+  CTX_GUARD(c, {
+    if (!avg_ms || reps <= 0) throw Error(RMD_E_INVALID, "bad arguments");
+    evaluate(c, RMD_TERM_ALL, true, nullptr);
+    hipEvent_t a, b;
+    HIP_CHECK(hipEventCreate(&a));
+    HIP_CHECK(hipEventCreate(&b));
+    int slots[E_TERM_COUNT];
+    auto once = [&]() {
+      switch (kernel) {
+        case 0: launchNonbonded(c->d, (float)c->pme.alpha, c->lanes, 0, c->stream); break;
+        case 1: launchPme(c->d, c->p, 0, c->stream, nullptr); break;
+        case 2: launchIntegrateOpen(c->d, c->ip, 1, 0, c->stream); break;
+        case 3: launchBonded(c->d, c->pme.alpha, RMD_TERM_ALL, 0, c->stream, nullptr); break;
+        default:
+          launchForceRebuildFlag(c->d, c->stream);
+          launchRebuild(c->d, c->stream);
+          break;
+      }
+    };
+    // The integrator moves atoms; time it on a copy of the state and put the
+    // state back afterwards.
+    if (kernel == 2) snapshot(c);
+    once();
+    HIP_CHECK(hipEventRecord(a, c->stream));
+    for (int r = 0; r < reps; ++r) once();
+    HIP_CHECK(hipEventRecord(b, c->stream));
+    HIP_CHECK(hipEventSynchronize(b));
+    float ms = 0;
+    HIP_CHECK(hipEventElapsedTime(&ms, a, b));
+    *avg_ms = ms / reps;
+    (void)hipEventDestroy(a);
+    (void)hipEventDestroy(b);
+    if (kernel == 2) restore(c);
+    (void)slots;
+    evaluate(c, RMD_TERM_ALL, true, nullptr);
   });
 }
 

@@ -73,7 +73,7 @@ struct Dev {
   int4 *torsionAtoms;
   double4 *torsionParams;   // k, n, phase
   int2 *excAtoms;
-  float4 *excParams;        // qq scaled (with 1/4pi eps0), A, B, full qq (with 1/4pi eps0)
+  double4 *excParams;       // qq scaled (with 1/4pi eps0), A, B, full qq (with 1/4pi eps0)
   int *restraintAtoms;
   double4 *restraintRef;    // reference xyz and k, FP64
   // integrator units

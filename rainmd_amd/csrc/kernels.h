@@ -20,7 +20,7 @@ void launchForceRebuildFlag(const Dev &d, hipStream_t s);
 int launchNonbonded(const Dev &d, float alpha, int lanes, int energies, hipStream_t s);
 
 // ---- bonded, exceptions, restraints (bonded.hip) ----
-int launchBonded(const Dev &d, float alpha, int terms, int energies, hipStream_t s, int *slotsUsed);
+int launchBonded(const Dev &d, double alpha, int terms, int energies, hipStream_t s, int *slotsUsed);
 
 // ---- PME (pme.hip) ----
 struct PmeDev {

@@ -125,8 +125,11 @@ __device__ __forceinline__ void gridCoordinate(const PmeDev &p, const double *re
   const double u = wrapped * p.K[d];
   int b = (int)floor(u);
   double off = u - b;
-  if (b >= p.K[d]) {
-    b -= p.K[d];
+  if (b >= p.K[d]) b -= p.K[d];
+  // A non-finite coordinate must not turn into an out-of-range index.
+  if (!(b >= 0 && b < p.K[d])) {
+    b = 0;
+    off = 0.0;
   }
   base = b;
   offset = (float)off;

@@ -464,6 +464,14 @@ class GpuSimulation:
         _check(L.load().rmd_get_stream(self._h, C.byref(s)), self._h)
         return s.value
 
+    def profile_kernel(self, kernel, reps=50):
+        """Average launch duration (ms) of a kernel family, HIP events on the
+        context's stream: 'nonbonded', 'pme', 'integrate', 'bonded', 'rebuild'."""
+        ids = {"nonbonded": 0, "pme": 1, "integrate": 2, "bonded": 3, "rebuild": 4}
+        ms = C.c_double()
+        _check(L.load().rmd_profile_kernel(self._h, ids[kernel], int(reps), C.byref(ms)), self._h)
+        return ms.value
+
     def release(self):
         if getattr(self, "_h", None):
             L.load().rmd_destroy(self._h)
@@ -471,6 +479,42 @@ class GpuSimulation:
 
     def __del__(self):
         self.release()
+
+
+# ---- initial velocities ----------------------------------------------------
+
+def _mix(v):
+    v = v ^ (v >> np.uint32(16))
+    v = v * np.uint32(0x7feb352d)
+    v = v ^ (v >> np.uint32(15))
+    v = v * np.uint32(0x846ca68b)
+    return v ^ (v >> np.uint32(16))
+
+
+def uniform_at(seed, a, b, c):
+    """uniformAt (shared/random.js:59-64), vectorised over numpy uint32 arrays."""
+    with np.errstate(over="ignore"):
+        a = np.asarray(a, dtype=np.uint32)
+        st = _mix(np.uint32(seed) ^ (a * np.uint32(0x9e3779b1)))
+        st = _mix(st ^ (np.asarray(b, dtype=np.uint32) * np.uint32(0x85ebca6b)))
+        st = _mix(st ^ (np.asarray(c, dtype=np.uint32) * np.uint32(0xc2b2ae35)))
+    return ((st >> np.uint32(9)).astype(np.float64) + 0.5) * (1.0 / 8388608.0)
+
+
+def maxwell_boltzmann(system, temperature, seed=0):
+    """setVelocitiesToTemperature (cpu/integrate/langevin.js:211-230): per atom
+    gaussianTriple(seed, atom, 0x5eed) scaled by sqrt(kT/m); massless atoms
+    get zero."""
+    n = system.atom_count
+    atoms = np.arange(n, dtype=np.uint32)
+    u = [uniform_at(seed, atoms, 0x5eed, k) for k in range(4)]
+    r1 = np.sqrt(-2.0 * np.log(u[0]))
+    r2 = np.sqrt(-2.0 * np.log(u[2]))
+    g = np.stack([r1 * np.cos(2 * np.pi * u[1]), r1 * np.sin(2 * np.pi * u[1]), r2 * np.cos(2 * np.pi * u[3])], 1)
+    kT = 0.008314462618 * temperature
+    m = system.masses
+    sigma = np.where(m > 0, np.sqrt(kT / np.where(m > 0, m, 1.0)), 0.0)
+    return (g * sigma[:, None]).reshape(-1)
 
 
 # ---- the synthetic workloads ------------------------------------------------

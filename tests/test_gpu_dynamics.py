@@ -1,0 +1,150 @@
+"""GPU dynamics: the fused BAOAB step against the FP64 host integrator, and
+the simulation surface (constraints, thermostat, barostat, minimiser,
+capacity growth, stop) through the C ABI."""
+import numpy as np
+import pytest
+
+import oracle as O
+import rainmd_amd as R
+from helpers import ALPHA_24, fixture_system, rel
+
+pytestmark = pytest.mark.gpu
+
+
+def constraint_error(system, positions):
+    x = np.asarray(positions, dtype=np.float64).reshape(-1, 3)
+    d = np.linalg.norm(x[system.constraint_a] - x[system.constraint_b], axis=1)
+    return np.max(np.abs(d - system.constraint_distance) / system.constraint_distance)
+
+
+def test_one_step_matches_host_integrator_with_shared_noise_and_forces():
+    # tools/gpu/check-integrator.mjs:68-127: one step, supplied noise, shared forces
+    system, coords = fixture_system("ala-wat", dict(constraints="HBonds", rigid_water=True, cutoff=1.0))
+    sim = R.GpuSimulation(system, alpha=ALPHA_24, grid=[24, 24, 24], tolerance=1e-10, temperature=298.0,
+                          friction=1.0, seed=9)
+    n = system.atom_count
+    x0 = coords.positions.astype(np.float32).astype(np.float64)
+    v0 = R.maxwell_boltzmann(system, 298.0, 9).astype(np.float32).astype(np.float64)
+    noise = np.random.default_rng(1).standard_normal(3 * n).astype(np.float32)
+    # put the start on the constraint manifold first, as createState does
+    o = O.Oracle(system)
+    x0, _ = o.shake(x0, x0, 1e-12)
+    v0, _ = o.rattle(x0, v0, 1e-12)
+    x0 = x0.astype(np.float32).astype(np.float64)
+    v0 = v0.astype(np.float32).astype(np.float64)
+    sim.set_state(x0, v0, step=0, supplied_noise=True)
+    sim.set_noise(noise)
+    _, f0 = sim.evaluate()
+    sim.step_once()
+    st = sim.read_state()
+    x1, v1 = st["positions"].astype(np.float64), st["velocities"].astype(np.float64)
+    xo, vo, rc = o.langevin_step(x0, v0, f0.reshape(-1), timestep=0.002, temperature=298.0, friction=1.0,
+                                 tolerance=1e-12, max_iterations=5000, noise=noise.astype(np.float64))
+    assert rc == 0
+    _, f1 = sim.evaluate()
+    vo, rc = o.langevin_finish(xo, vo, f1.reshape(-1), tolerance=1e-12, max_iterations=5000)
+    dx = np.abs(x1 - xo).max()
+    dv = np.abs(v1 - vo).max()
+    print("one step: max |dx| %.3e nm, max |dv| %.3e nm/ps" % (dx, dv))
+    assert dx < 1e-5
+    assert dv < 1e-3 * np.abs(vo).max()
+
+
+def test_nvt_run_holds_temperature_and_constraints():
+    system, coords = fixture_system("ala-wat", dict(constraints="HBonds", rigid_water=True, cutoff=1.0))
+    sim = R.GpuSimulation(system, alpha=ALPHA_24, grid=[24, 24, 24], temperature=298.0, seed=3)
+    sim.set_state(coords.positions, R.maxwell_boltzmann(system, 298.0, 3), step=0)
+    temps = []
+    for _ in range(20):
+        sim.run(100)
+        temps.append(sim.energy()["temperature"])
+    e = sim.energy()
+    print("temperatures", np.round(temps, 1))
+    assert np.isfinite(e["potential"])
+    assert 260 < np.mean(temps[5:]) < 340
+    assert constraint_error(system, sim.read_positions()) < 1e-5
+    d = sim.diagnostics()
+    assert d["rebuilds"] > 1 and d["steps"] == 2000
+
+
+def test_npt_run_moves_the_volume():
+    system = R.workload_system("ala-389w")
+    sim = R.GpuSimulation(system, temperature=300.0, pressure=1.0, barostat_interval=25, seed=4)
+    sim.minimise(max_iterations=50, force_tolerance=1000.0)
+    sim.set_state(sim.read_positions(), R.maxwell_boltzmann(system, 300.0, 4), step=0)
+    v0 = sim.energy()["volume"]
+    sim.run(1000)
+    e = sim.energy()
+    d = sim.diagnostics()
+    print("volume", v0, "->", e["volume"], "barostat", d["barostat_accepted"], "/", d["barostat_attempts"],
+          "T", e["temperature"])
+    assert d["barostat_attempts"] >= 39
+    assert 0 < d["barostat_accepted"] < d["barostat_attempts"]
+    assert e["volume"] != v0
+    assert np.isfinite(e["potential"]) and 240 < e["temperature"] < 360
+
+
+@pytest.mark.parametrize("scale", [1.02, 0.97])
+def test_scale_volume_matches_openmm_and_oracle(scale, golden):
+    # tools/gpu/check-barostat.mjs: energy at forced scales vs ala-wat-scale-*.json
+    ref = golden(f"ala-wat-scale-{scale:.2f}.json")
+    system, coords = fixture_system("ala-wat", dict(constraints="HBonds", rigid_water=True, cutoff=1.0))
+    sim = R.GpuSimulation(system, alpha=ALPHA_24, grid=[24, 24, 24], dispersion_correction=True)
+    u = sim.scale_volume(scale)
+    pos = sim.read_positions().astype(np.float64)
+    o = O.Oracle(system, box=system.box * scale)
+    e, _ = o.potential(pos, alpha=ALPHA_24, grid=[24, 24, 24], f32_membership=True, forces=False)
+    disp = o.dispersion(np.prod(np.diag(system.box.reshape(3, 3))) * scale ** 3, 1.0)
+    print(scale, "gpu", u, "oracle", e["total"] + disp, "openmm", ref["energy"]["total"])
+    assert rel(u, e["total"] + disp) < 1e-6
+    assert rel(u, ref["energy"]["total"]) < 5e-5
+
+
+def test_restraint_matches_openmm(golden):
+    ref = golden("ala-wat-restraint-500.json")
+    system, coords = fixture_system("ala-wat", dict(constraints="HBonds", rigid_water=True, cutoff=1.0))
+    atoms = np.array(ref["restraint"]["atoms"], dtype=np.int32)
+    sim = R.GpuSimulation(system, alpha=ALPHA_24, grid=[24, 24, 24], restraint_force=ref["restraint"]["force"],
+                          restraint_atoms=atoms, restraint_reference=coords.positions)
+    sim.set_state(np.array(ref["positions"]), None, step=0)
+    e, _ = sim.evaluate(terms=R._lib.TERM_RESTRAINT, forces=False)
+    theirs = ref["energy"]["byForceGroup"]["5:CustomExternalForce"]["energy"]
+    print("restraint", e["position_restraint"], theirs)
+    assert rel(e["position_restraint"], theirs) < 1e-6
+
+
+def test_minimiser_lowers_energy_and_force():
+    system = R.workload_system("ala-389w")
+    sim = R.GpuSimulation(system, temperature=300.0)
+    e0, f0 = sim.evaluate()
+    result = sim.minimise(max_iterations=60, force_tolerance=100.0)
+    print(result)
+    assert result["potential"] < e0["potential"]
+    assert result["max_force"] < np.linalg.norm(f0, axis=1).max()
+    assert constraint_error(system, sim.read_positions()) < 1e-5
+
+
+def test_neighbour_overflow_regrows_to_identical_forces():
+    # tools/gpu/check-nonbonded.mjs: a forced-overflow arm must regrow to identical results
+    system, coords = fixture_system("ala-wat", dict(constraints="HBonds", rigid_water=True, cutoff=1.0))
+    a = R.GpuSimulation(system, alpha=ALPHA_24, grid=[24, 24, 24])
+    b = R.GpuSimulation(system, alpha=ALPHA_24, grid=[24, 24, 24], max_neighbours=64)
+    ea, fa = a.evaluate()
+    eb, fb = b.evaluate()
+    db = b.diagnostics()
+    print("capacity", db["neighbour_capacity"], "growths", db["neighbour_growths"])
+    assert db["neighbour_growths"] >= 1 and db["neighbour_capacity"] >= db["worst_neighbour_required"]
+    assert np.array_equal(fa, fb)
+    assert ea["potential"] == eb["potential"]
+
+
+def test_stop_between_blocks_and_nonfinite_input_are_errors():
+    system, coords = fixture_system("ala-wat", dict(constraints="HBonds", rigid_water=True, cutoff=1.0))
+    sim = R.GpuSimulation(system, alpha=ALPHA_24, grid=[24, 24, 24])
+    sim.request_stop()
+    with pytest.raises(R.SimulationStopped):
+        sim.run(50)
+    bad = coords.positions.copy()
+    bad[7] = np.nan
+    with pytest.raises(R.NonFiniteStateError):
+        sim.set_state(bad)

@@ -130,6 +130,15 @@ def test_energies_and_forces_workloads(workload):
     pos = sim.read_positions().astype(np.float64)
     ref, ref_forces = O.Oracle(system).potential(pos, alpha=pme["pme_alpha"], grid=pme["pme_grid"],
                                                  f32_membership=True)
+    # per-piece breakdown: direct pairs and exception pairs on their own
+    oracle = O.Oracle(system)
+    for mask, keys in [(R._lib.TERM_DIRECT, ("lj_direct", "coulomb_direct")),
+                       (R._lib.TERM_EXCEPTIONS, ("lj_exception", "coulomb_exception"))]:
+        g, _ = sim.evaluate(terms=mask, forces=False)
+        r, _ = oracle.potential(pos, alpha=pme["pme_alpha"], grid=pme["pme_grid"], terms=mask, f32_membership=True,
+                                forces=False)
+        print(workload, keys, "lj rel %.2e coul rel %.2e" % (rel(g["lennard_jones"], r[keys[0]]),
+                                                             rel(g["coulomb_direct"], r[keys[1]])))
     _check_terms(workload, gpu, ref)
     frel = rel_rms(forces, ref_forces)
     print(workload, "forces rel rms", frel)
