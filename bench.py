@@ -99,6 +99,7 @@ def main():
     ap.add_argument("--block", type=int, default=25)
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-graphs", action="store_true", help="launch kernels directly (for profilers)")
     args = ap.parse_args()
 
     if args.gpus != 1:
@@ -115,6 +116,8 @@ def main():
     if args.lanes > 0:
         opts["lanes_per_atom"] = args.lanes
     sim = R.GpuSimulation(system, **opts)
+    if args.no_graphs:
+        sim.set_graphs(False)
     diag = sim.diagnostics()
     pme = {"alpha": diag["pme_alpha"], "grid": diag["pme_grid"]}
     log(f"workload {args.workload}: {system.atom_count} atoms, grid {pme['grid']}, skin {diag['skin']:.3f}")
@@ -171,6 +174,7 @@ def main():
             "skin_nm": diag["skin"],
             "lanes_per_atom": diag["lanes_per_atom"],
             "block": args.block,
+            "graphs": not args.no_graphs,
         },
         "roofline": {
             "kernel": "nonbonded (LJ + erfc real space)",
