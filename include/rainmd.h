@@ -303,6 +303,7 @@ typedef struct rmd_diagnostics {
   double pme_alpha;
   int32_t lanes_per_atom;
   int32_t water_units, cluster_units, free_units;
+  int32_t pme_planes; /* 1: plane PME pipeline, 0: line pipeline */
 } rmd_diagnostics;
 int rmd_diagnostics_get(rmd_ctx *ctx, rmd_diagnostics *out);
 
@@ -344,6 +345,9 @@ int rmd_debug_fft_forward(rmd_ctx *ctx, const float *real, float *out);
  * timing on that stream). */
 int rmd_set_lanes(rmd_ctx *ctx, int32_t lanes);
 int rmd_set_graphs(rmd_ctx *ctx, int32_t enabled);
+/* PME pipeline: 0 automatic (planes when one x plane of the grid fits in
+ * LDS), 1 planes (RMD_E_INVALID when it does not fit), 2 lines. */
+int rmd_set_pme_pipeline(rmd_ctx *ctx, int32_t mode);
 int rmd_get_stream(rmd_ctx *ctx, void **stream);
 /* Average launch duration (ms) of one kernel family at the current state,
  * timed with HIP events on the context's stream over `reps` launches:

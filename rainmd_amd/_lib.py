@@ -190,6 +190,7 @@ class Diagnostics(C.Structure):
         ("pme_alpha", C.c_double),
         ("lanes_per_atom", C.c_int32),
         ("water_units", C.c_int32), ("cluster_units", C.c_int32), ("free_units", C.c_int32),
+        ("pme_planes", C.c_int32),
     ]
 
 
@@ -244,6 +245,7 @@ SIGNATURES = {
     "rmd_debug_fft_forward": (C.c_int, [C.c_void_p, _fp, _fp]),
     "rmd_set_lanes": (C.c_int, [C.c_void_p, C.c_int32]),
     "rmd_set_graphs": (C.c_int, [C.c_void_p, C.c_int32]),
+    "rmd_set_pme_pipeline": (C.c_int, [C.c_void_p, C.c_int32]),
     "rmd_get_stream": (C.c_int, [C.c_void_p, C.POINTER(C.c_void_p)]),
     "rmd_profile_kernel": (C.c_int, [C.c_void_p, C.c_int32, C.c_int32, C.POINTER(C.c_double)]),
     "rmd_device_info_get": (C.c_int, [C.c_int32, C.POINTER(DeviceInfo)]),

@@ -83,6 +83,7 @@ struct rmd_ctx {
   bool dispersionOn = true;
   double skin = 0, slack = 2.0;
   int lanes = 16;
+  size_t maxLds = 65536;
   CellPlan plan;
   rmd_pme_parameters pme{};
   std::vector<double> moduli[3];
@@ -107,6 +108,7 @@ struct rmd_ctx {
   double *dTerms = nullptr;
   unsigned int *dMaxForce = nullptr;
   int nUnits = 0, waterUnits = 0, clusterUnits = 0, freeUnits = 0;
+  int clusterLdsSlots = 0;  // largest per-block cluster slot count held in LDS
   // graphs per block length
   std::map<int, hipGraphExec_t> graphs;
   bool useGraphs = true;
@@ -259,6 +261,61 @@ void buildUnits(rmd_ctx *c, std::vector<int4> &units, std::vector<double4> &geom
     geom.push_back(make_double4(0, 0, 0, 0));
     ++c->clusterUnits;
   }
+  // Wave-homogeneous layout: waters, then free atoms, then the clusters from
+  // a block boundary, each with its slot offset in the block's LDS (clusters
+  // too large for LDS keep global scratch). Padding units do nothing.
+  std::vector<int4> waters, frees, clusters;
+  std::vector<double4> waterGeom;
+  for (size_t u = 0; u < units.size(); ++u) {
+    if (units[u].x == UNIT_WATER) {
+      waters.push_back(units[u]);
+      waterGeom.push_back(geom[u]);
+    } else if (units[u].x == UNIT_FREE) {
+      frees.push_back(units[u]);
+    } else {
+      clusters.push_back(units[u]);
+    }
+  }
+  units.clear();
+  geom.clear();
+  const double4 none = make_double4(0, 0, 0, 0);
+  auto pad = [&](size_t multiple) {
+    while (units.size() % multiple) {
+      units.push_back(make_int4(UNIT_NONE, 0, 0, 0));
+      geom.push_back(none);
+    }
+  };
+  for (size_t k = 0; k < waters.size(); ++k) {
+    units.push_back(waters[k]);
+    geom.push_back(waterGeom[k]);
+  }
+  pad(64);
+  for (const int4 &u : frees) {
+    units.push_back(u);
+    geom.push_back(none);
+  }
+  c->clusterLdsSlots = 0;
+  if (!clusters.empty()) pad(INTEGRATE_BLOCK);
+  int used = 0;
+  for (int4 u : clusters) {
+    const int size = atomStart[u.y + 1] - atomStart[u.y];
+    if (units.size() % INTEGRATE_BLOCK == 0) used = 0;
+    if (size > CLUSTER_LDS_SLOTS) {
+      u.z = 1;
+      u.w = 0;
+    } else {
+      if (used + size > CLUSTER_LDS_SLOTS) {
+        pad(INTEGRATE_BLOCK);
+        used = 0;
+      }
+      u.z = 0;
+      u.w = used;
+      used += size;
+      c->clusterLdsSlots = std::max(c->clusterLdsSlots, used);
+    }
+    units.push_back(u);
+    geom.push_back(none);
+  }
 }
 
 void setupIntegrator(rmd_ctx *c, int suppliedNoise) {
@@ -271,6 +328,7 @@ void setupIntegrator(rmd_ctx *c, int suppliedNoise) {
   c->ip.seed = c->seed;
   c->ip.maxIterations = c->maxIterations;
   c->ip.suppliedNoise = suppliedNoise;
+  c->ip.clusterLdsBytes = (int)(c->clusterLdsSlots * CLUSTER_SLOT * sizeof(double));
   const double half = std::max(0.0, 0.5 * c->plan.skin - 1e-4);
   c->ip.dispThreshold2 = half * half;
 }
@@ -551,6 +609,11 @@ void create(rmd_ctx *c, const System &sys, const rmd_options &o) {
     throw Error(RMD_E_DEVICE, "no HIP device is available; this library runs on an MI355X (gfx950)");
   HIP_CHECK(hipSetDevice(c->device));
   HIP_CHECK(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
+  {
+    int lds = 0;
+    if (hipDeviceGetAttribute(&lds, hipDeviceAttributeMaxSharedMemoryPerBlock, c->device) == hipSuccess && lds > 0)
+      c->maxLds = (size_t)lds;
+  }
 
   const int n = t.atomCount;
   Dev &d = c->d;
@@ -669,7 +732,7 @@ void create(rmd_ctx *c, const System &sys, const rmd_options &o) {
     d.clusterAtoms = devAlloc<int>(atoms.size(), c->held);
     d.clusterCons = devAlloc<int4>(cons.size(), c->held);
     d.clusterConD = devAlloc<double>(conD.size(), c->held);
-    d.clusterScratch = devAlloc<double>(12 * std::max<size_t>(1, atoms.size()), c->held);
+    d.clusterScratch = devAlloc<double>(CLUSTER_SLOT * std::max<size_t>(1, atoms.size()), c->held);
     upload(d.units, units, c->stream);
     upload(d.waterGeom, geom, c->stream);
     upload(d.clusterConStart, conStart, c->stream);
@@ -718,6 +781,7 @@ void create(rmd_ctx *c, const System &sys, const rmd_options &o) {
   p.rgrid = devAlloc<float>(grid, c->held);
   p.cgrid = devAlloc<float2>(half, c->held);
   p.influence = devAlloc<float>(half, c->held);
+  pmeConfigure(p, PME_AUTO, c->maxLds);
   writeBox(c);
   setupIntegrator(c, 0);
   // Initial state from the system's own coordinates, if it has them.
@@ -1161,6 +1225,7 @@ int rmd_diagnostics_get(rmd_ctx *c, rmd_diagnostics *o) {
     o->constraint_tolerance = c->tolerance;
     o->pme_alpha = c->pme.alpha;
     o->lanes_per_atom = c->lanes;
+    o->pme_planes = c->p.planes;
     o->water_units = c->waterUnits;
     o->cluster_units = c->clusterUnits;
     o->free_units = c->freeUnits;
@@ -1215,7 +1280,7 @@ int rmd_debug_fft_forward(rmd_ctx *c, const float *real, float *out) {
     float *dr = devAlloc<float>(n, tmp);
     float2 *dc = devAlloc<float2>(nc, tmp);
     HIP_CHECK(hipMemcpy(dr, real, n * sizeof(float), hipMemcpyHostToDevice));
-    launchFftForward(c->p, dr, dc, c->stream);
+    launchFftForward(c->d, c->p, dr, dc, c->stream);
     HIP_CHECK(hipStreamSynchronize(c->stream));
     HIP_CHECK(hipMemcpy(out, dc, nc * sizeof(float2), hipMemcpyDeviceToHost));
     for (void *p : tmp) (void)hipFree(p);
@@ -1277,6 +1342,15 @@ int rmd_profile_kernel(rmd_ctx *c, int32_t kernel, int32_t reps, double *avg_ms)
     if (kernel == 2) restore(c);
     (void)slots;
     evaluate(c, RMD_TERM_ALL, true, nullptr);
+  });
+}
+
+int rmd_set_pme_pipeline(rmd_ctx *c, int32_t mode) {
+  CTX_GUARD(c, {
+    if (mode < PME_AUTO || mode > PME_LINES) throw Error(RMD_E_INVALID, "PME pipeline must be 0, 1 or 2");
+    if (!pmeConfigure(c->p, mode, c->maxLds))
+      throw Error(RMD_E_INVALID, "one x plane of this PME grid does not fit in LDS");
+    invalidateGraphs(c);
   });
 }
 

@@ -83,7 +83,7 @@ struct Dev {
   int *clusterConStart, *clusterAtomStart, *clusterAtoms;
   int4 *clusterCons;        // local slot a, local slot b, constraint index, unused
   double *clusterConD;      // constrained length per cluster constraint
-  double *clusterScratch;   // 12 doubles per cluster atom slot
+  double *clusterScratch;   // CLUSTER_SLOT doubles per cluster atom slot (clusters too large for LDS)
   // noise
   float *noise;             // 3N supplied normals (optional)
   // energies and scalars

@@ -10,15 +10,14 @@
 //   B R A S O R A S [forces] B R
 // Every constraint cluster is independent, so one thread owns one unit - a
 // free atom, a rigid water (SETTLE, closed form, FP64) or a general cluster
-// (iterative SHAKE/RATTLE in FP64) - and runs the whole B R A S O R A S
-// sequence for it in registers with no synchronisation. The closing B R of
-// step n is folded into the start of step n+1's launch, so a step is this
-// kernel plus the force kernels.
+// (iterative SHAKE/RATTLE in FP64, its working state in the block's LDS) -
+// and runs the whole B R A S O R A S sequence for it with no
+// synchronisation. The host lays the units out so that every wave holds one
+// kind. The closing B R of step n is folded into the start of step n+1's
+// launch, so a step is this kernel plus the force kernels.
 #include "kernels.h"
 
 namespace rmd {
-
-enum { UNIT_FREE = 0, UNIT_WATER = 1, UNIT_CLUSTER = 2 };
 
 struct Atom {
   double x[3], v[3], f[3];
@@ -219,19 +218,43 @@ __device__ void driftWater(Atom *w, double h, const double4 geom) {
     }
 }
 
-// ---- general clusters (iterative, FP64, in per-slot scratch) -------------
-// Scratch per cluster atom slot: x[3], ref[3], v[3], corr[3].
+// ---- general clusters (iterative, FP64) -------------------------------------
+// Working state per cluster atom slot (CLUSTER_SLOT doubles): x[3], ref[3],
+// v[3], corr[3], inverse mass. S points at the cluster's first slot, in LDS
+// or (clusters too large for LDS) in global scratch.
 
-__device__ void clusterShake(const Dev &d, const IntegratorParams &ip, int cluster, double h) {
+__device__ __forceinline__ double *clusterState(const Dev &d, const int4 desc, double *lds) {
+  const int ab = d.clusterAtomStart[desc.y];
+  return desc.z ? d.clusterScratch + (size_t)CLUSTER_SLOT * ab : lds + CLUSTER_SLOT * desc.w;
+}
+
+__device__ void clusterLoad(const Dev &d, int cluster, double *S) {
+  const int ab = d.clusterAtomStart[cluster], ae = d.clusterAtomStart[cluster + 1];
+  for (int s = ab; s < ae; ++s) {
+    const int a = d.clusterAtoms[s];
+    const float4 p = d.pos[a], v = d.vel[a];
+    double *x = S + CLUSTER_SLOT * (s - ab);
+    x[0] = p.x;
+    x[1] = p.y;
+    x[2] = p.z;
+    x[6] = v.x;
+    x[7] = v.y;
+    x[8] = v.z;
+    x[12] = v.w;
+  }
+}
+
+__device__ void clusterShake(const Dev &d, const IntegratorParams &ip, int cluster, double *S, double h) {
   const int cb = d.clusterConStart[cluster], ce = d.clusterConStart[cluster + 1];
   const int ab = d.clusterAtomStart[cluster], ae = d.clusterAtomStart[cluster + 1];
-  double *S = d.clusterScratch;
-  for (int s = ab; s < ae; ++s)
+  for (int s = 0; s < ae - ab; ++s) {
+    double *x = S + CLUSTER_SLOT * s;
     for (int c = 0; c < 3; ++c) {
-      S[12 * s + 3 + c] = S[12 * s + c];
-      S[12 * s + c] += h * S[12 * s + 6 + c];
-      S[12 * s + 9 + c] = 0.0;
+      x[3 + c] = x[c];
+      x[c] += h * x[6 + c];
+      x[9 + c] = 0.0;
     }
+  }
   int iterations = 0;
   bool converged = false, stalled = false;
   double previous = 1e300;
@@ -240,8 +263,8 @@ __device__ void clusterShake(const Dev &d, const IntegratorParams &ip, int clust
     for (int k = cb; k < ce; ++k) {
       const int4 con = d.clusterCons[k];
       const double target = d.clusterConD[k];
-      double *pa = S + 12 * con.x, *pb = S + 12 * con.y;
-      const double ia = d.vel[d.clusterAtoms[con.x]].w, ib = d.vel[d.clusterAtoms[con.y]].w;
+      double *pa = S + CLUSTER_SLOT * (con.x - ab), *pb = S + CLUSTER_SLOT * (con.y - ab);
+      const double ia = pa[12], ib = pb[12];
       const double dx = pa[0] - pb[0], dy = pa[1] - pb[1], dz = pa[2] - pb[2];
       const double err = dx * dx + dy * dy + dz * dz - target * target;
       const double rel = fabs(err) / (target * target);
@@ -271,15 +294,17 @@ __device__ void clusterShake(const Dev &d, const IntegratorParams &ip, int clust
     previous = worst;
   }
   const double ih = 1.0 / h;
-  for (int s = ab; s < ae; ++s)
-    for (int c = 0; c < 3; ++c) S[12 * s + 6 + c] += S[12 * s + 9 + c] * ih;
+  for (int s = 0; s < ae - ab; ++s) {
+    double *x = S + CLUSTER_SLOT * s;
+    for (int c = 0; c < 3; ++c) x[6 + c] += x[9 + c] * ih;
+  }
   atomicMax(&d.sc->shakeMaxIterations, (unsigned int)iterations);
   if (!converged && !stalled) atomicMax(&d.sc->shakeFailed, 1u);
 }
 
-__device__ void clusterRattle(const Dev &d, const IntegratorParams &ip, int cluster) {
+__device__ void clusterRattle(const Dev &d, const IntegratorParams &ip, int cluster, double *S) {
   const int cb = d.clusterConStart[cluster], ce = d.clusterConStart[cluster + 1];
-  double *S = d.clusterScratch;
+  const int ab = d.clusterAtomStart[cluster];
   bool converged = false, stalled = false;
   double previous = 1e300;
   for (int it = 0; it < ip.maxIterations; ++it) {
@@ -287,8 +312,8 @@ __device__ void clusterRattle(const Dev &d, const IntegratorParams &ip, int clus
     for (int k = cb; k < ce; ++k) {
       const int4 con = d.clusterCons[k];
       const double target = d.clusterConD[k];
-      double *pa = S + 12 * con.x, *pb = S + 12 * con.y;
-      const double ia = d.vel[d.clusterAtoms[con.x]].w, ib = d.vel[d.clusterAtoms[con.y]].w;
+      double *pa = S + CLUSTER_SLOT * (con.x - ab), *pb = S + CLUSTER_SLOT * (con.y - ab);
+      const double ia = pa[12], ib = pb[12];
       const double r[3] = {pa[0] - pb[0], pa[1] - pb[1], pa[2] - pb[2]};
       const double vr[3] = {pa[6] - pb[6], pa[7] - pb[7], pa[8] - pb[8]};
       const double proj = r[0] * vr[0] + r[1] * vr[1] + r[2] * vr[2];
@@ -316,15 +341,16 @@ __device__ void clusterRattle(const Dev &d, const IntegratorParams &ip, int clus
   if (!converged && !stalled) atomicMax(&d.sc->rattleFailed, 1u);
 }
 
-__device__ void clusterKick(const Dev &d, int cluster, double h) {
+__device__ void clusterKick(const Dev &d, int cluster, double *S, double h) {
   const int ab = d.clusterAtomStart[cluster], ae = d.clusterAtomStart[cluster + 1];
   for (int s = ab; s < ae; ++s) {
-    const int a = d.clusterAtoms[s];
-    const double im = d.vel[a].w;
+    double *x = S + CLUSTER_SLOT * (s - ab);
+    const double im = x[12];
     if (im <= 0) continue;
+    const int a = d.clusterAtoms[s];
     const double f[3] = {(double)d.force[a] * FORCE_UNSCALE, (double)d.force[d.nPad + a] * FORCE_UNSCALE,
                          (double)d.force[2 * d.nPad + a] * FORCE_UNSCALE};
-    for (int c = 0; c < 3; ++c) d.clusterScratch[12 * s + 6 + c] += h * im * f[c];
+    for (int c = 0; c < 3; ++c) x[6 + c] += h * im * f[c];
   }
 }
 
@@ -359,7 +385,8 @@ __device__ void blockMaxToWord(float v, unsigned int *word) {
   __syncthreads();
 }
 
-__global__ void __launch_bounds__(256) integrateOpenKernel(Dev d, IntegratorParams ip, int closing, int early) {
+__global__ void __launch_bounds__(INTEGRATE_BLOCK) integrateOpenKernel(Dev d, IntegratorParams ip, int closing, int early) {
+  extern __shared__ double clusterLds[];
   const int u = blockIdx.x * blockDim.x + threadIdx.x;
   const double h = ip.halfStep;
   const unsigned int step = (unsigned int)d.sc->step;
@@ -396,47 +423,38 @@ __global__ void __launch_bounds__(256) integrateOpenKernel(Dev d, IntegratorPara
         ke += kinetic(w[a]);
         disp = fmaxf(disp, storeAtom(d, atoms[a], w[a], true));
       }
-    } else {
+    } else if (desc.x == UNIT_CLUSTER) {
       const int cl = desc.y;
       const int ab = d.clusterAtomStart[cl], ae = d.clusterAtomStart[cl + 1];
-      double *S = d.clusterScratch;
-      for (int s = ab; s < ae; ++s) {
-        const int a = d.clusterAtoms[s];
-        const float4 p = d.pos[a], v = d.vel[a];
-        S[12 * s + 0] = p.x;
-        S[12 * s + 1] = p.y;
-        S[12 * s + 2] = p.z;
-        S[12 * s + 6] = v.x;
-        S[12 * s + 7] = v.y;
-        S[12 * s + 8] = v.z;
-      }
+      double *S = clusterState(d, desc, clusterLds);
+      clusterLoad(d, cl, S);
       if (closing) {
-        clusterKick(d, cl, h);
-        clusterRattle(d, ip, cl);
+        clusterKick(d, cl, S, h);
+        clusterRattle(d, ip, cl, S);
       }
-      clusterKick(d, cl, h);
-      clusterRattle(d, ip, cl);
-      clusterShake(d, ip, cl, h);
+      clusterKick(d, cl, S, h);
+      clusterRattle(d, ip, cl, S);
+      clusterShake(d, ip, cl, S, h);
       for (int s = ab; s < ae; ++s) {
-        const int a = d.clusterAtoms[s];
+        double *x = S + CLUSTER_SLOT * (s - ab);
         Atom t;
-        t.im = d.vel[a].w;
-        for (int c = 0; c < 3; ++c) t.v[c] = S[12 * s + 6 + c];
-        thermostat(d, ip, a, t, step);
-        for (int c = 0; c < 3; ++c) S[12 * s + 6 + c] = t.v[c];
+        t.im = x[12];
+        for (int c = 0; c < 3; ++c) t.v[c] = x[6 + c];
+        thermostat(d, ip, d.clusterAtoms[s], t, step);
+        for (int c = 0; c < 3; ++c) x[6 + c] = t.v[c];
       }
-      clusterRattle(d, ip, cl);
-      clusterShake(d, ip, cl, h);
+      clusterRattle(d, ip, cl, S);
+      clusterShake(d, ip, cl, S, h);
       for (int s = ab; s < ae; ++s) {
-        const int a = d.clusterAtoms[s];
+        const double *x = S + CLUSTER_SLOT * (s - ab);
         Atom t;
-        t.im = d.vel[a].w;
+        t.im = x[12];
         for (int c = 0; c < 3; ++c) {
-          t.x[c] = S[12 * s + c];
-          t.v[c] = S[12 * s + 6 + c];
+          t.x[c] = x[c];
+          t.v[c] = x[6 + c];
         }
         ke += kinetic(t);
-        disp = fmaxf(disp, storeAtom(d, a, t, true));
+        disp = fmaxf(disp, storeAtom(d, d.clusterAtoms[s], t, true));
       }
     }
   }
@@ -465,7 +483,8 @@ __global__ void __launch_bounds__(256) integrateOpenKernel(Dev d, IntegratorPara
   }
 }
 
-__global__ void __launch_bounds__(256) integrateCloseKernel(Dev d, IntegratorParams ip, int withKinetic) {
+__global__ void __launch_bounds__(INTEGRATE_BLOCK) integrateCloseKernel(Dev d, IntegratorParams ip, int withKinetic) {
+  extern __shared__ double clusterLds[];
   const int u = blockIdx.x * blockDim.x + threadIdx.x;
   const double h = ip.halfStep;
   double ke = 0.0;
@@ -487,30 +506,20 @@ __global__ void __launch_bounds__(256) integrateCloseKernel(Dev d, IntegratorPar
         ke += kinetic(w[a]);
         d.vel[atoms[a]] = make_float4((float)w[a].v[0], (float)w[a].v[1], (float)w[a].v[2], (float)w[a].im);
       }
-    } else {
+    } else if (desc.x == UNIT_CLUSTER) {
       const int cl = desc.y;
       const int ab = d.clusterAtomStart[cl], ae = d.clusterAtomStart[cl + 1];
-      double *S = d.clusterScratch;
+      double *S = clusterState(d, desc, clusterLds);
+      clusterLoad(d, cl, S);
+      clusterKick(d, cl, S, h);
+      clusterRattle(d, ip, cl, S);
       for (int s = ab; s < ae; ++s) {
-        const int a = d.clusterAtoms[s];
-        const float4 p = d.pos[a], v = d.vel[a];
-        S[12 * s + 0] = p.x;
-        S[12 * s + 1] = p.y;
-        S[12 * s + 2] = p.z;
-        S[12 * s + 6] = v.x;
-        S[12 * s + 7] = v.y;
-        S[12 * s + 8] = v.z;
-      }
-      clusterKick(d, cl, h);
-      clusterRattle(d, ip, cl);
-      for (int s = ab; s < ae; ++s) {
-        const int a = d.clusterAtoms[s];
-        const float im = d.vel[a].w;
+        const double *x = S + CLUSTER_SLOT * (s - ab);
         Atom t;
-        t.im = im;
-        for (int c = 0; c < 3; ++c) t.v[c] = S[12 * s + 6 + c];
+        t.im = x[12];
+        for (int c = 0; c < 3; ++c) t.v[c] = x[6 + c];
         ke += kinetic(t);
-        d.vel[a] = make_float4((float)t.v[0], (float)t.v[1], (float)t.v[2], im);
+        d.vel[d.clusterAtoms[s]] = make_float4((float)t.v[0], (float)t.v[1], (float)t.v[2], (float)t.im);
       }
     }
   }
@@ -657,44 +666,48 @@ __global__ void constrainPositionsKernel(Dev d, const float4 *reference, Integra
     }
     return;
   }
+  if (desc.x != UNIT_CLUSTER) return;
   const int cl = desc.y;
   const int ab = d.clusterAtomStart[cl], ae = d.clusterAtomStart[cl + 1];
-  double *S = d.clusterScratch;
+  double *S = d.clusterScratch + (size_t)CLUSTER_SLOT * ab;
   for (int s = ab; s < ae; ++s) {
     const int a = d.clusterAtoms[s];
     const float4 r = reference[a], p = d.pos[a];
+    double *x = S + CLUSTER_SLOT * (s - ab);
     // drift of length one from the reference to the trial position
-    S[12 * s + 0] = r.x;
-    S[12 * s + 1] = r.y;
-    S[12 * s + 2] = r.z;
-    S[12 * s + 6] = (double)p.x - r.x;
-    S[12 * s + 7] = (double)p.y - r.y;
-    S[12 * s + 8] = (double)p.z - r.z;
+    x[0] = r.x;
+    x[1] = r.y;
+    x[2] = r.z;
+    x[6] = (double)p.x - r.x;
+    x[7] = (double)p.y - r.y;
+    x[8] = (double)p.z - r.z;
+    x[12] = d.vel[a].w;
   }
-  clusterShake(d, ip, cl, 1.0);
+  clusterShake(d, ip, cl, S, 1.0);
   for (int s = ab; s < ae; ++s) {
     const int a = d.clusterAtoms[s];
+    const double *x = S + CLUSTER_SLOT * (s - ab);
     float4 p = d.pos[a];
-    p.x = (float)S[12 * s + 0];
-    p.y = (float)S[12 * s + 1];
-    p.z = (float)S[12 * s + 2];
+    p.x = (float)x[0];
+    p.y = (float)x[1];
+    p.z = (float)x[2];
     d.pos[a] = p;
   }
 }
 
 // ---- launchers ----------------------------------------------------------------
 
-static int unitBlocks(const Dev &d) { return (d.nUnits + 255) / 256; }
+static int unitBlocks(const Dev &d) { return (d.nUnits + INTEGRATE_BLOCK - 1) / INTEGRATE_BLOCK; }
 
 int launchIntegrateOpen(const Dev &d, const IntegratorParams &ip, int closing, int early, hipStream_t s) {
   const int blocks = unitBlocks(d);
-  integrateOpenKernel<<<blocks, 256, 0, s>>>(d, ip, closing, early);
+  integrateOpenKernel<<<blocks, INTEGRATE_BLOCK, ip.clusterLdsBytes, s>>>(d, ip, closing, early);
   return blocks;
 }
 
 int launchIntegrateClose(const Dev &d, const IntegratorParams &ip, int withKinetic, hipStream_t s) {
   const int blocks = unitBlocks(d);
-  integrateCloseKernel<<<blocks, 256, 0, s>>>(d, ip, withKinetic);
+  integrateCloseKernel<<<blocks, INTEGRATE_BLOCK, ip.clusterLdsBytes, s>>>(d, ip, withKinetic);
   return blocks;
 }
 
@@ -725,7 +738,7 @@ void launchDisplace(const Dev &d, float scale, float cap, hipStream_t s) {
 }
 
 void launchConstrainPositions(const Dev &d, const float4 *reference, const IntegratorParams &ip, hipStream_t s) {
-  constrainPositionsKernel<<<unitBlocks(d), 256, 0, s>>>(d, reference, ip);
+  constrainPositionsKernel<<<unitBlocks(d), INTEGRATE_BLOCK, 0, s>>>(d, reference, ip);
 }
 
 }  // namespace rmd

@@ -32,19 +32,33 @@ struct PmeDev {
   float *influence;         // K0*K1*K2h: pref * kernel(m) (0 at m = 0)
   float2 *twiddle[3];       // per axis, exp(+2 pi i k / K)
   int radices[3][8];        // factorisation per axis (0-terminated)
+  int planeSpan;            // max(ceil(K1/2) * K2, K2h * K1): one plane's complex work buffer
+  int planes;               // 1: plane pipeline, 0: line pipeline
 };
+enum { PME_AUTO = 0, PME_PLANES = 1, PME_LINES = 2 };
+// Chooses the pipeline (PME_AUTO: planes when a plane fits in maxLds bytes of
+// LDS). Returns false when PME_PLANES was asked for and does not fit.
+bool pmeConfigure(PmeDev &p, int mode, size_t maxLds);
 void pmeComputeInfluence(const PmeDev &p, const BoxF *box, const double *moduli[3], double alpha, hipStream_t s);
 int launchPme(const Dev &d, const PmeDev &p, int energies, hipStream_t s, int *slotsUsed);
 // Debug entry: forward R2C 3D FFT of a real grid into the half spectrum.
-void launchFftForward(const PmeDev &p, const float *real, float2 *out, hipStream_t s);
+void launchFftForward(const Dev &d, const PmeDev &p, const float *real, float2 *out, hipStream_t s);
 
 // ---- integrator and reductions (integrate.hip) ----
+// Unit kinds: a free atom, a rigid water (SETTLE), a general constraint
+// cluster (iterative SHAKE/RATTLE) and padding. A cluster unit is
+// (UNIT_CLUSTER, cluster, 1 = global scratch / 0 = LDS, LDS slot offset).
+enum { UNIT_FREE = 0, UNIT_WATER = 1, UNIT_CLUSTER = 2, UNIT_NONE = 3 };
+constexpr int INTEGRATE_BLOCK = 256;
+constexpr int CLUSTER_SLOT = 13;        // doubles per cluster atom: x, ref, v, corr, inverse mass
+constexpr int CLUSTER_LDS_SLOTS = 512;  // cluster atom slots per block held in LDS
 struct IntegratorParams {
   double halfStep, decay, noiseScale, kT, tolerance;
   double dispThreshold2;  // rebuild when max |x - x_rebuild|^2 exceeds this
   unsigned int seed;
   int maxIterations;
   int suppliedNoise;
+  int clusterLdsBytes;  // dynamic LDS per integrator block
 };
 // closing: apply the previous step's closing kick + RATTLE first.
 // early: record the in-step kinetic energy (E_KINETIC_EARLY).

@@ -1,7 +1,7 @@
 // Smooth PME reciprocal space: order-5 B-spline charge spread, a real-to-
-// complex 3D FFT done as line transforms in LDS, the convolution with the
-// Ewald influence function (energy read off it in FP64), the inverse
-// transform and the force gather.
+// complex 3D FFT done in LDS, the convolution with the Ewald influence
+// function (energy read off it in FP64), the inverse transform and the force
+// gather.
 //
 // Replaces encodePme (webgpu/kernels/pme.js:137-463) and encodeFft3d
 // (webgpu/kernels/pme-fft.js:42-197). Conventions are the reference's
@@ -9,71 +9,125 @@
 // the B-spline moduli divide the convolution, grid index for weight k is
 // (floor(u) + k - order + 1) mod K, and the inverse is unnormalised.
 //
-// MI355X shape: the charge grid is int64 fixed point (exact, order
-// independent atomics in L2); the z transform is real-to-complex with two
-// real lines packed in one complex transform; the convolution is fused into
-// the x transform so the spectrum is read and written once per pass; grid
-// sizes may carry a factor 7 (2^a 3^b 5^c 7^d).
+// MI355X shape. The default "plane" pipeline is three launches:
+//   1. one workgroup per x plane gathers the charges that land on its plane
+//      into an int64 fixed-point plane in LDS (exact and order independent,
+//      no global atomics), then runs the z (real-to-complex, two real lines
+//      per complex transform) and y transforms of that plane in LDS;
+//   2. x lines: forward transform, convolution with the influence function
+//      (and the energy), inverse transform;
+//   3. one workgroup per x plane runs the inverse y and z transforms into a
+//      real potential plane in LDS and adds each atom's share of the force
+//      from that plane.
+// Grids whose plane does not fit in LDS use the "line" pipeline: a global
+// int64 charge grid filled with atomics and one launch per axis pass.
+// Grid sizes may carry a factor 7 (2^a 3^b 5^c 7^d).
 #include "kernels.h"
 
 namespace rmd {
 
 constexpr int ORDER = 5;
-constexpr int LINES_PER_BLOCK = 4;  // one wave64 per line
+constexpr int PLANE_THREADS = 512;
+constexpr int XLINE_THREADS = 256;
+constexpr int XLINES = 8;           // x lines per workgroup in the plane pipeline
+constexpr int LINES_PER_BLOCK = 4;  // lines per workgroup in the line pipeline
 
 __device__ __forceinline__ float2 cmul(float2 a, float2 b) { return make_float2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x); }
+__device__ __forceinline__ float2 cadd(float2 a, float2 b) { return make_float2(a.x + b.x, a.y + b.y); }
+__device__ __forceinline__ float2 csub(float2 a, float2 b) { return make_float2(a.x - b.x, a.y - b.y); }
+__device__ __forceinline__ float2 cscale(float2 a, float s) { return make_float2(a.x * s, a.y * s); }
+// a + sign * i * b
+__device__ __forceinline__ float2 caddi(float2 a, float2 b, int sign) {
+  return sign > 0 ? make_float2(a.x - b.y, a.y + b.x) : make_float2(a.x + b.y, a.y - b.x);
+}
+__device__ __forceinline__ float2 csubi(float2 a, float2 b, int sign) { return caddi(a, b, -sign); }
 
-__device__ __forceinline__ float2 twiddle(const float2 *tw, int index, int sign) {
-  float2 w = tw[index];
-  if (sign < 0) w.y = -w.y;
-  return w;
+// In-register DFT of length R (2, 3, 4, 5 or 7): y_p = sum_q v_q exp(sign 2 pi i p q / R).
+__device__ __forceinline__ void smallDft(float2 *v, int R, int sign) {
+  if (R == 2) {
+    const float2 a = v[0], b = v[1];
+    v[0] = cadd(a, b);
+    v[1] = csub(a, b);
+  } else if (R == 4) {
+    const float2 s02 = cadd(v[0], v[2]), d02 = csub(v[0], v[2]);
+    const float2 s13 = cadd(v[1], v[3]), d13 = csub(v[1], v[3]);
+    v[0] = cadd(s02, s13);
+    v[2] = csub(s02, s13);
+    v[1] = caddi(d02, d13, sign);
+    v[3] = csubi(d02, d13, sign);
+  } else if (R == 3) {
+    const float c1 = -0.5f, s1 = 0.866025403784438647f;
+    const float2 t1 = cadd(v[1], v[2]);
+    const float2 t2 = make_float2(v[0].x + c1 * t1.x, v[0].y + c1 * t1.y);
+    const float2 t3 = cscale(csub(v[1], v[2]), s1);
+    v[0] = cadd(v[0], t1);
+    v[1] = caddi(t2, t3, sign);
+    v[2] = csubi(t2, t3, sign);
+  } else if (R == 5) {
+    const float c1 = 0.309016994374947424f, c2 = -0.809016994374947424f;
+    const float s1 = 0.951056516295153572f, s2 = 0.587785252292473129f;
+    const float2 a1 = cadd(v[1], v[4]), b1 = csub(v[1], v[4]);
+    const float2 a2 = cadd(v[2], v[3]), b2 = csub(v[2], v[3]);
+    const float2 x0 = v[0];
+    const float2 r1 = make_float2(x0.x + c1 * a1.x + c2 * a2.x, x0.y + c1 * a1.y + c2 * a2.y);
+    const float2 r2 = make_float2(x0.x + c2 * a1.x + c1 * a2.x, x0.y + c2 * a1.y + c1 * a2.y);
+    const float2 i1 = make_float2(s1 * b1.x + s2 * b2.x, s1 * b1.y + s2 * b2.y);
+    const float2 i2 = make_float2(s2 * b1.x - s1 * b2.x, s2 * b1.y - s1 * b2.y);
+    v[0] = cadd(x0, cadd(a1, a2));
+    v[1] = caddi(r1, i1, sign);
+    v[4] = csubi(r1, i1, sign);
+    v[2] = caddi(r2, i2, sign);
+    v[3] = csubi(r2, i2, sign);
+  } else {  // R == 7
+    const float C1 = 0.623489801858733531f, C2 = -0.222520933956314404f, C3 = -0.900968867902419126f;
+    const float S1 = 0.781831482468029809f, S2 = 0.974927912181823607f, S3 = 0.433883739117558120f;
+    const float2 a1 = cadd(v[1], v[6]), b1 = csub(v[1], v[6]);
+    const float2 a2 = cadd(v[2], v[5]), b2 = csub(v[2], v[5]);
+    const float2 a3 = cadd(v[3], v[4]), b3 = csub(v[3], v[4]);
+    const float2 x0 = v[0];
+    auto comb = [](float2 x, float p, float2 u, float q, float2 w, float r, float2 z) {
+      return make_float2(x.x + p * u.x + q * w.x + r * z.x, x.y + p * u.y + q * w.y + r * z.y);
+    };
+    const float2 zero = make_float2(0.f, 0.f);
+    const float2 r1 = comb(x0, C1, a1, C2, a2, C3, a3), i1 = comb(zero, S1, b1, S2, b2, S3, b3);
+    const float2 r2 = comb(x0, C2, a1, C3, a2, C1, a3), i2 = comb(zero, S2, b1, -S3, b2, -S1, b3);
+    const float2 r3 = comb(x0, C3, a1, C1, a2, C2, a3), i3 = comb(zero, S3, b1, -S1, b2, S2, b3);
+    v[0] = cadd(x0, cadd(a1, cadd(a2, a3)));
+    v[1] = caddi(r1, i1, sign);
+    v[6] = csubi(r1, i1, sign);
+    v[2] = caddi(r2, i2, sign);
+    v[5] = csubi(r2, i2, sign);
+    v[3] = caddi(r3, i3, sign);
+    v[4] = csubi(r3, i3, sign);
+  }
 }
 
-// A Stockham autosort FFT of one line of length K held in LDS, performed by
-// the `nt` threads of a group; `rad` lists the radices (0-terminated). Every
-// group of the block must run the same K (the stage barrier is block wide).
-// Returns the buffer holding the result.
-__device__ float2 *fftLine(float2 *a, float2 *b, int K, const int *rad, const float2 *tw, int sign, int t, int nt) {
-  float2 *src = a, *dst = b;
+// Stockham autosort FFT of `lines` lines of length K stored line-major in LDS
+// (line l at src + l * K), by all threads of the block. `rad` lists the
+// radices (0-terminated), `tw` holds exp(+2 pi i t / K) for t < K in LDS.
+// Every stage ends with a block barrier; returns the buffer with the result.
+__device__ float2 *fftBatch(float2 *src, float2 *dst, int lines, int K, const int *rad, const float2 *tw, int sign) {
   int Ns = 1;
   for (int s = 0; s < 8 && rad[s] > 0; ++s) {
     const int R = rad[s];
     const int L = K / R;
     const int twStride = K / (Ns * R);
-    for (int j = t; j < L; j += nt) {
+    const int total = lines * L;
+    for (int idx = threadIdx.x; idx < total; idx += blockDim.x) {
+      const int line = idx / L, j = idx - line * L;
+      const float2 *in = src + line * K;
+      float2 *out = dst + line * K;
       const int k = j % Ns;
       float2 v[7];
-      for (int q = 0; q < R; ++q) v[q] = src[j + q * L];
-      for (int q = 1; q < R; ++q) v[q] = cmul(v[q], twiddle(tw, (k * q * twStride) % K, sign));
-      const int base = (j / Ns) * Ns * R + k;
-      if (R == 2) {
-        dst[base] = make_float2(v[0].x + v[1].x, v[0].y + v[1].y);
-        dst[base + Ns] = make_float2(v[0].x - v[1].x, v[0].y - v[1].y);
-      } else if (R == 4) {
-        // radix-4 with the +/- i rotation chosen by the sign
-        const float2 s02 = make_float2(v[0].x + v[2].x, v[0].y + v[2].y);
-        const float2 d02 = make_float2(v[0].x - v[2].x, v[0].y - v[2].y);
-        const float2 s13 = make_float2(v[1].x + v[3].x, v[1].y + v[3].y);
-        const float2 d13 = make_float2(v[1].x - v[3].x, v[1].y - v[3].y);
-        // i*sign*d13
-        const float2 r13 = sign > 0 ? make_float2(-d13.y, d13.x) : make_float2(d13.y, -d13.x);
-        dst[base] = make_float2(s02.x + s13.x, s02.y + s13.y);
-        dst[base + Ns] = make_float2(d02.x + r13.x, d02.y + r13.y);
-        dst[base + 2 * Ns] = make_float2(s02.x - s13.x, s02.y - s13.y);
-        dst[base + 3 * Ns] = make_float2(d02.x - r13.x, d02.y - r13.y);
-      } else {
-        const int step = K / R;
-        for (int p = 0; p < R; ++p) {
-          float2 acc = v[0];
-          for (int q = 1; q < R; ++q) {
-            const float2 w = twiddle(tw, (p * q * step) % K, sign);
-            const float2 m = cmul(v[q], w);
-            acc.x += m.x;
-            acc.y += m.y;
-          }
-          dst[base + p * Ns] = acc;
-        }
+      v[0] = in[j];
+      for (int q = 1; q < R; ++q) {
+        float2 w = tw[k * q * twStride];
+        if (sign < 0) w.y = -w.y;
+        v[q] = cmul(in[j + q * L], w);
       }
+      smallDft(v, R, sign);
+      const int base = (j / Ns) * Ns * R + k;
+      for (int q = 0; q < R; ++q) out[base + q * Ns] = v[q];
     }
     __syncthreads();
     float2 *swap = src;
@@ -82,6 +136,10 @@ __device__ float2 *fftLine(float2 *a, float2 *b, int K, const int *rad, const fl
     Ns *= R;
   }
   return src;
+}
+
+__device__ __forceinline__ void loadTwiddles(float2 *dst, const float2 *src, int K) {
+  for (int k = threadIdx.x; k < K; k += blockDim.x) dst[k] = src[k];
 }
 
 // Order-5 cardinal B-spline weights and derivatives at an offset in [0, 1)
@@ -135,6 +193,268 @@ __device__ __forceinline__ void gridCoordinate(const PmeDev &p, const double *re
   offset = (float)off;
 }
 
+// Grid index of weight k for base b: (b + k - order + 1) mod K.
+__device__ __forceinline__ int gridIndex(int b, int k, int K) {
+  int g = b + k - (ORDER - 1);
+  while (g < 0) g += K;
+  while (g >= K) g -= K;
+  return g;
+}
+
+// The first weight index k with gridIndex(b, k, K) == plane, or ORDER if the
+// atom's support misses the plane (later matches are k + K, k + 2K, ...).
+__device__ __forceinline__ int planeWeight(int b, int plane, int K) {
+  int r = (plane - b + ORDER - 1) % K;
+  if (r < 0) r += K;
+  return r;
+}
+
+__device__ __forceinline__ long long chargeToFixed(float q, float wx, float wy, float wz) {
+  const float qab = (q * wx) * wy;
+  return __double2ll_rn((double)(qab * wz) * GRID_SCALE);
+}
+
+// Force on one atom from the derivatives of its fractional-coordinate
+// gradient (gu, gv, gw per grid unit), chained through the reciprocal rows.
+__device__ __forceinline__ void addGridForce(const Dev &d, const PmeDev &p, int atom, double q, float gu, float gv,
+                                             float gw) {
+  const double u = (double)gu * p.K[0] * q, v = (double)gv * p.K[1] * q, ww = (double)gw * p.K[2] * q;
+  const double *r = d.box->recip64;
+  const double fx = -(u * r[0] + v * r[1] + ww * r[2]);
+  const double fy = -(u * r[3] + v * r[4] + ww * r[5]);
+  const double fz = -(u * r[6] + v * r[7] + ww * r[8]);
+  atomicAddFixed(&d.force[atom], __double2ll_rn(fx * FORCE_SCALE));
+  atomicAddFixed(&d.force[d.nPad + atom], __double2ll_rn(fy * FORCE_SCALE));
+  atomicAddFixed(&d.force[2 * d.nPad + atom], __double2ll_rn(fz * FORCE_SCALE));
+}
+
+// ============================================================================
+// Plane pipeline
+// ============================================================================
+
+// 1. Spread onto plane i and forward z, y transforms of the plane. With
+// `real` set, the plane is read from a real K0*K1*K2 grid instead (the FFT
+// verification entry).
+__global__ void __launch_bounds__(PLANE_THREADS) pmePlaneForwardKernel(Dev d, PmeDev p, const float *real) {
+  extern __shared__ float2 lds[];
+  const int i = blockIdx.x;
+  const int K0 = p.K[0], K1 = p.K[1], K2 = p.K[2], K2h = p.K2h;
+  const int S = p.planeSpan, P = (K1 + 1) / 2;
+  const int t = threadIdx.x, nt = blockDim.x;
+  float2 *tw1 = lds, *tw2 = tw1 + K1;
+  float2 *A = tw2 + K2, *B = A + S;
+  long long *plane = reinterpret_cast<long long *>(B);
+  loadTwiddles(tw1, p.twiddle[1], K1);
+  loadTwiddles(tw2, p.twiddle[2], K2);
+  if (real) {
+    for (int idx = t; idx < P * K2; idx += nt) {
+      const int pr = idx / K2, k = idx - pr * K2;
+      const float *row = real + ((size_t)i * K1 + 2 * pr) * K2;
+      A[idx] = make_float2(row[k], 2 * pr + 1 < K1 ? row[K2 + k] : 0.f);
+    }
+  } else {
+    for (int idx = t; idx < K1 * K2; idx += nt) plane[idx] = 0;
+    __syncthreads();
+    const double *recip = d.box->recip64;
+    for (int a = t; a < d.n; a += nt) {
+      const float4 pos = d.pos[a];
+      if (pos.w == 0.f) continue;
+      int bx, by, bz;
+      float ox, oy, oz;
+      gridCoordinate(p, recip, 0, pos, bx, ox);
+      const int r = planeWeight(bx, i, K0);
+      if (r >= ORDER) continue;
+      float w[ORDER], wy[ORDER], wz[ORDER], dw[ORDER];
+      bspline5(ox, w, dw);
+      float wx = 0.f;
+      for (int k = r; k < ORDER; k += K0) wx += w[k];
+      gridCoordinate(p, recip, 1, pos, by, oy);
+      gridCoordinate(p, recip, 2, pos, bz, oz);
+      bspline5(oy, wy, dw);
+      bspline5(oz, wz, dw);
+      int iz[ORDER];
+      for (int c = 0; c < ORDER; ++c) iz[c] = gridIndex(bz, c, K2);
+      for (int b = 0; b < ORDER; ++b) {
+        unsigned long long *row = reinterpret_cast<unsigned long long *>(plane + gridIndex(by, b, K1) * K2);
+        for (int c = 0; c < ORDER; ++c)
+          atomicAdd(row + iz[c], (unsigned long long)chargeToFixed(pos.w, wx, wy[b], wz[c]));
+      }
+    }
+    __syncthreads();
+    // Two real z lines per complex line.
+    for (int idx = t; idx < P * K2; idx += nt) {
+      const int pr = idx / K2, k = idx - pr * K2;
+      const long long *row = plane + 2 * pr * K2;
+      const float re = (float)((double)row[k] * GRID_UNSCALE);
+      const float im = 2 * pr + 1 < K1 ? (float)((double)row[K2 + k] * GRID_UNSCALE) : 0.f;
+      A[idx] = make_float2(re, im);
+    }
+  }
+  __syncthreads();
+  float2 *Z = fftBatch(A, B, P, K2, p.radices[2], tw2, +1);
+  float2 *Y = (Z == A) ? B : A;
+  // Split the packed spectra: line 2p = (Z[m] + conj Z[-m]) / 2, line 2p+1 =
+  // (Z[m] - conj Z[-m]) / (2i); y lines laid out m-major.
+  for (int idx = t; idx < K2h * K1; idx += nt) {
+    const int m = idx / K1, j = idx - m * K1;
+    const float2 *zl = Z + (j >> 1) * K2;
+    const float2 z = zl[m], zc = zl[m == 0 ? 0 : K2 - m];
+    Y[idx] = (j & 1) == 0 ? make_float2(0.5f * (z.x + zc.x), 0.5f * (z.y - zc.y))
+                          : make_float2(0.5f * (z.y + zc.y), -0.5f * (z.x - zc.x));
+  }
+  __syncthreads();
+  float2 *R = fftBatch(Y, Z, K2h, K1, p.radices[1], tw1, +1);
+  float2 *out = p.cgrid + (size_t)i * K1 * K2h;
+  for (int idx = t; idx < K1 * K2h; idx += nt) {
+    const int j = idx / K2h, m = idx - j * K2h;
+    out[idx] = R[m * K1 + j];
+  }
+}
+
+// 2. x lines (j, m0 .. m0 + XLINES - 1): forward transform, then, with
+// `convolve`, the influence function (energy in FP64, each half-spectrum
+// column weighted by its multiplicity in the full spectrum) and the inverse
+// transform.
+__global__ void __launch_bounds__(XLINE_THREADS) pmeXLinesKernel(Dev d, PmeDev p, int convolve, int energies) {
+  extern __shared__ float2 lds[];
+  const int K0 = p.K[0], K1 = p.K[1], K2h = p.K2h;
+  const int chunks = (K2h + XLINES - 1) / XLINES;
+  const int j = blockIdx.x / chunks, m0 = (blockIdx.x - j * chunks) * XLINES;
+  const int lines = min(XLINES, K2h - m0);
+  const int t = threadIdx.x, nt = blockDim.x;
+  float2 *tw = lds, *A = tw + K0, *B = A + XLINES * K0;
+  loadTwiddles(tw, p.twiddle[0], K0);
+  const size_t stride = (size_t)K1 * K2h;
+  float2 *col = p.cgrid + (size_t)j * K2h + m0;
+  for (int idx = t; idx < lines * K0; idx += nt) {
+    const int x = idx / lines, l = idx - x * lines;
+    A[l * K0 + x] = col[x * stride + l];
+  }
+  __syncthreads();
+  float2 *R = fftBatch(A, B, lines, K0, p.radices[0], tw, +1);
+  double energy = 0.0;
+  if (convolve) {
+    const float *g0 = p.influence + (size_t)j * K2h + m0;
+    for (int idx = t; idx < lines * K0; idx += nt) {
+      const int x = idx / lines, l = idx - x * lines;
+      const int m = m0 + l;
+      const double weight = (m == 0 || (p.K[2] % 2 == 0 && m == p.K[2] / 2)) ? 1.0 : 2.0;
+      const float g = g0[x * stride + l];
+      const float2 f = R[l * K0 + x];
+      energy += weight * (double)g * ((double)f.x * f.x + (double)f.y * f.y);
+      R[l * K0 + x] = make_float2(2.f * g * f.x, 2.f * g * f.y);
+    }
+    __syncthreads();
+    R = fftBatch(R, R == A ? B : A, lines, K0, p.radices[0], tw, -1);
+  }
+  for (int idx = t; idx < lines * K0; idx += nt) {
+    const int x = idx / lines, l = idx - x * lines;
+    col[x * stride + l] = R[l * K0 + x];
+  }
+  if (convolve && energies) {
+    __shared__ double partial[XLINE_THREADS / 64];
+    energy = waveSum(energy);
+    if ((t & 63) == 0) partial[t >> 6] = energy;
+    __syncthreads();
+    if (t == 0) {
+      double s = 0.0;
+      for (int w = 0; w < XLINE_THREADS / 64; ++w) s += partial[w];
+      d.slots[E_RECIPROCAL * MAX_SLOTS + blockIdx.x] = s;
+    }
+  }
+}
+
+// 3. Inverse y and z transforms of plane i into the real potential, then each
+// atom whose support meets the plane takes that plane's share of its force
+// (webgpu/kernels/pme.js:281-336; cpu/forces/pme.js:222-257).
+__global__ void __launch_bounds__(PLANE_THREADS) pmePlaneGatherKernel(Dev d, PmeDev p) {
+  extern __shared__ float2 lds[];
+  const int i = blockIdx.x;
+  const int K0 = p.K[0], K1 = p.K[1], K2 = p.K[2], K2h = p.K2h;
+  const int S = p.planeSpan, P = (K1 + 1) / 2;
+  const int t = threadIdx.x, nt = blockDim.x;
+  float2 *tw1 = lds, *tw2 = tw1 + K1;
+  float2 *A = tw2 + K2, *B = A + S;
+  loadTwiddles(tw1, p.twiddle[1], K1);
+  loadTwiddles(tw2, p.twiddle[2], K2);
+  const float2 *in = p.cgrid + (size_t)i * K1 * K2h;
+  for (int idx = t; idx < K1 * K2h; idx += nt) {
+    const int j = idx / K2h, m = idx - j * K2h;
+    A[m * K1 + j] = in[idx];
+  }
+  __syncthreads();
+  float2 *Y = fftBatch(A, B, K2h, K1, p.radices[1], tw1, -1);
+  float2 *O = (Y == A) ? B : A;
+  // Hermitian extension of lines 2p and 2p+1 packed into one complex line.
+  for (int idx = t; idx < P * K2; idx += nt) {
+    const int pr = idx / K2, k = idx - pr * K2;
+    const int j0 = 2 * pr, j1 = 2 * pr + 1;
+    float2 a, b = make_float2(0.f, 0.f);
+    if (k < K2h) {
+      a = Y[k * K1 + j0];
+      if (j1 < K1) b = Y[k * K1 + j1];
+    } else {
+      a = Y[(K2 - k) * K1 + j0];
+      a.y = -a.y;
+      if (j1 < K1) {
+        b = Y[(K2 - k) * K1 + j1];
+        b.y = -b.y;
+      }
+    }
+    O[idx] = make_float2(a.x - b.y, a.y + b.x);
+  }
+  __syncthreads();
+  float2 *Z = fftBatch(O, Y, P, K2, p.radices[2], tw2, -1);
+  float *phi = reinterpret_cast<float *>(Z == A ? B : A);
+  for (int idx = t; idx < P * K2; idx += nt) {
+    const int pr = idx / K2, k = idx - pr * K2;
+    phi[2 * pr * K2 + k] = Z[idx].x;
+    if (2 * pr + 1 < K1) phi[(2 * pr + 1) * K2 + k] = Z[idx].y;
+  }
+  __syncthreads();
+  const double *recip = d.box->recip64;
+  for (int a = t; a < d.n; a += nt) {
+    const float4 pos = d.pos[a];
+    if (pos.w == 0.f) continue;
+    int bx, by, bz;
+    float ox, oy, oz;
+    gridCoordinate(p, recip, 0, pos, bx, ox);
+    const int r = planeWeight(bx, i, K0);
+    if (r >= ORDER) continue;
+    float w[ORDER], dw[ORDER], wy[ORDER], dwy[ORDER], wz[ORDER], dwz[ORDER];
+    bspline5(ox, w, dw);
+    float wx = 0.f, dwx = 0.f;
+    for (int k = r; k < ORDER; k += K0) {
+      wx += w[k];
+      dwx += dw[k];
+    }
+    gridCoordinate(p, recip, 1, pos, by, oy);
+    gridCoordinate(p, recip, 2, pos, bz, oz);
+    bspline5(oy, wy, dwy);
+    bspline5(oz, wz, dwz);
+    int iz[ORDER];
+    for (int c = 0; c < ORDER; ++c) iz[c] = gridIndex(bz, c, K2);
+    float s0 = 0.f, s1 = 0.f, s2 = 0.f;
+    for (int b = 0; b < ORDER; ++b) {
+      const float *row = phi + gridIndex(by, b, K1) * K2;
+      float r0 = 0.f, r2 = 0.f;
+      for (int c = 0; c < ORDER; ++c) {
+        const float v = row[iz[c]];
+        r0 += wz[c] * v;
+        r2 += dwz[c] * v;
+      }
+      s0 += wy[b] * r0;
+      s1 += dwy[b] * r0;
+      s2 += wy[b] * r2;
+    }
+    addGridForce(d, p, a, pos.w, dwx * s0, wx * s1, wx * s2);
+  }
+}
+
+// ============================================================================
+// Line pipeline (grids whose plane does not fit in LDS)
+// ============================================================================
+
 __global__ void __launch_bounds__(256) pmeSpreadKernel(Dev d, PmeDev p) {
   const int atom = blockIdx.x * blockDim.x + threadIdx.x;
   if (atom >= d.n) return;
@@ -150,21 +470,15 @@ __global__ void __launch_bounds__(256) pmeSpreadKernel(Dev d, PmeDev p) {
   }
   int ix[ORDER], iy[ORDER], iz[ORDER];
   for (int k = 0; k < ORDER; ++k) {
-    ix[k] = (base[0] + k - ORDER + 1 + p.K[0]) % p.K[0];
-    iy[k] = (base[1] + k - ORDER + 1 + p.K[1]) % p.K[1];
-    iz[k] = (base[2] + k - ORDER + 1 + p.K[2]) % p.K[2];
+    ix[k] = gridIndex(base[0], k, p.K[0]);
+    iy[k] = gridIndex(base[1], k, p.K[1]);
+    iz[k] = gridIndex(base[2], k, p.K[2]);
   }
-  for (int a = 0; a < ORDER; ++a) {
-    const float qa = q * w[0][a];
+  for (int a = 0; a < ORDER; ++a)
     for (int b = 0; b < ORDER; ++b) {
-      const float qab = qa * w[1][b];
       long long *row = p.qgrid + ((size_t)ix[a] * p.K[1] + iy[b]) * p.K[2];
-      for (int c = 0; c < ORDER; ++c) {
-        const long long v = __double2ll_rn((double)(qab * w[2][c]) * GRID_SCALE);
-        atomicAddFixed(row + iz[c], v);
-      }
+      for (int c = 0; c < ORDER; ++c) atomicAddFixed(row + iz[c], chargeToFixed(q, w[0][a], w[1][b], w[2][c]));
     }
-  }
 }
 
 // z pass, forward: two real lines per complex transform; the fixed-point
@@ -172,33 +486,32 @@ __global__ void __launch_bounds__(256) pmeSpreadKernel(Dev d, PmeDev p) {
 __global__ void __launch_bounds__(256) pmeZForwardKernel(PmeDev p) {
   extern __shared__ float2 lds[];
   const int K = p.K[2];
-  const int group = threadIdx.x / 64, t = threadIdx.x % 64;
-  float2 *a = lds + group * 2 * K, *b = a + K;
+  float2 *tw = lds, *A = tw + K, *B = A + LINES_PER_BLOCK * K;
+  loadTwiddles(tw, p.twiddle[2], K);
   const int lines = p.K[0] * p.K[1];
-  const int pair = blockIdx.x * LINES_PER_BLOCK + group;
-  const int l0 = 2 * pair, l1 = 2 * pair + 1;
-  const bool active = l0 < lines;
-  if (active) {
-    for (int k = t; k < K; k += 64) {
-      const size_t i0 = (size_t)l0 * K + k;
-      const float re = (float)((double)p.qgrid[i0] * GRID_UNSCALE);
-      p.qgrid[i0] = 0;
-      float im = 0.f;
-      if (l1 < lines) {
-        const size_t i1 = (size_t)l1 * K + k;
-        im = (float)((double)p.qgrid[i1] * GRID_UNSCALE);
-        p.qgrid[i1] = 0;
-      }
-      a[k] = make_float2(re, im);
+  const int pairs = (lines + 1) / 2;
+  const int first = blockIdx.x * LINES_PER_BLOCK;
+  const int count = min(LINES_PER_BLOCK, pairs - first);
+  for (int idx = threadIdx.x; idx < count * K; idx += blockDim.x) {
+    const int g = idx / K, k = idx - g * K;
+    const int l0 = 2 * (first + g), l1 = l0 + 1;
+    const size_t i0 = (size_t)l0 * K + k;
+    const float re = (float)((double)p.qgrid[i0] * GRID_UNSCALE);
+    p.qgrid[i0] = 0;
+    float im = 0.f;
+    if (l1 < lines) {
+      const size_t i1 = (size_t)l1 * K + k;
+      im = (float)((double)p.qgrid[i1] * GRID_UNSCALE);
+      p.qgrid[i1] = 0;
     }
+    A[idx] = make_float2(re, im);
   }
   __syncthreads();
-  float2 *r = fftLine(a, b, K, p.radices[2], p.twiddle[2], +1, t, 64);
-  if (!active) return;
-  for (int m = t; m < p.K2h; m += 64) {
-    const float2 z = r[m];
-    const float2 zc = r[(K - m) % K];
-    // A = (Z[m] + conj Z[-m]) / 2,  B = (Z[m] - conj Z[-m]) / (2i)
+  float2 *r = fftBatch(A, B, count, K, p.radices[2], tw, +1);
+  for (int idx = threadIdx.x; idx < count * p.K2h; idx += blockDim.x) {
+    const int g = idx / p.K2h, m = idx - g * p.K2h;
+    const int l0 = 2 * (first + g), l1 = l0 + 1;
+    const float2 z = r[g * K + m], zc = r[g * K + (m == 0 ? 0 : K - m)];
     p.cgrid[(size_t)l0 * p.K2h + m] = make_float2(0.5f * (z.x + zc.x), 0.5f * (z.y - zc.y));
     if (l1 < lines) p.cgrid[(size_t)l1 * p.K2h + m] = make_float2(0.5f * (z.y + zc.y), -0.5f * (z.x - zc.x));
   }
@@ -208,58 +521,64 @@ __global__ void __launch_bounds__(256) pmeZForwardKernel(PmeDev p) {
 __global__ void __launch_bounds__(256) pmeYKernel(PmeDev p, int sign) {
   extern __shared__ float2 lds[];
   const int K = p.K[1];
-  const int group = threadIdx.x / 64, t = threadIdx.x % 64;
-  float2 *a = lds + group * 2 * K, *b = a + K;
+  float2 *tw = lds, *A = tw + K, *B = A + LINES_PER_BLOCK * K;
+  loadTwiddles(tw, p.twiddle[1], K);
   const int lines = p.K[0] * p.K2h;
-  const int line = blockIdx.x * LINES_PER_BLOCK + group;
-  const bool active = line < lines;
-  const int x = line / p.K2h, m = line % p.K2h;
-  const size_t base = (size_t)x * p.K[1] * p.K2h + m;
-  if (active)
-    for (int k = t; k < K; k += 64) a[k] = p.cgrid[base + (size_t)k * p.K2h];
-  __syncthreads();
-  float2 *r = fftLine(a, b, K, p.radices[1], p.twiddle[1], sign, t, 64);
-  if (!active) return;
-  for (int k = t; k < K; k += 64) p.cgrid[base + (size_t)k * p.K2h] = r[k];
-}
-
-// x pass: forward transform, convolution with the influence function (the
-// reciprocal energy is summed here in FP64, each half-spectrum column
-// weighted by its multiplicity in the full spectrum), inverse transform.
-__global__ void __launch_bounds__(256) pmeXConvolveKernel(Dev d, PmeDev p, int energies) {
-  extern __shared__ float2 lds[];
-  const int K = p.K[0];
-  const int group = threadIdx.x / 64, t = threadIdx.x % 64;
-  float2 *a = lds + group * 2 * K, *b = a + K;
-  const int lines = p.K[1] * p.K2h;
-  const int line = blockIdx.x * LINES_PER_BLOCK + group;
-  const bool active = line < lines;
-  const int y = line / p.K2h, m = line % p.K2h;
-  const size_t base = (size_t)y * p.K2h + m;
-  const size_t stride = (size_t)p.K[1] * p.K2h;
-  if (active)
-    for (int k = t; k < K; k += 64) a[k] = p.cgrid[base + k * stride];
-  __syncthreads();
-  float2 *r = fftLine(a, b, K, p.radices[0], p.twiddle[0], +1, t, 64);
-  double energy = 0.0;
-  if (active) {
-    const double weight = (m == 0 || (p.K[2] % 2 == 0 && m == p.K[2] / 2)) ? 1.0 : 2.0;
-    for (int k = t; k < K; k += 64) {
-      const float g = p.influence[base + k * stride];
-      const float2 f = r[k];
-      energy += weight * (double)g * ((double)f.x * f.x + (double)f.y * f.y);
-      r[k] = make_float2(2.f * g * f.x, 2.f * g * f.y);
-    }
+  const int first = blockIdx.x * LINES_PER_BLOCK;
+  const int count = min(LINES_PER_BLOCK, lines - first);
+  for (int idx = threadIdx.x; idx < count * K; idx += blockDim.x) {
+    const int g = idx / K, k = idx - g * K;
+    const int line = first + g, x = line / p.K2h, m = line - x * p.K2h;
+    A[idx] = p.cgrid[(size_t)x * p.K[1] * p.K2h + m + (size_t)k * p.K2h];
   }
   __syncthreads();
-  float2 *other = (r == a) ? b : a;
-  float2 *s = fftLine(r, other, K, p.radices[0], p.twiddle[0], -1, t, 64);
-  if (active)
-    for (int k = t; k < K; k += 64) p.cgrid[base + k * stride] = s[k];
-  if (energies) {
+  float2 *r = fftBatch(A, B, count, K, p.radices[1], tw, sign);
+  for (int idx = threadIdx.x; idx < count * K; idx += blockDim.x) {
+    const int g = idx / K, k = idx - g * K;
+    const int line = first + g, x = line / p.K2h, m = line - x * p.K2h;
+    p.cgrid[(size_t)x * p.K[1] * p.K2h + m + (size_t)k * p.K2h] = r[idx];
+  }
+}
+
+// x pass: forward transform, convolution with the influence function (energy
+// in FP64), inverse transform.
+__global__ void __launch_bounds__(256) pmeXConvolveKernel(Dev d, PmeDev p, int convolve, int energies) {
+  extern __shared__ float2 lds[];
+  const int K = p.K[0];
+  float2 *tw = lds, *A = tw + K, *B = A + LINES_PER_BLOCK * K;
+  loadTwiddles(tw, p.twiddle[0], K);
+  const int lines = p.K[1] * p.K2h;
+  const int first = blockIdx.x * LINES_PER_BLOCK;
+  const int count = min(LINES_PER_BLOCK, lines - first);
+  const size_t stride = (size_t)p.K[1] * p.K2h;
+  for (int idx = threadIdx.x; idx < count * K; idx += blockDim.x) {
+    const int g = idx / K, k = idx - g * K;
+    A[idx] = p.cgrid[(size_t)(first + g) + k * stride];
+  }
+  __syncthreads();
+  float2 *r = fftBatch(A, B, count, K, p.radices[0], tw, +1);
+  double energy = 0.0;
+  if (convolve) {
+    for (int idx = threadIdx.x; idx < count * K; idx += blockDim.x) {
+      const int g = idx / K, k = idx - g * K;
+      const int m = (first + g) % p.K2h;
+      const double weight = (m == 0 || (p.K[2] % 2 == 0 && m == p.K[2] / 2)) ? 1.0 : 2.0;
+      const float gi = p.influence[(size_t)(first + g) + k * stride];
+      const float2 f = r[idx];
+      energy += weight * (double)gi * ((double)f.x * f.x + (double)f.y * f.y);
+      r[idx] = make_float2(2.f * gi * f.x, 2.f * gi * f.y);
+    }
+    __syncthreads();
+    r = fftBatch(r, r == A ? B : A, count, K, p.radices[0], tw, -1);
+  }
+  for (int idx = threadIdx.x; idx < count * K; idx += blockDim.x) {
+    const int g = idx / K, k = idx - g * K;
+    p.cgrid[(size_t)(first + g) + k * stride] = r[idx];
+  }
+  if (convolve && energies) {
     __shared__ double partial[4];
     energy = waveSum(energy);
-    if (t == 0) partial[group] = energy;
+    if ((threadIdx.x & 63) == 0) partial[threadIdx.x >> 6] = energy;
     __syncthreads();
     if (threadIdx.x == 0) d.slots[E_RECIPROCAL * MAX_SLOTS + blockIdx.x] = partial[0] + partial[1] + partial[2] + partial[3];
   }
@@ -270,41 +589,39 @@ __global__ void __launch_bounds__(256) pmeXConvolveKernel(Dev d, PmeDev p, int e
 __global__ void __launch_bounds__(256) pmeZInverseKernel(PmeDev p) {
   extern __shared__ float2 lds[];
   const int K = p.K[2];
-  const int group = threadIdx.x / 64, t = threadIdx.x % 64;
-  float2 *a = lds + group * 2 * K, *b = a + K;
+  float2 *tw = lds, *A = tw + K, *B = A + LINES_PER_BLOCK * K;
+  loadTwiddles(tw, p.twiddle[2], K);
   const int lines = p.K[0] * p.K[1];
-  const int pair = blockIdx.x * LINES_PER_BLOCK + group;
-  const int l0 = 2 * pair, l1 = 2 * pair + 1;
-  const bool active = l0 < lines;
-  if (active) {
-    for (int k = t; k < K; k += 64) {
-      float2 A, B = make_float2(0.f, 0.f);
-      if (k < p.K2h) {
-        A = p.cgrid[(size_t)l0 * p.K2h + k];
-        if (l1 < lines) B = p.cgrid[(size_t)l1 * p.K2h + k];
-      } else {
-        A = p.cgrid[(size_t)l0 * p.K2h + (K - k)];
-        A.y = -A.y;
-        if (l1 < lines) {
-          B = p.cgrid[(size_t)l1 * p.K2h + (K - k)];
-          B.y = -B.y;
-        }
+  const int pairs = (lines + 1) / 2;
+  const int first = blockIdx.x * LINES_PER_BLOCK;
+  const int count = min(LINES_PER_BLOCK, pairs - first);
+  for (int idx = threadIdx.x; idx < count * K; idx += blockDim.x) {
+    const int g = idx / K, k = idx - g * K;
+    const int l0 = 2 * (first + g), l1 = l0 + 1;
+    float2 a, b = make_float2(0.f, 0.f);
+    if (k < p.K2h) {
+      a = p.cgrid[(size_t)l0 * p.K2h + k];
+      if (l1 < lines) b = p.cgrid[(size_t)l1 * p.K2h + k];
+    } else {
+      a = p.cgrid[(size_t)l0 * p.K2h + (K - k)];
+      a.y = -a.y;
+      if (l1 < lines) {
+        b = p.cgrid[(size_t)l1 * p.K2h + (K - k)];
+        b.y = -b.y;
       }
-      a[k] = make_float2(A.x - B.y, A.y + B.x);
     }
+    A[idx] = make_float2(a.x - b.y, a.y + b.x);
   }
   __syncthreads();
-  float2 *r = fftLine(a, b, K, p.radices[2], p.twiddle[2], -1, t, 64);
-  if (!active) return;
-  for (int k = t; k < K; k += 64) {
-    p.rgrid[(size_t)l0 * K + k] = r[k].x;
-    if (l1 < lines) p.rgrid[(size_t)l1 * K + k] = r[k].y;
+  float2 *r = fftBatch(A, B, count, K, p.radices[2], tw, -1);
+  for (int idx = threadIdx.x; idx < count * K; idx += blockDim.x) {
+    const int g = idx / K, k = idx - g * K;
+    const int l0 = 2 * (first + g), l1 = l0 + 1;
+    p.rgrid[(size_t)l0 * K + k] = r[idx].x;
+    if (l1 < lines) p.rgrid[(size_t)l1 * K + k] = r[idx].y;
   }
 }
 
-// Force gather (webgpu/kernels/pme.js:281-336; cpu/forces/pme.js:222-257):
-// dE/dr = q * sum (dW/dr) * convolved grid, chained through the reciprocal
-// rows.
 __global__ void __launch_bounds__(256) pmeGatherKernel(Dev d, PmeDev p) {
   const int atom = blockIdx.x * blockDim.x + threadIdx.x;
   if (atom >= d.n) return;
@@ -319,9 +636,9 @@ __global__ void __launch_bounds__(256) pmeGatherKernel(Dev d, PmeDev p) {
   }
   int ix[ORDER], iy[ORDER], iz[ORDER];
   for (int k = 0; k < ORDER; ++k) {
-    ix[k] = (base[0] + k - ORDER + 1 + p.K[0]) % p.K[0];
-    iy[k] = (base[1] + k - ORDER + 1 + p.K[1]) % p.K[1];
-    iz[k] = (base[2] + k - ORDER + 1 + p.K[2]) % p.K[2];
+    ix[k] = gridIndex(base[0], k, p.K[0]);
+    iy[k] = gridIndex(base[1], k, p.K[1]);
+    iz[k] = gridIndex(base[2], k, p.K[2]);
   }
   float gu = 0.f, gv = 0.f, gw = 0.f;
   for (int a = 0; a < ORDER; ++a) {
@@ -336,15 +653,7 @@ __global__ void __launch_bounds__(256) pmeGatherKernel(Dev d, PmeDev p) {
       }
     }
   }
-  const double q = pos.w;
-  const double u = (double)gu * p.K[0] * q, v = (double)gv * p.K[1] * q, ww = (double)gw * p.K[2] * q;
-  const double *r = d.box->recip64;
-  const double fx = -(u * r[0] + v * r[1] + ww * r[2]);
-  const double fy = -(u * r[3] + v * r[4] + ww * r[5]);
-  const double fz = -(u * r[6] + v * r[7] + ww * r[8]);
-  atomicAddFixed(&d.force[atom], __double2ll_rn(fx * FORCE_SCALE));
-  atomicAddFixed(&d.force[d.nPad + atom], __double2ll_rn(fy * FORCE_SCALE));
-  atomicAddFixed(&d.force[2 * d.nPad + atom], __double2ll_rn(fz * FORCE_SCALE));
+  addGridForce(d, p, atom, pos.w, gu, gv, gw);
 }
 
 // The influence function over the half spectrum, in FP64, stored as f32:
@@ -383,26 +692,64 @@ void pmeComputeInfluence(const PmeDev &p, const BoxF *box, const double *moduli[
   pmeInfluenceKernel<<<(unsigned)((total + 255) / 256), 256, 0, s>>>(p, box, moduli[0], moduli[1], moduli[2], alpha);
 }
 
-static size_t ldsBytes(int K) { return (size_t)LINES_PER_BLOCK * 2 * K * sizeof(float2); }
+// ---- configuration and launchers --------------------------------------------
+
+static size_t planeForwardLds(const PmeDev &p) {
+  const size_t S = (size_t)p.planeSpan * sizeof(float2);
+  const size_t plane = (size_t)p.K[1] * p.K[2] * sizeof(long long);
+  return (size_t)(p.K[1] + p.K[2]) * sizeof(float2) + S + (S > plane ? S : plane);
+}
+static size_t planeGatherLds(const PmeDev &p) {
+  return (size_t)(p.K[1] + p.K[2]) * sizeof(float2) + 2 * (size_t)p.planeSpan * sizeof(float2);
+}
+static size_t xLinesLds(const PmeDev &p) { return (size_t)p.K[0] * sizeof(float2) * (1 + 2 * XLINES); }
+static size_t lineLds(int K) { return (size_t)K * sizeof(float2) * (1 + 2 * LINES_PER_BLOCK); }
+
+bool pmeConfigure(PmeDev &p, int mode, size_t maxLds) {
+  const int P = (p.K[1] + 1) / 2;
+  const int a = P * p.K[2], b = p.K2h * p.K[1];
+  p.planeSpan = a > b ? a : b;
+  const size_t need = std::max(planeForwardLds(p), planeGatherLds(p));
+  const bool fits = need <= maxLds && xLinesLds(p) <= maxLds;
+  p.planes = (mode == PME_PLANES || mode == PME_AUTO) && fits ? 1 : 0;
+  if (p.planes) {
+    (void)hipFuncSetAttribute((const void *)pmePlaneForwardKernel, hipFuncAttributeMaxDynamicSharedMemorySize,
+                              (int)planeForwardLds(p));
+    (void)hipFuncSetAttribute((const void *)pmePlaneGatherKernel, hipFuncAttributeMaxDynamicSharedMemorySize,
+                              (int)planeGatherLds(p));
+  }
+  return mode != PME_PLANES || fits;
+}
+
+static int xLineBlocks(const PmeDev &p) { return p.K[1] * ((p.K2h + XLINES - 1) / XLINES); }
 
 int launchPme(const Dev &d, const PmeDev &p, int energies, hipStream_t s, int *slotsUsed) {
+  if (p.planes) {
+    pmePlaneForwardKernel<<<p.K[0], PLANE_THREADS, planeForwardLds(p), s>>>(d, p, nullptr);
+    const int xBlocks = xLineBlocks(p);
+    pmeXLinesKernel<<<xBlocks, XLINE_THREADS, xLinesLds(p), s>>>(d, p, 1, energies);
+    pmePlaneGatherKernel<<<p.K[0], PLANE_THREADS, planeGatherLds(p), s>>>(d, p);
+    if (slotsUsed) *slotsUsed = xBlocks;
+    return xBlocks;
+  }
   const int atomBlocks = (d.n + 255) / 256;
   pmeSpreadKernel<<<atomBlocks, 256, 0, s>>>(d, p);
   const int zPairs = (p.K[0] * p.K[1] + 1) / 2;
   const int zBlocks = (zPairs + LINES_PER_BLOCK - 1) / LINES_PER_BLOCK;
-  pmeZForwardKernel<<<zBlocks, 256, ldsBytes(p.K[2]), s>>>(p);
+  pmeZForwardKernel<<<zBlocks, 256, lineLds(p.K[2]), s>>>(p);
   const int yBlocks = (p.K[0] * p.K2h + LINES_PER_BLOCK - 1) / LINES_PER_BLOCK;
-  pmeYKernel<<<yBlocks, 256, ldsBytes(p.K[1]), s>>>(p, +1);
+  pmeYKernel<<<yBlocks, 256, lineLds(p.K[1]), s>>>(p, +1);
   const int xBlocks = (p.K[1] * p.K2h + LINES_PER_BLOCK - 1) / LINES_PER_BLOCK;
-  pmeXConvolveKernel<<<xBlocks, 256, ldsBytes(p.K[0]), s>>>(d, p, energies);
-  pmeYKernel<<<yBlocks, 256, ldsBytes(p.K[1]), s>>>(p, -1);
-  pmeZInverseKernel<<<zBlocks, 256, ldsBytes(p.K[2]), s>>>(p);
+  pmeXConvolveKernel<<<xBlocks, 256, lineLds(p.K[0]), s>>>(d, p, 1, energies);
+  pmeYKernel<<<yBlocks, 256, lineLds(p.K[1]), s>>>(p, -1);
+  pmeZInverseKernel<<<zBlocks, 256, lineLds(p.K[2]), s>>>(p);
   pmeGatherKernel<<<atomBlocks, 256, 0, s>>>(d, p);
   if (slotsUsed) *slotsUsed = xBlocks;
   return xBlocks;
 }
 
-// Debug/verification entry: the forward R2C 3D transform of a real grid.
+// Verification entry: the forward R2C 3D transform of a real grid, through
+// the same kernels as the active pipeline.
 __global__ void realToFixedKernel(PmeDev p, const float *real) {
   const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= (size_t)p.K[0] * p.K[1] * p.K[2]) return;
@@ -415,34 +762,20 @@ __global__ void copyComplexKernel(PmeDev p, float2 *out) {
   out[i] = p.cgrid[i];
 }
 
-// Forward x transform alone (the verification path has no convolution).
-__global__ void __launch_bounds__(256) pmeXForwardKernel(PmeDev p) {
-  extern __shared__ float2 lds[];
-  const int K = p.K[0];
-  const int group = threadIdx.x / 64, t = threadIdx.x % 64;
-  float2 *a = lds + group * 2 * K, *b = a + K;
-  const int lines = p.K[1] * p.K2h;
-  const int line = blockIdx.x * LINES_PER_BLOCK + group;
-  const bool active = line < lines;
-  const size_t base = (size_t)(line / p.K2h) * p.K2h + line % p.K2h;
-  const size_t stride = (size_t)p.K[1] * p.K2h;
-  if (active)
-    for (int k = t; k < K; k += 64) a[k] = p.cgrid[base + k * stride];
-  __syncthreads();
-  float2 *r = fftLine(a, b, K, p.radices[0], p.twiddle[0], +1, t, 64);
-  if (!active) return;
-  for (int k = t; k < K; k += 64) p.cgrid[base + k * stride] = r[k];
-}
-
-void launchFftForward(const PmeDev &p, const float *real, float2 *out, hipStream_t s) {
-  const size_t n = (size_t)p.K[0] * p.K[1] * p.K[2];
-  realToFixedKernel<<<(unsigned)((n + 255) / 256), 256, 0, s>>>(p, real);
-  const int zPairs = (p.K[0] * p.K[1] + 1) / 2;
-  pmeZForwardKernel<<<(zPairs + LINES_PER_BLOCK - 1) / LINES_PER_BLOCK, 256, ldsBytes(p.K[2]), s>>>(p);
-  const int yBlocks = (p.K[0] * p.K2h + LINES_PER_BLOCK - 1) / LINES_PER_BLOCK;
-  pmeYKernel<<<yBlocks, 256, ldsBytes(p.K[1]), s>>>(p, +1);
-  const int xBlocks = (p.K[1] * p.K2h + LINES_PER_BLOCK - 1) / LINES_PER_BLOCK;
-  pmeXForwardKernel<<<xBlocks, 256, ldsBytes(p.K[0]), s>>>(p);
+void launchFftForward(const Dev &d, const PmeDev &p, const float *real, float2 *out, hipStream_t s) {
+  if (p.planes) {
+    pmePlaneForwardKernel<<<p.K[0], PLANE_THREADS, planeForwardLds(p), s>>>(d, p, real);
+    pmeXLinesKernel<<<xLineBlocks(p), XLINE_THREADS, xLinesLds(p), s>>>(d, p, 0, 0);
+  } else {
+    const size_t n = (size_t)p.K[0] * p.K[1] * p.K[2];
+    realToFixedKernel<<<(unsigned)((n + 255) / 256), 256, 0, s>>>(p, real);
+    const int zPairs = (p.K[0] * p.K[1] + 1) / 2;
+    pmeZForwardKernel<<<(zPairs + LINES_PER_BLOCK - 1) / LINES_PER_BLOCK, 256, lineLds(p.K[2]), s>>>(p);
+    const int yBlocks = (p.K[0] * p.K2h + LINES_PER_BLOCK - 1) / LINES_PER_BLOCK;
+    pmeYKernel<<<yBlocks, 256, lineLds(p.K[1]), s>>>(p, +1);
+    const int xBlocks = (p.K[1] * p.K2h + LINES_PER_BLOCK - 1) / LINES_PER_BLOCK;
+    pmeXConvolveKernel<<<xBlocks, 256, lineLds(p.K[0]), s>>>(d, p, 0, 0);
+  }
   const size_t nc = (size_t)p.K[0] * p.K[1] * p.K2h;
   copyComplexKernel<<<(unsigned)((nc + 255) / 256), 256, 0, s>>>(p, out);
 }

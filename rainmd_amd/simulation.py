@@ -459,6 +459,11 @@ class GpuSimulation:
     def set_graphs(self, enabled):
         _check(L.load().rmd_set_graphs(self._h, 1 if enabled else 0), self._h)
 
+    def set_pme_pipeline(self, mode):
+        """'auto', 'planes' or 'lines'."""
+        ids = {"auto": 0, "planes": 1, "lines": 2}
+        _check(L.load().rmd_set_pme_pipeline(self._h, ids[mode]), self._h)
+
     def stream(self):
         s = C.c_void_p()
         _check(L.load().rmd_get_stream(self._h, C.byref(s)), self._h)

@@ -74,10 +74,13 @@ def test_fft_matches_numpy(ala_wat):
     assert err < 20 * np.sqrt(3 * np.log2(grid[0])) * 2.0 ** -24
 
 
-@pytest.mark.parametrize("grid", [(30, 30, 30), (28, 21, 25), (27, 32, 20)])
-def test_fft_mixed_radix(grid):
+@pytest.mark.parametrize("pipeline", ["planes", "lines"])
+@pytest.mark.parametrize("grid", [(30, 30, 30), (28, 21, 25), (27, 32, 20), (35, 14, 21)])
+def test_fft_mixed_radix(grid, pipeline):
     system, _ = fixture_system("ala-wat", dict(constraints="HBonds", rigid_water=True, cutoff=1.0))
     sim = R.GpuSimulation(system, alpha=ALPHA_24, grid=list(grid))
+    sim.set_pme_pipeline(pipeline)
+    assert sim.diagnostics()["pme_planes"] == (pipeline == "planes")
     rng = np.random.default_rng(sum(grid))
     x = rng.standard_normal(grid).astype(np.float32)
     ours = sim.debug_fft_forward(x)
@@ -110,9 +113,12 @@ def test_cells_and_pairs_bit_exact(ala_wat):
         assert np.array_equal(np.sort(lists[i, :counts[i]]), ref_lists[i, :ref_counts[i]]), i
 
 
-def test_energies_and_forces_ala_wat(ala_wat):
+@pytest.mark.parametrize("pipeline", ["planes", "lines"])
+def test_energies_and_forces_ala_wat(ala_wat, pipeline):
     system, sim = ala_wat
+    sim.set_pme_pipeline(pipeline)
     gpu, forces = sim.evaluate()
+    sim.set_pme_pipeline("auto")
     pos = sim.read_positions().astype(np.float64)
     ref, ref_forces = O.Oracle(system).potential(pos, alpha=ALPHA_24, grid=[24, 24, 24], f32_membership=True)
     _check_terms("ala-wat", gpu, ref)
