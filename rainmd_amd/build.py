@@ -1,8 +1,11 @@
-"""Build the gfx950 shared library librainmd_amd.so in-tree.
+"""Build the gfx950 shared library librainmd_amd.so in-tree, and the Node
+N-API addon over it.
 
 Every source under csrc/ is compiled by hipcc for gfx950 (host C++ files go
 through the same driver so one toolchain links the result), in parallel, and
-linked into rainmd_amd/librainmd_amd.so next to this file.
+linked into rainmd_amd/librainmd_amd.so next to this file. The addon
+(node/addon.cc) is host C++ against Node's node_api.h, built with g++ into
+rainmd_amd/node/rainmd.node, linked to the library by an $ORIGIN rpath.
 """
 import os
 import subprocess
@@ -13,6 +16,9 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 BUILD = os.path.join(HERE, "_build")
 LIBRARY = os.path.join(HERE, "librainmd_amd.so")
+ADDON_SOURCE = os.path.join(HERE, "node", "addon.cc")
+ADDON = os.path.join(HERE, "node", "rainmd.node")
+NODE_INCLUDE = os.environ.get("NODE_INCLUDE", "/usr/include/node")
 SOURCES = [
     "host_amber.cpp",
     "host_system.cpp",
@@ -59,7 +65,23 @@ def build(verbose=False):
     if proc.returncode != 0:
         raise RuntimeError("link failed:\n" + proc.stderr)
     os.replace(tmp, LIBRARY)
+    build_addon()
     return LIBRARY
+
+
+def build_addon():
+    """The N-API addon; skipped (returns None) where Node's headers are absent."""
+    if not os.path.exists(os.path.join(NODE_INCLUDE, "node_api.h")):
+        return None
+    tmp = ADDON + ".tmp"
+    cmd = ["g++", "-O2", "-std=c++17", "-shared", "-fPIC", "-Wall", "-DNODE_GYP_MODULE_NAME=rainmd",
+           f"-I{NODE_INCLUDE}", ADDON_SOURCE, "-o", tmp, f"-L{HERE}", "-lrainmd_amd",
+           "-Wl,-rpath,$ORIGIN/.."]
+    proc = subprocess.run(cmd, capture_output=True, text=True)
+    if proc.returncode != 0:
+        raise RuntimeError("addon build failed:\n" + proc.stderr)
+    os.replace(tmp, ADDON)
+    return ADDON
 
 
 if __name__ == "__main__":

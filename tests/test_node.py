@@ -1,0 +1,132 @@
+"""The Node.js host (rainmd_amd/node): the N-API addon over the C ABI, the
+GpuSimulation facade, the DCD/rst7 writers and the RNG, run under the image's
+Node through the scripts in tests/node/. Known answers follow the reference's
+test/physics.test.js:253-356 (writers) and test/amber-parse.test.js:33-46
+(counts)."""
+import json
+import os
+import shutil
+import struct
+import subprocess
+
+import numpy as np
+import pytest
+
+import oracle as O
+import rainmd_amd as R
+from helpers import rel, rel_rms
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(HERE)
+NODE = shutil.which("node")
+ADDON = os.path.join(ROOT, "rainmd_amd", "node", "rainmd.node")
+
+pytestmark = pytest.mark.skipif(NODE is None, reason="no node executable")
+
+
+def _run(script, *args, timeout=120):
+    if not os.path.exists(ADDON):
+        from rainmd_amd import build
+        build.build_addon()
+    proc = subprocess.run([NODE, os.path.join(HERE, "node", script), *args], capture_output=True, text=True,
+                          timeout=timeout)
+    assert proc.returncode == 0, proc.stderr
+    return json.loads(proc.stdout.strip().splitlines()[-1])
+
+
+@pytest.fixture(scope="module")
+def cpu():
+    return _run("host_cpu.js")
+
+
+def test_addon_loads_and_builds_systems(cpu):
+    assert cpu["abi"] == 1
+    assert cpu["vac"] == dict(atoms=22, residues=3, molecules=1, bonds=21, angles=36, torsions=67, exceptions=98)
+    w = cpu["wat"]
+    assert (w["atoms"], w["constraints"], w["exceptions"], w["dof"]) == (1190, 1176, 1262, 3 * 1190 - 1176 - 3)
+    assert w["waters"] == 388
+    # the same build through the Python host
+    s = R.System.build(R.Topology.load("ala-wat.prmtop"), R.Coordinates.load("ala-wat.inpcrd", 1190),
+                       constraints="HBonds", cutoff=1.0)
+    assert np.array_equal(np.array(w["box"]), s.box)
+
+
+def test_refusals_carry_the_reference_codes(cpu):
+    assert cpu["refuseMismatch"]["code"] == -2 and "not from the same tleap run" in cpu["refuseMismatch"]["message"]
+    assert cpu["refuseCutoff"]["code"] == -3
+    if not cpu["device"]["ok"]:
+        assert cpu["createWithoutDevice"]["code"] == -4
+
+
+def test_dcd_layout_matches_openmm(cpu):
+    d = cpu["dcd"]
+    assert (d["marker"], d["cord"], d["frames"], d["interval"], d["last"]) == (84, "CORD", 2, 5, 5)
+    assert abs(d["delta"] - 0.002 / 0.04888821) < 1e-6
+    assert (d["cellFlag"], d["version"], d["atoms"]) == (1, 24, 3)
+    cell = d["cell"]
+    assert abs(cell[0] - 25) < 1e-9
+    assert abs(cell[2] - np.hypot(0.3, 2.6) * 10) < 1e-9
+    assert abs(cell[5] - np.sqrt(0.4 ** 2 + 0.5 ** 2 + 2.7 ** 2) * 10) < 1e-9
+    assert abs(cell[1] - 0.3 * 2.5 / (2.5 * np.hypot(0.3, 2.6))) < 1e-12
+    assert np.allclose(d["firstCell"], cell, rtol=0, atol=1e-12)
+    # header 276 B; per frame 56 B of cell + 3 axes of (4N + 8) B
+    assert d["length"] == 276 + 2 * (56 + 3 * (12 + 8))
+    assert np.allclose(d["firstX"], [0, 1.0, -4.0], atol=1e-6)
+    assert "needs its box" in cpu["dcdNoBox"]["message"]
+    assert "DCD writer" in cpu["dcdNaN"]["message"]
+
+
+def test_restart_round_trips_through_the_inpcrd_reader(cpu):
+    r = cpu["restart"]
+    assert r["atoms"] == 1190 and r["time"] == 1.5
+    assert r["dx"] < 1e-7 and r["dv"] < 1e-6 and r["db"] < 1e-7
+    assert r["head"][0] == "RainMD restart"
+    assert r["head"][1] == " 1190" + "1.5000000e+0".rjust(15)
+    assert "12-column" in cpu["restartOverflow"]["message"]
+    assert "restart writer" in cpu["restartNaN"]["message"]
+
+
+def test_imaging_moves_whole_molecules_by_lattice_vectors(cpu):
+    box = np.array([[2.5, 0, 0], [0.3, 2.6, 0], [0.4, 0.5, 2.7]])
+    raw = np.array([[2.6, -0.1, 5.5], [2.7, 0.1, 5.5], [-0.2, 0.3, 0.3]])
+    img = np.array(cpu["imaged"]).reshape(3, 3)
+    # molecule 0 (atoms 0, 1) moved together, by an integer lattice combination
+    shift = img[0] - raw[0]
+    assert np.allclose(img[1] - raw[1], shift)
+    n = np.linalg.solve(box.T, shift)
+    assert np.allclose(n, np.round(n), atol=1e-12)
+    # every centre inside the primary cell in fractional coordinates
+    for centre in (img[:2].mean(0), img[2]):
+        f = np.linalg.solve(box.T, centre)
+        assert np.all(f >= -1e-12) and np.all(f < 1 + 1e-12)
+
+
+def test_rng_and_velocities_match_the_python_host(cpu):
+    ref = [R.uniform_at(7, 3, 0x5eed, 1), R.uniform_at(123456, 99, 2, 3), R.uniform_at(0, 0, 0, 0)]
+    assert cpu["uniform"] == [float(x) for x in ref]
+    s = R.System.build(R.Topology.load("ala-wat.prmtop"), R.Coordinates.load("ala-wat.inpcrd", 1190),
+                       constraints="HBonds", cutoff=1.0)
+    v = R.maxwell_boltzmann(s, 300.0, 202)[:30]
+    assert np.max(np.abs(np.array(cpu["velocities"]) - v)) < 1e-14
+
+
+@pytest.mark.gpu
+def test_node_host_runs_on_the_gpu():
+    out = _run("host_gpu.js", timeout=300)
+    system = R.workload_system("ala-389w")
+    pos = np.array(out["positions"], dtype=np.float32).astype(np.float64)
+    e = out["energy"]
+    ref, _ = O.Oracle(system).potential(pos, alpha=out["pme"]["alpha"], grid=out["pme"]["grid"], f32_membership=True)
+    pairs = {"lennardJones": ref["lennard_jones"], "coulombDirect": ref["coulomb_direct_total"],
+             "coulombReciprocal": ref["reciprocal"], "bonded": ref["bonded"]}
+    for key, value in pairs.items():
+        print(key, e[key], value, rel(e[key], value))
+        assert rel(e[key], value) < 1e-6, key
+    total = e["potential"] - e["dispersion"]
+    assert rel(total, ref["total"]) < 1e-6
+    assert out["step"] == 100 and out["after"]["step"] == 100
+    assert np.isfinite(out["after"]["potential"]) and 200 < out["after"]["temperature"] < 400
+    assert out["move"] is not None and "accepted" in out["move"]
+    assert out["stopped"] == "SimulationStopped"
+    assert out["diagnostics"]["barostat"]["attempts"] >= 1
+    assert "released" in out["released"]
