@@ -82,7 +82,7 @@ struct rmd_ctx {
   int barostatInterval = 25;
   bool dispersionOn = true;
   double skin = 0, slack = 2.0;
-  int lanes = 16;
+  int lanes = 32;
   size_t maxLds = 65536;
   CellPlan plan;
   rmd_pme_parameters pme{};
@@ -248,6 +248,31 @@ void buildUnits(rmd_ctx *c, std::vector<int4> &units, std::vector<double4> &geom
     if (water) continue;
     const int cl = (int)conStart.size() - 1;
     const int base = (int)atoms.size();
+    int centre = -1;
+    if (ck.size() <= (size_t)STAR_MAX && ca.size() == ck.size() + 1)
+      for (int x : ca) {
+        bool inAll = true;
+        for (int k : ck)
+          if (s.constraints.a[k] != x && s.constraints.b[k] != x) inAll = false;
+        if (inAll) {
+          centre = x;
+          break;
+        }
+      }
+    if (centre >= 0) {
+      atoms.push_back(centre);
+      for (int k : ck) {
+        atoms.push_back(s.constraints.a[k] == centre ? s.constraints.b[k] : s.constraints.a[k]);
+        cons.push_back(make_int4((int)atoms.size() - 1, base, k, 0));
+        conD.push_back(s.constraints.distance[k]);
+      }
+      conStart.push_back((int)cons.size());
+      atomStart.push_back((int)atoms.size());
+      units.push_back(make_int4(UNIT_STAR, cl, (int)ck.size(), 0));
+      geom.push_back(make_double4(0, 0, 0, 0));
+      ++c->clusterUnits;
+      continue;
+    }
     for (int x : ca) atoms.push_back(x);
     for (int k : ck) {
       int sa = base + (int)(std::find(ca.begin(), ca.end(), s.constraints.a[k]) - ca.begin());
@@ -261,10 +286,10 @@ void buildUnits(rmd_ctx *c, std::vector<int4> &units, std::vector<double4> &geom
     geom.push_back(make_double4(0, 0, 0, 0));
     ++c->clusterUnits;
   }
-  // Wave-homogeneous layout: waters, then free atoms, then the clusters from
+  // Wave-homogeneous layout: waters, free atoms, stars, then the clusters from
   // a block boundary, each with its slot offset in the block's LDS (clusters
   // too large for LDS keep global scratch). Padding units do nothing.
-  std::vector<int4> waters, frees, clusters;
+  std::vector<int4> waters, frees, stars, clusters;
   std::vector<double4> waterGeom;
   for (size_t u = 0; u < units.size(); ++u) {
     if (units[u].x == UNIT_WATER) {
@@ -272,6 +297,8 @@ void buildUnits(rmd_ctx *c, std::vector<int4> &units, std::vector<double4> &geom
       waterGeom.push_back(geom[u]);
     } else if (units[u].x == UNIT_FREE) {
       frees.push_back(units[u]);
+    } else if (units[u].x == UNIT_STAR) {
+      stars.push_back(units[u]);
     } else {
       clusters.push_back(units[u]);
     }
@@ -291,6 +318,11 @@ void buildUnits(rmd_ctx *c, std::vector<int4> &units, std::vector<double4> &geom
   }
   pad(64);
   for (const int4 &u : frees) {
+    units.push_back(u);
+    geom.push_back(none);
+  }
+  pad(64);
+  for (const int4 &u : stars) {
     units.push_back(u);
     geom.push_back(none);
   }
@@ -589,7 +621,7 @@ void create(rmd_ctx *c, const System &sys, const rmd_options &o) {
   c->barostatInterval = o.barostat_interval > 0 ? o.barostat_interval : 25;
   c->dispersionOn = o.dispersion_correction != 0;
   c->slack = o.neighbour_slack > 0 ? o.neighbour_slack : 2.0;
-  c->lanes = o.lanes_per_atom > 0 ? o.lanes_per_atom : 16;
+  c->lanes = o.lanes_per_atom > 0 ? o.lanes_per_atom : 32;
   if (c->lanes & (c->lanes - 1) || c->lanes > 64) throw Error(RMD_E_INVALID, "lanes_per_atom must be a power of two <= 64");
   for (int k = 0; k < 9; ++k) c->box[k] = sys.box[k];
   c->skin = o.skin > 0 ? o.skin : skinFor(sys, c->timestep, c->temperature, nullptr);
@@ -639,6 +671,7 @@ void create(rmd_ctx *c, const System &sys, const rmd_options &o) {
   }
   d.cellOfAtom = devAlloc<int>(d.nPad, c->held);
   d.cellAtoms = devAlloc<int>(d.nPad, c->held);
+  d.cellPos = devAlloc<float4>(d.nPad, c->held);
   d.cellCapacity = 0;
   allocateCells(c, c->plan.cellCount);
   d.nbrCount = devAlloc<int>(d.nPad, c->held);

@@ -59,6 +59,7 @@ struct Dev {
   const BoxF *box;  // device-resident geometry (see BoxF)
   // cells (arrays sized for cellCapacity cells)
   int *cellOfAtom, *cellCount, *cellStart, *cellCursor, *cellAtoms;
+  float4 *cellPos;  // positions in cell order, the atom index in .w (int bits)
   int cellCapacity;
   // neighbours: full list, `cap` slots per atom
   int *nbrCount, *nbrList;

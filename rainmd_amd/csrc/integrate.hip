@@ -218,6 +218,120 @@ __device__ void driftWater(Atom *w, double h, const double4 geom) {
     }
 }
 
+// ---- star clusters (registers, FP64) ----------------------------------------
+// The same SHAKE/RATTLE sweeps as a general cluster, with the whole cluster
+// in registers: slot 0 is the centre, constraint k joins slot k + 1 to it.
+
+constexpr int STAR_ATOMS = STAR_MAX + 1;
+
+__device__ void starLoad(const Dev &d, int cluster, int m, Atom *w, int *atom, double *target) {
+  const int ab = d.clusterAtomStart[cluster], cb = d.clusterConStart[cluster];
+#pragma unroll
+  for (int k = 0; k < STAR_ATOMS; ++k) {
+    atom[k] = k <= m ? d.clusterAtoms[ab + k] : -1;
+    if (k <= m) {
+      loadAtom(d, atom[k], w[k]);
+    } else {
+      for (int c = 0; c < 3; ++c) w[k].x[c] = w[k].v[c] = w[k].f[c] = 0.0;
+      w[k].im = 0.0;
+    }
+  }
+#pragma unroll
+  for (int k = 0; k < STAR_MAX; ++k) target[k] = k < m ? d.clusterConD[cb + k] : 1.0;
+}
+
+__device__ void starRattle(const Dev &d, const IntegratorParams &ip, Atom *w, int m, const double *target) {
+  bool converged = false, stalled = false;
+  double previous = 1e300;
+  for (int it = 0; it < ip.maxIterations; ++it) {
+    double worst = 0;
+#pragma unroll
+    for (int k = 0; k < STAR_MAX; ++k) {
+      if (k >= m) continue;
+      Atom &a = w[k + 1], &b = w[0];
+      const double r[3] = {a.x[0] - b.x[0], a.x[1] - b.x[1], a.x[2] - b.x[2]};
+      const double vr[3] = {a.v[0] - b.v[0], a.v[1] - b.v[1], a.v[2] - b.v[2]};
+      const double proj = r[0] * vr[0] + r[1] * vr[1] + r[2] * vr[2];
+      const double rel = fabs(proj) / (target[k] * target[k]);
+      worst = fmax(worst, rel);
+      if (rel <= ip.tolerance) continue;
+      const double den = (a.im + b.im) * (r[0] * r[0] + r[1] * r[1] + r[2] * r[2]);
+      if (den < 1e-20) continue;
+      const double lam = -proj / den;
+      for (int c = 0; c < 3; ++c) {
+        a.v[c] += lam * a.im * r[c];
+        b.v[c] -= lam * b.im * r[c];
+      }
+    }
+    if (worst <= ip.tolerance) {
+      converged = true;
+      break;
+    }
+    if (it >= 2 && worst >= previous * 0.99) {
+      stalled = true;
+      break;
+    }
+    previous = worst;
+  }
+  if (!converged && !stalled) atomicMax(&d.sc->rattleFailed, 1u);
+}
+
+// A with SHAKE: half drift, then the position constraints along the bond
+// vectors before the drift; the velocity takes the constraint displacement.
+__device__ void starDrift(const Dev &d, const IntegratorParams &ip, Atom *w, int m, const double *target, double h) {
+  double ref[STAR_ATOMS][3], corr[STAR_ATOMS][3];
+#pragma unroll
+  for (int k = 0; k < STAR_ATOMS; ++k)
+    for (int c = 0; c < 3; ++c) {
+      ref[k][c] = w[k].x[c];
+      w[k].x[c] += h * w[k].v[c];
+      corr[k][c] = 0.0;
+    }
+  int iterations = 0;
+  bool converged = false, stalled = false;
+  double previous = 1e300;
+  for (int it = 0; it < ip.maxIterations; ++it) {
+    double worst = 0;
+#pragma unroll
+    for (int k = 0; k < STAR_MAX; ++k) {
+      if (k >= m) continue;
+      Atom &a = w[k + 1], &b = w[0];
+      const double t2 = target[k] * target[k];
+      const double dx = a.x[0] - b.x[0], dy = a.x[1] - b.x[1], dz = a.x[2] - b.x[2];
+      const double err = dx * dx + dy * dy + dz * dz - t2;
+      const double rel = fabs(err) / t2;
+      worst = fmax(worst, rel);
+      if (rel <= ip.tolerance) continue;
+      const double r[3] = {ref[k + 1][0] - ref[0][0], ref[k + 1][1] - ref[0][1], ref[k + 1][2] - ref[0][2]};
+      const double den = 2.0 * (a.im + b.im) * (dx * r[0] + dy * r[1] + dz * r[2]);
+      if (fabs(den) < 1e-20) continue;
+      const double lam = -err / den;
+      for (int c = 0; c < 3; ++c) {
+        a.x[c] += lam * a.im * r[c];
+        b.x[c] -= lam * b.im * r[c];
+        corr[k + 1][c] += lam * a.im * r[c];
+        corr[0][c] -= lam * b.im * r[c];
+      }
+    }
+    iterations = it + 1;
+    if (worst <= ip.tolerance) {
+      converged = true;
+      break;
+    }
+    if (it >= 2 && worst >= previous * 0.99) {
+      stalled = true;
+      break;
+    }
+    previous = worst;
+  }
+  const double ih = 1.0 / h;
+#pragma unroll
+  for (int k = 0; k < STAR_ATOMS; ++k)
+    for (int c = 0; c < 3; ++c) w[k].v[c] += corr[k][c] * ih;
+  atomicMax(&d.sc->shakeMaxIterations, (unsigned int)iterations);
+  if (!converged && !stalled) atomicMax(&d.sc->shakeFailed, 1u);
+}
+
 // ---- general clusters (iterative, FP64) -------------------------------------
 // Working state per cluster atom slot (CLUSTER_SLOT doubles): x[3], ref[3],
 // v[3], corr[3], inverse mass. S points at the cluster's first slot, in LDS
@@ -423,6 +537,32 @@ __global__ void __launch_bounds__(INTEGRATE_BLOCK) integrateOpenKernel(Dev d, In
         ke += kinetic(w[a]);
         disp = fmaxf(disp, storeAtom(d, atoms[a], w[a], true));
       }
+    } else if (desc.x == UNIT_STAR) {
+      const int m = desc.z;
+      Atom w[STAR_ATOMS];
+      int atom[STAR_ATOMS];
+      double target[STAR_MAX];
+      starLoad(d, desc.y, m, w, atom, target);
+      if (closing) {
+#pragma unroll
+        for (int k = 0; k < STAR_ATOMS; ++k) kick(w[k], h);
+        starRattle(d, ip, w, m, target);
+      }
+#pragma unroll
+      for (int k = 0; k < STAR_ATOMS; ++k) kick(w[k], h);
+      starRattle(d, ip, w, m, target);
+      starDrift(d, ip, w, m, target, h);
+#pragma unroll
+      for (int k = 0; k < STAR_ATOMS; ++k)
+        if (k <= m) thermostat(d, ip, atom[k], w[k], step);
+      starRattle(d, ip, w, m, target);
+      starDrift(d, ip, w, m, target, h);
+#pragma unroll
+      for (int k = 0; k < STAR_ATOMS; ++k)
+        if (k <= m) {
+          ke += kinetic(w[k]);
+          disp = fmaxf(disp, storeAtom(d, atom[k], w[k], true));
+        }
     } else if (desc.x == UNIT_CLUSTER) {
       const int cl = desc.y;
       const int ab = d.clusterAtomStart[cl], ae = d.clusterAtomStart[cl + 1];
@@ -506,6 +646,21 @@ __global__ void __launch_bounds__(INTEGRATE_BLOCK) integrateCloseKernel(Dev d, I
         ke += kinetic(w[a]);
         d.vel[atoms[a]] = make_float4((float)w[a].v[0], (float)w[a].v[1], (float)w[a].v[2], (float)w[a].im);
       }
+    } else if (desc.x == UNIT_STAR) {
+      const int m = desc.z;
+      Atom w[STAR_ATOMS];
+      int atom[STAR_ATOMS];
+      double target[STAR_MAX];
+      starLoad(d, desc.y, m, w, atom, target);
+#pragma unroll
+      for (int k = 0; k < STAR_ATOMS; ++k) kick(w[k], h);
+      starRattle(d, ip, w, m, target);
+#pragma unroll
+      for (int k = 0; k < STAR_ATOMS; ++k)
+        if (k <= m) {
+          ke += kinetic(w[k]);
+          d.vel[atom[k]] = make_float4((float)w[k].v[0], (float)w[k].v[1], (float)w[k].v[2], (float)w[k].im);
+        }
     } else if (desc.x == UNIT_CLUSTER) {
       const int cl = desc.y;
       const int ab = d.clusterAtomStart[cl], ae = d.clusterAtomStart[cl + 1];
@@ -666,7 +821,7 @@ __global__ void constrainPositionsKernel(Dev d, const float4 *reference, Integra
     }
     return;
   }
-  if (desc.x != UNIT_CLUSTER) return;
+  if (desc.x != UNIT_CLUSTER && desc.x != UNIT_STAR) return;
   const int cl = desc.y;
   const int ab = d.clusterAtomStart[cl], ae = d.clusterAtomStart[cl + 1];
   double *S = d.clusterScratch + (size_t)CLUSTER_SLOT * ab;

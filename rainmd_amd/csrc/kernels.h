@@ -46,9 +46,14 @@ void launchFftForward(const Dev &d, const PmeDev &p, const float *real, float2 *
 
 // ---- integrator and reductions (integrate.hip) ----
 // Unit kinds: a free atom, a rigid water (SETTLE), a general constraint
-// cluster (iterative SHAKE/RATTLE) and padding. A cluster unit is
-// (UNIT_CLUSTER, cluster, 1 = global scratch / 0 = LDS, LDS slot offset).
-enum { UNIT_FREE = 0, UNIT_WATER = 1, UNIT_CLUSTER = 2, UNIT_NONE = 3 };
+// cluster (iterative SHAKE/RATTLE), padding, and a star cluster (one centre
+// atom in every constraint, at most STAR_MAX constraints: the shape of every
+// H-bond cluster) solved in registers. A cluster unit is (UNIT_CLUSTER,
+// cluster, 1 = global scratch / 0 = LDS, LDS slot offset); a star unit is
+// (UNIT_STAR, cluster, constraint count, 0) with the centre first in the
+// cluster's atoms and constraint k joining atom k + 1 to the centre.
+enum { UNIT_FREE = 0, UNIT_WATER = 1, UNIT_CLUSTER = 2, UNIT_NONE = 3, UNIT_STAR = 4 };
+constexpr int STAR_MAX = 4;
 constexpr int INTEGRATE_BLOCK = 256;
 constexpr int CLUSTER_SLOT = 13;        // doubles per cluster atom: x, ref, v, corr, inverse mass
 constexpr int CLUSTER_LDS_SLOTS = 512;  // cluster atom slots per block held in LDS

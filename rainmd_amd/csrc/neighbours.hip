@@ -101,7 +101,8 @@ __global__ void cellsScatterKernel(Dev d) {
 
 // The scatter leaves each cell's atoms in arrival order; a bitonic sort in
 // LDS puts them in index order so the list, and every sum over it, is the
-// same on every run.
+// same on every run. The sorted cell also gets its positions copied in cell
+// order (atom index in .w) for the candidate scan.
 __global__ void __launch_bounds__(256) cellsSortKernel(Dev d) {
   if (!d.sc->rebuild) return;
   __shared__ int keys[SORT_CAPACITY];
@@ -109,7 +110,7 @@ __global__ void __launch_bounds__(256) cellsSortKernel(Dev d) {
   if (c >= d.box->cellTotal) return;
   const int begin = d.cellStart[c], end = d.cellStart[c + 1];
   const int len = end - begin;
-  if (len <= 1) return;
+  if (len <= 0) return;
   if (len > SORT_CAPACITY) {
     if (threadIdx.x == 0) atomicMax(&d.sc->cellOverflow, 1u);
     return;
@@ -134,7 +135,13 @@ __global__ void __launch_bounds__(256) cellsSortKernel(Dev d) {
       __syncthreads();
     }
   }
-  for (int k = threadIdx.x; k < len; k += blockDim.x) d.cellAtoms[begin + k] = keys[k];
+  for (int k = threadIdx.x; k < len; k += blockDim.x) {
+    const int atom = keys[k];
+    d.cellAtoms[begin + k] = atom;
+    float4 p = d.pos[atom];
+    p.w = __int_as_float(atom);
+    d.cellPos[begin + k] = p;
+  }
 }
 
 // Per-axis stencil offsets, deduplicated (shared/topology/cells.js:25-29):
@@ -157,11 +164,27 @@ __device__ __forceinline__ int wrapCell(int v, int count) {
   return v < 0 ? v + count : v;
 }
 
-// One wave per atom. Candidates are taken 64 at a time from each stencil
+// One wave per atom. Candidates are read 128 at a time (two per lane, both
+// loads in flight together) from the cell-ordered positions of each stencil
 // cell; a ballot compacts the accepted ones into the atom's slice in
-// candidate order. Every pair found is counted; only `cap` are stored, and
-// the true requirement reaches the host through Scalars::required
-// (the reference's overflow contract, webgpu/kernels/neighbours.js:128-146).
+// candidate order. Exclusions are checked only for candidates inside the
+// range of the atom's (sorted) exclusion partners. Every pair found is
+// counted; only `cap` are stored, and the true requirement reaches the host
+// through Scalars::required (the reference's overflow contract,
+// webgpu/kernels/neighbours.js:128-146).
+__device__ __forceinline__ bool acceptPair(const Dev &d, const BoxF &box, float4 pi, float4 pj, int atom, int exBegin,
+                                           int exEnd, int exMin, int exMax) {
+  const int j = __float_as_int(pj.w);
+  if (j == atom) return false;
+  float dx = pi.x - pj.x, dy = pi.y - pj.y, dz = pi.z - pj.z;
+  const float r2 = minimumImageR2(dx, dy, dz, box);
+  if (!(r2 <= box.rl2)) return false;
+  if (j >= exMin && j <= exMax)
+    for (int e = exBegin; e < exEnd; ++e)
+      if (d.exclPartners[e] == j) return false;
+  return true;
+}
+
 __global__ void __launch_bounds__(256) neighboursKernel(Dev d) {
   if (!d.sc->rebuild) return;
   const BoxF box = *d.box;
@@ -174,12 +197,23 @@ __global__ void __launch_bounds__(256) neighboursKernel(Dev d) {
   const int own = d.cellOfAtom[atom];
   const int c3[3] = {own / (cells[1] * cells[2]), (own / cells[2]) % cells[1], own % cells[2]};
   const int exBegin = d.exclStart[atom], exEnd = d.exclStart[atom + 1];
+  const int exMin = exBegin < exEnd ? d.exclPartners[exBegin] : 1;
+  const int exMax = exBegin < exEnd ? d.exclPartners[exEnd - 1] : 0;
   int bx, nx, by, ny, bz, nz;
   axisRange(cells[0], bx, nx);
   axisRange(cells[1], by, ny);
   axisRange(cells[2], bz, nz);
   int found = 0;
   int *out = d.nbrList + (size_t)atom * d.cap;
+  const float4 none = make_float4(0.f, 0.f, 0.f, __int_as_float(atom));
+  auto emit = [&](bool accept, int j) {
+    const unsigned long long mask = __ballot(accept);
+    if (accept) {
+      const int where = found + __popcll(mask & ((1ull << lane) - 1ull));
+      if (where < d.cap) out[where] = j;
+    }
+    found += __popcll(mask);
+  };
   for (int ox = 0; ox < nx; ++ox) {
     const int cx = wrapCell(c3[0] + bx + ox, cells[0]);
     for (int oy = 0; oy < ny; ++oy) {
@@ -188,32 +222,12 @@ __global__ void __launch_bounds__(256) neighboursKernel(Dev d) {
         const int cz = wrapCell(c3[2] + bz + oz, cells[2]);
         const int cell = (cx * cells[1] + cy) * cells[2] + cz;
         const int begin = d.cellStart[cell], end = d.cellStart[cell + 1];
-        for (int base = begin; base < end; base += 64) {
-          const int slot = base + lane;
-          bool accept = false;
-          int j = -1;
-          if (slot < end) {
-            j = d.cellAtoms[slot];
-            if (j != atom) {
-              const float4 pj = d.pos[j];
-              float dx = pi.x - pj.x, dy = pi.y - pj.y, dz = pi.z - pj.z;
-              const float r2 = minimumImageR2(dx, dy, dz, box);
-              accept = r2 <= box.rl2;
-              if (accept)
-                for (int e = exBegin; e < exEnd; ++e)
-                  if (d.exclPartners[e] == j) {
-                    accept = false;
-                    break;
-                  }
-            }
-          }
-          const unsigned long long mask = __ballot(accept);
-          if (accept) {
-            const int rank = __popcll(mask & ((1ull << lane) - 1ull));
-            const int where = found + rank;
-            if (where < d.cap) out[where] = j;
-          }
-          found += __popcll(mask);
+        for (int base = begin; base < end; base += 128) {
+          const int s0 = base + lane, s1 = base + 64 + lane;
+          const float4 p0 = s0 < end ? d.cellPos[s0] : none;
+          const float4 p1 = s1 < end ? d.cellPos[s1] : none;
+          emit(acceptPair(d, box, pi, p0, atom, exBegin, exEnd, exMin, exMax), __float_as_int(p0.w));
+          if (base + 64 < end) emit(acceptPair(d, box, pi, p1, atom, exBegin, exEnd, exMin, exMax), __float_as_int(p1.w));
         }
       }
     }

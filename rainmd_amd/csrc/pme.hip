@@ -31,6 +31,7 @@ constexpr int PLANE_THREADS = 512;
 constexpr int XLINE_THREADS = 256;
 constexpr int XLINES = 8;           // x lines per workgroup in the plane pipeline
 constexpr int LINES_PER_BLOCK = 4;  // lines per workgroup in the line pipeline
+constexpr int ATOM_BATCH = 4;       // atom positions loaded together in the plane scans
 
 __device__ __forceinline__ float2 cmul(float2 a, float2 b) { return make_float2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x); }
 __device__ __forceinline__ float2 cadd(float2 a, float2 b) { return make_float2(a.x + b.x, a.y + b.y); }
@@ -256,8 +257,16 @@ __global__ void __launch_bounds__(PLANE_THREADS) pmePlaneForwardKernel(Dev d, Pm
     for (int idx = t; idx < K1 * K2; idx += nt) plane[idx] = 0;
     __syncthreads();
     const double *recip = d.box->recip64;
-    for (int a = t; a < d.n; a += nt) {
-      const float4 pos = d.pos[a];
+    for (int a0 = t; a0 < d.n; a0 += nt * ATOM_BATCH) {
+      float4 batch[ATOM_BATCH];
+#pragma unroll
+      for (int u = 0; u < ATOM_BATCH; ++u) {
+        const int a = a0 + u * nt;
+        batch[u] = a < d.n ? d.pos[a] : make_float4(0.f, 0.f, 0.f, 0.f);
+      }
+#pragma unroll
+      for (int u = 0; u < ATOM_BATCH; ++u) {
+      const float4 pos = batch[u];
       if (pos.w == 0.f) continue;
       int bx, by, bz;
       float ox, oy, oz;
@@ -278,6 +287,7 @@ __global__ void __launch_bounds__(PLANE_THREADS) pmePlaneForwardKernel(Dev d, Pm
         unsigned long long *row = reinterpret_cast<unsigned long long *>(plane + gridIndex(by, b, K1) * K2);
         for (int c = 0; c < ORDER; ++c)
           atomicAdd(row + iz[c], (unsigned long long)chargeToFixed(pos.w, wx, wy[b], wz[c]));
+      }
       }
     }
     __syncthreads();
@@ -413,8 +423,17 @@ __global__ void __launch_bounds__(PLANE_THREADS) pmePlaneGatherKernel(Dev d, Pme
   }
   __syncthreads();
   const double *recip = d.box->recip64;
-  for (int a = t; a < d.n; a += nt) {
-    const float4 pos = d.pos[a];
+  for (int a0 = t; a0 < d.n; a0 += nt * ATOM_BATCH) {
+    float4 batch[ATOM_BATCH];
+#pragma unroll
+    for (int u = 0; u < ATOM_BATCH; ++u) {
+      const int a = a0 + u * nt;
+      batch[u] = a < d.n ? d.pos[a] : make_float4(0.f, 0.f, 0.f, 0.f);
+    }
+#pragma unroll
+    for (int u = 0; u < ATOM_BATCH; ++u) {
+    const int a = a0 + u * nt;
+    const float4 pos = batch[u];
     if (pos.w == 0.f) continue;
     int bx, by, bz;
     float ox, oy, oz;
@@ -448,6 +467,7 @@ __global__ void __launch_bounds__(PLANE_THREADS) pmePlaneGatherKernel(Dev d, Pme
       s2 += wy[b] * r2;
     }
     addGridForce(d, p, a, pos.w, dwx * s0, wx * s1, wx * s2);
+    }
   }
 }
 
