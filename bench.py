@@ -31,7 +31,21 @@ METRIC = "ns/day (single system & batched aggregate) at energy rel.err ≤1e-6 v
 TIMESTEP_PS = 0.002
 FLOP_PER_PAIR = 78
 FP32_PEAK_TFLOPS = 157.3
+COPY_PROBE_BYTES = 64 << 20  # the counter-calibration copy: 64 MiB read + 64 MiB written
 REFERENCE_NS_PER_DAY = 74.0  # ala-wat-big, Apple M-series WebGPU (docs/STATUS.md:199-203)
+
+# The single-GPU workloads: the 2854-atom NPT system the metric is quoted on
+# (default), the 1189-atom NVT system and the 100,200-atom water box.
+WORKLOADS = {
+    "ala-944w": dict(what="ACE-ALA-NME + 944 TIP3P", ensemble="NPT 1 bar 300 K, MC barostat every 25 steps",
+                     options=dict(pressure=1.0, barostat_interval=25), reference_ns_per_day=REFERENCE_NS_PER_DAY,
+                     reference_note="74 ns/day, same composition (ala-wat-big), Apple M-series WebGPU, "
+                                    "docs/STATUS.md:199-203"),
+    "ala-389w": dict(what="ACE-ALA-NME + 389 TIP3P", ensemble="NVT 300 K", options={}, reference_ns_per_day=154.0,
+                     reference_note="154 ns/day, ala-wat (1190 atoms) NVT, Apple M-series WebGPU, docs/STATUS.md:186"),
+    "water-33400": dict(what="33400 TIP3P", ensemble="NVT 300 K, skin 0.15 nm",
+                        options=dict(skin=0.15, grid=[112, 112, 112])),
+}
 
 
 def log(msg):
@@ -39,21 +53,26 @@ def log(msg):
 
 
 def in_cutoff_pairs(positions, box, cutoff, excl_start, excl_partners):
-    """Unique non-excluded pairs with r < rc at these positions (numpy, FP64)."""
+    """Unique non-excluded pairs with r < rc at these positions (FP64): a
+    periodic k-d tree count for rectangular boxes, minus the excluded pairs
+    inside the cutoff."""
+    from scipy.spatial import cKDTree
+
     x = positions.reshape(-1, 3).astype(np.float64)
     n = len(x)
     L = np.array([box[0], box[4], box[8]])
-    total = 0
-    rc2 = cutoff * cutoff
-    for i in range(n - 1):
-        d = x[i + 1:] - x[i]
-        d -= L * np.floor(d / L + 0.5)
-        r2 = np.einsum("ij,ij->i", d, d)
-        hit = np.nonzero(r2 <= rc2)[0] + i + 1
-        if len(hit):
-            ex = excl_partners[excl_start[i]:excl_start[i + 1]]
-            total += len(hit) - np.count_nonzero(np.isin(hit, ex))
-    return total
+    wrapped = x - L * np.floor(x / L)
+    wrapped[wrapped >= L] -= L[np.nonzero(wrapped >= L)[1]]
+    tree = cKDTree(wrapped, boxsize=L)
+    ordered = int(tree.count_neighbors(tree, cutoff))
+    unique = (ordered - n) // 2
+    a = np.repeat(np.arange(n), np.diff(excl_start))
+    b = np.asarray(excl_partners)
+    keep = a < b
+    d = x[a[keep]] - x[b[keep]]
+    d -= L * np.round(d / L)
+    excluded_inside = int(np.count_nonzero(np.einsum("ij,ij->i", d, d) <= cutoff * cutoff))
+    return unique - excluded_inside
 
 
 def cpu_baseline(system, pme, state, target_seconds):
@@ -69,7 +88,7 @@ def cpu_baseline(system, pme, state, target_seconds):
     t0 = time.perf_counter()
     ref, forces = o.potential(pos, alpha=pme["alpha"], grid=pme["grid"], f32_membership=True)
     first = time.perf_counter() - t0
-    steps = max(2, int(target_seconds / max(first, 1e-3)))
+    steps = max(1, int(target_seconds / max(first, 1e-3)))
     t0 = time.perf_counter()
     for k in range(steps):
         pos, vel, _ = o.langevin_step(pos, vel, forces, temperature=300.0, seed=202, step=k)
@@ -83,7 +102,7 @@ def cpu_baseline(system, pme, state, target_seconds):
         "cores": 1,
         "kind": "port",
         "ms_per_step": ms,
-        "sample": f"{steps} NVT BAOAB steps of the same 2854-atom system from the final GPU state "
+        "sample": f"{steps} NVT BAOAB steps of the same {system.atom_count}-atom system from the final GPU state "
                   "(FP64 all-pairs direct space, naive-DFT PME, global SHAKE/RATTLE), single-threaded C",
     }
 
@@ -100,6 +119,10 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-graphs", action="store_true", help="launch kernels directly (for profilers)")
+    ap.add_argument("--kernel-only", default=None,
+                    help="after warm-up, launch only this kernel family (nonbonded, pme, integrate, bonded, rebuild) "
+                         "and the calibration copy, for hardware-counter passes; prints their durations and exits")
+    ap.add_argument("--reps", type=int, default=50)
     args = ap.parse_args()
 
     if args.gpus != 1:
@@ -110,7 +133,8 @@ def main():
     import rainmd_amd as R
 
     system = R.workload_system(args.workload)
-    opts = dict(temperature=300.0, pressure=1.0, barostat_interval=25, seed=202)
+    spec = WORKLOADS[args.workload]
+    opts = dict(temperature=300.0, seed=202, **spec["options"])
     if args.skin > 0:
         opts["skin"] = args.skin
     if args.lanes > 0:
@@ -129,6 +153,16 @@ def main():
     t0 = time.perf_counter()
     sim.run(args.warmup, args.block)
     log(f"warm-up {args.warmup} steps in {time.perf_counter() - t0:.2f} s")
+
+    if args.kernel_only:
+        ms = sim.profile_kernel(args.kernel_only, reps=args.reps)
+        copy_bytes = COPY_PROBE_BYTES
+        copy_ms = sim.copy_probe(copy_bytes, reps=args.reps)
+        print(json.dumps({"kernel": args.kernel_only, "reps": args.reps, "launch_us": ms * 1e3,
+                          "copy_bytes": copy_bytes, "copy_us": copy_ms * 1e3,
+                          "copy_GBps": 2 * copy_bytes / (copy_ms * 1e-3) / 1e9}), flush=True)
+        sim.release()
+        return
 
     t0 = time.perf_counter()
     sim.run(args.steps, args.block)
@@ -162,14 +196,14 @@ def main():
         "ms_per_step": ms_per_step,
         "higher_is_better": True,
         "scaling": "weak",
-        "vs_baseline": ns_day / REFERENCE_NS_PER_DAY,
-        "baseline_reference": "74 ns/day, same composition (ala-wat-big), Apple M-series WebGPU, docs/STATUS.md:199-203",
+        "vs_baseline": ns_day / spec["reference_ns_per_day"] if spec.get("reference_ns_per_day") else None,
+        "baseline_reference": spec.get("reference_note"),
         "dtype": "f32",
         "data": "synthetic",
         "config": {
-            "workload": f"{args.workload}: ACE-ALA-NME + 944 TIP3P, {system.atom_count} atoms, "
-                        f"{system.box[0]:.4f} nm cube, NPT 1 bar 300 K, MC barostat every 25 steps, 2 fs, "
-                        f"rc 0.9 nm, PME {pme['grid'][0]}^3 order 5",
+            "workload": f"{args.workload}: {spec['what']}, {system.atom_count} atoms, "
+                        f"{system.box[0]:.4f} nm cube, {spec['ensemble']}, 2 fs, rc {system.cutoff} nm, "
+                        f"PME {pme['grid'][0]}^3 order 5",
             "atoms": system.atom_count,
             "skin_nm": diag["skin"],
             "lanes_per_atom": diag["lanes_per_atom"],

@@ -252,6 +252,10 @@ typedef struct rmd_energy_terms {
   int32_t kinetic_sampled_in_step; /* 1: "before the closing half kick"       */
   double volume;
   double box[9];
+  /* d(energy)/d(ln lambda) when every coordinate and the box scale by lambda:
+   * direct-space pairs (sum r dU/dr) and the PME reciprocal sum. Filled by
+   * evaluations (0 where not computed); features of the barostat's screen. */
+  double dscale_direct, dscale_reciprocal;
 } rmd_energy_terms;
 int rmd_energy(rmd_ctx *ctx, rmd_energy_terms *out);
 
@@ -353,6 +357,67 @@ int rmd_get_stream(rmd_ctx *ctx, void **stream);
  * timed with HIP events on the context's stream over `reps` launches:
  * 0 nonbonded, 1 PME chain, 2 integrator, 3 bonded/exceptions, 4 rebuild. */
 int rmd_profile_kernel(rmd_ctx *ctx, int32_t kernel, int32_t reps, double *avg_ms);
+/* Measurement calibration: `reps` launches of a plain device-to-device copy
+ * of `bytes` bytes (16-byte vector loads and stores) on the context's stream;
+ * its hardware byte counters scale the HBM traffic read for other kernels. */
+int rmd_copy_probe(rmd_ctx *ctx, int64_t bytes, int32_t reps, double *avg_ms);
+
+/* ======================================================================== */
+/* The barostat's learned screen (no counterpart in the reference)          */
+/* ======================================================================== */
+/* A 16 -> 64 -> 64 -> 1 tanh MLP (RMD_SURROGATE_PARAMS FP32 parameters:
+ * W1[16][64], b1[64], W2[64][64], b2[64], W3[64], b3, input-major) whose
+ * layer products run as BF16 MFMA with FP32 accumulation on the GPU, for
+ * inference and for training (Adam, mean-squared error). The reference's
+ * barostat evaluates every move exactly (webgpu/simulation.js:995-1070);
+ * this screen lets a move be turned down before its exact evaluation with
+ * a delayed-acceptance second stage that keeps the NPT ensemble exact. */
+#define RMD_SURROGATE_INPUTS 16
+#define RMD_SURROGATE_PARAMS 5313
+typedef struct rmd_surrogate rmd_surrogate;
+/* Parameters drawn from the counter hash with `seed` (uniform, scaled by
+ * sqrt(6 / (fan_in + fan_out))), biases zero. */
+int rmd_surrogate_create(int32_t device, uint32_t seed, rmd_surrogate **out);
+void rmd_surrogate_free(rmd_surrogate *net);
+int rmd_surrogate_get_params(rmd_surrogate *net, float *params);
+int rmd_surrogate_set_params(rmd_surrogate *net, const float *params);
+/* y[k] = f(x[k]), x: n x 16. */
+int rmd_surrogate_forward(rmd_surrogate *net, const float *x, int32_t n, float *y);
+/* loss = mean_k (f(x_k) - t_k)^2 and its gradient (RMD_SURROGATE_PARAMS). */
+int rmd_surrogate_gradient(rmd_surrogate *net, const float *x, const float *t, int32_t n, float *loss, float *grad);
+/* `steps` Adam steps (mini-batches of 64 drawn from the n samples);
+ * returns the mean mini-batch loss of the last 16 steps. */
+int rmd_surrogate_train(rmd_surrogate *net, const float *x, const float *t, int32_t n, int32_t steps,
+                        float learning_rate, float *loss);
+
+/* The screen on a simulation's barostat. Phase 0 runs plain moves and keeps
+ * (features, exact energy change) samples; after `training_moves` of them
+ * the network is trained (`training_steps` Adam steps) and frozen, and
+ * phase 1 screens: stage one accepts a move with min(1, exp(-w~/kT)) from
+ * the predicted weight w~ (no evaluation when it fails); a survivor is
+ * evaluated exactly and accepted with
+ * min(1, exp(-w/kT) * a1(reverse) / a1(forward)). */
+typedef struct rmd_screen_options {
+  int32_t enabled;
+  int32_t training_moves;   /* 2000 */
+  int32_t training_steps;   /* 4000 */
+  float learning_rate;      /* 1e-3 */
+  uint32_t seed;            /* network initialisation */
+} rmd_screen_options;
+void rmd_screen_options_default(rmd_screen_options *options);
+int rmd_screen_configure(rmd_ctx *ctx, const rmd_screen_options *options);
+typedef struct rmd_screen_stats {
+  int32_t phase;                 /* 0 collecting samples, 1 screening */
+  int64_t samples;               /* (features, exact change) pairs kept */
+  int64_t moves, exact_evaluations, screened_out, second_stage_rejections, accepted;
+  double training_loss;          /* normalised mean-squared error at the end of training */
+  double residual_rms;           /* kJ/mol: rms of (exact - predicted) energy change on the screened moves */
+  double estimate_rms;           /* kJ/mol: rms of (exact - first-order estimate) on the same moves */
+} rmd_screen_stats;
+int rmd_screen_stats_get(rmd_ctx *ctx, rmd_screen_stats *out);
+/* The kept samples: up to `capacity` rows of 16 raw features and the exact
+ * energy change residual target (kJ/mol); *count receives the number kept. */
+int rmd_screen_samples(rmd_ctx *ctx, float *features, float *targets, int32_t capacity, int32_t *count);
 
 /* Device and build identity. */
 typedef struct rmd_device_info {

@@ -150,6 +150,7 @@ class EnergyTerms(C.Structure):
         ("kinetic_sampled_in_step", C.c_int32),
         ("volume", C.c_double),
         ("box", C.c_double * 9),
+        ("dscale_direct", C.c_double), ("dscale_reciprocal", C.c_double),
     ]
 
 
@@ -192,6 +193,22 @@ class Diagnostics(C.Structure):
         ("water_units", C.c_int32), ("cluster_units", C.c_int32), ("free_units", C.c_int32),
         ("pme_planes", C.c_int32),
     ]
+
+
+class ScreenOptions(C.Structure):
+    _fields_ = [("enabled", C.c_int32), ("training_moves", C.c_int32), ("training_steps", C.c_int32),
+                ("learning_rate", C.c_float), ("seed", C.c_uint32)]
+
+
+class ScreenStats(C.Structure):
+    _fields_ = [("phase", C.c_int32), ("samples", C.c_int64), ("moves", C.c_int64),
+                ("exact_evaluations", C.c_int64), ("screened_out", C.c_int64),
+                ("second_stage_rejections", C.c_int64), ("accepted", C.c_int64),
+                ("training_loss", C.c_double), ("residual_rms", C.c_double), ("estimate_rms", C.c_double)]
+
+
+SURROGATE_INPUTS = 16
+SURROGATE_PARAMS = 5313
 
 
 class DeviceInfo(C.Structure):
@@ -246,6 +263,18 @@ SIGNATURES = {
     "rmd_set_lanes": (C.c_int, [C.c_void_p, C.c_int32]),
     "rmd_set_graphs": (C.c_int, [C.c_void_p, C.c_int32]),
     "rmd_set_pme_pipeline": (C.c_int, [C.c_void_p, C.c_int32]),
+    "rmd_copy_probe": (C.c_int, [C.c_void_p, C.c_int64, C.c_int32, C.POINTER(C.c_double)]),
+    "rmd_surrogate_create": (C.c_int, [C.c_int32, C.c_uint32, C.POINTER(C.c_void_p)]),
+    "rmd_surrogate_free": (None, [C.c_void_p]),
+    "rmd_surrogate_get_params": (C.c_int, [C.c_void_p, _fp]),
+    "rmd_surrogate_set_params": (C.c_int, [C.c_void_p, _fp]),
+    "rmd_surrogate_forward": (C.c_int, [C.c_void_p, _fp, C.c_int32, _fp]),
+    "rmd_surrogate_gradient": (C.c_int, [C.c_void_p, _fp, _fp, C.c_int32, _fp, _fp]),
+    "rmd_surrogate_train": (C.c_int, [C.c_void_p, _fp, _fp, C.c_int32, C.c_int32, C.c_float, _fp]),
+    "rmd_screen_options_default": (None, [C.POINTER(ScreenOptions)]),
+    "rmd_screen_configure": (C.c_int, [C.c_void_p, C.POINTER(ScreenOptions)]),
+    "rmd_screen_stats_get": (C.c_int, [C.c_void_p, C.POINTER(ScreenStats)]),
+    "rmd_screen_samples": (C.c_int, [C.c_void_p, _fp, _fp, C.c_int32, C.POINTER(C.c_int32)]),
     "rmd_get_stream": (C.c_int, [C.c_void_p, C.POINTER(C.c_void_p)]),
     "rmd_profile_kernel": (C.c_int, [C.c_void_p, C.c_int32, C.c_int32, C.POINTER(C.c_double)]),
     "rmd_device_info_get": (C.c_int, [C.c_int32, C.POINTER(DeviceInfo)]),

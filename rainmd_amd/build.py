@@ -27,6 +27,7 @@ SOURCES = [
     "bonded.hip",
     "pme.hip",
     "integrate.hip",
+    "surrogate.hip",
     "context.hip",
 ]
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")

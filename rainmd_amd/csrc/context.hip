@@ -15,6 +15,7 @@
 
 #include "host.h"
 #include "kernels.h"
+#include "surrogate.h"
 
 namespace rmd {
 void setGlobalError(const std::string &m);
@@ -104,8 +105,14 @@ struct rmd_ctx {
   long long *forceSaved = nullptr;
   Scalars *scSaved = nullptr;
   int *molStart = nullptr, *molAtoms = nullptr, *molOfAtom = nullptr;
-  int *dCounts = nullptr;
   double *dTerms = nullptr;
+  // Energy terms of the last step of the last block (E_TERM_COUNT doubles on
+  // the device, copied to the host with the block's scalars), valid while
+  // blockEndStep == step: the barostat's old potential without another
+  // evaluation.
+  double *dTermsBlock = nullptr;
+  double blockEndTerms[E_TERM_COUNT] = {0};
+  int64_t blockEndStep = -1;
   unsigned int *dMaxForce = nullptr;
   int nUnits = 0, waterUnits = 0, clusterUnits = 0, freeUnits = 0;
   int clusterLdsSlots = 0;  // largest per-block cluster slot count held in LDS
@@ -126,6 +133,18 @@ struct rmd_ctx {
   bool initialised = false;
   int slotCounts[E_TERM_COUNT] = {0};
   IntegratorParams ip{};
+  // The barostat's learned screen.
+  struct Screen {
+    rmd_screen_options options{};
+    rmd_surrogate *net = nullptr;
+    int phase = 0;
+    double refVolume = 0;
+    std::vector<float> x, t;  // raw features and residual targets
+    float mean[RMD_SURROGATE_INPUTS] = {0}, scale[RMD_SURROGATE_INPUTS] = {0};
+    double tMean = 0, tScale = 1;
+    int64_t moves = 0, exact = 0, screened = 0, secondStage = 0, accepted = 0, residualCount = 0;
+    double trainingLoss = 0, residual2 = 0, estimate2 = 0;
+  } screen;
 };
 
 namespace {
@@ -389,7 +408,7 @@ void enqueueForces(rmd_ctx *c, int terms, int energies, int slots[E_TERM_COUNT])
   const float alpha = (float)c->pme.alpha;
   if (terms & RMD_TERM_DIRECT) {
     const int b = launchNonbonded(d, alpha, c->lanes, energies, c->stream);
-    slots[E_LJ_DIRECT] = slots[E_COUL_DIRECT] = b;
+    slots[E_LJ_DIRECT] = slots[E_COUL_DIRECT] = slots[E_DSCALE_DIRECT] = b;
   }
   if (terms & (RMD_TERM_BONDED | RMD_TERM_EXCEPTIONS | RMD_TERM_RESTRAINT)) {
     int b = 0;
@@ -400,7 +419,7 @@ void enqueueForces(rmd_ctx *c, int terms, int energies, int slots[E_TERM_COUNT])
   if (terms & RMD_TERM_RECIPROCAL) {
     int b = 0;
     launchPme(d, c->p, energies, c->stream, &b);
-    slots[E_RECIPROCAL] = b;
+    slots[E_RECIPROCAL] = slots[E_DSCALE_RECIPROCAL] = b;
   }
 }
 
@@ -434,8 +453,9 @@ void evaluate(rmd_ctx *c, int terms, bool forceRebuild, double terms_out[E_TERM_
     }
     if (terms_out) {
       std::memcpy(c->slotCounts, slots, sizeof slots);
-      HIP_CHECK(hipMemcpyAsync(c->dCounts, slots, sizeof slots, hipMemcpyHostToDevice, c->stream));
-      launchReduceEnergies(c->d, c->dCounts, c->dTerms, c->stream);
+      SlotCounts counts;
+      std::memcpy(counts.c, slots, sizeof slots);
+      launchReduceEnergies(c->d, counts, c->dTerms, c->stream);
       HIP_CHECK(hipMemcpyAsync(terms_out, c->dTerms, sizeof(double) * E_TERM_COUNT, hipMemcpyDeviceToHost,
                                c->stream));
       HIP_CHECK(hipStreamSynchronize(c->stream));
@@ -454,6 +474,87 @@ double potentialFromTerms(rmd_ctx *c, const double *t) {
          t[E_COUL_EXCEPTION] + t[E_RECIPROCAL] + c->selfEnergy + c->netEnergy + dispersionNow(c) + t[E_RESTRAINT];
 }
 
+// ---- the barostat's learned screen ------------------------------------------
+
+double dispersionAt(rmd_ctx *c, double volume) {
+  return c->dispersionOn ? dispersionEnergy(c->disp, volume, c->sys.cutoff) : 0.0;
+}
+
+// The 16 raw features of a move from a state (its evaluated terms) at
+// `volume` to `newVolume`, and the first-order estimate of the energy change
+// from the scaling derivatives, dU/dln(lambda) * ln(lambda) with
+// ln(lambda) = ln(V'/V) / 3.
+void screenFeatures(rmd_ctx *c, const double *t, double volume, double newVolume, float *f, double *estimate) {
+  const double kT = BOLTZMANN * c->temperature;
+  const double nkT = std::max(1, c->sys.topology.moleculeCount) * kT;
+  const double L = std::log(newVolume / volume);
+  const double dD = t[E_DSCALE_DIRECT], dR = t[E_DSCALE_RECIPROCAL];
+  const double est = (dD + dR) * L / 3.0;
+  *estimate = est;
+  double g[RMD_SURROGATE_INPUTS];
+  g[0] = 100.0 * L;
+  g[1] = 1e4 * L * L;
+  g[2] = est / kT;
+  g[3] = dD / nkT;
+  g[4] = dR / nkT;
+  g[5] = volume / c->screen.refVolume - 1.0;
+  g[6] = t[E_LJ_DIRECT] / nkT;
+  g[7] = t[E_COUL_DIRECT] / nkT;
+  g[8] = t[E_RECIPROCAL] / nkT;
+  g[9] = g[0] * g[3];
+  g[10] = g[0] * g[4];
+  g[11] = g[1] * g[3];
+  g[12] = g[1] * g[4];
+  g[13] = g[0] * g[6];
+  g[14] = g[0] * g[7];
+  g[15] = g[0] * g[8];
+  for (int k = 0; k < RMD_SURROGATE_INPUTS; ++k) f[k] = (float)g[k];
+}
+
+// The network's residual (kJ/mol) for one raw feature row.
+double predictResidual(rmd_ctx *c, const float *raw) {
+  auto &sc = c->screen;
+  float x[RMD_SURROGATE_INPUTS], y = 0.f;
+  for (int k = 0; k < RMD_SURROGATE_INPUTS; ++k) x[k] = (raw[k] - sc.mean[k]) * sc.scale[k];
+  const int st = rmd_surrogate_forward(sc.net, x, 1, &y);
+  if (st != RMD_OK) throw Error(st, std::string("screen inference failed: ") + rmd_last_global_error());
+  return sc.tMean + sc.tScale * (double)y;
+}
+
+// Normalise the kept samples, train the network on the GPU and start screening.
+void screenTrain(rmd_ctx *c) {
+  auto &sc = c->screen;
+  const size_t n = sc.t.size();
+  const int F = RMD_SURROGATE_INPUTS;
+  for (int k = 0; k < F; ++k) {
+    double m = 0, v = 0;
+    for (size_t i = 0; i < n; ++i) m += sc.x[i * F + k];
+    m /= n;
+    for (size_t i = 0; i < n; ++i) v += (sc.x[i * F + k] - m) * (sc.x[i * F + k] - m);
+    const double sd = std::sqrt(v / n);
+    sc.mean[k] = (float)m;
+    sc.scale[k] = (float)(sd > 1e-12 ? 1.0 / sd : 0.0);
+  }
+  double tm = 0, tv = 0;
+  for (size_t i = 0; i < n; ++i) tm += sc.t[i];
+  tm /= n;
+  for (size_t i = 0; i < n; ++i) tv += (sc.t[i] - tm) * (sc.t[i] - tm);
+  sc.tMean = tm;
+  sc.tScale = std::max(std::sqrt(tv / n), 1e-9);
+  std::vector<float> x(n * F), t(n);
+  for (size_t i = 0; i < n; ++i) {
+    for (int k = 0; k < F; ++k) x[i * F + k] = (sc.x[i * F + k] - sc.mean[k]) * sc.scale[k];
+    t[i] = (float)((sc.t[i] - sc.tMean) / sc.tScale);
+  }
+  float loss = 0.f;
+  const int st = rmd_surrogate_train(sc.net, x.data(), t.data(), (int)n, sc.options.training_steps,
+                                     sc.options.learning_rate, &loss);
+  if (st != RMD_OK) throw Error(st, std::string("screen training failed: ") + rmd_last_global_error());
+  HIP_CHECK(hipSetDevice(c->device));
+  sc.trainingLoss = loss;
+  sc.phase = 1;
+}
+
 double evaluatePotential(rmd_ctx *c) {
   double t[E_TERM_COUNT];
   evaluate(c, RMD_TERM_ALL, true, t);
@@ -463,11 +564,20 @@ double evaluatePotential(rmd_ctx *c) {
 // ---- the block loop ---------------------------------------------------------
 
 void enqueueBlock(rmd_ctx *c, int steps) {
+  SlotCounts counts{};
   for (int s = 0; s < steps; ++s) {
-    launchIntegrateOpen(c->d, c->ip, s > 0 ? 1 : 0, s == steps - 1 ? 1 : 0, c->stream);
+    const bool last = s == steps - 1;
+    const int openBlocks = launchIntegrateOpen(c->d, c->ip, s > 0 ? 1 : 0, last ? 1 : 0, c->stream);
     int slots[E_TERM_COUNT];
-    enqueueForces(c, RMD_TERM_ALL, 0, slots);
+    enqueueForces(c, RMD_TERM_ALL, last ? 1 : 0, slots);
+    if (last) {
+      std::memcpy(counts.c, slots, sizeof slots);
+      counts.c[E_KINETIC] = 0;
+      counts.c[E_KINETIC_EARLY] = openBlocks;
+    }
   }
+  // The last step's energies, reduced in the same launch sequence.
+  launchReduceEnergies(c->d, counts, c->dTermsBlock, c->stream);
   launchIntegrateClose(c->d, c->ip, 0, c->stream);
 }
 
@@ -512,6 +622,8 @@ void runBlock(rmd_ctx *c, int steps) {
     HIP_CHECK(hipMemsetAsync(&c->d.sc->requiredMax, 0, sizeof(unsigned int), c->stream));
     snapshot(c);
     launchBlock(c, steps);
+    HIP_CHECK(hipMemcpyAsync(c->blockEndTerms, c->dTermsBlock, sizeof c->blockEndTerms, hipMemcpyDeviceToHost,
+                             c->stream));
     Scalars h;
     checkScalars(c, h);
     if (h.nonFinite || std::isnan(__builtin_bit_cast(float, h.maxDisp2Bits)))
@@ -533,6 +645,7 @@ void runBlock(rmd_ctx *c, int steps) {
     c->step += steps;
     c->stepsRun += steps;
     c->kineticSampledAtStep = c->step;
+    c->blockEndStep = c->step;
     return;
   }
   throw Error(RMD_E_NEIGHBOUR_OVERFLOW, "the neighbour list kept overflowing after growth");
@@ -542,6 +655,7 @@ void runBlock(rmd_ctx *c, int steps) {
 // cells and the capacity, update the self energy and the PME influence
 // function; the PME grid and alpha stay fixed.
 void updateBox(rmd_ctx *c, const double *nb) {
+  c->blockEndStep = -1;
   for (int k = 0; k < 9; ++k) c->box[k] = nb[k];
   CellPlan estimate = planCellList(c->sys, c->plan.skin, c->slack, 0, c->box);
   const int capacity = std::max(c->d.cap, estimate.maxNeighbours);
@@ -556,13 +670,16 @@ void updateBox(rmd_ctx *c, const double *nb) {
   setForceRebuildKernel<<<1, 1, 0, c->stream>>>(c->d.sc);
 }
 
-double scaleVolume(rmd_ctx *c, double s) {
+double scaleVolume(rmd_ctx *c, double s, double *terms = nullptr) {
   launchScaleMolecules(c->d, c->molStart, c->molAtoms, c->sys.topology.moleculeCount, c->centroids, c->molOfAtom,
                        (float)s, c->stream);
   double nb[9];
   for (int k = 0; k < 9; ++k) nb[k] = c->box[k] * s;
   updateBox(c, nb);
-  return evaluatePotential(c);
+  double t[E_TERM_COUNT];
+  evaluate(c, RMD_TERM_ALL, true, t);
+  if (terms) std::memcpy(terms, t, sizeof t);
+  return potentialFromTerms(c, t);
 }
 
 void toDouble(rmd_ctx *c, double *forces) {
@@ -592,6 +709,8 @@ void fillEnergy(rmd_ctx *c, const double *t, rmd_energy_terms *e, bool withKinet
   e->position_restraint = t[E_RESTRAINT];
   e->restrained_atoms = (int32_t)c->restraintAtoms.size();
   e->potential = e->bonded + e->lennard_jones + e->dispersion + e->coulomb + e->position_restraint;
+  e->dscale_direct = t[E_DSCALE_DIRECT];
+  e->dscale_reciprocal = t[E_DSCALE_RECIPROCAL];
   if (withKinetic) {
     e->kinetic = kinetic;
     e->total = e->potential + kinetic;
@@ -778,8 +897,8 @@ void create(rmd_ctx *c, const System &sys, const rmd_options &o) {
   d.slots = devAlloc<double>((size_t)E_TERM_COUNT * MAX_SLOTS, c->held);
   d.sc = devAlloc<Scalars>(1, c->held);
   c->scSaved = devAlloc<Scalars>(1, c->held);
-  c->dCounts = devAlloc<int>(E_TERM_COUNT, c->held);
   c->dTerms = devAlloc<double>(E_TERM_COUNT, c->held);
+  c->dTermsBlock = devAlloc<double>(E_TERM_COUNT, c->held);
   c->dMaxForce = devAlloc<unsigned int>(1, c->held);
   // Molecules for the barostat.
   c->molStart = devAlloc<int>(sys.moleculeStart.size(), c->held);
@@ -814,6 +933,7 @@ void create(rmd_ctx *c, const System &sys, const rmd_options &o) {
   p.rgrid = devAlloc<float>(grid, c->held);
   p.cgrid = devAlloc<float2>(half, c->held);
   p.influence = devAlloc<float>(half, c->held);
+  p.piOverAlpha2 = M_PI * M_PI / (c->pme.alpha * c->pme.alpha);
   pmeConfigure(p, PME_AUTO, c->maxLds);
   writeBox(c);
   setupIntegrator(c, 0);
@@ -822,6 +942,7 @@ void create(rmd_ctx *c, const System &sys, const rmd_options &o) {
 }
 
 void setState(rmd_ctx *c, const double *pos, const double *vel, int64_t step, int supplied) {
+  c->blockEndStep = -1;
   const System &s = c->sys;
   const int n = s.topology.atomCount;
   for (int k = 0; k < 3 * n; ++k)
@@ -933,6 +1054,8 @@ void rmd_destroy(rmd_ctx *c) {
   (void)hipSetDevice(c->device);
   if (c->stream) (void)hipStreamSynchronize(c->stream);
   invalidateGraphs(c);
+  if (c->screen.net) rmd_surrogate_free(c->screen.net);
+  (void)hipSetDevice(c->device);
   for (void *p : c->held) (void)hipFree(p);
   if (c->stream) (void)hipStreamDestroy(c->stream);
   delete c;
@@ -1021,9 +1144,10 @@ int rmd_energy(rmd_ctx *c, rmd_energy_terms *out) {
     std::memcpy(slots, c->slotCounts, sizeof slots);
     slots[E_KINETIC] = launchKinetic(c->d, E_KINETIC, c->stream);
     const bool earlyValid = c->kineticSampledAtStep == c->step;
-    slots[E_KINETIC_EARLY] = earlyValid ? (c->nUnits + 255) / 256 : 0;
-    HIP_CHECK(hipMemcpyAsync(c->dCounts, slots, sizeof slots, hipMemcpyHostToDevice, c->stream));
-    launchReduceEnergies(c->d, c->dCounts, c->dTerms, c->stream);
+    slots[E_KINETIC_EARLY] = earlyValid ? (c->nUnits + INTEGRATE_BLOCK - 1) / INTEGRATE_BLOCK : 0;
+    SlotCounts counts;
+    std::memcpy(counts.c, slots, sizeof slots);
+    launchReduceEnergies(c->d, counts, c->dTerms, c->stream);
     double all[E_TERM_COUNT];
     HIP_CHECK(hipMemcpyAsync(all, c->dTerms, sizeof all, hipMemcpyDeviceToHost, c->stream));
     HIP_CHECK(hipStreamSynchronize(c->stream));
@@ -1102,8 +1226,18 @@ int rmd_attempt_volume_change(rmd_ctx *c, rmd_move *m) {
     if (c->volumeScale < 0) c->volumeScale = 0.01 * volume;
     const double kT = BOLTZMANN * c->temperature;
     const double pressure = c->pressure * BAR_TO_KJ_PER_MOL_NM3;
-    const double oldPotential = evaluatePotential(c);
+    // The old potential: the last block's own energies when the state has not
+    // changed since (same positions and box), else a fresh evaluation.
+    double oldTerms[E_TERM_COUNT];
+    if (c->blockEndStep == c->step) {
+      std::memcpy(oldTerms, c->blockEndTerms, sizeof oldTerms);
+    } else {
+      evaluate(c, RMD_TERM_ALL, true, oldTerms);
+    }
+    const double oldPotential = potentialFromTerms(c, oldTerms);
     const size_t np = c->d.nPad;
+    double oldBox[9];
+    std::memcpy(oldBox, c->box, sizeof oldBox);
     HIP_CHECK(hipMemcpyAsync(c->posSaved, c->d.pos, np * sizeof(float4), hipMemcpyDeviceToDevice, c->stream));
     HIP_CHECK(hipMemcpyAsync(c->forceSaved, c->d.force, 3 * np * sizeof(long long), hipMemcpyDeviceToDevice, c->stream));
     c->barostatAttempts += 1;
@@ -1126,21 +1260,84 @@ int rmd_attempt_volume_change(rmd_ctx *c, rmd_move *m) {
       m->acceptance_rate = (double)c->barostatAccepted / c->barostatAttempts;
       return RMD_OK;
     }
-    const double newPotential = scaleVolume(c, s);
-    const double weight = (newPotential - oldPotential) + pressure * deltaVolume -
-                          c->sys.topology.moleculeCount * kT * std::log(newVolume / volume);
-    const bool accept = weight <= 0 || uniformAt(bseed, (uint32_t)c->barostatAttempts, 1, 0) < std::exp(-weight / kT);
+    auto &sc = c->screen;
+    const bool screening = sc.options.enabled && sc.phase == 1 && sc.net;
+    const double lnRatio = std::log(newVolume / volume);
+    const double work = pressure * deltaVolume - c->sys.topology.moleculeCount * kT * lnRatio;
+    const double dispersionChange = dispersionAt(c, newVolume) - dispersionAt(c, volume);
+    float fx[RMD_SURROGATE_INPUTS];
+    double estimateX = 0.0, predictedX = 0.0, firstStage = 1.0;
+    if (sc.options.enabled) screenFeatures(c, oldTerms, volume, newVolume, fx, &estimateX);
+    bool evaluated = true, accept = false;
+    double newPotential = oldPotential, weight = 0.0;
+    if (screening) {
+      // Stage one: the predicted weight alone; a move it turns down is never
+      // evaluated and the state is untouched.
+      sc.moves += 1;
+      predictedX = dispersionChange + estimateX + predictResidual(c, fx);
+      const double predictedWeight = predictedX + work;
+      firstStage = predictedWeight <= 0 ? 1.0 : std::exp(-predictedWeight / kT);
+      if (uniformAt(bseed, (uint32_t)c->barostatAttempts, 2, 0) >= firstStage) {
+        sc.screened += 1;
+        evaluated = false;
+        weight = predictedWeight;
+      }
+    }
+    if (evaluated) {
+      double newTerms[E_TERM_COUNT];
+      newPotential = scaleVolume(c, s, newTerms);
+      weight = (newPotential - oldPotential) + work;
+      const double change = newPotential - oldPotential;
+      if (sc.options.enabled) sc.exact += 1;
+      if (screening) {
+        // Stage two (delayed acceptance): exp(-w/kT) times the reverse move's
+        // stage-one probability over the forward one's.
+        float fy[RMD_SURROGATE_INPUTS];
+        double estimateY = 0.0;
+        screenFeatures(c, newTerms, newVolume, volume, fy, &estimateY);
+        const double predictedReverse = -dispersionChange + estimateY + predictResidual(c, fy) - work;
+        const double reverseStage = predictedReverse <= 0 ? 1.0 : std::exp(-predictedReverse / kT);
+        const double ratio = std::exp(-weight / kT) * reverseStage / firstStage;
+        accept = ratio >= 1.0 || uniformAt(bseed, (uint32_t)c->barostatAttempts, 1, 0) < ratio;
+        if (!accept) sc.secondStage += 1;
+        sc.residual2 += (change - predictedX) * (change - predictedX);
+        sc.estimate2 += (change - dispersionChange - estimateX) * (change - dispersionChange - estimateX);
+        sc.residualCount += 1;
+      } else {
+        accept = weight <= 0 || uniformAt(bseed, (uint32_t)c->barostatAttempts, 1, 0) < std::exp(-weight / kT);
+        if (sc.options.enabled && sc.phase == 0) {
+          // both directions of the move: the network sees the pair's antisymmetry
+          float fy[RMD_SURROGATE_INPUTS];
+          double estimateY = 0.0;
+          screenFeatures(c, newTerms, newVolume, volume, fy, &estimateY);
+          sc.x.insert(sc.x.end(), fx, fx + RMD_SURROGATE_INPUTS);
+          sc.t.push_back((float)(change - dispersionChange - estimateX));
+          sc.x.insert(sc.x.end(), fy, fy + RMD_SURROGATE_INPUTS);
+          sc.t.push_back((float)(-change + dispersionChange - estimateY));
+        }
+      }
+      if (accept) {
+        std::memcpy(c->blockEndTerms, newTerms, sizeof newTerms);
+      } else {
+        // Back to the old state exactly: positions, box (copied, not divided
+        // back), forces; the old energies describe it again.
+        HIP_CHECK(hipMemcpyAsync(c->d.pos, c->posSaved, np * sizeof(float4), hipMemcpyDeviceToDevice, c->stream));
+        updateBox(c, oldBox);
+        HIP_CHECK(hipMemcpyAsync(c->d.force, c->forceSaved, 3 * np * sizeof(long long), hipMemcpyDeviceToDevice,
+                                 c->stream));
+        HIP_CHECK(hipStreamSynchronize(c->stream));
+        std::memcpy(c->blockEndTerms, oldTerms, sizeof oldTerms);
+      }
+    }
     if (accept) {
       c->barostatAccepted += 1;
       c->windowAccepted += 1;
-    } else {
-      HIP_CHECK(hipMemcpyAsync(c->d.pos, c->posSaved, np * sizeof(float4), hipMemcpyDeviceToDevice, c->stream));
-      double nb[9];
-      for (int k = 0; k < 9; ++k) nb[k] = c->box[k] / s;
-      updateBox(c, nb);
-      HIP_CHECK(hipMemcpyAsync(c->d.force, c->forceSaved, 3 * np * sizeof(long long), hipMemcpyDeviceToDevice, c->stream));
-      HIP_CHECK(hipStreamSynchronize(c->stream));
+      if (sc.options.enabled) sc.accepted += 1;
     }
+    c->blockEndStep = c->step;
+    if (sc.options.enabled && sc.phase == 0 && sc.net &&
+        (int64_t)sc.t.size() >= 2 * (int64_t)sc.options.training_moves)
+      screenTrain(c);
     if (c->windowAttempts >= 10) {
       const double rate = (double)c->windowAccepted / c->windowAttempts;
       if (rate < 0.25)
@@ -1152,7 +1349,7 @@ int rmd_attempt_volume_change(rmd_ctx *c, rmd_move *m) {
     m->accepted = accept;
     m->weight = weight;
     m->old_potential = oldPotential;
-    m->new_potential = newPotential;
+    m->new_potential = evaluated ? newPotential : NAN;
     m->volume = volumeOf(c->box);
     m->volume_scale = c->volumeScale;
     m->acceptance_rate = (double)c->barostatAccepted / c->barostatAttempts;
@@ -1161,6 +1358,7 @@ int rmd_attempt_volume_change(rmd_ctx *c, rmd_move *m) {
 
 int rmd_minimise(rmd_ctx *c, const rmd_min_options *options, rmd_min_result *out) {
   CTX_GUARD(c, {
+    c->blockEndStep = -1;
     rmd_min_options o;
     rmd_min_options_default(&o);
     if (options) o = *options;
@@ -1378,12 +1576,100 @@ int rmd_profile_kernel(rmd_ctx *c, int32_t kernel, int32_t reps, double *avg_ms)
   });
 }
 
+void rmd_screen_options_default(rmd_screen_options *o) {
+  if (!o) return;
+  o->enabled = 1;
+  o->training_moves = 2000;
+  o->training_steps = 4000;
+  o->learning_rate = 1e-3f;
+  o->seed = 0x5eedu;
+}
+
+int rmd_screen_configure(rmd_ctx *c, const rmd_screen_options *options) {
+  CTX_GUARD(c, {
+    rmd_screen_options o;
+    rmd_screen_options_default(&o);
+    if (options) o = *options;
+    if (o.enabled && (o.training_moves <= 0 || o.training_steps <= 0 || !(o.learning_rate > 0)))
+      throw Error(RMD_E_INVALID, "the screen needs training_moves, training_steps and learning_rate > 0");
+    auto &sc = c->screen;
+    if (sc.net) rmd_surrogate_free(sc.net);
+    sc = rmd_ctx::Screen();
+    sc.options = o;
+    if (o.enabled) {
+      rmd_surrogate *net = nullptr;
+      const int st = rmd_surrogate_create(c->device, o.seed, &net);
+      if (st != RMD_OK) throw Error(st, std::string("screen network: ") + rmd_last_global_error());
+      HIP_CHECK(hipSetDevice(c->device));
+      sc.net = net;
+      sc.refVolume = volumeOf(c->box);
+    }
+  });
+}
+
+int rmd_screen_stats_get(rmd_ctx *c, rmd_screen_stats *out) {
+  CTX_GUARD(c, {
+    if (!out) throw Error(RMD_E_INVALID, "null output");
+    const auto &sc = c->screen;
+    std::memset(out, 0, sizeof *out);
+    out->phase = sc.phase;
+    out->samples = (int64_t)sc.t.size();
+    out->moves = sc.moves;
+    out->exact_evaluations = sc.exact;
+    out->screened_out = sc.screened;
+    out->second_stage_rejections = sc.secondStage;
+    out->accepted = sc.accepted;
+    out->training_loss = sc.trainingLoss;
+    out->residual_rms = sc.residualCount ? std::sqrt(sc.residual2 / sc.residualCount) : 0.0;
+    out->estimate_rms = sc.residualCount ? std::sqrt(sc.estimate2 / sc.residualCount) : 0.0;
+  });
+}
+
+int rmd_screen_samples(rmd_ctx *c, float *features, float *targets, int32_t capacity, int32_t *count) {
+  CTX_GUARD(c, {
+    const auto &sc = c->screen;
+    const int32_t n = (int32_t)std::min<size_t>(sc.t.size(), (size_t)std::max(0, capacity));
+    if (features) std::memcpy(features, sc.x.data(), (size_t)n * RMD_SURROGATE_INPUTS * sizeof(float));
+    if (targets) std::memcpy(targets, sc.t.data(), (size_t)n * sizeof(float));
+    if (count) *count = (int32_t)sc.t.size();
+  });
+}
+
 int rmd_set_pme_pipeline(rmd_ctx *c, int32_t mode) {
   CTX_GUARD(c, {
     if (mode < PME_AUTO || mode > PME_LINES) throw Error(RMD_E_INVALID, "PME pipeline must be 0, 1 or 2");
     if (!pmeConfigure(c->p, mode, c->maxLds))
       throw Error(RMD_E_INVALID, "one x plane of this PME grid does not fit in LDS");
     invalidateGraphs(c);
+  });
+}
+
+__global__ void copyProbeKernel(const float4 *__restrict__ src, float4 *__restrict__ dst, size_t n) {
+  for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
+    dst[i] = src[i];
+}
+
+int rmd_copy_probe(rmd_ctx *c, int64_t bytes, int32_t reps, double *avg_ms) {
+  CTX_GUARD(c, {
+    if (bytes <= 0 || bytes % 16 || reps <= 0 || !avg_ms) throw Error(RMD_E_INVALID, "bytes must be a positive multiple of 16");
+    const size_t n = (size_t)bytes / 16;
+    std::vector<void *> tmp;
+    float4 *src = devAlloc<float4>(n, tmp), *dst = devAlloc<float4>(n, tmp);
+    const int blocks = (int)std::min<size_t>((n + 255) / 256, 8192);
+    copyProbeKernel<<<blocks, 256, 0, c->stream>>>(src, dst, n);
+    hipEvent_t a, b;
+    HIP_CHECK(hipEventCreate(&a));
+    HIP_CHECK(hipEventCreate(&b));
+    HIP_CHECK(hipEventRecord(a, c->stream));
+    for (int r = 0; r < reps; ++r) copyProbeKernel<<<blocks, 256, 0, c->stream>>>(src, dst, n);
+    HIP_CHECK(hipEventRecord(b, c->stream));
+    HIP_CHECK(hipEventSynchronize(b));
+    float ms = 0;
+    HIP_CHECK(hipEventElapsedTime(&ms, a, b));
+    *avg_ms = ms / reps;
+    (void)hipEventDestroy(a);
+    (void)hipEventDestroy(b);
+    for (void *p : tmp) (void)hipFree(p);
   });
 }
 

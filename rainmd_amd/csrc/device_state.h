@@ -25,6 +25,12 @@ enum EnergyTerm {
   E_RECIPROCAL,
   E_KINETIC,
   E_KINETIC_EARLY,
+  // d(energy)/d(ln lambda) under a uniform scaling of every coordinate by
+  // lambda: direct-space pairs (sum r dU/dr) and the reciprocal sum
+  // (sum_k E_k (2 pi^2 m^2 / alpha^2 - 1)). Filled with the energies; the
+  // barostat's learned screen reads them as features.
+  E_DSCALE_DIRECT,
+  E_DSCALE_RECIPROCAL,
   E_TERM_COUNT
 };
 constexpr int MAX_SLOTS = 65536;

@@ -30,6 +30,9 @@ struct Error : std::runtime_error {
   Error(int c, const std::string &m) : std::runtime_error(m), code(c) {}
 };
 
+// The message rmd_last_global_error() returns (per thread).
+void setGlobalError(const std::string &m);
+
 struct Topology {
   int atomCount = 0, typeCount = 0, residueCount = 0;
   std::vector<std::string> atomNames, residueNames, atomTypeNames;

@@ -692,9 +692,9 @@ __global__ void __launch_bounds__(256) kineticKernel(Dev d, int term) {
 }
 
 // Sum each term's slot row in a fixed order: one block per term.
-__global__ void __launch_bounds__(256) reduceEnergiesKernel(const double *slots, const int *counts, double *out) {
+__global__ void __launch_bounds__(256) reduceEnergiesKernel(const double *slots, SlotCounts counts, double *out) {
   const int term = blockIdx.x;
-  const int count = counts[term];
+  const int count = counts.c[term];
   double s = 0.0;
   for (int k = threadIdx.x; k < count; k += blockDim.x) s += slots[term * MAX_SLOTS + k];
   __shared__ double part[256];
@@ -872,7 +872,7 @@ int launchKinetic(const Dev &d, int term, hipStream_t s) {
   return blocks;
 }
 
-void launchReduceEnergies(const Dev &d, const int *counts, double *out, hipStream_t s) {
+void launchReduceEnergies(const Dev &d, const SlotCounts &counts, double *out, hipStream_t s) {
   reduceEnergiesKernel<<<E_TERM_COUNT, 256, 0, s>>>(d.slots, counts, out);
 }
 

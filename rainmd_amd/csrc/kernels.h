@@ -32,6 +32,7 @@ struct PmeDev {
   float *influence;         // K0*K1*K2h: pref * kernel(m) (0 at m = 0)
   float2 *twiddle[3];       // per axis, exp(+2 pi i k / K)
   int radices[3][8];        // factorisation per axis (0-terminated)
+  double piOverAlpha2;      // pi^2 / alpha^2 (the scaling derivative of the reciprocal energy)
   int planeSpan;            // max(ceil(K1/2) * K2, K2h * K1): one plane's complex work buffer
   int planes;               // 1: plane pipeline, 0: line pipeline
 };
@@ -70,7 +71,12 @@ struct IntegratorParams {
 int launchIntegrateOpen(const Dev &d, const IntegratorParams &ip, int closing, int early, hipStream_t s);
 int launchIntegrateClose(const Dev &d, const IntegratorParams &ip, int kinetic, hipStream_t s);
 int launchKinetic(const Dev &d, int term, hipStream_t s);
-void launchReduceEnergies(const Dev &d, const int *slotsUsed, double *out, hipStream_t s);
+// Sums each term's slot row in a fixed order; the counts travel by value so a
+// captured graph keeps them.
+struct SlotCounts {
+  int c[E_TERM_COUNT];
+};
+void launchReduceEnergies(const Dev &d, const SlotCounts &counts, double *out, hipStream_t s);
 void launchZeroForces(const Dev &d, hipStream_t s);
 // Barostat: mass-weighted molecule centroids and the scaling about them.
 void launchScaleMolecules(const Dev &d, const int *molStart, const int *molAtoms, int nMol, float4 *centroids,

@@ -12,14 +12,25 @@
 
 namespace rmd {
 
+constexpr int LDS_TYPES = 64;  // type-pair tables up to 64^2 entries live in LDS
+
 template <int LANES>
 __global__ void __launch_bounds__(256) nonbondedKernel(Dev d, float alpha, int energies) {
+  // The LJ type-pair table in LDS: one lookup per pair, off the global
+  // latency chain (list index -> position and type -> table entry).
+  __shared__ float2 ldsTable[LDS_TYPES * LDS_TYPES];
+  const int tableSize = d.types * d.types;
+  const bool tableInLds = d.types <= LDS_TYPES;
+  if (tableInLds)
+    for (int k = threadIdx.x; k < tableSize; k += blockDim.x) ldsTable[k] = d.lj[k];
+  __syncthreads();
+  const float2 *table = tableInLds ? ldsTable : d.lj;
   const BoxF &box = *d.box;
   const int tid = blockIdx.x * blockDim.x + threadIdx.x;
   const int atom = tid / LANES;
   const int sub = tid % LANES;
   const bool valid = atom < d.n;
-  float fx = 0.f, fy = 0.f, fz = 0.f, elj = 0.f, ecoul = 0.f;
+  float fx = 0.f, fy = 0.f, fz = 0.f, elj = 0.f, ecoul = 0.f, dscale = 0.f;
   if (valid) {
     const float4 pi = d.pos[atom];
     const int ti = d.type[atom] * d.types;
@@ -27,13 +38,26 @@ __global__ void __launch_bounds__(256) nonbondedKernel(Dev d, float alpha, int e
     const float gaussScale = TWO_OVER_SQRT_PI_F * alpha;
     const int count = d.nbrCount[atom];
     const int *list = d.nbrList + (size_t)atom * d.cap;
-    for (int k = sub; k < count; k += LANES) {
-      const int j = list[k];
-      const float4 pj = d.pos[j];
+    // Software pipeline: the neighbour index two iterations ahead and the
+    // position and type one iteration ahead are in flight while a pair is
+    // computed. Past the end the prefetch reads the atom itself (in bounds).
+    int k = sub;
+    int jAhead = k + LANES < count ? list[k + LANES] : atom;
+    int jNext = k < count ? list[k] : atom;
+    float4 pNext = d.pos[jNext];
+    int tNext = d.type[jNext];
+    while (k < count) {
+      const float4 pj = pNext;
+      const int tj = tNext;
+      k += LANES;
+      jNext = jAhead;
+      jAhead = k + LANES < count ? list[k + LANES] : atom;
+      pNext = d.pos[jNext];
+      tNext = d.type[jNext];
       float dx = pi.x - pj.x, dy = pi.y - pj.y, dz = pi.z - pj.z;
       const float r2 = minimumImageR2(dx, dy, dz, box);
       if (r2 > box.rc2) continue;
-      const float2 ab = d.lj[ti + d.type[j]];
+      const float2 ab = table[ti + tj];
       const float invR2 = __builtin_amdgcn_rcpf(r2);
       const float invR = __builtin_amdgcn_rsqf(r2);
       const float r = r2 * invR;
@@ -47,6 +71,7 @@ __global__ void __launch_bounds__(256) nonbondedKernel(Dev d, float alpha, int e
       if (energies) {
         elj += (lj12 - ab.y) * invR6;
         ecoul += qq * screened * invR;
+        dscale -= scalar * r2;  // r dU/dr
       }
       fx += scalar * dx;
       fy += scalar * dy;
@@ -69,23 +94,27 @@ __global__ void __launch_bounds__(256) nonbondedKernel(Dev d, float alpha, int e
     atomicAddFixed(&d.force[2 * d.nPad + atom], iz);
   }
   if (energies) {
-    __shared__ double partial[2][4];
+    __shared__ double partial[3][4];
     double a = waveSum(0.5 * (double)elj);
     double b = waveSum(0.5 * (double)ecoul);
+    double c = waveSum(0.5 * (double)dscale);
     const int wave = threadIdx.x >> 6;
     if ((threadIdx.x & 63) == 0) {
       partial[0][wave] = a;
       partial[1][wave] = b;
+      partial[2][wave] = c;
     }
     __syncthreads();
     if (threadIdx.x == 0) {
-      double sa = 0, sb = 0;
+      double sa = 0, sb = 0, sc = 0;
       for (int w = 0; w < (int)(blockDim.x >> 6); ++w) {
         sa += partial[0][w];
         sb += partial[1][w];
+        sc += partial[2][w];
       }
       d.slots[E_LJ_DIRECT * MAX_SLOTS + blockIdx.x] = sa;
       d.slots[E_COUL_DIRECT * MAX_SLOTS + blockIdx.x] = sb;
+      d.slots[E_DSCALE_DIRECT * MAX_SLOTS + blockIdx.x] = sc;
     }
   }
 }

@@ -229,6 +229,39 @@ __device__ __forceinline__ void addGridForce(const Dev &d, const PmeDev &p, int 
   atomicAddFixed(&d.force[2 * d.nPad + atom], __double2ll_rn(fz * FORCE_SCALE));
 }
 
+// |m|^2 of the wave vector of half-spectrum point (x, y, m), reciprocal rows
+// (the influence function's convention).
+__device__ __forceinline__ double waveVector2(const PmeDev &p, const double *recip, int x, int y, int m) {
+  const int kx = x <= p.K[0] / 2 ? x : x - p.K[0];
+  const int ky = y <= p.K[1] / 2 ? y : y - p.K[1];
+  const double mx = kx * recip[0] + ky * recip[1] + m * recip[2];
+  const double my = kx * recip[3] + ky * recip[4] + m * recip[5];
+  const double mz = kx * recip[6] + ky * recip[7] + m * recip[8];
+  return mx * mx + my * my + mz * mz;
+}
+
+// The block's two energy sums into their slot rows.
+__device__ void storeReciprocal(const Dev &d, double energy, double dscale) {
+  __shared__ double partial[2][16];
+  energy = waveSum(energy);
+  dscale = waveSum(dscale);
+  const int waves = blockDim.x >> 6;
+  if ((threadIdx.x & 63) == 0) {
+    partial[0][threadIdx.x >> 6] = energy;
+    partial[1][threadIdx.x >> 6] = dscale;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double e = 0.0, s = 0.0;
+    for (int w = 0; w < waves; ++w) {
+      e += partial[0][w];
+      s += partial[1][w];
+    }
+    d.slots[E_RECIPROCAL * MAX_SLOTS + blockIdx.x] = e;
+    d.slots[E_DSCALE_RECIPROCAL * MAX_SLOTS + blockIdx.x] = s;
+  }
+}
+
 // ============================================================================
 // Plane pipeline
 // ============================================================================
@@ -342,16 +375,19 @@ __global__ void __launch_bounds__(XLINE_THREADS) pmeXLinesKernel(Dev d, PmeDev p
   }
   __syncthreads();
   float2 *R = fftBatch(A, B, lines, K0, p.radices[0], tw, +1);
-  double energy = 0.0;
+  double energy = 0.0, dscale = 0.0;
   if (convolve) {
     const float *g0 = p.influence + (size_t)j * K2h + m0;
+    const double *recip = d.box->recip64;
     for (int idx = t; idx < lines * K0; idx += nt) {
       const int x = idx / lines, l = idx - x * lines;
       const int m = m0 + l;
       const double weight = (m == 0 || (p.K[2] % 2 == 0 && m == p.K[2] / 2)) ? 1.0 : 2.0;
       const float g = g0[x * stride + l];
       const float2 f = R[l * K0 + x];
-      energy += weight * (double)g * ((double)f.x * f.x + (double)f.y * f.y);
+      const double e = weight * (double)g * ((double)f.x * f.x + (double)f.y * f.y);
+      energy += e;
+      if (energies) dscale += e * (2.0 * p.piOverAlpha2 * waveVector2(p, recip, x, j, m) - 1.0);
       R[l * K0 + x] = make_float2(2.f * g * f.x, 2.f * g * f.y);
     }
     __syncthreads();
@@ -361,17 +397,7 @@ __global__ void __launch_bounds__(XLINE_THREADS) pmeXLinesKernel(Dev d, PmeDev p
     const int x = idx / lines, l = idx - x * lines;
     col[x * stride + l] = R[l * K0 + x];
   }
-  if (convolve && energies) {
-    __shared__ double partial[XLINE_THREADS / 64];
-    energy = waveSum(energy);
-    if ((t & 63) == 0) partial[t >> 6] = energy;
-    __syncthreads();
-    if (t == 0) {
-      double s = 0.0;
-      for (int w = 0; w < XLINE_THREADS / 64; ++w) s += partial[w];
-      d.slots[E_RECIPROCAL * MAX_SLOTS + blockIdx.x] = s;
-    }
-  }
+  if (convolve && energies) storeReciprocal(d, energy, dscale);
 }
 
 // 3. Inverse y and z transforms of plane i into the real potential, then each
@@ -577,15 +603,18 @@ __global__ void __launch_bounds__(256) pmeXConvolveKernel(Dev d, PmeDev p, int c
   }
   __syncthreads();
   float2 *r = fftBatch(A, B, count, K, p.radices[0], tw, +1);
-  double energy = 0.0;
+  double energy = 0.0, dscale = 0.0;
   if (convolve) {
+    const double *recip = d.box->recip64;
     for (int idx = threadIdx.x; idx < count * K; idx += blockDim.x) {
       const int g = idx / K, k = idx - g * K;
-      const int m = (first + g) % p.K2h;
+      const int line = first + g, y = line / p.K2h, m = line - y * p.K2h;
       const double weight = (m == 0 || (p.K[2] % 2 == 0 && m == p.K[2] / 2)) ? 1.0 : 2.0;
-      const float gi = p.influence[(size_t)(first + g) + k * stride];
+      const float gi = p.influence[(size_t)line + k * stride];
       const float2 f = r[idx];
-      energy += weight * (double)gi * ((double)f.x * f.x + (double)f.y * f.y);
+      const double e = weight * (double)gi * ((double)f.x * f.x + (double)f.y * f.y);
+      energy += e;
+      if (energies) dscale += e * (2.0 * p.piOverAlpha2 * waveVector2(p, recip, k, y, m) - 1.0);
       r[idx] = make_float2(2.f * gi * f.x, 2.f * gi * f.y);
     }
     __syncthreads();
@@ -595,13 +624,7 @@ __global__ void __launch_bounds__(256) pmeXConvolveKernel(Dev d, PmeDev p, int c
     const int g = idx / K, k = idx - g * K;
     p.cgrid[(size_t)(first + g) + k * stride] = r[idx];
   }
-  if (convolve && energies) {
-    __shared__ double partial[4];
-    energy = waveSum(energy);
-    if ((threadIdx.x & 63) == 0) partial[threadIdx.x >> 6] = energy;
-    __syncthreads();
-    if (threadIdx.x == 0) d.slots[E_RECIPROCAL * MAX_SLOTS + blockIdx.x] = partial[0] + partial[1] + partial[2] + partial[3];
-  }
+  if (convolve && energies) storeReciprocal(d, energy, dscale);
 }
 
 // z pass, inverse: Hermitian extension of two half spectra into one complex

@@ -74,6 +74,10 @@ def _check(rc, ctx=None):
     raise _ERRORS.get(rc, RainmdError)(msg.decode(errors="replace"))
 
 
+def _error(rc, message):
+    return _ERRORS.get(rc, RainmdError)(message)
+
+
 def _arr(ptr, n, dtype):
     if not ptr or n == 0:
         return np.zeros(0, dtype=dtype)
@@ -459,6 +463,33 @@ class GpuSimulation:
     def set_graphs(self, enabled):
         _check(L.load().rmd_set_graphs(self._h, 1 if enabled else 0), self._h)
 
+    # -- the barostat's learned screen --
+    def configure_screen(self, enabled=True, training_moves=2000, training_steps=4000, learning_rate=1e-3,
+                         seed=0x5eed):
+        o = L.ScreenOptions(1 if enabled else 0, int(training_moves), int(training_steps), float(learning_rate),
+                            int(seed))
+        _check(L.load().rmd_screen_configure(self._h, C.byref(o)), self._h)
+
+    def screen_stats(self):
+        st = L.ScreenStats()
+        _check(L.load().rmd_screen_stats_get(self._h, C.byref(st)), self._h)
+        return _struct_dict(st)
+
+    def screen_samples(self):
+        n = C.c_int32()
+        _check(L.load().rmd_screen_samples(self._h, None, None, 0, C.byref(n)), self._h)
+        x = np.zeros((max(n.value, 1), L.SURROGATE_INPUTS), dtype=np.float32)
+        t = np.zeros(max(n.value, 1), dtype=np.float32)
+        _check(L.load().rmd_screen_samples(self._h, x.ctypes.data_as(L._fp), t.ctypes.data_as(L._fp), n.value,
+                                           C.byref(n)), self._h)
+        return x[:n.value], t[:n.value]
+
+    def copy_probe(self, nbytes, reps=20):
+        """Average duration (ms) of a plain device copy of nbytes (counter calibration)."""
+        ms = C.c_double()
+        _check(L.load().rmd_copy_probe(self._h, int(nbytes), int(reps), C.byref(ms)), self._h)
+        return ms.value
+
     def set_pme_pipeline(self, mode):
         """'auto', 'planes' or 'lines'."""
         ids = {"auto": 0, "planes": 1, "lines": 2}
@@ -520,6 +551,74 @@ def maxwell_boltzmann(system, temperature, seed=0):
     m = system.masses
     sigma = np.where(m > 0, np.sqrt(kT / np.where(m > 0, m, 1.0)), 0.0)
     return (g * sigma[:, None]).reshape(-1)
+
+
+# ---- the learned screen's network on its own ---------------------------------
+
+class Surrogate:
+    """The 16 -> 64 -> 64 -> 1 tanh MLP of the barostat's screen on one device
+    (include/rainmd.h, rmd_surrogate_*): BF16 MFMA layer products with FP32
+    accumulation, Adam training on the GPU."""
+
+    INPUTS = L.SURROGATE_INPUTS
+    PARAMS = L.SURROGATE_PARAMS
+
+    def __init__(self, seed=0, device=0):
+        h = C.c_void_p()
+        self._h = None
+        st = L.load().rmd_surrogate_create(int(device), int(seed), C.byref(h))
+        self._ok(st)
+        self._h = h
+
+    def release(self):
+        if self._h:
+            L.load().rmd_surrogate_free(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.release()
+        except Exception:
+            pass
+
+    def _ok(self, st):
+        if st != 0:
+            raise _error(st, (L.load().rmd_last_global_error() or b"").decode(errors="replace"))
+
+    @property
+    def params(self):
+        p = np.zeros(self.PARAMS, dtype=np.float32)
+        self._ok(L.load().rmd_surrogate_get_params(self._h, p.ctypes.data_as(L._fp)))
+        return p
+
+    @params.setter
+    def params(self, p):
+        p = np.ascontiguousarray(p, dtype=np.float32)
+        assert p.shape == (self.PARAMS,)
+        self._ok(L.load().rmd_surrogate_set_params(self._h, p.ctypes.data_as(L._fp)))
+
+    def forward(self, x):
+        x = np.ascontiguousarray(x, dtype=np.float32).reshape(-1, self.INPUTS)
+        y = np.zeros(len(x), dtype=np.float32)
+        self._ok(L.load().rmd_surrogate_forward(self._h, x.ctypes.data_as(L._fp), len(x), y.ctypes.data_as(L._fp)))
+        return y
+
+    def gradient(self, x, t):
+        x = np.ascontiguousarray(x, dtype=np.float32).reshape(-1, self.INPUTS)
+        t = np.ascontiguousarray(t, dtype=np.float32)
+        loss = np.zeros(1, dtype=np.float32)
+        g = np.zeros(self.PARAMS, dtype=np.float32)
+        self._ok(L.load().rmd_surrogate_gradient(self._h, x.ctypes.data_as(L._fp), t.ctypes.data_as(L._fp), len(x),
+                                                 loss.ctypes.data_as(L._fp), g.ctypes.data_as(L._fp)))
+        return float(loss[0]), g
+
+    def train(self, x, t, steps, learning_rate=1e-3):
+        x = np.ascontiguousarray(x, dtype=np.float32).reshape(-1, self.INPUTS)
+        t = np.ascontiguousarray(t, dtype=np.float32)
+        loss = np.zeros(1, dtype=np.float32)
+        self._ok(L.load().rmd_surrogate_train(self._h, x.ctypes.data_as(L._fp), t.ctypes.data_as(L._fp), len(x),
+                                              int(steps), float(learning_rate), loss.ctypes.data_as(L._fp)))
+        return float(loss[0])
 
 
 # ---- the synthetic workloads ------------------------------------------------

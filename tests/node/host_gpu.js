@@ -16,7 +16,7 @@ async function main() {
   out.energy = await sim.energy();
   out.energy.boxVectors = Array.from(out.energy.boxVectors);
   // dynamics: minimise, velocities, a few blocks, a stop signal, a volume move
-  const m = await sim.minimise({ maxIterations: 20 });
+  const m = await sim.minimise({ maxIterations: 150, forceTolerance: 1000 });
   out.minimise = m;
   sim.setState(Float64Array.from(await sim.readPositions()), rmd.maxwellBoltzmann(system, 300, 11), { step: 0 });
   out.step = await sim.run(100, { blockSize: 25 });

@@ -84,6 +84,34 @@ def test_npt_run_moves_the_volume():
     assert np.isfinite(e["potential"]) and 240 < e["temperature"] < 360
 
 
+def test_volume_move_reuses_the_block_energy_and_a_rejection_restores_the_state():
+    system = R.workload_system("ala-389w")
+    sim = R.GpuSimulation(system, temperature=300.0, pressure=1.0, barostat_interval=1000, seed=5)
+    sim.minimise(max_iterations=30, force_tolerance=1000.0)
+    sim.set_state(sim.read_positions(), R.maxwell_boltzmann(system, 300.0, 5), step=0)
+    sim.run(50)
+    seen = {True: 0, False: 0}
+    for attempt in range(12):
+        before = sim.read_state()
+        fresh, _ = sim.evaluate(forces=False)
+        move = sim.attempt_volume_change()
+        # the old potential came from the last step of the block (or the last
+        # move), not from a new evaluation, and agrees with one up to the f32
+        # per-lane summation order of a list built at another step
+        print(attempt, move["accepted"], move["old_potential"], fresh["potential"])
+        assert rel(move["old_potential"], fresh["potential"]) < 1e-7
+        if not move["accepted"]:
+            after = sim.read_state()
+            assert np.array_equal(after["positions"], before["positions"])
+            assert np.array_equal(after["box_vectors"], before["box_vectors"])
+            again, _ = sim.evaluate(forces=False)
+            assert again["potential"] == fresh["potential"]
+        seen[bool(move["accepted"])] += 1
+        sim.run(5)
+    print("moves", seen)
+    assert seen[False] > 0
+
+
 @pytest.mark.parametrize("scale", [1.02, 0.97])
 def test_scale_volume_matches_openmm_and_oracle(scale, golden):
     # tools/gpu/check-barostat.mjs: energy at forced scales vs ala-wat-scale-*.json
