@@ -41,7 +41,15 @@ T *devAlloc(size_t count, std::vector<void *> &held) {
     throw rmd::Error(e == hipErrorOutOfMemory ? RMD_E_BUDGET : RMD_E_DEVICE,
                      std::string("hipMalloc of ") + std::to_string(count * sizeof(T)) + " bytes failed: " +
                          hipGetErrorString(e));
-  (void)hipMemset(p, 0, count * sizeof(T));
+  // Zeroed and finished before returning: the context's stream is
+  // non-blocking, so work queued on it later is not ordered after the null
+  // stream, and a memset still in flight could clear an upload made after it.
+  e = hipMemsetAsync(p, 0, count * sizeof(T), nullptr);
+  if (e == hipSuccess) e = hipStreamSynchronize(nullptr);
+  if (e != hipSuccess) {
+    (void)hipFree(p);
+    throw rmd::Error(RMD_E_DEVICE, std::string("clearing a new buffer failed: ") + hipGetErrorString(e));
+  }
   held.push_back(p);
   return static_cast<T *>(p);
 }
@@ -1073,7 +1081,8 @@ int rmd_set_state(rmd_ctx *c, const double *pos, const double *vel, int64_t step
 int rmd_set_noise(rmd_ctx *c, const float *noise) {
   CTX_GUARD(c, {
     if (!noise) throw Error(RMD_E_INVALID, "noise is required");
-    HIP_CHECK(hipMemcpy(c->d.noise, noise, 3 * (size_t)c->d.n * sizeof(float), hipMemcpyHostToDevice));
+    HIP_CHECK(hipMemcpyAsync(c->d.noise, noise, 3 * (size_t)c->d.n * sizeof(float), hipMemcpyHostToDevice, c->stream));
+    HIP_CHECK(hipStreamSynchronize(c->stream));
   });
 }
 
@@ -1432,7 +1441,8 @@ int rmd_diagnostics_get(rmd_ctx *c, rmd_diagnostics *o) {
   CTX_GUARD(c, {
     std::memset(o, 0, sizeof *o);
     Scalars h;
-    HIP_CHECK(hipMemcpy(&h, c->d.sc, sizeof h, hipMemcpyDeviceToHost));
+    HIP_CHECK(hipMemcpyAsync(&h, c->d.sc, sizeof h, hipMemcpyDeviceToHost, c->stream));
+    HIP_CHECK(hipStreamSynchronize(c->stream));
     o->max_shake_iterations = (int32_t)h.shakeMaxIterations;
     o->shake_hit_iteration_cap = h.shakeFailed ? 1 : 0;
     o->rattle_hit_iteration_cap = h.rattleFailed ? 1 : 0;
@@ -1510,10 +1520,10 @@ int rmd_debug_fft_forward(rmd_ctx *c, const float *real, float *out) {
     std::vector<void *> tmp;
     float *dr = devAlloc<float>(n, tmp);
     float2 *dc = devAlloc<float2>(nc, tmp);
-    HIP_CHECK(hipMemcpy(dr, real, n * sizeof(float), hipMemcpyHostToDevice));
+    HIP_CHECK(hipMemcpyAsync(dr, real, n * sizeof(float), hipMemcpyHostToDevice, c->stream));
     launchFftForward(c->d, c->p, dr, dc, c->stream);
+    HIP_CHECK(hipMemcpyAsync(out, dc, nc * sizeof(float2), hipMemcpyDeviceToHost, c->stream));
     HIP_CHECK(hipStreamSynchronize(c->stream));
-    HIP_CHECK(hipMemcpy(out, dc, nc * sizeof(float2), hipMemcpyDeviceToHost));
     for (void *p : tmp) (void)hipFree(p);
   });
 }

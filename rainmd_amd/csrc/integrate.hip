@@ -179,19 +179,19 @@ __device__ void rattleWater(Atom *w) {
       {iO * dot(r02, r01), (iO + i2) * dot(r02, r02), i2 * dot(r02, r12)},
       {-i1 * dot(r12, r01), i2 * dot(r12, r02), (i1 + i2) * dot(r12, r12)}};
   const double bvec[3] = {-dot(r01, v01), -dot(r02, v02), -dot(r12, v12)};
-  const double det = A[0][0] * (A[1][1] * A[2][2] - A[1][2] * A[2][1]) -
-                     A[0][1] * (A[1][0] * A[2][2] - A[1][2] * A[2][0]) +
-                     A[0][2] * (A[1][0] * A[2][1] - A[1][1] * A[2][0]);
+  // Cramer's rule with each determinant written out (no array indexed in a
+  // loop, so everything stays in registers).
+  auto det3 = [](double a00, double a01, double a02, double a10, double a11, double a12, double a20, double a21,
+                 double a22) {
+    return a00 * (a11 * a22 - a12 * a21) - a01 * (a10 * a22 - a12 * a20) + a02 * (a10 * a21 - a11 * a20);
+  };
+  const double det = det3(A[0][0], A[0][1], A[0][2], A[1][0], A[1][1], A[1][2], A[2][0], A[2][1], A[2][2]);
   if (!(fabs(det) > 0)) return;
-  double l[3];
-  for (int k = 0; k < 3; ++k) {
-    double M[3][3];
-    for (int r = 0; r < 3; ++r)
-      for (int c = 0; c < 3; ++c) M[r][c] = c == k ? bvec[r] : A[r][c];
-    l[k] = (M[0][0] * (M[1][1] * M[2][2] - M[1][2] * M[2][1]) - M[0][1] * (M[1][0] * M[2][2] - M[1][2] * M[2][0]) +
-            M[0][2] * (M[1][0] * M[2][1] - M[1][1] * M[2][0])) /
-           det;
-  }
+  const double inv = 1.0 / det;
+  const double l[3] = {
+      det3(bvec[0], A[0][1], A[0][2], bvec[1], A[1][1], A[1][2], bvec[2], A[2][1], A[2][2]) * inv,
+      det3(A[0][0], bvec[0], A[0][2], A[1][0], bvec[1], A[1][2], A[2][0], bvec[2], A[2][2]) * inv,
+      det3(A[0][0], A[0][1], bvec[0], A[1][0], A[1][1], bvec[1], A[2][0], A[2][1], bvec[2]) * inv};
   for (int c = 0; c < 3; ++c) {
     w[0].v[c] += (l[0] * r01[c] + l[1] * r02[c]) * iO;
     w[1].v[c] += (-l[0] * r01[c] + l[2] * r12[c]) * i1;
@@ -224,20 +224,24 @@ __device__ void driftWater(Atom *w, double h, const double4 geom) {
 
 constexpr int STAR_ATOMS = STAR_MAX + 1;
 
+// Every slot is loaded (slots past the cluster repeat its last atom) and the
+// unused ones are then given zero inverse mass, velocity and force: no branch
+// around the loads, so the compiler keeps the whole cluster in registers.
 __device__ void starLoad(const Dev &d, int cluster, int m, Atom *w, int *atom, double *target) {
   const int ab = d.clusterAtomStart[cluster], cb = d.clusterConStart[cluster];
 #pragma unroll
   for (int k = 0; k < STAR_ATOMS; ++k) {
-    atom[k] = k <= m ? d.clusterAtoms[ab + k] : -1;
-    if (k <= m) {
-      loadAtom(d, atom[k], w[k]);
-    } else {
-      for (int c = 0; c < 3; ++c) w[k].x[c] = w[k].v[c] = w[k].f[c] = 0.0;
-      w[k].im = 0.0;
+    atom[k] = d.clusterAtoms[ab + (k <= m ? k : m)];
+    loadAtom(d, atom[k], w[k]);
+    const double use = k <= m ? 1.0 : 0.0;
+    w[k].im *= use;
+    for (int c = 0; c < 3; ++c) {
+      w[k].v[c] *= use;
+      w[k].f[c] *= use;
     }
   }
 #pragma unroll
-  for (int k = 0; k < STAR_MAX; ++k) target[k] = k < m ? d.clusterConD[cb + k] : 1.0;
+  for (int k = 0; k < STAR_MAX; ++k) target[k] = d.clusterConD[cb + (k < m ? k : m - 1)];
 }
 
 __device__ void starRattle(const Dev &d, const IntegratorParams &ip, Atom *w, int m, const double *target) {

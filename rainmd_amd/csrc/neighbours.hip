@@ -144,6 +144,75 @@ __global__ void __launch_bounds__(256) cellsSortKernel(Dev d) {
   }
 }
 
+// The whole cell list of a small system in one workgroup: assignment with
+// LDS-atomic counts, the scan, an LDS scatter, and a rank sort that puts each
+// cell's atoms in index order (rank = atoms of the same cell with a lower
+// index) - the same lists as the four-kernel path, with no global atomics
+// and three launches fewer on every step. Used when the atoms and the cells
+// fit the LDS arrays below.
+constexpr int SMALL_THREADS = 1024;
+constexpr int SMALL_MAX_ATOMS = 16384;
+constexpr int SMALL_MAX_CELLS = 4096;
+
+__global__ void __launch_bounds__(SMALL_THREADS) cellsBuildSmallKernel(Dev d) {
+  if (!d.sc->rebuild) return;
+  __shared__ int count[SMALL_MAX_CELLS];
+  __shared__ int begin[SMALL_MAX_CELLS + 1];
+  __shared__ int unsorted[SMALL_MAX_ATOMS];
+  __shared__ int partial[SMALL_THREADS];
+  const BoxF &box = *d.box;
+  const int total = box.cellTotal, n = d.n, t = threadIdx.x;
+  for (int c = t; c < total; c += SMALL_THREADS) count[c] = 0;
+  __syncthreads();
+  for (int i = t; i < n; i += SMALL_THREADS) {
+    const float4 p = d.pos[i];
+    int c3[3];
+    const int c = cellIndex(p.x, p.y, p.z, box, c3);
+    d.cellOfAtom[i] = c;
+    atomicAdd(&count[c], 1);
+    if (!(isfinite(p.x) && isfinite(p.y) && isfinite(p.z))) atomicMax(&d.sc->nonFinite, 1u);
+  }
+  __syncthreads();
+  // exclusive scan: each thread sums a contiguous run of cells, the runs are
+  // scanned across the block, then each run is written out
+  const int per = (total + SMALL_THREADS - 1) / SMALL_THREADS;
+  const int c0 = min(total, t * per), c1 = min(total, c0 + per);
+  int run = 0;
+  for (int c = c0; c < c1; ++c) run += count[c];
+  partial[t] = run;
+  __syncthreads();
+  for (int offset = 1; offset < SMALL_THREADS; offset <<= 1) {
+    const int add = t >= offset ? partial[t - offset] : 0;
+    __syncthreads();
+    partial[t] += add;
+    __syncthreads();
+  }
+  int at = partial[t] - run;
+  for (int c = c0; c < c1; ++c) {
+    begin[c] = at;
+    d.cellStart[c] = at;
+    at += count[c];
+    count[c] = begin[c];  // now the scatter cursor
+  }
+  if (t == SMALL_THREADS - 1) {
+    begin[total] = partial[t];
+    d.cellStart[total] = partial[t];
+  }
+  __syncthreads();
+  for (int i = t; i < n; i += SMALL_THREADS) unsorted[atomicAdd(&count[d.cellOfAtom[i]], 1)] = i;
+  __syncthreads();
+  for (int i = t; i < n; i += SMALL_THREADS) {
+    const int c = d.cellOfAtom[i];
+    const int b = begin[c], e = begin[c + 1];
+    int rank = 0;
+    for (int k = b; k < e; ++k) rank += unsorted[k] < i;
+    float4 p = d.pos[i];
+    p.w = __int_as_float(i);
+    d.cellAtoms[b + rank] = i;
+    d.cellPos[b + rank] = p;
+  }
+}
+
 // Per-axis stencil offsets, deduplicated (shared/topology/cells.js:25-29):
 // three or more cells visit {-1, 0, 1}, two visit {0, 1}, one visits {0}.
 __device__ __forceinline__ void axisRange(int count, int &base, int &len) {
@@ -242,11 +311,15 @@ __global__ void __launch_bounds__(256) neighboursKernel(Dev d) {
 void launchForceRebuildFlag(const Dev &d, hipStream_t s) { setRebuildKernel<<<1, 1, 0, s>>>(d.sc); }
 
 void launchRebuild(const Dev &d, hipStream_t s) {
-  const int blocks = (d.n + 255) / 256;
-  cellsAssignKernel<<<blocks, 256, 0, s>>>(d);
-  cellsScanKernel<<<1, 1024, 0, s>>>(d);
-  cellsScatterKernel<<<blocks, 256, 0, s>>>(d);
-  cellsSortKernel<<<d.cellCapacity, 256, 0, s>>>(d);
+  if (d.n <= SMALL_MAX_ATOMS && d.cellCapacity <= SMALL_MAX_CELLS) {
+    cellsBuildSmallKernel<<<1, SMALL_THREADS, 0, s>>>(d);
+  } else {
+    const int blocks = (d.n + 255) / 256;
+    cellsAssignKernel<<<blocks, 256, 0, s>>>(d);
+    cellsScanKernel<<<1, 1024, 0, s>>>(d);
+    cellsScatterKernel<<<blocks, 256, 0, s>>>(d);
+    cellsSortKernel<<<d.cellCapacity, 256, 0, s>>>(d);
+  }
   neighboursKernel<<<(d.n + 3) / 4, 256, 0, s>>>(d);
 }
 

@@ -103,6 +103,35 @@ __device__ __forceinline__ void smallDft(float2 *v, int R, int sign) {
   }
 }
 
+// One radix-R pass of a Stockham autosort FFT over `lines` lines of length K
+// (line l at src + l * K, in LDS); `Ns` is the product of the radices done.
+// R is a compile-time constant, so the butterfly stays in registers.
+template <int R>
+__device__ __forceinline__ void fftStage(const float2 *src, float2 *dst, int lines, int K, int Ns, const float2 *tw,
+                                         int sign) {
+  const int L = K / R;
+  const int twStride = K / (Ns * R);
+  const int total = lines * L;
+  for (int idx = threadIdx.x; idx < total; idx += blockDim.x) {
+    const int line = idx / L, j = idx - line * L;
+    const float2 *in = src + line * K;
+    float2 *out = dst + line * K;
+    const int k = j % Ns;
+    float2 v[8];
+    v[0] = in[j];
+#pragma unroll
+    for (int q = 1; q < R; ++q) {
+      float2 w = tw[k * q * twStride];
+      if (sign < 0) w.y = -w.y;
+      v[q] = cmul(in[j + q * L], w);
+    }
+    smallDft(v, R, sign);
+    const int base = (j / Ns) * Ns * R + k;
+#pragma unroll
+    for (int q = 0; q < R; ++q) out[base + q * Ns] = v[q];
+  }
+}
+
 // Stockham autosort FFT of `lines` lines of length K stored line-major in LDS
 // (line l at src + l * K), by all threads of the block. `rad` lists the
 // radices (0-terminated), `tw` holds exp(+2 pi i t / K) for t < K in LDS.
@@ -111,24 +140,12 @@ __device__ float2 *fftBatch(float2 *src, float2 *dst, int lines, int K, const in
   int Ns = 1;
   for (int s = 0; s < 8 && rad[s] > 0; ++s) {
     const int R = rad[s];
-    const int L = K / R;
-    const int twStride = K / (Ns * R);
-    const int total = lines * L;
-    for (int idx = threadIdx.x; idx < total; idx += blockDim.x) {
-      const int line = idx / L, j = idx - line * L;
-      const float2 *in = src + line * K;
-      float2 *out = dst + line * K;
-      const int k = j % Ns;
-      float2 v[7];
-      v[0] = in[j];
-      for (int q = 1; q < R; ++q) {
-        float2 w = tw[k * q * twStride];
-        if (sign < 0) w.y = -w.y;
-        v[q] = cmul(in[j + q * L], w);
-      }
-      smallDft(v, R, sign);
-      const int base = (j / Ns) * Ns * R + k;
-      for (int q = 0; q < R; ++q) out[base + q * Ns] = v[q];
+    switch (R) {
+      case 2: fftStage<2>(src, dst, lines, K, Ns, tw, sign); break;
+      case 3: fftStage<3>(src, dst, lines, K, Ns, tw, sign); break;
+      case 4: fftStage<4>(src, dst, lines, K, Ns, tw, sign); break;
+      case 5: fftStage<5>(src, dst, lines, K, Ns, tw, sign); break;
+      default: fftStage<7>(src, dst, lines, K, Ns, tw, sign); break;
     }
     __syncthreads();
     float2 *swap = src;
