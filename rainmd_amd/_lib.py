@@ -263,6 +263,7 @@ SIGNATURES = {
     "rmd_set_lanes": (C.c_int, [C.c_void_p, C.c_int32]),
     "rmd_set_graphs": (C.c_int, [C.c_void_p, C.c_int32]),
     "rmd_set_pme_pipeline": (C.c_int, [C.c_void_p, C.c_int32]),
+    "rmd_set_concurrent_forces": (C.c_int, [C.c_void_p, C.c_int32]),
     "rmd_copy_probe": (C.c_int, [C.c_void_p, C.c_int64, C.c_int32, C.POINTER(C.c_double)]),
     "rmd_surrogate_create": (C.c_int, [C.c_int32, C.c_uint32, C.POINTER(C.c_void_p)]),
     "rmd_surrogate_free": (None, [C.c_void_p]),

@@ -104,6 +104,11 @@ struct rmd_ctx {
   // device
   int device = 0;
   hipStream_t stream = nullptr;
+  // Force branches that need no neighbour list (PME, bonded) run on their own
+  // streams, forked from and joined back to `stream` around every evaluation.
+  bool concurrent = true;
+  hipStream_t side[2] = {nullptr, nullptr};
+  hipEvent_t fork = nullptr, join[2] = {nullptr, nullptr};
   std::vector<void *> held;
   Dev d{};
   PmeDev p{};
@@ -409,25 +414,43 @@ __global__ void decideRebuildKernel(Scalars *sc, float threshold2, int force) {
 __global__ void setForceRebuildKernel(Scalars *sc) { sc->forceRebuild = 1u; }
 __global__ void setStepKernel(Scalars *sc, long long step) { sc->step = step; }
 
+// One force evaluation. The neighbour-list branch (rebuild + direct space)
+// stays on the context's stream; PME and the bonded terms read only the
+// positions, so they run on side streams forked after the integrator and
+// joined before anything that reads the forces. Forces are int64 atomics and
+// every energy has its own slot row, so the results do not depend on the
+// overlap.
 void enqueueForces(rmd_ctx *c, int terms, int energies, int slots[E_TERM_COUNT]) {
   Dev &d = c->d;
-  launchRebuild(d, c->stream);
   for (int k = 0; k < E_TERM_COUNT; ++k) slots[k] = 0;
-  const float alpha = (float)c->pme.alpha;
-  if (terms & RMD_TERM_DIRECT) {
-    const int b = launchNonbonded(d, alpha, c->lanes, energies, c->stream);
-    slots[E_LJ_DIRECT] = slots[E_COUL_DIRECT] = slots[E_DSCALE_DIRECT] = b;
-  }
-  if (terms & (RMD_TERM_BONDED | RMD_TERM_EXCEPTIONS | RMD_TERM_RESTRAINT)) {
-    int b = 0;
-    launchBonded(d, c->pme.alpha, terms, energies, c->stream, &b);
-    slots[E_BOND] = slots[E_ANGLE] = slots[E_TORSION] = slots[E_LJ_EXCEPTION] = slots[E_COUL_EXCEPTION] =
-        slots[E_RESTRAINT] = b;
+  const bool fork = c->concurrent;
+  hipStream_t pmeStream = fork ? c->side[0] : c->stream, bondedStream = fork ? c->side[1] : c->stream;
+  if (fork) {
+    HIP_CHECK(hipEventRecord(c->fork, c->stream));
+    HIP_CHECK(hipStreamWaitEvent(c->side[0], c->fork, 0));
+    HIP_CHECK(hipStreamWaitEvent(c->side[1], c->fork, 0));
   }
   if (terms & RMD_TERM_RECIPROCAL) {
     int b = 0;
-    launchPme(d, c->p, energies, c->stream, &b);
+    launchPme(d, c->p, energies, pmeStream, &b);
     slots[E_RECIPROCAL] = slots[E_DSCALE_RECIPROCAL] = b;
+  }
+  if (terms & (RMD_TERM_BONDED | RMD_TERM_EXCEPTIONS | RMD_TERM_RESTRAINT)) {
+    int b = 0;
+    launchBonded(d, c->pme.alpha, terms, energies, bondedStream, &b);
+    slots[E_BOND] = slots[E_ANGLE] = slots[E_TORSION] = slots[E_LJ_EXCEPTION] = slots[E_COUL_EXCEPTION] =
+        slots[E_RESTRAINT] = b;
+  }
+  launchRebuild(d, c->stream);
+  if (terms & RMD_TERM_DIRECT) {
+    const int b = launchNonbonded(d, (float)c->pme.alpha, c->lanes, energies, c->stream);
+    slots[E_LJ_DIRECT] = slots[E_COUL_DIRECT] = slots[E_DSCALE_DIRECT] = b;
+  }
+  if (fork) {
+    for (int k = 0; k < 2; ++k) {
+      HIP_CHECK(hipEventRecord(c->join[k], c->side[k]));
+      HIP_CHECK(hipStreamWaitEvent(c->stream, c->join[k], 0));
+    }
   }
 }
 
@@ -768,6 +791,11 @@ void create(rmd_ctx *c, const System &sys, const rmd_options &o) {
     throw Error(RMD_E_DEVICE, "no HIP device is available; this library runs on an MI355X (gfx950)");
   HIP_CHECK(hipSetDevice(c->device));
   HIP_CHECK(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
+  for (int k = 0; k < 2; ++k) {
+    HIP_CHECK(hipStreamCreateWithFlags(&c->side[k], hipStreamNonBlocking));
+    HIP_CHECK(hipEventCreateWithFlags(&c->join[k], hipEventDisableTiming));
+  }
+  HIP_CHECK(hipEventCreateWithFlags(&c->fork, hipEventDisableTiming));
   {
     int lds = 0;
     if (hipDeviceGetAttribute(&lds, hipDeviceAttributeMaxSharedMemoryPerBlock, c->device) == hipSuccess && lds > 0)
@@ -1065,6 +1093,11 @@ void rmd_destroy(rmd_ctx *c) {
   if (c->screen.net) rmd_surrogate_free(c->screen.net);
   (void)hipSetDevice(c->device);
   for (void *p : c->held) (void)hipFree(p);
+  for (int k = 0; k < 2; ++k) {
+    if (c->side[k]) (void)hipStreamDestroy(c->side[k]);
+    if (c->join[k]) (void)hipEventDestroy(c->join[k]);
+  }
+  if (c->fork) (void)hipEventDestroy(c->fork);
   if (c->stream) (void)hipStreamDestroy(c->stream);
   delete c;
 }
@@ -1642,6 +1675,14 @@ int rmd_screen_samples(rmd_ctx *c, float *features, float *targets, int32_t capa
     if (features) std::memcpy(features, sc.x.data(), (size_t)n * RMD_SURROGATE_INPUTS * sizeof(float));
     if (targets) std::memcpy(targets, sc.t.data(), (size_t)n * sizeof(float));
     if (count) *count = (int32_t)sc.t.size();
+  });
+}
+
+int rmd_set_concurrent_forces(rmd_ctx *c, int32_t enabled) {
+  CTX_GUARD(c, {
+    HIP_CHECK(hipStreamSynchronize(c->stream));
+    c->concurrent = enabled != 0;
+    invalidateGraphs(c);
   });
 }
 

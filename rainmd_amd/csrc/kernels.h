@@ -34,6 +34,7 @@ struct PmeDev {
   int radices[3][8];        // factorisation per axis (0-terminated)
   double piOverAlpha2;      // pi^2 / alpha^2 (the scaling derivative of the reciprocal energy)
   int planeSpan;            // max(ceil(K1/2) * K2, K2h * K1): one plane's complex work buffer
+  int planeRegion;          // float2 slots of the second buffer: max(planeSpan, the int64 plane)
   int planes;               // 1: plane pipeline, 0: line pipeline
 };
 enum { PME_AUTO = 0, PME_PLANES = 1, PME_LINES = 2 };

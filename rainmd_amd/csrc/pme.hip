@@ -31,7 +31,6 @@ constexpr int PLANE_THREADS = 512;
 constexpr int XLINE_THREADS = 256;
 constexpr int XLINES = 8;           // x lines per workgroup in the plane pipeline
 constexpr int LINES_PER_BLOCK = 4;  // lines per workgroup in the line pipeline
-constexpr int ATOM_BATCH = 4;       // atom positions loaded together in the plane scans
 
 __device__ __forceinline__ float2 cmul(float2 a, float2 b) { return make_float2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x); }
 __device__ __forceinline__ float2 cadd(float2 a, float2 b) { return make_float2(a.x + b.x, a.y + b.y); }
@@ -227,9 +226,17 @@ __device__ __forceinline__ int planeWeight(int b, int plane, int K) {
   return r;
 }
 
+// round-to-nearest-even of v to an integer, exact for |v| < 2^51: adding
+// 1.5 * 2^52 puts v's integer part in the low mantissa bits.
+__device__ __forceinline__ long long roundToInt64(double v) {
+  const double magic = 6755399441055744.0;
+  return (long long)(__double_as_longlong(v + magic) - __double_as_longlong(magic));
+}
+
+// |q w w w| <= |q| < 2^19 here, far inside the exact range.
 __device__ __forceinline__ long long chargeToFixed(float q, float wx, float wy, float wz) {
   const float qab = (q * wx) * wy;
-  return __double2ll_rn((double)(qab * wz) * GRID_SCALE);
+  return roundToInt64((double)(qab * wz) * GRID_SCALE);
 }
 
 // Force on one atom from the derivatives of its fractional-coordinate
@@ -283,6 +290,44 @@ __device__ void storeReciprocal(const Dev &d, double energy, double dscale) {
 // Plane pipeline
 // ============================================================================
 
+// One round of a plane's atom scan: of atoms [a0, a0 + PLANE_LIST), those
+// whose x support meets `plane` are appended to `list` as (atom << 3 | weight
+// index), wave-aggregated into the shared `count`. The expensive per-atom work
+// then runs over the compacted list with every lane busy (about 5 in K0 atoms
+// land on a given plane).
+constexpr int SCAN_PER_THREAD = 4;
+constexpr int PLANE_LIST = PLANE_THREADS * SCAN_PER_THREAD;
+
+__device__ void planeScan(const Dev &d, const PmeDev &p, int plane, int a0, int *list, int *count) {
+  const double *recip = d.box->recip64;
+  const int lane = threadIdx.x & 63;
+  float4 batch[SCAN_PER_THREAD];
+#pragma unroll
+  for (int u = 0; u < SCAN_PER_THREAD; ++u) {
+    const int a = a0 + u * blockDim.x + threadIdx.x;
+    batch[u] = a < d.n ? d.pos[a] : make_float4(0.f, 0.f, 0.f, 0.f);
+  }
+#pragma unroll
+  for (int u = 0; u < SCAN_PER_THREAD; ++u) {
+    const int a = a0 + u * blockDim.x + threadIdx.x;
+    int r = ORDER;
+    if (a < d.n && batch[u].w != 0.f) {
+      int bx;
+      float ox;
+      gridCoordinate(p, recip, 0, batch[u], bx, ox);
+      r = planeWeight(bx, plane, p.K[0]);
+    }
+    const bool take = r < ORDER;
+    const unsigned long long mask = __ballot(take);
+    int base = 0;
+    if (lane == (int)(__ffsll((long long)mask) - 1)) base = atomicAdd(count, __popcll(mask));
+    base = __shfl(base, __ffsll((long long)mask) - 1, 64);
+    if (take) list[base + __popcll(mask & ((1ull << lane) - 1ull))] = (a << 3) | r;
+  }
+}
+
+
+
 // 1. Spread onto plane i and forward z, y transforms of the plane. With
 // `real` set, the plane is read from a real K0*K1*K2 grid instead (the FFT
 // verification entry).
@@ -295,6 +340,8 @@ __global__ void __launch_bounds__(PLANE_THREADS) pmePlaneForwardKernel(Dev d, Pm
   float2 *tw1 = lds, *tw2 = tw1 + K1;
   float2 *A = tw2 + K2, *B = A + S;
   long long *plane = reinterpret_cast<long long *>(B);
+  int *list = reinterpret_cast<int *>(B + p.planeRegion);
+  int *listCount = list + PLANE_LIST;
   loadTwiddles(tw1, p.twiddle[1], K1);
   loadTwiddles(tw2, p.twiddle[2], K2);
   if (real) {
@@ -305,40 +352,36 @@ __global__ void __launch_bounds__(PLANE_THREADS) pmePlaneForwardKernel(Dev d, Pm
     }
   } else {
     for (int idx = t; idx < K1 * K2; idx += nt) plane[idx] = 0;
-    __syncthreads();
     const double *recip = d.box->recip64;
-    for (int a0 = t; a0 < d.n; a0 += nt * ATOM_BATCH) {
-      float4 batch[ATOM_BATCH];
-#pragma unroll
-      for (int u = 0; u < ATOM_BATCH; ++u) {
-        const int a = a0 + u * nt;
-        batch[u] = a < d.n ? d.pos[a] : make_float4(0.f, 0.f, 0.f, 0.f);
+    for (int a0 = 0; a0 < d.n; a0 += PLANE_LIST) {
+      if (t == 0) *listCount = 0;
+      __syncthreads();
+      planeScan(d, p, i, a0, list, listCount);
+      __syncthreads();
+      const int count = *listCount;
+      for (int k = t; k < count; k += nt) {
+        const int a = list[k] >> 3, r = list[k] & 7;
+        const float4 pos = d.pos[a];
+        int bx, by, bz;
+        float ox, oy, oz;
+        gridCoordinate(p, recip, 0, pos, bx, ox);
+        float w[ORDER], wy[ORDER], wz[ORDER], dw[ORDER];
+        bspline5(ox, w, dw);
+        float wx = 0.f;
+        for (int q = r; q < ORDER; q += K0) wx += w[q];
+        gridCoordinate(p, recip, 1, pos, by, oy);
+        gridCoordinate(p, recip, 2, pos, bz, oz);
+        bspline5(oy, wy, dw);
+        bspline5(oz, wz, dw);
+        int iz[ORDER];
+        for (int c = 0; c < ORDER; ++c) iz[c] = gridIndex(bz, c, K2);
+        for (int b = 0; b < ORDER; ++b) {
+          unsigned long long *row = reinterpret_cast<unsigned long long *>(plane + gridIndex(by, b, K1) * K2);
+          for (int c = 0; c < ORDER; ++c)
+            atomicAdd(row + iz[c], (unsigned long long)chargeToFixed(pos.w, wx, wy[b], wz[c]));
+        }
       }
-#pragma unroll
-      for (int u = 0; u < ATOM_BATCH; ++u) {
-      const float4 pos = batch[u];
-      if (pos.w == 0.f) continue;
-      int bx, by, bz;
-      float ox, oy, oz;
-      gridCoordinate(p, recip, 0, pos, bx, ox);
-      const int r = planeWeight(bx, i, K0);
-      if (r >= ORDER) continue;
-      float w[ORDER], wy[ORDER], wz[ORDER], dw[ORDER];
-      bspline5(ox, w, dw);
-      float wx = 0.f;
-      for (int k = r; k < ORDER; k += K0) wx += w[k];
-      gridCoordinate(p, recip, 1, pos, by, oy);
-      gridCoordinate(p, recip, 2, pos, bz, oz);
-      bspline5(oy, wy, dw);
-      bspline5(oz, wz, dw);
-      int iz[ORDER];
-      for (int c = 0; c < ORDER; ++c) iz[c] = gridIndex(bz, c, K2);
-      for (int b = 0; b < ORDER; ++b) {
-        unsigned long long *row = reinterpret_cast<unsigned long long *>(plane + gridIndex(by, b, K1) * K2);
-        for (int c = 0; c < ORDER; ++c)
-          atomicAdd(row + iz[c], (unsigned long long)chargeToFixed(pos.w, wx, wy[b], wz[c]));
-      }
-      }
+      __syncthreads();
     }
     __syncthreads();
     // Two real z lines per complex line.
@@ -428,6 +471,8 @@ __global__ void __launch_bounds__(PLANE_THREADS) pmePlaneGatherKernel(Dev d, Pme
   const int t = threadIdx.x, nt = blockDim.x;
   float2 *tw1 = lds, *tw2 = tw1 + K1;
   float2 *A = tw2 + K2, *B = A + S;
+  int *list = reinterpret_cast<int *>(B + S);
+  int *listCount = list + PLANE_LIST;
   loadTwiddles(tw1, p.twiddle[1], K1);
   loadTwiddles(tw2, p.twiddle[2], K2);
   const float2 *in = p.cgrid + (size_t)i * K1 * K2h;
@@ -466,51 +511,47 @@ __global__ void __launch_bounds__(PLANE_THREADS) pmePlaneGatherKernel(Dev d, Pme
   }
   __syncthreads();
   const double *recip = d.box->recip64;
-  for (int a0 = t; a0 < d.n; a0 += nt * ATOM_BATCH) {
-    float4 batch[ATOM_BATCH];
-#pragma unroll
-    for (int u = 0; u < ATOM_BATCH; ++u) {
-      const int a = a0 + u * nt;
-      batch[u] = a < d.n ? d.pos[a] : make_float4(0.f, 0.f, 0.f, 0.f);
-    }
-#pragma unroll
-    for (int u = 0; u < ATOM_BATCH; ++u) {
-    const int a = a0 + u * nt;
-    const float4 pos = batch[u];
-    if (pos.w == 0.f) continue;
-    int bx, by, bz;
-    float ox, oy, oz;
-    gridCoordinate(p, recip, 0, pos, bx, ox);
-    const int r = planeWeight(bx, i, K0);
-    if (r >= ORDER) continue;
-    float w[ORDER], dw[ORDER], wy[ORDER], dwy[ORDER], wz[ORDER], dwz[ORDER];
-    bspline5(ox, w, dw);
-    float wx = 0.f, dwx = 0.f;
-    for (int k = r; k < ORDER; k += K0) {
-      wx += w[k];
-      dwx += dw[k];
-    }
-    gridCoordinate(p, recip, 1, pos, by, oy);
-    gridCoordinate(p, recip, 2, pos, bz, oz);
-    bspline5(oy, wy, dwy);
-    bspline5(oz, wz, dwz);
-    int iz[ORDER];
-    for (int c = 0; c < ORDER; ++c) iz[c] = gridIndex(bz, c, K2);
-    float s0 = 0.f, s1 = 0.f, s2 = 0.f;
-    for (int b = 0; b < ORDER; ++b) {
-      const float *row = phi + gridIndex(by, b, K1) * K2;
-      float r0 = 0.f, r2 = 0.f;
-      for (int c = 0; c < ORDER; ++c) {
-        const float v = row[iz[c]];
-        r0 += wz[c] * v;
-        r2 += dwz[c] * v;
+  for (int a0 = 0; a0 < d.n; a0 += PLANE_LIST) {
+    if (t == 0) *listCount = 0;
+    __syncthreads();
+    planeScan(d, p, i, a0, list, listCount);
+    __syncthreads();
+    const int count = *listCount;
+    for (int k = t; k < count; k += nt) {
+      const int a = list[k] >> 3, r = list[k] & 7;
+      const float4 pos = d.pos[a];
+      int bx, by, bz;
+      float ox, oy, oz;
+      gridCoordinate(p, recip, 0, pos, bx, ox);
+      float w[ORDER], dw[ORDER], wy[ORDER], dwy[ORDER], wz[ORDER], dwz[ORDER];
+      bspline5(ox, w, dw);
+      float wx = 0.f, dwx = 0.f;
+      for (int q = r; q < ORDER; q += K0) {
+        wx += w[q];
+        dwx += dw[q];
       }
-      s0 += wy[b] * r0;
-      s1 += dwy[b] * r0;
-      s2 += wy[b] * r2;
+      gridCoordinate(p, recip, 1, pos, by, oy);
+      gridCoordinate(p, recip, 2, pos, bz, oz);
+      bspline5(oy, wy, dwy);
+      bspline5(oz, wz, dwz);
+      int iz[ORDER];
+      for (int c = 0; c < ORDER; ++c) iz[c] = gridIndex(bz, c, K2);
+      float s0 = 0.f, s1 = 0.f, s2 = 0.f;
+      for (int b = 0; b < ORDER; ++b) {
+        const float *row = phi + gridIndex(by, b, K1) * K2;
+        float r0 = 0.f, r2 = 0.f;
+        for (int c = 0; c < ORDER; ++c) {
+          const float v = row[iz[c]];
+          r0 += wz[c] * v;
+          r2 += dwz[c] * v;
+        }
+        s0 += wy[b] * r0;
+        s1 += dwy[b] * r0;
+        s2 += wy[b] * r2;
+      }
+      addGridForce(d, p, a, pos.w, dwx * s0, wx * s1, wx * s2);
     }
-    addGridForce(d, p, a, pos.w, dwx * s0, wx * s1, wx * s2);
-    }
+    __syncthreads();
   }
 }
 
@@ -754,13 +795,14 @@ void pmeComputeInfluence(const PmeDev &p, const BoxF *box, const double *moduli[
 
 // ---- configuration and launchers --------------------------------------------
 
+// LDS: twiddles, two complex work buffers (the second also holds the int64
+// charge plane), then the compacted atom list and its count.
+static size_t listBytes() { return (size_t)(PLANE_LIST + 4) * sizeof(int); }
 static size_t planeForwardLds(const PmeDev &p) {
-  const size_t S = (size_t)p.planeSpan * sizeof(float2);
-  const size_t plane = (size_t)p.K[1] * p.K[2] * sizeof(long long);
-  return (size_t)(p.K[1] + p.K[2]) * sizeof(float2) + S + (S > plane ? S : plane);
+  return (size_t)(p.K[1] + p.K[2] + p.planeSpan + p.planeRegion) * sizeof(float2) + listBytes();
 }
 static size_t planeGatherLds(const PmeDev &p) {
-  return (size_t)(p.K[1] + p.K[2]) * sizeof(float2) + 2 * (size_t)p.planeSpan * sizeof(float2);
+  return (size_t)(p.K[1] + p.K[2]) * sizeof(float2) + 2 * (size_t)p.planeSpan * sizeof(float2) + listBytes();
 }
 static size_t xLinesLds(const PmeDev &p) { return (size_t)p.K[0] * sizeof(float2) * (1 + 2 * XLINES); }
 static size_t lineLds(int K) { return (size_t)K * sizeof(float2) * (1 + 2 * LINES_PER_BLOCK); }
@@ -769,6 +811,8 @@ bool pmeConfigure(PmeDev &p, int mode, size_t maxLds) {
   const int P = (p.K[1] + 1) / 2;
   const int a = P * p.K[2], b = p.K2h * p.K[1];
   p.planeSpan = a > b ? a : b;
+  const int planeWords = (p.K[1] * p.K[2] * (int)sizeof(long long) + (int)sizeof(float2) - 1) / (int)sizeof(float2);
+  p.planeRegion = p.planeSpan > planeWords ? p.planeSpan : planeWords;
   const size_t need = std::max(planeForwardLds(p), planeGatherLds(p));
   const bool fits = need <= maxLds && xLinesLds(p) <= maxLds;
   p.planes = (mode == PME_PLANES || mode == PME_AUTO) && fits ? 1 : 0;

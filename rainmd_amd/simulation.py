@@ -490,6 +490,9 @@ class GpuSimulation:
         _check(L.load().rmd_copy_probe(self._h, int(nbytes), int(reps), C.byref(ms)), self._h)
         return ms.value
 
+    def set_concurrent_forces(self, enabled):
+        _check(L.load().rmd_set_concurrent_forces(self._h, 1 if enabled else 0), self._h)
+
     def set_pme_pipeline(self, mode):
         """'auto', 'planes' or 'lines'."""
         ids = {"auto": 0, "planes": 1, "lines": 2}

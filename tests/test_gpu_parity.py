@@ -152,14 +152,21 @@ def test_energies_and_forces_workloads(workload):
     assert frel < FORCE_BAR
 
 
-def test_steps_are_deterministic(ala_wat):
+@pytest.mark.parametrize("concurrent, graphs", [(True, True), (False, True), (True, False)])
+def test_steps_are_deterministic(ala_wat, concurrent, graphs):
+    # the same 20 steps twice, and against the default mode (concurrent force
+    # branches in captured graphs): bit-identical states and energies
     system, _ = ala_wat
     runs = []
-    for _ in range(2):
+    for mode in [(True, True), (concurrent, graphs)]:
         sim = R.GpuSimulation(system, alpha=ALPHA_24, grid=[24, 24, 24], seed=5, temperature=298.0)
+        sim.set_concurrent_forces(mode[0])
+        sim.set_graphs(mode[1])
         sim.run(20)
         st = sim.read_state()
-        runs.append((st["positions"].copy(), st["velocities"].copy(), st["forces"].copy()))
+        e = sim.energy()
+        runs.append((st["positions"].copy(), st["velocities"].copy(), st["forces"].copy(),
+                     np.array([e["potential"], e["kinetic"], e["coulomb_reciprocal"]])))
         sim.release()
     for a, b in zip(runs[0], runs[1]):
         assert np.array_equal(a, b)
