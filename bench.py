@@ -75,6 +75,19 @@ def in_cutoff_pairs(positions, box, cutoff, excl_start, excl_partners):
     return unique - excluded_inside
 
 
+def counted_traffic(workload, lanes):
+    """HBM bytes per nonbonded launch from the committed hardware-counter
+    summary (FETCH_SIZE and WRITE_SIZE scaled by a plain copy's), when it
+    covers this workload and lane count; else None."""
+    path = os.path.join(ROOT, "profiles", f"nonbonded_{workload.replace('-', '')}_traffic.md")
+    if not os.path.exists(path) or lanes != 64:
+        return None, None
+    for line in open(path):
+        if line.startswith("traffic_bytes_per_launch:"):
+            return float(line.split(":")[1]), os.path.relpath(path, ROOT)
+    return None, None
+
+
 def cpu_baseline(system, pme, state, target_seconds):
     """The FP64 CPU port of the reference's host path (oracle/), timed on one
     core: BAOAB steps with an all-pairs direct space and a naive-DFT PME. Its
@@ -182,6 +195,7 @@ def main():
     pairs = in_cutoff_pairs(state["positions"], state["box"], system.cutoff, system.exclusion_start,
                             system.exclusion_partners)
     achieved = FLOP_PER_PAIR * pairs / (nb_ms * 1e-3) / 1e12
+    traffic, traffic_source = counted_traffic(args.workload, diag["lanes_per_atom"])
     kernels = {k: sim.profile_kernel(k, reps=100) for k in ("pme", "integrate", "bonded", "rebuild")}
     log(f"nonbonded {nb_ms * 1e3:.2f} us/launch over {pairs} pairs -> {achieved:.3f} TFLOP/s; "
         + ", ".join(f"{k} {v * 1e3:.2f} us" for k, v in kernels.items()))
@@ -217,7 +231,8 @@ def main():
             "peak": FP32_PEAK_TFLOPS,
             "unit": "TFLOP/s",
             "frac": achieved / FP32_PEAK_TFLOPS,
-            "traffic": None,
+            "traffic": traffic,
+            "traffic_source": traffic_source,
             "launch_us": nb_ms * 1e3,
             "pairs": pairs,
             "flop_per_pair": FLOP_PER_PAIR,

@@ -91,7 +91,7 @@ struct rmd_ctx {
   int barostatInterval = 25;
   bool dispersionOn = true;
   double skin = 0, slack = 2.0;
-  int lanes = 32;
+  int lanes = 64;
   size_t maxLds = 65536;
   CellPlan plan;
   rmd_pme_parameters pme{};
@@ -771,7 +771,7 @@ void create(rmd_ctx *c, const System &sys, const rmd_options &o) {
   c->barostatInterval = o.barostat_interval > 0 ? o.barostat_interval : 25;
   c->dispersionOn = o.dispersion_correction != 0;
   c->slack = o.neighbour_slack > 0 ? o.neighbour_slack : 2.0;
-  c->lanes = o.lanes_per_atom > 0 ? o.lanes_per_atom : 32;
+  c->lanes = o.lanes_per_atom > 0 ? o.lanes_per_atom : 64;
   if (c->lanes & (c->lanes - 1) || c->lanes > 64) throw Error(RMD_E_INVALID, "lanes_per_atom must be a power of two <= 64");
   for (int k = 0; k < 9; ++k) c->box[k] = sys.box[k];
   c->skin = o.skin > 0 ? o.skin : skinFor(sys, c->timestep, c->temperature, nullptr);
