@@ -74,14 +74,18 @@ __device__ __forceinline__ unsigned int mix(unsigned int v) {
   v ^= v >> 16;
   return v;
 }
-__device__ __forceinline__ double uniformAt(unsigned int seed, unsigned int a, unsigned int b, unsigned int c) {
+// ((h >> 9) + 0.5) * 2^-23: 24 significant bits at most, so exact in FP32.
+__device__ __forceinline__ float uniformAt(unsigned int seed, unsigned int a, unsigned int b, unsigned int c) {
   unsigned int st = mix(seed ^ (a * 0x9e3779b1u));
   st = mix(st ^ (b * 0x85ebca6bu));
   st = mix(st ^ (c * 0xc2b2ae35u));
-  return ((double)(st >> 9) + 0.5) * (1.0 / 8388608.0);
+  return ((float)(st >> 9) + 0.5f) * (1.0f / 8388608.0f);
 }
 
-// O: v <- a v + sqrt(1 - a^2) sqrt(kT/m) xi (integrate.js:406-437).
+// O: v <- a v + sqrt(1 - a^2) sqrt(kT/m) xi (integrate.js:406-437). The
+// three normals come from four counter-hash uniforms by Box-Muller
+// (shared/random.js:66-87) in FP32 - the noise is a random draw, and FP64 is
+// kept for the velocity update itself.
 __device__ __forceinline__ void thermostat(const Dev &d, const IntegratorParams &ip, int a, Atom &s,
                                            unsigned int step) {
   if (s.im <= 0) return;
@@ -89,12 +93,15 @@ __device__ __forceinline__ void thermostat(const Dev &d, const IntegratorParams 
   if (ip.suppliedNoise) {
     for (int c = 0; c < 3; ++c) g[c] = d.noise[3 * a + c];
   } else {
-    const double u1 = uniformAt(ip.seed, a, step, 0), u2 = uniformAt(ip.seed, a, step, 1);
-    const double u3 = uniformAt(ip.seed, a, step, 2), u4 = uniformAt(ip.seed, a, step, 3);
-    const double r1 = sqrt(-2.0 * log(u1)), r2 = sqrt(-2.0 * log(u3));
-    g[0] = r1 * cos(2.0 * M_PI * u2);
-    g[1] = r1 * sin(2.0 * M_PI * u2);
-    g[2] = r2 * cos(2.0 * M_PI * u4);
+    const float u1 = uniformAt(ip.seed, a, step, 0), u2 = uniformAt(ip.seed, a, step, 1);
+    const float u3 = uniformAt(ip.seed, a, step, 2), u4 = uniformAt(ip.seed, a, step, 3);
+    const float r1 = sqrtf(-2.0f * logf(u1)), r2 = sqrtf(-2.0f * logf(u3));
+    float s1, c1, s2, c2;
+    sincospif(2.0f * u2, &s1, &c1);
+    sincospif(2.0f * u4, &s2, &c2);
+    g[0] = r1 * c1;
+    g[1] = r1 * s1;
+    g[2] = r2 * c2;
   }
   const double amp = ip.noiseScale * sqrt(ip.kT * s.im);
   for (int c = 0; c < 3; ++c) s.v[c] = ip.decay * s.v[c] + amp * g[c];
