@@ -119,6 +119,18 @@ int rmd_system_build_solvated(const char *template_prmtop, size_t template_lengt
                               const char *solute_inpcrd, size_t solute_length,
                               const rmd_solvated_options *options, rmd_system **out);
 
+/* The system a later protocol stage runs: the same topology in the live box
+ * a barostat left behind, as the page rebuilds it with buildSystem(...,
+ * {boxVectors: currentBoxVectors()}) before each stage (web/app.js:354-364,
+ * 1103-1124). box is reduced and row-major; the cutoff must stay within half
+ * of its smallest width, as in buildSystem (shared/topology/system.js:80-87). */
+int rmd_system_rebox(const rmd_system *system, const double box[9], rmd_system **out);
+/* soluteHeavyAtoms (shared/topology/restraints.js:46-49): atoms whose residue
+ * is neither water nor an ion (the residue classes of
+ * shared/analysis/selection.js:9-24,70-77) and whose atomic number is not 1,
+ * ascending. Writes at most `capacity` indices; *count is the full count. */
+int rmd_system_solute_heavy_atoms(const rmd_system *system, int32_t *out, int32_t capacity, int32_t *count);
+
 /* A flat, read-only view of a built system: every array the kernels and the
  * oracle read. Pointers stay valid until rmd_system_free. */
 typedef struct rmd_system_view {

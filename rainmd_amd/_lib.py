@@ -233,6 +233,8 @@ SIGNATURES = {
     "rmd_system_build_solvated": (C.c_int, [C.c_char_p, C.c_size_t, C.c_char_p, C.c_size_t,
                                             C.POINTER(SolvatedOptions), C.POINTER(C.c_void_p)]),
     "rmd_system_view_get": (C.c_int, [C.c_void_p, C.POINTER(SystemView)]),
+    "rmd_system_rebox": (C.c_int, [C.c_void_p, _dp, C.POINTER(C.c_void_p)]),
+    "rmd_system_solute_heavy_atoms": (C.c_int, [C.c_void_p, _ip, C.c_int32, C.POINTER(C.c_int32)]),
     "rmd_pme_parameters_derive": (C.c_int, [_dp, C.c_double, C.c_double, C.c_double, _ip,
                                             C.POINTER(PmeParameters)]),
     "rmd_pme_moduli": (C.c_int, [C.c_int32, C.c_int32, _dp]),

@@ -833,6 +833,62 @@ int rmd_pme_moduli(int32_t gridSize, int32_t order, double *moduli) {
   });
 }
 
+namespace {
+// The selection's residue classes (shared/analysis/selection.js:9-24,70-77):
+// a residue is solute unless its trimmed, upper-cased name is a water or an
+// ion. The selection's water set is wider than buildSystem's rigid-water set
+// by T3P.
+bool isSoluteResidue(std::string name) {
+  static const std::set<std::string> solvent = {
+      "WAT", "HOH", "H2O", "SOL", "TIP", "TP3", "SPC", "T3P",                                   // water
+      "NA",  "NA+", "CL",  "CL-", "K",   "K+",  "MG",  "MG2", "CA", "CA2", "ZN", "ZN2", "LI",   // ions
+      "LI+", "RB",  "RB+", "CS",  "CS+", "F",   "F-",  "BR",  "BR-", "I",  "I-", "IB",  "CIO",
+      "MN",  "FE",  "FE2", "CU",  "CU1"};
+  const size_t first = name.find_first_not_of(" \t");
+  if (first == std::string::npos) return true;
+  name = name.substr(first, name.find_last_not_of(" \t") + 1 - first);
+  for (char &ch : name) ch = (char)std::toupper((unsigned char)ch);
+  return solvent.count(name) == 0;
+}
+}  // namespace
+
+int rmd_system_rebox(const rmd_system *sys, const double box[9], rmd_system **out) {
+  RMD_GUARD({
+    if (!sys || !box || !out) throw Error(RMD_E_INVALID, "null argument");
+    if (!sys->s.isPeriodic) throw Error(RMD_E_SYSTEM, "a non-periodic system has no box to replace");
+    for (int k = 0; k < 9; ++k)
+      if (!std::isfinite(box[k])) throw Error(RMD_E_INVALID, "box vectors must be finite");
+    if (box[1] != 0 || box[2] != 0 || box[5] != 0 || !(box[0] > 0) || !(box[4] > 0) || !(box[8] > 0))
+      throw Error(RMD_E_INVALID, "box vectors must be reduced: a along x, b in the xy plane, positive diagonal");
+    const double smallest = std::min({box[0], box[4], box[8]});
+    if (sys->s.cutoff > 0.5 * smallest) {
+      char buf[256];
+      std::snprintf(buf, sizeof buf,
+                    "a %g nm cutoff does not fit in a box whose smallest width is %.4f nm: the cutoff "
+                    "must be at most half of it (%.4f nm).",
+                    sys->s.cutoff, smallest, 0.5 * smallest);
+      throw Error(RMD_E_SYSTEM, buf);
+    }
+    rmd_system *copy = new rmd_system{sys->s};
+    std::memcpy(copy->s.box, box, sizeof copy->s.box);
+    *out = copy;
+  });
+}
+
+int rmd_system_solute_heavy_atoms(const rmd_system *sys, int32_t *out, int32_t capacity, int32_t *count) {
+  RMD_GUARD({
+    if (!sys || !count || (capacity > 0 && !out)) throw Error(RMD_E_INVALID, "null argument");
+    const Topology &t = sys->s.topology;
+    int32_t n = 0;
+    for (int a = 0; a < t.atomCount; ++a) {
+      if (!isSoluteResidue(t.residueNames[t.residueOfAtom[a]]) || t.atomicNumbers[a] == 1) continue;
+      if (n < capacity) out[n] = a;
+      ++n;
+    }
+    *count = n;
+  });
+}
+
 double rmd_skin_for(const rmd_system *s, double timestep, double temperature) {
   try {
     return skinFor(s->s, timestep > 0 ? timestep : 0.002, temperature > 0 ? temperature : 298.0, nullptr);

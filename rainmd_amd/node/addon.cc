@@ -410,6 +410,34 @@ napi_value BuildSolvated(napi_env env, napi_callback_info info) {
   return wrap(env, s);
 }
 
+// The same system in the live box of a finished stage (web/app.js:354-364).
+napi_value ReboxSystem(napi_env env, napi_callback_info info) {
+  napi_value argv[2];
+  args(env, info, 2, argv);
+  rmd_system *sys = unwrap<rmd_system>(env, argv[0]);
+  std::vector<double> box;
+  if (!getDoubles(env, argv[1], box) || box.size() != 9) {
+    napi_throw_type_error(env, nullptr, "reboxSystem expects boxVectors[9]");
+    return nullptr;
+  }
+  rmd_system *s = nullptr;
+  if (!check(env, rmd_system_rebox(sys, box.data(), &s), nullptr)) return nullptr;
+  return wrap(env, s);
+}
+
+// soluteHeavyAtoms (shared/topology/restraints.js:46-49).
+napi_value SoluteHeavyAtoms(napi_env env, napi_callback_info info) {
+  napi_value argv[1];
+  args(env, info, 1, argv);
+  rmd_system *sys = unwrap<rmd_system>(env, argv[0]);
+  int32_t count = 0;
+  if (!check(env, rmd_system_solute_heavy_atoms(sys, nullptr, 0, &count), nullptr)) return nullptr;
+  std::vector<int32_t> atoms(count > 0 ? count : 1);
+  if (!check(env, rmd_system_solute_heavy_atoms(sys, atoms.data(), (int32_t)atoms.size(), &count), nullptr))
+    return nullptr;
+  return i32(env, atoms.data(), (size_t)count);
+}
+
 napi_value SystemView(napi_env env, napi_callback_info info) {
   napi_value argv[1];
   args(env, info, 1, argv);
@@ -900,6 +928,8 @@ napi_value Init(napi_env env, napi_value exports) {
       {"buildSystem", BuildSystem},
       {"buildSolvated", BuildSolvated},
       {"systemView", SystemView},
+      {"reboxSystem", ReboxSystem},
+      {"soluteHeavyAtoms", SoluteHeavyAtoms},
       {"pmeParameters", PmeParameters},
       {"create", Create},
       {"destroy", Destroy},

@@ -313,10 +313,23 @@ class GpuSimulation {
   }
 }
 
+// The system in the live box a stage left behind: the page rebuilds its
+// system with the current box vectors before every stage (web/app.js:354-364).
+function reboxSystem(system, boxVectors) {
+  return wrapSystem(call(() => addon.reboxSystem(system.handle, Array.from(boxVectors))));
+}
+
+// soluteHeavyAtoms(system, selections) (shared/topology/restraints.js:46-49):
+// non-water, non-ion residues, atomic number other than 1.
+function soluteHeavyAtoms(system) {
+  return call(() => addon.soluteHeavyAtoms(system.handle));
+}
+
 module.exports = {
   GpuSimulation, StopSignal,
   NeighbourOverflowError, NonFiniteStateError, SimulationStopped, GpuBudgetError, ConstraintFailure,
   parsePrmtop, parseInpcrd, boxVectorsFromLengthsAndAngles, buildSystem, buildSolvated, pmeParameters,
+  reboxSystem, soluteHeavyAtoms,
   uniformAt, maxwellBoltzmann, WORKLOADS, workloadSystem,
   deviceInfo: (device) => addon.deviceInfo(device || 0),
   abiVersion: () => addon.abiVersion(),

@@ -256,6 +256,24 @@ class System:
         _check(lib.rmd_system_build_solvated(templ, len(templ), solute, len(solute), C.byref(o), C.byref(h)))
         return cls(h)
 
+    def rebox(self, box_vectors):
+        """The same system in another (reduced, row-major) box: the live box a
+        barostat stage leaves for the next (web/app.js:354-364)."""
+        h = C.c_void_p()
+        box = (C.c_double * 9)(*[float(x) for x in np.asarray(box_vectors).reshape(9)])
+        _check(L.load().rmd_system_rebox(self._h, box, C.byref(h)))
+        return System(h)
+
+    def solute_heavy_atoms(self):
+        """soluteHeavyAtoms (shared/topology/restraints.js:46-49)."""
+        lib = L.load()
+        count = C.c_int32()
+        _check(lib.rmd_system_solute_heavy_atoms(self._h, None, 0, C.byref(count)))
+        out = np.zeros(max(count.value, 1), np.int32)
+        _check(lib.rmd_system_solute_heavy_atoms(self._h, out.ctypes.data_as(C.POINTER(C.c_int32)), len(out),
+                                                 C.byref(count)))
+        return out[:count.value].copy()
+
     def skin_for(self, timestep=0.002, temperature=298.0):
         return L.load().rmd_skin_for(self._h, timestep, temperature)
 

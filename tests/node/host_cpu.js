@@ -30,6 +30,14 @@ if (!out.device.ok) {
   out.createWithoutDevice = code(() => new rmd.GpuSimulation(0, wat, {}));
 }
 
+// the protocol's helpers: the restrained selection and the live-box rebuild
+out.soluteHeavy = Array.from(rmd.soluteHeavyAtoms(wat));
+const grown = Array.from(wat.boxVectors).map((v, k) => (k % 4 === 0 ? v * 1.02 : v));
+const reboxed = rmd.reboxSystem(wat, grown);
+out.rebox = { box: Array.from(reboxed.boxVectors), atoms: reboxed.atomCount, constraints: reboxed.constraintCount,
+  original: Array.from(wat.boxVectors) };
+out.refuseRebox = code(() => rmd.reboxSystem(wat, [1.9, 0, 0, 0, 3, 0, 0, 0, 3]));
+
 // DCD layout (test/physics.test.js:253-287)
 const dcd = new traj.DcdWriter({ atomCount: 3, timestep: 0.002, interval: 5, periodic: true });
 const xyz = Float64Array.from([0, 0, 0, 0.1, 0.2, 0.3, -0.4, 0.5, -0.6]);

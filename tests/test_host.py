@@ -146,3 +146,29 @@ def test_solvated_builder_is_seeded_and_clean():
     assert abs(density - 1.0) < 0.01
     c = R.System.solvated(389, 2.2816, seed=102)
     assert not np.array_equal(a.positions, c.positions)
+
+
+def test_solute_heavy_atoms_are_the_restraint_fixture_selection(golden):
+    # ala-wat-restraint-500.json restrains soluteHeavyAtoms (shared/topology/restraints.js:46-49)
+    fixture = golden("ala-wat-restraint-500.json")["restraint"]
+    s = R.System.build(R.Topology.load("ala-wat.prmtop"), R.Coordinates.load("ala-wat.inpcrd", 1190),
+                       constraints="HBonds", cutoff=1.0)
+    assert s.solute_heavy_atoms().tolist() == fixture["atoms"]
+    # a pure water box has no solute
+    water = R.System.solvated(64, 1.9, 3, include_solute=False, cutoff=0.9)
+    assert water.solute_heavy_atoms().size == 0
+
+
+def test_rebox_keeps_the_topology_and_checks_the_cutoff():
+    s = R.System.build(R.Topology.load("ala-wat.prmtop"), R.Coordinates.load("ala-wat.inpcrd", 1190),
+                       constraints="HBonds", cutoff=1.0)
+    box = s.box.copy()
+    box[[0, 4, 8]] *= 1.013
+    t = s.rebox(box)
+    assert np.array_equal(t.box, box)
+    assert t.atom_count == s.atom_count and np.array_equal(t.constraint_a, s.constraint_a)
+    assert np.array_equal(t.exclusion_partners, s.exclusion_partners)
+    with pytest.raises(R.SystemError_, match="cutoff"):
+        s.rebox(np.diag([1.9, 3.0, 3.0]).reshape(9))
+    with pytest.raises(R.RainmdError):
+        s.rebox([3.0, 0.1, 0, 0, 3.0, 0, 0, 0, 3.0])

@@ -58,6 +58,15 @@ def test_refusals_carry_the_reference_codes(cpu):
         assert cpu["createWithoutDevice"]["code"] == -4
 
 
+def test_protocol_helpers(cpu, golden):
+    # soluteHeavyAtoms is the selection the restraint fixture restrains
+    assert cpu["soluteHeavy"] == golden("ala-wat-restraint-500.json")["restraint"]["atoms"]
+    r = cpu["rebox"]
+    assert r["atoms"] == 1190 and r["constraints"] == 1176
+    assert np.allclose(np.array(r["box"])[[0, 4, 8]], 1.02 * np.array(r["original"])[[0, 4, 8]], rtol=0, atol=1e-15)
+    assert cpu["refuseRebox"]["code"] == -3 and "cutoff" in cpu["refuseRebox"]["message"]
+
+
 def test_dcd_layout_matches_openmm(cpu):
     d = cpu["dcd"]
     assert (d["marker"], d["cord"], d["frames"], d["interval"], d["last"]) == (84, "CORD", 2, 5, 5)
@@ -131,3 +140,39 @@ def test_node_host_runs_on_the_gpu():
     assert out["stopped"] == "SimulationStopped"
     assert out["diagnostics"]["barostat"]["attempts"] >= 1
     assert "released" in out["released"]
+
+
+@pytest.mark.gpu
+def test_run_driver_stages_hand_over_the_whole_state(tmp_path):
+    # minimise -> restrained NPT equilibration -> unrestrained production
+    # (web/app.js:1041-1124): each stage a fresh simulation in the box the
+    # last one left
+    dcd, rst = str(tmp_path / "prod.dcd"), str(tmp_path / "final.rst7")
+    proc = subprocess.run([NODE, os.path.join(ROOT, "rainmd_amd", "node", "run.js"), "--workload", "ala-389w",
+                           "--minimise", "150", "--equilibrate", "300", "--restraint", "500", "--steps", "200",
+                           "--report", "100", "--pressure", "1", "--barostatInterval", "10", "--seed", "5",
+                           "--dcd", dcd, "--rst", rst], capture_output=True, text=True, timeout=300)
+    assert proc.returncode == 0, proc.stderr
+    lines = [json.loads(x) for x in proc.stdout.strip().splitlines()]
+    print(*lines, sep="\n")
+    assert lines[0]["stage"] == "minimise"
+    head = lines[1]
+    assert head["stage"] == "equilibration" and head["restrainedAtoms"] == 10 and head["restraintForce"] == 500
+    equil = [x for x in lines if x.get("stage") == "equilibration" and "step" in x]
+    prod = [x for x in lines if x.get("stage") == "production"]
+    assert [x["step"] for x in equil] == [100, 200, 300] and [x["step"] for x in prod] == [100, 200]
+    # the restraint acts in equilibration only; 10 atoms at k = 500 hold about 15 kT/2
+    assert all(0 < x["positionRestraint"] < 200 for x in equil)
+    assert all(x["positionRestraint"] == 0 for x in prod)
+    assert all(np.isfinite(x["potential"]) and 150 < x["temperature"] < 600 for x in equil + prod)
+    summary = lines[-1]
+    assert summary["done"] and [s["stage"] for s in summary["stages"]] == ["equilibration", "production"]
+    # a move before every block but the first: 300 steps at interval 10
+    assert [s["barostat"]["attempts"] for s in summary["stages"]] == [29, 19]
+    # DCD frames per stage, the restart in the live box
+    with open(str(tmp_path / "prod.equil.dcd"), "rb") as f:
+        assert struct.unpack("<i", f.read(12)[8:12])[0] == 3
+    with open(dcd, "rb") as f:
+        assert struct.unpack("<i", f.read(12)[8:12])[0] == 2
+    c = R.Coordinates.load(rst, 1189)
+    assert abs(np.prod(c.box["lengths"]) / prod[-1]["volume"] - 1) < 1e-5
