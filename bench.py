@@ -35,7 +35,8 @@ COPY_PROBE_BYTES = 64 << 20  # the counter-calibration copy: 64 MiB read + 64 Mi
 REFERENCE_NS_PER_DAY = 74.0  # ala-wat-big, Apple M-series WebGPU (docs/STATUS.md:199-203)
 
 # The single-GPU workloads: the 2854-atom NPT system the metric is quoted on
-# (default), the 1189-atom NVT system and the 100,200-atom water box.
+# (default), the 1189-atom NVT system and the 100,200-atom water box; and
+# "replicas-N", N uneven replicas stepped as one batch.
 WORKLOADS = {
     "ala-944w": dict(what="ACE-ALA-NME + 944 TIP3P", ensemble="NPT 1 bar 300 K, MC barostat every 25 steps",
                      options=dict(pressure=1.0, barostat_interval=25), reference_ns_per_day=REFERENCE_NS_PER_DAY,
@@ -120,6 +121,148 @@ def cpu_baseline(system, pme, state, target_seconds):
     }
 
 
+def energy_gate(sim, system):
+    """Every term of a GPU state against the FP64 oracle at the same f32
+    positions and box; returns the largest relative error and the terms."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle as O
+
+    st = sim.read_state()
+    diag = sim.diagnostics()
+    o = O.Oracle(system, box=np.array(st["box_vectors"]))
+    ref, _ = o.potential(st["positions"].astype(np.float64), alpha=diag["pme_alpha"], grid=list(diag["pme_grid"]),
+                         f32_membership=True, forces=False)
+    gpu, _ = sim.evaluate(forces=False)
+    pairs = {
+        "bonded": (gpu["bonded"], ref["bonded"]),
+        "lennard_jones": (gpu["lennard_jones"], ref["lennard_jones"]),
+        "coulomb_direct": (gpu["coulomb_direct"], ref["coulomb_direct_total"]),
+        "coulomb_reciprocal": (gpu["coulomb_reciprocal"], ref["reciprocal"]),
+        "total": (gpu["potential"] - gpu["dispersion"], ref["total"]),
+    }
+    check = {k: abs(a - b) / max(abs(b), 1.0) for k, (a, b) in pairs.items()}
+    return max(check.values()), check
+
+
+def bench_replicas(args, count):
+    """The replica workload: `count` uneven replicas (edges U(2.2, 5.0) nm) in
+    one batch, NPT. The aggregate is count x 2 fs per batch step; the
+    baseline is the same replicas run one system per launch (each alone
+    through its own context, graphs and concurrent force branches on)."""
+    import rainmd_amd as R
+
+    work = R.replica_workload(count)
+    seeds = [s for _, _, s in work]
+    t0 = time.perf_counter()
+    systems = R.replica_systems(count)
+    atoms = [s.atom_count for s in systems]
+    log(f"{count} replicas, {sum(atoms)} atoms ({min(atoms)}-{max(atoms)}), built in {time.perf_counter() - t0:.1f} s")
+    t0 = time.perf_counter()
+    batch = R.Batch(systems, seeds=seeds, temperature=300.0, pressure=1.0, barostat_interval=25)
+    if args.no_graphs:
+        batch.set_graphs(False)
+    log(f"batch created in {time.perf_counter() - t0:.1f} s: {batch.info()}")
+    t0 = time.perf_counter()
+    for r, system in enumerate(systems):
+        sim = batch.replica(r)
+        sim.minimise(max_iterations=args.replica_minimise, force_tolerance=1000.0)
+        sim.set_state(sim.read_positions().astype(np.float64), R.maxwell_boltzmann(system, 300.0, seeds[r]), step=0)
+    log(f"replicas minimised in {time.perf_counter() - t0:.1f} s")
+    t0 = time.perf_counter()
+    batch.run(args.warmup, args.block)
+    log(f"warm-up {args.warmup} steps in {time.perf_counter() - t0:.2f} s")
+    t0 = time.perf_counter()
+    batch.run(args.steps, args.block)
+    elapsed = time.perf_counter() - t0
+    ms_per_step = 1e3 * elapsed / args.steps
+    aggregate = count * TIMESTEP_PS * 1e-3 * 86400.0 / (ms_per_step * 1e-3)
+    log(f"timed {args.steps} batch steps: {ms_per_step:.3f} ms/step, aggregate {aggregate:.1f} ns/day")
+
+    # the batched direct-space launch at the final state
+    nb_ms = batch.profile_nonbonded(reps=20)
+    pairs = 0
+    for r, system in enumerate(systems):
+        st = batch.replica(r).read_state()
+        pairs += in_cutoff_pairs(st["positions"], np.array(st["box_vectors"]), system.cutoff,
+                                 system.exclusion_start, system.exclusion_partners)
+    achieved = FLOP_PER_PAIR * pairs / (nb_ms * 1e-3) / 1e12
+    log(f"batched nonbonded {nb_ms * 1e3:.1f} us over {pairs} pairs -> {achieved:.2f} TFLOP/s")
+    small, large = int(np.argmin(atoms)), int(np.argmax(atoms))
+    gates = {}
+    if not args.no_cpu_baseline:
+        for r in (small, large):
+            gates[r] = energy_gate(batch.replica(r), systems[r])
+            log(f"energy gate replica {r} ({atoms[r]} atoms): max rel {gates[r][0]:.2e}")
+    barostat = [batch.replica(r).diagnostics() for r in range(count)]
+    moves = sum(d["barostat_attempts"] for d in barostat)
+    accepted = sum(d["barostat_accepted"] for d in barostat)
+
+    if args.single_steps <= 0:
+        batch.release()
+        return
+    # one system per launch: every replica alone, the same kernels
+    single_total = 0.0
+    for r in range(count):
+        sim = batch.replica(r)
+        sim.run(args.block, args.block)  # captures its block graph
+        t0 = time.perf_counter()
+        sim.run(args.single_steps, args.block)
+        single_total += time.perf_counter() - t0
+    single_ms = 1e3 * single_total / args.single_steps
+    single_aggregate = count * TIMESTEP_PS * 1e-3 * 86400.0 / (single_ms * 1e-3)
+    log(f"one system per launch: {single_ms:.3f} ms per step of all replicas, aggregate {single_aggregate:.1f} ns/day; "
+        f"batch speedup {aggregate / single_aggregate:.2f}x")
+    result = {
+        "metric": METRIC,
+        "value": aggregate,
+        "unit": "ns/day",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms_per_step,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {
+            "workload": f"replicas-{count}: {count} replicas of ACE-ALA-NME + TIP3P at 1.000 g/cm^3, cube edges "
+                        f"U(2.2, 5.0) nm (seed 404), {min(atoms)}-{max(atoms)} atoms, {sum(atoms)} in all, NPT 1 bar "
+                        f"300 K, MC barostat every 25 steps per replica, 2 fs, rc 0.9 nm, PME grid per replica",
+            "replicas": count,
+            "atoms": sum(atoms),
+            "block": args.block,
+        },
+        "single_launch_baseline": {
+            "value": single_aggregate,
+            "unit": "ns/day",
+            "ms_per_step": single_ms,
+            "sample": f"every replica run alone for {args.single_steps} steps through its own context "
+                      "(graphs and concurrent force branches on)",
+        },
+        "batch_speedup": aggregate / single_aggregate,
+        "barostat": {"attempts": moves, "accepted": accepted},
+        "roofline": {
+            "kernel": "batched nonbonded (LJ + erfc real space, every replica in one launch)",
+            "bound": "mfma",
+            "achieved": achieved,
+            "peak": FP32_PEAK_TFLOPS,
+            "unit": "TFLOP/s",
+            "frac": achieved / FP32_PEAK_TFLOPS,
+            "traffic": None,
+            "launch_us": nb_ms * 1e3,
+            "pairs": pairs,
+            "flop_per_pair": FLOP_PER_PAIR,
+            "note": "FP32 vector ALU bound; peak is MI355X's published FP32 figure",
+        },
+    }
+    if gates:
+        result["energy_check"] = {"max_rel_err": max(g[0] for g in gates.values()),
+                                  "replicas": {str(r): g[1] for r, g in gates.items()}}
+    batch.release()
+    print(json.dumps(result), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -136,12 +279,19 @@ def main():
                     help="after warm-up, launch only this kernel family (nonbonded, pme, integrate, bonded, rebuild) "
                          "and the calibration copy, for hardware-counter passes; prints their durations and exits")
     ap.add_argument("--reps", type=int, default=50)
+    ap.add_argument("--single-steps", type=int, default=100,
+                    help="replica workloads: steps of each replica alone for the one-system-per-launch baseline")
+    ap.add_argument("--replica-minimise", type=int, default=150)
     args = ap.parse_args()
 
     if args.gpus != 1:
         print(f"bench.py measures one GPU: this build runs one system per MI355X and has no multi-GPU "
               f"path (asked for --gpus {args.gpus}).", file=sys.stderr)
         sys.exit(2)
+
+    if args.workload.startswith("replicas-"):
+        bench_replicas(args, int(args.workload.split("-")[1]))
+        return
 
     import rainmd_amd as R
 

@@ -446,6 +446,54 @@ typedef struct rmd_device_info {
 int rmd_device_info_get(int32_t device, rmd_device_info *out);
 int32_t rmd_abi_version(void);
 
+/* ======================================================================== */
+/* Replica batches                                                          */
+/* ======================================================================== */
+
+/* Many independent systems stepped together (uneven replicas, each with its
+ * own box, PME grid, seed and barostat): the "rmd_batch_* variants" a host
+ * needs for replica workloads. The reference steps one GpuSimulation per
+ * system (webgpu/simulation.js:82); a batch is R member contexts stepped by
+ * batched launches, each replica computing exactly what its own context
+ * would alone. Every replica takes the same options except its seed
+ * (seeds[r], or options->seed when seeds is NULL), needs the same Ewald
+ * alpha, at most 16384 atoms and 4096 cells, and a PME grid whose planes fit
+ * in LDS. */
+typedef struct rmd_batch rmd_batch;
+int rmd_batch_create(const rmd_system *const *systems, int32_t count, const rmd_options *options,
+                     const uint32_t *seeds, rmd_batch **out);
+void rmd_batch_destroy(rmd_batch *batch);
+const char *rmd_batch_last_error(const rmd_batch *batch);
+/* Replica r's own context, owned by the batch (rmd_destroy leaves it alone):
+ * every single-system call works on it - setState, minimise, energy,
+ * readState, diagnostics, and a run of that replica on its own. NULL when r
+ * is out of range. */
+rmd_ctx *rmd_batch_replica(rmd_batch *batch, int32_t replica);
+typedef struct rmd_batch_info {
+  int32_t replicas;
+  int64_t atoms;                 /* over all replicas                          */
+  int64_t blocks_run;            /* step blocks run by rmd_batch_run           */
+  int64_t step_workgroups;       /* workgroups launched per step, all kernels  */
+  int64_t nonbonded_workgroups;  /* of the batched direct-space launch         */
+} rmd_batch_info;
+int rmd_batch_info_get(rmd_batch *batch, rmd_batch_info *out);
+/* initialise() of every replica. */
+int rmd_batch_initialise(rmd_batch *batch);
+/* One evaluation of the given RMD_TERM_* terms of every replica at its
+ * current positions (the counterpart of rmd_evaluate); each replica's forces
+ * are then readable through its own context. */
+int rmd_batch_evaluate(rmd_batch *batch, int32_t terms);
+/* run(count, {blockSize}) of every replica at once (webgpu/simulation.js:
+ * 456-527); each replica's barostat moves when its own interval is due, and
+ * the moves of all due replicas are evaluated together. *step_out is replica
+ * 0's step. Errors name the replica. */
+int rmd_batch_run(rmd_batch *batch, int64_t steps, int32_t block, int64_t *step_out);
+int rmd_batch_request_stop(rmd_batch *batch);
+int rmd_batch_set_graphs(rmd_batch *batch, int32_t enabled);
+int rmd_batch_set_concurrent_forces(rmd_batch *batch, int32_t enabled);
+/* Average duration of the batched direct-space launch (profiling). */
+int rmd_batch_profile_nonbonded(rmd_batch *batch, int32_t reps, double *avg_ms);
+
 #ifdef __cplusplus
 }
 #endif

@@ -217,6 +217,11 @@ class DeviceInfo(C.Structure):
 
 
 # Every function the header declares, with its ctypes signature.
+class BatchInfo(C.Structure):
+    _fields_ = [("replicas", C.c_int32), ("atoms", C.c_int64), ("blocks_run", C.c_int64),
+                ("step_workgroups", C.c_int64), ("nonbonded_workgroups", C.c_int64)]
+
+
 SIGNATURES = {
     "rmd_abi_version": (C.c_int32, []),
     "rmd_last_global_error": (C.c_char_p, []),
@@ -234,6 +239,19 @@ SIGNATURES = {
                                             C.POINTER(SolvatedOptions), C.POINTER(C.c_void_p)]),
     "rmd_system_view_get": (C.c_int, [C.c_void_p, C.POINTER(SystemView)]),
     "rmd_system_rebox": (C.c_int, [C.c_void_p, _dp, C.POINTER(C.c_void_p)]),
+    "rmd_batch_create": (C.c_int, [C.POINTER(C.c_void_p), C.c_int32, C.POINTER(Options), C.POINTER(C.c_uint32),
+                                   C.POINTER(C.c_void_p)]),
+    "rmd_batch_destroy": (None, [C.c_void_p]),
+    "rmd_batch_last_error": (C.c_char_p, [C.c_void_p]),
+    "rmd_batch_replica": (C.c_void_p, [C.c_void_p, C.c_int32]),
+    "rmd_batch_info_get": (C.c_int, [C.c_void_p, C.POINTER(BatchInfo)]),
+    "rmd_batch_initialise": (C.c_int, [C.c_void_p]),
+    "rmd_batch_evaluate": (C.c_int, [C.c_void_p, C.c_int32]),
+    "rmd_batch_run": (C.c_int, [C.c_void_p, C.c_int64, C.c_int32, C.POINTER(C.c_int64)]),
+    "rmd_batch_request_stop": (C.c_int, [C.c_void_p]),
+    "rmd_batch_set_graphs": (C.c_int, [C.c_void_p, C.c_int32]),
+    "rmd_batch_set_concurrent_forces": (C.c_int, [C.c_void_p, C.c_int32]),
+    "rmd_batch_profile_nonbonded": (C.c_int, [C.c_void_p, C.c_int32, C.POINTER(C.c_double)]),
     "rmd_system_solute_heavy_atoms": (C.c_int, [C.c_void_p, _ip, C.c_int32, C.POINTER(C.c_int32)]),
     "rmd_pme_parameters_derive": (C.c_int, [_dp, C.c_double, C.c_double, C.c_double, _ip,
                                             C.POINTER(PmeParameters)]),

@@ -13,6 +13,10 @@
 // exception pair per atom; the cost is negligible next to the pair list.
 #include "kernels.h"
 
+// Multiply-adds fuse only within one expression, so each body rounds the same
+// way wherever it is inlined (the single-system and the batched kernels).
+#pragma clang fp contract(on)
+
 namespace rmd {
 
 __device__ __forceinline__ void addForce(const Dev &d, int atom, double fx, double fy, double fz) {
@@ -184,9 +188,9 @@ __device__ double restraintTerm(const Dev &d, const BoxF &box, int t) {
 
 enum { BT_BOND = 0, BT_ANGLE, BT_TORSION, BT_LJX, BT_COULX, BT_RESTRAINT, BT_COUNT };
 
-__global__ void __launch_bounds__(256) bondedKernel(Dev d, double alphaD, int terms, int energies) {
+__device__ __forceinline__ void bonded(const Dev &d, double alphaD, int terms, int energies, int block) {
   const BoxF &box = *d.box;
-  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  const int t = block * blockDim.x + threadIdx.x;
   double e[BT_COUNT] = {0, 0, 0, 0, 0, 0};
   int base = 0;
   const bool doBonded = terms & RMD_TERM_BONDED;
@@ -223,13 +227,31 @@ __global__ void __launch_bounds__(256) bondedKernel(Dev d, double alphaD, int te
                      : threadIdx.x == BT_LJX ? E_LJ_EXCEPTION
                      : threadIdx.x == BT_COULX ? E_COUL_EXCEPTION
                                                : E_RESTRAINT;
-    d.slots[term * MAX_SLOTS + blockIdx.x] = s;
+    d.slots[term * MAX_SLOTS + block] = s;
   }
 }
 
+__global__ void __launch_bounds__(256) bondedKernel(Dev d, double alphaD, int terms, int energies) {
+  bonded(d, alphaD, terms, energies, blockIdx.x);
+}
+
+__global__ void __launch_bounds__(256) bondedBatchKernel(const BatchView *__restrict__ v, const int2 *__restrict__ task,
+                                                         double alphaD, int terms, int energies) {
+  const int2 t = task[blockIdx.x];
+  if (t.x < 0) return;
+  bonded(v[t.x].d, alphaD, terms, energies, t.y);
+}
+
+int bondedBlocks(const Dev &d) {
+  return (d.nBond + d.nAngle + d.nTorsion + d.nException + d.nRestraint + 255) / 256;
+}
+
+void launchBondedBatch(const BatchView *v, const TaskTable &t, double alpha, int terms, int energies, hipStream_t s) {
+  if (t.blocks > 0) bondedBatchKernel<<<t.blocks, 256, 0, s>>>(v, t.task, alpha, terms, energies);
+}
+
 int launchBonded(const Dev &d, double alpha, int terms, int energies, hipStream_t s, int *slotsUsed) {
-  const int total = d.nBond + d.nAngle + d.nTorsion + d.nException + d.nRestraint;
-  const int blocks = (total + 255) / 256;
+  const int blocks = bondedBlocks(d);
   if (blocks > 0) bondedKernel<<<blocks, 256, 0, s>>>(d, alpha, terms, energies);
   if (slotsUsed) *slotsUsed = blocks;
   return blocks;

@@ -78,8 +78,20 @@ std::vector<int> factorRadices(int K) {
 
 }  // namespace
 
+// The streams and fork/join events of a context; a replica batch lends its
+// own to every member context.
+struct Streams {
+  hipStream_t stream = nullptr;
+  hipStream_t side[2] = {nullptr, nullptr};
+  hipEvent_t fork = nullptr, join[2] = {nullptr, nullptr};
+};
+
 struct rmd_ctx {
   System sys;
+  // A member of a replica batch: the batch owns it (rmd_destroy leaves it
+  // alone) and lends it its streams.
+  bool batchMember = false;
+  bool ownsStreams = true;
   rmd_options opt{};
   std::string lastError;
   // derived parameters
@@ -171,7 +183,7 @@ void invalidateGraphs(rmd_ctx *c) {
 
 // Write the device box: f32 rows and inverse widths for the pair kernels,
 // FP64 rows for PME and restraints, the cell grid shape.
-void writeBox(rmd_ctx *c) {
+BoxF boxFor(const rmd_ctx *c) {
   BoxF b{};
   double recip[9];
   reciprocalBoxVectors(c->box, recip);
@@ -194,7 +206,14 @@ void writeBox(rmd_ctx *c) {
   b.periodic = 1;
   for (int k = 0; k < 3; ++k) b.cells[k] = c->plan.counts[k];
   b.cellTotal = c->plan.cellCount;
+  return b;
+}
+
+void writeBox(rmd_ctx *c) {
+  const BoxF b = boxFor(c);
   HIP_CHECK(hipMemcpyAsync(c->boxDev, &b, sizeof b, hipMemcpyHostToDevice, c->stream));
+  // the box crosses from pageable host memory: finish before `b` goes
+  HIP_CHECK(hipStreamSynchronize(c->stream));
   ewaldSelfEnergy(c->sys, c->pme.alpha, volumeOf(c->box), &c->selfEnergy, &c->netEnergy);
   const double *mods[3] = {c->dModuli[0], c->dModuli[1], c->dModuli[2]};
   pmeComputeInfluence(c->p, c->boxDev, mods, c->pme.alpha, c->stream);
@@ -685,7 +704,9 @@ void runBlock(rmd_ctx *c, int steps) {
 // #updateBox (webgpu/simulation.js:681-731): adopt a new box, re-plan the
 // cells and the capacity, update the self energy and the PME influence
 // function; the PME grid and alpha stay fixed.
-void updateBox(rmd_ctx *c, const double *nb) {
+// The host half of a box change: the box, the cell plan and capacities, the
+// self energy. The device box and the influence function follow separately.
+void updateBoxHost(rmd_ctx *c, const double *nb) {
   c->blockEndStep = -1;
   for (int k = 0; k < 9; ++k) c->box[k] = nb[k];
   CellPlan estimate = planCellList(c->sys, c->plan.skin, c->slack, 0, c->box);
@@ -697,6 +718,11 @@ void updateBox(rmd_ctx *c, const double *nb) {
     allocateNeighbours(c, capacity);
     ++c->neighbourGrowths;
   }
+  ewaldSelfEnergy(c->sys, c->pme.alpha, volumeOf(c->box), &c->selfEnergy, &c->netEnergy);
+}
+
+void updateBox(rmd_ctx *c, const double *nb) {
+  updateBoxHost(c, nb);
   writeBox(c);
   setForceRebuildKernel<<<1, 1, 0, c->stream>>>(c->d.sc);
 }
@@ -755,7 +781,7 @@ void fillEnergy(rmd_ctx *c, const double *t, rmd_energy_terms *e, bool withKinet
   for (int k = 0; k < 9; ++k) e->box[k] = c->box[k];
 }
 
-void create(rmd_ctx *c, const System &sys, const rmd_options &o) {
+void create(rmd_ctx *c, const System &sys, const rmd_options &o, const Streams *shared = nullptr) {
   c->sys = sys;
   c->opt = o;
   const Topology &t = sys.topology;
@@ -790,12 +816,22 @@ void create(rmd_ctx *c, const System &sys, const rmd_options &o) {
   if (hipGetDeviceCount(&count) != hipSuccess || count <= 0)
     throw Error(RMD_E_DEVICE, "no HIP device is available; this library runs on an MI355X (gfx950)");
   HIP_CHECK(hipSetDevice(c->device));
-  HIP_CHECK(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
-  for (int k = 0; k < 2; ++k) {
-    HIP_CHECK(hipStreamCreateWithFlags(&c->side[k], hipStreamNonBlocking));
-    HIP_CHECK(hipEventCreateWithFlags(&c->join[k], hipEventDisableTiming));
+  if (shared) {
+    c->ownsStreams = false;
+    c->stream = shared->stream;
+    c->fork = shared->fork;
+    for (int k = 0; k < 2; ++k) {
+      c->side[k] = shared->side[k];
+      c->join[k] = shared->join[k];
+    }
+  } else {
+    HIP_CHECK(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
+    for (int k = 0; k < 2; ++k) {
+      HIP_CHECK(hipStreamCreateWithFlags(&c->side[k], hipStreamNonBlocking));
+      HIP_CHECK(hipEventCreateWithFlags(&c->join[k], hipEventDisableTiming));
+    }
+    HIP_CHECK(hipEventCreateWithFlags(&c->fork, hipEventDisableTiming));
   }
-  HIP_CHECK(hipEventCreateWithFlags(&c->fork, hipEventDisableTiming));
   {
     int lds = 0;
     if (hipDeviceGetAttribute(&lds, hipDeviceAttributeMaxSharedMemoryPerBlock, c->device) == hipSuccess && lds > 0)
@@ -1085,7 +1121,8 @@ int rmd_create(const rmd_system *system, const rmd_options *options, rmd_ctx **o
   }
 }
 
-void rmd_destroy(rmd_ctx *c) {
+namespace {
+void destroyContext(rmd_ctx *c) {
   if (!c) return;
   (void)hipSetDevice(c->device);
   if (c->stream) (void)hipStreamSynchronize(c->stream);
@@ -1093,13 +1130,22 @@ void rmd_destroy(rmd_ctx *c) {
   if (c->screen.net) rmd_surrogate_free(c->screen.net);
   (void)hipSetDevice(c->device);
   for (void *p : c->held) (void)hipFree(p);
-  for (int k = 0; k < 2; ++k) {
-    if (c->side[k]) (void)hipStreamDestroy(c->side[k]);
-    if (c->join[k]) (void)hipEventDestroy(c->join[k]);
+  if (c->ownsStreams) {
+    for (int k = 0; k < 2; ++k) {
+      if (c->side[k]) (void)hipStreamDestroy(c->side[k]);
+      if (c->join[k]) (void)hipEventDestroy(c->join[k]);
+    }
+    if (c->fork) (void)hipEventDestroy(c->fork);
+    if (c->stream) (void)hipStreamDestroy(c->stream);
   }
-  if (c->fork) (void)hipEventDestroy(c->fork);
-  if (c->stream) (void)hipStreamDestroy(c->stream);
   delete c;
+}
+}  // namespace
+
+void rmd_destroy(rmd_ctx *c) {
+  // a batch member belongs to its batch (rmd_batch_destroy)
+  if (!c || c->batchMember) return;
+  destroyContext(c);
 }
 
 const char *rmd_last_error(const rmd_ctx *c) { return c ? c->lastError.c_str() : rmd_last_global_error(); }
@@ -1726,6 +1772,659 @@ int rmd_copy_probe(rmd_ctx *c, int64_t bytes, int32_t reps, double *avg_ms) {
 
 int rmd_get_stream(rmd_ctx *c, void **stream) {
   CTX_GUARD(c, { *stream = (void *)c->stream; });
+}
+
+}  // extern "C"
+
+// ============================================================================
+// Replica batches
+// ============================================================================
+//
+// A batch is R member contexts, each a complete single-system context (its
+// own buffers, Scalars, energy slots, PME descriptor and seeds), stepped by
+// batched launches: one launch per kernel covers every replica's workgroups
+// through a task table, so a step of the whole batch is the same dozen
+// launches as a step of one system, however many replicas there are. Each
+// replica computes exactly what its own context would compute alone (the
+// kernel bodies and per-replica grids are the single-system ones), so a
+// batch run reproduces the single runs bit for bit. Member contexts stay
+// usable through the single-system calls (set state, minimise, energy,
+// read state, diagnostics, and a run of that replica alone).
+
+struct rmd_batch {
+  std::string lastError;
+  int device = 0;
+  int replicas = 0;
+  Streams streams;
+  std::vector<rmd_ctx *> reps;
+  std::vector<void *> held;
+  std::vector<BatchView> views;  // host copy of the device views
+  BatchView *dViews = nullptr;
+  TaskTable tables[BK_COUNT];
+  Scalars *scAll = nullptr, *scSaved = nullptr;  // every replica's scalars, contiguous
+  BoxF *boxAll = nullptr;
+  double *termsAll = nullptr;  // replicas * E_TERM_COUNT
+  std::vector<Scalars> hostScalars;
+  std::vector<double> hostTerms;
+  size_t ldsForward = 0, ldsXlines = 0, ldsGather = 0;
+  int ldsOpen = 0;
+  double alpha = 0;
+  bool concurrent = true, useGraphs = true;
+  std::map<int, hipGraphExec_t> graphs;
+  std::atomic<int> stop{0};
+  int64_t blocksRun = 0;
+};
+
+namespace {
+
+constexpr int XCDS = 8;
+
+int replicaBlocks(const rmd_ctx *c, int kind) {
+  const Dev &d = c->d;
+  switch (kind) {
+    case BK_OPEN: return (d.nUnits + INTEGRATE_BLOCK - 1) / INTEGRATE_BLOCK;
+    case BK_ATOMS64: return (d.n + 3) / 4;  // 64 lanes per atom, 256 threads per workgroup
+    case BK_BONDED: return bondedBlocks(d);
+    case BK_PLANES: return c->p.K[0];
+    case BK_XLINES: return pmeXLineBlocks(c->p);
+    case BK_REPLICA: return 1;
+    case BK_ATOMS256: return (d.nPad + 255) / 256;
+    case BK_MOLECULES: return std::max(1, (c->sys.topology.moleculeCount + 255) / 256);
+    case BK_INFLUENCE: return (int)(((size_t)c->p.K[0] * c->p.K[1] * c->p.K2h + 255) / 256);
+  }
+  return 0;
+}
+
+// Workgroups reach the eight XCDs round robin (workgroup b runs on XCD
+// b % 8), and each XCD has its own L2. The table gives all of a replica's
+// workgroups to one XCD, so a replica's positions, lists and grids stay in
+// one L2; replicas go to the least loaded XCD, largest first, and the
+// shorter XCD queues are padded with empty tasks.
+TaskTable buildTasks(rmd_batch *b, int kind) {
+  const int R = b->replicas;
+  std::vector<int> per(R), order(R);
+  for (int r = 0; r < R; ++r) per[r] = replicaBlocks(b->reps[r], kind);
+  std::iota(order.begin(), order.end(), 0);
+  std::stable_sort(order.begin(), order.end(), [&](int x, int y) { return per[x] > per[y]; });
+  std::vector<std::vector<int2>> bins(XCDS);
+  std::vector<long long> load(XCDS, 0);
+  for (int r : order) {
+    const int x = (int)(std::min_element(load.begin(), load.end()) - load.begin());
+    for (int k = 0; k < per[r]; ++k) bins[x].push_back(make_int2(r, k));
+    load[x] += per[r];
+  }
+  size_t longest = 0;
+  for (auto &bin : bins) longest = std::max(longest, bin.size());
+  std::vector<int2> task(longest * XCDS, make_int2(-1, 0));
+  for (int x = 0; x < XCDS; ++x)
+    for (size_t k = 0; k < bins[x].size(); ++k) task[k * XCDS + x] = bins[x][k];
+  TaskTable t;
+  t.blocks = (int)task.size();
+  t.task = devAlloc<int2>(std::max<size_t>(1, task.size()), b->held);
+  upload(t.task, task, b->streams.stream);
+  HIP_CHECK(hipStreamSynchronize(b->streams.stream));
+  return t;
+}
+
+// Refresh the device views from the member contexts (their buffers move when
+// a list grows or the cell grid changes) with the flags the host has set.
+void uploadViews(rmd_batch *b) {
+  for (int r = 0; r < b->replicas; ++r) {
+    const rmd_ctx *c = b->reps[r];
+    BatchView &v = b->views[r];
+    v.d = c->d;
+    v.p = c->p;
+    v.ip = c->ip;
+    v.openBlocks = replicaBlocks(c, BK_OPEN);
+    std::memset(&v.counts, 0, sizeof v.counts);
+    const int nb = replicaBlocks(c, BK_ATOMS64), bb = replicaBlocks(c, BK_BONDED), xb = replicaBlocks(c, BK_XLINES);
+    v.counts.c[E_LJ_DIRECT] = v.counts.c[E_COUL_DIRECT] = v.counts.c[E_DSCALE_DIRECT] = nb;
+    v.counts.c[E_BOND] = v.counts.c[E_ANGLE] = v.counts.c[E_TORSION] = v.counts.c[E_LJ_EXCEPTION] =
+        v.counts.c[E_COUL_EXCEPTION] = v.counts.c[E_RESTRAINT] = bb;
+    v.counts.c[E_RECIPROCAL] = v.counts.c[E_DSCALE_RECIPROCAL] = xb;
+    v.terms = b->termsAll + (size_t)r * E_TERM_COUNT;
+    v.molStart = c->molStart;
+    v.molAtoms = c->molAtoms;
+    v.molOfAtom = c->molOfAtom;
+    v.nMol = c->sys.topology.moleculeCount;
+    v.centroids = c->centroids;
+    v.posSaved = c->posSaved;
+    v.velSaved = c->velSaved;
+    v.forceSaved = c->forceSaved;
+    for (int k = 0; k < 3; ++k) v.moduli[k] = c->dModuli[k];
+    v.alpha = c->pme.alpha;
+  }
+  HIP_CHECK(hipMemcpyAsync(b->dViews, b->views.data(), b->views.size() * sizeof(BatchView), hipMemcpyHostToDevice,
+                           b->streams.stream));
+  HIP_CHECK(hipStreamSynchronize(b->streams.stream));
+}
+
+void clearFlags(rmd_batch *b) {
+  for (auto &v : b->views) {
+    v.rebuild = 0;
+    v.restore = 0;
+    v.scale = 1.0f;
+  }
+}
+
+// Every replica's device box from its host box, in one copy.
+void uploadBoxes(rmd_batch *b) {
+  std::vector<BoxF> boxes(b->replicas);
+  for (int r = 0; r < b->replicas; ++r) boxes[r] = boxFor(b->reps[r]);
+  HIP_CHECK(hipMemcpyAsync(b->boxAll, boxes.data(), boxes.size() * sizeof(BoxF), hipMemcpyHostToDevice,
+                           b->streams.stream));
+  HIP_CHECK(hipStreamSynchronize(b->streams.stream));
+}
+
+void readBack(rmd_batch *b, bool terms) {
+  hipStream_t s = b->streams.stream;
+  HIP_CHECK(hipMemcpyAsync(b->hostScalars.data(), b->scAll, b->hostScalars.size() * sizeof(Scalars),
+                           hipMemcpyDeviceToHost, s));
+  if (terms)
+    HIP_CHECK(hipMemcpyAsync(b->hostTerms.data(), b->termsAll, b->hostTerms.size() * sizeof(double),
+                             hipMemcpyDeviceToHost, s));
+  HIP_CHECK(hipStreamSynchronize(s));
+  for (int r = 0; r < b->replicas; ++r)
+    if (b->hostScalars[r].cellOverflow)
+      throw Error(RMD_E_INVALID, "replica " + std::to_string(r) +
+                                     ": a cell held more atoms than the cell sort supports; the box is too dense "
+                                     "or too small for this cutoff");
+}
+
+std::string replicaLabel(int r) { return "replica " + std::to_string(r) + ": "; }
+
+// Grow every overflowed replica's list; true when any grew.
+bool growOverflowed(rmd_batch *b) {
+  bool grew = false;
+  for (int r = 0; r < b->replicas; ++r) {
+    rmd_ctx *c = b->reps[r];
+    const Scalars &h = b->hostScalars[r];
+    c->worstRequired = std::max(c->worstRequired, h.requiredMax);
+    if (h.requiredMax > (unsigned int)c->d.cap) {
+      allocateNeighbours(c, std::max(c->d.cap + 32, (int)std::ceil(h.requiredMax * 1.25)));
+      ++c->neighbourGrowths;
+      b->views[r].rebuild = 1;
+      grew = true;
+    }
+  }
+  return grew;
+}
+
+// One force evaluation of every replica (PME, bonded, then the lists and the
+// direct space), forked and joined like the single-system evaluation.
+void enqueueBatchForces(rmd_batch *b, int energies, int terms = RMD_TERM_ALL) {
+  const Streams &st = b->streams;
+  const bool fork = b->concurrent;
+  hipStream_t pmeStream = fork ? st.side[0] : st.stream, bondedStream = fork ? st.side[1] : st.stream;
+  if (fork) {
+    HIP_CHECK(hipEventRecord(st.fork, st.stream));
+    HIP_CHECK(hipStreamWaitEvent(st.side[0], st.fork, 0));
+    HIP_CHECK(hipStreamWaitEvent(st.side[1], st.fork, 0));
+  }
+  if (terms & RMD_TERM_RECIPROCAL)
+    launchPmeBatch(b->dViews, b->tables[BK_PLANES], b->tables[BK_XLINES], energies, b->ldsForward, b->ldsXlines,
+                   b->ldsGather, pmeStream);
+  if (terms & (RMD_TERM_BONDED | RMD_TERM_EXCEPTIONS | RMD_TERM_RESTRAINT))
+    launchBondedBatch(b->dViews, b->tables[BK_BONDED], b->alpha, terms, energies, bondedStream);
+  launchRebuildBatch(b->dViews, b->tables[BK_REPLICA], b->tables[BK_ATOMS64], st.stream);
+  if (terms & RMD_TERM_DIRECT) launchNonbondedBatch(b->dViews, b->tables[BK_ATOMS64], (float)b->alpha, energies, st.stream);
+  if (fork) {
+    for (int k = 0; k < 2; ++k) {
+      HIP_CHECK(hipEventRecord(st.join[k], st.side[k]));
+      HIP_CHECK(hipStreamWaitEvent(st.stream, st.join[k], 0));
+    }
+  }
+}
+
+void enqueueBatchBlock(rmd_batch *b, int steps) {
+  hipStream_t s = b->streams.stream;
+  for (int k = 0; k < steps; ++k) {
+    const bool last = k == steps - 1;
+    launchIntegrateOpenBatch(b->dViews, b->tables[BK_OPEN], k > 0 ? 1 : 0, last ? 1 : 0, b->ldsOpen, s);
+    enqueueBatchForces(b, last ? 1 : 0);
+  }
+  launchReduceEnergiesBatch(b->dViews, b->replicas, 1, s);
+  launchIntegrateCloseBatch(b->dViews, b->tables[BK_OPEN], b->ldsOpen, s);
+}
+
+void launchBatchBlock(rmd_batch *b, int steps) {
+  hipStream_t s = b->streams.stream;
+  if (!b->useGraphs) {
+    enqueueBatchBlock(b, steps);
+    HIP_CHECK(hipGetLastError());
+    return;
+  }
+  auto it = b->graphs.find(steps);
+  if (it == b->graphs.end()) {
+    hipGraph_t graph;
+    HIP_CHECK(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+    enqueueBatchBlock(b, steps);
+    HIP_CHECK(hipStreamEndCapture(s, &graph));
+    hipGraphExec_t exec;
+    HIP_CHECK(hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0));
+    HIP_CHECK(hipGraphDestroy(graph));
+    it = b->graphs.emplace(steps, exec).first;
+  }
+  HIP_CHECK(hipGraphLaunch(it->second, s));
+}
+
+// One evaluation of every replica at its current positions; the replicas
+// whose view has `rebuild` set (or all, with forceAll) rebuild their lists.
+// With energies the reduced terms land in hostTerms.
+void batchEvaluate(rmd_batch *b, bool energies, bool forceAll, int terms = RMD_TERM_ALL) {
+  hipStream_t s = b->streams.stream;
+  for (int attempt = 0; attempt < 4; ++attempt) {
+    uploadViews(b);
+    launchClearRequiredBatch(b->scAll, b->replicas, s);
+    launchDecideRebuildBatch(b->dViews, b->replicas, forceAll ? 1 : 0, s);
+    launchZeroForcesBatch(b->dViews, b->tables[BK_ATOMS256], s);
+    enqueueBatchForces(b, energies ? 1 : 0, terms);
+    if (energies) launchReduceEnergiesBatch(b->dViews, b->replicas, 0, s);
+    HIP_CHECK(hipGetLastError());
+    readBack(b, energies);
+    if (growOverflowed(b)) continue;
+    for (auto &v : b->views) v.rebuild = 0;
+    for (rmd_ctx *c : b->reps) c->initialised = true;
+    return;
+  }
+  throw Error(RMD_E_NEIGHBOUR_OVERFLOW, "the neighbour list kept overflowing after growth");
+}
+
+// A block of `steps` steps of every replica, with the single-system block's
+// overflow contract: a list that overflowed anywhere re-runs the whole block
+// from its saved start after growing.
+void batchRunBlock(rmd_batch *b, int steps) {
+  hipStream_t s = b->streams.stream;
+  const int R = b->replicas;
+  for (int attempt = 0; attempt < 4; ++attempt) {
+    uploadViews(b);
+    launchClearRequiredBatch(b->scAll, R, s);
+    HIP_CHECK(hipMemcpyAsync(b->scSaved, b->scAll, R * sizeof(Scalars), hipMemcpyDeviceToDevice, s));
+    launchSnapshotBatch(b->dViews, b->tables[BK_ATOMS256], SNAP_VELOCITIES, s);
+    launchBatchBlock(b, steps);
+    readBack(b, true);
+    for (int r = 0; r < R; ++r) {
+      const Scalars &h = b->hostScalars[r];
+      rmd_ctx *c = b->reps[r];
+      if (h.nonFinite || std::isnan(__builtin_bit_cast(float, h.maxDisp2Bits)))
+        throw Error(RMD_E_NONFINITE, replicaLabel(r) + "a coordinate became non-finite in the block of " +
+                                         std::to_string(steps) + " steps that ended at step " +
+                                         std::to_string(c->step + steps));
+      if (h.shakeFailed)
+        throw Error(RMD_E_CONSTRAINT, replicaLabel(r) + "constraints did not converge in the iteration limit");
+    }
+    if (growOverflowed(b)) {
+      launchSnapshotBatch(b->dViews, b->tables[BK_ATOMS256], SNAP_RESTORE | SNAP_VELOCITIES, s);
+      HIP_CHECK(hipMemcpyAsync(b->scAll, b->scSaved, R * sizeof(Scalars), hipMemcpyDeviceToDevice, s));
+      for (int r = 0; r < R; ++r)
+        if (b->views[r].rebuild) setForceRebuildKernel<<<1, 1, 0, s>>>(b->reps[r]->d.sc);
+      HIP_CHECK(hipStreamSynchronize(s));
+      for (auto &v : b->views) v.rebuild = 0;
+      continue;
+    }
+    for (int r = 0; r < R; ++r) {
+      rmd_ctx *c = b->reps[r];
+      c->step += steps;
+      c->stepsRun += steps;
+      c->kineticSampledAtStep = c->step;
+      c->blockEndStep = c->step;
+      c->stepsSinceBarostat += steps;
+      std::memcpy(c->blockEndTerms, &b->hostTerms[(size_t)r * E_TERM_COUNT], sizeof c->blockEndTerms);
+    }
+    ++b->blocksRun;
+    return;
+  }
+  throw Error(RMD_E_NEIGHBOUR_OVERFLOW, "the neighbour list kept overflowing after growth");
+}
+
+// The Monte Carlo volume move of every replica that is due one, with the
+// single-system move's arithmetic, random numbers and bookkeeping
+// (webgpu/simulation.js:995-1070): the trial boxes of all of them are
+// evaluated together, then each replica accepts or goes back on its own.
+void batchBarostat(rmd_batch *b) {
+  const int R = b->replicas;
+  hipStream_t s = b->streams.stream;
+  struct Trial {
+    bool active = false, moved = false;
+    double volume = 0, kT = 0, work = 0, oldPotential = 0;
+    double oldBox[9];
+    double oldTerms[E_TERM_COUNT];
+  };
+  std::vector<Trial> tr(R);
+  bool any = false, stale = false;
+  for (int r = 0; r < R; ++r) {
+    const rmd_ctx *c = b->reps[r];
+    if (!c->hasPressure || c->stepsSinceBarostat < c->barostatInterval) continue;
+    tr[r].active = any = true;
+    stale = stale || c->blockEndStep != c->step;
+  }
+  if (!any) return;
+  if (stale) batchEvaluate(b, true, true);
+  bool moved = false;
+  for (int r = 0; r < R; ++r) {
+    if (!tr[r].active) continue;
+    rmd_ctx *c = b->reps[r];
+    Trial &t = tr[r];
+    c->stepsSinceBarostat = 0;
+    const double volume = volumeOf(c->box);
+    if (c->volumeScale < 0) c->volumeScale = 0.01 * volume;
+    t.volume = volume;
+    t.kT = BOLTZMANN * c->temperature;
+    const double pressure = c->pressure * BAR_TO_KJ_PER_MOL_NM3;
+    std::memcpy(t.oldTerms, c->blockEndStep == c->step ? c->blockEndTerms : &b->hostTerms[(size_t)r * E_TERM_COUNT],
+                sizeof t.oldTerms);
+    t.oldPotential = potentialFromTerms(c, t.oldTerms);
+    std::memcpy(t.oldBox, c->box, sizeof t.oldBox);
+    c->barostatAttempts += 1;
+    c->windowAttempts += 1;
+    const uint32_t bseed = c->seed ^ 0x5a17u;
+    const double draw = uniformAt(bseed, (uint32_t)c->barostatAttempts, 0, 0);
+    const double deltaVolume = c->volumeScale * 2 * (draw - 0.5);
+    const double newVolume = volume + deltaVolume;
+    if (!(newVolume > 0)) continue;
+    const double sc = std::cbrt(newVolume / volume);
+    const double smallest = sc * std::min({std::fabs(c->box[0]), std::fabs(c->box[4]), std::fabs(c->box[8])});
+    if (c->sys.cutoff > 0.5 * smallest) {
+      c->barostatRefusals += 1;
+      continue;
+    }
+    t.work = pressure * deltaVolume - c->sys.topology.moleculeCount * t.kT * std::log(newVolume / volume);
+    t.moved = moved = true;
+    double nb[9];
+    for (int k = 0; k < 9; ++k) nb[k] = c->box[k] * sc;
+    updateBoxHost(c, nb);
+    if (!fitsSmallCellBuild(c->d.n, c->plan.cellCount))
+      throw Error(RMD_E_INVALID, replicaLabel(r) + "the trial box needs more cells than a batch replica supports");
+    b->views[r].scale = (float)sc;
+    b->views[r].rebuild = 1;
+  }
+  if (!moved) {
+    clearFlags(b);
+    return;
+  }
+  uploadViews(b);
+  launchSnapshotBatch(b->dViews, b->tables[BK_ATOMS256], 0, s);
+  launchScaleBatch(b->dViews, b->tables[BK_MOLECULES], b->tables[BK_ATOMS256], s);
+  uploadBoxes(b);
+  launchInfluenceBatch(b->dViews, b->tables[BK_INFLUENCE], s);
+  batchEvaluate(b, true, false);
+  bool restore = false;
+  for (int r = 0; r < R; ++r) {
+    if (!tr[r].moved) continue;
+    rmd_ctx *c = b->reps[r];
+    const Trial &t = tr[r];
+    const double *newTerms = &b->hostTerms[(size_t)r * E_TERM_COUNT];
+    const double weight = (potentialFromTerms(c, newTerms) - t.oldPotential) + t.work;
+    const uint32_t bseed = c->seed ^ 0x5a17u;
+    const bool accept =
+        weight <= 0 || uniformAt(bseed, (uint32_t)c->barostatAttempts, 1, 0) < std::exp(-weight / t.kT);
+    if (accept) {
+      std::memcpy(c->blockEndTerms, newTerms, sizeof c->blockEndTerms);
+      c->barostatAccepted += 1;
+      c->windowAccepted += 1;
+    } else {
+      // back to the old state exactly: positions, forces, the box (copied)
+      updateBoxHost(c, t.oldBox);
+      std::memcpy(c->blockEndTerms, t.oldTerms, sizeof c->blockEndTerms);
+      b->views[r].restore = 1;
+      b->views[r].rebuild = 1;
+      restore = true;
+    }
+    c->blockEndStep = c->step;
+    if (c->windowAttempts >= 10) {
+      const double rate = (double)c->windowAccepted / c->windowAttempts;
+      if (rate < 0.25)
+        c->volumeScale /= 1.1;
+      else if (rate > 0.75)
+        c->volumeScale = std::min(c->volumeScale * 1.1, 0.3 * t.volume);
+      c->windowAttempts = c->windowAccepted = 0;
+    }
+  }
+  if (restore) {
+    uploadViews(b);
+    launchSnapshotBatch(b->dViews, b->tables[BK_ATOMS256], SNAP_RESTORE | SNAP_FLAGGED, s);
+    uploadBoxes(b);
+    launchInfluenceBatch(b->dViews, b->tables[BK_INFLUENCE], s);
+    launchForceRebuildFlaggedBatch(b->dViews, R, s);
+    HIP_CHECK(hipGetLastError());
+    HIP_CHECK(hipStreamSynchronize(s));
+  }
+  clearFlags(b);
+}
+
+void destroyBatch(rmd_batch *b) {
+  if (!b) return;
+  (void)hipSetDevice(b->device);
+  if (b->streams.stream) (void)hipStreamSynchronize(b->streams.stream);
+  for (auto &g : b->graphs) (void)hipGraphExecDestroy(g.second);
+  for (rmd_ctx *c : b->reps) destroyContext(c);
+  for (void *p : b->held) (void)hipFree(p);
+  for (int k = 0; k < 2; ++k) {
+    if (b->streams.side[k]) (void)hipStreamDestroy(b->streams.side[k]);
+    if (b->streams.join[k]) (void)hipEventDestroy(b->streams.join[k]);
+  }
+  if (b->streams.fork) (void)hipEventDestroy(b->streams.fork);
+  if (b->streams.stream) (void)hipStreamDestroy(b->streams.stream);
+  delete b;
+}
+
+void createBatch(rmd_batch *b, const rmd_system *const *systems, int count, const rmd_options &o,
+                 const uint32_t *seeds) {
+  int devices = 0;
+  if (hipGetDeviceCount(&devices) != hipSuccess || devices <= 0)
+    throw Error(RMD_E_DEVICE, "no HIP device is available; this library runs on an MI355X (gfx950)");
+  b->device = o.device;
+  HIP_CHECK(hipSetDevice(b->device));
+  Streams &st = b->streams;
+  HIP_CHECK(hipStreamCreateWithFlags(&st.stream, hipStreamNonBlocking));
+  for (int k = 0; k < 2; ++k) {
+    HIP_CHECK(hipStreamCreateWithFlags(&st.side[k], hipStreamNonBlocking));
+    HIP_CHECK(hipEventCreateWithFlags(&st.join[k], hipEventDisableTiming));
+  }
+  HIP_CHECK(hipEventCreateWithFlags(&st.fork, hipEventDisableTiming));
+  b->replicas = count;
+  b->scAll = devAlloc<Scalars>(count, b->held);
+  b->scSaved = devAlloc<Scalars>(count, b->held);
+  b->boxAll = devAlloc<BoxF>(count, b->held);
+  b->termsAll = devAlloc<double>((size_t)count * E_TERM_COUNT, b->held);
+  b->dViews = devAlloc<BatchView>(count, b->held);
+  b->hostScalars.resize(count);
+  b->hostTerms.assign((size_t)count * E_TERM_COUNT, 0.0);
+  b->views.assign(count, BatchView{});
+  clearFlags(b);
+  for (int r = 0; r < count; ++r) {
+    if (!systems[r]) throw Error(RMD_E_INVALID, replicaLabel(r) + "null system");
+    rmd_options ro = o;
+    ro.seed = seeds ? seeds[r] : o.seed;
+    ro.lanes_per_atom = 64;
+    rmd_ctx *c = new rmd_ctx();
+    c->batchMember = true;
+    b->reps.push_back(c);
+    try {
+      create(c, systems[r]->s, ro, &st);
+    } catch (const Error &e) {
+      throw Error(e.code, replicaLabel(r) + e.what());
+    }
+    // the member's scalars and box live in the batch's contiguous arrays
+    c->d.sc = b->scAll + r;
+    c->boxDev = b->boxAll + r;
+    c->d.box = c->boxDev;
+    writeBox(c);
+    HIP_CHECK(hipMemsetAsync(c->d.sc, 0, sizeof(Scalars), st.stream));
+    setForceRebuildKernel<<<1, 1, 0, st.stream>>>(c->d.sc);
+    HIP_CHECK(hipStreamSynchronize(st.stream));
+    const System &sys = systems[r]->s;
+    if (!sys.positions.empty())
+      setState(c, sys.positions.data(), sys.velocities.empty() ? nullptr : sys.velocities.data(), 0, 0);
+    if (!c->p.planes)
+      throw Error(RMD_E_INVALID, replicaLabel(r) + "its PME grid needs the line pipeline, which a batch does not run");
+    if (!fitsSmallCellBuild(c->d.n, c->plan.cellCount))
+      throw Error(RMD_E_INVALID, replicaLabel(r) + "a batch replica takes at most 16384 atoms and 4096 cells");
+    if (r == 0)
+      b->alpha = c->pme.alpha;
+    else if (c->pme.alpha != b->alpha)
+      throw Error(RMD_E_INVALID, replicaLabel(r) + "every replica of a batch needs the same Ewald alpha (the same "
+                                                   "cutoff and tolerance)");
+    b->ldsForward = std::max(b->ldsForward, pmePlaneForwardLds(c->p));
+    b->ldsXlines = std::max(b->ldsXlines, pmeXLinesLds(c->p));
+    b->ldsGather = std::max(b->ldsGather, pmePlaneGatherLds(c->p));
+    b->ldsOpen = std::max(b->ldsOpen, c->ip.clusterLdsBytes);
+  }
+  pmeBatchLdsLimits(b->ldsForward, b->ldsXlines, b->ldsGather);
+  for (int k = 0; k < BK_COUNT; ++k) b->tables[k] = buildTasks(b, k);
+  uploadViews(b);
+}
+
+}  // namespace
+
+#define BATCH_GUARD(batch, ...)                               \
+  do {                                                         \
+    if (!(batch)) {                                            \
+      rmd::setGlobalError("null batch");                       \
+      return RMD_E_INVALID;                                    \
+    }                                                          \
+    try {                                                      \
+      HIP_CHECK(hipSetDevice((batch)->device));                \
+      __VA_ARGS__;                                             \
+      return RMD_OK;                                           \
+    } catch (const rmd::Error &e) {                            \
+      (batch)->lastError = e.what();                           \
+      rmd::setGlobalError(e.what());                           \
+      return e.code;                                           \
+    } catch (const std::exception &e) {                        \
+      (batch)->lastError = e.what();                           \
+      rmd::setGlobalError(e.what());                           \
+      return RMD_E_INVALID;                                    \
+    }                                                          \
+  } while (0)
+
+extern "C" {
+
+int rmd_batch_create(const rmd_system *const *systems, int32_t count, const rmd_options *options,
+                     const uint32_t *seeds, rmd_batch **out) {
+  if (!systems || !out || count <= 0) {
+    setGlobalError(count <= 0 ? "a batch needs at least one replica" : "null argument");
+    return RMD_E_INVALID;
+  }
+  rmd_batch *b = new rmd_batch();
+  try {
+    rmd_options d;
+    rmd_options_default(&d);
+    createBatch(b, systems, count, options ? *options : d, seeds);
+    *out = b;
+    return RMD_OK;
+  } catch (const rmd::Error &e) {
+    setGlobalError(e.what());
+    destroyBatch(b);
+    return e.code;
+  } catch (const std::exception &e) {
+    setGlobalError(e.what());
+    destroyBatch(b);
+    return RMD_E_INVALID;
+  }
+}
+
+void rmd_batch_destroy(rmd_batch *b) { destroyBatch(b); }
+
+const char *rmd_batch_last_error(const rmd_batch *b) { return b ? b->lastError.c_str() : rmd_last_global_error(); }
+
+rmd_ctx *rmd_batch_replica(rmd_batch *b, int32_t replica) {
+  if (!b || replica < 0 || replica >= b->replicas) return nullptr;
+  return b->reps[replica];
+}
+
+int rmd_batch_info_get(rmd_batch *b, rmd_batch_info *out) {
+  BATCH_GUARD(b, {
+    if (!out) throw Error(RMD_E_INVALID, "null output");
+    std::memset(out, 0, sizeof *out);
+    out->replicas = b->replicas;
+    for (rmd_ctx *c : b->reps) out->atoms += c->d.n;
+    out->blocks_run = b->blocksRun;
+    out->step_workgroups = b->tables[BK_OPEN].blocks + b->tables[BK_ATOMS64].blocks * 2 +
+                           b->tables[BK_BONDED].blocks + 2 * b->tables[BK_PLANES].blocks +
+                           b->tables[BK_XLINES].blocks + b->tables[BK_REPLICA].blocks;
+    out->nonbonded_workgroups = b->tables[BK_ATOMS64].blocks;
+  });
+}
+
+int rmd_batch_initialise(rmd_batch *b) {
+  BATCH_GUARD(b, { batchEvaluate(b, false, true); });
+}
+
+int rmd_batch_evaluate(rmd_batch *b, int32_t terms) {
+  BATCH_GUARD(b, {
+    batchEvaluate(b, true, true, terms ? terms : RMD_TERM_ALL);
+    // the replicas' forces now hold only these terms: the next run starts
+    // from a full evaluation
+    for (rmd_ctx *c : b->reps) {
+      c->initialised = false;
+      c->blockEndStep = -1;
+    }
+  });
+}
+
+int rmd_batch_run(rmd_batch *b, int64_t steps, int32_t block, int64_t *stepOut) {
+  BATCH_GUARD(b, {
+    bool ready = true, pressure = false;
+    for (rmd_ctx *c : b->reps) {
+      ready = ready && c->initialised;
+      pressure = pressure || c->hasPressure;
+    }
+    if (!ready) batchEvaluate(b, false, true);
+    const int blockSize = block > 0 ? block : (pressure ? b->reps[0]->barostatInterval : 25);
+    int64_t remaining = steps;
+    while (remaining > 0) {
+      if (b->stop.exchange(0)) throw Error(RMD_E_STOPPED, "the batch was stopped between blocks");
+      batchBarostat(b);
+      const int n = (int)std::min<int64_t>(blockSize, remaining);
+      batchRunBlock(b, n);
+      remaining -= n;
+    }
+    if (stepOut) *stepOut = b->reps[0]->step;
+  });
+}
+
+int rmd_batch_request_stop(rmd_batch *b) {
+  if (!b) return RMD_E_INVALID;
+  b->stop.store(1);
+  return RMD_OK;
+}
+
+int rmd_batch_set_graphs(rmd_batch *b, int32_t enabled) {
+  BATCH_GUARD(b, { b->useGraphs = enabled != 0; });
+}
+
+int rmd_batch_set_concurrent_forces(rmd_batch *b, int32_t enabled) {
+  BATCH_GUARD(b, {
+    b->concurrent = enabled != 0;
+    for (auto &g : b->graphs) (void)hipGraphExecDestroy(g.second);
+    b->graphs.clear();
+  });
+}
+
+// Average duration of the batched nonbonded launch (every replica's direct
+// space in one launch), timed with events on the batch's stream.
+int rmd_batch_profile_nonbonded(rmd_batch *b, int32_t reps, double *avg_ms) {
+  BATCH_GUARD(b, {
+    if (reps <= 0 || !avg_ms) throw Error(RMD_E_INVALID, "reps must be positive");
+    hipStream_t s = b->streams.stream;
+    uploadViews(b);
+    launchNonbondedBatch(b->dViews, b->tables[BK_ATOMS64], (float)b->alpha, 0, s);
+    hipEvent_t e0, e1;
+    HIP_CHECK(hipEventCreate(&e0));
+    HIP_CHECK(hipEventCreate(&e1));
+    HIP_CHECK(hipEventRecord(e0, s));
+    for (int k = 0; k < reps; ++k) launchNonbondedBatch(b->dViews, b->tables[BK_ATOMS64], (float)b->alpha, 0, s);
+    HIP_CHECK(hipEventRecord(e1, s));
+    HIP_CHECK(hipEventSynchronize(e1));
+    float ms = 0;
+    HIP_CHECK(hipEventElapsedTime(&ms, e0, e1));
+    *avg_ms = ms / reps;
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    // the timed launches added forces on top of the state's: evaluate again
+    batchEvaluate(b, false, false);
+  });
 }
 
 }  // extern "C"

@@ -17,6 +17,10 @@
 // launch, so a step is this kernel plus the force kernels.
 #include "kernels.h"
 
+// Multiply-adds fuse only within one expression, so each body rounds the same
+// way wherever it is inlined (the single-system and the batched kernels).
+#pragma clang fp contract(on)
+
 namespace rmd {
 
 struct Atom {
@@ -510,9 +514,10 @@ __device__ void blockMaxToWord(float v, unsigned int *word) {
   __syncthreads();
 }
 
-__global__ void __launch_bounds__(INTEGRATE_BLOCK) integrateOpenKernel(Dev d, IntegratorParams ip, int closing, int early) {
-  extern __shared__ double clusterLds[];
-  const int u = blockIdx.x * blockDim.x + threadIdx.x;
+// One step's opening half for the units of workgroup `block` of `blocks`.
+__device__ __forceinline__ void integrateOpen(const Dev &d, const IntegratorParams &ip, int closing, int early,
+                                              int block, int blocks, double *clusterLds) {
+  const int u = block * blockDim.x + threadIdx.x;
   const double h = ip.halfStep;
   const unsigned int step = (unsigned int)d.sc->step;
   double ke = 0.0;
@@ -610,12 +615,12 @@ __global__ void __launch_bounds__(INTEGRATE_BLOCK) integrateOpenKernel(Dev d, In
     }
   }
   blockMaxToWord(disp, &d.sc->maxDisp2Bits);
-  if (early) blockSumToSlot(ke, &d.slots[E_KINETIC_EARLY * MAX_SLOTS + blockIdx.x]);
+  if (early) blockSumToSlot(ke, &d.slots[E_KINETIC_EARLY * MAX_SLOTS + block]);
   // Last block: the displacement maximum is final; decide the rebuild and
   // advance the noise counter (integrate.js:456-464).
   __shared__ bool last;
   __threadfence();
-  if (threadIdx.x == 0) last = atomicAdd(&d.sc->ticket, 1u) == gridDim.x - 1;
+  if (threadIdx.x == 0) last = atomicAdd(&d.sc->ticket, 1u) == (unsigned int)blocks - 1;
   __syncthreads();
   if (last && threadIdx.x == 0) {
     __threadfence();
@@ -634,9 +639,23 @@ __global__ void __launch_bounds__(INTEGRATE_BLOCK) integrateOpenKernel(Dev d, In
   }
 }
 
-__global__ void __launch_bounds__(INTEGRATE_BLOCK) integrateCloseKernel(Dev d, IntegratorParams ip, int withKinetic) {
+__global__ void __launch_bounds__(INTEGRATE_BLOCK) integrateOpenKernel(Dev d, IntegratorParams ip, int closing, int early) {
   extern __shared__ double clusterLds[];
-  const int u = blockIdx.x * blockDim.x + threadIdx.x;
+  integrateOpen(d, ip, closing, early, blockIdx.x, gridDim.x, clusterLds);
+}
+
+__global__ void __launch_bounds__(INTEGRATE_BLOCK)
+    integrateOpenBatchKernel(const BatchView *__restrict__ v, const int2 *__restrict__ task, int closing, int early) {
+  extern __shared__ double clusterLds[];
+  const int2 t = task[blockIdx.x];
+  if (t.x < 0) return;
+  integrateOpen(v[t.x].d, v[t.x].ip, closing, early, t.y, v[t.x].openBlocks, clusterLds);
+}
+
+// The closing kick + RATTLE of a block's last step.
+__device__ __forceinline__ void integrateClose(const Dev &d, const IntegratorParams &ip, int withKinetic, int block,
+                                               double *clusterLds) {
+  const int u = block * blockDim.x + threadIdx.x;
   const double h = ip.halfStep;
   double ke = 0.0;
   if (u < d.nUnits) {
@@ -689,7 +708,20 @@ __global__ void __launch_bounds__(INTEGRATE_BLOCK) integrateCloseKernel(Dev d, I
       }
     }
   }
-  if (withKinetic) blockSumToSlot(ke, &d.slots[E_KINETIC * MAX_SLOTS + blockIdx.x]);
+  if (withKinetic) blockSumToSlot(ke, &d.slots[E_KINETIC * MAX_SLOTS + block]);
+}
+
+__global__ void __launch_bounds__(INTEGRATE_BLOCK) integrateCloseKernel(Dev d, IntegratorParams ip, int withKinetic) {
+  extern __shared__ double clusterLds[];
+  integrateClose(d, ip, withKinetic, blockIdx.x, clusterLds);
+}
+
+__global__ void __launch_bounds__(INTEGRATE_BLOCK)
+    integrateCloseBatchKernel(const BatchView *__restrict__ v, const int2 *__restrict__ task) {
+  extern __shared__ double clusterLds[];
+  const int2 t = task[blockIdx.x];
+  if (t.x < 0) return;
+  integrateClose(v[t.x].d, v[t.x].ip, 0, t.y, clusterLds);
 }
 
 __global__ void __launch_bounds__(256) kineticKernel(Dev d, int term) {
@@ -703,9 +735,7 @@ __global__ void __launch_bounds__(256) kineticKernel(Dev d, int term) {
 }
 
 // Sum each term's slot row in a fixed order: one block per term.
-__global__ void __launch_bounds__(256) reduceEnergiesKernel(const double *slots, SlotCounts counts, double *out) {
-  const int term = blockIdx.x;
-  const int count = counts.c[term];
+__device__ __forceinline__ void reduceEnergies(const double *slots, int term, int count, double *out) {
   double s = 0.0;
   for (int k = threadIdx.x; k < count; k += blockDim.x) s += slots[term * MAX_SLOTS + k];
   __shared__ double part[256];
@@ -718,18 +748,56 @@ __global__ void __launch_bounds__(256) reduceEnergiesKernel(const double *slots,
   if (threadIdx.x == 0) out[term] = part[0];
 }
 
-__global__ void zeroForcesKernel(Dev d) {
-  const int a = blockIdx.x * blockDim.x + threadIdx.x;
+__global__ void __launch_bounds__(256) reduceEnergiesKernel(const double *slots, SlotCounts counts, double *out) {
+  reduceEnergies(slots, blockIdx.x, counts.c[blockIdx.x], out);
+}
+
+// Batched: block (term, replica). The kinetic rows are summed only at a
+// block's last step (withEarly), as the single-system block does.
+__global__ void __launch_bounds__(256) reduceEnergiesBatchKernel(const BatchView *__restrict__ v, int withEarly) {
+  const int term = blockIdx.x % E_TERM_COUNT, r = blockIdx.x / E_TERM_COUNT;
+  const int count = term == E_KINETIC ? 0 : term == E_KINETIC_EARLY ? (withEarly ? v[r].openBlocks : 0)
+                                                                    : v[r].counts.c[term];
+  reduceEnergies(v[r].d.slots, term, count, v[r].terms);
+}
+
+__device__ __forceinline__ void zeroForces(const Dev &d, int a) {
   if (a >= d.nPad) return;
   d.force[a] = 0;
   d.force[d.nPad + a] = 0;
   d.force[2 * d.nPad + a] = 0;
 }
 
+__global__ void zeroForcesKernel(Dev d) { zeroForces(d, blockIdx.x * blockDim.x + threadIdx.x); }
+
+__global__ void __launch_bounds__(256) zeroForcesBatchKernel(const BatchView *__restrict__ v,
+                                                             const int2 *__restrict__ task) {
+  const int2 t = task[blockIdx.x];
+  if (t.x < 0) return;
+  zeroForces(v[t.x].d, t.y * blockDim.x + threadIdx.x);
+}
+
+// The rebuild decision of an evaluation outside the step loop, one thread per
+// replica (the single-system form lives beside the context).
+__global__ void decideRebuildBatchKernel(const BatchView *__restrict__ v, int replicas, int force) {
+  const int r = blockIdx.x * blockDim.x + threadIdx.x;
+  if (r >= replicas) return;
+  Scalars *sc = v[r].d.sc;
+  const float maxDisp2 = __uint_as_float(sc->maxDisp2Bits);
+  const bool rebuild = force || v[r].rebuild || sc->forceRebuild || !(maxDisp2 <= (float)v[r].ip.dispThreshold2);
+  sc->rebuild = rebuild ? 1u : 0u;
+  if (rebuild) {
+    sc->maxDisp2Bits = 0u;
+    sc->required = 0u;
+    sc->rebuilds += 1u;
+    sc->forceRebuild = 0u;
+  }
+}
+
 // Barostat (webgpu/kernels/barostat.js:25-107): mass-weighted molecule
 // centroids (geometric when massless), then x += (s - 1) * centroid.
-__global__ void centroidKernel(Dev d, const int *molStart, const int *molAtoms, int nMol, float4 *centroids) {
-  const int m = blockIdx.x * blockDim.x + threadIdx.x;
+__device__ __forceinline__ void centroid(const Dev &d, const int *molStart, const int *molAtoms, int nMol,
+                                         float4 *centroids, int m) {
   if (m >= nMol) return;
   double w[3] = {0, 0, 0}, total = 0;
   for (int k = molStart[m]; k < molStart[m + 1]; ++k) {
@@ -757,8 +825,12 @@ __global__ void centroidKernel(Dev d, const int *molStart, const int *molAtoms, 
   centroids[m] = make_float4((float)w[0], (float)w[1], (float)w[2], 0.f);
 }
 
-__global__ void scaleKernel(Dev d, const float4 *centroids, const int *molOfAtom, float factor) {
-  const int a = blockIdx.x * blockDim.x + threadIdx.x;
+__global__ void centroidKernel(Dev d, const int *molStart, const int *molAtoms, int nMol, float4 *centroids) {
+  centroid(d, molStart, molAtoms, nMol, centroids, blockIdx.x * blockDim.x + threadIdx.x);
+}
+
+__device__ __forceinline__ void scaleAtom(const Dev &d, const float4 *centroids, const int *molOfAtom, float factor,
+                                          int a) {
   if (a >= d.n) return;
   const float4 c = centroids[molOfAtom[a]];
   float4 p = d.pos[a];
@@ -766,6 +838,59 @@ __global__ void scaleKernel(Dev d, const float4 *centroids, const int *molOfAtom
   p.y += (factor - 1.0f) * c.y;
   p.z += (factor - 1.0f) * c.z;
   d.pos[a] = p;
+}
+
+__global__ void scaleKernel(Dev d, const float4 *centroids, const int *molOfAtom, float factor) {
+  scaleAtom(d, centroids, molOfAtom, factor, blockIdx.x * blockDim.x + threadIdx.x);
+}
+
+// Batched barostat scaling: replicas whose scale is 1 are left alone.
+__global__ void __launch_bounds__(256) centroidBatchKernel(const BatchView *__restrict__ v,
+                                                           const int2 *__restrict__ task) {
+  const int2 t = task[blockIdx.x];
+  if (t.x < 0 || v[t.x].scale == 1.0f) return;
+  const BatchView &b = v[t.x];
+  centroid(b.d, b.molStart, b.molAtoms, b.nMol, b.centroids, t.y * blockDim.x + threadIdx.x);
+}
+
+__global__ void __launch_bounds__(256) scaleBatchKernel(const BatchView *__restrict__ v,
+                                                        const int2 *__restrict__ task) {
+  const int2 t = task[blockIdx.x];
+  if (t.x < 0 || v[t.x].scale == 1.0f) return;
+  const BatchView &b = v[t.x];
+  scaleAtom(b.d, b.centroids, b.molOfAtom, b.scale, t.y * blockDim.x + threadIdx.x);
+}
+
+// Batched save or restore of positions and forces (and velocities) around a
+// block or a barostat trial; see SNAP_*.
+__global__ void __launch_bounds__(256) snapshotBatchKernel(const BatchView *__restrict__ v,
+                                                           const int2 *__restrict__ task, int mode) {
+  const int2 t = task[blockIdx.x];
+  if (t.x < 0) return;
+  const BatchView &b = v[t.x];
+  if ((mode & SNAP_FLAGGED) && !b.restore) return;
+  const int a = t.y * blockDim.x + threadIdx.x;
+  const int np = b.d.nPad;
+  if (a >= np) return;
+  if (mode & SNAP_RESTORE) {
+    b.d.pos[a] = b.posSaved[a];
+    if (mode & SNAP_VELOCITIES) b.d.vel[a] = b.velSaved[a];
+    for (int c = 0; c < 3; ++c) b.d.force[c * np + a] = b.forceSaved[c * np + a];
+  } else {
+    b.posSaved[a] = b.d.pos[a];
+    if (mode & SNAP_VELOCITIES) b.velSaved[a] = b.d.vel[a];
+    for (int c = 0; c < 3; ++c) b.forceSaved[c * np + a] = b.d.force[c * np + a];
+  }
+}
+
+__global__ void forceRebuildFlaggedBatchKernel(const BatchView *__restrict__ v, int replicas) {
+  const int r = blockIdx.x * blockDim.x + threadIdx.x;
+  if (r < replicas && v[r].restore) v[r].d.sc->forceRebuild = 1u;
+}
+
+__global__ void clearRequiredBatchKernel(Scalars *sc, int replicas) {
+  const int r = blockIdx.x * blockDim.x + threadIdx.x;
+  if (r < replicas) sc[r].requiredMax = 0u;
 }
 
 // Minimiser: the largest per-atom force magnitude, as float bits.
@@ -905,6 +1030,45 @@ void launchDisplace(const Dev &d, float scale, float cap, hipStream_t s) {
 
 void launchConstrainPositions(const Dev &d, const float4 *reference, const IntegratorParams &ip, hipStream_t s) {
   constrainPositionsKernel<<<unitBlocks(d), INTEGRATE_BLOCK, 0, s>>>(d, reference, ip);
+}
+
+// ---- batched launchers ---------------------------------------------------------
+
+void launchIntegrateOpenBatch(const BatchView *v, const TaskTable &t, int closing, int early, int lds, hipStream_t s) {
+  integrateOpenBatchKernel<<<t.blocks, INTEGRATE_BLOCK, lds, s>>>(v, t.task, closing, early);
+}
+
+void launchIntegrateCloseBatch(const BatchView *v, const TaskTable &t, int lds, hipStream_t s) {
+  integrateCloseBatchKernel<<<t.blocks, INTEGRATE_BLOCK, lds, s>>>(v, t.task);
+}
+
+void launchReduceEnergiesBatch(const BatchView *v, int replicas, int withEarly, hipStream_t s) {
+  reduceEnergiesBatchKernel<<<replicas * E_TERM_COUNT, 256, 0, s>>>(v, withEarly);
+}
+
+void launchZeroForcesBatch(const BatchView *v, const TaskTable &t, hipStream_t s) {
+  zeroForcesBatchKernel<<<t.blocks, 256, 0, s>>>(v, t.task);
+}
+
+void launchDecideRebuildBatch(const BatchView *v, int replicas, int force, hipStream_t s) {
+  decideRebuildBatchKernel<<<(replicas + 255) / 256, 256, 0, s>>>(v, replicas, force);
+}
+
+void launchScaleBatch(const BatchView *v, const TaskTable &molecules, const TaskTable &atoms, hipStream_t s) {
+  centroidBatchKernel<<<molecules.blocks, 256, 0, s>>>(v, molecules.task);
+  scaleBatchKernel<<<atoms.blocks, 256, 0, s>>>(v, atoms.task);
+}
+
+void launchSnapshotBatch(const BatchView *v, const TaskTable &atoms, int mode, hipStream_t s) {
+  snapshotBatchKernel<<<atoms.blocks, 256, 0, s>>>(v, atoms.task, mode);
+}
+
+void launchForceRebuildFlaggedBatch(const BatchView *v, int replicas, hipStream_t s) {
+  forceRebuildFlaggedBatchKernel<<<(replicas + 255) / 256, 256, 0, s>>>(v, replicas);
+}
+
+void launchClearRequiredBatch(Scalars *sc, int replicas, hipStream_t s) {
+  clearRequiredBatchKernel<<<(replicas + 255) / 256, 256, 0, s>>>(sc, replicas);
 }
 
 }  // namespace rmd

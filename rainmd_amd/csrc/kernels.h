@@ -87,4 +87,80 @@ void launchMaxForce(const Dev &d, unsigned int *out, hipStream_t s);
 void launchDisplace(const Dev &d, float scale, float cap, hipStream_t s);
 void launchConstrainPositions(const Dev &d, const float4 *reference, const IntegratorParams &ip, hipStream_t s);
 
+// ---- replica batches ----------------------------------------------------------
+// A batch runs R independent systems, each a complete single-system context
+// with buffers of its own (atom indices stay local to the replica). A batched
+// launch covers every replica's workgroups at once: a task table maps each
+// workgroup to (replica, the workgroup's index within that replica's own
+// grid), so the kernel bodies are the single-system ones. A task with
+// replica -1 is padding and returns at once.
+struct BatchView {
+  Dev d;
+  PmeDev p;
+  IntegratorParams ip;
+  int openBlocks;        // the integrator's grid for this replica (its last-block ticket)
+  int rebuild;           // 1: this replica's lists are rebuilt at the coming evaluation
+  float scale;           // the barostat's length scale for this replica (1: no move)
+  int restore;           // 1: the barostat puts this replica back
+  SlotCounts counts;     // energy slot rows a block's last step fills
+  double *terms;         // E_TERM_COUNT reduced energies
+  const int *molStart, *molAtoms, *molOfAtom;
+  int nMol;
+  float4 *centroids, *posSaved, *velSaved;
+  long long *forceSaved;
+  const double *moduli[3];
+  double alpha;
+};
+
+struct TaskTable {
+  int2 *task = nullptr;  // (replica, block within the replica), device memory
+  int blocks = 0;        // grid size (padding included)
+};
+
+enum BatchKernel {
+  BK_OPEN = 0,      // integrator (units)
+  BK_ATOMS64,       // one wave per atom (nonbonded at 64 lanes, neighbour list)
+  BK_BONDED,        // bonded, exception and restraint terms
+  BK_PLANES,        // PME plane kernels (one workgroup per x plane)
+  BK_XLINES,        // PME x lines
+  BK_REPLICA,       // one workgroup per replica (cell lists)
+  BK_ATOMS256,      // 256 atoms per workgroup (zeroing, scaling, copies)
+  BK_MOLECULES,     // 256 molecules per workgroup (centroids)
+  BK_INFLUENCE,     // 256 half-spectrum points per workgroup
+  BK_COUNT
+};
+
+// Batched launchers. `lds` is the largest dynamic LDS any replica needs.
+void launchIntegrateOpenBatch(const BatchView *v, const TaskTable &t, int closing, int early, int lds, hipStream_t s);
+void launchIntegrateCloseBatch(const BatchView *v, const TaskTable &t, int lds, hipStream_t s);
+// withEarly: a block's last step (the in-step kinetic energy is summed too).
+void launchReduceEnergiesBatch(const BatchView *v, int replicas, int withEarly, hipStream_t s);
+void launchZeroForcesBatch(const BatchView *v, const TaskTable &t, hipStream_t s);
+void launchDecideRebuildBatch(const BatchView *v, int replicas, int force, hipStream_t s);
+void launchScaleBatch(const BatchView *v, const TaskTable &molecules, const TaskTable &atoms, hipStream_t s);
+// Saves (or restores) positions and forces, with SNAP_VELOCITIES the
+// velocities too, of every replica (with SNAP_FLAGGED, of those whose
+// `restore` is set).
+enum { SNAP_RESTORE = 1, SNAP_VELOCITIES = 2, SNAP_FLAGGED = 4 };
+void launchSnapshotBatch(const BatchView *v, const TaskTable &atoms, int mode, hipStream_t s);
+// Sets Scalars::forceRebuild of the replicas whose `restore` is set.
+void launchForceRebuildFlaggedBatch(const BatchView *v, int replicas, hipStream_t s);
+// Clears Scalars::requiredMax of `replicas` consecutive scalar blocks.
+void launchClearRequiredBatch(Scalars *sc, int replicas, hipStream_t s);
+// Whether a system's cell list fits the single-workgroup build (a batch needs it).
+bool fitsSmallCellBuild(int atoms, int cells);
+void launchRebuildBatch(const BatchView *v, const TaskTable &replicas, const TaskTable &atoms64, hipStream_t s);
+void launchNonbondedBatch(const BatchView *v, const TaskTable &atoms64, float alpha, int energies, hipStream_t s);
+void launchBondedBatch(const BatchView *v, const TaskTable &t, double alpha, int terms, int energies, hipStream_t s);
+void launchPmeBatch(const BatchView *v, const TaskTable &planes, const TaskTable &xlines, int energies,
+                    size_t forwardLds, size_t xlinesLds, size_t gatherLds, hipStream_t s);
+void launchInfluenceBatch(const BatchView *v, const TaskTable &t, hipStream_t s);
+void pmeBatchLdsLimits(size_t forwardLds, size_t xlinesLds, size_t gatherLds);
+// Per-replica grid sizes and LDS of the PME plane pipeline (for the task tables).
+int pmeXLineBlocks(const PmeDev &p);
+size_t pmePlaneForwardLds(const PmeDev &p);
+size_t pmeXLinesLds(const PmeDev &p);
+size_t pmePlaneGatherLds(const PmeDev &p);
+int bondedBlocks(const Dev &d);
+
 }  // namespace rmd

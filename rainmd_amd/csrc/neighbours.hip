@@ -13,6 +13,10 @@
 // assignment and every pair decision bit for bit.
 #include "kernels.h"
 
+// Multiply-adds fuse only within one expression, so each body rounds the same
+// way wherever it is inlined (the single-system and the batched kernels).
+#pragma clang fp contract(on)
+
 namespace rmd {
 
 constexpr int SORT_CAPACITY = 4096;  // atoms per cell the LDS sort takes
@@ -154,7 +158,7 @@ constexpr int SMALL_THREADS = 1024;
 constexpr int SMALL_MAX_ATOMS = 16384;
 constexpr int SMALL_MAX_CELLS = 4096;
 
-__global__ void __launch_bounds__(SMALL_THREADS) cellsBuildSmallKernel(Dev d) {
+__device__ __forceinline__ void cellsBuildSmall(const Dev &d) {
   if (!d.sc->rebuild) return;
   __shared__ int count[SMALL_MAX_CELLS];
   __shared__ int begin[SMALL_MAX_CELLS + 1];
@@ -213,6 +217,16 @@ __global__ void __launch_bounds__(SMALL_THREADS) cellsBuildSmallKernel(Dev d) {
   }
 }
 
+__global__ void __launch_bounds__(SMALL_THREADS) cellsBuildSmallKernel(Dev d) { cellsBuildSmall(d); }
+
+// Batched: one workgroup per replica.
+__global__ void __launch_bounds__(SMALL_THREADS) cellsBuildSmallBatchKernel(const BatchView *__restrict__ v,
+                                                                           const int2 *__restrict__ task) {
+  const int2 t = task[blockIdx.x];
+  if (t.x < 0) return;
+  cellsBuildSmall(v[t.x].d);
+}
+
 // Per-axis stencil offsets, deduplicated (shared/topology/cells.js:25-29):
 // three or more cells visit {-1, 0, 1}, two visit {0, 1}, one visits {0}.
 __device__ __forceinline__ void axisRange(int count, int &base, int &len) {
@@ -254,12 +268,12 @@ __device__ __forceinline__ bool acceptPair(const Dev &d, const BoxF &box, float4
   return true;
 }
 
-__global__ void __launch_bounds__(256) neighboursKernel(Dev d) {
+__device__ __forceinline__ void neighbours(const Dev &d, int block) {
   if (!d.sc->rebuild) return;
   const BoxF box = *d.box;
   const int *cells = box.cells;
   const int lane = threadIdx.x & 63;
-  const int atom = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+  const int atom = (block * blockDim.x + threadIdx.x) >> 6;
   if (atom >= d.n) return;
   const float4 pi = d.pos[atom];
   if (lane == 0) d.posRebuild[atom] = pi;
@@ -308,6 +322,15 @@ __global__ void __launch_bounds__(256) neighboursKernel(Dev d) {
   }
 }
 
+__global__ void __launch_bounds__(256) neighboursKernel(Dev d) { neighbours(d, blockIdx.x); }
+
+__global__ void __launch_bounds__(256) neighboursBatchKernel(const BatchView *__restrict__ v,
+                                                             const int2 *__restrict__ task) {
+  const int2 t = task[blockIdx.x];
+  if (t.x < 0) return;
+  neighbours(v[t.x].d, t.y);
+}
+
 void launchForceRebuildFlag(const Dev &d, hipStream_t s) { setRebuildKernel<<<1, 1, 0, s>>>(d.sc); }
 
 void launchRebuild(const Dev &d, hipStream_t s) {
@@ -322,5 +345,14 @@ void launchRebuild(const Dev &d, hipStream_t s) {
   }
   neighboursKernel<<<(d.n + 3) / 4, 256, 0, s>>>(d);
 }
+
+// Batched rebuild: every replica's cell list in one workgroup of its own (a
+// batch takes replicas the single-workgroup build fits), then the lists.
+void launchRebuildBatch(const BatchView *v, const TaskTable &replicas, const TaskTable &atoms64, hipStream_t s) {
+  cellsBuildSmallBatchKernel<<<replicas.blocks, SMALL_THREADS, 0, s>>>(v, replicas.task);
+  neighboursBatchKernel<<<atoms64.blocks, 256, 0, s>>>(v, atoms64.task);
+}
+
+bool fitsSmallCellBuild(int atoms, int cells) { return atoms <= SMALL_MAX_ATOMS && cells <= SMALL_MAX_CELLS; }
 
 }  // namespace rmd

@@ -6,16 +6,23 @@
 // FP32. The machinery is MI355X's: a group of `LANES` lanes of a wave64
 // shares one atom's list, each lane's partial force is converted to int64
 // fixed point and the group reduces it exactly, so the sum does not depend
-// on the lane count or the order. Energies are halved (each pair appears in
-// both atoms' lists) and reduced in FP64 into per-block slots.
+// on the lane count or the order. Energies (a separate instantiation, used on
+// the steps that report them) are accumulated per lane in FP64 - a small
+// box's LJ total is a large cancellation that FP32 sums would not keep to
+// 1e-6 - halved (each pair appears in both atoms' lists) and reduced in FP64
+// into per-block slots.
 #include "kernels.h"
+
+// Multiply-adds fuse only within one expression, so each body rounds the same
+// way wherever it is inlined (the single-system and the batched kernels).
+#pragma clang fp contract(on)
 
 namespace rmd {
 
 constexpr int LDS_TYPES = 64;  // type-pair tables up to 64^2 entries live in LDS
 
-template <int LANES>
-__global__ void __launch_bounds__(256) nonbondedKernel(Dev d, float alpha, int energies) {
+template <int LANES, bool ENERGIES>
+__device__ __forceinline__ void nonbonded(const Dev &d, float alpha, int block) {
   // The LJ type-pair table in LDS: one lookup per pair, off the global
   // latency chain (list index -> position and type -> table entry).
   __shared__ float2 ldsTable[LDS_TYPES * LDS_TYPES];
@@ -26,11 +33,12 @@ __global__ void __launch_bounds__(256) nonbondedKernel(Dev d, float alpha, int e
   __syncthreads();
   const float2 *table = tableInLds ? ldsTable : d.lj;
   const BoxF &box = *d.box;
-  const int tid = blockIdx.x * blockDim.x + threadIdx.x;
+  const int tid = block * blockDim.x + threadIdx.x;
   const int atom = tid / LANES;
   const int sub = tid % LANES;
   const bool valid = atom < d.n;
-  float fx = 0.f, fy = 0.f, fz = 0.f, elj = 0.f, ecoul = 0.f, dscale = 0.f;
+  float fx = 0.f, fy = 0.f, fz = 0.f;
+  double elj = 0.0, ecoul = 0.0, dscale = 0.0;
   if (valid) {
     const float4 pi = d.pos[atom];
     const int ti = d.type[atom] * d.types;
@@ -68,10 +76,10 @@ __global__ void __launch_bounds__(256) nonbondedKernel(Dev d, float alpha, int e
       float gaussian;
       const float screened = erfcWithGaussian(alpha * r, gaussian);
       scalar += qq * (screened * invR + gaussScale * gaussian) * invR2;
-      if (energies) {
-        elj += (lj12 - ab.y) * invR6;
-        ecoul += qq * screened * invR;
-        dscale -= scalar * r2;  // r dU/dr
+      if (ENERGIES) {
+        elj += (double)((lj12 - ab.y) * invR6);
+        ecoul += (double)(qq * screened * invR);
+        dscale -= (double)(scalar * r2);  // r dU/dr
       }
       fx += scalar * dx;
       fy += scalar * dy;
@@ -93,11 +101,11 @@ __global__ void __launch_bounds__(256) nonbondedKernel(Dev d, float alpha, int e
     atomicAddFixed(&d.force[d.nPad + atom], iy);
     atomicAddFixed(&d.force[2 * d.nPad + atom], iz);
   }
-  if (energies) {
+  if (ENERGIES) {
     __shared__ double partial[3][4];
-    double a = waveSum(0.5 * (double)elj);
-    double b = waveSum(0.5 * (double)ecoul);
-    double c = waveSum(0.5 * (double)dscale);
+    double a = waveSum(0.5 * elj);
+    double b = waveSum(0.5 * ecoul);
+    double c = waveSum(0.5 * dscale);
     const int wave = threadIdx.x >> 6;
     if ((threadIdx.x & 63) == 0) {
       partial[0][wave] = a;
@@ -112,26 +120,55 @@ __global__ void __launch_bounds__(256) nonbondedKernel(Dev d, float alpha, int e
         sb += partial[1][w];
         sc += partial[2][w];
       }
-      d.slots[E_LJ_DIRECT * MAX_SLOTS + blockIdx.x] = sa;
-      d.slots[E_COUL_DIRECT * MAX_SLOTS + blockIdx.x] = sb;
-      d.slots[E_DSCALE_DIRECT * MAX_SLOTS + blockIdx.x] = sc;
+      d.slots[E_LJ_DIRECT * MAX_SLOTS + block] = sa;
+      d.slots[E_COUL_DIRECT * MAX_SLOTS + block] = sb;
+      d.slots[E_DSCALE_DIRECT * MAX_SLOTS + block] = sc;
     }
+  }
+}
+
+template <int LANES, bool ENERGIES>
+__global__ void __launch_bounds__(256) nonbondedKernel(Dev d, float alpha) {
+  nonbonded<LANES, ENERGIES>(d, alpha, blockIdx.x);
+}
+
+// Batched, one wave per atom.
+template <bool ENERGIES>
+__global__ void __launch_bounds__(256) nonbondedBatchKernel(const BatchView *__restrict__ v,
+                                                            const int2 *__restrict__ task, float alpha) {
+  const int2 t = task[blockIdx.x];
+  if (t.x < 0) return;
+  nonbonded<64, ENERGIES>(v[t.x].d, alpha, t.y);
+}
+
+template <bool ENERGIES>
+static void launchLanes(const Dev &d, float alpha, int lanes, int blocks, hipStream_t s) {
+  switch (lanes) {
+    case 1: nonbondedKernel<1, ENERGIES><<<blocks, 256, 0, s>>>(d, alpha); break;
+    case 2: nonbondedKernel<2, ENERGIES><<<blocks, 256, 0, s>>>(d, alpha); break;
+    case 4: nonbondedKernel<4, ENERGIES><<<blocks, 256, 0, s>>>(d, alpha); break;
+    case 8: nonbondedKernel<8, ENERGIES><<<blocks, 256, 0, s>>>(d, alpha); break;
+    case 16: nonbondedKernel<16, ENERGIES><<<blocks, 256, 0, s>>>(d, alpha); break;
+    case 32: nonbondedKernel<32, ENERGIES><<<blocks, 256, 0, s>>>(d, alpha); break;
+    default: nonbondedKernel<64, ENERGIES><<<blocks, 256, 0, s>>>(d, alpha); break;
   }
 }
 
 int launchNonbonded(const Dev &d, float alpha, int lanes, int energies, hipStream_t s) {
   const long long threads = (long long)d.n * lanes;
   const int blocks = (int)((threads + 255) / 256);
-  switch (lanes) {
-    case 1: nonbondedKernel<1><<<blocks, 256, 0, s>>>(d, alpha, energies); break;
-    case 2: nonbondedKernel<2><<<blocks, 256, 0, s>>>(d, alpha, energies); break;
-    case 4: nonbondedKernel<4><<<blocks, 256, 0, s>>>(d, alpha, energies); break;
-    case 8: nonbondedKernel<8><<<blocks, 256, 0, s>>>(d, alpha, energies); break;
-    case 16: nonbondedKernel<16><<<blocks, 256, 0, s>>>(d, alpha, energies); break;
-    case 32: nonbondedKernel<32><<<blocks, 256, 0, s>>>(d, alpha, energies); break;
-    default: nonbondedKernel<64><<<blocks, 256, 0, s>>>(d, alpha, energies); break;
-  }
+  if (energies)
+    launchLanes<true>(d, alpha, lanes, blocks, s);
+  else
+    launchLanes<false>(d, alpha, lanes, blocks, s);
   return blocks;
+}
+
+void launchNonbondedBatch(const BatchView *v, const TaskTable &atoms64, float alpha, int energies, hipStream_t s) {
+  if (energies)
+    nonbondedBatchKernel<true><<<atoms64.blocks, 256, 0, s>>>(v, atoms64.task, alpha);
+  else
+    nonbondedBatchKernel<false><<<atoms64.blocks, 256, 0, s>>>(v, atoms64.task, alpha);
 }
 
 }  // namespace rmd

@@ -24,6 +24,12 @@
 // Grid sizes may carry a factor 7 (2^a 3^b 5^c 7^d).
 #include "kernels.h"
 
+// Multiply-adds are fused only within one expression (decided in the front
+// end), never across statements by the backend: each kernel body then rounds
+// the same way wherever it is inlined, so a replica batch reproduces the
+// single-system kernels bit for bit.
+#pragma clang fp contract(on)
+
 namespace rmd {
 
 constexpr int ORDER = 5;
@@ -265,7 +271,7 @@ __device__ __forceinline__ double waveVector2(const PmeDev &p, const double *rec
 }
 
 // The block's two energy sums into their slot rows.
-__device__ void storeReciprocal(const Dev &d, double energy, double dscale) {
+__device__ void storeReciprocal(const Dev &d, double energy, double dscale, int block) {
   __shared__ double partial[2][16];
   energy = waveSum(energy);
   dscale = waveSum(dscale);
@@ -281,8 +287,8 @@ __device__ void storeReciprocal(const Dev &d, double energy, double dscale) {
       e += partial[0][w];
       s += partial[1][w];
     }
-    d.slots[E_RECIPROCAL * MAX_SLOTS + blockIdx.x] = e;
-    d.slots[E_DSCALE_RECIPROCAL * MAX_SLOTS + blockIdx.x] = s;
+    d.slots[E_RECIPROCAL * MAX_SLOTS + block] = e;
+    d.slots[E_DSCALE_RECIPROCAL * MAX_SLOTS + block] = s;
   }
 }
 
@@ -331,9 +337,7 @@ __device__ void planeScan(const Dev &d, const PmeDev &p, int plane, int a0, int 
 // 1. Spread onto plane i and forward z, y transforms of the plane. With
 // `real` set, the plane is read from a real K0*K1*K2 grid instead (the FFT
 // verification entry).
-__global__ void __launch_bounds__(PLANE_THREADS) pmePlaneForwardKernel(Dev d, PmeDev p, const float *real) {
-  extern __shared__ float2 lds[];
-  const int i = blockIdx.x;
+__device__ __forceinline__ void pmePlaneForward(const Dev &d, const PmeDev &p, const float *real, int i, float2 *lds) {
   const int K0 = p.K[0], K1 = p.K[1], K2 = p.K[2], K2h = p.K2h;
   const int S = p.planeSpan, P = (K1 + 1) / 2;
   const int t = threadIdx.x, nt = blockDim.x;
@@ -414,15 +418,28 @@ __global__ void __launch_bounds__(PLANE_THREADS) pmePlaneForwardKernel(Dev d, Pm
   }
 }
 
+__global__ void __launch_bounds__(PLANE_THREADS) pmePlaneForwardKernel(Dev d, PmeDev p, const float *real) {
+  extern __shared__ float2 lds[];
+  pmePlaneForward(d, p, real, blockIdx.x, lds);
+}
+
+__global__ void __launch_bounds__(PLANE_THREADS) pmePlaneForwardBatchKernel(const BatchView *__restrict__ v,
+                                                                          const int2 *__restrict__ task) {
+  extern __shared__ float2 lds[];
+  const int2 t = task[blockIdx.x];
+  if (t.x < 0) return;
+  pmePlaneForward(v[t.x].d, v[t.x].p, nullptr, t.y, lds);
+}
+
 // 2. x lines (j, m0 .. m0 + XLINES - 1): forward transform, then, with
 // `convolve`, the influence function (energy in FP64, each half-spectrum
 // column weighted by its multiplicity in the full spectrum) and the inverse
 // transform.
-__global__ void __launch_bounds__(XLINE_THREADS) pmeXLinesKernel(Dev d, PmeDev p, int convolve, int energies) {
-  extern __shared__ float2 lds[];
+__device__ __forceinline__ void pmeXLines(const Dev &d, const PmeDev &p, int convolve, int energies, int block,
+                                          float2 *lds) {
   const int K0 = p.K[0], K1 = p.K[1], K2h = p.K2h;
   const int chunks = (K2h + XLINES - 1) / XLINES;
-  const int j = blockIdx.x / chunks, m0 = (blockIdx.x - j * chunks) * XLINES;
+  const int j = block / chunks, m0 = (block - j * chunks) * XLINES;
   const int lines = min(XLINES, K2h - m0);
   const int t = threadIdx.x, nt = blockDim.x;
   float2 *tw = lds, *A = tw + K0, *B = A + XLINES * K0;
@@ -457,15 +474,26 @@ __global__ void __launch_bounds__(XLINE_THREADS) pmeXLinesKernel(Dev d, PmeDev p
     const int x = idx / lines, l = idx - x * lines;
     col[x * stride + l] = R[l * K0 + x];
   }
-  if (convolve && energies) storeReciprocal(d, energy, dscale);
+  if (convolve && energies) storeReciprocal(d, energy, dscale, block);
+}
+
+__global__ void __launch_bounds__(XLINE_THREADS) pmeXLinesKernel(Dev d, PmeDev p, int convolve, int energies) {
+  extern __shared__ float2 lds[];
+  pmeXLines(d, p, convolve, energies, blockIdx.x, lds);
+}
+
+__global__ void __launch_bounds__(XLINE_THREADS) pmeXLinesBatchKernel(const BatchView *__restrict__ v,
+                                                                    const int2 *__restrict__ task, int energies) {
+  extern __shared__ float2 lds[];
+  const int2 t = task[blockIdx.x];
+  if (t.x < 0) return;
+  pmeXLines(v[t.x].d, v[t.x].p, 1, energies, t.y, lds);
 }
 
 // 3. Inverse y and z transforms of plane i into the real potential, then each
 // atom whose support meets the plane takes that plane's share of its force
 // (webgpu/kernels/pme.js:281-336; cpu/forces/pme.js:222-257).
-__global__ void __launch_bounds__(PLANE_THREADS) pmePlaneGatherKernel(Dev d, PmeDev p) {
-  extern __shared__ float2 lds[];
-  const int i = blockIdx.x;
+__device__ __forceinline__ void pmePlaneGather(const Dev &d, const PmeDev &p, int i, float2 *lds) {
   const int K0 = p.K[0], K1 = p.K[1], K2 = p.K[2], K2h = p.K2h;
   const int S = p.planeSpan, P = (K1 + 1) / 2;
   const int t = threadIdx.x, nt = blockDim.x;
@@ -553,6 +581,19 @@ __global__ void __launch_bounds__(PLANE_THREADS) pmePlaneGatherKernel(Dev d, Pme
     }
     __syncthreads();
   }
+}
+
+__global__ void __launch_bounds__(PLANE_THREADS) pmePlaneGatherKernel(Dev d, PmeDev p) {
+  extern __shared__ float2 lds[];
+  pmePlaneGather(d, p, blockIdx.x, lds);
+}
+
+__global__ void __launch_bounds__(PLANE_THREADS) pmePlaneGatherBatchKernel(const BatchView *__restrict__ v,
+                                                                         const int2 *__restrict__ task) {
+  extern __shared__ float2 lds[];
+  const int2 t = task[blockIdx.x];
+  if (t.x < 0) return;
+  pmePlaneGather(v[t.x].d, v[t.x].p, t.y, lds);
 }
 
 // ============================================================================
@@ -682,7 +723,7 @@ __global__ void __launch_bounds__(256) pmeXConvolveKernel(Dev d, PmeDev p, int c
     const int g = idx / K, k = idx - g * K;
     p.cgrid[(size_t)(first + g) + k * stride] = r[idx];
   }
-  if (convolve && energies) storeReciprocal(d, energy, dscale);
+  if (convolve && energies) storeReciprocal(d, energy, dscale, blockIdx.x);
 }
 
 // z pass, inverse: Hermitian extension of two half spectra into one complex
@@ -760,12 +801,11 @@ __global__ void __launch_bounds__(256) pmeGatherKernel(Dev d, PmeDev p) {
 // The influence function over the half spectrum, in FP64, stored as f32:
 // pref * exp(-pi^2 m^2 / alpha^2) / (m^2 * B(m)) with the reference's row
 // convention for the wave vector (cpu/forces/pme.js:176-209).
-__global__ void pmeInfluenceKernel(PmeDev p, const BoxF *box, const double *mod0, const double *mod1,
-                                   const double *mod2, double alpha) {
+__device__ __forceinline__ void pmeInfluence(const PmeDev &p, const BoxF *box, const double *mod0, const double *mod1,
+                                             const double *mod2, double alpha, size_t idx) {
   const double *recip = box->recip64;
   const double *bx = box->box64;
   const double volume = bx[0] * bx[4] * bx[8];
-  const size_t idx = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
   const size_t total = (size_t)p.K[0] * p.K[1] * p.K2h;
   if (idx >= total) return;
   const int m = idx % p.K2h;
@@ -788,6 +828,21 @@ __global__ void pmeInfluenceKernel(PmeDev p, const BoxF *box, const double *mod0
   p.influence[idx] = (float)(pref * exp(-factor * m2) / (m2 * modulus));
 }
 
+__global__ void pmeInfluenceKernel(PmeDev p, const BoxF *box, const double *mod0, const double *mod1,
+                                   const double *mod2, double alpha) {
+  pmeInfluence(p, box, mod0, mod1, mod2, alpha, (size_t)blockIdx.x * blockDim.x + threadIdx.x);
+}
+
+// Batched: the replicas whose box changed (`rebuild` set by the barostat).
+__global__ void __launch_bounds__(256) pmeInfluenceBatchKernel(const BatchView *__restrict__ v,
+                                                               const int2 *__restrict__ task) {
+  const int2 t = task[blockIdx.x];
+  if (t.x < 0 || !v[t.x].rebuild) return;
+  const BatchView &b = v[t.x];
+  pmeInfluence(b.p, b.d.box, b.moduli[0], b.moduli[1], b.moduli[2], b.alpha,
+               (size_t)t.y * blockDim.x + threadIdx.x);
+}
+
 void pmeComputeInfluence(const PmeDev &p, const BoxF *box, const double *moduli[3], double alpha, hipStream_t s) {
   const size_t total = (size_t)p.K[0] * p.K[1] * p.K2h;
   pmeInfluenceKernel<<<(unsigned)((total + 255) / 256), 256, 0, s>>>(p, box, moduli[0], moduli[1], moduli[2], alpha);
@@ -798,13 +853,13 @@ void pmeComputeInfluence(const PmeDev &p, const BoxF *box, const double *moduli[
 // LDS: twiddles, two complex work buffers (the second also holds the int64
 // charge plane), then the compacted atom list and its count.
 static size_t listBytes() { return (size_t)(PLANE_LIST + 4) * sizeof(int); }
-static size_t planeForwardLds(const PmeDev &p) {
+size_t planeForwardLds(const PmeDev &p) {
   return (size_t)(p.K[1] + p.K[2] + p.planeSpan + p.planeRegion) * sizeof(float2) + listBytes();
 }
-static size_t planeGatherLds(const PmeDev &p) {
+size_t planeGatherLds(const PmeDev &p) {
   return (size_t)(p.K[1] + p.K[2]) * sizeof(float2) + 2 * (size_t)p.planeSpan * sizeof(float2) + listBytes();
 }
-static size_t xLinesLds(const PmeDev &p) { return (size_t)p.K[0] * sizeof(float2) * (1 + 2 * XLINES); }
+size_t xLinesLds(const PmeDev &p) { return (size_t)p.K[0] * sizeof(float2) * (1 + 2 * XLINES); }
 static size_t lineLds(int K) { return (size_t)K * sizeof(float2) * (1 + 2 * LINES_PER_BLOCK); }
 
 bool pmeConfigure(PmeDev &p, int mode, size_t maxLds) {
@@ -826,6 +881,32 @@ bool pmeConfigure(PmeDev &p, int mode, size_t maxLds) {
 }
 
 static int xLineBlocks(const PmeDev &p) { return p.K[1] * ((p.K2h + XLINES - 1) / XLINES); }
+int pmeXLineBlocks(const PmeDev &p) { return xLineBlocks(p); }
+size_t pmePlaneForwardLds(const PmeDev &p) { return planeForwardLds(p); }
+size_t pmeXLinesLds(const PmeDev &p) { return xLinesLds(p); }
+size_t pmePlaneGatherLds(const PmeDev &p) { return planeGatherLds(p); }
+
+// Batched plane pipeline; the LDS sizes are the largest over the replicas and
+// the kernels' LDS limits were raised to them at batch creation.
+void launchPmeBatch(const BatchView *v, const TaskTable &planes, const TaskTable &xlines, int energies,
+                    size_t forwardLds, size_t xlinesLds, size_t gatherLds, hipStream_t s) {
+  pmePlaneForwardBatchKernel<<<planes.blocks, PLANE_THREADS, forwardLds, s>>>(v, planes.task);
+  pmeXLinesBatchKernel<<<xlines.blocks, XLINE_THREADS, xlinesLds, s>>>(v, xlines.task, energies);
+  pmePlaneGatherBatchKernel<<<planes.blocks, PLANE_THREADS, gatherLds, s>>>(v, planes.task);
+}
+
+void launchInfluenceBatch(const BatchView *v, const TaskTable &t, hipStream_t s) {
+  pmeInfluenceBatchKernel<<<t.blocks, 256, 0, s>>>(v, t.task);
+}
+
+void pmeBatchLdsLimits(size_t forwardLds, size_t xlinesLds, size_t gatherLds) {
+  (void)hipFuncSetAttribute((const void *)pmePlaneForwardBatchKernel, hipFuncAttributeMaxDynamicSharedMemorySize,
+                            (int)forwardLds);
+  (void)hipFuncSetAttribute((const void *)pmeXLinesBatchKernel, hipFuncAttributeMaxDynamicSharedMemorySize,
+                            (int)xlinesLds);
+  (void)hipFuncSetAttribute((const void *)pmePlaneGatherBatchKernel, hipFuncAttributeMaxDynamicSharedMemorySize,
+                            (int)gatherLds);
+}
 
 int launchPme(const Dev &d, const PmeDev &p, int energies, hipStream_t s, int *slotsUsed) {
   if (p.planes) {

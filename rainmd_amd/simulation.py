@@ -320,6 +320,39 @@ def _struct_dict(s):
 
 # ---- the simulation -------------------------------------------------------
 
+def _options(options):
+    """rmd_options from the reference's constructor options; returns the
+    options and the arrays they point into (kept alive by the caller)."""
+    lib = L.load()
+    o = L.Options()
+    lib.rmd_options_default(C.byref(o))
+    simple = ["timestep", "temperature", "friction", "seed", "tolerance", "max_constraint_iterations",
+              "rebuild_interval", "barostat_interval", "skin", "neighbour_slack", "max_neighbours",
+              "ewald_tolerance", "alpha", "device", "lanes_per_atom"]
+    for key in simple:
+        if options.get(key) is not None:
+            setattr(o, key, options[key])
+    if options.get("adaptive_rebuild") is not None:
+        o.adaptive_rebuild = 1 if options["adaptive_rebuild"] else 0
+    if options.get("dispersion_correction") is not None:
+        o.dispersion_correction = 1 if options["dispersion_correction"] else 0
+    if options.get("pressure") is not None:
+        o.has_pressure = 1
+        o.pressure = options["pressure"]
+    if options.get("grid") is not None:
+        o.grid[:] = list(options["grid"])
+    keep = []
+    if options.get("restraint_force"):
+        atoms = np.ascontiguousarray(options["restraint_atoms"], dtype=np.int32)
+        ref = np.ascontiguousarray(options["restraint_reference"], dtype=np.float64)
+        keep += [atoms, ref]
+        o.restraint_force = options["restraint_force"]
+        o.restraint_count = len(atoms)
+        o.restraint_atoms = atoms.ctypes.data_as(L._ip)
+        o.restraint_reference = ref.ctypes.data_as(L._dp)
+    return o, keep
+
+
 class GpuSimulation:
     """new GpuSimulation(device, system, options) (webgpu/simulation.js:83-190).
 
@@ -334,36 +367,21 @@ class GpuSimulation:
     def __init__(self, system, **options):
         lib = L.load()
         self.system = system
-        o = L.Options()
-        lib.rmd_options_default(C.byref(o))
-        simple = ["timestep", "temperature", "friction", "seed", "tolerance", "max_constraint_iterations",
-                  "rebuild_interval", "barostat_interval", "skin", "neighbour_slack", "max_neighbours",
-                  "ewald_tolerance", "alpha", "device", "lanes_per_atom"]
-        for key in simple:
-            if options.get(key) is not None:
-                setattr(o, key, options[key])
-        if options.get("adaptive_rebuild") is not None:
-            o.adaptive_rebuild = 1 if options["adaptive_rebuild"] else 0
-        if options.get("dispersion_correction") is not None:
-            o.dispersion_correction = 1 if options["dispersion_correction"] else 0
-        if options.get("pressure") is not None:
-            o.has_pressure = 1
-            o.pressure = options["pressure"]
-        if options.get("grid") is not None:
-            o.grid[:] = list(options["grid"])
-        self._keep = []
-        if options.get("restraint_force"):
-            atoms = np.ascontiguousarray(options["restraint_atoms"], dtype=np.int32)
-            ref = np.ascontiguousarray(options["restraint_reference"], dtype=np.float64)
-            self._keep += [atoms, ref]
-            o.restraint_force = options["restraint_force"]
-            o.restraint_count = len(atoms)
-            o.restraint_atoms = atoms.ctypes.data_as(L._ip)
-            o.restraint_reference = ref.ctypes.data_as(L._dp)
+        o, self._keep = _options(options)
         h = C.c_void_p()
         _check(lib.rmd_create(system._h, C.byref(o), C.byref(h)))
         self._h = h
         self.atom_count = system.atom_count
+
+    @classmethod
+    def _member(cls, handle, system, owner):
+        """A batch member's own context (not owned: the batch frees it)."""
+        sim = cls.__new__(cls)
+        sim.system = system
+        sim._keep = [owner]
+        sim._h = handle
+        sim.atom_count = system.atom_count
+        return sim
 
     # -- state --
     def set_state(self, positions, velocities=None, step=0, supplied_noise=False):
@@ -538,6 +556,83 @@ class GpuSimulation:
         self.release()
 
 
+class Batch:
+    """A batch of replicas stepped together (include/rainmd.h, rmd_batch_*):
+    many independent systems, each with its own box, PME grid, seed and
+    barostat, advanced by batched launches. replica(r) is replica r's own
+    GpuSimulation (set_state, minimise, energy, read_state, diagnostics, and
+    a run of that replica alone)."""
+
+    def __init__(self, systems, seeds=None, **options):
+        lib = L.load()
+        self.systems = list(systems)
+        self.replicas = len(self.systems)
+        o, self._keep = _options(options)
+        handles = (C.c_void_p * self.replicas)(*[sys_._h.value if isinstance(sys_._h, C.c_void_p) else sys_._h
+                                                 for sys_ in self.systems])
+        sd = None if seeds is None else (C.c_uint32 * self.replicas)(*[int(x) for x in seeds])
+        h = C.c_void_p()
+        _check(lib.rmd_batch_create(handles, self.replicas, C.byref(o), sd, C.byref(h)))
+        self._h = h
+        self._members = {}
+
+    def _check(self, rc):
+        if rc == L.RMD_OK:
+            return
+        msg = L.load().rmd_batch_last_error(self._h) or b""
+        raise _ERRORS.get(rc, RainmdError)(msg.decode(errors="replace"))
+
+    def replica(self, r):
+        if r not in self._members:
+            handle = L.load().rmd_batch_replica(self._h, int(r))
+            if not handle:
+                raise RainmdError(f"no replica {r} in a batch of {self.replicas}")
+            self._members[r] = GpuSimulation._member(C.c_void_p(handle), self.systems[r], self)
+        return self._members[r]
+
+    def initialise(self):
+        self._check(L.load().rmd_batch_initialise(self._h))
+
+    def evaluate(self, terms=L.TERM_ALL):
+        """Evaluate `terms` for every replica; forces via replica(r).read_state()."""
+        self._check(L.load().rmd_batch_evaluate(self._h, int(terms)))
+
+    def run(self, count, block_size=0):
+        step = C.c_int64()
+        self._check(L.load().rmd_batch_run(self._h, int(count), int(block_size), C.byref(step)))
+        return step.value
+
+    def request_stop(self):
+        L.load().rmd_batch_request_stop(self._h)
+
+    def set_graphs(self, enabled):
+        self._check(L.load().rmd_batch_set_graphs(self._h, 1 if enabled else 0))
+
+    def set_concurrent_forces(self, enabled):
+        self._check(L.load().rmd_batch_set_concurrent_forces(self._h, 1 if enabled else 0))
+
+    def profile_nonbonded(self, reps=20):
+        ms = C.c_double()
+        self._check(L.load().rmd_batch_profile_nonbonded(self._h, int(reps), C.byref(ms)))
+        return ms.value
+
+    def info(self):
+        i = L.BatchInfo()
+        self._check(L.load().rmd_batch_info_get(self._h, C.byref(i)))
+        return {k: getattr(i, k) for k, _ in L.BatchInfo._fields_}
+
+    def release(self):
+        if getattr(self, "_h", None):
+            for m in self._members.values():
+                m._h = None
+            self._members = {}
+            L.load().rmd_batch_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        self.release()
+
+
 # ---- initial velocities ----------------------------------------------------
 
 def _mix(v):
@@ -655,3 +750,27 @@ WORKLOADS = {
 
 def workload_system(name):
     return System.solvated(**WORKLOADS[name])
+
+
+# The replica workload: `count` replicas of the ACE-ALA-NME + TIP3P composition
+# with cube edges drawn uniformly from [2.2, 5.0] nm by the counter hash
+# (seed 404, replica r at counter r), each filled with waters to 1.000 g/cm^3
+# (ACE-ALA-NME 144.17 g/mol, TIP3P 18.016 g/mol) and built with seed 4000 + r.
+REPLICA_SOLUTE_MASS = 144.17
+REPLICA_WATER_MASS = 18.016
+
+
+def replica_workload(count=256, seed=404, low=2.2, high=5.0):
+    """[(edge_nm, water_count, seed)] of the replica workload."""
+    out = []
+    for r in range(count):
+        edge = low + (high - low) * float(uniform_at(seed, r, 0, 0))
+        mass = 602.214076 * edge ** 3  # g/mol in edge^3 nm^3 at 1 g/cm^3
+        waters = int(round((mass - REPLICA_SOLUTE_MASS) / REPLICA_WATER_MASS))
+        out.append((edge, waters, 4000 + r))
+    return out
+
+
+def replica_systems(count=256, seed=404, cutoff=0.9):
+    return [System.solvated(water_count=w, box_edge=e, seed=s, include_solute=True, cutoff=cutoff)
+            for e, w, s in replica_workload(count, seed)]
