@@ -158,9 +158,12 @@ def bench_replicas(args, count):
     atoms = [s.atom_count for s in systems]
     log(f"{count} replicas, {sum(atoms)} atoms ({min(atoms)}-{max(atoms)}), built in {time.perf_counter() - t0:.1f} s")
     t0 = time.perf_counter()
-    batch = R.Batch(systems, seeds=seeds, temperature=300.0, pressure=1.0, barostat_interval=25)
+    extra = dict(skin=args.replica_skin) if args.replica_skin > 0 else {}
+    batch = R.Batch(systems, seeds=seeds, temperature=300.0, pressure=1.0, barostat_interval=25, **extra)
     if args.no_graphs:
         batch.set_graphs(False)
+    if args.serial_forces:
+        batch.set_concurrent_forces(False)
     log(f"batch created in {time.perf_counter() - t0:.1f} s: {batch.info()}")
     t0 = time.perf_counter()
     for r, system in enumerate(systems):
@@ -282,6 +285,9 @@ def main():
     ap.add_argument("--single-steps", type=int, default=100,
                     help="replica workloads: steps of each replica alone for the one-system-per-launch baseline")
     ap.add_argument("--replica-minimise", type=int, default=150)
+    ap.add_argument("--serial-forces", action="store_true",
+                    help="run the force branches one after another (for per-kernel profiles)")
+    ap.add_argument("--replica-skin", type=float, default=0.0, help="replica workloads: nm; 0 derives skinFor")
     args = ap.parse_args()
 
     if args.gpus != 1:

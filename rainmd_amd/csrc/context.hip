@@ -206,6 +206,7 @@ BoxF boxFor(const rmd_ctx *c) {
   b.periodic = 1;
   for (int k = 0; k < 3; ++k) b.cells[k] = c->plan.counts[k];
   b.cellTotal = c->plan.cellCount;
+  b.reach = c->plan.reach;
   return b;
 }
 

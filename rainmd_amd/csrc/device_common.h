@@ -36,6 +36,7 @@ struct BoxF {
   double recip64[9];
   int cells[3];
   int cellTotal;
+  int reach;  // stencil cells either side (cells are at least rl / reach wide)
 };
 
 __device__ __forceinline__ long long toFixed(double v) { return static_cast<long long>(__double2ll_rn(v * FORCE_SCALE)); }

@@ -131,9 +131,17 @@ struct CellPlan {
   double skin = 0, cutoff = 0, listCutoff = 0;
   int counts[3] = {1, 1, 1};
   int cellCount = 1;
+  // Cells are at least listCutoff / reach wide and the search visits reach
+  // cells either side (reach 1 is the reference's plan).
+  int reach = 1;
   int maxNeighbours = 32;
   double estimatedNeighbours = 0;
 };
+// The neighbour search's limits: the widest stencil reach, and the cell and
+// atom counts of the single-workgroup cell build.
+constexpr int CELL_REACH_MAX = 3;
+constexpr int CELL_LIMIT_SMALL = 4096;
+constexpr int CELL_ATOMS_SMALL = 16384;
 CellPlan planCellList(const System &system, double skin, double slack, int maxNeighbours,
                       const double *box);
 

@@ -580,7 +580,31 @@ CellPlan planCellList(const System &s, double skin, double slack, int maxNeighbo
   p.cutoff = s.cutoff;
   p.listCutoff = s.cutoff + skin;
   const double widths[3] = {b[0], b[4], b[8]};
-  for (int k = 0; k < 3; ++k) p.counts[k] = std::max(1, static_cast<int>(std::floor(std::fabs(widths[k]) / p.listCutoff)));
+  // The reference's cells are at least one list cutoff wide (reach 1), so in
+  // a box under four cutoffs wide the 3-cell stencil covers every cell and
+  // the search is all-pairs. Cells of listCutoff / reach with a stencil of
+  // reach cells either side find the same pairs; the plan takes the reach
+  // (up to 3) that leaves the fewest candidates, keeping the reference's plan
+  // when a finer one would not cull, and at most CELL_LIMIT_SMALL cells for
+  // systems the single-workgroup build takes.
+  const int n = s.topology.atomCount;
+  double best = 2.0;
+  for (int reach = 1; reach <= CELL_REACH_MAX; ++reach) {
+    int counts[3];
+    double fraction = 1.0;
+    long long cells = 1;
+    for (int k = 0; k < 3; ++k) {
+      counts[k] = std::max(1, static_cast<int>(std::floor(std::fabs(widths[k]) * reach / p.listCutoff)));
+      fraction *= (double)std::min(2 * reach + 1, counts[k]) / counts[k];
+      cells *= counts[k];
+    }
+    if (reach > 1 && n <= CELL_ATOMS_SMALL && cells > CELL_LIMIT_SMALL) break;
+    if (reach == 1 || fraction < best - 1e-12) {
+      best = fraction;
+      p.reach = reach;
+      for (int k = 0; k < 3; ++k) p.counts[k] = counts[k];
+    }
+  }
   p.cellCount = p.counts[0] * p.counts[1] * p.counts[2];
   const double volume = std::fabs(b[0] * b[4] * b[8]);
   const double density = s.topology.atomCount / volume;

@@ -227,18 +227,20 @@ __global__ void __launch_bounds__(SMALL_THREADS) cellsBuildSmallBatchKernel(cons
   cellsBuildSmall(v[t.x].d);
 }
 
-// Per-axis stencil offsets, deduplicated (shared/topology/cells.js:25-29):
-// three or more cells visit {-1, 0, 1}, two visit {0, 1}, one visits {0}.
-__device__ __forceinline__ void axisRange(int count, int &base, int &len) {
-  if (count >= 3) {
-    base = -1;
-    len = 3;
-  } else if (count == 2) {
-    base = 0;
-    len = 2;
+// Per-axis stencil (shared/topology/cells.js:25-29, generalised to cells of
+// rl / reach): `reach` cells either side when the axis has at least
+// 2 * reach + 1 cells, else every cell of the axis once.
+constexpr int REACH_MAX = 3;
+constexpr int STENCIL_ROWS = (2 * REACH_MAX + 1) * (2 * REACH_MAX + 1);
+constexpr int STENCIL_RANGES = 2 * STENCIL_ROWS;  // a row's z run splits in two where it wraps
+
+__device__ __forceinline__ void axisStencil(int count, int reach, int &base, int &len) {
+  if (count >= 2 * reach + 1) {
+    base = -reach;
+    len = 2 * reach + 1;
   } else {
     base = 0;
-    len = 1;
+    len = count;
   }
 }
 
@@ -247,14 +249,9 @@ __device__ __forceinline__ int wrapCell(int v, int count) {
   return v < 0 ? v + count : v;
 }
 
-// One wave per atom. Candidates are read 128 at a time (two per lane, both
-// loads in flight together) from the cell-ordered positions of each stencil
-// cell; a ballot compacts the accepted ones into the atom's slice in
-// candidate order. Exclusions are checked only for candidates inside the
-// range of the atom's (sorted) exclusion partners. Every pair found is
-// counted; only `cap` are stored, and the true requirement reaches the host
-// through Scalars::required (the reference's overflow contract,
-// webgpu/kernels/neighbours.js:128-146).
+// Whether candidate pj joins atom's list: not itself, within rl (the FP32
+// minimum image of the kernels), not excluded. Exclusions are checked only
+// for candidates inside the range of the atom's (sorted) exclusion partners.
 __device__ __forceinline__ bool acceptPair(const Dev &d, const BoxF &box, float4 pi, float4 pj, int atom, int exBegin,
                                            int exEnd, int exMin, int exMax) {
   const int j = __float_as_int(pj.w);
@@ -268,11 +265,23 @@ __device__ __forceinline__ bool acceptPair(const Dev &d, const BoxF &box, float4
   return true;
 }
 
+// One wave per atom. The stencil is a set of z runs, one per (x, y) stencil
+// row, each a contiguous range of the cell-ordered positions (two where the
+// run wraps). The wave writes the ranges and their prefix sums to LDS and
+// streams the concatenated candidates 128 at a time (two per lane, both loads
+// in flight together), each lane keeping a cursor into the ranges; a ballot
+// compacts the accepted ones into the atom's slice in candidate order, so the
+// list is the same on every run. Every pair found is counted; only `cap` are
+// stored, and the true requirement reaches the host through
+// Scalars::required (the reference's overflow contract,
+// webgpu/kernels/neighbours.js:128-146).
 __device__ __forceinline__ void neighbours(const Dev &d, int block) {
   if (!d.sc->rebuild) return;
+  __shared__ int rangeBegin[4][STENCIL_RANGES];
+  __shared__ int rangeStart[4][STENCIL_RANGES + 1];
   const BoxF box = *d.box;
   const int *cells = box.cells;
-  const int lane = threadIdx.x & 63;
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const int atom = (block * blockDim.x + threadIdx.x) >> 6;
   if (atom >= d.n) return;
   const float4 pi = d.pos[atom];
@@ -282,10 +291,63 @@ __device__ __forceinline__ void neighbours(const Dev &d, int block) {
   const int exBegin = d.exclStart[atom], exEnd = d.exclStart[atom + 1];
   const int exMin = exBegin < exEnd ? d.exclPartners[exBegin] : 1;
   const int exMax = exBegin < exEnd ? d.exclPartners[exEnd - 1] : 0;
+  const int reach = box.reach;
   int bx, nx, by, ny, bz, nz;
-  axisRange(cells[0], bx, nx);
-  axisRange(cells[1], by, ny);
-  axisRange(cells[2], bz, nz);
+  axisStencil(cells[0], reach, bx, nx);
+  axisStencil(cells[1], reach, by, ny);
+  axisStencil(cells[2], reach, bz, nz);
+  // the z segments, the same for every row: the whole row, or [lo, hi)
+  // split where it wraps
+  int za0, zb0, za1 = 0, zb1 = 0, segs = 1;
+  if (nz >= cells[2]) {
+    za0 = 0;
+    zb0 = cells[2];
+  } else {
+    const int lo = c3[2] + bz, hi = lo + nz;
+    if (lo < 0) {
+      segs = 2;
+      za0 = 0, zb0 = hi;
+      za1 = lo + cells[2], zb1 = cells[2];
+    } else if (hi > cells[2]) {
+      segs = 2;
+      za0 = 0, zb0 = hi - cells[2];
+      za1 = lo, zb1 = cells[2];
+    } else {
+      za0 = lo, zb0 = hi;
+    }
+  }
+  const int ranges = nx * ny * segs;
+  int *rb = rangeBegin[wave], *rs = rangeStart[wave];
+  // ranges r = lane and lane + 64: their slots and lengths, then an exclusive
+  // prefix over all of them
+  int len[2] = {0, 0};
+  for (int h = 0; h < 2; ++h) {
+    const int r = lane + 64 * h;
+    if (r < ranges) {
+      const int row = r / segs, seg = r - row * segs;
+      const int cx = wrapCell(c3[0] + bx + row / ny, cells[0]);
+      const int cy = wrapCell(c3[1] + by + row % ny, cells[1]);
+      const int rowBase = (cx * cells[1] + cy) * cells[2];
+      const int begin = d.cellStart[rowBase + (seg ? za1 : za0)];
+      const int end = d.cellStart[rowBase + (seg ? zb1 : zb0)];
+      rb[r] = begin;
+      len[h] = end - begin;
+    }
+  }
+  int carry = 0;
+  for (int h = 0; h < 2; ++h) {
+    int inclusive = len[h];
+    for (int offset = 1; offset < 64; offset <<= 1) {
+      const int v = __shfl_up(inclusive, offset, 64);
+      if (lane >= offset) inclusive += v;
+    }
+    const int r = lane + 64 * h;
+    if (r < ranges) rs[r] = carry + inclusive - len[h];
+    carry += __shfl(inclusive, 63, 64);
+  }
+  if (lane == 0) rs[ranges] = carry;
+  __builtin_amdgcn_wave_barrier();
+  const int total = carry;
   int found = 0;
   int *out = d.nbrList + (size_t)atom * d.cap;
   const float4 none = make_float4(0.f, 0.f, 0.f, __int_as_float(atom));
@@ -297,24 +359,23 @@ __device__ __forceinline__ void neighbours(const Dev &d, int block) {
     }
     found += __popcll(mask);
   };
-  for (int ox = 0; ox < nx; ++ox) {
-    const int cx = wrapCell(c3[0] + bx + ox, cells[0]);
-    for (int oy = 0; oy < ny; ++oy) {
-      const int cy = wrapCell(c3[1] + by + oy, cells[1]);
-      for (int oz = 0; oz < nz; ++oz) {
-        const int cz = wrapCell(c3[2] + bz + oz, cells[2]);
-        const int cell = (cx * cells[1] + cy) * cells[2] + cz;
-        const int begin = d.cellStart[cell], end = d.cellStart[cell + 1];
-        for (int base = begin; base < end; base += 128) {
-          const int s0 = base + lane, s1 = base + 64 + lane;
-          const float4 p0 = s0 < end ? d.cellPos[s0] : none;
-          const float4 p1 = s1 < end ? d.cellPos[s1] : none;
-          emit(acceptPair(d, box, pi, p0, atom, exBegin, exEnd, exMin, exMax), __float_as_int(p0.w));
-          if (base + 64 < end) emit(acceptPair(d, box, pi, p1, atom, exBegin, exEnd, exMin, exMax), __float_as_int(p1.w));
-        }
-      }
+  int cur = 0;  // the range of this lane's first candidate of the batch
+  for (int base = 0; base < total; base += 128) {
+    const int g0 = base + lane, g1 = g0 + 64;
+    float4 p0 = none, p1 = none;
+    if (g0 < total) {
+      while (rs[cur + 1] <= g0) ++cur;
+      p0 = d.cellPos[rb[cur] + (g0 - rs[cur])];
     }
+    if (g1 < total) {
+      int c1 = cur;
+      while (rs[c1 + 1] <= g1) ++c1;
+      p1 = d.cellPos[rb[c1] + (g1 - rs[c1])];
+    }
+    emit(acceptPair(d, box, pi, p0, atom, exBegin, exEnd, exMin, exMax), __float_as_int(p0.w));
+    if (base + 64 < total) emit(acceptPair(d, box, pi, p1, atom, exBegin, exEnd, exMin, exMax), __float_as_int(p1.w));
   }
+  __builtin_amdgcn_wave_barrier();
   if (lane == 0) {
     d.nbrCount[atom] = found < d.cap ? found : d.cap;
     atomicMax(&d.sc->required, (unsigned int)found);
@@ -324,11 +385,18 @@ __device__ __forceinline__ void neighbours(const Dev &d, int block) {
 
 __global__ void __launch_bounds__(256) neighboursKernel(Dev d) { neighbours(d, blockIdx.x); }
 
+// Batched, grid-stride over the task table: on most steps only the few
+// replicas due a rebuild have work, and a workgroup skips the others' tasks
+// with one scalar load each instead of the dispatch of one workgroup per task.
+// The stride is a multiple of the XCD count, so a workgroup stays on the
+// replicas of its own XCD.
 __global__ void __launch_bounds__(256) neighboursBatchKernel(const BatchView *__restrict__ v,
-                                                             const int2 *__restrict__ task) {
-  const int2 t = task[blockIdx.x];
-  if (t.x < 0) return;
-  neighbours(v[t.x].d, t.y);
+                                                             const int2 *__restrict__ task, int tasks) {
+  for (int k = blockIdx.x; k < tasks; k += gridDim.x) {
+    const int2 t = task[k];
+    if (t.x < 0 || !v[t.x].d.sc->rebuild) continue;
+    neighbours(v[t.x].d, t.y);
+  }
 }
 
 void launchForceRebuildFlag(const Dev &d, hipStream_t s) { setRebuildKernel<<<1, 1, 0, s>>>(d.sc); }
@@ -350,7 +418,10 @@ void launchRebuild(const Dev &d, hipStream_t s) {
 // batch takes replicas the single-workgroup build fits), then the lists.
 void launchRebuildBatch(const BatchView *v, const TaskTable &replicas, const TaskTable &atoms64, hipStream_t s) {
   cellsBuildSmallBatchKernel<<<replicas.blocks, SMALL_THREADS, 0, s>>>(v, replicas.task);
-  neighboursBatchKernel<<<atoms64.blocks, 256, 0, s>>>(v, atoms64.task);
+  // 2048 workgroups per XCD at most: enough to fill the chip when many
+  // replicas rebuild together
+  const int grid = atoms64.blocks < 16384 ? atoms64.blocks : 16384;
+  neighboursBatchKernel<<<grid, 256, 0, s>>>(v, atoms64.task, atoms64.blocks);
 }
 
 bool fitsSmallCellBuild(int atoms, int cells) { return atoms <= SMALL_MAX_ATOMS && cells <= SMALL_MAX_CELLS; }

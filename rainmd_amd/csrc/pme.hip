@@ -141,7 +141,8 @@ __device__ __forceinline__ void fftStage(const float2 *src, float2 *dst, int lin
 // (line l at src + l * K), by all threads of the block. `rad` lists the
 // radices (0-terminated), `tw` holds exp(+2 pi i t / K) for t < K in LDS.
 // Every stage ends with a block barrier; returns the buffer with the result.
-__device__ float2 *fftBatch(float2 *src, float2 *dst, int lines, int K, const int *rad, const float2 *tw, int sign) {
+__device__ __forceinline__ float2 *fftBatch(float2 *src, float2 *dst, int lines, int K, const int *rad,
+                                            const float2 *tw, int sign) {
   int Ns = 1;
   for (int s = 0; s < 8 && rad[s] > 0; ++s) {
     const int R = rad[s];
