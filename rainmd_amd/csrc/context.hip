@@ -489,6 +489,7 @@ void evaluate(rmd_ctx *c, int terms, bool forceRebuild, double terms_out[E_TERM_
     HIP_CHECK(hipMemsetAsync(&c->d.sc->requiredMax, 0, sizeof(unsigned int), c->stream));
     decideRebuildKernel<<<1, 1, 0, c->stream>>>(c->d.sc, (float)c->ip.dispThreshold2, forceRebuild ? 1 : 0);
     launchZeroForces(c->d, c->stream);
+    launchRefreshSlots(c->d, c->stream);
     int slots[E_TERM_COUNT];
     enqueueForces(c, terms, terms_out ? 1 : 0, slots);
     HIP_CHECK(hipGetLastError());
@@ -864,6 +865,9 @@ void create(rmd_ctx *c, const System &sys, const rmd_options &o, const Streams *
   d.cellOfAtom = devAlloc<int>(d.nPad, c->held);
   d.cellAtoms = devAlloc<int>(d.nPad, c->held);
   d.cellPos = devAlloc<float4>(d.nPad, c->held);
+  d.slotPos = devAlloc<float4>(d.nPad, c->held);
+  d.slotType = devAlloc<int>(d.nPad, c->held);
+  d.slotOf = devAlloc<int>(d.nPad, c->held);
   d.cellCapacity = 0;
   allocateCells(c, c->plan.cellCount);
   d.nbrCount = devAlloc<int>(d.nPad, c->held);
@@ -1565,10 +1569,23 @@ int rmd_read_cells(rmd_ctx *c, int32_t *cellOfAtom, int32_t *cellStart, int32_t 
 
 int rmd_read_neighbours(rmd_ctx *c, int32_t *counts, int32_t *lists) {
   CTX_GUARD(c, {
+    // rows and entries are cell slots on the device; the caller gets them
+    // by atom
+    const int n = c->d.n, cap = c->d.cap;
     HIP_CHECK(hipStreamSynchronize(c->stream));
-    if (counts) HIP_CHECK(hipMemcpy(counts, c->d.nbrCount, c->d.n * sizeof(int), hipMemcpyDeviceToHost));
-    if (lists)
-      HIP_CHECK(hipMemcpy(lists, c->d.nbrList, (size_t)c->d.n * c->d.cap * sizeof(int), hipMemcpyDeviceToHost));
+    std::vector<int> atomOf(n), rowCount(n);
+    HIP_CHECK(hipMemcpy(atomOf.data(), c->d.cellAtoms, n * sizeof(int), hipMemcpyDeviceToHost));
+    HIP_CHECK(hipMemcpy(rowCount.data(), c->d.nbrCount, n * sizeof(int), hipMemcpyDeviceToHost));
+    if (counts)
+      for (int r = 0; r < n; ++r) counts[atomOf[r]] = rowCount[r];
+    if (lists) {
+      std::vector<int> rows((size_t)n * cap);
+      HIP_CHECK(hipMemcpy(rows.data(), c->d.nbrList, rows.size() * sizeof(int), hipMemcpyDeviceToHost));
+      for (int r = 0; r < n; ++r) {
+        int32_t *dst = lists + (size_t)atomOf[r] * cap;
+        for (int k = 0; k < cap; ++k) dst[k] = k < rowCount[r] ? atomOf[rows[(size_t)r * cap + k]] : 0;
+      }
+    }
   });
 }
 
@@ -2019,6 +2036,7 @@ void batchEvaluate(rmd_batch *b, bool energies, bool forceAll, int terms = RMD_T
     launchClearRequiredBatch(b->scAll, b->replicas, s);
     launchDecideRebuildBatch(b->dViews, b->replicas, forceAll ? 1 : 0, s);
     launchZeroForcesBatch(b->dViews, b->tables[BK_ATOMS256], s);
+    launchRefreshSlotsBatch(b->dViews, b->tables[BK_ATOMS256], s);
     enqueueBatchForces(b, energies ? 1 : 0, terms);
     if (energies) launchReduceEnergiesBatch(b->dViews, b->replicas, 0, s);
     HIP_CHECK(hipGetLastError());

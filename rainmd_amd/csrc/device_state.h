@@ -65,8 +65,14 @@ struct Dev {
   const BoxF *box;  // device-resident geometry (see BoxF)
   // cells (arrays sized for cellCapacity cells)
   int *cellOfAtom, *cellCount, *cellStart, *cellCursor, *cellAtoms;
-  float4 *cellPos;  // positions in cell order, the atom index in .w (int bits)
+  float4 *cellPos;  // positions in cell order at the last rebuild, the atom index in .w (int bits)
   int cellCapacity;
+  // The current positions (.w the charge) and types in cell order ("slots":
+  // slot k holds atom cellAtoms[k]); the integrator keeps slotPos current
+  // through slotOf. The neighbour list's rows and entries are slots, so the
+  // direct space reads a cell's atoms from consecutive addresses.
+  float4 *slotPos;
+  int *slotType, *slotOf;
   // neighbours: full list, `cap` slots per atom
   int *nbrCount, *nbrList;
   int cap;

@@ -48,6 +48,7 @@ __device__ __forceinline__ float storeAtom(const Dev &d, int a, const Atom &s, b
   const float4 old = d.pos[a];
   const float4 p = make_float4((float)s.x[0], (float)s.x[1], (float)s.x[2], old.w);
   d.pos[a] = p;
+  d.slotPos[d.slotOf[a]] = p;
   d.vel[a] = make_float4((float)s.v[0], (float)s.v[1], (float)s.v[2], (float)s.im);
   if (clearForce) {
     d.force[a] = 0;

@@ -13,6 +13,9 @@ namespace rmd {
 // the integrator's last block (or the host) asked for a rebuild.
 void launchRebuild(const Dev &d, hipStream_t s);
 void launchForceRebuildFlag(const Dev &d, hipStream_t s);
+// Refresh the cell-ordered position copy from the positions (evaluations
+// outside the step loop, where positions may have been set or restored).
+void launchRefreshSlots(const Dev &d, hipStream_t s);
 
 // ---- nonbonded direct space (nonbonded.hip) ----
 // lanes: lanes per atom (1, 2, 4, ..., 64). energies != 0 fills the
@@ -149,6 +152,7 @@ void launchForceRebuildFlaggedBatch(const BatchView *v, int replicas, hipStream_
 void launchClearRequiredBatch(Scalars *sc, int replicas, hipStream_t s);
 // Whether a system's cell list fits the single-workgroup build (a batch needs it).
 bool fitsSmallCellBuild(int atoms, int cells);
+void launchRefreshSlotsBatch(const BatchView *v, const TaskTable &atoms256, hipStream_t s);
 void launchRebuildBatch(const BatchView *v, const TaskTable &replicas, const TaskTable &atoms64, hipStream_t s);
 void launchNonbondedBatch(const BatchView *v, const TaskTable &atoms64, float alpha, int energies, hipStream_t s);
 void launchBondedBatch(const BatchView *v, const TaskTable &t, double alpha, int terms, int energies, hipStream_t s);

@@ -42,6 +42,18 @@ __device__ __forceinline__ int cellIndex(float x, float y, float z, const BoxF &
   return (c[0] * cells[1] + c[1]) * cells[2] + c[2];
 }
 
+// Slot k of the cell order gets atom `atom`: its rebuild-time position with
+// the atom index (for the search), its current position and charge, its type,
+// and the inverse map.
+__device__ __forceinline__ void writeSlot(const Dev &d, int k, int atom) {
+  const float4 p = d.pos[atom];
+  d.cellAtoms[k] = atom;
+  d.cellPos[k] = make_float4(p.x, p.y, p.z, __int_as_float(atom));
+  d.slotPos[k] = p;
+  d.slotType[k] = d.type[atom];
+  d.slotOf[atom] = k;
+}
+
 __global__ void setRebuildKernel(Scalars *sc) {
   sc->rebuild = 1u;
   sc->maxDisp2Bits = 0u;
@@ -141,10 +153,7 @@ __global__ void __launch_bounds__(256) cellsSortKernel(Dev d) {
   }
   for (int k = threadIdx.x; k < len; k += blockDim.x) {
     const int atom = keys[k];
-    d.cellAtoms[begin + k] = atom;
-    float4 p = d.pos[atom];
-    p.w = __int_as_float(atom);
-    d.cellPos[begin + k] = p;
+    writeSlot(d, begin + k, atom);
   }
 }
 
@@ -210,10 +219,7 @@ __device__ __forceinline__ void cellsBuildSmall(const Dev &d) {
     const int b = begin[c], e = begin[c + 1];
     int rank = 0;
     for (int k = b; k < e; ++k) rank += unsorted[k] < i;
-    float4 p = d.pos[i];
-    p.w = __int_as_float(i);
-    d.cellAtoms[b + rank] = i;
-    d.cellPos[b + rank] = p;
+    writeSlot(d, b + rank, i);
   }
 }
 
@@ -282,8 +288,9 @@ __device__ __forceinline__ void neighbours(const Dev &d, int block) {
   const BoxF box = *d.box;
   const int *cells = box.cells;
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-  const int atom = (block * blockDim.x + threadIdx.x) >> 6;
-  if (atom >= d.n) return;
+  const int row = (block * blockDim.x + threadIdx.x) >> 6;  // a slot
+  if (row >= d.n) return;
+  const int atom = d.cellAtoms[row];
   const float4 pi = d.pos[atom];
   if (lane == 0) d.posRebuild[atom] = pi;
   const int own = d.cellOfAtom[atom];
@@ -349,9 +356,9 @@ __device__ __forceinline__ void neighbours(const Dev &d, int block) {
   __builtin_amdgcn_wave_barrier();
   const int total = carry;
   int found = 0;
-  int *out = d.nbrList + (size_t)atom * d.cap;
+  int *out = d.nbrList + (size_t)row * d.cap;
   const float4 none = make_float4(0.f, 0.f, 0.f, __int_as_float(atom));
-  auto emit = [&](bool accept, int j) {
+  auto emit = [&](bool accept, int j) {  // j: the candidate's slot
     const unsigned long long mask = __ballot(accept);
     if (accept) {
       const int where = found + __popcll(mask & ((1ull << lane) - 1ull));
@@ -363,21 +370,24 @@ __device__ __forceinline__ void neighbours(const Dev &d, int block) {
   for (int base = 0; base < total; base += 128) {
     const int g0 = base + lane, g1 = g0 + 64;
     float4 p0 = none, p1 = none;
+    int s0 = row, s1 = row;
     if (g0 < total) {
       while (rs[cur + 1] <= g0) ++cur;
-      p0 = d.cellPos[rb[cur] + (g0 - rs[cur])];
+      s0 = rb[cur] + (g0 - rs[cur]);
+      p0 = d.cellPos[s0];
     }
     if (g1 < total) {
       int c1 = cur;
       while (rs[c1 + 1] <= g1) ++c1;
-      p1 = d.cellPos[rb[c1] + (g1 - rs[c1])];
+      s1 = rb[c1] + (g1 - rs[c1]);
+      p1 = d.cellPos[s1];
     }
-    emit(acceptPair(d, box, pi, p0, atom, exBegin, exEnd, exMin, exMax), __float_as_int(p0.w));
-    if (base + 64 < total) emit(acceptPair(d, box, pi, p1, atom, exBegin, exEnd, exMin, exMax), __float_as_int(p1.w));
+    emit(acceptPair(d, box, pi, p0, atom, exBegin, exEnd, exMin, exMax), s0);
+    if (base + 64 < total) emit(acceptPair(d, box, pi, p1, atom, exBegin, exEnd, exMin, exMax), s1);
   }
   __builtin_amdgcn_wave_barrier();
   if (lane == 0) {
-    d.nbrCount[atom] = found < d.cap ? found : d.cap;
+    d.nbrCount[row] = found < d.cap ? found : d.cap;
     atomicMax(&d.sc->required, (unsigned int)found);
     atomicMax(&d.sc->requiredMax, (unsigned int)found);
   }
@@ -400,6 +410,28 @@ __global__ void __launch_bounds__(256) neighboursBatchKernel(const BatchView *__
 }
 
 void launchForceRebuildFlag(const Dev &d, hipStream_t s) { setRebuildKernel<<<1, 1, 0, s>>>(d.sc); }
+
+// The slot copy from the positions through the current cell order, for an
+// evaluation that did not follow an integrator step (which keeps it current).
+__global__ void refreshSlotsKernel(Dev d) {
+  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k < d.n) d.slotPos[k] = d.pos[d.cellAtoms[k]];
+}
+
+__global__ void __launch_bounds__(256) refreshSlotsBatchKernel(const BatchView *__restrict__ v,
+                                                               const int2 *__restrict__ task) {
+  const int2 t = task[blockIdx.x];
+  if (t.x < 0) return;
+  const Dev &d = v[t.x].d;
+  const int k = t.y * blockDim.x + threadIdx.x;
+  if (k < d.n) d.slotPos[k] = d.pos[d.cellAtoms[k]];
+}
+
+void launchRefreshSlots(const Dev &d, hipStream_t s) { refreshSlotsKernel<<<(d.n + 255) / 256, 256, 0, s>>>(d); }
+
+void launchRefreshSlotsBatch(const BatchView *v, const TaskTable &atoms256, hipStream_t s) {
+  refreshSlotsBatchKernel<<<atoms256.blocks, 256, 0, s>>>(v, atoms256.task);
+}
 
 void launchRebuild(const Dev &d, hipStream_t s) {
   if (d.n <= SMALL_MAX_ATOMS && d.cellCapacity <= SMALL_MAX_CELLS) {

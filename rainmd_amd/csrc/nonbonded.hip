@@ -34,34 +34,34 @@ __device__ __forceinline__ void nonbonded(const Dev &d, float alpha, int block) 
   const float2 *table = tableInLds ? ldsTable : d.lj;
   const BoxF &box = *d.box;
   const int tid = block * blockDim.x + threadIdx.x;
-  const int atom = tid / LANES;
+  const int row = tid / LANES;  // a slot: rows and list entries follow the cell order
   const int sub = tid % LANES;
-  const bool valid = atom < d.n;
+  const bool valid = row < d.n;
   float fx = 0.f, fy = 0.f, fz = 0.f;
   double elj = 0.0, ecoul = 0.0, dscale = 0.0;
   if (valid) {
-    const float4 pi = d.pos[atom];
-    const int ti = d.type[atom] * d.types;
+    const float4 pi = d.slotPos[row];
+    const int ti = d.slotType[row] * d.types;
     const float qi = ONE_4PI_EPS0_F * pi.w;
     const float gaussScale = TWO_OVER_SQRT_PI_F * alpha;
-    const int count = d.nbrCount[atom];
-    const int *list = d.nbrList + (size_t)atom * d.cap;
-    // Software pipeline: the neighbour index two iterations ahead and the
+    const int count = d.nbrCount[row];
+    const int *list = d.nbrList + (size_t)row * d.cap;
+    // Software pipeline: the neighbour slot two iterations ahead and the
     // position and type one iteration ahead are in flight while a pair is
-    // computed. Past the end the prefetch reads the atom itself (in bounds).
+    // computed. Past the end the prefetch reads the row's own slot (in bounds).
     int k = sub;
-    int jAhead = k + LANES < count ? list[k + LANES] : atom;
-    int jNext = k < count ? list[k] : atom;
-    float4 pNext = d.pos[jNext];
-    int tNext = d.type[jNext];
+    int jAhead = k + LANES < count ? list[k + LANES] : row;
+    int jNext = k < count ? list[k] : row;
+    float4 pNext = d.slotPos[jNext];
+    int tNext = d.slotType[jNext];
     while (k < count) {
       const float4 pj = pNext;
       const int tj = tNext;
       k += LANES;
       jNext = jAhead;
-      jAhead = k + LANES < count ? list[k + LANES] : atom;
-      pNext = d.pos[jNext];
-      tNext = d.type[jNext];
+      jAhead = k + LANES < count ? list[k + LANES] : row;
+      pNext = d.slotPos[jNext];
+      tNext = d.slotType[jNext];
       float dx = pi.x - pj.x, dy = pi.y - pj.y, dz = pi.z - pj.z;
       const float r2 = minimumImageR2(dx, dy, dz, box);
       if (r2 > box.rc2) continue;
@@ -97,6 +97,7 @@ __device__ __forceinline__ void nonbonded(const Dev &d, float alpha, int block) 
     iz += __shfl_xor(iz, offset, LANES);
   }
   if (valid && sub == 0) {
+    const int atom = d.cellAtoms[row];
     atomicAddFixed(&d.force[atom], ix);
     atomicAddFixed(&d.force[d.nPad + atom], iy);
     atomicAddFixed(&d.force[2 * d.nPad + atom], iz);
