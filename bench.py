@@ -159,6 +159,8 @@ def bench_replicas(args, count):
     log(f"{count} replicas, {sum(atoms)} atoms ({min(atoms)}-{max(atoms)}), built in {time.perf_counter() - t0:.1f} s")
     t0 = time.perf_counter()
     extra = dict(skin=args.replica_skin) if args.replica_skin > 0 else {}
+    if args.lanes > 0:
+        extra["lanes_per_atom"] = args.lanes
     batch = R.Batch(systems, seeds=seeds, temperature=300.0, pressure=1.0, barostat_interval=25, **extra)
     if args.no_graphs:
         batch.set_graphs(False)
@@ -203,14 +205,18 @@ def bench_replicas(args, count):
     if args.single_steps <= 0:
         batch.release()
         return
-    # one system per launch: every replica alone, the same kernels
+    # one system per launch: every replica alone through its own context, the
+    # same kernels and options, at the single system's 64 lanes per atom
     single_total = 0.0
     for r in range(count):
         sim = batch.replica(r)
+        lanes = sim.diagnostics()["lanes_per_atom"]
+        sim.set_lanes(64)
         sim.run(args.block, args.block)  # captures its block graph
         t0 = time.perf_counter()
         sim.run(args.single_steps, args.block)
         single_total += time.perf_counter() - t0
+        sim.set_lanes(lanes)
     single_ms = 1e3 * single_total / args.single_steps
     single_aggregate = count * TIMESTEP_PS * 1e-3 * 86400.0 / (single_ms * 1e-3)
     log(f"one system per launch: {single_ms:.3f} ms per step of all replicas, aggregate {single_aggregate:.1f} ns/day; "
@@ -235,13 +241,16 @@ def bench_replicas(args, count):
             "replicas": count,
             "atoms": sum(atoms),
             "block": args.block,
+            "skin_nm": batch.replica(0).diagnostics()["skin"],
+            "lanes_per_atom": batch.replica(0).diagnostics()["lanes_per_atom"],
+            "force_branches": "serial" if args.serial_forces else "concurrent",
         },
         "single_launch_baseline": {
             "value": single_aggregate,
             "unit": "ns/day",
             "ms_per_step": single_ms,
             "sample": f"every replica run alone for {args.single_steps} steps through its own context "
-                      "(graphs and concurrent force branches on)",
+                      "(the batch's options; graphs, concurrent force branches and 64 lanes per atom)",
         },
         "batch_speedup": aggregate / single_aggregate,
         "barostat": {"attempts": moves, "accepted": accepted},

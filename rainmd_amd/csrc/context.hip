@@ -1827,6 +1827,7 @@ struct rmd_batch {
   size_t ldsForward = 0, ldsXlines = 0, ldsGather = 0;
   int ldsOpen = 0;
   double alpha = 0;
+  int lanes = 16;  // direct-space lanes per atom of every member
   bool concurrent = true, useGraphs = true;
   std::map<int, hipGraphExec_t> graphs;
   std::atomic<int> stop{0};
@@ -1849,6 +1850,7 @@ int replicaBlocks(const rmd_ctx *c, int kind) {
     case BK_ATOMS256: return (d.nPad + 255) / 256;
     case BK_MOLECULES: return std::max(1, (c->sys.topology.moleculeCount + 255) / 256);
     case BK_INFLUENCE: return (int)(((size_t)c->p.K[0] * c->p.K[1] * c->p.K2h + 255) / 256);
+    case BK_NONBONDED: return (int)(((long long)d.n * c->lanes + 255) / 256);
   }
   return 0;
 }
@@ -1895,7 +1897,7 @@ void uploadViews(rmd_batch *b) {
     v.ip = c->ip;
     v.openBlocks = replicaBlocks(c, BK_OPEN);
     std::memset(&v.counts, 0, sizeof v.counts);
-    const int nb = replicaBlocks(c, BK_ATOMS64), bb = replicaBlocks(c, BK_BONDED), xb = replicaBlocks(c, BK_XLINES);
+    const int nb = replicaBlocks(c, BK_NONBONDED), bb = replicaBlocks(c, BK_BONDED), xb = replicaBlocks(c, BK_XLINES);
     v.counts.c[E_LJ_DIRECT] = v.counts.c[E_COUL_DIRECT] = v.counts.c[E_DSCALE_DIRECT] = nb;
     v.counts.c[E_BOND] = v.counts.c[E_ANGLE] = v.counts.c[E_TORSION] = v.counts.c[E_LJ_EXCEPTION] =
         v.counts.c[E_COUL_EXCEPTION] = v.counts.c[E_RESTRAINT] = bb;
@@ -1985,7 +1987,8 @@ void enqueueBatchForces(rmd_batch *b, int energies, int terms = RMD_TERM_ALL) {
   if (terms & (RMD_TERM_BONDED | RMD_TERM_EXCEPTIONS | RMD_TERM_RESTRAINT))
     launchBondedBatch(b->dViews, b->tables[BK_BONDED], b->alpha, terms, energies, bondedStream);
   launchRebuildBatch(b->dViews, b->tables[BK_REPLICA], b->tables[BK_ATOMS64], st.stream);
-  if (terms & RMD_TERM_DIRECT) launchNonbondedBatch(b->dViews, b->tables[BK_ATOMS64], (float)b->alpha, energies, st.stream);
+  if (terms & RMD_TERM_DIRECT)
+    launchNonbondedBatch(b->dViews, b->tables[BK_NONBONDED], b->lanes, (float)b->alpha, energies, st.stream);
   if (fork) {
     for (int k = 0; k < 2; ++k) {
       HIP_CHECK(hipEventRecord(st.join[k], st.side[k]));
@@ -2242,6 +2245,12 @@ void createBatch(rmd_batch *b, const rmd_system *const *systems, int count, cons
   }
   HIP_CHECK(hipEventCreateWithFlags(&st.fork, hipEventDisableTiming));
   b->replicas = count;
+  // Lanes per atom: with hundreds of thousands of atoms in flight, fewer
+  // lanes per atom and longer per-lane loops hide the gather latency better
+  // than the single system's 64.
+  b->lanes = o.lanes_per_atom > 0 ? o.lanes_per_atom : 16;
+  if (b->lanes != 8 && b->lanes != 16 && b->lanes != 32 && b->lanes != 64)
+    throw Error(RMD_E_INVALID, "a batch runs 8, 16, 32 or 64 lanes per atom");
   b->scAll = devAlloc<Scalars>(count, b->held);
   b->scSaved = devAlloc<Scalars>(count, b->held);
   b->boxAll = devAlloc<BoxF>(count, b->held);
@@ -2255,7 +2264,7 @@ void createBatch(rmd_batch *b, const rmd_system *const *systems, int count, cons
     if (!systems[r]) throw Error(RMD_E_INVALID, replicaLabel(r) + "null system");
     rmd_options ro = o;
     ro.seed = seeds ? seeds[r] : o.seed;
-    ro.lanes_per_atom = 64;
+    ro.lanes_per_atom = b->lanes;
     rmd_ctx *c = new rmd_ctx();
     c->batchMember = true;
     b->reps.push_back(c);
@@ -2359,10 +2368,10 @@ int rmd_batch_info_get(rmd_batch *b, rmd_batch_info *out) {
     out->replicas = b->replicas;
     for (rmd_ctx *c : b->reps) out->atoms += c->d.n;
     out->blocks_run = b->blocksRun;
-    out->step_workgroups = b->tables[BK_OPEN].blocks + b->tables[BK_ATOMS64].blocks * 2 +
+    out->step_workgroups = b->tables[BK_OPEN].blocks + b->tables[BK_ATOMS64].blocks + b->tables[BK_NONBONDED].blocks +
                            b->tables[BK_BONDED].blocks + 2 * b->tables[BK_PLANES].blocks +
                            b->tables[BK_XLINES].blocks + b->tables[BK_REPLICA].blocks;
-    out->nonbonded_workgroups = b->tables[BK_ATOMS64].blocks;
+    out->nonbonded_workgroups = b->tables[BK_NONBONDED].blocks;
   });
 }
 
@@ -2428,12 +2437,13 @@ int rmd_batch_profile_nonbonded(rmd_batch *b, int32_t reps, double *avg_ms) {
     if (reps <= 0 || !avg_ms) throw Error(RMD_E_INVALID, "reps must be positive");
     hipStream_t s = b->streams.stream;
     uploadViews(b);
-    launchNonbondedBatch(b->dViews, b->tables[BK_ATOMS64], (float)b->alpha, 0, s);
+    launchNonbondedBatch(b->dViews, b->tables[BK_NONBONDED], b->lanes, (float)b->alpha, 0, s);
     hipEvent_t e0, e1;
     HIP_CHECK(hipEventCreate(&e0));
     HIP_CHECK(hipEventCreate(&e1));
     HIP_CHECK(hipEventRecord(e0, s));
-    for (int k = 0; k < reps; ++k) launchNonbondedBatch(b->dViews, b->tables[BK_ATOMS64], (float)b->alpha, 0, s);
+    for (int k = 0; k < reps; ++k)
+      launchNonbondedBatch(b->dViews, b->tables[BK_NONBONDED], b->lanes, (float)b->alpha, 0, s);
     HIP_CHECK(hipEventRecord(e1, s));
     HIP_CHECK(hipEventSynchronize(e1));
     float ms = 0;

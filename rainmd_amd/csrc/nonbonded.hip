@@ -133,13 +133,13 @@ __global__ void __launch_bounds__(256) nonbondedKernel(Dev d, float alpha) {
   nonbonded<LANES, ENERGIES>(d, alpha, blockIdx.x);
 }
 
-// Batched, one wave per atom.
-template <bool ENERGIES>
+// Batched.
+template <int LANES, bool ENERGIES>
 __global__ void __launch_bounds__(256) nonbondedBatchKernel(const BatchView *__restrict__ v,
                                                             const int2 *__restrict__ task, float alpha) {
   const int2 t = task[blockIdx.x];
   if (t.x < 0) return;
-  nonbonded<64, ENERGIES>(v[t.x].d, alpha, t.y);
+  nonbonded<LANES, ENERGIES>(v[t.x].d, alpha, t.y);
 }
 
 template <bool ENERGIES>
@@ -165,11 +165,21 @@ int launchNonbonded(const Dev &d, float alpha, int lanes, int energies, hipStrea
   return blocks;
 }
 
-void launchNonbondedBatch(const BatchView *v, const TaskTable &atoms64, float alpha, int energies, hipStream_t s) {
+template <bool ENERGIES>
+static void launchBatchLanes(const BatchView *v, const TaskTable &t, int lanes, float alpha, hipStream_t s) {
+  switch (lanes) {
+    case 8: nonbondedBatchKernel<8, ENERGIES><<<t.blocks, 256, 0, s>>>(v, t.task, alpha); break;
+    case 16: nonbondedBatchKernel<16, ENERGIES><<<t.blocks, 256, 0, s>>>(v, t.task, alpha); break;
+    case 32: nonbondedBatchKernel<32, ENERGIES><<<t.blocks, 256, 0, s>>>(v, t.task, alpha); break;
+    default: nonbondedBatchKernel<64, ENERGIES><<<t.blocks, 256, 0, s>>>(v, t.task, alpha); break;
+  }
+}
+
+void launchNonbondedBatch(const BatchView *v, const TaskTable &t, int lanes, float alpha, int energies, hipStream_t s) {
   if (energies)
-    nonbondedBatchKernel<true><<<atoms64.blocks, 256, 0, s>>>(v, atoms64.task, alpha);
+    launchBatchLanes<true>(v, t, lanes, alpha, s);
   else
-    nonbondedBatchKernel<false><<<atoms64.blocks, 256, 0, s>>>(v, atoms64.task, alpha);
+    launchBatchLanes<false>(v, t, lanes, alpha, s);
 }
 
 }  // namespace rmd

@@ -22,8 +22,9 @@ def _systems(picks):
             for e, w, s in chosen], [s for _, _, s in chosen]
 
 
-# small and mid-sized replicas of the workload (1.1k - 5k atoms)
+# small and mid-sized replicas of the workload (1.1k - 10.6k atoms)
 PICKS = [1, 2, 5, 9]
+LANES = 16  # a batch's default lanes per atom; the single runs compared use the same
 
 
 def _start_states(systems, seeds):
@@ -44,14 +45,14 @@ def _run_pair(options, steps, graphs=True, max_neighbours=None):
     extra = dict(options)
     if max_neighbours:
         extra["max_neighbours"] = max_neighbours
-    batch = R.Batch(systems, seeds=seeds, lanes_per_atom=64, **extra)
+    batch = R.Batch(systems, seeds=seeds, lanes_per_atom=LANES, **extra)
     batch.set_graphs(graphs)
     for r, (x, v) in enumerate(starts):
         batch.replica(r).set_state(x, v, step=0)
     batch.run(steps)
     singles = []
     for r, (sys_, seed) in enumerate(zip(systems, seeds)):
-        sim = R.GpuSimulation(sys_, seed=seed, lanes_per_atom=64, **extra)
+        sim = R.GpuSimulation(sys_, seed=seed, lanes_per_atom=LANES, **extra)
         sim.set_graphs(graphs)
         sim.set_state(*starts[r], step=0)
         sim.run(steps)
