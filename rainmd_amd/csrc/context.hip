@@ -860,6 +860,11 @@ void create(rmd_ctx *c, const System &sys, const rmd_options &o, const Streams *
     for (size_t k = 0; k < lj.size(); ++k) lj[k] = make_float2((float)t.ljA[k], (float)t.ljB[k]);
     d.lj = devAlloc<float2>(lj.size(), c->held);
     upload(d.lj, lj, c->stream);
+    std::vector<double2> lj64(t.ljA.size());
+    for (size_t k = 0; k < lj64.size(); ++k) lj64[k] = make_double2(t.ljA[k], t.ljB[k]);
+    d.lj64 = devAlloc<double2>(lj64.size(), c->held);
+    upload(d.lj64, lj64, c->stream);
+    HIP_CHECK(hipStreamSynchronize(c->stream));
     upload(d.type, t.atomTypes, c->stream);
   }
   d.cellOfAtom = devAlloc<int>(d.nPad, c->held);

@@ -50,22 +50,40 @@ __device__ __forceinline__ void atomicAddFixed(long long *address, long long val
 // then b, then a, with s = floor(d * inv + 0.5) (the reference's kernel
 // order, webgpu/kernels/wgsl-common.js:81-88, with the division replaced by
 // a multiply by the f32 inverse width).
-__device__ __forceinline__ float minimumImageR2(float &dx, float &dy, float &dz, const BoxF &b) {
+__device__ __forceinline__ float minimumImageR2(float &dx, float &dy, float &dz, const BoxF &b, float &s0, float &s1,
+                                                float &s2) {
 #pragma clang fp contract(off)
-  float s2 = floorf(dz * b.inv[2] + 0.5f);
+  s2 = floorf(dz * b.inv[2] + 0.5f);
   dx = dx - s2 * b.c[0];
   dy = dy - s2 * b.c[1];
   dz = dz - s2 * b.c[2];
-  float s1 = floorf(dy * b.inv[1] + 0.5f);
+  s1 = floorf(dy * b.inv[1] + 0.5f);
   dx = dx - s1 * b.b[0];
   dy = dy - s1 * b.b[1];
-  float s0 = floorf(dx * b.inv[0] + 0.5f);
+  s0 = floorf(dx * b.inv[0] + 0.5f);
   dx = dx - s0 * b.a[0];
   float xx = dx * dx;
   float yy = dy * dy;
   float zz = dz * dz;
   float sum = xx + yy;
   return sum + zz;
+}
+
+__device__ __forceinline__ float minimumImageR2(float &dx, float &dy, float &dz, const BoxF &b) {
+  float s0, s1, s2;
+  return minimumImageR2(dx, dy, dz, b, s0, s1, s2);
+}
+
+// The squared distance of the same image in FP64: the FP32 box rows differ
+// from the FP64 box by up to half an ulp, a bias every boundary-crossing pair
+// shares, so the energies take the separation from the positions and the
+// FP64 rows with the images the FP32 test chose.
+__device__ __forceinline__ double imageR2Double(float4 pi, float4 pj, const BoxF &b, float s0, float s1, float s2) {
+  const double *r = b.box64;
+  const double ex = ((double)pi.x - (double)pj.x) - s2 * r[6] - s1 * r[3] - s0 * r[0];
+  const double ey = ((double)pi.y - (double)pj.y) - s2 * r[7] - s1 * r[4] - s0 * r[1];
+  const double ez = ((double)pi.z - (double)pj.z) - s2 * r[8] - s1 * r[5] - s0 * r[2];
+  return ex * ex + ey * ey + ez * ez;
 }
 
 // erfc(x) for x >= 0 as exp(-x^2) * t * P(t), t = 1/(1 + x/2). P is a

@@ -61,6 +61,7 @@ struct Dev {
   float4 *posRebuild;
   int *type;
   float2 *lj;     // (A, B) per type pair
+  double2 *lj64;  // the same in FP64, for the energies
   int types;
   const BoxF *box;  // device-resident geometry (see BoxF)
   // cells (arrays sized for cellCapacity cells)

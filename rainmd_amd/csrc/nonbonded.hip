@@ -7,10 +7,10 @@
 // shares one atom's list, each lane's partial force is converted to int64
 // fixed point and the group reduces it exactly, so the sum does not depend
 // on the lane count or the order. Energies (a separate instantiation, used on
-// the steps that report them) are accumulated per lane in FP64 - a small
-// box's LJ total is a large cancellation that FP32 sums would not keep to
-// 1e-6 - halved (each pair appears in both atoms' lists) and reduced in FP64
-// into per-block slots.
+// the steps that report them) are evaluated and accumulated per lane in FP64
+// from the FP32 distance - a small box's LJ total is a large cancellation
+// that FP32 terms and sums would not keep to 1e-6 - halved (each pair appears
+// in both atoms' lists) and reduced in FP64 into per-block slots.
 #include "kernels.h"
 
 // Multiply-adds fuse only within one expression, so each body rounds the same
@@ -63,7 +63,8 @@ __device__ __forceinline__ void nonbonded(const Dev &d, float alpha, int block) 
       pNext = d.slotPos[jNext];
       tNext = d.slotType[jNext];
       float dx = pi.x - pj.x, dy = pi.y - pj.y, dz = pi.z - pj.z;
-      const float r2 = minimumImageR2(dx, dy, dz, box);
+      float s0, s1, s2;
+      const float r2 = minimumImageR2(dx, dy, dz, box, s0, s1, s2);
       if (r2 > box.rc2) continue;
       const float2 ab = table[ti + tj];
       const float invR2 = __builtin_amdgcn_rcpf(r2);
@@ -77,8 +78,15 @@ __device__ __forceinline__ void nonbonded(const Dev &d, float alpha, int block) 
       const float screened = erfcWithGaussian(alpha * r, gaussian);
       scalar += qq * (screened * invR + gaussScale * gaussian) * invR2;
       if (ENERGIES) {
-        elj += (double)((lj12 - ab.y) * invR6);
-        ecoul += (double)(qq * screened * invR);
+        // FP64 at the same image: the FP32 box rows, the hardware
+        // reciprocals' one-ulp bias and the FP32 table would otherwise be
+        // systematic over the box, and a water box's LJ total is a small
+        // difference of large sums
+        const double2 ab64 = d.lj64[ti + tj];
+        const double ir2 = 1.0 / imageR2Double(pi, pj, box, s0, s1, s2);
+        const double ir6 = ir2 * ir2 * ir2;
+        elj += (ab64.x * ir6 - ab64.y) * ir6;
+        ecoul += (double)(qq * screened) * sqrt(ir2);
         dscale -= (double)(scalar * r2);  // r dU/dr
       }
       fx += scalar * dx;
