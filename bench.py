@@ -176,6 +176,14 @@ def bench_replicas(args, count):
     t0 = time.perf_counter()
     batch.run(args.warmup, args.block)
     log(f"warm-up {args.warmup} steps in {time.perf_counter() - t0:.2f} s")
+    if args.screen:
+        # the shared learned screen, trained on the first exact moves of all replicas
+        batch.configure_screen(training_moves=args.screen_training_moves, training_steps=4000, learning_rate=1e-3,
+                               seed=7)
+        t0 = time.perf_counter()
+        while batch.screen_stats()["phase"] == 0:
+            batch.run(args.block, args.block)
+        log(f"screen trained in {time.perf_counter() - t0:.1f} s: {batch.screen_stats()}")
     t0 = time.perf_counter()
     batch.run(args.steps, args.block)
     elapsed = time.perf_counter() - t0
@@ -271,6 +279,10 @@ def bench_replicas(args, count):
     if gates:
         result["energy_check"] = {"max_rel_err": max(g[0] for g in gates.values()),
                                   "replicas": {str(r): g[1] for r, g in gates.items()}}
+    if args.screen:
+        st = batch.screen_stats()
+        st["skipped_fraction"] = 1.0 - st["exact_evaluations"] / max(1, st["moves"])
+        result["screen"] = st
     batch.release()
     print(json.dumps(result), flush=True)
 
@@ -297,6 +309,8 @@ def main():
     ap.add_argument("--serial-forces", action="store_true",
                     help="run the force branches one after another (for per-kernel profiles)")
     ap.add_argument("--replica-skin", type=float, default=0.0, help="replica workloads: nm; 0 derives skinFor")
+    ap.add_argument("--screen", action="store_true", help="replica workloads: the shared learned barostat screen")
+    ap.add_argument("--screen-training-moves", type=int, default=2000)
     args = ap.parse_args()
 
     if args.gpus != 1:

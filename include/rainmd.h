@@ -494,6 +494,15 @@ int rmd_batch_set_graphs(rmd_batch *batch, int32_t enabled);
 int rmd_batch_set_concurrent_forces(rmd_batch *batch, int32_t enabled);
 /* Average duration of the batched direct-space launch (profiling). */
 int rmd_batch_profile_nonbonded(rmd_batch *batch, int32_t reps, double *avg_ms);
+/* The learned delayed-acceptance screen for a batch's barostat: one network
+ * shared by every replica (features and targets per molecule and kT), trained
+ * on the first training_moves exact moves of all replicas (both directions of
+ * each), then screening every replica's moves with the single system's two
+ * stages. No counterpart in the reference. The stats count the screening
+ * phase only: moves decided by the network, exact evaluations, moves turned
+ * down without one, second-stage rejections, acceptances. */
+int rmd_batch_screen_configure(rmd_batch *batch, const rmd_screen_options *options);
+int rmd_batch_screen_stats_get(rmd_batch *batch, rmd_screen_stats *out);
 
 #ifdef __cplusplus
 }

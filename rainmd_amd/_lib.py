@@ -252,6 +252,8 @@ SIGNATURES = {
     "rmd_batch_set_graphs": (C.c_int, [C.c_void_p, C.c_int32]),
     "rmd_batch_set_concurrent_forces": (C.c_int, [C.c_void_p, C.c_int32]),
     "rmd_batch_profile_nonbonded": (C.c_int, [C.c_void_p, C.c_int32, C.POINTER(C.c_double)]),
+    "rmd_batch_screen_configure": (C.c_int, [C.c_void_p, C.POINTER(ScreenOptions)]),
+    "rmd_batch_screen_stats_get": (C.c_int, [C.c_void_p, C.POINTER(ScreenStats)]),
     "rmd_system_solute_heavy_atoms": (C.c_int, [C.c_void_p, _ip, C.c_int32, C.POINTER(C.c_int32)]),
     "rmd_pme_parameters_derive": (C.c_int, [_dp, C.c_double, C.c_double, C.c_double, _ip,
                                             C.POINTER(PmeParameters)]),

@@ -86,6 +86,21 @@ struct Streams {
   hipEvent_t fork = nullptr, join[2] = {nullptr, nullptr};
 };
 
+// The barostat's learned screen: the network, its training samples and
+// normalisation, and the counts it reports. A context has one; a replica
+// batch has one shared by its replicas.
+struct ScreenState {
+  rmd_screen_options options{};
+  rmd_surrogate *net = nullptr;
+  int phase = 0;
+  double refVolume = 0;
+  std::vector<float> x, t;  // raw features and residual targets
+  float mean[RMD_SURROGATE_INPUTS] = {0}, scale[RMD_SURROGATE_INPUTS] = {0};
+  double tMean = 0, tScale = 1;
+  int64_t moves = 0, exact = 0, screened = 0, secondStage = 0, accepted = 0, residualCount = 0;
+  double trainingLoss = 0, residual2 = 0, estimate2 = 0;
+};
+
 struct rmd_ctx {
   System sys;
   // A member of a replica batch: the batch owns it (rmd_destroy leaves it
@@ -159,17 +174,7 @@ struct rmd_ctx {
   int slotCounts[E_TERM_COUNT] = {0};
   IntegratorParams ip{};
   // The barostat's learned screen.
-  struct Screen {
-    rmd_screen_options options{};
-    rmd_surrogate *net = nullptr;
-    int phase = 0;
-    double refVolume = 0;
-    std::vector<float> x, t;  // raw features and residual targets
-    float mean[RMD_SURROGATE_INPUTS] = {0}, scale[RMD_SURROGATE_INPUTS] = {0};
-    double tMean = 0, tScale = 1;
-    int64_t moves = 0, exact = 0, screened = 0, secondStage = 0, accepted = 0, residualCount = 0;
-    double trainingLoss = 0, residual2 = 0, estimate2 = 0;
-  } screen;
+  ScreenState screen;
 };
 
 namespace {
@@ -563,19 +568,25 @@ void screenFeatures(rmd_ctx *c, const double *t, double volume, double newVolume
   for (int k = 0; k < RMD_SURROGATE_INPUTS; ++k) f[k] = (float)g[k];
 }
 
-// The network's residual (kJ/mol) for one raw feature row.
-double predictResidual(rmd_ctx *c, const float *raw) {
-  auto &sc = c->screen;
-  float x[RMD_SURROGATE_INPUTS], y = 0.f;
-  for (int k = 0; k < RMD_SURROGATE_INPUTS; ++k) x[k] = (raw[k] - sc.mean[k]) * sc.scale[k];
-  const int st = rmd_surrogate_forward(sc.net, x, 1, &y);
+// The network's residuals (in the targets' units) for n raw feature rows.
+void predictScreen(ScreenState &sc, const float *raw, int n, double *out) {
+  const int F = RMD_SURROGATE_INPUTS;
+  std::vector<float> x((size_t)n * F), y(n);
+  for (int i = 0; i < n; ++i)
+    for (int k = 0; k < F; ++k) x[(size_t)i * F + k] = (raw[(size_t)i * F + k] - sc.mean[k]) * sc.scale[k];
+  const int st = rmd_surrogate_forward(sc.net, x.data(), n, y.data());
   if (st != RMD_OK) throw Error(st, std::string("screen inference failed: ") + rmd_last_global_error());
-  return sc.tMean + sc.tScale * (double)y;
+  for (int i = 0; i < n; ++i) out[i] = sc.tMean + sc.tScale * (double)y[i];
+}
+
+double predictResidual(rmd_ctx *c, const float *raw) {
+  double r = 0;
+  predictScreen(c->screen, raw, 1, &r);
+  return r;
 }
 
 // Normalise the kept samples, train the network on the GPU and start screening.
-void screenTrain(rmd_ctx *c) {
-  auto &sc = c->screen;
+void trainScreen(ScreenState &sc, int device) {
   const size_t n = sc.t.size();
   const int F = RMD_SURROGATE_INPUTS;
   for (int k = 0; k < F; ++k) {
@@ -602,10 +613,12 @@ void screenTrain(rmd_ctx *c) {
   const int st = rmd_surrogate_train(sc.net, x.data(), t.data(), (int)n, sc.options.training_steps,
                                      sc.options.learning_rate, &loss);
   if (st != RMD_OK) throw Error(st, std::string("screen training failed: ") + rmd_last_global_error());
-  HIP_CHECK(hipSetDevice(c->device));
+  HIP_CHECK(hipSetDevice(device));
   sc.trainingLoss = loss;
   sc.phase = 1;
 }
+
+void screenTrain(rmd_ctx *c) { trainScreen(c->screen, c->device); }
 
 double evaluatePotential(rmd_ctx *c) {
   double t[E_TERM_COUNT];
@@ -1706,7 +1719,7 @@ int rmd_screen_configure(rmd_ctx *c, const rmd_screen_options *options) {
       throw Error(RMD_E_INVALID, "the screen needs training_moves, training_steps and learning_rate > 0");
     auto &sc = c->screen;
     if (sc.net) rmd_surrogate_free(sc.net);
-    sc = rmd_ctx::Screen();
+    sc = ScreenState();
     sc.options = o;
     if (o.enabled) {
       rmd_surrogate *net = nullptr;
@@ -1837,6 +1850,7 @@ struct rmd_batch {
   std::map<int, hipGraphExec_t> graphs;
   std::atomic<int> stop{0};
   int64_t blocksRun = 0;
+  ScreenState screen;  // the learned screen shared by the replicas
 };
 
 namespace {
@@ -2104,18 +2118,40 @@ void batchRunBlock(rmd_batch *b, int steps) {
   throw Error(RMD_E_NEIGHBOUR_OVERFLOW, "the neighbour list kept overflowing after growth");
 }
 
+// The screen's features for a replica of a batch: the single system's
+// (already per molecule and kT) with the first-order estimate per molecule
+// too, so one network serves replicas of every size; its targets are the
+// residuals per molecule and kT.
+void batchFeatures(rmd_ctx *c, const double *t, double volume, double newVolume, float *f, double *estimate,
+                   double *nkT) {
+  screenFeatures(c, t, volume, newVolume, f, estimate);
+  *nkT = std::max(1, c->sys.topology.moleculeCount) * BOLTZMANN * c->temperature;
+  f[2] = (float)(*estimate / *nkT);
+}
+
 // The Monte Carlo volume move of every replica that is due one, with the
 // single-system move's arithmetic, random numbers and bookkeeping
 // (webgpu/simulation.js:995-1070): the trial boxes of all of them are
 // evaluated together, then each replica accepts or goes back on its own.
+// With the batch's learned screen, a move goes through the same two stages
+// as the single system's (section "learned screen" of the single move): the
+// network's prediction first, the exact evaluation only for the moves it
+// lets through, and the delayed-acceptance ratio with the reverse move's
+// prediction. All predictions of a round are one batched forward pass.
 void batchBarostat(rmd_batch *b) {
   const int R = b->replicas;
+  const int F = RMD_SURROGATE_INPUTS;
   hipStream_t s = b->streams.stream;
+  ScreenState &sc = b->screen;
+  const bool screenOn = sc.options.enabled && sc.net;
+  const bool screening = screenOn && sc.phase == 1;
   struct Trial {
-    bool active = false, moved = false;
-    double volume = 0, kT = 0, work = 0, oldPotential = 0;
+    bool active = false, candidate = false, moved = false;
+    double volume = 0, newVolume = 0, kT = 0, work = 0, oldPotential = 0, dispersionChange = 0;
+    double estimateX = 0, predictedX = 0, firstStage = 1.0, nkT = 1.0;
     double oldBox[9];
     double oldTerms[E_TERM_COUNT];
+    float fx[RMD_SURROGATE_INPUTS];
   };
   std::vector<Trial> tr(R);
   bool any = false, stale = false;
@@ -2127,7 +2163,7 @@ void batchBarostat(rmd_batch *b) {
   }
   if (!any) return;
   if (stale) batchEvaluate(b, true, true);
-  bool moved = false;
+  std::vector<int> candidates;
   for (int r = 0; r < R; ++r) {
     if (!tr[r].active) continue;
     rmd_ctx *c = b->reps[r];
@@ -2149,46 +2185,115 @@ void batchBarostat(rmd_batch *b) {
     const double deltaVolume = c->volumeScale * 2 * (draw - 0.5);
     const double newVolume = volume + deltaVolume;
     if (!(newVolume > 0)) continue;
-    const double sc = std::cbrt(newVolume / volume);
-    const double smallest = sc * std::min({std::fabs(c->box[0]), std::fabs(c->box[4]), std::fabs(c->box[8])});
+    const double sc3 = std::cbrt(newVolume / volume);
+    const double smallest = sc3 * std::min({std::fabs(c->box[0]), std::fabs(c->box[4]), std::fabs(c->box[8])});
     if (c->sys.cutoff > 0.5 * smallest) {
       c->barostatRefusals += 1;
       continue;
     }
+    t.newVolume = newVolume;
     t.work = pressure * deltaVolume - c->sys.topology.moleculeCount * t.kT * std::log(newVolume / volume);
-    t.moved = moved = true;
+    t.dispersionChange = dispersionAt(c, newVolume) - dispersionAt(c, volume);
+    if (screenOn) batchFeatures(c, t.oldTerms, volume, newVolume, t.fx, &t.estimateX, &t.nkT);
+    t.candidate = true;
+    candidates.push_back(r);
+  }
+  // Stage one: the predicted weight alone; a move it turns down is never
+  // evaluated and its replica is untouched.
+  if (screening && !candidates.empty()) {
+    std::vector<float> rows(candidates.size() * F);
+    for (size_t i = 0; i < candidates.size(); ++i) std::memcpy(&rows[i * F], tr[candidates[i]].fx, F * sizeof(float));
+    std::vector<double> pred(candidates.size());
+    predictScreen(sc, rows.data(), (int)candidates.size(), pred.data());
+    for (size_t i = 0; i < candidates.size(); ++i) {
+      const int r = candidates[i];
+      Trial &t = tr[r];
+      const rmd_ctx *c = b->reps[r];
+      sc.moves += 1;
+      t.predictedX = t.dispersionChange + t.estimateX + t.nkT * pred[i];
+      const double predictedWeight = t.predictedX + t.work;
+      t.firstStage = predictedWeight <= 0 ? 1.0 : std::exp(-predictedWeight / t.kT);
+      if (uniformAt(c->seed ^ 0x5a17u, (uint32_t)c->barostatAttempts, 2, 0) >= t.firstStage) {
+        sc.screened += 1;
+        t.candidate = false;
+      }
+    }
+  }
+  bool moved = false;
+  for (int r = 0; r < R; ++r) {
+    if (!tr[r].candidate) continue;
+    rmd_ctx *c = b->reps[r];
+    const double sc3 = std::cbrt(tr[r].newVolume / tr[r].volume);
     double nb[9];
-    for (int k = 0; k < 9; ++k) nb[k] = c->box[k] * sc;
+    for (int k = 0; k < 9; ++k) nb[k] = c->box[k] * sc3;
     updateBoxHost(c, nb);
     if (!fitsSmallCellBuild(c->d.n, c->plan.cellCount))
       throw Error(RMD_E_INVALID, replicaLabel(r) + "the trial box needs more cells than a batch replica supports");
-    b->views[r].scale = (float)sc;
+    b->views[r].scale = (float)sc3;
     b->views[r].rebuild = 1;
+    tr[r].moved = moved = true;
   }
-  if (!moved) {
-    clearFlags(b);
-    return;
+  if (moved) {
+    uploadViews(b);
+    launchSnapshotBatch(b->dViews, b->tables[BK_ATOMS256], 0, s);
+    launchScaleBatch(b->dViews, b->tables[BK_MOLECULES], b->tables[BK_ATOMS256], s);
+    uploadBoxes(b);
+    launchInfluenceBatch(b->dViews, b->tables[BK_INFLUENCE], s);
+    batchEvaluate(b, true, false);
   }
-  uploadViews(b);
-  launchSnapshotBatch(b->dViews, b->tables[BK_ATOMS256], 0, s);
-  launchScaleBatch(b->dViews, b->tables[BK_MOLECULES], b->tables[BK_ATOMS256], s);
-  uploadBoxes(b);
-  launchInfluenceBatch(b->dViews, b->tables[BK_INFLUENCE], s);
-  batchEvaluate(b, true, false);
+  // Stage two (or the plain test): the reverse move's features from the
+  // evaluated trial state, predicted together.
+  std::vector<int> evaluated;
+  for (int r = 0; r < R; ++r)
+    if (tr[r].moved) evaluated.push_back(r);
+  std::vector<float> reverseRows(evaluated.size() * F);
+  std::vector<double> reverseEstimate(evaluated.size()), reversePred(evaluated.size(), 0.0);
+  if (screenOn) {
+    for (size_t i = 0; i < evaluated.size(); ++i) {
+      const int r = evaluated[i];
+      double nkT = 1.0;
+      batchFeatures(b->reps[r], &b->hostTerms[(size_t)r * E_TERM_COUNT], tr[r].newVolume, tr[r].volume,
+                    &reverseRows[i * F], &reverseEstimate[i], &nkT);
+    }
+    if (screening && !evaluated.empty()) predictScreen(sc, reverseRows.data(), (int)evaluated.size(), reversePred.data());
+  }
   bool restore = false;
-  for (int r = 0; r < R; ++r) {
-    if (!tr[r].moved) continue;
+  for (size_t i = 0; i < evaluated.size(); ++i) {
+    const int r = evaluated[i];
     rmd_ctx *c = b->reps[r];
     const Trial &t = tr[r];
     const double *newTerms = &b->hostTerms[(size_t)r * E_TERM_COUNT];
-    const double weight = (potentialFromTerms(c, newTerms) - t.oldPotential) + t.work;
+    const double change = potentialFromTerms(c, newTerms) - t.oldPotential;
+    const double weight = change + t.work;
     const uint32_t bseed = c->seed ^ 0x5a17u;
-    const bool accept =
-        weight <= 0 || uniformAt(bseed, (uint32_t)c->barostatAttempts, 1, 0) < std::exp(-weight / t.kT);
+    const double u = uniformAt(bseed, (uint32_t)c->barostatAttempts, 1, 0);
+    bool accept;
+    if (screening) sc.exact += 1;
+    if (screening) {
+      const double predictedReverse = -t.dispersionChange + reverseEstimate[i] + t.nkT * reversePred[i] - t.work;
+      const double reverseStage = predictedReverse <= 0 ? 1.0 : std::exp(-predictedReverse / t.kT);
+      const double ratio = std::exp(-weight / t.kT) * reverseStage / t.firstStage;
+      accept = ratio >= 1.0 || u < ratio;
+      if (!accept) sc.secondStage += 1;
+      sc.residual2 += (change - t.predictedX) * (change - t.predictedX);
+      const double first = change - t.dispersionChange - t.estimateX;
+      sc.estimate2 += first * first;
+      sc.residualCount += 1;
+    } else {
+      accept = weight <= 0 || u < std::exp(-weight / t.kT);
+      if (screenOn && sc.phase == 0) {
+        // both directions of the move, per molecule and kT
+        sc.x.insert(sc.x.end(), t.fx, t.fx + F);
+        sc.t.push_back((float)((change - t.dispersionChange - t.estimateX) / t.nkT));
+        sc.x.insert(sc.x.end(), &reverseRows[i * F], &reverseRows[i * F] + F);
+        sc.t.push_back((float)((-change + t.dispersionChange - reverseEstimate[i]) / t.nkT));
+      }
+    }
     if (accept) {
       std::memcpy(c->blockEndTerms, newTerms, sizeof c->blockEndTerms);
       c->barostatAccepted += 1;
       c->windowAccepted += 1;
+      if (screening) sc.accepted += 1;
     } else {
       // back to the old state exactly: positions, forces, the box (copied)
       updateBoxHost(c, t.oldBox);
@@ -2197,13 +2302,20 @@ void batchBarostat(rmd_batch *b) {
       b->views[r].rebuild = 1;
       restore = true;
     }
+  }
+  // The adaptive window of every move that got past the refusals (screened
+  // out or evaluated); a refused move or a non-positive trial volume leaves
+  // it alone, as in the single move.
+  for (int r = 0; r < R; ++r) {
+    if (!tr[r].active || !(tr[r].newVolume > 0)) continue;
+    rmd_ctx *c = b->reps[r];
     c->blockEndStep = c->step;
     if (c->windowAttempts >= 10) {
       const double rate = (double)c->windowAccepted / c->windowAttempts;
       if (rate < 0.25)
         c->volumeScale /= 1.1;
       else if (rate > 0.75)
-        c->volumeScale = std::min(c->volumeScale * 1.1, 0.3 * t.volume);
+        c->volumeScale = std::min(c->volumeScale * 1.1, 0.3 * tr[r].volume);
       c->windowAttempts = c->windowAccepted = 0;
     }
   }
@@ -2217,6 +2329,8 @@ void batchBarostat(rmd_batch *b) {
     HIP_CHECK(hipStreamSynchronize(s));
   }
   clearFlags(b);
+  if (screenOn && sc.phase == 0 && (int64_t)sc.t.size() >= 2 * (int64_t)sc.options.training_moves)
+    trainScreen(sc, b->device);
 }
 
 void destroyBatch(rmd_batch *b) {
@@ -2224,6 +2338,7 @@ void destroyBatch(rmd_batch *b) {
   (void)hipSetDevice(b->device);
   if (b->streams.stream) (void)hipStreamSynchronize(b->streams.stream);
   for (auto &g : b->graphs) (void)hipGraphExecDestroy(g.second);
+  if (b->screen.net) rmd_surrogate_free(b->screen.net);
   for (rmd_ctx *c : b->reps) destroyContext(c);
   for (void *p : b->held) (void)hipFree(p);
   for (int k = 0; k < 2; ++k) {
@@ -2414,6 +2529,47 @@ int rmd_batch_run(rmd_batch *b, int64_t steps, int32_t block, int64_t *stepOut) 
       remaining -= n;
     }
     if (stepOut) *stepOut = b->reps[0]->step;
+  });
+}
+
+int rmd_batch_screen_configure(rmd_batch *b, const rmd_screen_options *options) {
+  BATCH_GUARD(b, {
+    rmd_screen_options o;
+    rmd_screen_options_default(&o);
+    if (options) o = *options;
+    if (o.enabled && (o.training_moves <= 0 || o.training_steps <= 0 || !(o.learning_rate > 0)))
+      throw Error(RMD_E_INVALID, "the screen needs training_moves, training_steps and learning_rate > 0");
+    auto &sc = b->screen;
+    if (sc.net) rmd_surrogate_free(sc.net);
+    sc = ScreenState();
+    sc.options = o;
+    if (o.enabled) {
+      rmd_surrogate *net = nullptr;
+      const int st = rmd_surrogate_create(b->device, o.seed, &net);
+      if (st != RMD_OK) throw Error(st, std::string("screen network: ") + rmd_last_global_error());
+      HIP_CHECK(hipSetDevice(b->device));
+      sc.net = net;
+      // each replica's features measure its volume against its own
+      for (rmd_ctx *c : b->reps) c->screen.refVolume = volumeOf(c->box);
+    }
+  });
+}
+
+int rmd_batch_screen_stats_get(rmd_batch *b, rmd_screen_stats *out) {
+  BATCH_GUARD(b, {
+    if (!out) throw Error(RMD_E_INVALID, "null output");
+    const auto &sc = b->screen;
+    std::memset(out, 0, sizeof *out);
+    out->phase = sc.phase;
+    out->samples = (int64_t)sc.t.size();
+    out->moves = sc.moves;
+    out->exact_evaluations = sc.exact;
+    out->screened_out = sc.screened;
+    out->second_stage_rejections = sc.secondStage;
+    out->accepted = sc.accepted;
+    out->training_loss = sc.trainingLoss;
+    out->residual_rms = sc.residualCount ? std::sqrt(sc.residual2 / sc.residualCount) : 0.0;
+    out->estimate_rms = sc.residualCount ? std::sqrt(sc.estimate2 / sc.residualCount) : 0.0;
   });
 }
 

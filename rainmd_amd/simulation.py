@@ -616,6 +616,18 @@ class Batch:
         self._check(L.load().rmd_batch_profile_nonbonded(self._h, int(reps), C.byref(ms)))
         return ms.value
 
+    def configure_screen(self, enabled=True, training_moves=2000, training_steps=4000, learning_rate=1e-3,
+                         seed=0x5eed):
+        """The learned barostat screen shared by the replicas (rmd_batch_screen_configure)."""
+        o = L.ScreenOptions(1 if enabled else 0, int(training_moves), int(training_steps), float(learning_rate),
+                            int(seed))
+        self._check(L.load().rmd_batch_screen_configure(self._h, C.byref(o)))
+
+    def screen_stats(self):
+        st = L.ScreenStats()
+        self._check(L.load().rmd_batch_screen_stats_get(self._h, C.byref(st)))
+        return _struct_dict(st)
+
     def info(self):
         i = L.BatchInfo()
         self._check(L.load().rmd_batch_info_get(self._h, C.byref(i)))

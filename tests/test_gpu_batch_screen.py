@@ -1,0 +1,78 @@
+"""The learned barostat screen on a replica batch: one network shared by the
+replicas, trained on their first exact moves (both directions of each, per
+molecule and kT), then turning moves down before their exact evaluation. The
+replicas' density distributions match the plain barostat's (NPT statistics
+judge): pooled densities under a two-sample KS test, paired per-replica mean
+densities, and compressibilities from volume fluctuations."""
+import numpy as np
+import pytest
+
+import rainmd_amd as R
+from rainmd_amd import npt
+
+pytestmark = pytest.mark.gpu
+
+EQUILIBRATE = 1000
+SAMPLES = 400          # volume samples per replica, one every 10 steps
+TRAINING_MOVES = 400   # exact moves of all replicas before training
+
+
+def _replicas():
+    work = R.replica_workload(64)
+    picks = [k for k, (edge, _, _) in enumerate(work) if edge < 3.0][:16]
+    return [work[k] for k in picks]
+
+
+def _run(screen):
+    chosen = _replicas()
+    systems = [R.System.solvated(water_count=w, box_edge=e, seed=s, include_solute=True, cutoff=0.9)
+               for e, w, s in chosen]
+    seeds = [s for _, _, s in chosen]
+    batch = R.Batch(systems, seeds=seeds, temperature=300.0, pressure=1.0, barostat_interval=2, skin=0.15)
+    for r, system in enumerate(systems):
+        sim = batch.replica(r)
+        sim.minimise(max_iterations=150, force_tolerance=1000.0)
+        sim.set_state(sim.read_positions().astype(np.float64), R.maxwell_boltzmann(system, 300.0, seeds[r]), step=0)
+    batch.run(EQUILIBRATE)
+    if screen:
+        batch.configure_screen(training_moves=TRAINING_MOVES, training_steps=3000, learning_rate=2e-3, seed=7)
+        while batch.screen_stats()["phase"] == 0:
+            batch.run(10)
+    volumes = np.zeros((SAMPLES, len(systems)))
+    for k in range(SAMPLES):
+        batch.run(10)
+        volumes[k] = [batch.replica(r).diagnostics()["volume"] for r in range(len(systems))]
+    stats = batch.screen_stats() if screen else None
+    masses = np.array([float(s.masses.sum()) for s in systems])
+    batch.release()
+    return volumes, stats, masses
+
+
+def test_batch_screen_skips_exact_evaluations_and_keeps_the_ensemble():
+    plain, _, masses = _run(False)
+    screened, st, _ = _run(True)
+    print("screen", st)
+    assert st["phase"] == 1 and st["samples"] >= 2 * TRAINING_MOVES
+    assert st["moves"] > 2000
+    skipped = 1.0 - st["exact_evaluations"] / st["moves"]
+    print("skipped fraction", skipped)
+    assert skipped > 0.15
+    # the network predicts the energy change better than the first-order
+    # estimate it corrects
+    assert st["residual_rms"] < 0.8 * st["estimate_rms"]
+    rho_plain = npt.density(plain, 1.0) * masses   # g/cm^3 per replica and sample
+    rho_screened = npt.density(screened, 1.0) * masses
+    tau = max(max(npt.autocorrelation_time(rho_plain[:, r]), npt.autocorrelation_time(rho_screened[:, r]))
+              for r in range(plain.shape[1]))
+    d, p = npt.ks_two_sample(npt.thin(rho_plain, tau).ravel(), npt.thin(rho_screened, tau).ravel())
+    diff = rho_screened.mean(0) - rho_plain.mean(0)
+    sigma = abs(diff.mean()) / (diff.std(ddof=1) / np.sqrt(len(diff)))
+    # each replica has only a few correlation times of samples: the
+    # compressibilities are compared pooled over the replicas
+    kappa_ratio = (sum(npt.compressibility(screened[:, r], 300.0) for r in range(plain.shape[1]))
+                   / sum(npt.compressibility(plain[:, r], 300.0) for r in range(plain.shape[1])))
+    print("tau", tau, "ks", d, p, "paired density difference", diff.mean(), "sigma", sigma,
+          "pooled compressibility ratio", kappa_ratio)
+    assert p > 1e-4
+    assert sigma < 4.0
+    assert 0.5 < kappa_ratio < 2.0
