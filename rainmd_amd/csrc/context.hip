@@ -141,7 +141,7 @@ struct rmd_ctx {
   PmeDev p{};
   BoxF *boxDev = nullptr;
   double *dModuli[3] = {nullptr, nullptr, nullptr};
-  float4 *posSaved = nullptr, *velSaved = nullptr, *centroids = nullptr;
+  float4 *posSaved = nullptr, *velSaved = nullptr, *centroids = nullptr, *rebuildSaved = nullptr;
   long long *forceSaved = nullptr;
   Scalars *scSaved = nullptr;
   int *molStart = nullptr, *molAtoms = nullptr, *molOfAtom = nullptr;
@@ -168,6 +168,7 @@ struct rmd_ctx {
   int64_t stepsSinceBarostat = 0;
   int neighbourGrowths = 0;
   unsigned int worstRequired = 0;
+  unsigned int rebuildCount = 0;  // Scalars::rebuilds at the last read-back
   int64_t kineticSampledAtStep = -1;
   int64_t stepsRun = 0;
   bool initialised = false;
@@ -418,18 +419,18 @@ void setupIntegrator(rmd_ctx *c, int suppliedNoise) {
   c->ip.maxIterations = c->maxIterations;
   c->ip.suppliedNoise = suppliedNoise;
   c->ip.clusterLdsBytes = (int)(c->clusterLdsSlots * CLUSTER_SLOT * sizeof(double));
-  const double half = std::max(0.0, 0.5 * c->plan.skin - 1e-4);
-  c->ip.dispThreshold2 = half * half;
+  c->ip.dispHalf = std::max(0.0, 0.5 * c->plan.skin - 1e-4);
 }
 
 // ---- force evaluation ---------------------------------------------------
 
-__global__ void decideRebuildKernel(Scalars *sc, float threshold2, int force) {
+__global__ void decideRebuildKernel(Scalars *sc, double dispHalf, int force) {
   const float maxDisp2 = __uint_as_float(sc->maxDisp2Bits);
-  const bool rebuild = force || sc->forceRebuild || !(maxDisp2 <= threshold2);
+  const bool rebuild = force || sc->forceRebuild || displacementExceeded(sc, maxDisp2, dispHalf);
   sc->rebuild = rebuild ? 1u : 0u;
   if (rebuild) {
     sc->maxDisp2Bits = 0u;
+    sc->slack = 0.f;
     sc->required = 0u;
     sc->rebuilds += 1u;
     sc->forceRebuild = 0u;
@@ -492,7 +493,7 @@ void checkScalars(rmd_ctx *c, Scalars &h) {
 void evaluate(rmd_ctx *c, int terms, bool forceRebuild, double terms_out[E_TERM_COUNT]) {
   for (int attempt = 0; attempt < 4; ++attempt) {
     HIP_CHECK(hipMemsetAsync(&c->d.sc->requiredMax, 0, sizeof(unsigned int), c->stream));
-    decideRebuildKernel<<<1, 1, 0, c->stream>>>(c->d.sc, (float)c->ip.dispThreshold2, forceRebuild ? 1 : 0);
+    decideRebuildKernel<<<1, 1, 0, c->stream>>>(c->d.sc, c->ip.dispHalf, forceRebuild ? 1 : 0);
     launchZeroForces(c->d, c->stream);
     launchRefreshSlots(c->d, c->stream);
     int slots[E_TERM_COUNT];
@@ -500,6 +501,7 @@ void evaluate(rmd_ctx *c, int terms, bool forceRebuild, double terms_out[E_TERM_
     HIP_CHECK(hipGetLastError());
     Scalars h;
     checkScalars(c, h);
+    c->rebuildCount = h.rebuilds;
     c->worstRequired = std::max(c->worstRequired, h.requiredMax);
     if (h.requiredMax > (unsigned int)c->d.cap) {
       const int grown = std::max(c->d.cap + 32, (int)std::ceil(h.requiredMax * 1.25));
@@ -696,6 +698,7 @@ void runBlock(rmd_ctx *c, int steps) {
                                        " steps that ended at step " + std::to_string(c->step + steps) +
                                        "; everything written during it should be discarded too.");
     if (h.shakeFailed) throw Error(RMD_E_CONSTRAINT, "constraints did not converge in the iteration limit");
+    c->rebuildCount = h.rebuilds;
     c->worstRequired = std::max(c->worstRequired, h.requiredMax);
     if (h.requiredMax > (unsigned int)c->d.cap) {
       // The list overflowed somewhere in the block: grow it and run the
@@ -721,7 +724,7 @@ void runBlock(rmd_ctx *c, int steps) {
 // function; the PME grid and alpha stay fixed.
 // The host half of a box change: the box, the cell plan and capacities, the
 // self energy. The device box and the influence function follow separately.
-void updateBoxHost(rmd_ctx *c, const double *nb) {
+bool updateBoxHost(rmd_ctx *c, const double *nb) {
   c->blockEndStep = -1;
   for (int k = 0; k < 9; ++k) c->box[k] = nb[k];
   CellPlan estimate = planCellList(c->sys, c->plan.skin, c->slack, 0, c->box);
@@ -729,11 +732,13 @@ void updateBoxHost(rmd_ctx *c, const double *nb) {
   CellPlan plan = planCellList(c->sys, c->plan.skin, c->slack, capacity, c->box);
   c->plan = plan;
   allocateCells(c, plan.cellCount);
-  if (capacity > c->d.cap) {
+  const bool replaced = capacity > c->d.cap;
+  if (replaced) {
     allocateNeighbours(c, capacity);
     ++c->neighbourGrowths;
   }
   ewaldSelfEnergy(c->sys, c->pme.alpha, volumeOf(c->box), &c->selfEnergy, &c->netEnergy);
+  return replaced;
 }
 
 void updateBox(rmd_ctx *c, const double *nb) {
@@ -750,6 +755,22 @@ double scaleVolume(rmd_ctx *c, double s, double *terms = nullptr) {
   updateBox(c, nb);
   double t[E_TERM_COUNT];
   evaluate(c, RMD_TERM_ALL, true, t);
+  if (terms) std::memcpy(terms, t, sizeof t);
+  return potentialFromTerms(c, t);
+}
+
+// The barostat's trial state: molecules scaled about their centroids (the
+// rebuild references with them, the move's share of the skin spent), the new
+// box, and an evaluation that rebuilds only if the list is due or was replaced.
+double moveVolume(rmd_ctx *c, double s, double *terms) {
+  launchScaleMolecules(c->d, c->molStart, c->molAtoms, c->sys.topology.moleculeCount, c->centroids, c->molOfAtom,
+                       (float)s, c->stream);
+  double nb[9];
+  for (int k = 0; k < 9; ++k) nb[k] = c->box[k] * s;
+  const bool replaced = updateBoxHost(c, nb);
+  writeBox(c);
+  double t[E_TERM_COUNT];
+  evaluate(c, RMD_TERM_ALL, replaced, t);
   if (terms) std::memcpy(terms, t, sizeof t);
   return potentialFromTerms(c, t);
 }
@@ -862,6 +883,7 @@ void create(rmd_ctx *c, const System &sys, const rmd_options &o, const Streams *
   d.force = devAlloc<long long>(3 * (size_t)d.nPad, c->held);
   d.posRebuild = devAlloc<float4>(d.nPad, c->held);
   c->posSaved = devAlloc<float4>(d.nPad, c->held);
+  c->rebuildSaved = devAlloc<float4>(d.nPad, c->held);
   c->velSaved = devAlloc<float4>(d.nPad, c->held);
   c->forceSaved = devAlloc<long long>(3 * (size_t)d.nPad, c->held);
   c->boxDev = devAlloc<BoxF>(1, c->held);
@@ -1350,7 +1372,10 @@ int rmd_attempt_volume_change(rmd_ctx *c, rmd_move *m) {
     double oldBox[9];
     std::memcpy(oldBox, c->box, sizeof oldBox);
     HIP_CHECK(hipMemcpyAsync(c->posSaved, c->d.pos, np * sizeof(float4), hipMemcpyDeviceToDevice, c->stream));
+    HIP_CHECK(hipMemcpyAsync(c->rebuildSaved, c->d.posRebuild, np * sizeof(float4), hipMemcpyDeviceToDevice,
+                             c->stream));
     HIP_CHECK(hipMemcpyAsync(c->forceSaved, c->d.force, 3 * np * sizeof(long long), hipMemcpyDeviceToDevice, c->stream));
+    const unsigned int rebuildsBefore = c->rebuildCount;
     c->barostatAttempts += 1;
     c->windowAttempts += 1;
     const uint32_t bseed = c->seed ^ 0x5a17u;
@@ -1396,7 +1421,7 @@ int rmd_attempt_volume_change(rmd_ctx *c, rmd_move *m) {
     }
     if (evaluated) {
       double newTerms[E_TERM_COUNT];
-      newPotential = scaleVolume(c, s, newTerms);
+      newPotential = moveVolume(c, s, newTerms);
       weight = (newPotential - oldPotential) + work;
       const double change = newPotential - oldPotential;
       if (sc.options.enabled) sc.exact += 1;
@@ -1431,11 +1456,19 @@ int rmd_attempt_volume_change(rmd_ctx *c, rmd_move *m) {
         std::memcpy(c->blockEndTerms, newTerms, sizeof newTerms);
       } else {
         // Back to the old state exactly: positions, box (copied, not divided
-        // back), forces; the old energies describe it again.
+        // back), forces; the old energies describe it again. The list stays
+        // with its rebuild references restored, unless the trial rebuilt it
+        // (then it describes the trial state) or replaced it.
         HIP_CHECK(hipMemcpyAsync(c->d.pos, c->posSaved, np * sizeof(float4), hipMemcpyDeviceToDevice, c->stream));
-        updateBox(c, oldBox);
+        const bool replaced = updateBoxHost(c, oldBox);
+        writeBox(c);
         HIP_CHECK(hipMemcpyAsync(c->d.force, c->forceSaved, 3 * np * sizeof(long long), hipMemcpyDeviceToDevice,
                                  c->stream));
+        if (replaced || c->rebuildCount != rebuildsBefore)
+          setForceRebuildKernel<<<1, 1, 0, c->stream>>>(c->d.sc);
+        else
+          HIP_CHECK(hipMemcpyAsync(c->d.posRebuild, c->rebuildSaved, np * sizeof(float4), hipMemcpyDeviceToDevice,
+                                   c->stream));
         HIP_CHECK(hipStreamSynchronize(c->stream));
         std::memcpy(c->blockEndTerms, oldTerms, sizeof oldTerms);
       }
@@ -1929,6 +1962,7 @@ void uploadViews(rmd_batch *b) {
     v.centroids = c->centroids;
     v.posSaved = c->posSaved;
     v.velSaved = c->velSaved;
+    v.rebuildSaved = c->rebuildSaved;
     v.forceSaved = c->forceSaved;
     for (int k = 0; k < 3; ++k) v.moduli[k] = c->dModuli[k];
     v.alpha = c->pme.alpha;
@@ -1942,6 +1976,7 @@ void clearFlags(rmd_batch *b) {
   for (auto &v : b->views) {
     v.rebuild = 0;
     v.restore = 0;
+    v.boxChanged = 0;
     v.scale = 1.0f;
   }
 }
@@ -2219,6 +2254,10 @@ void batchBarostat(rmd_batch *b) {
       }
     }
   }
+  // Rebuild counts before the trial: a replica whose trial evaluation rebuilt
+  // its list cannot keep it when the move is turned down.
+  std::vector<unsigned int> rebuildsBefore(R);
+  for (int r = 0; r < R; ++r) rebuildsBefore[r] = b->hostScalars[r].rebuilds;
   bool moved = false;
   for (int r = 0; r < R; ++r) {
     if (!tr[r].candidate) continue;
@@ -2226,16 +2265,17 @@ void batchBarostat(rmd_batch *b) {
     const double sc3 = std::cbrt(tr[r].newVolume / tr[r].volume);
     double nb[9];
     for (int k = 0; k < 9; ++k) nb[k] = c->box[k] * sc3;
-    updateBoxHost(c, nb);
+    const bool replaced = updateBoxHost(c, nb);
     if (!fitsSmallCellBuild(c->d.n, c->plan.cellCount))
       throw Error(RMD_E_INVALID, replicaLabel(r) + "the trial box needs more cells than a batch replica supports");
     b->views[r].scale = (float)sc3;
-    b->views[r].rebuild = 1;
+    b->views[r].boxChanged = 1;
+    b->views[r].rebuild = replaced ? 1 : 0;  // else the list is kept (rebuild references scaled, slack spent)
     tr[r].moved = moved = true;
   }
   if (moved) {
     uploadViews(b);
-    launchSnapshotBatch(b->dViews, b->tables[BK_ATOMS256], 0, s);
+    launchSnapshotBatch(b->dViews, b->tables[BK_ATOMS256], SNAP_REBUILD, s);
     launchScaleBatch(b->dViews, b->tables[BK_MOLECULES], b->tables[BK_ATOMS256], s);
     uploadBoxes(b);
     launchInfluenceBatch(b->dViews, b->tables[BK_INFLUENCE], s);
@@ -2295,11 +2335,13 @@ void batchBarostat(rmd_batch *b) {
       c->windowAccepted += 1;
       if (screening) sc.accepted += 1;
     } else {
-      // back to the old state exactly: positions, forces, the box (copied)
-      updateBoxHost(c, t.oldBox);
+      // back to the old state exactly: positions, forces, rebuild references,
+      // the box (copied); a list the trial rebuilt or replaced is rebuilt
+      const bool replaced = updateBoxHost(c, t.oldBox);
       std::memcpy(c->blockEndTerms, t.oldTerms, sizeof c->blockEndTerms);
       b->views[r].restore = 1;
-      b->views[r].rebuild = 1;
+      b->views[r].boxChanged = 1;
+      b->views[r].rebuild = (replaced || b->hostScalars[r].rebuilds != rebuildsBefore[r]) ? 1 : 0;
       restore = true;
     }
   }
@@ -2321,7 +2363,7 @@ void batchBarostat(rmd_batch *b) {
   }
   if (restore) {
     uploadViews(b);
-    launchSnapshotBatch(b->dViews, b->tables[BK_ATOMS256], SNAP_RESTORE | SNAP_FLAGGED, s);
+    launchSnapshotBatch(b->dViews, b->tables[BK_ATOMS256], SNAP_RESTORE | SNAP_FLAGGED | SNAP_REBUILD, s);
     uploadBoxes(b);
     launchInfluenceBatch(b->dViews, b->tables[BK_INFLUENCE], s);
     launchForceRebuildFlaggedBatch(b->dViews, R, s);

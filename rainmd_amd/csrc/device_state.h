@@ -49,8 +49,17 @@ struct Scalars {
   unsigned int cellOverflow;  // a cell held more atoms than the sort can take
   unsigned int nonFinite;     // a non-finite coordinate was seen
   unsigned int requiredMax;   // largest requirement since the host last cleared it
+  float slack;                // nm of the skin spent by volume moves since the last rebuild
+  unsigned int spanBits;      // largest atom-to-centroid distance of any molecule seen (float bits)
   long long step;             // the noise counter (completed steps)
 };
+
+// The rebuild test: the largest displacement since the last rebuild against
+// half the skin less half of what volume moves have spent of it (NaN rebuilds).
+__device__ __forceinline__ bool displacementExceeded(const Scalars *sc, float maxDisp2, double dispHalf) {
+  const float half = (float)dispHalf - 0.5f * sc->slack;
+  return !(half > 0.f) || !(maxDisp2 <= half * half);
+}
 
 struct Dev {
   int n;          // atoms

@@ -627,10 +627,11 @@ __device__ __forceinline__ void integrateOpen(const Dev &d, const IntegratorPara
     __threadfence();
     const unsigned int bits = atomicAdd(&d.sc->maxDisp2Bits, 0u);
     const float maxDisp2 = __uint_as_float(bits);
-    const bool rebuild = d.sc->forceRebuild || !(maxDisp2 <= (float)ip.dispThreshold2);
+    const bool rebuild = d.sc->forceRebuild || displacementExceeded(d.sc, maxDisp2, ip.dispHalf);
     d.sc->rebuild = rebuild ? 1u : 0u;
     if (rebuild) {
       d.sc->maxDisp2Bits = 0u;
+      d.sc->slack = 0.f;
       d.sc->required = 0u;
       d.sc->rebuilds += 1u;
       d.sc->forceRebuild = 0u;
@@ -785,10 +786,11 @@ __global__ void decideRebuildBatchKernel(const BatchView *__restrict__ v, int re
   if (r >= replicas) return;
   Scalars *sc = v[r].d.sc;
   const float maxDisp2 = __uint_as_float(sc->maxDisp2Bits);
-  const bool rebuild = force || v[r].rebuild || sc->forceRebuild || !(maxDisp2 <= (float)v[r].ip.dispThreshold2);
+  const bool rebuild = force || v[r].rebuild || sc->forceRebuild || displacementExceeded(sc, maxDisp2, v[r].ip.dispHalf);
   sc->rebuild = rebuild ? 1u : 0u;
   if (rebuild) {
     sc->maxDisp2Bits = 0u;
+    sc->slack = 0.f;
     sc->required = 0u;
     sc->rebuilds += 1u;
     sc->forceRebuild = 0u;
@@ -824,6 +826,14 @@ __device__ __forceinline__ void centroid(const Dev &d, const int *molStart, cons
     for (int c = 0; c < 3; ++c) w[c] /= (count > 0 ? count : 1);
   }
   centroids[m] = make_float4((float)w[0], (float)w[1], (float)w[2], 0.f);
+  // the molecule's span: its atoms' largest distance from the centroid
+  double span2 = 0.0;
+  for (int k = molStart[m]; k < molStart[m + 1]; ++k) {
+    const float4 p = d.pos[molAtoms[k]];
+    const double ex = p.x - w[0], ey = p.y - w[1], ez = p.z - w[2];
+    span2 = fmax(span2, ex * ex + ey * ey + ez * ez);
+  }
+  atomicMax(&d.sc->spanBits, __float_as_uint((float)sqrt(span2) * 1.0001f));
 }
 
 __global__ void centroidKernel(Dev d, const int *molStart, const int *molAtoms, int nMol, float4 *centroids) {
@@ -832,6 +842,15 @@ __global__ void centroidKernel(Dev d, const int *molStart, const int *molAtoms, 
 
 __device__ __forceinline__ void scaleAtom(const Dev &d, const float4 *centroids, const int *molOfAtom, float factor,
                                           int a) {
+  if (a == 0) {
+    // A pair's separation in the rebuild frame differs from the scaled
+    // frame's by |s - 1| times the molecules' centroid separation, which is
+    // at most rl + skin + 2 * span for a pair the list must hold.
+    const BoxF &b = *d.box;
+    const float rl = sqrtf(b.rl2), skin = rl - b.cutoff;
+    const float span = __uint_as_float(d.sc->spanBits);
+    d.sc->slack += fabsf(factor - 1.0f) * (rl + skin + 2.0f * span) * 1.01f;
+  }
   if (a >= d.n) return;
   const float4 c = centroids[molOfAtom[a]];
   float4 p = d.pos[a];
@@ -839,6 +858,11 @@ __device__ __forceinline__ void scaleAtom(const Dev &d, const float4 *centroids,
   p.y += (factor - 1.0f) * c.y;
   p.z += (factor - 1.0f) * c.z;
   d.pos[a] = p;
+  float4 r = d.posRebuild[a];
+  r.x += (factor - 1.0f) * c.x;
+  r.y += (factor - 1.0f) * c.y;
+  r.z += (factor - 1.0f) * c.z;
+  d.posRebuild[a] = r;
 }
 
 __global__ void scaleKernel(Dev d, const float4 *centroids, const int *molOfAtom, float factor) {
@@ -876,17 +900,19 @@ __global__ void __launch_bounds__(256) snapshotBatchKernel(const BatchView *__re
   if (mode & SNAP_RESTORE) {
     b.d.pos[a] = b.posSaved[a];
     if (mode & SNAP_VELOCITIES) b.d.vel[a] = b.velSaved[a];
+    if (mode & SNAP_REBUILD) b.d.posRebuild[a] = b.rebuildSaved[a];
     for (int c = 0; c < 3; ++c) b.d.force[c * np + a] = b.forceSaved[c * np + a];
   } else {
     b.posSaved[a] = b.d.pos[a];
     if (mode & SNAP_VELOCITIES) b.velSaved[a] = b.d.vel[a];
+    if (mode & SNAP_REBUILD) b.rebuildSaved[a] = b.d.posRebuild[a];
     for (int c = 0; c < 3; ++c) b.forceSaved[c * np + a] = b.d.force[c * np + a];
   }
 }
 
 __global__ void forceRebuildFlaggedBatchKernel(const BatchView *__restrict__ v, int replicas) {
   const int r = blockIdx.x * blockDim.x + threadIdx.x;
-  if (r < replicas && v[r].restore) v[r].d.sc->forceRebuild = 1u;
+  if (r < replicas && v[r].restore && v[r].rebuild) v[r].d.sc->forceRebuild = 1u;
 }
 
 __global__ void clearRequiredBatchKernel(Scalars *sc, int replicas) {

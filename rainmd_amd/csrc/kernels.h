@@ -64,7 +64,7 @@ constexpr int CLUSTER_SLOT = 13;        // doubles per cluster atom: x, ref, v, 
 constexpr int CLUSTER_LDS_SLOTS = 512;  // cluster atom slots per block held in LDS
 struct IntegratorParams {
   double halfStep, decay, noiseScale, kT, tolerance;
-  double dispThreshold2;  // rebuild when max |x - x_rebuild|^2 exceeds this
+  double dispHalf;  // the displacement since the last rebuild that triggers one: half the skin (less a margin)
   unsigned int seed;
   int maxIterations;
   int suppliedNoise;
@@ -82,7 +82,10 @@ struct SlotCounts {
 };
 void launchReduceEnergies(const Dev &d, const SlotCounts &counts, double *out, hipStream_t s);
 void launchZeroForces(const Dev &d, hipStream_t s);
-// Barostat: mass-weighted molecule centroids and the scaling about them.
+// Barostat: mass-weighted molecule centroids and the scaling about them. The
+// rebuild references are scaled with the positions, and the move's share of
+// the skin is added to Scalars::slack, so the neighbour list stays valid
+// without a rebuild.
 void launchScaleMolecules(const Dev &d, const int *molStart, const int *molAtoms, int nMol, float4 *centroids,
                           const int *molOfAtom, float scale, hipStream_t s);
 // Minimiser helpers.
@@ -105,11 +108,12 @@ struct BatchView {
   int rebuild;           // 1: this replica's lists are rebuilt at the coming evaluation
   float scale;           // the barostat's length scale for this replica (1: no move)
   int restore;           // 1: the barostat puts this replica back
+  int boxChanged;        // 1: recompute this replica's PME influence function
   SlotCounts counts;     // energy slot rows a block's last step fills
   double *terms;         // E_TERM_COUNT reduced energies
   const int *molStart, *molAtoms, *molOfAtom;
   int nMol;
-  float4 *centroids, *posSaved, *velSaved;
+  float4 *centroids, *posSaved, *velSaved, *rebuildSaved;
   long long *forceSaved;
   const double *moduli[3];
   double alpha;
@@ -145,9 +149,9 @@ void launchScaleBatch(const BatchView *v, const TaskTable &molecules, const Task
 // Saves (or restores) positions and forces, with SNAP_VELOCITIES the
 // velocities too, of every replica (with SNAP_FLAGGED, of those whose
 // `restore` is set).
-enum { SNAP_RESTORE = 1, SNAP_VELOCITIES = 2, SNAP_FLAGGED = 4 };
+enum { SNAP_RESTORE = 1, SNAP_VELOCITIES = 2, SNAP_FLAGGED = 4, SNAP_REBUILD = 8 };
 void launchSnapshotBatch(const BatchView *v, const TaskTable &atoms, int mode, hipStream_t s);
-// Sets Scalars::forceRebuild of the replicas whose `restore` is set.
+// Sets Scalars::forceRebuild of the replicas whose `restore` and `rebuild` are set.
 void launchForceRebuildFlaggedBatch(const BatchView *v, int replicas, hipStream_t s);
 // Clears Scalars::requiredMax of `replicas` consecutive scalar blocks.
 void launchClearRequiredBatch(Scalars *sc, int replicas, hipStream_t s);

@@ -57,6 +57,7 @@ __device__ __forceinline__ void writeSlot(const Dev &d, int k, int atom) {
 __global__ void setRebuildKernel(Scalars *sc) {
   sc->rebuild = 1u;
   sc->maxDisp2Bits = 0u;
+  sc->slack = 0.f;
   sc->required = 0u;
   sc->rebuilds += 1u;
 }

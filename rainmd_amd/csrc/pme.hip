@@ -834,11 +834,11 @@ __global__ void pmeInfluenceKernel(PmeDev p, const BoxF *box, const double *mod0
   pmeInfluence(p, box, mod0, mod1, mod2, alpha, (size_t)blockIdx.x * blockDim.x + threadIdx.x);
 }
 
-// Batched: the replicas whose box changed (`rebuild` set by the barostat).
+// Batched: the replicas whose box changed.
 __global__ void __launch_bounds__(256) pmeInfluenceBatchKernel(const BatchView *__restrict__ v,
                                                                const int2 *__restrict__ task) {
   const int2 t = task[blockIdx.x];
-  if (t.x < 0 || !v[t.x].rebuild) return;
+  if (t.x < 0 || !v[t.x].boxChanged) return;
   const BatchView &b = v[t.x];
   pmeInfluence(b.p, b.d.box, b.moduli[0], b.moduli[1], b.moduli[2], b.alpha,
                (size_t)t.y * blockDim.x + threadIdx.x);
