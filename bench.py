@@ -210,13 +210,10 @@ def bench_replicas(args, count):
     moves = sum(d["barostat_attempts"] for d in barostat)
     accepted = sum(d["barostat_accepted"] for d in barostat)
 
-    if args.single_steps <= 0:
-        batch.release()
-        return
     # one system per launch: every replica alone through its own context, the
     # same kernels and options, at the single system's 64 lanes per atom
     single_total = 0.0
-    for r in range(count):
+    for r in range(count if args.single_steps > 0 else 0):
         sim = batch.replica(r)
         lanes = sim.diagnostics()["lanes_per_atom"]
         sim.set_lanes(64)
@@ -225,10 +222,11 @@ def bench_replicas(args, count):
         sim.run(args.single_steps, args.block)
         single_total += time.perf_counter() - t0
         sim.set_lanes(lanes)
-    single_ms = 1e3 * single_total / args.single_steps
-    single_aggregate = count * TIMESTEP_PS * 1e-3 * 86400.0 / (single_ms * 1e-3)
-    log(f"one system per launch: {single_ms:.3f} ms per step of all replicas, aggregate {single_aggregate:.1f} ns/day; "
-        f"batch speedup {aggregate / single_aggregate:.2f}x")
+    single_ms = 1e3 * single_total / args.single_steps if args.single_steps > 0 else None
+    single_aggregate = count * TIMESTEP_PS * 1e-3 * 86400.0 / (single_ms * 1e-3) if single_ms else None
+    if single_ms:
+        log(f"one system per launch: {single_ms:.3f} ms per step of all replicas, aggregate {single_aggregate:.1f} "
+            f"ns/day; batch speedup {aggregate / single_aggregate:.2f}x")
     result = {
         "metric": METRIC,
         "value": aggregate,
@@ -259,8 +257,8 @@ def bench_replicas(args, count):
             "ms_per_step": single_ms,
             "sample": f"every replica run alone for {args.single_steps} steps through its own context "
                       "(the batch's options; graphs, concurrent force branches and 64 lanes per atom)",
-        },
-        "batch_speedup": aggregate / single_aggregate,
+        } if single_ms else None,
+        "batch_speedup": aggregate / single_aggregate if single_ms else None,
         "barostat": {"attempts": moves, "accepted": accepted},
         "roofline": {
             "kernel": "batched nonbonded (LJ + erfc real space, every replica in one launch)",

@@ -266,8 +266,11 @@ typedef struct rmd_energy_terms {
   double box[9];
   /* d(energy)/d(ln lambda) when every coordinate and the box scale by lambda:
    * direct-space pairs (sum r dU/dr) and the PME reciprocal sum. Filled by
-   * evaluations (0 where not computed); features of the barostat's screen. */
+   * evaluations (0 where not computed); features of the barostat's screen.
+   * dscale_molecular: the derivative when molecules are scaled about their
+   * centroids with their geometry kept (the barostat's move), every term. */
   double dscale_direct, dscale_reciprocal;
+  double dscale_molecular;
 } rmd_energy_terms;
 int rmd_energy(rmd_ctx *ctx, rmd_energy_terms *out);
 

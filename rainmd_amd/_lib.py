@@ -151,6 +151,7 @@ class EnergyTerms(C.Structure):
         ("volume", C.c_double),
         ("box", C.c_double * 9),
         ("dscale_direct", C.c_double), ("dscale_reciprocal", C.c_double),
+        ("dscale_molecular", C.c_double),
     ]
 
 

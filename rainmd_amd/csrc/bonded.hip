@@ -33,7 +33,10 @@ __device__ __forceinline__ void loadPos(const Dev &d, int a, double &x, double &
 }
 
 // Harmonic bond, (1/2) k (r - r0)^2 (cpu/forces/bonded.js:22-46).
-__device__ double bondTerm(const Dev &d, int t) {
+// Each term also adds sum_a F_a . (r_a - r_ref) over its atoms to `vir` (the
+// negative of its scaling derivative; translation invariant since its forces
+// sum to zero).
+__device__ double bondTerm(const Dev &d, int t, double &vir) {
   const int2 ab = d.bondAtoms[t];
   const double2 kp = d.bondParams[t];
   double ax, ay, az, bx, by, bz;
@@ -45,12 +48,13 @@ __device__ double bondTerm(const Dev &d, int t) {
   const double scale = r > 0 ? (double)kp.x * stretch / r : 0.0;
   addForce(d, ab.x, -scale * dx, -scale * dy, -scale * dz);
   addForce(d, ab.y, scale * dx, scale * dy, scale * dz);
+  vir -= scale * r * r;
   return 0.5 * (double)kp.x * stretch * stretch;
 }
 
 // Harmonic angle via atan2 with the clamped cross product
 // (cpu/forces/bonded.js:58-114).
-__device__ double angleTerm(const Dev &d, int t) {
+__device__ double angleTerm(const Dev &d, int t, double &vir) {
   const int4 abc = d.angleAtoms[t];
   const double2 kp = d.angleParams[t];
   double ax, ay, az, bx, by, bz, cx, cy, cz;
@@ -73,12 +77,13 @@ __device__ double angleTerm(const Dev &d, int t) {
   addForce(d, abc.x, fax, fay, faz);
   addForce(d, abc.z, fcx, fcy, fcz);
   addForce(d, abc.y, -(fax + fcx), -(fay + fcy), -(faz + fcz));
+  vir += fax * v1x + fay * v1y + faz * v1z + fcx * v2x + fcy * v2y + fcz * v2z;
   return 0.5 * (double)kp.x * dev * dev;
 }
 
 // Periodic torsion k (1 + cos(n phi - phase)) with the reference's middle-atom
 // coefficients (cpu/forces/bonded.js:126-207).
-__device__ double torsionTerm(const Dev &d, int t) {
+__device__ double torsionTerm(const Dev &d, int t, double &vir) {
   const int4 q = d.torsionAtoms[t];
   const double4 kp = d.torsionParams[t];
   double ax, ay, az, bx, by, bz, cx, cy, cz, dx_, dy_, dz_;
@@ -107,10 +112,15 @@ __device__ double torsionTerm(const Dev &d, int t) {
   const double inv = 1.0 / (b2l * b2l);
   const double tt = (b1x * b2x + b1y * b2y + b1z * b2z) * inv;
   const double u = (b3x * b2x + b3y * b2y + b3z * b2z) * inv;
+  const double fbx = -(1 + tt) * fax + u * fdx, fby = -(1 + tt) * fay + u * fdy, fbz = -(1 + tt) * faz + u * fdz;
+  const double fcx = tt * fax - (1 + u) * fdx, fcy = tt * fay - (1 + u) * fdy, fcz = tt * faz - (1 + u) * fdz;
   addForce(d, q.x, fax, fay, faz);
   addForce(d, q.w, fdx, fdy, fdz);
-  addForce(d, q.y, -(1 + tt) * fax + u * fdx, -(1 + tt) * fay + u * fdy, -(1 + tt) * faz + u * fdz);
-  addForce(d, q.z, tt * fax - (1 + u) * fdx, tt * fay - (1 + u) * fdy, tt * faz - (1 + u) * fdz);
+  addForce(d, q.y, fbx, fby, fbz);
+  addForce(d, q.z, fcx, fcy, fcz);
+  // relative to atom a: b - a = b1, c - a = b1 + b2, d - a = b1 + b2 + b3
+  vir += fbx * b1x + fby * b1y + fbz * b1z + fcx * (b1x + b2x) + fcy * (b1y + b2y) + fcz * (b1z + b2z) +
+         fdx * (b1x + b2x + b3x) + fdy * (b1y + b2y + b3y) + fdz * (b1z + b2z + b3z);
   return k * (1.0 + cos(arg));
 }
 
@@ -119,7 +129,8 @@ __device__ double torsionTerm(const Dev &d, int t) {
 // FP64: a solvated system has one such pair per atom, and a rigid water's
 // corrections are the same large number (~150 kJ/mol) for every water, so an
 // FP32 rounding bias would add up coherently across the box.
-__device__ void exceptionTerm(const Dev &d, const BoxF &box, double alpha, int t, double &elj, double &ecoul) {
+__device__ void exceptionTerm(const Dev &d, const BoxF &box, double alpha, int t, double &elj, double &ecoul,
+                              double &vir) {
   const int2 ij = d.excAtoms[t];
   const double4 prm = d.excParams[t];  // qq scaled, A, B, qq full (both with 1/4 pi eps0)
   double xi, yi, zi, xj, yj, zj;
@@ -158,6 +169,7 @@ __device__ void exceptionTerm(const Dev &d, const BoxF &box, double alpha, int t
   }
   addForce(d, ij.x, scalar * dx, scalar * dy, scalar * dz);
   addForce(d, ij.y, -scalar * dx, -scalar * dy, -scalar * dz);
+  vir += scalar * r2;
 }
 
 // Positional restraint k |minImage(r - r0)|^2, no half
@@ -186,24 +198,25 @@ __device__ double restraintTerm(const Dev &d, const BoxF &box, int t) {
   return k * (dx * dx + dy * dy + dz * dz);
 }
 
-enum { BT_BOND = 0, BT_ANGLE, BT_TORSION, BT_LJX, BT_COULX, BT_RESTRAINT, BT_COUNT };
+enum { BT_BOND = 0, BT_ANGLE, BT_TORSION, BT_LJX, BT_COULX, BT_RESTRAINT, BT_DSCALE, BT_COUNT };
 
 __device__ __forceinline__ void bonded(const Dev &d, double alphaD, int terms, int energies, int block) {
   const BoxF &box = *d.box;
   const int t = block * blockDim.x + threadIdx.x;
-  double e[BT_COUNT] = {0, 0, 0, 0, 0, 0};
+  double e[BT_COUNT] = {0, 0, 0, 0, 0, 0, 0};
+  double vir = 0.0;
   int base = 0;
   const bool doBonded = terms & RMD_TERM_BONDED;
   if (t < base + d.nBond) {
-    if (doBonded) e[BT_BOND] = bondTerm(d, t - base);
+    if (doBonded) e[BT_BOND] = bondTerm(d, t - base, vir);
   } else if ((base += d.nBond, t < base + d.nAngle)) {
-    if (doBonded) e[BT_ANGLE] = angleTerm(d, t - base);
+    if (doBonded) e[BT_ANGLE] = angleTerm(d, t - base, vir);
   } else if ((base += d.nAngle, t < base + d.nTorsion)) {
-    if (doBonded) e[BT_TORSION] = torsionTerm(d, t - base);
+    if (doBonded) e[BT_TORSION] = torsionTerm(d, t - base, vir);
   } else if ((base += d.nTorsion, t < base + d.nException)) {
     if (terms & RMD_TERM_EXCEPTIONS) {
       double elj = 0.0, ecoul = 0.0;
-      exceptionTerm(d, box, alphaD, t - base, elj, ecoul);
+      exceptionTerm(d, box, alphaD, t - base, elj, ecoul, vir);
       e[BT_LJX] = elj;
       e[BT_COULX] = ecoul;
     }
@@ -211,6 +224,7 @@ __device__ __forceinline__ void bonded(const Dev &d, double alphaD, int terms, i
     if (terms & RMD_TERM_RESTRAINT) e[BT_RESTRAINT] = restraintTerm(d, box, t - base);
   }
   if (!energies) return;
+  e[BT_DSCALE] = -vir;
   __shared__ double partial[BT_COUNT][4];
   const int wave = threadIdx.x >> 6;
   for (int k = 0; k < BT_COUNT; ++k) {
@@ -226,7 +240,8 @@ __device__ __forceinline__ void bonded(const Dev &d, double alphaD, int terms, i
                      : threadIdx.x == BT_TORSION ? E_TORSION
                      : threadIdx.x == BT_LJX ? E_LJ_EXCEPTION
                      : threadIdx.x == BT_COULX ? E_COUL_EXCEPTION
-                                               : E_RESTRAINT;
+                     : threadIdx.x == BT_RESTRAINT ? E_RESTRAINT
+                                                   : E_DSCALE_BONDED;
     d.slots[term * MAX_SLOTS + block] = s;
   }
 }

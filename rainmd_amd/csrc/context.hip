@@ -465,7 +465,7 @@ void enqueueForces(rmd_ctx *c, int terms, int energies, int slots[E_TERM_COUNT])
     int b = 0;
     launchBonded(d, c->pme.alpha, terms, energies, bondedStream, &b);
     slots[E_BOND] = slots[E_ANGLE] = slots[E_TORSION] = slots[E_LJ_EXCEPTION] = slots[E_COUL_EXCEPTION] =
-        slots[E_RESTRAINT] = b;
+        slots[E_RESTRAINT] = slots[E_DSCALE_BONDED] = b;
   }
   launchRebuild(d, c->stream);
   if (terms & RMD_TERM_DIRECT) {
@@ -478,6 +478,9 @@ void enqueueForces(rmd_ctx *c, int terms, int energies, int slots[E_TERM_COUNT])
       HIP_CHECK(hipStreamWaitEvent(c->stream, c->join[k], 0));
     }
   }
+  if (energies)
+    slots[E_DSCALE_MOLECULAR] =
+        launchMolecularTerm(d, c->molStart, c->molAtoms, c->sys.topology.moleculeCount, c->stream);
 }
 
 void checkScalars(rmd_ctx *c, Scalars &h) {
@@ -535,6 +538,13 @@ double potentialFromTerms(rmd_ctx *c, const double *t) {
 
 // ---- the barostat's learned screen ------------------------------------------
 
+// d(energy)/d(ln lambda) when molecules are scaled about their centroids with
+// their geometry kept: the atomic scaling derivatives of every term plus
+// sum_i F_i . (r_i - c_mol(i)).
+double molecularDerivative(const double *t) {
+  return t[E_DSCALE_DIRECT] + t[E_DSCALE_RECIPROCAL] + t[E_DSCALE_BONDED] + t[E_DSCALE_MOLECULAR];
+}
+
 double dispersionAt(rmd_ctx *c, double volume) {
   return c->dispersionOn ? dispersionEnergy(c->disp, volume, c->sys.cutoff) : 0.0;
 }
@@ -548,7 +558,8 @@ void screenFeatures(rmd_ctx *c, const double *t, double volume, double newVolume
   const double nkT = std::max(1, c->sys.topology.moleculeCount) * kT;
   const double L = std::log(newVolume / volume);
   const double dD = t[E_DSCALE_DIRECT], dR = t[E_DSCALE_RECIPROCAL];
-  const double est = (dD + dR) * L / 3.0;
+  // first order in ln(lambda) under the barostat's molecular scaling
+  const double est = molecularDerivative(t) * L / 3.0;
   *estimate = est;
   double g[RMD_SURROGATE_INPUTS];
   g[0] = 100.0 * L;
@@ -804,6 +815,7 @@ void fillEnergy(rmd_ctx *c, const double *t, rmd_energy_terms *e, bool withKinet
   e->potential = e->bonded + e->lennard_jones + e->dispersion + e->coulomb + e->position_restraint;
   e->dscale_direct = t[E_DSCALE_DIRECT];
   e->dscale_reciprocal = t[E_DSCALE_RECIPROCAL];
+  e->dscale_molecular = molecularDerivative(t);
   if (withKinetic) {
     e->kinetic = kinetic;
     e->total = e->potential + kinetic;
@@ -1952,7 +1964,8 @@ void uploadViews(rmd_batch *b) {
     const int nb = replicaBlocks(c, BK_NONBONDED), bb = replicaBlocks(c, BK_BONDED), xb = replicaBlocks(c, BK_XLINES);
     v.counts.c[E_LJ_DIRECT] = v.counts.c[E_COUL_DIRECT] = v.counts.c[E_DSCALE_DIRECT] = nb;
     v.counts.c[E_BOND] = v.counts.c[E_ANGLE] = v.counts.c[E_TORSION] = v.counts.c[E_LJ_EXCEPTION] =
-        v.counts.c[E_COUL_EXCEPTION] = v.counts.c[E_RESTRAINT] = bb;
+        v.counts.c[E_COUL_EXCEPTION] = v.counts.c[E_RESTRAINT] = v.counts.c[E_DSCALE_BONDED] = bb;
+    v.counts.c[E_DSCALE_MOLECULAR] = replicaBlocks(c, BK_MOLECULES);
     v.counts.c[E_RECIPROCAL] = v.counts.c[E_DSCALE_RECIPROCAL] = xb;
     v.terms = b->termsAll + (size_t)r * E_TERM_COUNT;
     v.molStart = c->molStart;
@@ -2049,6 +2062,7 @@ void enqueueBatchForces(rmd_batch *b, int energies, int terms = RMD_TERM_ALL) {
       HIP_CHECK(hipStreamWaitEvent(st.stream, st.join[k], 0));
     }
   }
+  if (energies) launchMolecularTermBatch(b->dViews, b->tables[BK_MOLECULES], st.stream);
 }
 
 void enqueueBatchBlock(rmd_batch *b, int steps) {

@@ -31,6 +31,12 @@ enum EnergyTerm {
   // barostat's learned screen reads them as features.
   E_DSCALE_DIRECT,
   E_DSCALE_RECIPROCAL,
+  // The same for the bonded and exception terms (all within molecules), and
+  // sum_i F_i . (r_i - c_mol(i)) with the total forces: together with the
+  // two above, the derivative under the barostat's molecular scaling, where
+  // intramolecular geometry is kept.
+  E_DSCALE_BONDED,
+  E_DSCALE_MOLECULAR,
   E_TERM_COUNT
 };
 constexpr int MAX_SLOTS = 65536;

@@ -836,6 +836,60 @@ __device__ __forceinline__ void centroid(const Dev &d, const int *molStart, cons
   atomicMax(&d.sc->spanBits, __float_as_uint((float)sqrt(span2) * 1.0001f));
 }
 
+// sum_i F_i . (r_i - c_mol(i)) over molecule m's atoms with the total forces
+// (its share of the molecular-scaling correction, E_DSCALE_MOLECULAR), about
+// the same mass-weighted centroid as the barostat's, in FP64.
+__device__ __forceinline__ double molecularTerm(const Dev &d, const int *molStart, const int *molAtoms, int nMol,
+                                                int m) {
+  if (m >= nMol) return 0.0;
+  double w[3] = {0, 0, 0}, total = 0;
+  for (int k = molStart[m]; k < molStart[m + 1]; ++k) {
+    const int a = molAtoms[k];
+    const float4 p = d.pos[a];
+    const double im = d.vel[a].w;
+    const double mass = im > 0 ? 1.0 / im : 0.0;
+    w[0] += mass * p.x;
+    w[1] += mass * p.y;
+    w[2] += mass * p.z;
+    total += mass;
+  }
+  const int count = molStart[m + 1] - molStart[m];
+  if (total > 0) {
+    for (int c = 0; c < 3; ++c) w[c] /= total;
+  } else {
+    w[0] = w[1] = w[2] = 0.0;
+    for (int k = molStart[m]; k < molStart[m + 1]; ++k) {
+      const float4 p = d.pos[molAtoms[k]];
+      w[0] += p.x;
+      w[1] += p.y;
+      w[2] += p.z;
+    }
+    for (int c = 0; c < 3; ++c) w[c] /= (count > 0 ? count : 1);
+  }
+  double s = 0.0;
+  for (int k = molStart[m]; k < molStart[m + 1]; ++k) {
+    const int a = molAtoms[k];
+    const float4 p = d.pos[a];
+    s += (double)d.force[a] * FORCE_UNSCALE * (p.x - w[0]) + (double)d.force[d.nPad + a] * FORCE_UNSCALE * (p.y - w[1]) +
+         (double)d.force[2 * d.nPad + a] * FORCE_UNSCALE * (p.z - w[2]);
+  }
+  return s;
+}
+
+__global__ void __launch_bounds__(256) molecularTermKernel(Dev d, const int *molStart, const int *molAtoms, int nMol) {
+  const double s = molecularTerm(d, molStart, molAtoms, nMol, blockIdx.x * blockDim.x + threadIdx.x);
+  blockSumToSlot(s, &d.slots[E_DSCALE_MOLECULAR * MAX_SLOTS + blockIdx.x]);
+}
+
+__global__ void __launch_bounds__(256) molecularTermBatchKernel(const BatchView *__restrict__ v,
+                                                                const int2 *__restrict__ task) {
+  const int2 t = task[blockIdx.x];
+  if (t.x < 0) return;
+  const BatchView &b = v[t.x];
+  const double s = molecularTerm(b.d, b.molStart, b.molAtoms, b.nMol, t.y * blockDim.x + threadIdx.x);
+  blockSumToSlot(s, &b.d.slots[E_DSCALE_MOLECULAR * MAX_SLOTS + t.y]);
+}
+
 __global__ void centroidKernel(Dev d, const int *molStart, const int *molAtoms, int nMol, float4 *centroids) {
   centroid(d, molStart, molAtoms, nMol, centroids, blockIdx.x * blockDim.x + threadIdx.x);
 }
@@ -1084,6 +1138,16 @@ void launchDecideRebuildBatch(const BatchView *v, int replicas, int force, hipSt
 void launchScaleBatch(const BatchView *v, const TaskTable &molecules, const TaskTable &atoms, hipStream_t s) {
   centroidBatchKernel<<<molecules.blocks, 256, 0, s>>>(v, molecules.task);
   scaleBatchKernel<<<atoms.blocks, 256, 0, s>>>(v, atoms.task);
+}
+
+int launchMolecularTerm(const Dev &d, const int *molStart, const int *molAtoms, int nMol, hipStream_t s) {
+  const int blocks = (nMol + 255) / 256 > 0 ? (nMol + 255) / 256 : 1;
+  molecularTermKernel<<<blocks, 256, 0, s>>>(d, molStart, molAtoms, nMol);
+  return blocks;
+}
+
+void launchMolecularTermBatch(const BatchView *v, const TaskTable &molecules, hipStream_t s) {
+  molecularTermBatchKernel<<<molecules.blocks, 256, 0, s>>>(v, molecules.task);
 }
 
 void launchSnapshotBatch(const BatchView *v, const TaskTable &atoms, int mode, hipStream_t s) {

@@ -88,6 +88,9 @@ void launchZeroForces(const Dev &d, hipStream_t s);
 // without a rebuild.
 void launchScaleMolecules(const Dev &d, const int *molStart, const int *molAtoms, int nMol, float4 *centroids,
                           const int *molOfAtom, float scale, hipStream_t s);
+// The molecular-scaling correction, sum_i F_i . (r_i - c_mol(i)), into the
+// E_DSCALE_MOLECULAR slot row (after every force kernel); returns the rows used.
+int launchMolecularTerm(const Dev &d, const int *molStart, const int *molAtoms, int nMol, hipStream_t s);
 // Minimiser helpers.
 void launchMaxForce(const Dev &d, unsigned int *out, hipStream_t s);
 void launchDisplace(const Dev &d, float scale, float cap, hipStream_t s);
@@ -146,6 +149,7 @@ void launchReduceEnergiesBatch(const BatchView *v, int replicas, int withEarly, 
 void launchZeroForcesBatch(const BatchView *v, const TaskTable &t, hipStream_t s);
 void launchDecideRebuildBatch(const BatchView *v, int replicas, int force, hipStream_t s);
 void launchScaleBatch(const BatchView *v, const TaskTable &molecules, const TaskTable &atoms, hipStream_t s);
+void launchMolecularTermBatch(const BatchView *v, const TaskTable &molecules, hipStream_t s);
 // Saves (or restores) positions and forces, with SNAP_VELOCITIES the
 // velocities too, of every replica (with SNAP_FLAGGED, of those whose
 // `restore` is set).
