@@ -49,6 +49,11 @@ WORKLOADS = {
 }
 
 
+SCREEN_SEED = 505         # the learned screen's network initialisation
+PLAIN_SEED_OFFSET = 7919  # the plain-barostat comparison arm's replica seeds
+SAMPLE_STEPS = 250        # steps between the volume samples of the ensemble comparison
+
+
 def log(msg):
     print(msg, file=sys.stderr, flush=True)
 
@@ -168,25 +173,39 @@ def bench_replicas(args, count):
         batch.set_concurrent_forces(False)
     log(f"batch created in {time.perf_counter() - t0:.1f} s: {batch.info()}")
     t0 = time.perf_counter()
+    starts = []
     for r, system in enumerate(systems):
         sim = batch.replica(r)
         sim.minimise(max_iterations=args.replica_minimise, force_tolerance=1000.0)
-        sim.set_state(sim.read_positions().astype(np.float64), R.maxwell_boltzmann(system, 300.0, seeds[r]), step=0)
+        starts.append((sim.read_positions().astype(np.float64), R.maxwell_boltzmann(system, 300.0, seeds[r])))
+        sim.set_state(*starts[-1], step=0)
     log(f"replicas minimised in {time.perf_counter() - t0:.1f} s")
     t0 = time.perf_counter()
     batch.run(args.warmup, args.block)
     log(f"warm-up {args.warmup} steps in {time.perf_counter() - t0:.2f} s")
+    collection_steps = 0
     if args.screen:
         # the shared learned screen, trained on the first exact moves of all replicas
         batch.configure_screen(training_moves=args.screen_training_moves, training_steps=4000, learning_rate=1e-3,
-                               seed=7)
+                               seed=SCREEN_SEED)
         t0 = time.perf_counter()
         while batch.screen_stats()["phase"] == 0:
             batch.run(args.block, args.block)
-        log(f"screen trained in {time.perf_counter() - t0:.1f} s: {batch.screen_stats()}")
-    t0 = time.perf_counter()
-    batch.run(args.steps, args.block)
-    elapsed = time.perf_counter() - t0
+            collection_steps += args.block
+        log(f"screen trained in {time.perf_counter() - t0:.1f} s after {collection_steps} steps: "
+            f"{batch.screen_stats()}")
+    # the timed run; with the screen, in chunks with every replica's volume
+    # read between them (outside the clock) for the ensemble comparison
+    chunk = SAMPLE_STEPS if args.screen else args.steps
+    volumes = []
+    elapsed = 0.0
+    for done in range(0, args.steps, chunk):
+        steps = min(chunk, args.steps - done)
+        t0 = time.perf_counter()
+        batch.run(steps, args.block)
+        elapsed += time.perf_counter() - t0
+        if args.screen:
+            volumes.append([batch.replica(r).diagnostics()["volume"] for r in range(count)])
     ms_per_step = 1e3 * elapsed / args.steps
     aggregate = count * TIMESTEP_PS * 1e-3 * 86400.0 / (ms_per_step * 1e-3)
     log(f"timed {args.steps} batch steps: {ms_per_step:.3f} ms/step, aggregate {aggregate:.1f} ns/day")
@@ -280,9 +299,63 @@ def bench_replicas(args, count):
     if args.screen:
         st = batch.screen_stats()
         st["skipped_fraction"] = 1.0 - st["exact_evaluations"] / max(1, st["moves"])
+        st["network_seed"] = SCREEN_SEED
         result["screen"] = st
     batch.release()
+    if args.screen:
+        result["screen"]["ensemble"] = plain_barostat_comparison(args, systems, starts, extra, collection_steps,
+                                                                 np.array(volumes))
     print(json.dumps(result), flush=True)
+
+
+def plain_barostat_comparison(args, systems, starts, extra, collection_steps, screened):
+    """The same replicas from the same minimised states under the plain
+    barostat (independent seeds), stepped through the same protocol - warm-up,
+    the screen's collection steps, then the timed steps sampled at the same
+    points - and the two arms' densities compared: a KS test on the pooled
+    densities thinned to one sample per correlation time, the replicas' paired
+    mean densities, and the compressibilities pooled over the replicas."""
+    import rainmd_amd as R
+    from rainmd_amd import npt
+
+    count = len(systems)
+    seeds = [s + PLAIN_SEED_OFFSET for _, _, s in R.replica_workload(count)]
+    plain = R.Batch(systems, seeds=seeds, temperature=300.0, pressure=1.0, barostat_interval=25, **extra)
+    if args.no_graphs:
+        plain.set_graphs(False)
+    if args.serial_forces:
+        plain.set_concurrent_forces(False)
+    t0 = time.perf_counter()
+    for r, (x, v) in enumerate(starts):
+        plain.replica(r).set_state(x, v, step=0)
+    plain.run(args.warmup + collection_steps, args.block)
+    volumes = []
+    for done in range(0, args.steps, SAMPLE_STEPS):
+        plain.run(min(SAMPLE_STEPS, args.steps - done), args.block)
+        volumes.append([plain.replica(r).diagnostics()["volume"] for r in range(count)])
+    moves = sum(plain.replica(r).diagnostics()["barostat_attempts"] for r in range(count))
+    plain.release()
+    log(f"plain barostat arm: {args.warmup + collection_steps + args.steps} steps in {time.perf_counter() - t0:.1f} s")
+    volumes = np.array(volumes)
+    masses = np.array([float(s.masses.sum()) for s in systems])
+    rho_plain = npt.density(volumes, 1.0) * masses
+    rho_screened = npt.density(screened, 1.0) * masses
+    taus = [max(npt.autocorrelation_time(rho_plain[:, r]), npt.autocorrelation_time(rho_screened[:, r]))
+            for r in range(count)]
+    tau = float(np.median(taus))
+    d, p = npt.ks_two_sample(npt.thin(rho_plain, tau).ravel(), npt.thin(rho_screened, tau).ravel())
+    diff = rho_screened.mean(0) - rho_plain.mean(0)
+    sigma = float(abs(diff.mean()) / (diff.std(ddof=1) / np.sqrt(count)))
+    kappa = (sum(npt.compressibility(screened[:, r], 300.0) for r in range(count))
+             / sum(npt.compressibility(volumes[:, r], 300.0) for r in range(count)))
+    out = {"samples_per_replica": int(volumes.shape[0]), "sample_interval_steps": SAMPLE_STEPS,
+           "correlation_time_samples": tau, "ks_statistic": d, "ks_p": p,
+           "density_plain": float(rho_plain.mean()), "density_screened": float(rho_screened.mean()),
+           "paired_density_difference": float(diff.mean()), "paired_difference_sigma": sigma,
+           "pooled_compressibility_ratio": float(kappa), "plain_moves": int(moves),
+           "plain_seed_offset": PLAIN_SEED_OFFSET}
+    log(f"ensemble: {out}")
+    return out
 
 
 def main():
