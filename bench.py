@@ -198,6 +198,7 @@ def bench_replicas(args, count):
     # read between them (outside the clock) for the ensemble comparison
     chunk = SAMPLE_STEPS if args.screen else args.steps
     volumes = []
+    rebuilds = sum(batch.replica(r).diagnostics()["rebuilds"] for r in range(count))
     elapsed = 0.0
     for done in range(0, args.steps, chunk):
         steps = min(chunk, args.steps - done)
@@ -208,7 +209,9 @@ def bench_replicas(args, count):
             volumes.append([batch.replica(r).diagnostics()["volume"] for r in range(count)])
     ms_per_step = 1e3 * elapsed / args.steps
     aggregate = count * TIMESTEP_PS * 1e-3 * 86400.0 / (ms_per_step * 1e-3)
-    log(f"timed {args.steps} batch steps: {ms_per_step:.3f} ms/step, aggregate {aggregate:.1f} ns/day")
+    rebuilds = sum(batch.replica(r).diagnostics()["rebuilds"] for r in range(count)) - rebuilds
+    log(f"timed {args.steps} batch steps: {ms_per_step:.3f} ms/step, aggregate {aggregate:.1f} ns/day; "
+        f"{rebuilds} list rebuilds, one per {count * args.steps / max(1, rebuilds):.1f} replica steps")
 
     # the batched direct-space launch at the final state
     nb_ms = batch.profile_nonbonded(reps=20)
@@ -269,6 +272,7 @@ def bench_replicas(args, count):
             "skin_nm": batch.replica(0).diagnostics()["skin"],
             "lanes_per_atom": batch.replica(0).diagnostics()["lanes_per_atom"],
             "force_branches": "serial" if args.serial_forces else "concurrent",
+            "replica_steps_per_rebuild": count * args.steps / max(1, rebuilds),
         },
         "single_launch_baseline": {
             "value": single_aggregate,

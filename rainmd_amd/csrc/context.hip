@@ -1891,6 +1891,7 @@ struct rmd_batch {
   int ldsOpen = 0;
   double alpha = 0;
   int lanes = 16;  // direct-space lanes per atom of every member
+  int maxTypes = 0;  // the largest LJ type count of the members
   bool concurrent = true, useGraphs = true;
   std::map<int, hipGraphExec_t> graphs;
   std::atomic<int> stop{0};
@@ -2055,7 +2056,8 @@ void enqueueBatchForces(rmd_batch *b, int energies, int terms = RMD_TERM_ALL) {
     launchBondedBatch(b->dViews, b->tables[BK_BONDED], b->alpha, terms, energies, bondedStream);
   launchRebuildBatch(b->dViews, b->tables[BK_REPLICA], b->tables[BK_ATOMS64], st.stream);
   if (terms & RMD_TERM_DIRECT)
-    launchNonbondedBatch(b->dViews, b->tables[BK_NONBONDED], b->lanes, (float)b->alpha, energies, st.stream);
+    launchNonbondedBatch(b->dViews, b->tables[BK_NONBONDED], b->lanes, (float)b->alpha, energies, b->maxTypes,
+                         st.stream);
   if (fork) {
     for (int k = 0; k < 2; ++k) {
       HIP_CHECK(hipEventRecord(st.join[k], st.side[k]));
@@ -2469,6 +2471,7 @@ void createBatch(rmd_batch *b, const rmd_system *const *systems, int count, cons
     else if (c->pme.alpha != b->alpha)
       throw Error(RMD_E_INVALID, replicaLabel(r) + "every replica of a batch needs the same Ewald alpha (the same "
                                                    "cutoff and tolerance)");
+    b->maxTypes = std::max(b->maxTypes, c->d.types);
     b->ldsForward = std::max(b->ldsForward, pmePlaneForwardLds(c->p));
     b->ldsXlines = std::max(b->ldsXlines, pmeXLinesLds(c->p));
     b->ldsGather = std::max(b->ldsGather, pmePlaneGatherLds(c->p));
@@ -2654,13 +2657,13 @@ int rmd_batch_profile_nonbonded(rmd_batch *b, int32_t reps, double *avg_ms) {
     if (reps <= 0 || !avg_ms) throw Error(RMD_E_INVALID, "reps must be positive");
     hipStream_t s = b->streams.stream;
     uploadViews(b);
-    launchNonbondedBatch(b->dViews, b->tables[BK_NONBONDED], b->lanes, (float)b->alpha, 0, s);
+    launchNonbondedBatch(b->dViews, b->tables[BK_NONBONDED], b->lanes, (float)b->alpha, 0, b->maxTypes, s);
     hipEvent_t e0, e1;
     HIP_CHECK(hipEventCreate(&e0));
     HIP_CHECK(hipEventCreate(&e1));
     HIP_CHECK(hipEventRecord(e0, s));
     for (int k = 0; k < reps; ++k)
-      launchNonbondedBatch(b->dViews, b->tables[BK_NONBONDED], b->lanes, (float)b->alpha, 0, s);
+      launchNonbondedBatch(b->dViews, b->tables[BK_NONBONDED], b->lanes, (float)b->alpha, 0, b->maxTypes, s);
     HIP_CHECK(hipEventRecord(e1, s));
     HIP_CHECK(hipEventSynchronize(e1));
     float ms = 0;

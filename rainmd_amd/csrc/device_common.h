@@ -86,13 +86,60 @@ __device__ __forceinline__ double imageR2Double(float4 pi, float4 pj, const BoxF
   return ex * ex + ey * ey + ez * ez;
 }
 
+// A pointer into global memory typed as such: loads through it are
+// global_load (in-order completion counter) rather than flat loads, whose
+// completion the compiler can only wait for as a whole. For pointers the
+// compiler cannot place itself, such as those read from a view in memory.
+// (The host pass of the same source, which never runs device code, sees a
+// plain pointer.)
+#if defined(__HIP_DEVICE_COMPILE__)
+template <typename T>
+using GlobalPtr = const __attribute__((address_space(1))) T *;
+#else
+template <typename T>
+using GlobalPtr = const T *;
+#endif
+template <typename T>
+__device__ __forceinline__ GlobalPtr<T> globalPtr(const T *p) {
+  return (GlobalPtr<T>)p;
+}
+
+// A value the same in every lane of the wave (read from lane 0 or the first
+// active lane), held in a scalar register.
+__device__ __forceinline__ float uniformFloat(float x) {
+  return __int_as_float(__builtin_amdgcn_readfirstlane(__float_as_int(x)));
+}
+__device__ __forceinline__ double uniformDouble(double x) {
+  const long long bits = __double_as_longlong(x);
+  const int lo = __builtin_amdgcn_readfirstlane((int)(bits & 0xffffffffll));
+  const int hi = __builtin_amdgcn_readfirstlane((int)(bits >> 32));
+  return __longlong_as_double(((long long)hi << 32) | (unsigned)lo);
+}
+
+// The fields of a box the pair test reads (rows, inverse lengths, rc^2, and
+// with `rows64` the FP64 rows), in scalar registers: a box is uniform over a
+// workgroup. Call with every lane active.
+__device__ __forceinline__ BoxF uniformPairBox(const BoxF &g, bool rows64) {
+  BoxF b;
+  for (int k = 0; k < 3; ++k) {
+    b.a[k] = uniformFloat(g.a[k]);
+    b.b[k] = uniformFloat(g.b[k]);
+    b.c[k] = uniformFloat(g.c[k]);
+    b.inv[k] = uniformFloat(g.inv[k]);
+  }
+  b.rc2 = uniformFloat(g.rc2);
+  if (rows64)
+    for (int k = 0; k < 9; ++k) b.box64[k] = uniformDouble(g.box64[k]);
+  return b;
+}
+
 // erfc(x) for x >= 0 as exp(-x^2) * t * P(t), t = 1/(1 + x/2). P is a
 // degree-8 relative-error minimax fit of erfcx(x)/t over x in [0, 4] made for
 // this file (max fit error 2e-10; the f32 evaluation is ~1e-7 rms). x^2 is
 // split into hi + lo with an FMA so the exponential's argument is exact.
 // Also returns exp(-x^2), which the force needs.
 __device__ __forceinline__ float erfcWithGaussian(float x, float &gaussian) {
-  const float t = 1.0f / fmaf(0.5f, x, 1.0f);
+  const float t = __builtin_amdgcn_rcpf(fmaf(0.5f, x, 1.0f));  // 1 ulp
   float p = -4.261296242e-02f;
   p = fmaf(p, t, 1.989050657e-01f);
   p = fmaf(p, t, -2.959875166e-01f);

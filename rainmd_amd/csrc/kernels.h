@@ -164,7 +164,8 @@ bool fitsSmallCellBuild(int atoms, int cells);
 void launchRefreshSlotsBatch(const BatchView *v, const TaskTable &atoms256, hipStream_t s);
 void launchRebuildBatch(const BatchView *v, const TaskTable &replicas, const TaskTable &atoms64, hipStream_t s);
 // lanes: lanes per atom (8, 16, 32 or 64), the table's grid at that count.
-void launchNonbondedBatch(const BatchView *v, const TaskTable &t, int lanes, float alpha, int energies, hipStream_t s);
+void launchNonbondedBatch(const BatchView *v, const TaskTable &t, int lanes, float alpha, int energies, int types,
+                          hipStream_t s);
 void launchBondedBatch(const BatchView *v, const TaskTable &t, double alpha, int terms, int energies, hipStream_t s);
 void launchPmeBatch(const BatchView *v, const TaskTable &planes, const TaskTable &xlines, int energies,
                     size_t forwardLds, size_t xlinesLds, size_t gatherLds, hipStream_t s);

@@ -19,20 +19,72 @@
 
 namespace rmd {
 
-constexpr int LDS_TYPES = 64;  // type-pair tables up to 64^2 entries live in LDS
+// Type-pair tables up to 90^2 entries (64.8 KB) are copied to LDS, sized to
+// the system; larger ones are read from global memory.
+constexpr int LDS_TYPES = 90;
 
-template <int LANES, bool ENERGIES>
-__device__ __forceinline__ void nonbonded(const Dev &d, float alpha, int block) {
+inline bool tableInLds(int types) { return types <= LDS_TYPES; }
+inline size_t tableLdsBytes(int types) { return tableInLds(types) ? (size_t)types * types * sizeof(float2) : 0; }
+
+// One pair: the force scalar F/r (added to the lane's force with the
+// separation), and with ENERGIES the pair's LJ, screened Coulomb and r dU/dr
+// in FP64. A pair that is not `live` (an entry past the row's count) or
+// beyond the cutoff adds nothing. The force-only variant computes every pair
+// and selects zero for those (a branch around them would be taken by whole
+// waves almost never, and lets the compiler sink the pair's loads behind
+// it); the energy variant branches around its FP64 work.
+template <bool ENERGIES, bool LDS_TABLE>
+__device__ __forceinline__ void pairTerm(const float4 &pi, float qi, int ti, const float4 &pj, int tj, bool live,
+                                         const BoxF &box, float alpha, const float2 *ldsTable,
+                                         GlobalPtr<float2> table, const double2 *lj64, float &fx, float &fy, float &fz,
+                                         double &elj, double &ecoul, double &dscale) {
+  float dx = pi.x - pj.x, dy = pi.y - pj.y, dz = pi.z - pj.z;
+  float s0, s1, s2;
+  const float r2 = minimumImageR2(dx, dy, dz, box, s0, s1, s2);
+  const bool inside = live && r2 <= box.rc2;
+  if (ENERGIES && !inside) return;
+  const float2 ab = LDS_TABLE ? ldsTable[ti + tj] : table[ti + tj];
+  const float invR2 = __builtin_amdgcn_rcpf(r2);
+  const float invR = __builtin_amdgcn_rsqf(r2);
+  const float r = r2 * invR;
+  const float invR6 = invR2 * invR2 * invR2;
+  const float lj12 = ab.x * invR6;
+  float scalar = (12.0f * lj12 - 6.0f * ab.y) * invR6 * invR2;
+  const float qq = qi * pj.w;
+  float gaussian;
+  const float screened = erfcWithGaussian(alpha * r, gaussian);
+  scalar += qq * (screened * invR + TWO_OVER_SQRT_PI_F * alpha * gaussian) * invR2;
+  if (ENERGIES) {
+    // FP64 at the same image: the FP32 box rows, the hardware reciprocals'
+    // one-ulp bias and the FP32 table would otherwise be systematic over the
+    // box, and a water box's LJ total is a small difference of large sums
+    const double2 ab64 = lj64[ti + tj];
+    const double ir2 = 1.0 / imageR2Double(pi, pj, box, s0, s1, s2);
+    const double ir6 = ir2 * ir2 * ir2;
+    elj += (ab64.x * ir6 - ab64.y) * ir6;
+    ecoul += (double)(qq * screened) * sqrt(ir2);
+    dscale -= (double)(scalar * r2);  // r dU/dr
+  }
+  // a select, not a product: a pair that is not computed may hold r2 = 0
+  const float f = inside ? scalar : 0.0f;
+  fx += f * dx;
+  fy += f * dy;
+  fz += f * dz;
+}
+
+template <int LANES, bool ENERGIES, bool LDS_TABLE>
+__device__ __forceinline__ void nonbonded(const Dev &d, float alpha, int block, float2 *ldsTable) {
   // The LJ type-pair table in LDS: one lookup per pair, off the global
   // latency chain (list index -> position and type -> table entry).
-  __shared__ float2 ldsTable[LDS_TYPES * LDS_TYPES];
-  const int tableSize = d.types * d.types;
-  const bool tableInLds = d.types <= LDS_TYPES;
-  if (tableInLds)
-    for (int k = threadIdx.x; k < tableSize; k += blockDim.x) ldsTable[k] = d.lj[k];
-  __syncthreads();
-  const float2 *table = tableInLds ? ldsTable : d.lj;
-  const BoxF &box = *d.box;
+  const GlobalPtr<float2> table = globalPtr(d.lj);
+  if (LDS_TABLE) {
+    const int tableSize = d.types * d.types;
+    for (int k = threadIdx.x; k < tableSize; k += blockDim.x) ldsTable[k] = table[k];
+    __syncthreads();
+  }
+  const BoxF box = uniformPairBox(*d.box, ENERGIES);
+  const GlobalPtr<float4> slotPos = globalPtr(d.slotPos);
+  const GlobalPtr<int> slotType = globalPtr(d.slotType);
   const int tid = block * blockDim.x + threadIdx.x;
   const int row = tid / LANES;  // a slot: rows and list entries follow the cell order
   const int sub = tid % LANES;
@@ -40,58 +92,40 @@ __device__ __forceinline__ void nonbonded(const Dev &d, float alpha, int block) 
   float fx = 0.f, fy = 0.f, fz = 0.f;
   double elj = 0.0, ecoul = 0.0, dscale = 0.0;
   if (valid) {
-    const float4 pi = d.slotPos[row];
-    const int ti = d.slotType[row] * d.types;
+    const float4 pi = slotPos[row];
+    const int ti = slotType[row] * d.types;
     const float qi = ONE_4PI_EPS0_F * pi.w;
-    const float gaussScale = TWO_OVER_SQRT_PI_F * alpha;
-    const int count = d.nbrCount[row];
-    const int *list = d.nbrList + (size_t)row * d.cap;
-    // Software pipeline: the neighbour slot two iterations ahead and the
-    // position and type one iteration ahead are in flight while a pair is
-    // computed. Past the end the prefetch reads the row's own slot (in bounds).
-    int k = sub;
-    int jAhead = k + LANES < count ? list[k + LANES] : row;
-    int jNext = k < count ? list[k] : row;
-    float4 pNext = d.slotPos[jNext];
-    int tNext = d.slotType[jNext];
-    while (k < count) {
-      const float4 pj = pNext;
-      const int tj = tNext;
-      k += LANES;
-      jNext = jAhead;
-      jAhead = k + LANES < count ? list[k + LANES] : row;
-      pNext = d.slotPos[jNext];
-      tNext = d.slotType[jNext];
-      float dx = pi.x - pj.x, dy = pi.y - pj.y, dz = pi.z - pj.z;
-      float s0, s1, s2;
-      const float r2 = minimumImageR2(dx, dy, dz, box, s0, s1, s2);
-      if (r2 > box.rc2) continue;
-      const float2 ab = table[ti + tj];
-      const float invR2 = __builtin_amdgcn_rcpf(r2);
-      const float invR = __builtin_amdgcn_rsqf(r2);
-      const float r = r2 * invR;
-      const float invR6 = invR2 * invR2 * invR2;
-      const float lj12 = ab.x * invR6;
-      float scalar = (12.0f * lj12 - 6.0f * ab.y) * invR6 * invR2;
-      const float qq = qi * pj.w;
-      float gaussian;
-      const float screened = erfcWithGaussian(alpha * r, gaussian);
-      scalar += qq * (screened * invR + gaussScale * gaussian) * invR2;
-      if (ENERGIES) {
-        // FP64 at the same image: the FP32 box rows, the hardware
-        // reciprocals' one-ulp bias and the FP32 table would otherwise be
-        // systematic over the box, and a water box's LJ total is a small
-        // difference of large sums
-        const double2 ab64 = d.lj64[ti + tj];
-        const double ir2 = 1.0 / imageR2Double(pi, pj, box, s0, s1, s2);
-        const double ir6 = ir2 * ir2 * ir2;
-        elj += (ab64.x * ir6 - ab64.y) * ir6;
-        ecoul += (double)(qq * screened) * sqrt(ir2);
-        dscale -= (double)(scalar * r2);  // r dU/dr
+    const int count = globalPtr(d.nbrCount)[row];
+    const GlobalPtr<int> list = globalPtr(d.nbrList + (size_t)row * d.cap);
+    // UNROLL entries per iteration, k + u * LANES: their indices are loaded
+    // together, then their positions and types together, then the pairs are
+    // computed, so a lane waits two memory round trips per UNROLL pairs and
+    // the other waves of the SIMD fill the wait. Every load is issued
+    // unconditionally: an entry index past the row's count is clamped into
+    // the row's capacity and the slot it holds (unwritten memory past the
+    // count) into the slots, so both loads stay in bounds; such a pair is
+    // loaded but not computed.
+    constexpr int UNROLL = 4;
+    const int last = d.cap - 1;
+    const unsigned lastSlot = (unsigned)(d.n - 1);
+    for (int k = sub; k < count; k += UNROLL * LANES) {
+      int j[UNROLL];
+#pragma unroll
+      for (int u = 0; u < UNROLL; ++u) {
+        const int e = k + u * LANES;
+        j[u] = (int)min((unsigned)list[e < last ? e : last], lastSlot);
       }
-      fx += scalar * dx;
-      fy += scalar * dy;
-      fz += scalar * dz;
+      float4 p[UNROLL];
+      int t[UNROLL];
+#pragma unroll
+      for (int u = 0; u < UNROLL; ++u) {
+        p[u] = slotPos[j[u]];
+        t[u] = slotType[j[u]];
+      }
+#pragma unroll
+      for (int u = 0; u < UNROLL; ++u)
+        pairTerm<ENERGIES, LDS_TABLE>(pi, qi, ti, p[u], t[u], k + u * LANES < count, box, alpha, ldsTable, table,
+                                      d.lj64, fx, fy, fz, elj, ecoul, dscale);
     }
   }
   // Exact group reduction in fixed point.
@@ -136,58 +170,80 @@ __device__ __forceinline__ void nonbonded(const Dev &d, float alpha, int block) 
   }
 }
 
-template <int LANES, bool ENERGIES>
+template <int LANES, bool ENERGIES, bool LDS_TABLE>
 __global__ void __launch_bounds__(256) nonbondedKernel(Dev d, float alpha) {
-  nonbonded<LANES, ENERGIES>(d, alpha, blockIdx.x);
+  extern __shared__ float2 ldsTable[];
+  nonbonded<LANES, ENERGIES, LDS_TABLE>(d, alpha, blockIdx.x, ldsTable);
 }
 
 // Batched.
-template <int LANES, bool ENERGIES>
+template <int LANES, bool ENERGIES, bool LDS_TABLE>
 __global__ void __launch_bounds__(256) nonbondedBatchKernel(const BatchView *__restrict__ v,
                                                             const int2 *__restrict__ task, float alpha) {
+  extern __shared__ float2 ldsTable[];
   const int2 t = task[blockIdx.x];
   if (t.x < 0) return;
-  nonbonded<LANES, ENERGIES>(v[t.x].d, alpha, t.y);
+  nonbonded<LANES, ENERGIES, LDS_TABLE>(v[t.x].d, alpha, t.y, ldsTable);
 }
 
-template <bool ENERGIES>
-static void launchLanes(const Dev &d, float alpha, int lanes, int blocks, hipStream_t s) {
+template <bool ENERGIES, bool LDS_TABLE>
+static void launchLanes(const Dev &d, float alpha, int lanes, int blocks, size_t lds, hipStream_t s) {
   switch (lanes) {
-    case 1: nonbondedKernel<1, ENERGIES><<<blocks, 256, 0, s>>>(d, alpha); break;
-    case 2: nonbondedKernel<2, ENERGIES><<<blocks, 256, 0, s>>>(d, alpha); break;
-    case 4: nonbondedKernel<4, ENERGIES><<<blocks, 256, 0, s>>>(d, alpha); break;
-    case 8: nonbondedKernel<8, ENERGIES><<<blocks, 256, 0, s>>>(d, alpha); break;
-    case 16: nonbondedKernel<16, ENERGIES><<<blocks, 256, 0, s>>>(d, alpha); break;
-    case 32: nonbondedKernel<32, ENERGIES><<<blocks, 256, 0, s>>>(d, alpha); break;
-    default: nonbondedKernel<64, ENERGIES><<<blocks, 256, 0, s>>>(d, alpha); break;
+    case 1: nonbondedKernel<1, ENERGIES, LDS_TABLE><<<blocks, 256, lds, s>>>(d, alpha); break;
+    case 2: nonbondedKernel<2, ENERGIES, LDS_TABLE><<<blocks, 256, lds, s>>>(d, alpha); break;
+    case 4: nonbondedKernel<4, ENERGIES, LDS_TABLE><<<blocks, 256, lds, s>>>(d, alpha); break;
+    case 8: nonbondedKernel<8, ENERGIES, LDS_TABLE><<<blocks, 256, lds, s>>>(d, alpha); break;
+    case 16: nonbondedKernel<16, ENERGIES, LDS_TABLE><<<blocks, 256, lds, s>>>(d, alpha); break;
+    case 32: nonbondedKernel<32, ENERGIES, LDS_TABLE><<<blocks, 256, lds, s>>>(d, alpha); break;
+    default: nonbondedKernel<64, ENERGIES, LDS_TABLE><<<blocks, 256, lds, s>>>(d, alpha); break;
   }
 }
 
 int launchNonbonded(const Dev &d, float alpha, int lanes, int energies, hipStream_t s) {
   const long long threads = (long long)d.n * lanes;
   const int blocks = (int)((threads + 255) / 256);
-  if (energies)
-    launchLanes<true>(d, alpha, lanes, blocks, s);
-  else
-    launchLanes<false>(d, alpha, lanes, blocks, s);
+  const size_t lds = tableLdsBytes(d.types);
+  if (tableInLds(d.types)) {
+    if (energies)
+      launchLanes<true, true>(d, alpha, lanes, blocks, lds, s);
+    else
+      launchLanes<false, true>(d, alpha, lanes, blocks, lds, s);
+  } else {
+    if (energies)
+      launchLanes<true, false>(d, alpha, lanes, blocks, 0, s);
+    else
+      launchLanes<false, false>(d, alpha, lanes, blocks, 0, s);
+  }
   return blocks;
 }
 
-template <bool ENERGIES>
-static void launchBatchLanes(const BatchView *v, const TaskTable &t, int lanes, float alpha, hipStream_t s) {
+template <bool ENERGIES, bool LDS_TABLE>
+static void launchBatchLanes(const BatchView *v, const TaskTable &t, int lanes, float alpha, size_t lds,
+                             hipStream_t s) {
   switch (lanes) {
-    case 8: nonbondedBatchKernel<8, ENERGIES><<<t.blocks, 256, 0, s>>>(v, t.task, alpha); break;
-    case 16: nonbondedBatchKernel<16, ENERGIES><<<t.blocks, 256, 0, s>>>(v, t.task, alpha); break;
-    case 32: nonbondedBatchKernel<32, ENERGIES><<<t.blocks, 256, 0, s>>>(v, t.task, alpha); break;
-    default: nonbondedBatchKernel<64, ENERGIES><<<t.blocks, 256, 0, s>>>(v, t.task, alpha); break;
+    case 8: nonbondedBatchKernel<8, ENERGIES, LDS_TABLE><<<t.blocks, 256, lds, s>>>(v, t.task, alpha); break;
+    case 16: nonbondedBatchKernel<16, ENERGIES, LDS_TABLE><<<t.blocks, 256, lds, s>>>(v, t.task, alpha); break;
+    case 32: nonbondedBatchKernel<32, ENERGIES, LDS_TABLE><<<t.blocks, 256, lds, s>>>(v, t.task, alpha); break;
+    default: nonbondedBatchKernel<64, ENERGIES, LDS_TABLE><<<t.blocks, 256, lds, s>>>(v, t.task, alpha); break;
   }
 }
 
-void launchNonbondedBatch(const BatchView *v, const TaskTable &t, int lanes, float alpha, int energies, hipStream_t s) {
-  if (energies)
-    launchBatchLanes<true>(v, t, lanes, alpha, s);
-  else
-    launchBatchLanes<false>(v, t, lanes, alpha, s);
+// `types` is the largest type count of the batch's replicas: every workgroup
+// gets LDS for that table.
+void launchNonbondedBatch(const BatchView *v, const TaskTable &t, int lanes, float alpha, int energies, int types,
+                          hipStream_t s) {
+  const size_t lds = tableLdsBytes(types);
+  if (tableInLds(types)) {
+    if (energies)
+      launchBatchLanes<true, true>(v, t, lanes, alpha, lds, s);
+    else
+      launchBatchLanes<false, true>(v, t, lanes, alpha, lds, s);
+  } else {
+    if (energies)
+      launchBatchLanes<true, false>(v, t, lanes, alpha, 0, s);
+    else
+      launchBatchLanes<false, false>(v, t, lanes, alpha, 0, s);
+  }
 }
 
 }  // namespace rmd
