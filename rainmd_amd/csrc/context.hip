@@ -213,6 +213,7 @@ BoxF boxFor(const rmd_ctx *c) {
   for (int k = 0; k < 3; ++k) b.cells[k] = c->plan.counts[k];
   b.cellTotal = c->plan.cellCount;
   b.reach = c->plan.reach;
+  b.cellSearch = cellSearchFits(c->d.n, c->plan.cellCount) ? 1 : 0;
   return b;
 }
 

@@ -37,6 +37,7 @@ struct BoxF {
   int cells[3];
   int cellTotal;
   int reach;  // stencil cells either side (cells are at least rl / reach wide)
+  int cellSearch;  // the list search runs one wave per cell (else one per atom)
 };
 
 __device__ __forceinline__ long long toFixed(double v) { return static_cast<long long>(__double2ll_rn(v * FORCE_SCALE)); }
@@ -95,13 +96,21 @@ __device__ __forceinline__ double imageR2Double(float4 pi, float4 pj, const BoxF
 #if defined(__HIP_DEVICE_COMPILE__)
 template <typename T>
 using GlobalPtr = const __attribute__((address_space(1))) T *;
+template <typename T>
+using GlobalOut = __attribute__((address_space(1))) T *;
 #else
 template <typename T>
 using GlobalPtr = const T *;
+template <typename T>
+using GlobalOut = T *;
 #endif
 template <typename T>
 __device__ __forceinline__ GlobalPtr<T> globalPtr(const T *p) {
   return (GlobalPtr<T>)p;
+}
+template <typename T>
+__device__ __forceinline__ GlobalOut<T> globalOut(T *p) {
+  return (GlobalOut<T>)p;
 }
 
 // A value the same in every lane of the wave (read from lane 0 or the first
@@ -116,9 +125,9 @@ __device__ __forceinline__ double uniformDouble(double x) {
   return __longlong_as_double(((long long)hi << 32) | (unsigned)lo);
 }
 
-// The fields of a box the pair test reads (rows, inverse lengths, rc^2, and
-// with `rows64` the FP64 rows), in scalar registers: a box is uniform over a
-// workgroup. Call with every lane active.
+// The fields of a box the pair tests read (rows, inverse lengths, rc^2,
+// rl^2, and with `rows64` the FP64 rows), in scalar registers: a box is
+// uniform over a workgroup. Call with every lane active.
 __device__ __forceinline__ BoxF uniformPairBox(const BoxF &g, bool rows64) {
   BoxF b;
   for (int k = 0; k < 3; ++k) {
@@ -128,6 +137,7 @@ __device__ __forceinline__ BoxF uniformPairBox(const BoxF &g, bool rows64) {
     b.inv[k] = uniformFloat(g.inv[k]);
   }
   b.rc2 = uniformFloat(g.rc2);
+  b.rl2 = uniformFloat(g.rl2);
   if (rows64)
     for (int k = 0; k < 9; ++k) b.box64[k] = uniformDouble(g.box64[k]);
   return b;

@@ -161,6 +161,10 @@ void launchForceRebuildFlaggedBatch(const BatchView *v, int replicas, hipStream_
 void launchClearRequiredBatch(Scalars *sc, int replicas, hipStream_t s);
 // Whether a system's cell list fits the single-workgroup build (a batch needs it).
 bool fitsSmallCellBuild(int atoms, int cells);
+// Whether the list search runs one wave per cell: small cells (a few dozen
+// atoms at most on average), and a cell count the per-atom launch's blocks
+// cover at four cells each.
+bool cellSearchFits(int atoms, int cells);
 void launchRefreshSlotsBatch(const BatchView *v, const TaskTable &atoms256, hipStream_t s);
 void launchRebuildBatch(const BatchView *v, const TaskTable &replicas, const TaskTable &atoms64, hipStream_t s);
 // lanes: lanes per atom (8, 16, 32 or 64), the table's grid at that count.

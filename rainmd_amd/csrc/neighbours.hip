@@ -259,35 +259,39 @@ __device__ __forceinline__ int wrapCell(int v, int count) {
 // Whether candidate pj joins atom's list: not itself, within rl (the FP32
 // minimum image of the kernels), not excluded. Exclusions are checked only
 // for candidates inside the range of the atom's (sorted) exclusion partners.
-__device__ __forceinline__ bool acceptPair(const Dev &d, const BoxF &box, float4 pi, float4 pj, int atom, int exBegin,
-                                           int exEnd, int exMin, int exMax) {
+__device__ __forceinline__ bool acceptPair(GlobalPtr<int> excl, const BoxF &box, float4 pi, float4 pj, int atom,
+                                           int exBegin, int exEnd, int exMin, int exMax) {
+  // the distance test first and without a branch, so the candidate's loads
+  // are used whole where they are issued
   const int j = __float_as_int(pj.w);
-  if (j == atom) return false;
   float dx = pi.x - pj.x, dy = pi.y - pj.y, dz = pi.z - pj.z;
   const float r2 = minimumImageR2(dx, dy, dz, box);
-  if (!(r2 <= box.rl2)) return false;
-  if (j >= exMin && j <= exMax)
+  bool accept = (j != atom) & (r2 <= box.rl2);
+  if (accept & (j >= exMin) & (j <= exMax))
     for (int e = exBegin; e < exEnd; ++e)
-      if (d.exclPartners[e] == j) return false;
-  return true;
+      if (excl[e] == j) accept = false;
+  return accept;
 }
 
 // One wave per atom. The stencil is a set of z runs, one per (x, y) stencil
 // row, each a contiguous range of the cell-ordered positions (two where the
 // run wraps). The wave writes the ranges and their prefix sums to LDS and
-// streams the concatenated candidates 128 at a time (two per lane, both loads
-// in flight together), each lane keeping a cursor into the ranges; a ballot
-// compacts the accepted ones into the atom's slice in candidate order, so the
-// list is the same on every run. Every pair found is counted; only `cap` are
-// stored, and the true requirement reaches the host through
+// streams the concatenated candidates 256 at a time (four per lane, their
+// loads issued together), each lane keeping a cursor into the ranges; a
+// ballot compacts the accepted ones into the atom's slice in candidate
+// order, so the list is the same on every run. Every pair found is counted;
+// only `cap` are stored, and the true requirement reaches the host through
 // Scalars::required (the reference's overflow contract,
 // webgpu/kernels/neighbours.js:128-146).
 __device__ __forceinline__ void neighbours(const Dev &d, int block) {
   if (!d.sc->rebuild) return;
   __shared__ int rangeBegin[4][STENCIL_RANGES];
   __shared__ int rangeStart[4][STENCIL_RANGES + 1];
-  const BoxF box = *d.box;
-  const int *cells = box.cells;
+  const BoxF box = uniformPairBox(*d.box, false);
+  const int cells[3] = {d.box->cells[0], d.box->cells[1], d.box->cells[2]};
+  const GlobalPtr<float4> cellPos = globalPtr(d.cellPos);
+  const GlobalPtr<int> cellStart = globalPtr(d.cellStart);
+  const GlobalPtr<int> excl = globalPtr(d.exclPartners);
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const int row = (block * blockDim.x + threadIdx.x) >> 6;  // a slot
   if (row >= d.n) return;
@@ -297,9 +301,9 @@ __device__ __forceinline__ void neighbours(const Dev &d, int block) {
   const int own = d.cellOfAtom[atom];
   const int c3[3] = {own / (cells[1] * cells[2]), (own / cells[2]) % cells[1], own % cells[2]};
   const int exBegin = d.exclStart[atom], exEnd = d.exclStart[atom + 1];
-  const int exMin = exBegin < exEnd ? d.exclPartners[exBegin] : 1;
-  const int exMax = exBegin < exEnd ? d.exclPartners[exEnd - 1] : 0;
-  const int reach = box.reach;
+  const int exMin = exBegin < exEnd ? excl[exBegin] : 1;
+  const int exMax = exBegin < exEnd ? excl[exEnd - 1] : 0;
+  const int reach = d.box->reach;
   int bx, nx, by, ny, bz, nz;
   axisStencil(cells[0], reach, bx, nx);
   axisStencil(cells[1], reach, by, ny);
@@ -336,8 +340,8 @@ __device__ __forceinline__ void neighbours(const Dev &d, int block) {
       const int cx = wrapCell(c3[0] + bx + row / ny, cells[0]);
       const int cy = wrapCell(c3[1] + by + row % ny, cells[1]);
       const int rowBase = (cx * cells[1] + cy) * cells[2];
-      const int begin = d.cellStart[rowBase + (seg ? za1 : za0)];
-      const int end = d.cellStart[rowBase + (seg ? zb1 : zb0)];
+      const int begin = cellStart[rowBase + (seg ? za1 : za0)];
+      const int end = cellStart[rowBase + (seg ? zb1 : zb0)];
       rb[r] = begin;
       len[h] = end - begin;
     }
@@ -357,44 +361,208 @@ __device__ __forceinline__ void neighbours(const Dev &d, int block) {
   __builtin_amdgcn_wave_barrier();
   const int total = carry;
   int found = 0;
-  int *out = d.nbrList + (size_t)row * d.cap;
-  const float4 none = make_float4(0.f, 0.f, 0.f, __int_as_float(atom));
+  const GlobalOut<int> out = globalOut(d.nbrList + (size_t)row * d.cap);
+  const int cap = d.cap;
   auto emit = [&](bool accept, int j) {  // j: the candidate's slot
     const unsigned long long mask = __ballot(accept);
     if (accept) {
       const int where = found + __popcll(mask & ((1ull << lane) - 1ull));
-      if (where < d.cap) out[where] = j;
+      if (where < cap) out[where] = j;
     }
     found += __popcll(mask);
   };
-  int cur = 0;  // the range of this lane's first candidate of the batch
-  for (int base = 0; base < total; base += 128) {
-    const int g0 = base + lane, g1 = g0 + 64;
-    float4 p0 = none, p1 = none;
-    int s0 = row, s1 = row;
-    if (g0 < total) {
-      while (rs[cur + 1] <= g0) ++cur;
-      s0 = rb[cur] + (g0 - rs[cur]);
-      p0 = d.cellPos[s0];
+  // Four candidates per lane per pass, base + lane + 64 u. A candidate past
+  // the total loads the row's own slot, which the pair test turns down (the
+  // atom itself), so every load is in bounds and issued with the others.
+  constexpr int CAND = 4;
+  int cur = 0;  // the range of this lane's latest candidate
+  for (int base = 0; base < total; base += 64 * CAND) {
+    int slot[CAND];
+#pragma unroll
+    for (int u = 0; u < CAND; ++u) {
+      const int g = base + lane + 64 * u;
+      slot[u] = row;
+      if (g < total) {
+        while (rs[cur + 1] <= g) ++cur;
+        slot[u] = rb[cur] + (g - rs[cur]);
+      }
     }
-    if (g1 < total) {
-      int c1 = cur;
-      while (rs[c1 + 1] <= g1) ++c1;
-      s1 = rb[c1] + (g1 - rs[c1]);
-      p1 = d.cellPos[s1];
-    }
-    emit(acceptPair(d, box, pi, p0, atom, exBegin, exEnd, exMin, exMax), s0);
-    if (base + 64 < total) emit(acceptPair(d, box, pi, p1, atom, exBegin, exEnd, exMin, exMax), s1);
+    float4 p[CAND];
+#pragma unroll
+    for (int u = 0; u < CAND; ++u) p[u] = cellPos[slot[u]];
+    // the tests of all four, then the stores (which share the loads'
+    // completion counter)
+    bool accept[CAND];
+#pragma unroll
+    for (int u = 0; u < CAND; ++u) accept[u] = acceptPair(excl, box, pi, p[u], atom, exBegin, exEnd, exMin, exMax);
+#pragma unroll
+    for (int u = 0; u < CAND; ++u)
+      if (base + 64 * u < total) emit(accept[u], slot[u]);
   }
   __builtin_amdgcn_wave_barrier();
   if (lane == 0) {
-    d.nbrCount[row] = found < d.cap ? found : d.cap;
+    d.nbrCount[row] = found < cap ? found : cap;
     atomicMax(&d.sc->required, (unsigned int)found);
     atomicMax(&d.sc->requiredMax, (unsigned int)found);
   }
 }
 
-__global__ void __launch_bounds__(256) neighboursKernel(Dev d) { neighbours(d, blockIdx.x); }
+// One wave per cell. Every atom of a cell has the same stencil, so the wave
+// streams the stencil's candidates as above (the same ranges, in the same
+// order) and tests each candidate against every atom of the cell, up to 64
+// at a time, one per lane, broadcast from its lane in turn. Each atom's
+// accepted candidates are compacted into its own slice in candidate order:
+// the lists are entry for entry those of one wave per atom, for a fraction
+// of the loads and of the per-atom set-up.
+__device__ __forceinline__ void neighboursByCell(const Dev &d, int block) {
+  __shared__ int rangeBegin[4][STENCIL_RANGES];
+  __shared__ int rangeStart[4][STENCIL_RANGES + 1];
+  const BoxF box = uniformPairBox(*d.box, false);
+  const int cells[3] = {d.box->cells[0], d.box->cells[1], d.box->cells[2]};
+  const GlobalPtr<float4> cellPos = globalPtr(d.cellPos);
+  const GlobalPtr<int> cellStart = globalPtr(d.cellStart);
+  const GlobalPtr<int> excl = globalPtr(d.exclPartners);
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int cell = block * 4 + wave;
+  if (cell >= d.box->cellTotal) return;
+  const int cellBegin = cellStart[cell], cellEnd = cellStart[cell + 1];
+  if (cellBegin == cellEnd) return;
+  const int c3[3] = {cell / (cells[1] * cells[2]), (cell / cells[2]) % cells[1], cell % cells[2]};
+  const int reach = d.box->reach;
+  int bx, nx, by, ny, bz, nz;
+  axisStencil(cells[0], reach, bx, nx);
+  axisStencil(cells[1], reach, by, ny);
+  axisStencil(cells[2], reach, bz, nz);
+  int za0, zb0, za1 = 0, zb1 = 0, segs = 1;
+  if (nz >= cells[2]) {
+    za0 = 0;
+    zb0 = cells[2];
+  } else {
+    const int lo = c3[2] + bz, hi = lo + nz;
+    if (lo < 0) {
+      segs = 2;
+      za0 = 0, zb0 = hi;
+      za1 = lo + cells[2], zb1 = cells[2];
+    } else if (hi > cells[2]) {
+      segs = 2;
+      za0 = 0, zb0 = hi - cells[2];
+      za1 = lo, zb1 = cells[2];
+    } else {
+      za0 = lo, zb0 = hi;
+    }
+  }
+  const int ranges = nx * ny * segs;
+  int *rb = rangeBegin[wave], *rs = rangeStart[wave];
+  int len[2] = {0, 0};
+  for (int h = 0; h < 2; ++h) {
+    const int r = lane + 64 * h;
+    if (r < ranges) {
+      const int row = r / segs, seg = r - row * segs;
+      const int cx = wrapCell(c3[0] + bx + row / ny, cells[0]);
+      const int cy = wrapCell(c3[1] + by + row % ny, cells[1]);
+      const int rowBase = (cx * cells[1] + cy) * cells[2];
+      const int begin = cellStart[rowBase + (seg ? za1 : za0)];
+      const int end = cellStart[rowBase + (seg ? zb1 : zb0)];
+      rb[r] = begin;
+      len[h] = end - begin;
+    }
+  }
+  int carry = 0;
+  for (int h = 0; h < 2; ++h) {
+    int inclusive = len[h];
+    for (int offset = 1; offset < 64; offset <<= 1) {
+      const int v = __shfl_up(inclusive, offset, 64);
+      if (lane >= offset) inclusive += v;
+    }
+    const int r = lane + 64 * h;
+    if (r < ranges) rs[r] = carry + inclusive - len[h];
+    carry += __shfl(inclusive, 63, 64);
+  }
+  if (lane == 0) rs[ranges] = carry;
+  __builtin_amdgcn_wave_barrier();
+  const int total = carry;
+  const int cap = d.cap;
+  constexpr int CAND = 4;
+  unsigned int required = 0;
+  for (int first = cellBegin; first < cellEnd; first += 64) {
+    // lane l holds the cell's atom first + l (the last one again past the end)
+    const int atoms = min(64, cellEnd - first);
+    const int own = first + min(lane, atoms - 1);
+    const float4 mine = cellPos[own];
+    const int myAtom = __float_as_int(mine.w);
+    const int myExBegin = d.exclStart[myAtom], myExEnd = d.exclStart[myAtom + 1];
+    const int myExMin = myExBegin < myExEnd ? excl[myExBegin] : 1;
+    const int myExMax = myExBegin < myExEnd ? excl[myExEnd - 1] : 0;
+    if (lane < atoms) d.posRebuild[myAtom] = d.pos[myAtom];
+    int myFound = 0;
+    int cur = 0;
+    for (int base = 0; base < total; base += 64 * CAND) {
+      int slot[CAND];
+#pragma unroll
+      for (int u = 0; u < CAND; ++u) {
+        const int g = base + lane + 64 * u;
+        slot[u] = own;  // past the total: an in-bounds load, never accepted
+        if (g < total) {
+          while (rs[cur + 1] <= g) ++cur;
+          slot[u] = rb[cur] + (g - rs[cur]);
+        }
+      }
+      float4 p[CAND];
+#pragma unroll
+      for (int u = 0; u < CAND; ++u) p[u] = cellPos[slot[u]];
+      for (int i = 0; i < atoms; ++i) {
+        const float4 pi = make_float4(__int_as_float(__builtin_amdgcn_readlane(__float_as_int(mine.x), i)),
+                                      __int_as_float(__builtin_amdgcn_readlane(__float_as_int(mine.y), i)),
+                                      __int_as_float(__builtin_amdgcn_readlane(__float_as_int(mine.z), i)), 0.f);
+        const int atom = __builtin_amdgcn_readlane(myAtom, i);
+        const int exBegin = __builtin_amdgcn_readlane(myExBegin, i);
+        const int exEnd = __builtin_amdgcn_readlane(myExEnd, i);
+        const int exMin = __builtin_amdgcn_readlane(myExMin, i);
+        const int exMax = __builtin_amdgcn_readlane(myExMax, i);
+        int found = __builtin_amdgcn_readlane(myFound, i);
+        // a lane past the total holds a real atom of the cell: turned down by
+        // its position in the stream, not by the pair test
+        bool accept[CAND];
+#pragma unroll
+        for (int u = 0; u < CAND; ++u)
+          accept[u] = (base + lane + 64 * u < total) &
+                      acceptPair(excl, box, pi, p[u], atom, exBegin, exEnd, exMin, exMax);
+        const GlobalOut<int> out = globalOut(d.nbrList + (size_t)(first + i) * cap);
+#pragma unroll
+        for (int u = 0; u < CAND; ++u) {
+          const unsigned long long mask = __ballot(accept[u]);
+          if (accept[u]) {
+            const int where = found + __popcll(mask & ((1ull << lane) - 1ull));
+            if (where < cap) out[where] = slot[u];
+          }
+          found += __popcll(mask);
+        }
+        if (lane == i) myFound = found;
+      }
+    }
+    if (lane < atoms) {
+      d.nbrCount[own] = myFound < cap ? myFound : cap;
+      required = max(required, (unsigned int)myFound);
+    }
+  }
+  // one update of the requirement per wave
+  for (int offset = 32; offset > 0; offset >>= 1) required = max(required, (unsigned int)__shfl_xor((int)required, offset, 64));
+  if (lane == 0) {
+    atomicMax(&d.sc->required, required);
+    atomicMax(&d.sc->requiredMax, required);
+  }
+}
+
+// A block is four atoms or four cells, as the box's plan says.
+__device__ __forceinline__ void searchBlock(const Dev &d, int block) {
+  if (!d.sc->rebuild) return;
+  if (d.box->cellSearch)
+    neighboursByCell(d, block);
+  else
+    neighbours(d, block);
+}
+
+__global__ void __launch_bounds__(256) neighboursKernel(Dev d) { searchBlock(d, blockIdx.x); }
 
 // Batched, grid-stride over the task table: on most steps only the few
 // replicas due a rebuild have work, and a workgroup skips the others' tasks
@@ -406,7 +574,7 @@ __global__ void __launch_bounds__(256) neighboursBatchKernel(const BatchView *__
   for (int k = blockIdx.x; k < tasks; k += gridDim.x) {
     const int2 t = task[k];
     if (t.x < 0 || !v[t.x].d.sc->rebuild) continue;
-    neighbours(v[t.x].d, t.y);
+    searchBlock(v[t.x].d, t.y);
   }
 }
 
@@ -458,5 +626,10 @@ void launchRebuildBatch(const BatchView *v, const TaskTable &replicas, const Tas
 }
 
 bool fitsSmallCellBuild(int atoms, int cells) { return atoms <= SMALL_MAX_ATOMS && cells <= SMALL_MAX_CELLS; }
+
+bool cellSearchFits(int atoms, int cells) {
+  constexpr int MAX_AVERAGE = 32;  // atoms per cell
+  return cells > 0 && cells <= 4 * ((atoms + 3) / 4) && atoms <= MAX_AVERAGE * cells;
+}
 
 }  // namespace rmd
