@@ -375,7 +375,7 @@ void buildUnits(rmd_ctx *c, std::vector<int4> &units, std::vector<double4> &geom
     units.push_back(waters[k]);
     geom.push_back(waterGeom[k]);
   }
-  pad(64);
+  pad(INTEGRATE_BLOCK);  // whole blocks of waters: their own kernel
   for (const int4 &u : frees) {
     units.push_back(u);
     geom.push_back(none);
@@ -1007,6 +1007,9 @@ void create(rmd_ctx *c, const System &sys, const rmd_options &o, const Streams *
     buildUnits(c, units, geom, conStart, atomStart, atoms, cons, conD);
     d.nUnits = (int)units.size();
     c->nUnits = d.nUnits;
+    int waterUnits = 0;
+    for (const int4 &u : units) waterUnits += u.x == UNIT_WATER ? 1 : 0;
+    d.waterBlocks = (waterUnits + INTEGRATE_BLOCK - 1) / INTEGRATE_BLOCK;
     d.units = devAlloc<int4>(units.size(), c->held);
     d.waterGeom = devAlloc<double4>(geom.size(), c->held);
     d.clusterConStart = devAlloc<int>(conStart.size(), c->held);
@@ -1290,7 +1293,7 @@ int rmd_energy(rmd_ctx *c, rmd_energy_terms *out) {
     std::memcpy(slots, c->slotCounts, sizeof slots);
     slots[E_KINETIC] = launchKinetic(c->d, E_KINETIC, c->stream);
     const bool earlyValid = c->kineticSampledAtStep == c->step;
-    slots[E_KINETIC_EARLY] = earlyValid ? (c->nUnits + INTEGRATE_BLOCK - 1) / INTEGRATE_BLOCK : 0;
+    slots[E_KINETIC_EARLY] = earlyValid ? openSlotCount(c->d) : 0;
     SlotCounts counts;
     std::memcpy(counts.c, slots, sizeof slots);
     launchReduceEnergies(c->d, counts, c->dTerms, c->stream);
@@ -1907,7 +1910,9 @@ constexpr int XCDS = 8;
 int replicaBlocks(const rmd_ctx *c, int kind) {
   const Dev &d = c->d;
   switch (kind) {
-    case BK_OPEN: return (d.nUnits + INTEGRATE_BLOCK - 1) / INTEGRATE_BLOCK;
+    case BK_OPEN: return generalBlockCount(d);
+    case BK_WATER: return d.waterBlocks;
+    case BK_UNITS: return unitBlockCount(d);
     case BK_ATOMS64: return (d.n + 3) / 4;  // 64 lanes per atom, 256 threads per workgroup
     case BK_BONDED: return bondedBlocks(d);
     case BK_PLANES: return c->p.K[0];
@@ -1962,6 +1967,7 @@ void uploadViews(rmd_batch *b) {
     v.p = c->p;
     v.ip = c->ip;
     v.openBlocks = replicaBlocks(c, BK_OPEN);
+    v.openSlots = openSlotCount(c->d);
     std::memset(&v.counts, 0, sizeof v.counts);
     const int nb = replicaBlocks(c, BK_NONBONDED), bb = replicaBlocks(c, BK_BONDED), xb = replicaBlocks(c, BK_XLINES);
     v.counts.c[E_LJ_DIRECT] = v.counts.c[E_COUL_DIRECT] = v.counts.c[E_DSCALE_DIRECT] = nb;
@@ -2072,11 +2078,12 @@ void enqueueBatchBlock(rmd_batch *b, int steps) {
   hipStream_t s = b->streams.stream;
   for (int k = 0; k < steps; ++k) {
     const bool last = k == steps - 1;
-    launchIntegrateOpenBatch(b->dViews, b->tables[BK_OPEN], k > 0 ? 1 : 0, last ? 1 : 0, b->ldsOpen, s);
+    launchIntegrateOpenBatch(b->dViews, b->tables[BK_WATER], b->tables[BK_OPEN], k > 0 ? 1 : 0, last ? 1 : 0,
+                             b->ldsOpen, s);
     enqueueBatchForces(b, last ? 1 : 0);
   }
   launchReduceEnergiesBatch(b->dViews, b->replicas, 1, s);
-  launchIntegrateCloseBatch(b->dViews, b->tables[BK_OPEN], b->ldsOpen, s);
+  launchIntegrateCloseBatch(b->dViews, b->tables[BK_UNITS], b->ldsOpen, s);
 }
 
 void launchBatchBlock(rmd_batch *b, int steps) {
@@ -2548,7 +2555,7 @@ int rmd_batch_info_get(rmd_batch *b, rmd_batch_info *out) {
     out->replicas = b->replicas;
     for (rmd_ctx *c : b->reps) out->atoms += c->d.n;
     out->blocks_run = b->blocksRun;
-    out->step_workgroups = b->tables[BK_OPEN].blocks + b->tables[BK_ATOMS64].blocks + b->tables[BK_NONBONDED].blocks +
+    out->step_workgroups = b->tables[BK_WATER].blocks + b->tables[BK_OPEN].blocks + b->tables[BK_ATOMS64].blocks + b->tables[BK_NONBONDED].blocks +
                            b->tables[BK_BONDED].blocks + 2 * b->tables[BK_PLANES].blocks +
                            b->tables[BK_XLINES].blocks + b->tables[BK_REPLICA].blocks;
     out->nonbonded_workgroups = b->tables[BK_NONBONDED].blocks;

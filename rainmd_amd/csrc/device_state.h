@@ -107,6 +107,7 @@ struct Dev {
   double4 *restraintRef;    // reference xyz and k, FP64
   // integrator units
   int nUnits;
+  int waterBlocks;          // leading unit blocks holding water units only
   int4 *units;              // kind, a, b, c (see integrate.hip)
   double4 *waterGeom;       // dOH, dHH, mO, mH per unit (used by water units)
   int *clusterConStart, *clusterAtomStart, *clusterAtoms;

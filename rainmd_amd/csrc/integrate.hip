@@ -515,7 +515,64 @@ __device__ void blockMaxToWord(float v, unsigned int *word) {
   __syncthreads();
 }
 
-// One step's opening half for the units of workgroup `block` of `blocks`.
+// One step's opening half for a water unit.
+__device__ __forceinline__ void openWater(const Dev &d, const IntegratorParams &ip, int u, int4 desc, int closing,
+                                          unsigned int step, double h, double &ke, float &disp) {
+  Atom w[3];
+  const int atoms[3] = {desc.y, desc.z, desc.w};
+  for (int a = 0; a < 3; ++a) loadAtom(d, atoms[a], w[a]);
+  const double4 geom = d.waterGeom[u];
+  if (closing) {
+    for (int a = 0; a < 3; ++a) kick(w[a], h);
+    rattleWater(w);
+  }
+  for (int a = 0; a < 3; ++a) kick(w[a], h);
+  rattleWater(w);
+  driftWater(w, h, geom);
+  for (int a = 0; a < 3; ++a) thermostat(d, ip, atoms[a], w[a], step);
+  rattleWater(w);
+  driftWater(w, h, geom);
+  for (int a = 0; a < 3; ++a) {
+    ke += kinetic(w[a]);
+    disp = fmaxf(disp, storeAtom(d, atoms[a], w[a], true));
+  }
+}
+
+// The water blocks (the unit table's leading blocks hold water units and
+// padding only) run the water path alone, in a kernel of their own, so that
+// its register budget is the water path's and not that of the star and
+// cluster paths. Their displacement maximum and kinetic slots join those of
+// the general blocks' kernel, which runs after them on the same stream and
+// decides the rebuild.
+__device__ __forceinline__ void integrateWater(const Dev &d, const IntegratorParams &ip, int closing, int early,
+                                               int block) {
+  const int u = block * blockDim.x + threadIdx.x;
+  const double h = ip.halfStep;
+  const unsigned int step = (unsigned int)d.sc->step;
+  double ke = 0.0;
+  float disp = 0.f;
+  if (u < d.nUnits) {
+    const int4 desc = d.units[u];
+    if (desc.x == UNIT_WATER) openWater(d, ip, u, desc, closing, step, h, ke, disp);
+  }
+  blockMaxToWord(disp, &d.sc->maxDisp2Bits);
+  if (early) blockSumToSlot(ke, &d.slots[E_KINETIC_EARLY * MAX_SLOTS + block]);
+}
+
+__global__ void __launch_bounds__(INTEGRATE_BLOCK)
+    integrateWaterKernel(Dev d, IntegratorParams ip, int closing, int early) {
+  integrateWater(d, ip, closing, early, blockIdx.x);
+}
+
+__global__ void __launch_bounds__(INTEGRATE_BLOCK)
+    integrateWaterBatchKernel(const BatchView *__restrict__ v, const int2 *__restrict__ task, int closing, int early) {
+  const int2 t = task[blockIdx.x];
+  if (t.x < 0) return;
+  integrateWater(v[t.x].d, v[t.x].ip, closing, early, t.y);
+}
+
+// One step's opening half for the units of workgroup `block`, one of the
+// `blocks` general workgroups (whose last one decides the rebuild).
 __device__ __forceinline__ void integrateOpen(const Dev &d, const IntegratorParams &ip, int closing, int early,
                                               int block, int blocks, double *clusterLds) {
   const int u = block * blockDim.x + threadIdx.x;
@@ -536,24 +593,7 @@ __device__ __forceinline__ void integrateOpen(const Dev &d, const IntegratorPara
       ke = kinetic(s);
       disp = storeAtom(d, desc.y, s, true);
     } else if (desc.x == UNIT_WATER) {
-      Atom w[3];
-      const int atoms[3] = {desc.y, desc.z, desc.w};
-      for (int a = 0; a < 3; ++a) loadAtom(d, atoms[a], w[a]);
-      const double4 geom = d.waterGeom[u];
-      if (closing) {
-        for (int a = 0; a < 3; ++a) kick(w[a], h);
-        rattleWater(w);
-      }
-      for (int a = 0; a < 3; ++a) kick(w[a], h);
-      rattleWater(w);
-      driftWater(w, h, geom);
-      for (int a = 0; a < 3; ++a) thermostat(d, ip, atoms[a], w[a], step);
-      rattleWater(w);
-      driftWater(w, h, geom);
-      for (int a = 0; a < 3; ++a) {
-        ke += kinetic(w[a]);
-        disp = fmaxf(disp, storeAtom(d, atoms[a], w[a], true));
-      }
+      openWater(d, ip, u, desc, closing, step, h, ke, disp);
     } else if (desc.x == UNIT_STAR) {
       const int m = desc.z;
       Atom w[STAR_ATOMS];
@@ -643,7 +683,7 @@ __device__ __forceinline__ void integrateOpen(const Dev &d, const IntegratorPara
 
 __global__ void __launch_bounds__(INTEGRATE_BLOCK) integrateOpenKernel(Dev d, IntegratorParams ip, int closing, int early) {
   extern __shared__ double clusterLds[];
-  integrateOpen(d, ip, closing, early, blockIdx.x, gridDim.x, clusterLds);
+  integrateOpen(d, ip, closing, early, d.waterBlocks + blockIdx.x, gridDim.x, clusterLds);
 }
 
 __global__ void __launch_bounds__(INTEGRATE_BLOCK)
@@ -651,7 +691,7 @@ __global__ void __launch_bounds__(INTEGRATE_BLOCK)
   extern __shared__ double clusterLds[];
   const int2 t = task[blockIdx.x];
   if (t.x < 0) return;
-  integrateOpen(v[t.x].d, v[t.x].ip, closing, early, t.y, v[t.x].openBlocks, clusterLds);
+  integrateOpen(v[t.x].d, v[t.x].ip, closing, early, v[t.x].d.waterBlocks + t.y, v[t.x].openBlocks, clusterLds);
 }
 
 // The closing kick + RATTLE of a block's last step.
@@ -758,7 +798,7 @@ __global__ void __launch_bounds__(256) reduceEnergiesKernel(const double *slots,
 // block's last step (withEarly), as the single-system block does.
 __global__ void __launch_bounds__(256) reduceEnergiesBatchKernel(const BatchView *__restrict__ v, int withEarly) {
   const int term = blockIdx.x % E_TERM_COUNT, r = blockIdx.x / E_TERM_COUNT;
-  const int count = term == E_KINETIC ? 0 : term == E_KINETIC_EARLY ? (withEarly ? v[r].openBlocks : 0)
+  const int count = term == E_KINETIC ? 0 : term == E_KINETIC_EARLY ? (withEarly ? v[r].openSlots : 0)
                                                                     : v[r].counts.c[term];
   reduceEnergies(v[r].d.slots, term, count, v[r].terms);
 }
@@ -1069,16 +1109,16 @@ __global__ void constrainPositionsKernel(Dev d, const float4 *reference, Integra
 
 // ---- launchers ----------------------------------------------------------------
 
-static int unitBlocks(const Dev &d) { return (d.nUnits + INTEGRATE_BLOCK - 1) / INTEGRATE_BLOCK; }
-
+// The water blocks, then the general blocks; returns the in-step kinetic
+// slots written.
 int launchIntegrateOpen(const Dev &d, const IntegratorParams &ip, int closing, int early, hipStream_t s) {
-  const int blocks = unitBlocks(d);
-  integrateOpenKernel<<<blocks, INTEGRATE_BLOCK, ip.clusterLdsBytes, s>>>(d, ip, closing, early);
-  return blocks;
+  if (d.waterBlocks > 0) integrateWaterKernel<<<d.waterBlocks, INTEGRATE_BLOCK, 0, s>>>(d, ip, closing, early);
+  integrateOpenKernel<<<generalBlockCount(d), INTEGRATE_BLOCK, ip.clusterLdsBytes, s>>>(d, ip, closing, early);
+  return openSlotCount(d);
 }
 
 int launchIntegrateClose(const Dev &d, const IntegratorParams &ip, int withKinetic, hipStream_t s) {
-  const int blocks = unitBlocks(d);
+  const int blocks = unitBlockCount(d);
   integrateCloseKernel<<<blocks, INTEGRATE_BLOCK, ip.clusterLdsBytes, s>>>(d, ip, withKinetic);
   return blocks;
 }
@@ -1110,13 +1150,15 @@ void launchDisplace(const Dev &d, float scale, float cap, hipStream_t s) {
 }
 
 void launchConstrainPositions(const Dev &d, const float4 *reference, const IntegratorParams &ip, hipStream_t s) {
-  constrainPositionsKernel<<<unitBlocks(d), INTEGRATE_BLOCK, 0, s>>>(d, reference, ip);
+  constrainPositionsKernel<<<unitBlockCount(d), INTEGRATE_BLOCK, 0, s>>>(d, reference, ip);
 }
 
 // ---- batched launchers ---------------------------------------------------------
 
-void launchIntegrateOpenBatch(const BatchView *v, const TaskTable &t, int closing, int early, int lds, hipStream_t s) {
-  integrateOpenBatchKernel<<<t.blocks, INTEGRATE_BLOCK, lds, s>>>(v, t.task, closing, early);
+void launchIntegrateOpenBatch(const BatchView *v, const TaskTable &water, const TaskTable &general, int closing,
+                              int early, int lds, hipStream_t s) {
+  if (water.blocks > 0) integrateWaterBatchKernel<<<water.blocks, INTEGRATE_BLOCK, 0, s>>>(v, water.task, closing, early);
+  integrateOpenBatchKernel<<<general.blocks, INTEGRATE_BLOCK, lds, s>>>(v, general.task, closing, early);
 }
 
 void launchIntegrateCloseBatch(const BatchView *v, const TaskTable &t, int lds, hipStream_t s) {

@@ -72,6 +72,15 @@ struct IntegratorParams {
 };
 // closing: apply the previous step's closing kick + RATTLE first.
 // early: record the in-step kinetic energy (E_KINETIC_EARLY).
+// The unit table's workgroups: the leading water blocks (their own kernel),
+// then the general blocks (at least one: its last workgroup decides the
+// rebuild); the in-step kinetic slots of a step cover both.
+inline int unitBlockCount(const Dev &d) { return (d.nUnits + INTEGRATE_BLOCK - 1) / INTEGRATE_BLOCK; }
+inline int generalBlockCount(const Dev &d) {
+  const int rest = unitBlockCount(d) - d.waterBlocks;
+  return rest > 1 ? rest : 1;
+}
+inline int openSlotCount(const Dev &d) { return d.waterBlocks + generalBlockCount(d); }
 int launchIntegrateOpen(const Dev &d, const IntegratorParams &ip, int closing, int early, hipStream_t s);
 int launchIntegrateClose(const Dev &d, const IntegratorParams &ip, int kinetic, hipStream_t s);
 int launchKinetic(const Dev &d, int term, hipStream_t s);
@@ -107,7 +116,8 @@ struct BatchView {
   Dev d;
   PmeDev p;
   IntegratorParams ip;
-  int openBlocks;        // the integrator's grid for this replica (its last-block ticket)
+  int openBlocks;        // the integrator's general blocks for this replica (its last-block ticket)
+  int openSlots;         // in-step kinetic slots (water and general blocks)
   int rebuild;           // 1: this replica's lists are rebuilt at the coming evaluation
   float scale;           // the barostat's length scale for this replica (1: no move)
   int restore;           // 1: the barostat puts this replica back
@@ -128,7 +138,7 @@ struct TaskTable {
 };
 
 enum BatchKernel {
-  BK_OPEN = 0,      // integrator (units)
+  BK_OPEN = 0,      // integrator general blocks (after the water blocks)
   BK_ATOMS64,       // one wave per atom (nonbonded at 64 lanes, neighbour list)
   BK_BONDED,        // bonded, exception and restraint terms
   BK_PLANES,        // PME plane kernels (one workgroup per x plane)
@@ -138,11 +148,14 @@ enum BatchKernel {
   BK_MOLECULES,     // 256 molecules per workgroup (centroids)
   BK_INFLUENCE,     // 256 half-spectrum points per workgroup
   BK_NONBONDED,     // the direct space at the batch's lanes per atom
+  BK_WATER,         // integrator water blocks
+  BK_UNITS,         // every integrator unit block (the closing kick)
   BK_COUNT
 };
 
 // Batched launchers. `lds` is the largest dynamic LDS any replica needs.
-void launchIntegrateOpenBatch(const BatchView *v, const TaskTable &t, int closing, int early, int lds, hipStream_t s);
+void launchIntegrateOpenBatch(const BatchView *v, const TaskTable &water, const TaskTable &general, int closing,
+                              int early, int lds, hipStream_t s);
 void launchIntegrateCloseBatch(const BatchView *v, const TaskTable &t, int lds, hipStream_t s);
 // withEarly: a block's last step (the in-step kinetic energy is summed too).
 void launchReduceEnergiesBatch(const BatchView *v, int replicas, int withEarly, hipStream_t s);
