@@ -217,7 +217,18 @@ BoxF boxFor(const rmd_ctx *c) {
   return b;
 }
 
+bool rectangular(const double *box) {
+  return box[1] == 0.0 && box[2] == 0.0 && box[3] == 0.0 && box[5] == 0.0 && box[6] == 0.0 && box[7] == 0.0;
+}
+
 void writeBox(rmd_ctx *c) {
+  // the pair kernels have a rectangular-box form, chosen at launch: a box
+  // that changes shape invalidates the captured launches
+  const int rect = rectangular(c->box) ? 1 : 0;
+  if (rect != c->d.rect) {
+    c->d.rect = rect;
+    invalidateGraphs(c);
+  }
   const BoxF b = boxFor(c);
   HIP_CHECK(hipMemcpyAsync(c->boxDev, &b, sizeof b, hipMemcpyHostToDevice, c->stream));
   // the box crosses from pageable host memory: finish before `b` goes
@@ -1293,7 +1304,7 @@ int rmd_energy(rmd_ctx *c, rmd_energy_terms *out) {
     std::memcpy(slots, c->slotCounts, sizeof slots);
     slots[E_KINETIC] = launchKinetic(c->d, E_KINETIC, c->stream);
     const bool earlyValid = c->kineticSampledAtStep == c->step;
-    slots[E_KINETIC_EARLY] = earlyValid ? openSlotCount(c->d) : 0;
+    slots[E_KINETIC_EARLY] = earlyValid ? singleOpenSlotCount(c->d) : 0;
     SlotCounts counts;
     std::memcpy(counts.c, slots, sizeof slots);
     launchReduceEnergies(c->d, counts, c->dTerms, c->stream);
@@ -1896,6 +1907,7 @@ struct rmd_batch {
   double alpha = 0;
   int lanes = 16;  // direct-space lanes per atom of every member
   int maxTypes = 0;  // the largest LJ type count of the members
+  int allRect = 1;   // every member's box is rectangular
   bool concurrent = true, useGraphs = true;
   std::map<int, hipGraphExec_t> graphs;
   std::atomic<int> stop{0};
@@ -1959,7 +1971,23 @@ TaskTable buildTasks(rmd_batch *b, int kind) {
 
 // Refresh the device views from the member contexts (their buffers move when
 // a list grows or the cell grid changes) with the flags the host has set.
+// The batch launches the rectangular-box form of the pair kernels when every
+// member's box is rectangular; a change drops the captured launches.
+void refreshRect(rmd_batch *b) {
+  int all = 1;
+  for (rmd_ctx *c : b->reps) {
+    c->d.rect = rectangular(c->box) ? 1 : 0;
+    all = all && c->d.rect;
+  }
+  if (all != b->allRect) {
+    b->allRect = all;
+    for (auto &g : b->graphs) (void)hipGraphExecDestroy(g.second);
+    b->graphs.clear();
+  }
+}
+
 void uploadViews(rmd_batch *b) {
+  refreshRect(b);
   for (int r = 0; r < b->replicas; ++r) {
     const rmd_ctx *c = b->reps[r];
     BatchView &v = b->views[r];
@@ -2004,6 +2032,7 @@ void clearFlags(rmd_batch *b) {
 
 // Every replica's device box from its host box, in one copy.
 void uploadBoxes(rmd_batch *b) {
+  refreshRect(b);
   std::vector<BoxF> boxes(b->replicas);
   for (int r = 0; r < b->replicas; ++r) boxes[r] = boxFor(b->reps[r]);
   HIP_CHECK(hipMemcpyAsync(b->boxAll, boxes.data(), boxes.size() * sizeof(BoxF), hipMemcpyHostToDevice,
@@ -2061,10 +2090,10 @@ void enqueueBatchForces(rmd_batch *b, int energies, int terms = RMD_TERM_ALL) {
                    b->ldsGather, pmeStream);
   if (terms & (RMD_TERM_BONDED | RMD_TERM_EXCEPTIONS | RMD_TERM_RESTRAINT))
     launchBondedBatch(b->dViews, b->tables[BK_BONDED], b->alpha, terms, energies, bondedStream);
-  launchRebuildBatch(b->dViews, b->tables[BK_REPLICA], b->tables[BK_ATOMS64], st.stream);
+  launchRebuildBatch(b->dViews, b->tables[BK_REPLICA], b->tables[BK_ATOMS64], b->allRect, st.stream);
   if (terms & RMD_TERM_DIRECT)
     launchNonbondedBatch(b->dViews, b->tables[BK_NONBONDED], b->lanes, (float)b->alpha, energies, b->maxTypes,
-                         st.stream);
+                         b->allRect, st.stream);
   if (fork) {
     for (int k = 0; k < 2; ++k) {
       HIP_CHECK(hipEventRecord(st.join[k], st.side[k]));
@@ -2480,6 +2509,7 @@ void createBatch(rmd_batch *b, const rmd_system *const *systems, int count, cons
       throw Error(RMD_E_INVALID, replicaLabel(r) + "every replica of a batch needs the same Ewald alpha (the same "
                                                    "cutoff and tolerance)");
     b->maxTypes = std::max(b->maxTypes, c->d.types);
+    b->allRect = b->allRect && c->d.rect;
     b->ldsForward = std::max(b->ldsForward, pmePlaneForwardLds(c->p));
     b->ldsXlines = std::max(b->ldsXlines, pmeXLinesLds(c->p));
     b->ldsGather = std::max(b->ldsGather, pmePlaneGatherLds(c->p));
@@ -2665,13 +2695,15 @@ int rmd_batch_profile_nonbonded(rmd_batch *b, int32_t reps, double *avg_ms) {
     if (reps <= 0 || !avg_ms) throw Error(RMD_E_INVALID, "reps must be positive");
     hipStream_t s = b->streams.stream;
     uploadViews(b);
-    launchNonbondedBatch(b->dViews, b->tables[BK_NONBONDED], b->lanes, (float)b->alpha, 0, b->maxTypes, s);
+    launchNonbondedBatch(b->dViews, b->tables[BK_NONBONDED], b->lanes, (float)b->alpha, 0, b->maxTypes, b->allRect,
+                         s);
     hipEvent_t e0, e1;
     HIP_CHECK(hipEventCreate(&e0));
     HIP_CHECK(hipEventCreate(&e1));
     HIP_CHECK(hipEventRecord(e0, s));
     for (int k = 0; k < reps; ++k)
-      launchNonbondedBatch(b->dViews, b->tables[BK_NONBONDED], b->lanes, (float)b->alpha, 0, b->maxTypes, s);
+      launchNonbondedBatch(b->dViews, b->tables[BK_NONBONDED], b->lanes, (float)b->alpha, 0, b->maxTypes, b->allRect,
+                         s);
     HIP_CHECK(hipEventRecord(e1, s));
     HIP_CHECK(hipEventSynchronize(e1));
     float ms = 0;

@@ -70,6 +70,31 @@ __device__ __forceinline__ float minimumImageR2(float &dx, float &dy, float &dz,
   return sum + zz;
 }
 
+// The same minimum image for a rectangular box: the terms of the general form
+// that multiply an off-diagonal row entry are products with exact zeros and
+// leave every bit as it is, so they are left out.
+__device__ __forceinline__ float minimumImageR2Rect(float &dx, float &dy, float &dz, const BoxF &b, float &s0,
+                                                    float &s1, float &s2) {
+#pragma clang fp contract(off)
+  s2 = floorf(dz * b.inv[2] + 0.5f);
+  dz = dz - s2 * b.c[2];
+  s1 = floorf(dy * b.inv[1] + 0.5f);
+  dy = dy - s1 * b.b[1];
+  s0 = floorf(dx * b.inv[0] + 0.5f);
+  dx = dx - s0 * b.a[0];
+  float xx = dx * dx;
+  float yy = dy * dy;
+  float zz = dz * dz;
+  float sum = xx + yy;
+  return sum + zz;
+}
+
+template <bool RECT>
+__device__ __forceinline__ float imageR2(float &dx, float &dy, float &dz, const BoxF &b, float &s0, float &s1,
+                                         float &s2) {
+  return RECT ? minimumImageR2Rect(dx, dy, dz, b, s0, s1, s2) : minimumImageR2(dx, dy, dz, b, s0, s1, s2);
+}
+
 __device__ __forceinline__ float minimumImageR2(float &dx, float &dy, float &dz, const BoxF &b) {
   float s0, s1, s2;
   return minimumImageR2(dx, dy, dz, b, s0, s1, s2);

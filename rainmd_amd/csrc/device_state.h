@@ -106,6 +106,7 @@ struct Dev {
   int *restraintAtoms;
   double4 *restraintRef;    // reference xyz and k, FP64
   // integrator units
+  int rect;                 // the box is rectangular (and stays so: volume moves are isotropic)
   int nUnits;
   int waterBlocks;          // leading unit blocks holding water units only
   int4 *units;              // kind, a, b, c (see integrate.hip)

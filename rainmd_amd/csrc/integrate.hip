@@ -681,9 +681,12 @@ __device__ __forceinline__ void integrateOpen(const Dev &d, const IntegratorPara
   }
 }
 
-__global__ void __launch_bounds__(INTEGRATE_BLOCK) integrateOpenKernel(Dev d, IntegratorParams ip, int closing, int early) {
+// The units of blocks firstBlock.. (0: every unit; the water blocks' count:
+// the general units after the water kernel).
+__global__ void __launch_bounds__(INTEGRATE_BLOCK)
+    integrateOpenKernel(Dev d, IntegratorParams ip, int closing, int early, int firstBlock) {
   extern __shared__ double clusterLds[];
-  integrateOpen(d, ip, closing, early, d.waterBlocks + blockIdx.x, gridDim.x, clusterLds);
+  integrateOpen(d, ip, closing, early, firstBlock + blockIdx.x, gridDim.x, clusterLds);
 }
 
 __global__ void __launch_bounds__(INTEGRATE_BLOCK)
@@ -1112,8 +1115,13 @@ __global__ void constrainPositionsKernel(Dev d, const float4 *reference, Integra
 // The water blocks, then the general blocks; returns the in-step kinetic
 // slots written.
 int launchIntegrateOpen(const Dev &d, const IntegratorParams &ip, int closing, int early, hipStream_t s) {
-  if (d.waterBlocks > 0) integrateWaterKernel<<<d.waterBlocks, INTEGRATE_BLOCK, 0, s>>>(d, ip, closing, early);
-  integrateOpenKernel<<<generalBlockCount(d), INTEGRATE_BLOCK, ip.clusterLdsBytes, s>>>(d, ip, closing, early);
+  if (!splitWater(d)) {
+    integrateOpenKernel<<<unitBlockCount(d), INTEGRATE_BLOCK, ip.clusterLdsBytes, s>>>(d, ip, closing, early, 0);
+    return unitBlockCount(d);
+  }
+  integrateWaterKernel<<<d.waterBlocks, INTEGRATE_BLOCK, 0, s>>>(d, ip, closing, early);
+  integrateOpenKernel<<<generalBlockCount(d), INTEGRATE_BLOCK, ip.clusterLdsBytes, s>>>(d, ip, closing, early,
+                                                                                       d.waterBlocks);
   return openSlotCount(d);
 }
 

@@ -81,6 +81,13 @@ inline int generalBlockCount(const Dev &d) {
   return rest > 1 ? rest : 1;
 }
 inline int openSlotCount(const Dev &d) { return d.waterBlocks + generalBlockCount(d); }
+// A single system runs its water blocks in their own kernel when there are
+// many (the water path's occupancy), else one kernel for every unit (one
+// launch, and the general units' serial solves overlap the waters). A batch
+// always splits.
+constexpr int SPLIT_WATER_BLOCKS = 32;
+inline bool splitWater(const Dev &d) { return d.waterBlocks >= SPLIT_WATER_BLOCKS; }
+inline int singleOpenSlotCount(const Dev &d) { return splitWater(d) ? openSlotCount(d) : unitBlockCount(d); }
 int launchIntegrateOpen(const Dev &d, const IntegratorParams &ip, int closing, int early, hipStream_t s);
 int launchIntegrateClose(const Dev &d, const IntegratorParams &ip, int kinetic, hipStream_t s);
 int launchKinetic(const Dev &d, int term, hipStream_t s);
@@ -179,10 +186,11 @@ bool fitsSmallCellBuild(int atoms, int cells);
 // cover at four cells each.
 bool cellSearchFits(int atoms, int cells);
 void launchRefreshSlotsBatch(const BatchView *v, const TaskTable &atoms256, hipStream_t s);
-void launchRebuildBatch(const BatchView *v, const TaskTable &replicas, const TaskTable &atoms64, hipStream_t s);
+void launchRebuildBatch(const BatchView *v, const TaskTable &replicas, const TaskTable &atoms64, int rect,
+                        hipStream_t s);
 // lanes: lanes per atom (8, 16, 32 or 64), the table's grid at that count.
 void launchNonbondedBatch(const BatchView *v, const TaskTable &t, int lanes, float alpha, int energies, int types,
-                          hipStream_t s);
+                          int rect, hipStream_t s);
 void launchBondedBatch(const BatchView *v, const TaskTable &t, double alpha, int terms, int energies, hipStream_t s);
 void launchPmeBatch(const BatchView *v, const TaskTable &planes, const TaskTable &xlines, int energies,
                     size_t forwardLds, size_t xlinesLds, size_t gatherLds, hipStream_t s);

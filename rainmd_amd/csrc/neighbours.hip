@@ -259,13 +259,15 @@ __device__ __forceinline__ int wrapCell(int v, int count) {
 // Whether candidate pj joins atom's list: not itself, within rl (the FP32
 // minimum image of the kernels), not excluded. Exclusions are checked only
 // for candidates inside the range of the atom's (sorted) exclusion partners.
+template <bool RECT>
 __device__ __forceinline__ bool acceptPair(GlobalPtr<int> excl, const BoxF &box, float4 pi, float4 pj, int atom,
                                            int exBegin, int exEnd, int exMin, int exMax) {
   // the distance test first and without a branch, so the candidate's loads
   // are used whole where they are issued
   const int j = __float_as_int(pj.w);
   float dx = pi.x - pj.x, dy = pi.y - pj.y, dz = pi.z - pj.z;
-  const float r2 = minimumImageR2(dx, dy, dz, box);
+  float s0, s1, s2;
+  const float r2 = imageR2<RECT>(dx, dy, dz, box, s0, s1, s2);
   bool accept = (j != atom) & (r2 <= box.rl2);
   if (accept & (j >= exMin) & (j <= exMax))
     for (int e = exBegin; e < exEnd; ++e)
@@ -283,6 +285,7 @@ __device__ __forceinline__ bool acceptPair(GlobalPtr<int> excl, const BoxF &box,
 // only `cap` are stored, and the true requirement reaches the host through
 // Scalars::required (the reference's overflow contract,
 // webgpu/kernels/neighbours.js:128-146).
+template <bool RECT>
 __device__ __forceinline__ void neighbours(const Dev &d, int block) {
   if (!d.sc->rebuild) return;
   __shared__ int rangeBegin[4][STENCIL_RANGES];
@@ -394,7 +397,7 @@ __device__ __forceinline__ void neighbours(const Dev &d, int block) {
     // completion counter)
     bool accept[CAND];
 #pragma unroll
-    for (int u = 0; u < CAND; ++u) accept[u] = acceptPair(excl, box, pi, p[u], atom, exBegin, exEnd, exMin, exMax);
+    for (int u = 0; u < CAND; ++u) accept[u] = acceptPair<RECT>(excl, box, pi, p[u], atom, exBegin, exEnd, exMin, exMax);
 #pragma unroll
     for (int u = 0; u < CAND; ++u)
       if (base + 64 * u < total) emit(accept[u], slot[u]);
@@ -414,6 +417,7 @@ __device__ __forceinline__ void neighbours(const Dev &d, int block) {
 // accepted candidates are compacted into its own slice in candidate order:
 // the lists are entry for entry those of one wave per atom, for a fraction
 // of the loads and of the per-atom set-up.
+template <bool RECT>
 __device__ __forceinline__ void neighboursByCell(const Dev &d, int block) {
   __shared__ int rangeBegin[4][STENCIL_RANGES];
   __shared__ int rangeStart[4][STENCIL_RANGES + 1];
@@ -526,7 +530,7 @@ __device__ __forceinline__ void neighboursByCell(const Dev &d, int block) {
 #pragma unroll
         for (int u = 0; u < CAND; ++u)
           accept[u] = (base + lane + 64 * u < total) &
-                      acceptPair(excl, box, pi, p[u], atom, exBegin, exEnd, exMin, exMax);
+                      acceptPair<RECT>(excl, box, pi, p[u], atom, exBegin, exEnd, exMin, exMax);
         const GlobalOut<int> out = globalOut(d.nbrList + (size_t)(first + i) * cap);
 #pragma unroll
         for (int u = 0; u < CAND; ++u) {
@@ -554,27 +558,32 @@ __device__ __forceinline__ void neighboursByCell(const Dev &d, int block) {
 }
 
 // A block is four atoms or four cells, as the box's plan says.
+template <bool RECT>
 __device__ __forceinline__ void searchBlock(const Dev &d, int block) {
   if (!d.sc->rebuild) return;
   if (d.box->cellSearch)
-    neighboursByCell(d, block);
+    neighboursByCell<RECT>(d, block);
   else
-    neighbours(d, block);
+    neighbours<RECT>(d, block);
 }
 
-__global__ void __launch_bounds__(256) neighboursKernel(Dev d) { searchBlock(d, blockIdx.x); }
+template <bool RECT>
+__global__ void __launch_bounds__(256) neighboursKernel(Dev d) {
+  searchBlock<RECT>(d, blockIdx.x);
+}
 
 // Batched, grid-stride over the task table: on most steps only the few
 // replicas due a rebuild have work, and a workgroup skips the others' tasks
 // with one scalar load each instead of the dispatch of one workgroup per task.
 // The stride is a multiple of the XCD count, so a workgroup stays on the
 // replicas of its own XCD.
+template <bool RECT>
 __global__ void __launch_bounds__(256) neighboursBatchKernel(const BatchView *__restrict__ v,
                                                              const int2 *__restrict__ task, int tasks) {
   for (int k = blockIdx.x; k < tasks; k += gridDim.x) {
     const int2 t = task[k];
     if (t.x < 0 || !v[t.x].d.sc->rebuild) continue;
-    searchBlock(v[t.x].d, t.y);
+    searchBlock<RECT>(v[t.x].d, t.y);
   }
 }
 
@@ -612,17 +621,24 @@ void launchRebuild(const Dev &d, hipStream_t s) {
     cellsScatterKernel<<<blocks, 256, 0, s>>>(d);
     cellsSortKernel<<<d.cellCapacity, 256, 0, s>>>(d);
   }
-  neighboursKernel<<<(d.n + 3) / 4, 256, 0, s>>>(d);
+  if (d.rect)
+    neighboursKernel<true><<<(d.n + 3) / 4, 256, 0, s>>>(d);
+  else
+    neighboursKernel<false><<<(d.n + 3) / 4, 256, 0, s>>>(d);
 }
 
 // Batched rebuild: every replica's cell list in one workgroup of its own (a
 // batch takes replicas the single-workgroup build fits), then the lists.
-void launchRebuildBatch(const BatchView *v, const TaskTable &replicas, const TaskTable &atoms64, hipStream_t s) {
+void launchRebuildBatch(const BatchView *v, const TaskTable &replicas, const TaskTable &atoms64, int rect,
+                        hipStream_t s) {
   cellsBuildSmallBatchKernel<<<replicas.blocks, SMALL_THREADS, 0, s>>>(v, replicas.task);
   // 2048 workgroups per XCD at most: enough to fill the chip when many
   // replicas rebuild together
   const int grid = atoms64.blocks < 16384 ? atoms64.blocks : 16384;
-  neighboursBatchKernel<<<grid, 256, 0, s>>>(v, atoms64.task, atoms64.blocks);
+  if (rect)
+    neighboursBatchKernel<true><<<grid, 256, 0, s>>>(v, atoms64.task, atoms64.blocks);
+  else
+    neighboursBatchKernel<false><<<grid, 256, 0, s>>>(v, atoms64.task, atoms64.blocks);
 }
 
 bool fitsSmallCellBuild(int atoms, int cells) { return atoms <= SMALL_MAX_ATOMS && cells <= SMALL_MAX_CELLS; }

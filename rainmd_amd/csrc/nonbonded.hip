@@ -33,14 +33,14 @@ inline size_t tableLdsBytes(int types) { return tableInLds(types) ? (size_t)type
 // and selects zero for those (a branch around them would be taken by whole
 // waves almost never, and lets the compiler sink the pair's loads behind
 // it); the energy variant branches around its FP64 work.
-template <bool ENERGIES, bool LDS_TABLE>
+template <bool ENERGIES, bool LDS_TABLE, bool RECT>
 __device__ __forceinline__ void pairTerm(const float4 &pi, float qi, int ti, const float4 &pj, int tj, bool live,
                                          const BoxF &box, float alpha, const float2 *ldsTable,
                                          GlobalPtr<float2> table, const double2 *lj64, float &fx, float &fy, float &fz,
                                          double &elj, double &ecoul, double &dscale) {
   float dx = pi.x - pj.x, dy = pi.y - pj.y, dz = pi.z - pj.z;
   float s0, s1, s2;
-  const float r2 = minimumImageR2(dx, dy, dz, box, s0, s1, s2);
+  const float r2 = imageR2<RECT>(dx, dy, dz, box, s0, s1, s2);
   const bool inside = live && r2 <= box.rc2;
   if (ENERGIES && !inside) return;
   const float2 ab = LDS_TABLE ? ldsTable[ti + tj] : table[ti + tj];
@@ -72,7 +72,7 @@ __device__ __forceinline__ void pairTerm(const float4 &pi, float qi, int ti, con
   fz += f * dz;
 }
 
-template <int LANES, bool ENERGIES, bool LDS_TABLE>
+template <int LANES, bool ENERGIES, bool LDS_TABLE, bool RECT>
 __device__ __forceinline__ void nonbonded(const Dev &d, float alpha, int block, float2 *ldsTable) {
   // The LJ type-pair table in LDS: one lookup per pair, off the global
   // latency chain (list index -> position and type -> table entry).
@@ -124,7 +124,7 @@ __device__ __forceinline__ void nonbonded(const Dev &d, float alpha, int block, 
       }
 #pragma unroll
       for (int u = 0; u < UNROLL; ++u)
-        pairTerm<ENERGIES, LDS_TABLE>(pi, qi, ti, p[u], t[u], k + u * LANES < count, box, alpha, ldsTable, table,
+        pairTerm<ENERGIES, LDS_TABLE, RECT>(pi, qi, ti, p[u], t[u], k + u * LANES < count, box, alpha, ldsTable, table,
                                       d.lj64, fx, fy, fz, elj, ecoul, dscale);
     }
   }
@@ -170,80 +170,84 @@ __device__ __forceinline__ void nonbonded(const Dev &d, float alpha, int block, 
   }
 }
 
-template <int LANES, bool ENERGIES, bool LDS_TABLE>
+template <int LANES, bool ENERGIES, bool LDS_TABLE, bool RECT>
 __global__ void __launch_bounds__(256) nonbondedKernel(Dev d, float alpha) {
   extern __shared__ float2 ldsTable[];
-  nonbonded<LANES, ENERGIES, LDS_TABLE>(d, alpha, blockIdx.x, ldsTable);
+  nonbonded<LANES, ENERGIES, LDS_TABLE, RECT>(d, alpha, blockIdx.x, ldsTable);
 }
 
 // Batched.
-template <int LANES, bool ENERGIES, bool LDS_TABLE>
+template <int LANES, bool ENERGIES, bool LDS_TABLE, bool RECT>
 __global__ void __launch_bounds__(256) nonbondedBatchKernel(const BatchView *__restrict__ v,
                                                             const int2 *__restrict__ task, float alpha) {
   extern __shared__ float2 ldsTable[];
   const int2 t = task[blockIdx.x];
   if (t.x < 0) return;
-  nonbonded<LANES, ENERGIES, LDS_TABLE>(v[t.x].d, alpha, t.y, ldsTable);
+  nonbonded<LANES, ENERGIES, LDS_TABLE, RECT>(v[t.x].d, alpha, t.y, ldsTable);
 }
 
-template <bool ENERGIES, bool LDS_TABLE>
+template <bool ENERGIES, bool LDS_TABLE, bool RECT>
 static void launchLanes(const Dev &d, float alpha, int lanes, int blocks, size_t lds, hipStream_t s) {
   switch (lanes) {
-    case 1: nonbondedKernel<1, ENERGIES, LDS_TABLE><<<blocks, 256, lds, s>>>(d, alpha); break;
-    case 2: nonbondedKernel<2, ENERGIES, LDS_TABLE><<<blocks, 256, lds, s>>>(d, alpha); break;
-    case 4: nonbondedKernel<4, ENERGIES, LDS_TABLE><<<blocks, 256, lds, s>>>(d, alpha); break;
-    case 8: nonbondedKernel<8, ENERGIES, LDS_TABLE><<<blocks, 256, lds, s>>>(d, alpha); break;
-    case 16: nonbondedKernel<16, ENERGIES, LDS_TABLE><<<blocks, 256, lds, s>>>(d, alpha); break;
-    case 32: nonbondedKernel<32, ENERGIES, LDS_TABLE><<<blocks, 256, lds, s>>>(d, alpha); break;
-    default: nonbondedKernel<64, ENERGIES, LDS_TABLE><<<blocks, 256, lds, s>>>(d, alpha); break;
+    case 1: nonbondedKernel<1, ENERGIES, LDS_TABLE, RECT><<<blocks, 256, lds, s>>>(d, alpha); break;
+    case 2: nonbondedKernel<2, ENERGIES, LDS_TABLE, RECT><<<blocks, 256, lds, s>>>(d, alpha); break;
+    case 4: nonbondedKernel<4, ENERGIES, LDS_TABLE, RECT><<<blocks, 256, lds, s>>>(d, alpha); break;
+    case 8: nonbondedKernel<8, ENERGIES, LDS_TABLE, RECT><<<blocks, 256, lds, s>>>(d, alpha); break;
+    case 16: nonbondedKernel<16, ENERGIES, LDS_TABLE, RECT><<<blocks, 256, lds, s>>>(d, alpha); break;
+    case 32: nonbondedKernel<32, ENERGIES, LDS_TABLE, RECT><<<blocks, 256, lds, s>>>(d, alpha); break;
+    default: nonbondedKernel<64, ENERGIES, LDS_TABLE, RECT><<<blocks, 256, lds, s>>>(d, alpha); break;
   }
+}
+
+template <bool ENERGIES, bool RECT>
+static void launchTable(const Dev &d, float alpha, int lanes, int blocks, hipStream_t s) {
+  if (tableInLds(d.types))
+    launchLanes<ENERGIES, true, RECT>(d, alpha, lanes, blocks, tableLdsBytes(d.types), s);
+  else
+    launchLanes<ENERGIES, false, RECT>(d, alpha, lanes, blocks, 0, s);
 }
 
 int launchNonbonded(const Dev &d, float alpha, int lanes, int energies, hipStream_t s) {
   const long long threads = (long long)d.n * lanes;
   const int blocks = (int)((threads + 255) / 256);
-  const size_t lds = tableLdsBytes(d.types);
-  if (tableInLds(d.types)) {
-    if (energies)
-      launchLanes<true, true>(d, alpha, lanes, blocks, lds, s);
-    else
-      launchLanes<false, true>(d, alpha, lanes, blocks, lds, s);
-  } else {
-    if (energies)
-      launchLanes<true, false>(d, alpha, lanes, blocks, 0, s);
-    else
-      launchLanes<false, false>(d, alpha, lanes, blocks, 0, s);
-  }
+  if (energies)
+    d.rect ? launchTable<true, true>(d, alpha, lanes, blocks, s) : launchTable<true, false>(d, alpha, lanes, blocks, s);
+  else
+    d.rect ? launchTable<false, true>(d, alpha, lanes, blocks, s)
+           : launchTable<false, false>(d, alpha, lanes, blocks, s);
   return blocks;
 }
 
-template <bool ENERGIES, bool LDS_TABLE>
+template <bool ENERGIES, bool LDS_TABLE, bool RECT>
 static void launchBatchLanes(const BatchView *v, const TaskTable &t, int lanes, float alpha, size_t lds,
                              hipStream_t s) {
   switch (lanes) {
-    case 8: nonbondedBatchKernel<8, ENERGIES, LDS_TABLE><<<t.blocks, 256, lds, s>>>(v, t.task, alpha); break;
-    case 16: nonbondedBatchKernel<16, ENERGIES, LDS_TABLE><<<t.blocks, 256, lds, s>>>(v, t.task, alpha); break;
-    case 32: nonbondedBatchKernel<32, ENERGIES, LDS_TABLE><<<t.blocks, 256, lds, s>>>(v, t.task, alpha); break;
-    default: nonbondedBatchKernel<64, ENERGIES, LDS_TABLE><<<t.blocks, 256, lds, s>>>(v, t.task, alpha); break;
+    case 8: nonbondedBatchKernel<8, ENERGIES, LDS_TABLE, RECT><<<t.blocks, 256, lds, s>>>(v, t.task, alpha); break;
+    case 16: nonbondedBatchKernel<16, ENERGIES, LDS_TABLE, RECT><<<t.blocks, 256, lds, s>>>(v, t.task, alpha); break;
+    case 32: nonbondedBatchKernel<32, ENERGIES, LDS_TABLE, RECT><<<t.blocks, 256, lds, s>>>(v, t.task, alpha); break;
+    default: nonbondedBatchKernel<64, ENERGIES, LDS_TABLE, RECT><<<t.blocks, 256, lds, s>>>(v, t.task, alpha); break;
   }
 }
 
+template <bool ENERGIES, bool RECT>
+static void launchBatchTable(const BatchView *v, const TaskTable &t, int lanes, float alpha, int types,
+                             hipStream_t s) {
+  if (tableInLds(types))
+    launchBatchLanes<ENERGIES, true, RECT>(v, t, lanes, alpha, tableLdsBytes(types), s);
+  else
+    launchBatchLanes<ENERGIES, false, RECT>(v, t, lanes, alpha, 0, s);
+}
+
 // `types` is the largest type count of the batch's replicas: every workgroup
-// gets LDS for that table.
+// gets LDS for that table. `rect`: every replica's box is rectangular.
 void launchNonbondedBatch(const BatchView *v, const TaskTable &t, int lanes, float alpha, int energies, int types,
-                          hipStream_t s) {
-  const size_t lds = tableLdsBytes(types);
-  if (tableInLds(types)) {
-    if (energies)
-      launchBatchLanes<true, true>(v, t, lanes, alpha, lds, s);
-    else
-      launchBatchLanes<false, true>(v, t, lanes, alpha, lds, s);
-  } else {
-    if (energies)
-      launchBatchLanes<true, false>(v, t, lanes, alpha, 0, s);
-    else
-      launchBatchLanes<false, false>(v, t, lanes, alpha, 0, s);
-  }
+                          int rect, hipStream_t s) {
+  if (energies)
+    rect ? launchBatchTable<true, true>(v, t, lanes, alpha, types, s)
+         : launchBatchTable<true, false>(v, t, lanes, alpha, types, s);
+  else
+    rect ? launchBatchTable<false, true>(v, t, lanes, alpha, types, s)
+         : launchBatchTable<false, false>(v, t, lanes, alpha, types, s);
 }
 
 }  // namespace rmd
