@@ -481,7 +481,8 @@ void enqueueForces(rmd_ctx *c, int terms, int energies, int slots[E_TERM_COUNT])
   }
   launchRebuild(d, c->stream);
   if (terms & RMD_TERM_DIRECT) {
-    const int b = launchNonbonded(d, (float)c->pme.alpha, c->lanes, energies, c->stream);
+    const int b = launchNonbonded(d, (float)c->pme.alpha, ewaldRationalFits(c->pme.alpha, c->sys.cutoff) ? 1 : 0,
+                                  c->lanes, energies, c->stream);
     slots[E_LJ_DIRECT] = slots[E_COUL_DIRECT] = slots[E_DSCALE_DIRECT] = b;
   }
   if (fork) {
@@ -1732,7 +1733,10 @@ int rmd_profile_kernel(rmd_ctx *c, int32_t kernel, int32_t reps, double *avg_ms)
     int slots[E_TERM_COUNT];
     auto once = [&]() {
       switch (kernel) {
-        case 0: launchNonbonded(c->d, (float)c->pme.alpha, c->lanes, 0, c->stream); break;
+        case 0:
+          launchNonbonded(c->d, (float)c->pme.alpha, ewaldRationalFits(c->pme.alpha, c->sys.cutoff) ? 1 : 0, c->lanes,
+                          0, c->stream);
+          break;
         case 1: launchPme(c->d, c->p, 0, c->stream, nullptr); break;
         case 2: launchIntegrateOpen(c->d, c->ip, 1, 0, c->stream); break;
         case 3: launchBonded(c->d, c->pme.alpha, RMD_TERM_ALL, 0, c->stream, nullptr); break;
@@ -1908,6 +1912,7 @@ struct rmd_batch {
   int lanes = 16;  // direct-space lanes per atom of every member
   int maxTypes = 0;  // the largest LJ type count of the members
   int allRect = 1;   // every member's box is rectangular
+  int rational = 1;  // the direct-space Coulomb force from the rational fit for every member
   bool concurrent = true, useGraphs = true;
   std::map<int, hipGraphExec_t> graphs;
   std::atomic<int> stop{0};
@@ -2092,8 +2097,8 @@ void enqueueBatchForces(rmd_batch *b, int energies, int terms = RMD_TERM_ALL) {
     launchBondedBatch(b->dViews, b->tables[BK_BONDED], b->alpha, terms, energies, bondedStream);
   launchRebuildBatch(b->dViews, b->tables[BK_REPLICA], b->tables[BK_ATOMS64], b->allRect, st.stream);
   if (terms & RMD_TERM_DIRECT)
-    launchNonbondedBatch(b->dViews, b->tables[BK_NONBONDED], b->lanes, (float)b->alpha, energies, b->maxTypes,
-                         b->allRect, st.stream);
+    launchNonbondedBatch(b->dViews, b->tables[BK_NONBONDED], b->lanes, (float)b->alpha, b->rational, energies,
+                         b->maxTypes, b->allRect, st.stream);
   if (fork) {
     for (int k = 0; k < 2; ++k) {
       HIP_CHECK(hipEventRecord(st.join[k], st.side[k]));
@@ -2510,6 +2515,7 @@ void createBatch(rmd_batch *b, const rmd_system *const *systems, int count, cons
                                                    "cutoff and tolerance)");
     b->maxTypes = std::max(b->maxTypes, c->d.types);
     b->allRect = b->allRect && c->d.rect;
+    b->rational = b->rational && ewaldRationalFits(c->pme.alpha, c->sys.cutoff);
     b->ldsForward = std::max(b->ldsForward, pmePlaneForwardLds(c->p));
     b->ldsXlines = std::max(b->ldsXlines, pmeXLinesLds(c->p));
     b->ldsGather = std::max(b->ldsGather, pmePlaneGatherLds(c->p));
@@ -2695,15 +2701,15 @@ int rmd_batch_profile_nonbonded(rmd_batch *b, int32_t reps, double *avg_ms) {
     if (reps <= 0 || !avg_ms) throw Error(RMD_E_INVALID, "reps must be positive");
     hipStream_t s = b->streams.stream;
     uploadViews(b);
-    launchNonbondedBatch(b->dViews, b->tables[BK_NONBONDED], b->lanes, (float)b->alpha, 0, b->maxTypes, b->allRect,
-                         s);
+    launchNonbondedBatch(b->dViews, b->tables[BK_NONBONDED], b->lanes, (float)b->alpha, b->rational, 0, b->maxTypes,
+                         b->allRect, s);
     hipEvent_t e0, e1;
     HIP_CHECK(hipEventCreate(&e0));
     HIP_CHECK(hipEventCreate(&e1));
     HIP_CHECK(hipEventRecord(e0, s));
     for (int k = 0; k < reps; ++k)
-      launchNonbondedBatch(b->dViews, b->tables[BK_NONBONDED], b->lanes, (float)b->alpha, 0, b->maxTypes, b->allRect,
-                         s);
+      launchNonbondedBatch(b->dViews, b->tables[BK_NONBONDED], b->lanes, (float)b->alpha, b->rational, 0, b->maxTypes,
+                         b->allRect, s);
     HIP_CHECK(hipEventRecord(e1, s));
     HIP_CHECK(hipEventSynchronize(e1));
     float ms = 0;

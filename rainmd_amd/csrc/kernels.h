@@ -20,7 +20,10 @@ void launchRefreshSlots(const Dev &d, hipStream_t s);
 // ---- nonbonded direct space (nonbonded.hip) ----
 // lanes: lanes per atom (1, 2, 4, ..., 64). energies != 0 fills the
 // E_LJ_DIRECT / E_COUL_DIRECT slot rows and returns the slot count used.
-int launchNonbonded(const Dev &d, float alpha, int lanes, int energies, hipStream_t s);
+// rational: the screened Coulomb force from the rational fit (alpha * cutoff
+// within its range, ewaldRationalFits), else from erfc.
+bool ewaldRationalFits(double alpha, double cutoff);
+int launchNonbonded(const Dev &d, float alpha, int rational, int lanes, int energies, hipStream_t s);
 
 // ---- bonded, exceptions, restraints (bonded.hip) ----
 int launchBonded(const Dev &d, double alpha, int terms, int energies, hipStream_t s, int *slotsUsed);
@@ -189,8 +192,8 @@ void launchRefreshSlotsBatch(const BatchView *v, const TaskTable &atoms256, hipS
 void launchRebuildBatch(const BatchView *v, const TaskTable &replicas, const TaskTable &atoms64, int rect,
                         hipStream_t s);
 // lanes: lanes per atom (8, 16, 32 or 64), the table's grid at that count.
-void launchNonbondedBatch(const BatchView *v, const TaskTable &t, int lanes, float alpha, int energies, int types,
-                          int rect, hipStream_t s);
+void launchNonbondedBatch(const BatchView *v, const TaskTable &t, int lanes, float alpha, int rational, int energies,
+                          int types, int rect, hipStream_t s);
 void launchBondedBatch(const BatchView *v, const TaskTable &t, double alpha, int terms, int energies, hipStream_t s);
 void launchPmeBatch(const BatchView *v, const TaskTable &planes, const TaskTable &xlines, int energies,
                     size_t forwardLds, size_t xlinesLds, size_t gatherLds, hipStream_t s);

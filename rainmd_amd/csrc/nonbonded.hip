@@ -26,6 +26,30 @@ constexpr int LDS_TYPES = 90;
 inline bool tableInLds(int types) { return types <= LDS_TYPES; }
 inline size_t tableLdsBytes(int types) { return tableInLds(types) ? (size_t)types * types * sizeof(float2) : 0; }
 
+// G(w) = (erf(z) - 2 z / sqrt(pi) exp(-w)) / z^3 with w = z^2 = (alpha r)^2,
+// as P(w) / Q(w) of degree 5 (a relative-error fit on [0, 12] made for this
+// file by ewald_force_fit.py: 6.0e-8 in FP64, 4e-7 evaluated in FP32; Q >= 1
+// there). The screened Coulomb force scalar of a pair is
+// qq (1/r^3 - alpha^3 G(w)): no exponential, no erfc.
+constexpr float EWALD_G_WMAX = 12.0f;
+__device__ __forceinline__ float ewaldG(float w) {
+  float p = -3.737052279e-07f;
+  p = fmaf(p, w, 3.548286411e-05f);
+  p = fmaf(p, w, 1.812391234e-04f);
+  p = fmaf(p, w, 1.684099673e-02f);
+  p = fmaf(p, w, -1.957190408e-02f);
+  p = fmaf(p, w, 7.522528230e-01f);
+  float q = 1.580766103e-04f;
+  q = fmaf(q, w, 2.452907735e-03f);
+  q = fmaf(q, w, 2.431532518e-02f);
+  q = fmaf(q, w, 1.524804702e-01f);
+  q = fmaf(q, w, 5.739843349e-01f);
+  q = fmaf(q, w, 1.0f);
+  return p * __builtin_amdgcn_rcpf(q);
+}
+
+bool ewaldRationalFits(double alpha, double cutoff) { return alpha * cutoff * alpha * cutoff <= EWALD_G_WMAX; }
+
 // One pair: the force scalar F/r (added to the lane's force with the
 // separation), and with ENERGIES the pair's LJ, screened Coulomb and r dU/dr
 // in FP64. A pair that is not `live` (an entry past the row's count) or
@@ -33,9 +57,12 @@ inline size_t tableLdsBytes(int types) { return tableInLds(types) ? (size_t)type
 // and selects zero for those (a branch around them would be taken by whole
 // waves almost never, and lets the compiler sink the pair's loads behind
 // it); the energy variant branches around its FP64 work.
-template <bool ENERGIES, bool LDS_TABLE, bool RECT>
+// With RATIONAL (alpha rc within the fit) the Coulomb force takes ewaldG;
+// else erfc and the Gaussian.
+template <bool ENERGIES, bool LDS_TABLE, bool RECT, bool RATIONAL>
 __device__ __forceinline__ void pairTerm(const float4 &pi, float qi, int ti, const float4 &pj, int tj, bool live,
-                                         const BoxF &box, float alpha, const float2 *ldsTable,
+                                         const BoxF &box, float alpha, float alpha2, float alpha3,
+                                         const float2 *ldsTable,
                                          GlobalPtr<float2> table, const double2 *lj64, float &fx, float &fy, float &fz,
                                          double &elj, double &ecoul, double &dscale) {
   float dx = pi.x - pj.x, dy = pi.y - pj.y, dz = pi.z - pj.z;
@@ -44,16 +71,26 @@ __device__ __forceinline__ void pairTerm(const float4 &pi, float qi, int ti, con
   const bool inside = live && r2 <= box.rc2;
   if (ENERGIES && !inside) return;
   const float2 ab = LDS_TABLE ? ldsTable[ti + tj] : table[ti + tj];
-  const float invR2 = __builtin_amdgcn_rcpf(r2);
   const float invR = __builtin_amdgcn_rsqf(r2);
-  const float r = r2 * invR;
+  const float invR2 = invR * invR;
   const float invR6 = invR2 * invR2 * invR2;
   const float lj12 = ab.x * invR6;
   float scalar = (12.0f * lj12 - 6.0f * ab.y) * invR6 * invR2;
   const float qq = qi * pj.w;
-  float gaussian;
-  const float screened = erfcWithGaussian(alpha * r, gaussian);
-  scalar += qq * (screened * invR + TWO_OVER_SQRT_PI_F * alpha * gaussian) * invR2;
+  float screened = 0.f;  // erfc(alpha r), for the energy
+  float coulomb;
+  if (RATIONAL) {
+    coulomb = invR * invR2 - alpha3 * ewaldG(alpha2 * r2);
+    if (ENERGIES) {
+      float gaussian;
+      screened = erfcWithGaussian(alpha * (r2 * invR), gaussian);
+    }
+  } else {
+    float gaussian;
+    screened = erfcWithGaussian(alpha * (r2 * invR), gaussian);
+    coulomb = (screened * invR + TWO_OVER_SQRT_PI_F * alpha * gaussian) * invR2;
+  }
+  scalar += qq * coulomb;
   if (ENERGIES) {
     // FP64 at the same image: the FP32 box rows, the hardware reciprocals'
     // one-ulp bias and the FP32 table would otherwise be systematic over the
@@ -72,8 +109,9 @@ __device__ __forceinline__ void pairTerm(const float4 &pi, float qi, int ti, con
   fz += f * dz;
 }
 
-template <int LANES, bool ENERGIES, bool LDS_TABLE, bool RECT>
+template <int LANES, bool ENERGIES, bool LDS_TABLE, bool RECT, bool RATIONAL>
 __device__ __forceinline__ void nonbonded(const Dev &d, float alpha, int block, float2 *ldsTable) {
+  const float alpha2 = alpha * alpha, alpha3 = alpha2 * alpha;
   // The LJ type-pair table in LDS: one lookup per pair, off the global
   // latency chain (list index -> position and type -> table entry).
   const GlobalPtr<float2> table = globalPtr(d.lj);
@@ -124,8 +162,8 @@ __device__ __forceinline__ void nonbonded(const Dev &d, float alpha, int block, 
       }
 #pragma unroll
       for (int u = 0; u < UNROLL; ++u)
-        pairTerm<ENERGIES, LDS_TABLE, RECT>(pi, qi, ti, p[u], t[u], k + u * LANES < count, box, alpha, ldsTable, table,
-                                      d.lj64, fx, fy, fz, elj, ecoul, dscale);
+        pairTerm<ENERGIES, LDS_TABLE, RECT, RATIONAL>(pi, qi, ti, p[u], t[u], k + u * LANES < count, box, alpha, alpha2,
+                                                      alpha3, ldsTable, table, d.lj64, fx, fy, fz, elj, ecoul, dscale);
     }
   }
   // Exact group reduction in fixed point.
@@ -170,84 +208,99 @@ __device__ __forceinline__ void nonbonded(const Dev &d, float alpha, int block, 
   }
 }
 
-template <int LANES, bool ENERGIES, bool LDS_TABLE, bool RECT>
+template <int LANES, bool ENERGIES, bool LDS_TABLE, bool RECT, bool RATIONAL>
 __global__ void __launch_bounds__(256) nonbondedKernel(Dev d, float alpha) {
   extern __shared__ float2 ldsTable[];
-  nonbonded<LANES, ENERGIES, LDS_TABLE, RECT>(d, alpha, blockIdx.x, ldsTable);
+  nonbonded<LANES, ENERGIES, LDS_TABLE, RECT, RATIONAL>(d, alpha, blockIdx.x, ldsTable);
 }
 
 // Batched.
-template <int LANES, bool ENERGIES, bool LDS_TABLE, bool RECT>
+template <int LANES, bool ENERGIES, bool LDS_TABLE, bool RECT, bool RATIONAL>
 __global__ void __launch_bounds__(256) nonbondedBatchKernel(const BatchView *__restrict__ v,
                                                             const int2 *__restrict__ task, float alpha) {
   extern __shared__ float2 ldsTable[];
   const int2 t = task[blockIdx.x];
   if (t.x < 0) return;
-  nonbonded<LANES, ENERGIES, LDS_TABLE, RECT>(v[t.x].d, alpha, t.y, ldsTable);
+  nonbonded<LANES, ENERGIES, LDS_TABLE, RECT, RATIONAL>(v[t.x].d, alpha, t.y, ldsTable);
 }
 
-template <bool ENERGIES, bool LDS_TABLE, bool RECT>
+template <bool ENERGIES, bool LDS_TABLE, bool RECT, bool RATIONAL>
 static void launchLanes(const Dev &d, float alpha, int lanes, int blocks, size_t lds, hipStream_t s) {
   switch (lanes) {
-    case 1: nonbondedKernel<1, ENERGIES, LDS_TABLE, RECT><<<blocks, 256, lds, s>>>(d, alpha); break;
-    case 2: nonbondedKernel<2, ENERGIES, LDS_TABLE, RECT><<<blocks, 256, lds, s>>>(d, alpha); break;
-    case 4: nonbondedKernel<4, ENERGIES, LDS_TABLE, RECT><<<blocks, 256, lds, s>>>(d, alpha); break;
-    case 8: nonbondedKernel<8, ENERGIES, LDS_TABLE, RECT><<<blocks, 256, lds, s>>>(d, alpha); break;
-    case 16: nonbondedKernel<16, ENERGIES, LDS_TABLE, RECT><<<blocks, 256, lds, s>>>(d, alpha); break;
-    case 32: nonbondedKernel<32, ENERGIES, LDS_TABLE, RECT><<<blocks, 256, lds, s>>>(d, alpha); break;
-    default: nonbondedKernel<64, ENERGIES, LDS_TABLE, RECT><<<blocks, 256, lds, s>>>(d, alpha); break;
+    case 1: nonbondedKernel<1, ENERGIES, LDS_TABLE, RECT, RATIONAL><<<blocks, 256, lds, s>>>(d, alpha); break;
+    case 2: nonbondedKernel<2, ENERGIES, LDS_TABLE, RECT, RATIONAL><<<blocks, 256, lds, s>>>(d, alpha); break;
+    case 4: nonbondedKernel<4, ENERGIES, LDS_TABLE, RECT, RATIONAL><<<blocks, 256, lds, s>>>(d, alpha); break;
+    case 8: nonbondedKernel<8, ENERGIES, LDS_TABLE, RECT, RATIONAL><<<blocks, 256, lds, s>>>(d, alpha); break;
+    case 16: nonbondedKernel<16, ENERGIES, LDS_TABLE, RECT, RATIONAL><<<blocks, 256, lds, s>>>(d, alpha); break;
+    case 32: nonbondedKernel<32, ENERGIES, LDS_TABLE, RECT, RATIONAL><<<blocks, 256, lds, s>>>(d, alpha); break;
+    default: nonbondedKernel<64, ENERGIES, LDS_TABLE, RECT, RATIONAL><<<blocks, 256, lds, s>>>(d, alpha); break;
   }
 }
 
-template <bool ENERGIES, bool RECT>
-static void launchTable(const Dev &d, float alpha, int lanes, int blocks, hipStream_t s) {
-  if (tableInLds(d.types))
-    launchLanes<ENERGIES, true, RECT>(d, alpha, lanes, blocks, tableLdsBytes(d.types), s);
-  else
-    launchLanes<ENERGIES, false, RECT>(d, alpha, lanes, blocks, 0, s);
-}
-
-int launchNonbonded(const Dev &d, float alpha, int lanes, int energies, hipStream_t s) {
-  const long long threads = (long long)d.n * lanes;
-  const int blocks = (int)((threads + 255) / 256);
-  if (energies)
-    d.rect ? launchTable<true, true>(d, alpha, lanes, blocks, s) : launchTable<true, false>(d, alpha, lanes, blocks, s);
-  else
-    d.rect ? launchTable<false, true>(d, alpha, lanes, blocks, s)
-           : launchTable<false, false>(d, alpha, lanes, blocks, s);
-  return blocks;
-}
-
-template <bool ENERGIES, bool LDS_TABLE, bool RECT>
+template <bool ENERGIES, bool LDS_TABLE, bool RECT, bool RATIONAL>
 static void launchBatchLanes(const BatchView *v, const TaskTable &t, int lanes, float alpha, size_t lds,
                              hipStream_t s) {
   switch (lanes) {
-    case 8: nonbondedBatchKernel<8, ENERGIES, LDS_TABLE, RECT><<<t.blocks, 256, lds, s>>>(v, t.task, alpha); break;
-    case 16: nonbondedBatchKernel<16, ENERGIES, LDS_TABLE, RECT><<<t.blocks, 256, lds, s>>>(v, t.task, alpha); break;
-    case 32: nonbondedBatchKernel<32, ENERGIES, LDS_TABLE, RECT><<<t.blocks, 256, lds, s>>>(v, t.task, alpha); break;
-    default: nonbondedBatchKernel<64, ENERGIES, LDS_TABLE, RECT><<<t.blocks, 256, lds, s>>>(v, t.task, alpha); break;
+    case 8:
+      nonbondedBatchKernel<8, ENERGIES, LDS_TABLE, RECT, RATIONAL><<<t.blocks, 256, lds, s>>>(v, t.task, alpha);
+      break;
+    case 16:
+      nonbondedBatchKernel<16, ENERGIES, LDS_TABLE, RECT, RATIONAL><<<t.blocks, 256, lds, s>>>(v, t.task, alpha);
+      break;
+    case 32:
+      nonbondedBatchKernel<32, ENERGIES, LDS_TABLE, RECT, RATIONAL><<<t.blocks, 256, lds, s>>>(v, t.task, alpha);
+      break;
+    default:
+      nonbondedBatchKernel<64, ENERGIES, LDS_TABLE, RECT, RATIONAL><<<t.blocks, 256, lds, s>>>(v, t.task, alpha);
+      break;
   }
 }
 
-template <bool ENERGIES, bool RECT>
-static void launchBatchTable(const BatchView *v, const TaskTable &t, int lanes, float alpha, int types,
-                             hipStream_t s) {
-  if (tableInLds(types))
-    launchBatchLanes<ENERGIES, true, RECT>(v, t, lanes, alpha, tableLdsBytes(types), s);
+// The kernel variant of a launch: energies, LJ table in LDS, rectangular
+// box, rational Coulomb force - each a compile-time choice.
+struct Variant {
+  bool energies, lds, rect, rational;
+};
+
+template <bool E, bool L, bool R, bool Q>
+static void launchOne(bool batch, const Dev *d, const BatchView *v, const TaskTable *t, float alpha, int lanes,
+                      int blocks, size_t lds, hipStream_t s) {
+  if (batch)
+    launchBatchLanes<E, L, R, Q>(v, *t, lanes, alpha, lds, s);
   else
-    launchBatchLanes<ENERGIES, false, RECT>(v, t, lanes, alpha, 0, s);
+    launchLanes<E, L, R, Q>(*d, alpha, lanes, blocks, lds, s);
+}
+
+static void dispatch(const Variant &k, bool batch, const Dev *d, const BatchView *v, const TaskTable *t, float alpha,
+                     int lanes, int blocks, size_t lds, hipStream_t s) {
+  const int code = (k.energies ? 8 : 0) | (k.lds ? 4 : 0) | (k.rect ? 2 : 0) | (k.rational ? 1 : 0);
+  switch (code) {
+#define RMD_NB_CASE(C)                                                                                     \
+  case C:                                                                                                  \
+    launchOne<(C & 8) != 0, (C & 4) != 0, (C & 2) != 0, (C & 1) != 0>(batch, d, v, t, alpha, lanes, blocks, \
+                                                                       lds, s);                           \
+    break;
+    RMD_NB_CASE(0) RMD_NB_CASE(1) RMD_NB_CASE(2) RMD_NB_CASE(3) RMD_NB_CASE(4) RMD_NB_CASE(5) RMD_NB_CASE(6)
+    RMD_NB_CASE(7) RMD_NB_CASE(8) RMD_NB_CASE(9) RMD_NB_CASE(10) RMD_NB_CASE(11) RMD_NB_CASE(12) RMD_NB_CASE(13)
+    RMD_NB_CASE(14) RMD_NB_CASE(15)
+#undef RMD_NB_CASE
+  }
+}
+
+int launchNonbonded(const Dev &d, float alpha, int rational, int lanes, int energies, hipStream_t s) {
+  const long long threads = (long long)d.n * lanes;
+  const int blocks = (int)((threads + 255) / 256);
+  const Variant k{energies != 0, tableInLds(d.types), d.rect != 0, rational != 0};
+  dispatch(k, false, &d, nullptr, nullptr, alpha, lanes, blocks, tableLdsBytes(d.types), s);
+  return blocks;
 }
 
 // `types` is the largest type count of the batch's replicas: every workgroup
 // gets LDS for that table. `rect`: every replica's box is rectangular.
-void launchNonbondedBatch(const BatchView *v, const TaskTable &t, int lanes, float alpha, int energies, int types,
-                          int rect, hipStream_t s) {
-  if (energies)
-    rect ? launchBatchTable<true, true>(v, t, lanes, alpha, types, s)
-         : launchBatchTable<true, false>(v, t, lanes, alpha, types, s);
-  else
-    rect ? launchBatchTable<false, true>(v, t, lanes, alpha, types, s)
-         : launchBatchTable<false, false>(v, t, lanes, alpha, types, s);
+void launchNonbondedBatch(const BatchView *v, const TaskTable &t, int lanes, float alpha, int rational, int energies,
+                          int types, int rect, hipStream_t s) {
+  const Variant k{energies != 0, tableInLds(types), rect != 0, rational != 0};
+  dispatch(k, true, nullptr, v, &t, alpha, lanes, 0, tableLdsBytes(types), s);
 }
 
 }  // namespace rmd
