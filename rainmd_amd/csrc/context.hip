@@ -1080,6 +1080,16 @@ void create(rmd_ctx *c, const System &sys, const rmd_options &o, const Streams *
   p.influence = devAlloc<float>(half, c->held);
   p.piOverAlpha2 = M_PI * M_PI / (c->pme.alpha * c->pme.alpha);
   pmeConfigure(p, PME_AUTO, c->maxLds);
+  // The plane pipeline's atom bins. A single system scans every atom per
+  // plane (one launch fewer on a short critical path); replica batches bin
+  // their members (the same results: see pme.hip).
+  p.binned = 0;
+  if (shared && pmeBinsFit(p, d.n)) {
+    p.binned = 1;
+    p.binStart = devAlloc<int>(p.K[0] + 1, c->held);
+    p.atomBin = devAlloc<int2>(d.nPad, c->held);
+    p.binAtoms = devAlloc<int>(d.nPad, c->held);
+  }
   writeBox(c);
   setupIntegrator(c, 0);
   // Initial state from the system's own coordinates, if it has them.
@@ -2091,8 +2101,8 @@ void enqueueBatchForces(rmd_batch *b, int energies, int terms = RMD_TERM_ALL) {
     HIP_CHECK(hipStreamWaitEvent(st.side[1], st.fork, 0));
   }
   if (terms & RMD_TERM_RECIPROCAL)
-    launchPmeBatch(b->dViews, b->tables[BK_PLANES], b->tables[BK_XLINES], energies, b->ldsForward, b->ldsXlines,
-                   b->ldsGather, pmeStream);
+    launchPmeBatch(b->dViews, b->tables[BK_PLANES], b->tables[BK_XLINES], b->tables[BK_REPLICA], energies,
+                   b->ldsForward, b->ldsXlines, b->ldsGather, pmeStream);
   if (terms & (RMD_TERM_BONDED | RMD_TERM_EXCEPTIONS | RMD_TERM_RESTRAINT))
     launchBondedBatch(b->dViews, b->tables[BK_BONDED], b->alpha, terms, energies, bondedStream);
   launchRebuildBatch(b->dViews, b->tables[BK_REPLICA], b->tables[BK_ATOMS64], b->allRect, st.stream);

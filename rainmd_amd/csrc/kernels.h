@@ -42,11 +42,20 @@ struct PmeDev {
   int planeSpan;            // max(ceil(K1/2) * K2, K2h * K1): one plane's complex work buffer
   int planeRegion;          // float2 slots of the second buffer: max(planeSpan, the int64 plane)
   int planes;               // 1: plane pipeline, 0: line pipeline
+  // The plane pipeline's atoms by the base index of their x support (K0
+  // bins): a plane reads the ORDER bins that reach it instead of every atom.
+  int binned;               // 1: the bins are built each evaluation (replica batches)
+  int *binStart;            // K0 + 1 offsets into binAtoms
+  int2 *atomBin;            // per atom: (bin or -1 for no charge, rank in the bin)
+  int *binAtoms;            // atoms grouped by bin
 };
 enum { PME_AUTO = 0, PME_PLANES = 1, PME_LINES = 2 };
 // Chooses the pipeline (PME_AUTO: planes when a plane fits in maxLds bytes of
 // LDS). Returns false when PME_PLANES was asked for and does not fit.
 bool pmeConfigure(PmeDev &p, int mode, size_t maxLds);
+// Whether a plane-pipeline system can bin its atoms in one workgroup.
+constexpr int BIN_MAX_ATOMS = 16384;
+bool pmeBinsFit(const PmeDev &p, int atoms);
 void pmeComputeInfluence(const PmeDev &p, const BoxF *box, const double *moduli[3], double alpha, hipStream_t s);
 int launchPme(const Dev &d, const PmeDev &p, int energies, hipStream_t s, int *slotsUsed);
 // Debug entry: forward R2C 3D FFT of a real grid into the half spectrum.
@@ -195,8 +204,8 @@ void launchRebuildBatch(const BatchView *v, const TaskTable &replicas, const Tas
 void launchNonbondedBatch(const BatchView *v, const TaskTable &t, int lanes, float alpha, int rational, int energies,
                           int types, int rect, hipStream_t s);
 void launchBondedBatch(const BatchView *v, const TaskTable &t, double alpha, int terms, int energies, hipStream_t s);
-void launchPmeBatch(const BatchView *v, const TaskTable &planes, const TaskTable &xlines, int energies,
-                    size_t forwardLds, size_t xlinesLds, size_t gatherLds, hipStream_t s);
+void launchPmeBatch(const BatchView *v, const TaskTable &planes, const TaskTable &xlines, const TaskTable &replicas,
+                    int energies, size_t forwardLds, size_t xlinesLds, size_t gatherLds, hipStream_t s);
 void launchInfluenceBatch(const BatchView *v, const TaskTable &t, hipStream_t s);
 void pmeBatchLdsLimits(size_t forwardLds, size_t xlinesLds, size_t gatherLds);
 // Per-replica grid sizes and LDS of the PME plane pipeline (for the task tables).

@@ -297,6 +297,91 @@ __device__ void storeReciprocal(const Dev &d, double energy, double dscale, int 
 // Plane pipeline
 // ============================================================================
 
+// The atoms by the base index of their x support, rebuilt each evaluation by
+// one workgroup per system (at most BIN_MAX_ATOMS atoms): each charged atom
+// takes its rank in its bin by an LDS atomic count, the counts become
+// offsets, and each atom is written at its offset plus rank. The order inside
+// a bin varies from run to run; nothing depends on it (the spread
+// accumulates in fixed point, the gather is per atom), so the binned planes
+// give the scanned planes' results bit for bit.
+constexpr int BIN_THREADS = 1024;
+constexpr int BIN_MAX_K = 1024;  // grid sizes are at most 1024
+
+__device__ __forceinline__ void pmeBinBuild(const Dev &d, const PmeDev &p) {
+  __shared__ int start[BIN_MAX_K];
+  __shared__ int waveTotal[BIN_THREADS / 64];
+  const int K0 = p.K[0], t = threadIdx.x, nt = blockDim.x;
+  for (int b = t; b < K0; b += nt) start[b] = 0;
+  __syncthreads();
+  const double *recip = d.box->recip64;
+  for (int a = t; a < d.n; a += nt) {
+    const float4 pos = d.pos[a];
+    int2 e = make_int2(-1, 0);
+    if (pos.w != 0.f) {
+      int bx;
+      float ox;
+      gridCoordinate(p, recip, 0, pos, bx, ox);
+      e = make_int2(bx, atomicAdd(&start[bx], 1));
+    }
+    p.atomBin[a] = e;
+  }
+  __syncthreads();
+  // exclusive prefix over the counts, in place: a few bins per thread, then
+  // the waves in order
+  const int per = (K0 + nt - 1) / nt;
+  const int lo = t * per, hi = min(K0, lo + per);
+  int sum = 0;
+  for (int b = lo; b < hi; ++b) sum += start[b];
+  const int lane = t & 63, wave = t >> 6;
+  int inclusive = sum;
+  for (int offset = 1; offset < 64; offset <<= 1) {
+    const int v = __shfl_up(inclusive, offset, 64);
+    if (lane >= offset) inclusive += v;
+  }
+  if (lane == 63) waveTotal[wave] = inclusive;
+  __syncthreads();
+  int before = 0;
+  for (int w = 0; w < wave; ++w) before += waveTotal[w];
+  int offset = before + inclusive - sum;
+  for (int b = lo; b < hi; ++b) {
+    const int c = start[b];
+    start[b] = offset;
+    p.binStart[b] = offset;
+    offset += c;
+  }
+  if (t == nt - 1) p.binStart[K0] = before + inclusive;
+  __syncthreads();
+  for (int a = t; a < d.n; a += nt) {
+    const int2 e = p.atomBin[a];
+    if (e.x >= 0) p.binAtoms[start[e.x] + e.y] = a;
+  }
+}
+
+__global__ void __launch_bounds__(BIN_THREADS) pmeBinKernel(Dev d, PmeDev p) { pmeBinBuild(d, p); }
+__global__ void __launch_bounds__(BIN_THREADS) pmeBinBatchKernel(const BatchView *__restrict__ v,
+                                                                 const int2 *__restrict__ task) {
+  const int2 t = task[blockIdx.x];
+  if (t.x < 0 || !v[t.x].p.binned) return;
+  pmeBinBuild(v[t.x].d, v[t.x].p);
+}
+
+bool pmeBinsFit(const PmeDev &p, int atoms) {
+  return p.planes && p.K[0] >= ORDER && p.K[0] <= BIN_MAX_K && atoms <= BIN_MAX_ATOMS;
+}
+
+// The binned atoms that reach plane i: bins i .. i + ORDER - 1 (mod K0), one
+// contiguous range of binAtoms or two where the bins wrap.
+__device__ __forceinline__ void planeRanges(const PmeDev &p, int i, int2 range[2]) {
+  const int K0 = p.K[0], last = i + ORDER - 1;
+  if (last < K0) {
+    range[0] = make_int2(p.binStart[i], p.binStart[last + 1]);
+    range[1] = make_int2(0, 0);
+  } else {
+    range[0] = make_int2(p.binStart[i], p.binStart[K0]);
+    range[1] = make_int2(p.binStart[0], p.binStart[last - K0 + 1]);
+  }
+}
+
 // One round of a plane's atom scan: of atoms [a0, a0 + PLANE_LIST), those
 // whose x support meets `plane` are appended to `list` as (atom << 3 | weight
 // index), wave-aggregated into the shared `count`. The expensive per-atom work
@@ -358,35 +443,51 @@ __device__ __forceinline__ void pmePlaneForward(const Dev &d, const PmeDev &p, c
   } else {
     for (int idx = t; idx < K1 * K2; idx += nt) plane[idx] = 0;
     const double *recip = d.box->recip64;
-    for (int a0 = 0; a0 < d.n; a0 += PLANE_LIST) {
-      if (t == 0) *listCount = 0;
-      __syncthreads();
-      planeScan(d, p, i, a0, list, listCount);
-      __syncthreads();
-      const int count = *listCount;
-      for (int k = t; k < count; k += nt) {
-        const int a = list[k] >> 3, r = list[k] & 7;
-        const float4 pos = d.pos[a];
-        int bx, by, bz;
-        float ox, oy, oz;
-        gridCoordinate(p, recip, 0, pos, bx, ox);
-        float w[ORDER], wy[ORDER], wz[ORDER], dw[ORDER];
-        bspline5(ox, w, dw);
-        float wx = 0.f;
-        for (int q = r; q < ORDER; q += K0) wx += w[q];
-        gridCoordinate(p, recip, 1, pos, by, oy);
-        gridCoordinate(p, recip, 2, pos, bz, oz);
-        bspline5(oy, wy, dw);
-        bspline5(oz, wz, dw);
-        int iz[ORDER];
-        for (int c = 0; c < ORDER; ++c) iz[c] = gridIndex(bz, c, K2);
-        for (int b = 0; b < ORDER; ++b) {
-          unsigned long long *row = reinterpret_cast<unsigned long long *>(plane + gridIndex(by, b, K1) * K2);
-          for (int c = 0; c < ORDER; ++c)
-            atomicAdd(row + iz[c], (unsigned long long)chargeToFixed(pos.w, wx, wy[b], wz[c]));
-        }
+    // atom a's charge onto this plane; r: the first weight index that lands
+    // on the plane, or -1 to derive it from a binned atom's base index (in
+    // i .. i + ORDER - 1 mod K0)
+    auto spread = [&](int a, int r) {
+      const float4 pos = d.pos[a];
+      int bx, by, bz;
+      float ox, oy, oz;
+      gridCoordinate(p, recip, 0, pos, bx, ox);
+      if (r < 0) {
+        r = i - bx + ORDER - 1;
+        r += r < 0 ? K0 : 0;
+        r -= r >= K0 ? K0 : 0;
       }
-      __syncthreads();
+      float w[ORDER], wy[ORDER], wz[ORDER], dw[ORDER];
+      bspline5(ox, w, dw);
+      float wx = 0.f;
+      for (int q = r; q < ORDER; q += K0) wx += w[q];
+      gridCoordinate(p, recip, 1, pos, by, oy);
+      gridCoordinate(p, recip, 2, pos, bz, oz);
+      bspline5(oy, wy, dw);
+      bspline5(oz, wz, dw);
+      int iz[ORDER];
+      for (int c = 0; c < ORDER; ++c) iz[c] = gridIndex(bz, c, K2);
+      for (int b = 0; b < ORDER; ++b) {
+        unsigned long long *row = reinterpret_cast<unsigned long long *>(plane + gridIndex(by, b, K1) * K2);
+        for (int c = 0; c < ORDER; ++c)
+          atomicAdd(row + iz[c], (unsigned long long)chargeToFixed(pos.w, wx, wy[b], wz[c]));
+      }
+    };
+    __syncthreads();
+    if (p.binned) {
+      int2 range[2];
+      planeRanges(p, i, range);
+      for (int h = 0; h < 2; ++h)
+        for (int k = range[h].x + t; k < range[h].y; k += nt) spread(p.binAtoms[k], -1);
+    } else {
+      for (int a0 = 0; a0 < d.n; a0 += PLANE_LIST) {
+        if (t == 0) *listCount = 0;
+        __syncthreads();
+        planeScan(d, p, i, a0, list, listCount);
+        __syncthreads();
+        const int count = *listCount;
+        for (int k = t; k < count; k += nt) spread(list[k] >> 3, list[k] & 7);
+        __syncthreads();
+      }
     }
     __syncthreads();
     // Two real z lines per complex line.
@@ -889,8 +990,10 @@ size_t pmePlaneGatherLds(const PmeDev &p) { return planeGatherLds(p); }
 
 // Batched plane pipeline; the LDS sizes are the largest over the replicas and
 // the kernels' LDS limits were raised to them at batch creation.
-void launchPmeBatch(const BatchView *v, const TaskTable &planes, const TaskTable &xlines, int energies,
-                    size_t forwardLds, size_t xlinesLds, size_t gatherLds, hipStream_t s) {
+void launchPmeBatch(const BatchView *v, const TaskTable &planes, const TaskTable &xlines, const TaskTable &replicas,
+                    int energies, size_t forwardLds, size_t xlinesLds, size_t gatherLds,
+                    hipStream_t s) {
+  pmeBinBatchKernel<<<replicas.blocks, BIN_THREADS, 0, s>>>(v, replicas.task);
   pmePlaneForwardBatchKernel<<<planes.blocks, PLANE_THREADS, forwardLds, s>>>(v, planes.task);
   pmeXLinesBatchKernel<<<xlines.blocks, XLINE_THREADS, xlinesLds, s>>>(v, xlines.task, energies);
   pmePlaneGatherBatchKernel<<<planes.blocks, PLANE_THREADS, gatherLds, s>>>(v, planes.task);
@@ -911,6 +1014,7 @@ void pmeBatchLdsLimits(size_t forwardLds, size_t xlinesLds, size_t gatherLds) {
 
 int launchPme(const Dev &d, const PmeDev &p, int energies, hipStream_t s, int *slotsUsed) {
   if (p.planes) {
+    if (p.binned) pmeBinKernel<<<1, BIN_THREADS, 0, s>>>(d, p);
     pmePlaneForwardKernel<<<p.K[0], PLANE_THREADS, planeForwardLds(p), s>>>(d, p, nullptr);
     const int xBlocks = xLineBlocks(p);
     pmeXLinesKernel<<<xBlocks, XLINE_THREADS, xLinesLds(p), s>>>(d, p, 1, energies);
