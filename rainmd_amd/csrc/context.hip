@@ -2101,8 +2101,8 @@ void enqueueBatchForces(rmd_batch *b, int energies, int terms = RMD_TERM_ALL) {
     HIP_CHECK(hipStreamWaitEvent(st.side[1], st.fork, 0));
   }
   if (terms & RMD_TERM_RECIPROCAL)
-    launchPmeBatch(b->dViews, b->tables[BK_PLANES], b->tables[BK_XLINES], b->tables[BK_REPLICA], energies,
-                   b->ldsForward, b->ldsXlines, b->ldsGather, pmeStream);
+    launchPmeBatch(b->dViews, b->tables[BK_PLANES], b->tables[BK_XLINES], b->tables[BK_REPLICA],
+                   b->tables[BK_ATOMS256], energies, b->ldsForward, b->ldsXlines, b->ldsGather, pmeStream);
   if (terms & (RMD_TERM_BONDED | RMD_TERM_EXCEPTIONS | RMD_TERM_RESTRAINT))
     launchBondedBatch(b->dViews, b->tables[BK_BONDED], b->alpha, terms, energies, bondedStream);
   launchRebuildBatch(b->dViews, b->tables[BK_REPLICA], b->tables[BK_ATOMS64], b->allRect, st.stream);

@@ -205,7 +205,8 @@ void launchNonbondedBatch(const BatchView *v, const TaskTable &t, int lanes, flo
                           int types, int rect, hipStream_t s);
 void launchBondedBatch(const BatchView *v, const TaskTable &t, double alpha, int terms, int energies, hipStream_t s);
 void launchPmeBatch(const BatchView *v, const TaskTable &planes, const TaskTable &xlines, const TaskTable &replicas,
-                    int energies, size_t forwardLds, size_t xlinesLds, size_t gatherLds, hipStream_t s);
+                    const TaskTable &atoms256, int energies, size_t forwardLds, size_t xlinesLds, size_t gatherLds,
+                    hipStream_t s);
 void launchInfluenceBatch(const BatchView *v, const TaskTable &t, hipStream_t s);
 void pmeBatchLdsLimits(size_t forwardLds, size_t xlinesLds, size_t gatherLds);
 // Per-replica grid sizes and LDS of the PME plane pipeline (for the task tables).

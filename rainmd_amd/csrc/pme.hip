@@ -248,16 +248,26 @@ __device__ __forceinline__ long long chargeToFixed(float q, float wx, float wy, 
 
 // Force on one atom from the derivatives of its fractional-coordinate
 // gradient (gu, gv, gw per grid unit), chained through the reciprocal rows.
-__device__ __forceinline__ void addGridForce(const Dev &d, const PmeDev &p, int atom, double q, float gu, float gv,
-                                             float gw) {
+// The same force in fixed point, without adding it.
+__device__ __forceinline__ void gridForceFixed(const Dev &d, const PmeDev &p, double q, float gu, float gv, float gw,
+                                               long long f[3]) {
   const double u = (double)gu * p.K[0] * q, v = (double)gv * p.K[1] * q, ww = (double)gw * p.K[2] * q;
   const double *r = d.box->recip64;
   const double fx = -(u * r[0] + v * r[1] + ww * r[2]);
   const double fy = -(u * r[3] + v * r[4] + ww * r[5]);
   const double fz = -(u * r[6] + v * r[7] + ww * r[8]);
-  atomicAddFixed(&d.force[atom], __double2ll_rn(fx * FORCE_SCALE));
-  atomicAddFixed(&d.force[d.nPad + atom], __double2ll_rn(fy * FORCE_SCALE));
-  atomicAddFixed(&d.force[2 * d.nPad + atom], __double2ll_rn(fz * FORCE_SCALE));
+  f[0] = __double2ll_rn(fx * FORCE_SCALE);
+  f[1] = __double2ll_rn(fy * FORCE_SCALE);
+  f[2] = __double2ll_rn(fz * FORCE_SCALE);
+}
+
+__device__ __forceinline__ void addGridForce(const Dev &d, const PmeDev &p, int atom, double q, float gu, float gv,
+                                             float gw) {
+  long long f[3];
+  gridForceFixed(d, p, q, gu, gv, gw, f);
+  atomicAddFixed(&d.force[atom], f[0]);
+  atomicAddFixed(&d.force[d.nPad + atom], f[1]);
+  atomicAddFixed(&d.force[2 * d.nPad + atom], f[2]);
 }
 
 // |m|^2 of the wave vector of half-spectrum point (x, y, m), reciprocal rows
@@ -640,6 +650,13 @@ __device__ __forceinline__ void pmePlaneGather(const Dev &d, const PmeDev &p, in
     if (2 * pr + 1 < K1) phi[(2 * pr + 1) * K2 + k] = Z[idx].y;
   }
   __syncthreads();
+  if (p.binned) {
+    // a binned system gathers per atom from the potential grid
+    // (pmeAtomGather)
+    float *out = p.rgrid + (size_t)i * K1 * K2;
+    for (int idx = t; idx < K1 * K2; idx += nt) out[idx] = phi[idx];
+    return;
+  }
   const double *recip = d.box->recip64;
   for (int a0 = 0; a0 < d.n; a0 += PLANE_LIST) {
     if (t == 0) *listCount = 0;
@@ -696,6 +713,72 @@ __global__ void __launch_bounds__(PLANE_THREADS) pmePlaneGatherBatchKernel(const
   const int2 t = task[blockIdx.x];
   if (t.x < 0) return;
   pmePlaneGather(v[t.x].d, v[t.x].p, t.y, lds);
+}
+
+// The gather of a binned system, one thread per charged atom in bin order (a
+// wave's atoms share an x slab of the grid; the cell order is not used: the
+// list rebuild may be rewriting it meanwhile on another stream): each of the
+// atom's ORDER planes' shares is computed from the potential grid with the
+// plane gather's own arithmetic and rounded to fixed point as it would be
+// there, and the shares are summed in integers - one set of force atomics per
+// atom instead of one per plane, and the same force bit for bit (the grid has
+// at least ORDER planes, so each weight lands on a plane of its own).
+__device__ __forceinline__ void pmeAtomGather(const Dev &d, const PmeDev &p, int k) {
+  if (k >= p.binStart[p.K[0]]) return;
+  const int a = p.binAtoms[k];
+  const float4 pos = d.pos[a];
+  const int K0 = p.K[0], K1 = p.K[1], K2 = p.K[2];
+  const double *recip = d.box->recip64;
+  int bx, by, bz;
+  float ox, oy, oz;
+  gridCoordinate(p, recip, 0, pos, bx, ox);
+  float w[ORDER], dw[ORDER], wy[ORDER], dwy[ORDER], wz[ORDER], dwz[ORDER];
+  bspline5(ox, w, dw);
+  gridCoordinate(p, recip, 1, pos, by, oy);
+  gridCoordinate(p, recip, 2, pos, bz, oz);
+  bspline5(oy, wy, dwy);
+  bspline5(oz, wz, dwz);
+  int iz[ORDER], iy[ORDER];
+  for (int c = 0; c < ORDER; ++c) {
+    iz[c] = gridIndex(bz, c, K2);
+    iy[c] = gridIndex(by, c, K1);
+  }
+  const GlobalPtr<float> grid = globalPtr(p.rgrid);
+  long long total[3] = {0, 0, 0};
+  for (int r = 0; r < ORDER; ++r) {
+    const size_t plane = (size_t)gridIndex(bx, r, K0) * K1;
+    const float wx = 0.f + w[r], dwx = 0.f + dw[r];
+    float s0 = 0.f, s1 = 0.f, s2 = 0.f;
+    for (int b = 0; b < ORDER; ++b) {
+      const GlobalPtr<float> row = grid + (plane + iy[b]) * K2;
+      float r0 = 0.f, r2 = 0.f;
+      for (int c = 0; c < ORDER; ++c) {
+        const float v = row[iz[c]];
+        r0 += wz[c] * v;
+        r2 += dwz[c] * v;
+      }
+      s0 += wy[b] * r0;
+      s1 += dwy[b] * r0;
+      s2 += wy[b] * r2;
+    }
+    long long f[3];
+    gridForceFixed(d, p, pos.w, dwx * s0, wx * s1, wx * s2, f);
+    for (int k = 0; k < 3; ++k) total[k] += f[k];
+  }
+  atomicAddFixed(&d.force[a], total[0]);
+  atomicAddFixed(&d.force[d.nPad + a], total[1]);
+  atomicAddFixed(&d.force[2 * d.nPad + a], total[2]);
+}
+
+__global__ void __launch_bounds__(256) pmeAtomGatherKernel(Dev d, PmeDev p) {
+  pmeAtomGather(d, p, blockIdx.x * blockDim.x + threadIdx.x);
+}
+
+__global__ void __launch_bounds__(256) pmeAtomGatherBatchKernel(const BatchView *__restrict__ v,
+                                                                const int2 *__restrict__ task) {
+  const int2 t = task[blockIdx.x];
+  if (t.x < 0 || !v[t.x].p.binned) return;
+  pmeAtomGather(v[t.x].d, v[t.x].p, t.y * blockDim.x + threadIdx.x);
 }
 
 // ============================================================================
@@ -991,12 +1074,13 @@ size_t pmePlaneGatherLds(const PmeDev &p) { return planeGatherLds(p); }
 // Batched plane pipeline; the LDS sizes are the largest over the replicas and
 // the kernels' LDS limits were raised to them at batch creation.
 void launchPmeBatch(const BatchView *v, const TaskTable &planes, const TaskTable &xlines, const TaskTable &replicas,
-                    int energies, size_t forwardLds, size_t xlinesLds, size_t gatherLds,
+                    const TaskTable &atoms256, int energies, size_t forwardLds, size_t xlinesLds, size_t gatherLds,
                     hipStream_t s) {
   pmeBinBatchKernel<<<replicas.blocks, BIN_THREADS, 0, s>>>(v, replicas.task);
   pmePlaneForwardBatchKernel<<<planes.blocks, PLANE_THREADS, forwardLds, s>>>(v, planes.task);
   pmeXLinesBatchKernel<<<xlines.blocks, XLINE_THREADS, xlinesLds, s>>>(v, xlines.task, energies);
   pmePlaneGatherBatchKernel<<<planes.blocks, PLANE_THREADS, gatherLds, s>>>(v, planes.task);
+  pmeAtomGatherBatchKernel<<<atoms256.blocks, 256, 0, s>>>(v, atoms256.task);
 }
 
 void launchInfluenceBatch(const BatchView *v, const TaskTable &t, hipStream_t s) {
@@ -1019,6 +1103,7 @@ int launchPme(const Dev &d, const PmeDev &p, int energies, hipStream_t s, int *s
     const int xBlocks = xLineBlocks(p);
     pmeXLinesKernel<<<xBlocks, XLINE_THREADS, xLinesLds(p), s>>>(d, p, 1, energies);
     pmePlaneGatherKernel<<<p.K[0], PLANE_THREADS, planeGatherLds(p), s>>>(d, p);
+    if (p.binned) pmeAtomGatherKernel<<<(d.n + 255) / 256, 256, 0, s>>>(d, p);
     if (slotsUsed) *slotsUsed = xBlocks;
     return xBlocks;
   }
