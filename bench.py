@@ -284,7 +284,7 @@ def bench_replicas(args, count):
         "batch_speedup": aggregate / single_aggregate if single_ms else None,
         "barostat": {"attempts": moves, "accepted": accepted},
         "roofline": {
-            "kernel": "batched nonbonded (LJ + erfc real space, every replica in one launch)",
+            "kernel": "batched nonbonded (LJ + screened Coulomb real space, every replica in one launch)",
             "bound": "mfma",
             "achieved": achieved,
             "peak": FP32_PEAK_TFLOPS,
@@ -479,7 +479,7 @@ def main():
             "graphs": not args.no_graphs,
         },
         "roofline": {
-            "kernel": "nonbonded (LJ + erfc real space)",
+            "kernel": "nonbonded (LJ + screened Coulomb real space)",
             "bound": "mfma",
             "achieved": achieved,
             "peak": FP32_PEAK_TFLOPS,
