@@ -455,6 +455,41 @@ __device__ __forceinline__ void neighboursByCell(const Dev &d, int block) {
       za0 = lo, zb0 = hi;
     }
   }
+  // Rectangular boxes: each stencil z run is trimmed at both ends to the
+  // cells within reach of the box around this cell's atoms (wrapped into the
+  // cell frame), by the periodic gap between the box and each cell's extent,
+  // with a margin far above the FP32 rounding of either. A trimmed cell holds
+  // no candidate the pair test could accept, so the lists are unchanged.
+  // per axis: the box's centre and half width (uniform), the cell width
+  float centre[3] = {0.f, 0.f, 0.f}, half[3] = {0.f, 0.f, 0.f};
+  const float limit = sqrtf(box.rl2) + 1e-4f, limit2 = limit * limit;
+  const float edge[3] = {box.a[0], box.b[1], box.c[2]};
+  if (RECT) {
+    for (int dd = 0; dd < 3; ++dd) {
+      float lo = __builtin_inff(), hi = -__builtin_inff();
+      for (int k = cellBegin + lane; k < cellEnd; k += 64) {
+        const float4 q = cellPos[k];
+        const float x = dd == 0 ? q.x : dd == 1 ? q.y : q.z;
+        const float w = x - edge[dd] * floorf(x * box.inv[dd]);
+        lo = fminf(lo, w);
+        hi = fmaxf(hi, w);
+      }
+      for (int offset = 32; offset > 0; offset >>= 1) {
+        lo = fminf(lo, __shfl_xor(lo, offset, 64));
+        hi = fmaxf(hi, __shfl_xor(hi, offset, 64));
+      }
+      centre[dd] = uniformFloat(0.5f * (lo + hi));
+      half[dd] = uniformFloat(0.5f * (hi - lo));
+    }
+  }
+  // the periodic gap along axis dd between the atoms' box and cell index c:
+  // the centres' distance folded into [0, L/2], less both half widths
+  auto gap = [&](int dd, int c) {
+    const float width = edge[dd] / (float)cells[dd];
+    float dist = fabsf((c + 0.5f) * width - centre[dd]);
+    dist = fminf(dist, edge[dd] - dist);
+    return fmaxf(0.f, dist - half[dd] - 0.5f * width);
+  };
   const int ranges = nx * ny * segs;
   int *rb = rangeBegin[wave], *rs = rangeStart[wave];
   int len[2] = {0, 0};
@@ -465,8 +500,26 @@ __device__ __forceinline__ void neighboursByCell(const Dev &d, int block) {
       const int cx = wrapCell(c3[0] + bx + row / ny, cells[0]);
       const int cy = wrapCell(c3[1] + by + row % ny, cells[1]);
       const int rowBase = (cx * cells[1] + cy) * cells[2];
-      const int begin = cellStart[rowBase + (seg ? za1 : za0)];
-      const int end = cellStart[rowBase + (seg ? zb1 : zb0)];
+      int zFirst = seg ? za1 : za0, zEnd = seg ? zb1 : zb0;
+      if (RECT) {
+        const float gx = gap(0, cx), gy = gap(1, cy), gxy = gx * gx + gy * gy;
+        if (gxy > limit2) {
+          zEnd = zFirst;
+        } else {
+          while (zFirst < zEnd) {
+            const float gz = gap(2, zFirst);
+            if (gxy + gz * gz <= limit2) break;
+            ++zFirst;
+          }
+          while (zEnd > zFirst) {
+            const float gz = gap(2, zEnd - 1);
+            if (gxy + gz * gz <= limit2) break;
+            --zEnd;
+          }
+        }
+      }
+      const int begin = cellStart[rowBase + zFirst];
+      const int end = cellStart[rowBase + zEnd];
       rb[r] = begin;
       len[h] = end - begin;
     }
@@ -577,8 +630,10 @@ __global__ void __launch_bounds__(256) neighboursKernel(Dev d) {
 // with one scalar load each instead of the dispatch of one workgroup per task.
 // The stride is a multiple of the XCD count, so a workgroup stays on the
 // replicas of its own XCD.
+// (Seven waves per SIMD: the register count the search body needs, without
+// spills.)
 template <bool RECT>
-__global__ void __launch_bounds__(256) neighboursBatchKernel(const BatchView *__restrict__ v,
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(7))) neighboursBatchKernel(const BatchView *__restrict__ v,
                                                              const int2 *__restrict__ task, int tasks) {
   for (int k = blockIdx.x; k < tasks; k += gridDim.x) {
     const int2 t = task[k];
