@@ -129,13 +129,11 @@ __device__ double torsionTerm(const Dev &d, int t, double &vir) {
 // FP64: a solvated system has one such pair per atom, and a rigid water's
 // corrections are the same large number (~150 kJ/mol) for every water, so an
 // FP32 rounding bias would add up coherently across the box.
-__device__ void exceptionTerm(const Dev &d, const BoxF &box, double alpha, int t, double &elj, double &ecoul,
-                              double &vir) {
-  const int2 ij = d.excAtoms[t];
-  const double4 prm = d.excParams[t];  // qq scaled, A, B, qq full (both with 1/4 pi eps0)
-  double xi, yi, zi, xj, yj, zj;
-  loadPos(d, ij.x, xi, yi, zi);
-  loadPos(d, ij.y, xj, yj, zj);
+// One exception pair at the given positions: its energies and virial added
+// in, and its force on atom i (the negative on j) returned.
+__device__ __forceinline__ void exceptionPair(const BoxF &box, double alpha, double4 prm, double xi, double yi,
+                                              double zi, double xj, double yj, double zj, double &elj,
+                                              double &ecoul, double &vir, double &fx, double &fy, double &fz) {
   double dx = xi - xj, dy = yi - yj, dz = zi - zj;
   if (box.periodic) {
     const double *bx = box.box64;
@@ -167,9 +165,64 @@ __device__ void exceptionTerm(const Dev &d, const BoxF &box, double alpha, int t
     ecoul -= prm.w * unscreened * invR;
     scalar -= prm.w * (unscreened * invR - gaussian) * invR2;
   }
-  addForce(d, ij.x, scalar * dx, scalar * dy, scalar * dz);
-  addForce(d, ij.y, -scalar * dx, -scalar * dy, -scalar * dz);
+  fx = scalar * dx;
+  fy = scalar * dy;
+  fz = scalar * dz;
   vir += scalar * r2;
+}
+
+__device__ void exceptionTerm(const Dev &d, const BoxF &box, double alpha, int t, double &elj, double &ecoul,
+                              double &vir) {
+  const int2 ij = d.excAtoms[t];
+  const double4 prm = d.excParams[t];  // qq scaled, A, B, qq full (both with 1/4 pi eps0)
+  double xi, yi, zi, xj, yj, zj;
+  loadPos(d, ij.x, xi, yi, zi);
+  loadPos(d, ij.y, xj, yj, zj);
+  double fx, fy, fz;
+  exceptionPair(box, alpha, prm, xi, yi, zi, xj, yj, zj, elj, ecoul, vir, fx, fy, fz);
+  addForce(d, ij.x, fx, fy, fz);
+  addForce(d, ij.y, -fx, -fy, -fz);
+}
+
+// The three exception pairs of a three-atom molecule (a rigid water), at
+// 3k .. 3k + 2: the atoms' positions loaded once, and each pair's force
+// rounded to fixed point as exceptionTerm rounds it, the three atoms' sums
+// added once - nine atomics for the molecule instead of eighteen, the same
+// forces bit for bit.
+__device__ void exceptionTriplet(const Dev &d, const BoxF &box, double alpha, int k, double &elj, double &ecoul,
+                                 double &vir) {
+  int2 e[3];
+  double4 prm[3];
+  for (int m = 0; m < 3; ++m) {
+    e[m] = d.excAtoms[3 * k + m];
+    prm[m] = d.excParams[3 * k + m];
+  }
+  const int atom[3] = {e[0].x, e[0].y, (e[1].x != e[0].x && e[1].x != e[0].y) ? e[1].x : e[1].y};
+  double px[3], py[3], pz[3];
+  for (int q = 0; q < 3; ++q) loadPos(d, atom[q], px[q], py[q], pz[q]);
+  long long f[3][3] = {{0, 0, 0}, {0, 0, 0}, {0, 0, 0}};
+  for (int m = 0; m < 3; ++m) {
+    const int qi = e[m].x == atom[0] ? 0 : e[m].x == atom[1] ? 1 : 2;
+    const int qj = e[m].y == atom[0] ? 0 : e[m].y == atom[1] ? 1 : 2;
+    // selects, not indexing: the arrays stay in registers
+    const double xi = qi == 0 ? px[0] : qi == 1 ? px[1] : px[2];
+    const double yi = qi == 0 ? py[0] : qi == 1 ? py[1] : py[2];
+    const double zi = qi == 0 ? pz[0] : qi == 1 ? pz[1] : pz[2];
+    const double xj = qj == 0 ? px[0] : qj == 1 ? px[1] : px[2];
+    const double yj = qj == 0 ? py[0] : qj == 1 ? py[1] : py[2];
+    const double zj = qj == 0 ? pz[0] : qj == 1 ? pz[1] : pz[2];
+    double fx, fy, fz;
+    exceptionPair(box, alpha, prm[m], xi, yi, zi, xj, yj, zj, elj, ecoul, vir, fx, fy, fz);
+    const long long g[3] = {__double2ll_rn(fx * FORCE_SCALE), __double2ll_rn(fy * FORCE_SCALE),
+                            __double2ll_rn(fz * FORCE_SCALE)};
+    for (int q = 0; q < 3; ++q)
+      for (int c = 0; c < 3; ++c) f[q][c] += (qi == q ? g[c] : 0) - (qj == q ? g[c] : 0);
+  }
+  for (int q = 0; q < 3; ++q) {
+    atomicAddFixed(&d.force[atom[q]], f[q][0]);
+    atomicAddFixed(&d.force[d.nPad + atom[q]], f[q][1]);
+    atomicAddFixed(&d.force[2 * d.nPad + atom[q]], f[q][2]);
+  }
 }
 
 // Positional restraint k |minImage(r - r0)|^2, no half
@@ -200,6 +253,9 @@ __device__ double restraintTerm(const Dev &d, const BoxF &box, int t) {
 
 enum { BT_BOND = 0, BT_ANGLE, BT_TORSION, BT_LJX, BT_COULX, BT_RESTRAINT, BT_DSCALE, BT_COUNT };
 
+// One thread per exception triplet, then one per remaining exception.
+__host__ __device__ inline int exceptionThreads(const Dev &d) { return d.nException - 2 * d.nExcTriplets; }
+
 __device__ __forceinline__ void bonded(const Dev &d, double alphaD, int terms, int energies, int block) {
   const BoxF &box = *d.box;
   const int t = block * blockDim.x + threadIdx.x;
@@ -213,14 +269,18 @@ __device__ __forceinline__ void bonded(const Dev &d, double alphaD, int terms, i
     if (doBonded) e[BT_ANGLE] = angleTerm(d, t - base, vir);
   } else if ((base += d.nAngle, t < base + d.nTorsion)) {
     if (doBonded) e[BT_TORSION] = torsionTerm(d, t - base, vir);
-  } else if ((base += d.nTorsion, t < base + d.nException)) {
+  } else if ((base += d.nTorsion, t < base + exceptionThreads(d))) {
     if (terms & RMD_TERM_EXCEPTIONS) {
       double elj = 0.0, ecoul = 0.0;
-      exceptionTerm(d, box, alphaD, t - base, elj, ecoul, vir);
+      const int u = t - base;
+      if (u < d.nExcTriplets)
+        exceptionTriplet(d, box, alphaD, u, elj, ecoul, vir);
+      else
+        exceptionTerm(d, box, alphaD, 2 * d.nExcTriplets + u, elj, ecoul, vir);
       e[BT_LJX] = elj;
       e[BT_COULX] = ecoul;
     }
-  } else if ((base += d.nException, t < base + d.nRestraint)) {
+  } else if ((base += exceptionThreads(d), t < base + d.nRestraint)) {
     if (terms & RMD_TERM_RESTRAINT) e[BT_RESTRAINT] = restraintTerm(d, box, t - base);
   }
   if (!energies) return;
@@ -258,7 +318,7 @@ __global__ void __launch_bounds__(256) bondedBatchKernel(const BatchView *__rest
 }
 
 int bondedBlocks(const Dev &d) {
-  return (d.nBond + d.nAngle + d.nTorsion + d.nException + d.nRestraint + 255) / 256;
+  return (d.nBond + d.nAngle + d.nTorsion + exceptionThreads(d) + d.nRestraint + 255) / 256;
 }
 
 void launchBondedBatch(const BatchView *v, const TaskTable &t, double alpha, int terms, int energies, hipStream_t s) {

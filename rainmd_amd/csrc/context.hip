@@ -9,6 +9,7 @@
 #include <cstring>
 #include <functional>
 #include <map>
+#include <unordered_map>
 #include <numeric>
 #include <string>
 #include <vector>
@@ -978,12 +979,44 @@ void create(rmd_ctx *c, const System &sys, const rmd_options &o, const Streams *
     d.torsionParams = devAlloc<double4>(tp.size(), c->held);
     upload(d.torsionAtoms, ta, c->stream);
     upload(d.torsionParams, tp, c->stream);
+    // The exceptions of each three-atom molecule whose three pairs are all
+    // exceptions (a rigid water) come first, three to a molecule: the
+    // bonded kernel takes them together. The rest keep their order.
+    std::vector<int> order;
+    order.reserve(d.nException);
+    {
+      std::unordered_map<long long, int> pairIndex;
+      const long long n = t.atomCount;
+      auto key = [n](int i, int j) { return (long long)std::min(i, j) * n + std::max(i, j); };
+      for (int k = 0; k < d.nException; ++k) pairIndex.emplace(key(sys.exceptions.a[k], sys.exceptions.b[k]), k);
+      std::vector<char> taken(d.nException, 0);
+      d.nExcTriplets = 0;
+      for (size_t m = 0; m + 1 < sys.moleculeStart.size(); ++m) {
+        if (sys.moleculeStart[m + 1] - sys.moleculeStart[m] != 3) continue;
+        const int *at = &sys.moleculeAtoms[sys.moleculeStart[m]];
+        const long long keys[3] = {key(at[0], at[1]), key(at[0], at[2]), key(at[1], at[2])};
+        int found[3], count = 0;
+        for (int q = 0; q < 3; ++q) {
+          const auto it = pairIndex.find(keys[q]);
+          if (it != pairIndex.end() && !taken[it->second]) found[count++] = it->second;
+        }
+        if (count != 3) continue;
+        for (int q = 0; q < 3; ++q) {
+          taken[found[q]] = 1;
+          order.push_back(found[q]);
+        }
+        ++d.nExcTriplets;
+      }
+      for (int k = 0; k < d.nException; ++k)
+        if (!taken[k]) order.push_back(k);
+    }
     std::vector<int2> ea(d.nException);
     std::vector<double4> ep(d.nException);
-    for (int k = 0; k < d.nException; ++k) {
+    for (int q = 0; q < d.nException; ++q) {
+      const int k = order[q];
       const int i = sys.exceptions.a[k], j = sys.exceptions.b[k];
-      ea[k] = make_int2(i, j);
-      ep[k] = make_double4(ONE_4PI_EPS0 * sys.exceptions.chargeProduct[k], sys.exceptions.ljA[k],
+      ea[q] = make_int2(i, j);
+      ep[q] = make_double4(ONE_4PI_EPS0 * sys.exceptions.chargeProduct[k], sys.exceptions.ljA[k],
                            sys.exceptions.ljB[k], ONE_4PI_EPS0 * t.charges[i] * t.charges[j]);
     }
     d.excAtoms = devAlloc<int2>(ea.size(), c->held);

@@ -95,6 +95,7 @@ struct Dev {
   int *exclStart, *exclPartners;
   // bonded terms (f32 parameters)
   int nBond, nAngle, nTorsion, nException, nRestraint;
+  int nExcTriplets;         // leading exception triplets: the three pairs of a three-atom molecule
   int2 *bondAtoms;
   double2 *bondParams;      // k, r0 (FP64: r - r0 cancels)
   int4 *angleAtoms;         // a, b, c
