@@ -138,6 +138,21 @@ __device__ __forceinline__ GlobalOut<T> globalOut(T *p) {
   return (GlobalOut<T>)p;
 }
 
+// A raw buffer over `bytes` bytes at p, for buffer loads with 32-bit byte
+// offsets: the descriptor lives in scalar registers (the pointer must be the
+// same in every lane), and a load past `bytes` (or at a negative offset, taken
+// as unsigned) returns zero instead of touching memory.
+template <typename T>
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t bufferOf(const T *p, unsigned int bytes) {
+  const unsigned long long a = (unsigned long long)p;
+  const unsigned int lo = __builtin_amdgcn_readfirstlane((unsigned int)a);
+  const unsigned int hi = __builtin_amdgcn_readfirstlane((unsigned int)(a >> 32));
+  void *base = (void *)(((unsigned long long)hi << 32) | lo);
+  constexpr int RAW_BUFFER_FLAGS = 0x00020000;  // the gfx9 raw-buffer format word
+  return __builtin_amdgcn_make_buffer_rsrc(base, (short)0, (int)__builtin_amdgcn_readfirstlane(bytes),
+                                           RAW_BUFFER_FLAGS);
+}
+
 // A value the same in every lane of the wave (read from lane 0 or the first
 // active lane), held in a scalar register.
 __device__ __forceinline__ float uniformFloat(float x) {

@@ -123,10 +123,17 @@ __device__ __forceinline__ void nonbonded(const Dev &d, float alpha, int block, 
   const BoxF box = uniformPairBox(*d.box, ENERGIES);
   const GlobalPtr<float4> slotPos = globalPtr(d.slotPos);
   const GlobalPtr<int> slotType = globalPtr(d.slotType);
+  const __amdgpu_buffer_rsrc_t posBuffer = bufferOf(d.slotPos, (unsigned int)d.n * 16u);
+  const __amdgpu_buffer_rsrc_t typeBuffer = bufferOf(d.slotType, (unsigned int)d.n * 4u);
   const int tid = block * blockDim.x + threadIdx.x;
   const int row = tid / LANES;  // a slot: rows and list entries follow the cell order
   const int sub = tid % LANES;
   const bool valid = row < d.n;
+  // the list rows of this wave (64 / LANES consecutive rows from firstRow)
+  const int firstRow = __builtin_amdgcn_readfirstlane(row);
+  const int waveRows = min(64 / LANES, max(0, d.n - firstRow));
+  const __amdgpu_buffer_rsrc_t listBuffer =
+      bufferOf(d.nbrList + (size_t)firstRow * d.cap, (unsigned int)waveRows * (unsigned int)d.cap * 4u);
   float fx = 0.f, fy = 0.f, fz = 0.f;
   double elj = 0.0, ecoul = 0.0, dscale = 0.0;
   if (valid) {
@@ -134,31 +141,29 @@ __device__ __forceinline__ void nonbonded(const Dev &d, float alpha, int block, 
     const int ti = slotType[row] * d.types;
     const float qi = ONE_4PI_EPS0_F * pi.w;
     const int count = globalPtr(d.nbrCount)[row];
-    const GlobalPtr<int> list = globalPtr(d.nbrList + (size_t)row * d.cap);
     // UNROLL entries per iteration, k + u * LANES: their indices are loaded
     // together, then their positions and types together, then the pairs are
     // computed, so a lane waits two memory round trips per UNROLL pairs and
     // the other waves of the SIMD fill the wait. Every load is issued
-    // unconditionally: an entry index past the row's count is clamped into
-    // the row's capacity and the slot it holds (unwritten memory past the
-    // count) into the slots, so both loads stay in bounds; such a pair is
-    // loaded but not computed.
+    // unconditionally, through buffer descriptors that return zero for any
+    // offset outside them: the list's over this wave's rows (an entry past a
+    // row's count reads unwritten memory or the next row's entries) and the
+    // slots' (whatever slot such an entry holds). Such a pair is loaded but
+    // not computed.
     constexpr int UNROLL = 4;
-    const int last = d.cap - 1;
-    const unsigned lastSlot = (unsigned)(d.n - 1);
+    const unsigned int rowOffset = (unsigned int)(row - firstRow) * (unsigned int)d.cap * 4u;
     for (int k = sub; k < count; k += UNROLL * LANES) {
       int j[UNROLL];
 #pragma unroll
-      for (int u = 0; u < UNROLL; ++u) {
-        const int e = k + u * LANES;
-        j[u] = (int)min((unsigned)list[e < last ? e : last], lastSlot);
-      }
+      for (int u = 0; u < UNROLL; ++u)
+        j[u] = (int)__builtin_amdgcn_raw_buffer_load_b32(listBuffer, rowOffset + (unsigned int)(k + u * LANES) * 4u, 0,
+                                                         0);
       float4 p[UNROLL];
       int t[UNROLL];
 #pragma unroll
       for (int u = 0; u < UNROLL; ++u) {
-        p[u] = slotPos[j[u]];
-        t[u] = slotType[j[u]];
+        p[u] = __builtin_bit_cast(float4, __builtin_amdgcn_raw_buffer_load_b128(posBuffer, j[u] * 16, 0, 0));
+        t[u] = (int)__builtin_amdgcn_raw_buffer_load_b32(typeBuffer, j[u] * 4, 0, 0);
       }
 #pragma unroll
       for (int u = 0; u < UNROLL; ++u)
