@@ -1067,9 +1067,10 @@ bool pmeConfigure(PmeDev &p, int mode, size_t maxLds) {
 
 static int xLineBlocks(const PmeDev &p) { return p.K[1] * ((p.K2h + XLINES - 1) / XLINES); }
 int pmeXLineBlocks(const PmeDev &p) { return xLineBlocks(p); }
-size_t pmePlaneForwardLds(const PmeDev &p) { return planeForwardLds(p); }
+// (A binned system's planes never use the scan list: its LDS is left out.)
+size_t pmePlaneForwardLds(const PmeDev &p) { return planeForwardLds(p) - (p.binned ? listBytes() : 0); }
 size_t pmeXLinesLds(const PmeDev &p) { return xLinesLds(p); }
-size_t pmePlaneGatherLds(const PmeDev &p) { return planeGatherLds(p); }
+size_t pmePlaneGatherLds(const PmeDev &p) { return planeGatherLds(p) - (p.binned ? listBytes() : 0); }
 
 // Batched plane pipeline; the LDS sizes are the largest over the replicas and
 // the kernels' LDS limits were raised to them at batch creation.
