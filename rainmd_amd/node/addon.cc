@@ -234,9 +234,11 @@ void finalizeHandle<rmd_system>(napi_env, void *data, void *) {
 struct CtxBox {
   rmd_ctx *ctx = nullptr;
   int busy = 0;
+  bool owned = true;  // false: a replica of a batch (the batch owns it and this box)
 };
 void finalizeCtx(napi_env, void *data, void *) {
   CtxBox *box = static_cast<CtxBox *>(data);
+  if (!box->owned) return;
   if (box->ctx) rmd_destroy(box->ctx);
   delete box;
 }
@@ -505,6 +507,9 @@ napi_value PmeParameters(napi_env env, napi_callback_info info) {
 
 // ---- the simulation context ----
 
+void readOptions(napi_env env, napi_value opt, rmd_options &o, std::vector<int32_t> &restrained,
+                 std::vector<double> &reference);
+
 napi_value Create(napi_env env, napi_callback_info info) {
   napi_value argv[2];
   args(env, info, 2, argv);
@@ -513,8 +518,23 @@ napi_value Create(napi_env env, napi_callback_info info) {
     napi_throw_type_error(env, nullptr, "create expects a built system");
     return nullptr;
   }
-  napi_value opt = argv[1];
   rmd_options o;
+  std::vector<int32_t> restrained;
+  std::vector<double> reference;
+  readOptions(env, argv[1], o, restrained, reference);
+  rmd_ctx *ctx = nullptr;
+  if (!check(env, rmd_create(s, &o, &ctx), nullptr)) return nullptr;
+  CtxBox *box = new CtxBox();
+  box->ctx = ctx;
+  napi_value out;
+  napi_create_external(env, box, finalizeCtx, nullptr, &out);
+  return out;
+}
+
+// The reference's constructor options (webgpu/simulation.js:82-180) into
+// rmd_options; the restraint arrays live in the caller's vectors.
+void readOptions(napi_env env, napi_value opt, rmd_options &o, std::vector<int32_t> &restrained,
+                 std::vector<double> &reference) {
   rmd_options_default(&o);
   o.timestep = prop(env, opt, "timestep", o.timestep);
   o.temperature = prop(env, opt, "temperature", o.temperature);
@@ -539,8 +559,6 @@ napi_value Create(napi_env env, napi_callback_info info) {
     napi_get_value_bool(env, v, &b);
     o.dispersion_correction = b ? 1 : 0;
   }
-  std::vector<int32_t> restrained;
-  std::vector<double> reference;
   o.restraint_force = prop(env, opt, "restraintForce", 0.0);
   if (propValue(env, opt, "restraintAtoms", &v)) getInts(env, v, restrained);
   if (propValue(env, opt, "restraintReference", &v)) getDoubles(env, v, reference);
@@ -557,13 +575,6 @@ napi_value Create(napi_env env, napi_callback_info info) {
     for (int k = 0; k < 3; ++k) o.grid[k] = grid[k];
   o.device = (int32_t)prop(env, opt, "device", o.device);
   o.lanes_per_atom = (int32_t)prop(env, opt, "lanesPerAtom", o.lanes_per_atom);
-  rmd_ctx *ctx = nullptr;
-  if (!check(env, rmd_create(s, &o, &ctx), nullptr)) return nullptr;
-  CtxBox *box = new CtxBox();
-  box->ctx = ctx;
-  napi_value out;
-  napi_create_external(env, box, finalizeCtx, nullptr, &out);
-  return out;
 }
 
 napi_value Destroy(napi_env env, napi_callback_info info) {
@@ -571,6 +582,10 @@ napi_value Destroy(napi_env env, napi_callback_info info) {
   args(env, info, 1, argv);
   CtxBox *box = unwrap<CtxBox>(env, argv[0]);
   if (box && box->ctx) {
+    if (!box->owned) {
+      napi_throw(env, makeError(env, RMD_E_INVALID, "a batch replica is released with its batch"));
+      return nullptr;
+    }
     if (box->busy) {
       napi_throw(env, makeError(env, RMD_E_INVALID, "release() while a call is still running"));
       return nullptr;
@@ -891,6 +906,309 @@ napi_value AttemptVolumeChange(napi_env env, napi_callback_info info) {
   return startAsync(env, argv[0], call, "rainmd.attemptVolumeChange");
 }
 
+// ---- replica batches (rmd_batch_*) ----
+// A batch handle owns the boxes of its replicas' contexts: a replica view
+// (batchReplica) is a non-owning external of such a box, usable through every
+// context entry point above while the batch lives and is not running. The
+// JavaScript facade keeps the batch alive from each view.
+
+struct BatchBox {
+  rmd_batch *batch = nullptr;
+  int busy = 0;
+  std::vector<CtxBox *> members;
+};
+void finalizeBatch(napi_env, void *data, void *) {
+  BatchBox *b = static_cast<BatchBox *>(data);
+  if (b->batch) rmd_batch_destroy(b->batch);
+  for (CtxBox *m : b->members) delete m;
+  delete b;
+}
+void finalizeView(napi_env, void *, void *) {}
+
+BatchBox *unwrapBatch(napi_env env, napi_value v) {
+  BatchBox *b = unwrap<BatchBox>(env, v);
+  if (!b || !b->batch) {
+    napi_throw(env, makeError(env, RMD_E_INVALID, "the batch was released"));
+    return nullptr;
+  }
+  return b;
+}
+
+bool checkBatch(napi_env env, int status, rmd_batch *b) {
+  if (status == RMD_OK) return true;
+  const char *m = rmd_batch_last_error(b);
+  napi_throw(env, makeError(env, status, m ? m : "rainmd error"));
+  return false;
+}
+
+bool batchIdle(napi_env env, BatchBox *b) {
+  if (!b->busy) return true;
+  napi_throw(env, makeError(env, RMD_E_INVALID, "another call on this batch is still running"));
+  return false;
+}
+
+// batchCreate(systems[], options, seeds[] | null)
+napi_value BatchCreate(napi_env env, napi_callback_info info) {
+  napi_value argv[3];
+  args(env, info, 3, argv);
+  bool isArray = false;
+  napi_is_array(env, argv[0], &isArray);
+  if (!isArray) {
+    napi_throw_type_error(env, nullptr, "batchCreate expects an array of built systems");
+    return nullptr;
+  }
+  uint32_t count = 0;
+  napi_get_array_length(env, argv[0], &count);
+  std::vector<const rmd_system *> systems(count);
+  for (uint32_t k = 0; k < count; ++k) {
+    napi_value e;
+    napi_get_element(env, argv[0], k, &e);
+    systems[k] = unwrap<rmd_system>(env, e);
+    if (!systems[k]) {
+      napi_throw_type_error(env, nullptr, "batchCreate expects an array of built systems");
+      return nullptr;
+    }
+  }
+  rmd_options o;
+  std::vector<int32_t> restrained;
+  std::vector<double> reference;
+  readOptions(env, argv[1], o, restrained, reference);
+  std::vector<int32_t> given;
+  std::vector<uint32_t> seeds;
+  if (!isNullish(env, argv[2]) && getInts(env, argv[2], given)) {
+    if (given.size() != count) {
+      napi_throw(env, makeError(env, RMD_E_INVALID, "batchCreate needs one seed per system"));
+      return nullptr;
+    }
+    for (int32_t v : given) seeds.push_back((uint32_t)v);
+  }
+  rmd_batch *batch = nullptr;
+  if (!check(env, rmd_batch_create(systems.data(), (int32_t)count, &o, seeds.empty() ? nullptr : seeds.data(), &batch),
+             nullptr))
+    return nullptr;
+  BatchBox *b = new BatchBox();
+  b->batch = batch;
+  for (uint32_t r = 0; r < count; ++r) {
+    CtxBox *m = new CtxBox();
+    m->ctx = rmd_batch_replica(batch, (int32_t)r);
+    m->owned = false;
+    b->members.push_back(m);
+  }
+  napi_value out;
+  napi_create_external(env, b, finalizeBatch, nullptr, &out);
+  return out;
+}
+
+napi_value BatchDestroy(napi_env env, napi_callback_info info) {
+  napi_value argv[1];
+  args(env, info, 1, argv);
+  BatchBox *b = unwrap<BatchBox>(env, argv[0]);
+  if (b && b->batch) {
+    if (!batchIdle(env, b)) return nullptr;
+    for (CtxBox *m : b->members)
+      if (m->busy) {
+        napi_throw(env, makeError(env, RMD_E_INVALID, "release() while a call on a replica is still running"));
+        return nullptr;
+      }
+    rmd_batch_destroy(b->batch);
+    b->batch = nullptr;
+    for (CtxBox *m : b->members) m->ctx = nullptr;
+  }
+  return undefinedValue(env);
+}
+
+napi_value BatchReplica(napi_env env, napi_callback_info info) {
+  napi_value argv[2];
+  args(env, info, 2, argv);
+  BatchBox *b = unwrapBatch(env, argv[0]);
+  if (!b) return nullptr;
+  const double r = getNumber(env, argv[1], -1);
+  if (!(r >= 0 && r < (double)b->members.size())) {
+    napi_throw(env, makeError(env, RMD_E_INVALID, "no such replica"));
+    return nullptr;
+  }
+  napi_value out;
+  napi_create_external(env, b->members[(size_t)r], finalizeView, nullptr, &out);
+  return out;
+}
+
+napi_value BatchInfo(napi_env env, napi_callback_info info) {
+  napi_value argv[1];
+  args(env, info, 1, argv);
+  BatchBox *b = unwrapBatch(env, argv[0]);
+  if (!b) return nullptr;
+  rmd_batch_info i;
+  if (!checkBatch(env, rmd_batch_info_get(b->batch, &i), b->batch)) return nullptr;
+  napi_value o = newObject(env);
+  setNumber(env, o, "replicas", i.replicas);
+  setNumber(env, o, "atoms", (double)i.atoms);
+  setNumber(env, o, "blocksRun", (double)i.blocks_run);
+  setNumber(env, o, "stepWorkgroups", (double)i.step_workgroups);
+  setNumber(env, o, "nonbondedWorkgroups", (double)i.nonbonded_workgroups);
+  return o;
+}
+
+napi_value BatchSwitch(napi_env env, napi_callback_info info, int (*fn)(rmd_batch *, int32_t)) {
+  napi_value argv[2];
+  args(env, info, 2, argv);
+  BatchBox *b = unwrapBatch(env, argv[0]);
+  if (!b || !batchIdle(env, b)) return nullptr;
+  bool on = true;
+  napi_get_value_bool(env, argv[1], &on);
+  if (!checkBatch(env, fn(b->batch, on ? 1 : 0), b->batch)) return nullptr;
+  return undefinedValue(env);
+}
+napi_value BatchSetGraphs(napi_env env, napi_callback_info info) {
+  return BatchSwitch(env, info, rmd_batch_set_graphs);
+}
+napi_value BatchSetConcurrentForces(napi_env env, napi_callback_info info) {
+  return BatchSwitch(env, info, rmd_batch_set_concurrent_forces);
+}
+
+napi_value BatchRequestStop(napi_env env, napi_callback_info info) {
+  napi_value argv[1];
+  args(env, info, 1, argv);
+  BatchBox *b = unwrap<BatchBox>(env, argv[0]);
+  if (b && b->batch) rmd_batch_request_stop(b->batch);
+  return undefinedValue(env);
+}
+
+// batchScreenConfigure(batch, {enabled, trainingMoves, trainingSteps, learningRate, seed})
+napi_value BatchScreenConfigure(napi_env env, napi_callback_info info) {
+  napi_value argv[2];
+  args(env, info, 2, argv);
+  BatchBox *b = unwrapBatch(env, argv[0]);
+  if (!b || !batchIdle(env, b)) return nullptr;
+  rmd_screen_options o;
+  rmd_screen_options_default(&o);
+  napi_value v;
+  if (propValue(env, argv[1], "enabled", &v)) {
+    bool e = true;
+    napi_get_value_bool(env, v, &e);
+    o.enabled = e ? 1 : 0;
+  }
+  o.training_moves = (int32_t)prop(env, argv[1], "trainingMoves", o.training_moves);
+  o.training_steps = (int32_t)prop(env, argv[1], "trainingSteps", o.training_steps);
+  o.learning_rate = (float)prop(env, argv[1], "learningRate", o.learning_rate);
+  o.seed = (uint32_t)prop(env, argv[1], "seed", o.seed);
+  if (!checkBatch(env, rmd_batch_screen_configure(b->batch, &o), b->batch)) return nullptr;
+  return undefinedValue(env);
+}
+
+napi_value BatchScreenStats(napi_env env, napi_callback_info info) {
+  napi_value argv[1];
+  args(env, info, 1, argv);
+  BatchBox *b = unwrapBatch(env, argv[0]);
+  if (!b) return nullptr;
+  rmd_screen_stats st;
+  if (!checkBatch(env, rmd_batch_screen_stats_get(b->batch, &st), b->batch)) return nullptr;
+  napi_value o = newObject(env);
+  setNumber(env, o, "phase", st.phase);
+  setNumber(env, o, "samples", (double)st.samples);
+  setNumber(env, o, "moves", (double)st.moves);
+  setNumber(env, o, "exactEvaluations", (double)st.exact_evaluations);
+  setNumber(env, o, "screenedOut", (double)st.screened_out);
+  setNumber(env, o, "secondStageRejections", (double)st.second_stage_rejections);
+  setNumber(env, o, "accepted", (double)st.accepted);
+  setNumber(env, o, "trainingLoss", st.training_loss);
+  setNumber(env, o, "residualRms", st.residual_rms);
+  setNumber(env, o, "estimateRms", st.estimate_rms);
+  return o;
+}
+
+// initialise and run, off the JavaScript thread; the batch and its replicas
+// are busy until the promise settles
+enum BatchKind { B_INITIALISE, B_RUN };
+struct BatchCall {
+  BatchKind kind;
+  BatchBox *box = nullptr;
+  napi_ref boxRef = nullptr;
+  napi_deferred deferred = nullptr;
+  napi_async_work work = nullptr;
+  int status = RMD_OK;
+  std::string error;
+  int64_t steps = 0;
+  int32_t block = 0;
+  int64_t stepOut = 0;
+};
+
+void setBatchBusy(BatchBox *b, int busy) {
+  b->busy = busy;
+  for (CtxBox *m : b->members) m->busy = busy;
+}
+
+void executeBatch(napi_env, void *data) {
+  BatchCall *call = static_cast<BatchCall *>(data);
+  rmd_batch *batch = call->box->batch;
+  if (call->kind == B_INITIALISE)
+    call->status = rmd_batch_initialise(batch);
+  else
+    call->status = rmd_batch_run(batch, call->steps, call->block, &call->stepOut);
+  if (call->status != RMD_OK) {
+    const char *m = rmd_batch_last_error(batch);
+    call->error = m ? m : "rainmd error";
+  }
+}
+
+void completeBatch(napi_env env, napi_status, void *data) {
+  BatchCall *call = static_cast<BatchCall *>(data);
+  setBatchBusy(call->box, 0);
+  if (call->status != RMD_OK) {
+    napi_reject_deferred(env, call->deferred, makeError(env, call->status, call->error));
+  } else {
+    napi_value result = undefinedValue(env);
+    if (call->kind == B_RUN) napi_create_double(env, (double)call->stepOut, &result);
+    napi_resolve_deferred(env, call->deferred, result);
+  }
+  napi_delete_async_work(env, call->work);
+  if (call->boxRef) napi_delete_reference(env, call->boxRef);
+  delete call;
+}
+
+napi_value startBatch(napi_env env, napi_value handle, BatchCall *call, const char *name) {
+  BatchBox *b = unwrapBatch(env, handle);
+  bool idle = b && batchIdle(env, b);
+  if (idle)
+    for (CtxBox *m : b->members)
+      if (m->busy) {
+        napi_throw(env, makeError(env, RMD_E_INVALID, "a call on a replica of this batch is still running"));
+        idle = false;
+        break;
+      }
+  if (!idle) {
+    delete call;
+    return nullptr;
+  }
+  call->box = b;
+  setBatchBusy(b, 1);
+  napi_value promise, resource;
+  napi_create_promise(env, &call->deferred, &promise);
+  napi_create_reference(env, handle, 1, &call->boxRef);
+  napi_create_string_utf8(env, name, NAPI_AUTO_LENGTH, &resource);
+  napi_create_async_work(env, nullptr, resource, executeBatch, completeBatch, call, &call->work);
+  napi_queue_async_work(env, call->work);
+  return promise;
+}
+
+napi_value BatchInitialise(napi_env env, napi_callback_info info) {
+  napi_value argv[1];
+  args(env, info, 1, argv);
+  BatchCall *call = new BatchCall();
+  call->kind = B_INITIALISE;
+  return startBatch(env, argv[0], call, "rainmd.batchInitialise");
+}
+
+// batchRun(batch, steps, block) -> the replicas' step afterwards
+napi_value BatchRun(napi_env env, napi_callback_info info) {
+  napi_value argv[3];
+  args(env, info, 3, argv);
+  BatchCall *call = new BatchCall();
+  call->kind = B_RUN;
+  call->steps = (int64_t)getNumber(env, argv[1], 0);
+  call->block = (int32_t)getNumber(env, argv[2], 0);
+  return startBatch(env, argv[0], call, "rainmd.batchRun");
+}
+
 napi_value DeviceInfo(napi_env env, napi_callback_info info) {
   napi_value argv[1];
   args(env, info, 1, argv);
@@ -941,6 +1259,17 @@ napi_value Init(napi_env env, napi_value exports) {
       {"minimise", Minimise},
       {"scaleVolume", ScaleVolume},
       {"attemptVolumeChange", AttemptVolumeChange},
+      {"batchCreate", BatchCreate},
+      {"batchDestroy", BatchDestroy},
+      {"batchReplica", BatchReplica},
+      {"batchInfo", BatchInfo},
+      {"batchSetGraphs", BatchSetGraphs},
+      {"batchSetConcurrentForces", BatchSetConcurrentForces},
+      {"batchRequestStop", BatchRequestStop},
+      {"batchScreenConfigure", BatchScreenConfigure},
+      {"batchScreenStats", BatchScreenStats},
+      {"batchInitialise", BatchInitialise},
+      {"batchRun", BatchRun},
       {"energy", Energy},
       {"readPositions", ReadPositions},
       {"readState", ReadState},

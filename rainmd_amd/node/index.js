@@ -213,19 +213,25 @@ class GpuSimulation {
   constructor(device, system, options) {
     const o = Object.assign({}, options || {});
     if (typeof device === 'number') o.device = device;
-    this.system = system;
-    this.atomCount = system.atomCount;
-    this.timestep = o.timestep === undefined ? 0.002 : o.timestep;
-    this.pressure = o.pressure === undefined ? null : o.pressure;
-    this._handle = call(() => addon.create(system.handle, o));
-    this._stopSignals = [];
-    this.step = 0;
-    const diag = addon.diagnostics(this._handle);
+    GpuSimulation._attach(this, call(() => addon.create(system.handle, o)), system, o);
+  }
+
+  // The fields of a simulation over a context handle (its own, or a replica
+  // of a batch).
+  static _attach(sim, handle, system, o) {
+    sim.system = system;
+    sim.atomCount = system.atomCount;
+    sim.timestep = o.timestep === undefined ? 0.002 : o.timestep;
+    sim.pressure = o.pressure === undefined ? null : o.pressure;
+    sim._handle = handle;
+    sim._stopSignals = [];
+    sim.step = 0;
+    const diag = addon.diagnostics(handle);
     // public fields callers read (web/app.js:1141-1143)
-    this.pme = { grid: Array.from(diag.pmeGrid), alpha: diag.pmeAlpha };
-    this.plan = { counts: Array.from(diag.cellCounts), skin: diag.skin, maxNeighbours: diag.neighbourCapacity };
-    this.skin = diag.skin;
-    this.clusters = { layout: { clusterCount: diag.clusterUnits + diag.waterUnits } };
+    sim.pme = { grid: Array.from(diag.pmeGrid), alpha: diag.pmeAlpha };
+    sim.plan = { counts: Array.from(diag.cellCounts), skin: diag.skin, maxNeighbours: diag.neighbourCapacity };
+    sim.skin = diag.skin;
+    sim.clusters = { layout: { clusterCount: diag.clusterUnits + diag.waterUnits } };
   }
 
   _context(extra) {
@@ -325,8 +331,67 @@ function soluteHeavyAtoms(system) {
   return call(() => addon.soluteHeavyAtoms(system.handle));
 }
 
+// A replica batch (rmd_batch_*): many systems, each with its own box, PME
+// grid, seed and barostat, stepped together by batched launches; every
+// replica is bit for bit its own single simulation. No counterpart in the
+// reference. replica(r) is the GpuSimulation surface of one replica (released
+// with the batch); run() steps them all.
+class Batch {
+  constructor(device, systems, options, seeds) {
+    const o = Object.assign({}, options || {});
+    if (typeof device === 'number') o.device = device;
+    this.systems = systems;
+    this._options = o;
+    this._handle = call(() => addon.batchCreate(systems.map((sys) => sys.handle), o, seeds || null));
+    this._views = [];
+    this.step = 0;
+  }
+
+  get replicas() { return this.systems.length; }
+
+  replica(r) {
+    if (!this._views[r]) {
+      const view = Object.create(GpuSimulation.prototype);
+      GpuSimulation._attach(view, call(() => addon.batchReplica(this._handle, r)), this.systems[r], this._options);
+      view._batch = this;  // a view keeps its batch alive
+      this._views[r] = view;
+    }
+    return this._views[r];
+  }
+
+  async _async(fn) {
+    try {
+      return await fn();
+    } catch (e) {
+      throw translate(e, { step: this.step });
+    }
+  }
+
+  async initialise() { await this._async(() => addon.batchInitialise(this._handle)); }
+
+  // run(count, {blockSize}) -> step
+  async run(count, options) {
+    const o = options || {};
+    this.step = await this._async(() => addon.batchRun(this._handle, count, o.blockSize || 0));
+    for (const view of this._views) if (view) view.step = this.step;
+    return this.step;
+  }
+
+  info() { return call(() => addon.batchInfo(this._handle)); }
+  setGraphs(enabled) { call(() => addon.batchSetGraphs(this._handle, !!enabled)); }
+  setConcurrentForces(enabled) { call(() => addon.batchSetConcurrentForces(this._handle, !!enabled)); }
+  requestStop() { addon.batchRequestStop(this._handle); }
+  // the shared learned barostat screen: {enabled, trainingMoves, trainingSteps, learningRate, seed}
+  configureScreen(options) { call(() => addon.batchScreenConfigure(this._handle, options || {})); }
+  screenStats() { return call(() => addon.batchScreenStats(this._handle)); }
+
+  release() {
+    if (this._handle) addon.batchDestroy(this._handle);
+  }
+}
+
 module.exports = {
-  GpuSimulation, StopSignal,
+  GpuSimulation, Batch, StopSignal,
   NeighbourOverflowError, NonFiniteStateError, SimulationStopped, GpuBudgetError, ConstraintFailure,
   parsePrmtop, parseInpcrd, boxVectorsFromLengthsAndAngles, buildSystem, buildSolvated, pmeParameters,
   reboxSystem, soluteHeavyAtoms,

@@ -143,6 +143,32 @@ def test_node_host_runs_on_the_gpu():
 
 
 @pytest.mark.gpu
+def test_node_batch_equals_the_python_batch():
+    # the same replica batch through the Node facade and through the Python
+    # host: one C ABI, so the replicas end bit for bit alike
+    out = _run("batch_gpu.js", timeout=300)
+    names, seeds = ["ala-389w", "ala-944w"], [31, 32]
+    systems = [R.workload_system(n) for n in names]
+    batch = R.Batch(systems, seeds=seeds, temperature=300.0, pressure=1.0, barostat_interval=10, lanes_per_atom=16)
+    for r, sys_ in enumerate(systems):
+        sim = batch.replica(r)
+        sim.minimise(max_iterations=60, force_tolerance=1000.0)
+        sim.set_state(sim.read_positions().astype(np.float64), R.maxwell_boltzmann(sys_, 300.0, seeds[r]), step=0)
+    batch.run(60)
+    assert out["step"] == 60 and out["info"]["replicas"] == 2 and out["infoAfter"]["blocksRun"] >= 1
+    for r in range(2):
+        node = np.array(out["replicas"][r]["positions"], dtype=np.float32)
+        py = batch.replica(r).read_positions().reshape(-1)
+        print("replica", r, "potential", out["replicas"][r]["potential"], batch.replica(r).energy()["potential"])
+        assert np.array_equal(node, py), r
+        assert out["replicas"][r]["step"] == 60
+        assert out["replicas"][r]["attempts"] == batch.replica(r).diagnostics()["barostat_attempts"]
+    batch.release()
+    assert "released with its batch" in out["replicaRelease"]
+    assert "released" in out["afterRelease"]
+
+
+@pytest.mark.gpu
 def test_run_driver_stages_hand_over_the_whole_state(tmp_path):
     # minimise -> restrained NPT equilibration -> unrestrained production
     # (web/app.js:1041-1124): each stage a fresh simulation in the box the
