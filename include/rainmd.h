@@ -461,8 +461,10 @@ int32_t rmd_abi_version(void);
  * would alone. Every replica takes the same options except its seed
  * (seeds[r], or options->seed when seeds is NULL), needs the same Ewald
  * alpha, at most 16384 atoms and 4096 cells, and a PME grid whose planes fit
- * in LDS. options->lanes_per_atom picks the direct space's lanes per atom for
- * every member (8, 16, 32 or 64; 0 = 16). */
+ * in LDS; a batch holds at most RMD_BATCH_MAX_REPLICAS replicas.
+ * options->lanes_per_atom picks the direct space's lanes per atom for every
+ * member (8, 16, 32 or 64; 0 = 16). */
+#define RMD_BATCH_MAX_REPLICAS 4096
 typedef struct rmd_batch rmd_batch;
 int rmd_batch_create(const rmd_system *const *systems, int32_t count, const rmd_options *options,
                      const uint32_t *seeds, rmd_batch **out);

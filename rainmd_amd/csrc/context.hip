@@ -1944,6 +1944,8 @@ struct rmd_batch {
   std::vector<BatchView> views;  // host copy of the device views
   BatchView *dViews = nullptr;
   TaskTable tables[BK_COUNT];
+  int *searchQueue = nullptr;  // the due replicas' list search blocks (searchQueueInts ints)
+  int searchGrid = 0;          // workgroups of the batched list search
   Scalars *scAll = nullptr, *scSaved = nullptr;  // every replica's scalars, contiguous
   BoxF *boxAll = nullptr;
   double *termsAll = nullptr;  // replicas * E_TERM_COUNT
@@ -1965,15 +1967,12 @@ struct rmd_batch {
 
 namespace {
 
-constexpr int XCDS = 8;
-
 int replicaBlocks(const rmd_ctx *c, int kind) {
   const Dev &d = c->d;
   switch (kind) {
     case BK_OPEN: return generalBlockCount(d);
     case BK_WATER: return d.waterBlocks;
     case BK_UNITS: return unitBlockCount(d);
-    case BK_ATOMS64: return (d.n + 3) / 4;  // 64 lanes per atom, 256 threads per workgroup
     case BK_BONDED: return bondedBlocks(d);
     case BK_PLANES: return c->p.K[0];
     case BK_XLINES: return pmeXLineBlocks(c->p);
@@ -1997,8 +1996,8 @@ TaskTable buildTasks(rmd_batch *b, int kind) {
   for (int r = 0; r < R; ++r) per[r] = replicaBlocks(b->reps[r], kind);
   std::iota(order.begin(), order.end(), 0);
   std::stable_sort(order.begin(), order.end(), [&](int x, int y) { return per[x] > per[y]; });
-  std::vector<std::vector<int2>> bins(XCDS);
-  std::vector<long long> load(XCDS, 0);
+  std::vector<std::vector<int2>> bins(XCD_COUNT);
+  std::vector<long long> load(XCD_COUNT, 0);
   for (int r : order) {
     const int x = (int)(std::min_element(load.begin(), load.end()) - load.begin());
     for (int k = 0; k < per[r]; ++k) bins[x].push_back(make_int2(r, k));
@@ -2006,9 +2005,9 @@ TaskTable buildTasks(rmd_batch *b, int kind) {
   }
   size_t longest = 0;
   for (auto &bin : bins) longest = std::max(longest, bin.size());
-  std::vector<int2> task(longest * XCDS, make_int2(-1, 0));
-  for (int x = 0; x < XCDS; ++x)
-    for (size_t k = 0; k < bins[x].size(); ++k) task[k * XCDS + x] = bins[x][k];
+  std::vector<int2> task(longest * XCD_COUNT, make_int2(-1, 0));
+  for (int x = 0; x < XCD_COUNT; ++x)
+    for (size_t k = 0; k < bins[x].size(); ++k) task[k * XCD_COUNT + x] = bins[x][k];
   TaskTable t;
   t.blocks = (int)task.size();
   t.task = devAlloc<int2>(std::max<size_t>(1, task.size()), b->held);
@@ -2138,7 +2137,8 @@ void enqueueBatchForces(rmd_batch *b, int energies, int terms = RMD_TERM_ALL) {
                    b->tables[BK_ATOMS256], energies, b->ldsForward, b->ldsXlines, b->ldsGather, pmeStream);
   if (terms & (RMD_TERM_BONDED | RMD_TERM_EXCEPTIONS | RMD_TERM_RESTRAINT))
     launchBondedBatch(b->dViews, b->tables[BK_BONDED], b->alpha, terms, energies, bondedStream);
-  launchRebuildBatch(b->dViews, b->tables[BK_REPLICA], b->tables[BK_ATOMS64], b->allRect, st.stream);
+  launchRebuildBatch(b->dViews, b->tables[BK_REPLICA], b->replicas, b->searchQueue, b->searchGrid, b->allRect,
+                     st.stream);
   if (terms & RMD_TERM_DIRECT)
     launchNonbondedBatch(b->dViews, b->tables[BK_NONBONDED], b->lanes, (float)b->alpha, b->rational, energies,
                          b->maxTypes, b->allRect, st.stream);
@@ -2566,6 +2566,13 @@ void createBatch(rmd_batch *b, const rmd_system *const *systems, int count, cons
   }
   pmeBatchLdsLimits(b->ldsForward, b->ldsXlines, b->ldsGather);
   for (int k = 0; k < BK_COUNT; ++k) b->tables[k] = buildTasks(b, k);
+  // The list search: at most 1024 workgroups per XCD (enough to fill the chip
+  // when every replica rebuilds; a search block is four cells or four atoms),
+  // fewer when the replicas hold fewer blocks.
+  long long searchBlocks = 0;
+  for (rmd_ctx *c : b->reps) searchBlocks += (c->d.n + 3) / 4;
+  b->searchGrid = XCD_COUNT * (int)std::min<long long>(1024, (searchBlocks + XCD_COUNT - 1) / XCD_COUNT);
+  b->searchQueue = devAlloc<int>(searchQueueInts(b->replicas), b->held);
   uploadViews(b);
 }
 
@@ -2598,6 +2605,11 @@ int rmd_batch_create(const rmd_system *const *systems, int32_t count, const rmd_
                      const uint32_t *seeds, rmd_batch **out) {
   if (!systems || !out || count <= 0) {
     setGlobalError(count <= 0 ? "a batch needs at least one replica" : "null argument");
+    return RMD_E_INVALID;
+  }
+  if (count > RMD_BATCH_MAX_REPLICAS) {
+    setGlobalError("a batch holds at most " + std::to_string(RMD_BATCH_MAX_REPLICAS) +
+                   " replicas (the list search stages two ints per replica in LDS)");
     return RMD_E_INVALID;
   }
   rmd_batch *b = new rmd_batch();
@@ -2634,7 +2646,7 @@ int rmd_batch_info_get(rmd_batch *b, rmd_batch_info *out) {
     out->replicas = b->replicas;
     for (rmd_ctx *c : b->reps) out->atoms += c->d.n;
     out->blocks_run = b->blocksRun;
-    out->step_workgroups = b->tables[BK_WATER].blocks + b->tables[BK_OPEN].blocks + b->tables[BK_ATOMS64].blocks + b->tables[BK_NONBONDED].blocks +
+    out->step_workgroups = b->tables[BK_WATER].blocks + b->tables[BK_OPEN].blocks + b->searchGrid + b->tables[BK_NONBONDED].blocks +
                            b->tables[BK_BONDED].blocks + 2 * b->tables[BK_PLANES].blocks +
                            b->tables[BK_XLINES].blocks + b->tables[BK_REPLICA].blocks;
     out->nonbonded_workgroups = b->tables[BK_NONBONDED].blocks;

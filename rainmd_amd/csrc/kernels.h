@@ -156,9 +156,11 @@ struct TaskTable {
   int blocks = 0;        // grid size (padding included)
 };
 
+// MI355X's XCDs: workgroup b of a launch runs on XCD b % XCD_COUNT.
+constexpr int XCD_COUNT = 8;
+
 enum BatchKernel {
   BK_OPEN = 0,      // integrator general blocks (after the water blocks)
-  BK_ATOMS64,       // one wave per atom (nonbonded at 64 lanes, neighbour list)
   BK_BONDED,        // bonded, exception and restraint terms
   BK_PLANES,        // PME plane kernels (one workgroup per x plane)
   BK_XLINES,        // PME x lines
@@ -198,8 +200,12 @@ bool fitsSmallCellBuild(int atoms, int cells);
 // cover at four cells each.
 bool cellSearchFits(int atoms, int cells);
 void launchRefreshSlotsBatch(const BatchView *v, const TaskTable &atoms256, hipStream_t s);
-void launchRebuildBatch(const BatchView *v, const TaskTable &replicas, const TaskTable &atoms64, int rect,
-                        hipStream_t s);
+// The cell lists of the replicas due a rebuild, then their neighbour lists;
+// `queue` holds searchQueueInts(replicas) ints of device memory, `grid` is a
+// multiple of XCD_COUNT.
+void launchRebuildBatch(const BatchView *v, const TaskTable &replicaTable, int replicas, int *queue, int grid,
+                        int rect, hipStream_t s);
+int searchQueueInts(int replicas);
 // lanes: lanes per atom (8, 16, 32 or 64), the table's grid at that count.
 void launchNonbondedBatch(const BatchView *v, const TaskTable &t, int lanes, float alpha, int rational, int energies,
                           int types, int rect, hipStream_t s);

@@ -625,20 +625,86 @@ __global__ void __launch_bounds__(256) neighboursKernel(Dev d) {
   searchBlock<RECT>(d, blockIdx.x);
 }
 
-// Batched, grid-stride over the task table: on most steps only the few
-// replicas due a rebuild have work, and a workgroup skips the others' tasks
-// with one scalar load each instead of the dispatch of one workgroup per task.
-// The stride is a multiple of the XCD count, so a workgroup stays on the
-// replicas of its own XCD.
+// The replicas due a rebuild and their search blocks, densely, for the
+// batched search: queue[0] the due replicas m, queue[1] their search blocks
+// in all, queue[2 .. 2 + m] the prefix of the blocks (its last entry the
+// total), queue[3 + replicas ..] the replica of each. One workgroup scans the
+// replicas in order, SEARCH_QUEUE_THREADS at a time.
+constexpr int SEARCH_QUEUE_THREADS = 1024;
+
+__global__ void __launch_bounds__(SEARCH_QUEUE_THREADS) searchQueueBatchKernel(const BatchView *__restrict__ v,
+                                                                                int replicas, int *__restrict__ queue) {
+  // (due replicas << 32) + search blocks, scanned together
+  __shared__ long long partial[SEARCH_QUEUE_THREADS];
+  int *start = queue + 2, *replica = queue + 3 + replicas;
+  const int t = threadIdx.x;
+  int dueBase = 0, blockBase = 0;
+  for (int r0 = 0; r0 < replicas; r0 += SEARCH_QUEUE_THREADS) {
+    const int r = r0 + t;
+    long long mine = 0;
+    if (r < replicas && v[r].d.sc->rebuild) {
+      const BoxF &box = *v[r].d.box;
+      const int blocks = box.cellSearch ? (box.cellTotal + 3) / 4 : (v[r].d.n + 3) / 4;
+      mine = (1LL << 32) + blocks;
+    }
+    partial[t] = mine;
+    __syncthreads();
+    for (int offset = 1; offset < SEARCH_QUEUE_THREADS; offset <<= 1) {
+      const long long add = t >= offset ? partial[t - offset] : 0;
+      __syncthreads();
+      partial[t] += add;
+      __syncthreads();
+    }
+    const long long before = partial[t] - mine, sum = partial[SEARCH_QUEUE_THREADS - 1];
+    if (mine) {
+      const int at = dueBase + (int)(before >> 32);
+      start[at] = blockBase + (int)(before & 0xffffffffLL);
+      replica[at] = r;
+    }
+    __syncthreads();  // the next chunk reuses partial
+    dueBase += (int)(sum >> 32);
+    blockBase += (int)(sum & 0xffffffffLL);
+  }
+  if (t == 0) {
+    queue[0] = dueBase;
+    queue[1] = blockBase;
+    start[dueBase] = blockBase;
+  }
+}
+
+// Batched search over the queue: the due replicas' blocks, in replica order,
+// are cut into eight equal ranges, one per XCD (workgroup b runs on XCD
+// b % 8, and each XCD has its own L2), so a replica's neighbouring cells are
+// searched through one L2 and every XCD has the same share whichever
+// replicas are due. A workgroup strides through its XCD's range and finds
+// each block's replica by bisection of the prefix, staged in LDS.
 // (Seven waves per SIMD: the register count the search body needs, without
 // spills.)
 template <bool RECT>
-__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(7))) neighboursBatchKernel(const BatchView *__restrict__ v,
-                                                             const int2 *__restrict__ task, int tasks) {
-  for (int k = blockIdx.x; k < tasks; k += gridDim.x) {
-    const int2 t = task[k];
-    if (t.x < 0 || !v[t.x].d.sc->rebuild) continue;
-    searchBlock<RECT>(v[t.x].d, t.y);
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(7)))
+neighboursBatchKernel(const BatchView *__restrict__ v, const int *__restrict__ queue, int replicas) {
+  extern __shared__ int queueLds[];  // the prefix (due + 1 entries), then the replicas (due)
+  const int due = queue[0], total = queue[1];
+  const int xcd = blockIdx.x % XCD_COUNT, perXcd = gridDim.x / XCD_COUNT;
+  const int first = (int)((long long)total * xcd / XCD_COUNT), last = (int)((long long)total * (xcd + 1) / XCD_COUNT);
+  int k = first + (int)(blockIdx.x / XCD_COUNT);
+  if (k >= last) return;
+  int *start = queueLds, *replica = queueLds + due + 1;
+  for (int i = threadIdx.x; i <= due; i += blockDim.x) start[i] = queue[2 + i];
+  for (int i = threadIdx.x; i < due; i += blockDim.x) replica[i] = queue[3 + replicas + i];
+  __syncthreads();
+  for (; k < last; k += perXcd) {
+    int lo = 0, hi = due - 1;  // the last entry whose prefix is at most k
+    while (lo < hi) {
+      const int mid = (lo + hi + 1) >> 1;
+      if (start[mid] <= k)
+        lo = mid;
+      else
+        hi = mid - 1;
+    }
+    const int r = __builtin_amdgcn_readfirstlane(replica[lo]);
+    const int block = __builtin_amdgcn_readfirstlane(k - start[lo]);
+    searchBlock<RECT>(v[r].d, block);
   }
 }
 
@@ -683,18 +749,20 @@ void launchRebuild(const Dev &d, hipStream_t s) {
 }
 
 // Batched rebuild: every replica's cell list in one workgroup of its own (a
-// batch takes replicas the single-workgroup build fits), then the lists.
-void launchRebuildBatch(const BatchView *v, const TaskTable &replicas, const TaskTable &atoms64, int rect,
-                        hipStream_t s) {
-  cellsBuildSmallBatchKernel<<<replicas.blocks, SMALL_THREADS, 0, s>>>(v, replicas.task);
-  // 2048 workgroups per XCD at most: enough to fill the chip when many
-  // replicas rebuild together
-  const int grid = atoms64.blocks < 16384 ? atoms64.blocks : 16384;
+// batch takes replicas the single-workgroup build fits), the queue of the due
+// replicas' search blocks, then the lists.
+void launchRebuildBatch(const BatchView *v, const TaskTable &replicaTable, int replicas, int *queue, int grid,
+                        int rect, hipStream_t s) {
+  cellsBuildSmallBatchKernel<<<replicaTable.blocks, SMALL_THREADS, 0, s>>>(v, replicaTable.task);
+  searchQueueBatchKernel<<<1, SEARCH_QUEUE_THREADS, 0, s>>>(v, replicas, queue);
+  const size_t lds = (size_t)(2 * replicas + 1) * sizeof(int);
   if (rect)
-    neighboursBatchKernel<true><<<grid, 256, 0, s>>>(v, atoms64.task, atoms64.blocks);
+    neighboursBatchKernel<true><<<grid, 256, lds, s>>>(v, queue, replicas);
   else
-    neighboursBatchKernel<false><<<grid, 256, 0, s>>>(v, atoms64.task, atoms64.blocks);
+    neighboursBatchKernel<false><<<grid, 256, lds, s>>>(v, queue, replicas);
 }
+
+int searchQueueInts(int replicas) { return 3 + 2 * replicas; }
 
 bool fitsSmallCellBuild(int atoms, int cells) { return atoms <= SMALL_MAX_ATOMS && cells <= SMALL_MAX_CELLS; }
 
