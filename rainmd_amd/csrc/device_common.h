@@ -104,8 +104,15 @@ __device__ __forceinline__ float minimumImageR2(float &dx, float &dy, float &dz,
 // from the FP64 box by up to half an ulp, a bias every boundary-crossing pair
 // shares, so the energies take the separation from the positions and the
 // FP64 rows with the images the FP32 test chose.
+template <bool RECT>
 __device__ __forceinline__ double imageR2Double(float4 pi, float4 pj, const BoxF &b, float s0, float s1, float s2) {
   const double *r = b.box64;
+  if (RECT) {  // the same sums without the off-diagonal rows' zero terms
+    const double ex = ((double)pi.x - (double)pj.x) - s0 * r[0];
+    const double ey = ((double)pi.y - (double)pj.y) - s1 * r[4];
+    const double ez = ((double)pi.z - (double)pj.z) - s2 * r[8];
+    return ex * ex + ey * ey + ez * ez;
+  }
   const double ex = ((double)pi.x - (double)pj.x) - s2 * r[6] - s1 * r[3] - s0 * r[0];
   const double ey = ((double)pi.y - (double)pj.y) - s2 * r[7] - s1 * r[4] - s0 * r[1];
   const double ez = ((double)pi.z - (double)pj.z) - s2 * r[8] - s1 * r[5] - s0 * r[2];

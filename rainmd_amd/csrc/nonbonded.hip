@@ -95,11 +95,17 @@ __device__ __forceinline__ void pairTerm(const float4 &pi, float qi, int ti, con
     // FP64 at the same image: the FP32 box rows, the hardware reciprocals'
     // one-ulp bias and the FP32 table would otherwise be systematic over the
     // box, and a water box's LJ total is a small difference of large sums
+    // 1/r from the hardware FP64 reciprocal square root and one Newton step
+    // (relative error about the square of the estimate's), in place of a
+    // correctly rounded division and square root
     const double2 ab64 = lj64[ti + tj];
-    const double ir2 = 1.0 / imageR2Double(pi, pj, box, s0, s1, s2);
+    const double r2d = imageR2Double<RECT>(pi, pj, box, s0, s1, s2);
+    const double y = __builtin_amdgcn_rsq(r2d);
+    const double ir = fma(0.5 * y, fma(-r2d * y, y, 1.0), y);
+    const double ir2 = ir * ir;
     const double ir6 = ir2 * ir2 * ir2;
     elj += (ab64.x * ir6 - ab64.y) * ir6;
-    ecoul += (double)(qq * screened) * sqrt(ir2);
+    ecoul += (double)(qq * screened) * ir;
     dscale -= (double)(scalar * r2);  // r dU/dr
   }
   // a select, not a product: a pair that is not computed may hold r2 = 0
