@@ -236,10 +236,19 @@ __device__ void driftWater(Atom *w, double h, const double4 geom) {
 
 constexpr int STAR_ATOMS = STAR_MAX + 1;
 
+// 1/x from the hardware FP64 reciprocal refined by one Newton step (about one
+// ulp): the sweeps below divide on their serial dependency chain, where a
+// correctly rounded division is several times longer.
+__device__ __forceinline__ double reciprocal(double x) {
+  const double y = __builtin_amdgcn_rcp(x);
+  return fma(fma(-x, y, 1.0), y, y);
+}
+
 // Every slot is loaded (slots past the cluster repeat its last atom) and the
 // unused ones are then given zero inverse mass, velocity and force: no branch
 // around the loads, so the compiler keeps the whole cluster in registers.
-__device__ void starLoad(const Dev &d, int cluster, int m, Atom *w, int *atom, double *target) {
+// t2 and invT2: each constraint's squared length and its reciprocal.
+__device__ void starLoad(const Dev &d, int cluster, int m, Atom *w, int *atom, double *t2, double *invT2) {
   const int ab = d.clusterAtomStart[cluster], cb = d.clusterConStart[cluster];
 #pragma unroll
   for (int k = 0; k < STAR_ATOMS; ++k) {
@@ -253,10 +262,14 @@ __device__ void starLoad(const Dev &d, int cluster, int m, Atom *w, int *atom, d
     }
   }
 #pragma unroll
-  for (int k = 0; k < STAR_MAX; ++k) target[k] = d.clusterConD[cb + (k < m ? k : m - 1)];
+  for (int k = 0; k < STAR_MAX; ++k) {
+    const double target = d.clusterConD[cb + (k < m ? k : m - 1)];
+    t2[k] = target * target;
+    invT2[k] = reciprocal(t2[k]);
+  }
 }
 
-__device__ void starRattle(const Dev &d, const IntegratorParams &ip, Atom *w, int m, const double *target) {
+__device__ void starRattle(const Dev &d, const IntegratorParams &ip, Atom *w, int m, const double *invT2) {
   bool converged = false, stalled = false;
   double previous = 1e300;
   for (int it = 0; it < ip.maxIterations; ++it) {
@@ -268,12 +281,12 @@ __device__ void starRattle(const Dev &d, const IntegratorParams &ip, Atom *w, in
       const double r[3] = {a.x[0] - b.x[0], a.x[1] - b.x[1], a.x[2] - b.x[2]};
       const double vr[3] = {a.v[0] - b.v[0], a.v[1] - b.v[1], a.v[2] - b.v[2]};
       const double proj = r[0] * vr[0] + r[1] * vr[1] + r[2] * vr[2];
-      const double rel = fabs(proj) / (target[k] * target[k]);
+      const double rel = fabs(proj) * invT2[k];
       worst = fmax(worst, rel);
       if (rel <= ip.tolerance) continue;
       const double den = (a.im + b.im) * (r[0] * r[0] + r[1] * r[1] + r[2] * r[2]);
       if (den < 1e-20) continue;
-      const double lam = -proj / den;
+      const double lam = -proj * reciprocal(den);
       for (int c = 0; c < 3; ++c) {
         a.v[c] += lam * a.im * r[c];
         b.v[c] -= lam * b.im * r[c];
@@ -294,7 +307,8 @@ __device__ void starRattle(const Dev &d, const IntegratorParams &ip, Atom *w, in
 
 // A with SHAKE: half drift, then the position constraints along the bond
 // vectors before the drift; the velocity takes the constraint displacement.
-__device__ void starDrift(const Dev &d, const IntegratorParams &ip, Atom *w, int m, const double *target, double h) {
+__device__ void starDrift(const Dev &d, const IntegratorParams &ip, Atom *w, int m, const double *t2,
+                          const double *invT2, double h) {
   double ref[STAR_ATOMS][3], corr[STAR_ATOMS][3];
 #pragma unroll
   for (int k = 0; k < STAR_ATOMS; ++k)
@@ -312,16 +326,15 @@ __device__ void starDrift(const Dev &d, const IntegratorParams &ip, Atom *w, int
     for (int k = 0; k < STAR_MAX; ++k) {
       if (k >= m) continue;
       Atom &a = w[k + 1], &b = w[0];
-      const double t2 = target[k] * target[k];
       const double dx = a.x[0] - b.x[0], dy = a.x[1] - b.x[1], dz = a.x[2] - b.x[2];
-      const double err = dx * dx + dy * dy + dz * dz - t2;
-      const double rel = fabs(err) / t2;
+      const double err = dx * dx + dy * dy + dz * dz - t2[k];
+      const double rel = fabs(err) * invT2[k];
       worst = fmax(worst, rel);
       if (rel <= ip.tolerance) continue;
       const double r[3] = {ref[k + 1][0] - ref[0][0], ref[k + 1][1] - ref[0][1], ref[k + 1][2] - ref[0][2]};
       const double den = 2.0 * (a.im + b.im) * (dx * r[0] + dy * r[1] + dz * r[2]);
       if (fabs(den) < 1e-20) continue;
-      const double lam = -err / den;
+      const double lam = -err * reciprocal(den);
       for (int c = 0; c < 3; ++c) {
         a.x[c] += lam * a.im * r[c];
         b.x[c] -= lam * b.im * r[c];
@@ -598,22 +611,22 @@ __device__ __forceinline__ void integrateOpen(const Dev &d, const IntegratorPara
       const int m = desc.z;
       Atom w[STAR_ATOMS];
       int atom[STAR_ATOMS];
-      double target[STAR_MAX];
-      starLoad(d, desc.y, m, w, atom, target);
+      double t2[STAR_MAX], invT2[STAR_MAX];
+      starLoad(d, desc.y, m, w, atom, t2, invT2);
       if (closing) {
 #pragma unroll
         for (int k = 0; k < STAR_ATOMS; ++k) kick(w[k], h);
-        starRattle(d, ip, w, m, target);
+        starRattle(d, ip, w, m, invT2);
       }
 #pragma unroll
       for (int k = 0; k < STAR_ATOMS; ++k) kick(w[k], h);
-      starRattle(d, ip, w, m, target);
-      starDrift(d, ip, w, m, target, h);
+      starRattle(d, ip, w, m, invT2);
+      starDrift(d, ip, w, m, t2, invT2, h);
 #pragma unroll
       for (int k = 0; k < STAR_ATOMS; ++k)
         if (k <= m) thermostat(d, ip, atom[k], w[k], step);
-      starRattle(d, ip, w, m, target);
-      starDrift(d, ip, w, m, target, h);
+      starRattle(d, ip, w, m, invT2);
+      starDrift(d, ip, w, m, t2, invT2, h);
 #pragma unroll
       for (int k = 0; k < STAR_ATOMS; ++k)
         if (k <= m) {
@@ -725,11 +738,11 @@ __device__ __forceinline__ void integrateClose(const Dev &d, const IntegratorPar
       const int m = desc.z;
       Atom w[STAR_ATOMS];
       int atom[STAR_ATOMS];
-      double target[STAR_MAX];
-      starLoad(d, desc.y, m, w, atom, target);
+      double t2[STAR_MAX], invT2[STAR_MAX];
+      starLoad(d, desc.y, m, w, atom, t2, invT2);
 #pragma unroll
       for (int k = 0; k < STAR_ATOMS; ++k) kick(w[k], h);
-      starRattle(d, ip, w, m, target);
+      starRattle(d, ip, w, m, invT2);
 #pragma unroll
       for (int k = 0; k < STAR_ATOMS; ++k)
         if (k <= m) {
