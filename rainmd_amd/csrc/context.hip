@@ -453,17 +453,18 @@ __global__ void decideRebuildKernel(Scalars *sc, double dispHalf, int force) {
 __global__ void setForceRebuildKernel(Scalars *sc) { sc->forceRebuild = 1u; }
 __global__ void setStepKernel(Scalars *sc, long long step) { sc->step = step; }
 
-// One force evaluation. The neighbour-list branch (rebuild + direct space)
-// stays on the context's stream; PME and the bonded terms read only the
-// positions, so they run on side streams forked after the integrator and
-// joined before anything that reads the forces. Forces are int64 atomics and
-// every energy has its own slot row, so the results do not depend on the
-// overlap.
+// One force evaluation. PME, the longest branch, stays on the context's
+// stream; the neighbour-list branch (rebuild + direct space) and the bonded
+// terms read only the positions, so they run on side streams forked after the
+// integrator and joined before anything that reads the forces. Forces are
+// int64 atomics and every energy has its own slot row, so the results do not
+// depend on the overlap.
 void enqueueForces(rmd_ctx *c, int terms, int energies, int slots[E_TERM_COUNT]) {
   Dev &d = c->d;
   for (int k = 0; k < E_TERM_COUNT; ++k) slots[k] = 0;
   const bool fork = c->concurrent;
-  hipStream_t pmeStream = fork ? c->side[0] : c->stream, bondedStream = fork ? c->side[1] : c->stream;
+  hipStream_t pmeStream = c->stream, listStream = fork ? c->side[0] : c->stream,
+              bondedStream = fork ? c->side[1] : c->stream;
   if (fork) {
     HIP_CHECK(hipEventRecord(c->fork, c->stream));
     HIP_CHECK(hipStreamWaitEvent(c->side[0], c->fork, 0));
@@ -480,10 +481,10 @@ void enqueueForces(rmd_ctx *c, int terms, int energies, int slots[E_TERM_COUNT])
     slots[E_BOND] = slots[E_ANGLE] = slots[E_TORSION] = slots[E_LJ_EXCEPTION] = slots[E_COUL_EXCEPTION] =
         slots[E_RESTRAINT] = slots[E_DSCALE_BONDED] = b;
   }
-  launchRebuild(d, c->stream);
+  launchRebuild(d, listStream);
   if (terms & RMD_TERM_DIRECT) {
     const int b = launchNonbonded(d, (float)c->pme.alpha, ewaldRationalFits(c->pme.alpha, c->sys.cutoff) ? 1 : 0,
-                                  c->lanes, energies, c->stream);
+                                  c->lanes, energies, listStream);
     slots[E_LJ_DIRECT] = slots[E_COUL_DIRECT] = slots[E_DSCALE_DIRECT] = b;
   }
   if (fork) {
