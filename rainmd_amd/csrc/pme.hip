@@ -393,14 +393,18 @@ __device__ __forceinline__ void planeRanges(const PmeDev &p, int i, int2 range[2
 }
 
 // One round of a plane's atom scan: of atoms [a0, a0 + PLANE_LIST), those
-// whose x support meets `plane` are appended to `list` as (atom << 3 | weight
-// index), wave-aggregated into the shared `count`. The expensive per-atom work
-// then runs over the compacted list with every lane busy (about 5 in K0 atoms
-// land on a given plane).
-constexpr int SCAN_PER_THREAD = 4;
+// whose x support meets `plane` are appended to `list` as 16 bits, (atom - a0)
+// << 3 | weight index, wave-aggregated into the shared `count`. The expensive
+// per-atom work then runs over the compacted list with every lane busy (about
+// 5 in K0 atoms land on a given plane). Six atoms per thread: a 3,072-atom
+// system scans in one round (each round is a global-load latency and three
+// barriers on the kernel's serial path), in less LDS than 2,048 int entries.
+constexpr int SCAN_PER_THREAD = 6;
 constexpr int PLANE_LIST = PLANE_THREADS * SCAN_PER_THREAD;
+static_assert(PLANE_LIST <= (1 << 13), "a list entry holds the atom's offset in its round in 13 bits");
+typedef unsigned short PlaneEntry;
 
-__device__ void planeScan(const Dev &d, const PmeDev &p, int plane, int a0, int *list, int *count) {
+__device__ void planeScan(const Dev &d, const PmeDev &p, int plane, int a0, PlaneEntry *list, int *count) {
   const double *recip = d.box->recip64;
   const int lane = threadIdx.x & 63;
   float4 batch[SCAN_PER_THREAD];
@@ -424,7 +428,7 @@ __device__ void planeScan(const Dev &d, const PmeDev &p, int plane, int a0, int 
     int base = 0;
     if (lane == (int)(__ffsll((long long)mask) - 1)) base = atomicAdd(count, __popcll(mask));
     base = __shfl(base, __ffsll((long long)mask) - 1, 64);
-    if (take) list[base + __popcll(mask & ((1ull << lane) - 1ull))] = (a << 3) | r;
+    if (take) list[base + __popcll(mask & ((1ull << lane) - 1ull))] = (PlaneEntry)(((a - a0) << 3) | r);
   }
 }
 
@@ -440,8 +444,8 @@ __device__ __forceinline__ void pmePlaneForward(const Dev &d, const PmeDev &p, c
   float2 *tw1 = lds, *tw2 = tw1 + K1;
   float2 *A = tw2 + K2, *B = A + S;
   long long *plane = reinterpret_cast<long long *>(B);
-  int *list = reinterpret_cast<int *>(B + p.planeRegion);
-  int *listCount = list + PLANE_LIST;
+  PlaneEntry *list = reinterpret_cast<PlaneEntry *>(B + p.planeRegion);
+  int *listCount = reinterpret_cast<int *>(list + PLANE_LIST);
   loadTwiddles(tw1, p.twiddle[1], K1);
   loadTwiddles(tw2, p.twiddle[2], K2);
   if (real) {
@@ -495,7 +499,7 @@ __device__ __forceinline__ void pmePlaneForward(const Dev &d, const PmeDev &p, c
         planeScan(d, p, i, a0, list, listCount);
         __syncthreads();
         const int count = *listCount;
-        for (int k = t; k < count; k += nt) spread(list[k] >> 3, list[k] & 7);
+        for (int k = t; k < count; k += nt) spread(a0 + (list[k] >> 3), list[k] & 7);
         __syncthreads();
       }
     }
@@ -611,8 +615,8 @@ __device__ __forceinline__ void pmePlaneGather(const Dev &d, const PmeDev &p, in
   const int t = threadIdx.x, nt = blockDim.x;
   float2 *tw1 = lds, *tw2 = tw1 + K1;
   float2 *A = tw2 + K2, *B = A + S;
-  int *list = reinterpret_cast<int *>(B + S);
-  int *listCount = list + PLANE_LIST;
+  PlaneEntry *list = reinterpret_cast<PlaneEntry *>(B + S);
+  int *listCount = reinterpret_cast<int *>(list + PLANE_LIST);
   loadTwiddles(tw1, p.twiddle[1], K1);
   loadTwiddles(tw2, p.twiddle[2], K2);
   const float2 *in = p.cgrid + (size_t)i * K1 * K2h;
@@ -665,7 +669,7 @@ __device__ __forceinline__ void pmePlaneGather(const Dev &d, const PmeDev &p, in
     __syncthreads();
     const int count = *listCount;
     for (int k = t; k < count; k += nt) {
-      const int a = list[k] >> 3, r = list[k] & 7;
+      const int a = a0 + (list[k] >> 3), r = list[k] & 7;
       const float4 pos = d.pos[a];
       int bx, by, bz;
       float ox, oy, oz;
@@ -1037,7 +1041,7 @@ void pmeComputeInfluence(const PmeDev &p, const BoxF *box, const double *moduli[
 
 // LDS: twiddles, two complex work buffers (the second also holds the int64
 // charge plane), then the compacted atom list and its count.
-static size_t listBytes() { return (size_t)(PLANE_LIST + 4) * sizeof(int); }
+static size_t listBytes() { return (size_t)PLANE_LIST * sizeof(PlaneEntry) + 4 * sizeof(int); }
 size_t planeForwardLds(const PmeDev &p) {
   return (size_t)(p.K[1] + p.K[2] + p.planeSpan + p.planeRegion) * sizeof(float2) + listBytes();
 }
