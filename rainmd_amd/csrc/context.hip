@@ -456,9 +456,11 @@ __global__ void setStepKernel(Scalars *sc, long long step) { sc->step = step; }
 // One force evaluation. PME, the longest branch, stays on the context's
 // stream; the neighbour-list branch (rebuild + direct space) and the bonded
 // terms read only the positions, so they run on side streams forked after the
-// integrator and joined before anything that reads the forces. Forces are
-// int64 atomics and every energy has its own slot row, so the results do not
-// depend on the overlap.
+// integrator and joined before anything that reads the forces. The bonded
+// terms join the direct space's launch where its lane count allows (a
+// captured step runs the branches one after another; one launch overlaps
+// them). Forces are int64 atomics and every energy has its own slot row, so
+// the results do not depend on the overlap.
 void enqueueForces(rmd_ctx *c, int terms, int energies, int slots[E_TERM_COUNT]) {
   Dev &d = c->d;
   for (int k = 0; k < E_TERM_COUNT; ++k) slots[k] = 0;
@@ -475,16 +477,20 @@ void enqueueForces(rmd_ctx *c, int terms, int energies, int slots[E_TERM_COUNT])
     launchPme(d, c->p, energies, pmeStream, &b);
     slots[E_RECIPROCAL] = slots[E_DSCALE_RECIPROCAL] = b;
   }
-  if (terms & (RMD_TERM_BONDED | RMD_TERM_EXCEPTIONS | RMD_TERM_RESTRAINT)) {
-    int b = 0;
-    launchBonded(d, c->pme.alpha, terms, energies, bondedStream, &b);
+  const bool bondedTerms = (terms & (RMD_TERM_BONDED | RMD_TERM_EXCEPTIONS | RMD_TERM_RESTRAINT)) != 0;
+  const bool joined = bondedTerms && (terms & RMD_TERM_DIRECT) && nonbondedJoinsBonded(c->lanes, d.n);
+  if (bondedTerms) {
+    int b = bondedBlocks(d);
+    if (!joined) launchBonded(d, c->pme.alpha, terms, energies, bondedStream, &b);
     slots[E_BOND] = slots[E_ANGLE] = slots[E_TORSION] = slots[E_LJ_EXCEPTION] = slots[E_COUL_EXCEPTION] =
         slots[E_RESTRAINT] = slots[E_DSCALE_BONDED] = b;
   }
   launchRebuild(d, listStream);
   if (terms & RMD_TERM_DIRECT) {
-    const int b = launchNonbonded(d, (float)c->pme.alpha, ewaldRationalFits(c->pme.alpha, c->sys.cutoff) ? 1 : 0,
-                                  c->lanes, energies, listStream);
+    const int rational = ewaldRationalFits(c->pme.alpha, c->sys.cutoff) ? 1 : 0;
+    const int b = joined ? launchNonbondedWithBonded(d, (float)c->pme.alpha, rational, c->lanes, energies,
+                                                     c->pme.alpha, terms, listStream)
+                         : launchNonbonded(d, (float)c->pme.alpha, rational, c->lanes, energies, listStream);
     slots[E_LJ_DIRECT] = slots[E_COUL_DIRECT] = slots[E_DSCALE_DIRECT] = b;
   }
   if (fork) {

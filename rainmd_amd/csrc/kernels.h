@@ -24,6 +24,12 @@ void launchRefreshSlots(const Dev &d, hipStream_t s);
 // within its range, ewaldRationalFits), else from erfc.
 bool ewaldRationalFits(double alpha, double cutoff);
 int launchNonbonded(const Dev &d, float alpha, int rational, int lanes, int energies, hipStream_t s);
+// The direct space with the bonded terms joined to its launch, for lanes 16,
+// 32 and 64 and small systems (nonbondedJoinsBonded); returns the direct
+// space's workgroups, the bonded terms taking bondedBlocks(d) more.
+bool nonbondedJoinsBonded(int lanes, int atoms);
+int launchNonbondedWithBonded(const Dev &d, float alpha, int rational, int lanes, int energies, double alphaD,
+                              int terms, hipStream_t s);
 
 // ---- bonded, exceptions, restraints (bonded.hip) ----
 int launchBonded(const Dev &d, double alpha, int terms, int energies, hipStream_t s, int *slotsUsed);

@@ -11,6 +11,7 @@
 // from the FP32 distance - a small box's LJ total is a large cancellation
 // that FP32 terms and sums would not keep to 1e-6 - halved (each pair appears
 // in both atoms' lists) and reduced in FP64 into per-block slots.
+#include "bonded_terms.h"
 #include "kernels.h"
 
 // Multiply-adds fuse only within one expression, so each body rounds the same
@@ -225,6 +226,21 @@ __global__ void __launch_bounds__(256) nonbondedKernel(Dev d, float alpha) {
   nonbonded<LANES, ENERGIES, LDS_TABLE, RECT, RATIONAL>(d, alpha, blockIdx.x, ldsTable);
 }
 
+// The direct space and the bonded terms of one system in one launch:
+// workgroups [0, pairBlocks) run the direct space, the rest the bonded terms,
+// each with the same body as its own kernel. A captured step runs its force
+// branches one after another, so two launches cost the sum of their
+// durations, where one launch overlaps them.
+template <int LANES, bool ENERGIES, bool LDS_TABLE, bool RECT, bool RATIONAL>
+__global__ void __launch_bounds__(256)
+    nonbondedBondedKernel(Dev d, float alpha, double alphaD, int terms, int pairBlocks) {
+  extern __shared__ float2 ldsTable[];
+  if ((int)blockIdx.x < pairBlocks)
+    nonbonded<LANES, ENERGIES, LDS_TABLE, RECT, RATIONAL>(d, alpha, blockIdx.x, ldsTable);
+  else
+    bonded(d, alphaD, terms, ENERGIES ? 1 : 0, (int)blockIdx.x - pairBlocks);
+}
+
 // Batched.
 template <int LANES, bool ENERGIES, bool LDS_TABLE, bool RECT, bool RATIONAL>
 __global__ void __launch_bounds__(256) nonbondedBatchKernel(const BatchView *__restrict__ v,
@@ -235,8 +251,34 @@ __global__ void __launch_bounds__(256) nonbondedBatchKernel(const BatchView *__r
   nonbonded<LANES, ENERGIES, LDS_TABLE, RECT, RATIONAL>(v[t.x].d, alpha, t.y, ldsTable);
 }
 
+// The bonded terms joined to a single system's direct-space launch (no
+// workgroups: none joined).
+struct BondedPart {
+  int blocks;
+  double alpha;
+  int terms;
+};
+
 template <bool ENERGIES, bool LDS_TABLE, bool RECT, bool RATIONAL>
-static void launchLanes(const Dev &d, float alpha, int lanes, int blocks, size_t lds, hipStream_t s) {
+static void launchLanes(const Dev &d, float alpha, int lanes, int blocks, size_t lds, hipStream_t s,
+                        const BondedPart &b) {
+  if (b.blocks > 0) {
+    const int all = blocks + b.blocks;
+    switch (lanes) {
+      case 16:
+        nonbondedBondedKernel<16, ENERGIES, LDS_TABLE, RECT, RATIONAL><<<all, 256, lds, s>>>(d, alpha, b.alpha, b.terms,
+                                                                                           blocks);
+        return;
+      case 32:
+        nonbondedBondedKernel<32, ENERGIES, LDS_TABLE, RECT, RATIONAL><<<all, 256, lds, s>>>(d, alpha, b.alpha, b.terms,
+                                                                                           blocks);
+        return;
+      case 64:
+        nonbondedBondedKernel<64, ENERGIES, LDS_TABLE, RECT, RATIONAL><<<all, 256, lds, s>>>(d, alpha, b.alpha, b.terms,
+                                                                                           blocks);
+        return;
+    }
+  }
   switch (lanes) {
     case 1: nonbondedKernel<1, ENERGIES, LDS_TABLE, RECT, RATIONAL><<<blocks, 256, lds, s>>>(d, alpha); break;
     case 2: nonbondedKernel<2, ENERGIES, LDS_TABLE, RECT, RATIONAL><<<blocks, 256, lds, s>>>(d, alpha); break;
@@ -275,21 +317,21 @@ struct Variant {
 
 template <bool E, bool L, bool R, bool Q>
 static void launchOne(bool batch, const Dev *d, const BatchView *v, const TaskTable *t, float alpha, int lanes,
-                      int blocks, size_t lds, hipStream_t s) {
+                      int blocks, size_t lds, hipStream_t s, const BondedPart &b) {
   if (batch)
     launchBatchLanes<E, L, R, Q>(v, *t, lanes, alpha, lds, s);
   else
-    launchLanes<E, L, R, Q>(*d, alpha, lanes, blocks, lds, s);
+    launchLanes<E, L, R, Q>(*d, alpha, lanes, blocks, lds, s, b);
 }
 
 static void dispatch(const Variant &k, bool batch, const Dev *d, const BatchView *v, const TaskTable *t, float alpha,
-                     int lanes, int blocks, size_t lds, hipStream_t s) {
+                     int lanes, int blocks, size_t lds, hipStream_t s, const BondedPart &b = BondedPart{0, 0.0, 0}) {
   const int code = (k.energies ? 8 : 0) | (k.lds ? 4 : 0) | (k.rect ? 2 : 0) | (k.rational ? 1 : 0);
   switch (code) {
 #define RMD_NB_CASE(C)                                                                                     \
   case C:                                                                                                  \
     launchOne<(C & 8) != 0, (C & 4) != 0, (C & 2) != 0, (C & 1) != 0>(batch, d, v, t, alpha, lanes, blocks, \
-                                                                       lds, s);                           \
+                                                                       lds, s, b);                        \
     break;
     RMD_NB_CASE(0) RMD_NB_CASE(1) RMD_NB_CASE(2) RMD_NB_CASE(3) RMD_NB_CASE(4) RMD_NB_CASE(5) RMD_NB_CASE(6)
     RMD_NB_CASE(7) RMD_NB_CASE(8) RMD_NB_CASE(9) RMD_NB_CASE(10) RMD_NB_CASE(11) RMD_NB_CASE(12) RMD_NB_CASE(13)
@@ -303,6 +345,25 @@ int launchNonbonded(const Dev &d, float alpha, int rational, int lanes, int ener
   const int blocks = (int)((threads + 255) / 256);
   const Variant k{energies != 0, tableInLds(d.types), d.rect != 0, rational != 0};
   dispatch(k, false, &d, nullptr, nullptr, alpha, lanes, blocks, tableLdsBytes(d.types), s);
+  return blocks;
+}
+
+// The joined kernel holds the bonded body's registers (92 VGPRs against the
+// direct space's 63 at 64 lanes, so five waves per SIMD rather than eight): a
+// system joins when its direct space fits in one pass at four waves per SIMD
+// of the 256 CUs, as small systems' launch-bound steps do.
+bool nonbondedJoinsBonded(int lanes, int atoms) {
+  constexpr long long ONE_PASS = 256LL * 4 * 4 * 64;  // threads
+  return (lanes == 16 || lanes == 32 || lanes == 64) && (long long)atoms * lanes <= ONE_PASS;
+}
+
+int launchNonbondedWithBonded(const Dev &d, float alpha, int rational, int lanes, int energies, double alphaD,
+                              int terms, hipStream_t s) {
+  const long long threads = (long long)d.n * lanes;
+  const int blocks = (int)((threads + 255) / 256);
+  const Variant k{energies != 0, tableInLds(d.types), d.rect != 0, rational != 0};
+  dispatch(k, false, &d, nullptr, nullptr, alpha, lanes, blocks, tableLdsBytes(d.types), s,
+           BondedPart{bondedBlocks(d), alphaD, terms});
   return blocks;
 }
 
