@@ -364,9 +364,11 @@ int rmd_debug_fft_forward(rmd_ctx *ctx, const float *real, float *out);
  * timing on that stream). */
 int rmd_set_lanes(rmd_ctx *ctx, int32_t lanes);
 int rmd_set_graphs(rmd_ctx *ctx, int32_t enabled);
-/* Force branches on concurrent streams (1, the default: PME and the bonded
- * terms overlap the neighbour list and the direct space) or all on the
- * context's stream (0). Results are identical either way. */
+/* Overlapped force branches (1, the default) or each branch launched on its
+ * own, one after another on the context's stream (0). Overlapped, a small
+ * system runs its direct space, bonded terms and PME forward planes as one
+ * launch; a larger one runs PME and the bonded terms on streams beside the
+ * neighbour list and the direct space. Results are identical either way. */
 int rmd_set_concurrent_forces(rmd_ctx *ctx, int32_t enabled);
 /* PME pipeline: 0 automatic (planes when one x plane of the grid fits in
  * LDS), 1 planes (RMD_E_INVALID when it does not fit), 2 lines. */

@@ -27,7 +27,7 @@ __global__ void __launch_bounds__(256) bondedBatchKernel(const BatchView *__rest
 }
 
 int bondedBlocks(const Dev &d) {
-  return (d.nBond + d.nAngle + d.nTorsion + exceptionThreads(d) + d.nRestraint + 255) / 256;
+  return (bondedThreads(d) + 255) / 256;
 }
 
 void launchBondedBatch(const BatchView *v, const TaskTable &t, double alpha, int terms, int energies, hipStream_t s) {

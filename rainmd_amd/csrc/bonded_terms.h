@@ -249,6 +249,11 @@ enum { BT_BOND = 0, BT_ANGLE, BT_TORSION, BT_LJX, BT_COULX, BT_RESTRAINT, BT_DSC
 // One thread per exception triplet, then one per remaining exception.
 __host__ __device__ inline int exceptionThreads(const Dev &d) { return d.nException - 2 * d.nExcTriplets; }
 
+// Threads of the bonded launch: one per term.
+__host__ __device__ inline int bondedThreads(const Dev &d) {
+  return d.nBond + d.nAngle + d.nTorsion + exceptionThreads(d) + d.nRestraint;
+}
+
 __device__ __forceinline__ void bonded(const Dev &d, double alphaD, int terms, int energies, int block) {
   const BoxF &box = *d.box;
   const int t = block * blockDim.x + threadIdx.x;
@@ -278,24 +283,30 @@ __device__ __forceinline__ void bonded(const Dev &d, double alphaD, int terms, i
   }
   if (!energies) return;
   e[BT_DSCALE] = -vir;
-  __shared__ double partial[BT_COUNT][4];
+  __shared__ double partial[BT_COUNT][8];  // up to 512 threads
   const int wave = threadIdx.x >> 6;
   for (int k = 0; k < BT_COUNT; ++k) {
     const double v = waveSum(e[k]);
     if ((threadIdx.x & 63) == 0) partial[k][wave] = v;
   }
   __syncthreads();
-  if (threadIdx.x < BT_COUNT) {
+  // One slot per 256 threads, numbered as a 256-thread launch numbers its
+  // workgroups, whatever this launch's workgroup size: the slot sums, and so
+  // the energies, are bit for bit those of the 256-thread kernels.
+  const int group = threadIdx.x >> 8, k = threadIdx.x & 255;
+  if (k < BT_COUNT) {
+    const int slot = block * (int)(blockDim.x >> 8) + group;
+    const int waves = min(4, (int)(blockDim.x >> 6) - 4 * group);
     double s = 0;
-    for (int w = 0; w < (int)(blockDim.x >> 6); ++w) s += partial[threadIdx.x][w];
-    const int term = threadIdx.x == BT_BOND ? E_BOND
-                     : threadIdx.x == BT_ANGLE ? E_ANGLE
-                     : threadIdx.x == BT_TORSION ? E_TORSION
-                     : threadIdx.x == BT_LJX ? E_LJ_EXCEPTION
-                     : threadIdx.x == BT_COULX ? E_COUL_EXCEPTION
-                     : threadIdx.x == BT_RESTRAINT ? E_RESTRAINT
-                                                   : E_DSCALE_BONDED;
-    d.slots[term * MAX_SLOTS + block] = s;
+    for (int w = 4 * group; w < 4 * group + waves; ++w) s += partial[k][w];
+    const int term = k == BT_BOND ? E_BOND
+                     : k == BT_ANGLE ? E_ANGLE
+                     : k == BT_TORSION ? E_TORSION
+                     : k == BT_LJX ? E_LJ_EXCEPTION
+                     : k == BT_COULX ? E_COUL_EXCEPTION
+                     : k == BT_RESTRAINT ? E_RESTRAINT
+                                         : E_DSCALE_BONDED;
+    if (slot < (bondedThreads(d) + 255) / 256) d.slots[term * MAX_SLOTS + slot] = s;
   }
 }
 

@@ -453,50 +453,64 @@ __global__ void decideRebuildKernel(Scalars *sc, double dispHalf, int force) {
 __global__ void setForceRebuildKernel(Scalars *sc) { sc->forceRebuild = 1u; }
 __global__ void setStepKernel(Scalars *sc, long long step) { sc->step = step; }
 
-// One force evaluation. PME, the longest branch, stays on the context's
-// stream; the neighbour-list branch (rebuild + direct space) and the bonded
-// terms read only the positions, so they run on side streams forked after the
-// integrator and joined before anything that reads the forces. The bonded
-// terms join the direct space's launch where its lane count allows (a
-// captured step runs the branches one after another; one launch overlaps
-// them). Forces are int64 atomics and every energy has its own slot row, so
-// the results do not depend on the overlap.
+// One force evaluation. A small system runs its direct space, bonded terms
+// and PME forward planes as one launch after the list rebuild, then the rest
+// of PME, all on the context's stream: a captured step runs forked branches
+// one after another, so separate launches cost the sum of their durations,
+// where one launch overlaps them. Otherwise PME, the longest branch, stays on
+// the context's stream, and the neighbour-list branch (rebuild + direct
+// space) and the bonded terms, which read only the positions, run on side
+// streams forked after the integrator and joined before anything that reads
+// the forces. Forces are int64 atomics and every energy has its own slot row,
+// so the results do not depend on the overlap.
 void enqueueForces(rmd_ctx *c, int terms, int energies, int slots[E_TERM_COUNT]) {
   Dev &d = c->d;
   for (int k = 0; k < E_TERM_COUNT; ++k) slots[k] = 0;
-  const bool fork = c->concurrent;
-  hipStream_t pmeStream = c->stream, listStream = fork ? c->side[0] : c->stream,
-              bondedStream = fork ? c->side[1] : c->stream;
-  if (fork) {
-    HIP_CHECK(hipEventRecord(c->fork, c->stream));
-    HIP_CHECK(hipStreamWaitEvent(c->side[0], c->fork, 0));
-    HIP_CHECK(hipStreamWaitEvent(c->side[1], c->fork, 0));
-  }
-  if (terms & RMD_TERM_RECIPROCAL) {
-    int b = 0;
-    launchPme(d, c->p, energies, pmeStream, &b);
-    slots[E_RECIPROCAL] = slots[E_DSCALE_RECIPROCAL] = b;
-  }
-  const bool bondedTerms = (terms & (RMD_TERM_BONDED | RMD_TERM_EXCEPTIONS | RMD_TERM_RESTRAINT)) != 0;
-  const bool joined = bondedTerms && (terms & RMD_TERM_DIRECT) && nonbondedJoinsBonded(c->lanes, d.n);
-  if (bondedTerms) {
-    int b = bondedBlocks(d);
-    if (!joined) launchBonded(d, c->pme.alpha, terms, energies, bondedStream, &b);
+  const int bondedMask = RMD_TERM_BONDED | RMD_TERM_EXCEPTIONS | RMD_TERM_RESTRAINT;
+  const int rational = ewaldRationalFits(c->pme.alpha, c->sys.cutoff) ? 1 : 0;
+  auto setBondedSlots = [&](int b) {
     slots[E_BOND] = slots[E_ANGLE] = slots[E_TORSION] = slots[E_LJ_EXCEPTION] = slots[E_COUL_EXCEPTION] =
         slots[E_RESTRAINT] = slots[E_DSCALE_BONDED] = b;
-  }
-  launchRebuild(d, listStream);
-  if (terms & RMD_TERM_DIRECT) {
-    const int rational = ewaldRationalFits(c->pme.alpha, c->sys.cutoff) ? 1 : 0;
-    const int b = joined ? launchNonbondedWithBonded(d, (float)c->pme.alpha, rational, c->lanes, energies,
-                                                     c->pme.alpha, terms, listStream)
-                         : launchNonbonded(d, (float)c->pme.alpha, rational, c->lanes, energies, listStream);
-    slots[E_LJ_DIRECT] = slots[E_COUL_DIRECT] = slots[E_DSCALE_DIRECT] = b;
-  }
-  if (fork) {
-    for (int k = 0; k < 2; ++k) {
-      HIP_CHECK(hipEventRecord(c->join[k], c->side[k]));
-      HIP_CHECK(hipStreamWaitEvent(c->stream, c->join[k], 0));
+  };
+  const bool joined = c->concurrent && (terms & RMD_TERM_DIRECT) && (terms & RMD_TERM_RECIPROCAL) &&
+                      (terms & bondedMask) && forcesJoin(d, c->p, c->lanes);
+  if (joined) {
+    launchRebuild(d, c->stream);
+    int pair = 0, bonded = 0;
+    launchForcesJoined(d, c->p, (float)c->pme.alpha, rational, c->lanes, energies, c->pme.alpha, terms, c->stream,
+                       &pair, &bonded);
+    slots[E_LJ_DIRECT] = slots[E_COUL_DIRECT] = slots[E_DSCALE_DIRECT] = pair;
+    setBondedSlots(bonded);
+    slots[E_RECIPROCAL] = slots[E_DSCALE_RECIPROCAL] = launchPmeAfterForward(d, c->p, energies, c->stream);
+  } else {
+    const bool fork = c->concurrent;
+    hipStream_t pmeStream = c->stream, listStream = fork ? c->side[0] : c->stream,
+                bondedStream = fork ? c->side[1] : c->stream;
+    if (fork) {
+      HIP_CHECK(hipEventRecord(c->fork, c->stream));
+      HIP_CHECK(hipStreamWaitEvent(c->side[0], c->fork, 0));
+      HIP_CHECK(hipStreamWaitEvent(c->side[1], c->fork, 0));
+    }
+    if (terms & RMD_TERM_RECIPROCAL) {
+      int b = 0;
+      launchPme(d, c->p, energies, pmeStream, &b);
+      slots[E_RECIPROCAL] = slots[E_DSCALE_RECIPROCAL] = b;
+    }
+    if (terms & bondedMask) {
+      int b = 0;
+      launchBonded(d, c->pme.alpha, terms, energies, bondedStream, &b);
+      setBondedSlots(b);
+    }
+    launchRebuild(d, listStream);
+    if (terms & RMD_TERM_DIRECT) {
+      const int b = launchNonbonded(d, (float)c->pme.alpha, rational, c->lanes, energies, listStream);
+      slots[E_LJ_DIRECT] = slots[E_COUL_DIRECT] = slots[E_DSCALE_DIRECT] = b;
+    }
+    if (fork) {
+      for (int k = 0; k < 2; ++k) {
+        HIP_CHECK(hipEventRecord(c->join[k], c->side[k]));
+        HIP_CHECK(hipStreamWaitEvent(c->stream, c->join[k], 0));
+      }
     }
   }
   if (energies)

@@ -24,12 +24,6 @@ void launchRefreshSlots(const Dev &d, hipStream_t s);
 // within its range, ewaldRationalFits), else from erfc.
 bool ewaldRationalFits(double alpha, double cutoff);
 int launchNonbonded(const Dev &d, float alpha, int rational, int lanes, int energies, hipStream_t s);
-// The direct space with the bonded terms joined to its launch, for lanes 16,
-// 32 and 64 and small systems (nonbondedJoinsBonded); returns the direct
-// space's workgroups, the bonded terms taking bondedBlocks(d) more.
-bool nonbondedJoinsBonded(int lanes, int atoms);
-int launchNonbondedWithBonded(const Dev &d, float alpha, int rational, int lanes, int energies, double alphaD,
-                              int terms, hipStream_t s);
 
 // ---- bonded, exceptions, restraints (bonded.hip) ----
 int launchBonded(const Dev &d, double alpha, int terms, int energies, hipStream_t s, int *slotsUsed);
@@ -64,6 +58,14 @@ constexpr int BIN_MAX_ATOMS = 16384;
 bool pmeBinsFit(const PmeDev &p, int atoms);
 void pmeComputeInfluence(const PmeDev &p, const BoxF *box, const double *moduli[3], double alpha, hipStream_t s);
 int launchPme(const Dev &d, const PmeDev &p, int energies, hipStream_t s, int *slotsUsed);
+// A small system's direct space, bonded terms and PME forward planes in one
+// launch (where forcesJoin allows), with the energy slots each part used;
+// launchPmeAfterForward then runs the rest of the plane pipeline and returns
+// its slots.
+bool forcesJoin(const Dev &d, const PmeDev &p, int lanes);
+void launchForcesJoined(const Dev &d, const PmeDev &p, float alpha, int rational, int lanes, int energies,
+                        double alphaD, int terms, hipStream_t s, int *pairSlots, int *bondedSlots);
+int launchPmeAfterForward(const Dev &d, const PmeDev &p, int energies, hipStream_t s);
 // Debug entry: forward R2C 3D FFT of a real grid into the half spectrum.
 void launchFftForward(const Dev &d, const PmeDev &p, const float *real, float2 *out, hipStream_t s);
 

@@ -13,6 +13,7 @@
 // in both atoms' lists) and reduced in FP64 into per-block slots.
 #include "bonded_terms.h"
 #include "kernels.h"
+#include "pme_plane.h"
 
 // Multiply-adds fuse only within one expression, so each body rounds the same
 // way wherever it is inlined (the single-system and the batched kernels).
@@ -195,7 +196,7 @@ __device__ __forceinline__ void nonbonded(const Dev &d, float alpha, int block, 
     atomicAddFixed(&d.force[2 * d.nPad + atom], iz);
   }
   if (ENERGIES) {
-    __shared__ double partial[3][4];
+    __shared__ double partial[3][8];  // up to 512 threads
     double a = waveSum(0.5 * elj);
     double b = waveSum(0.5 * ecoul);
     double c = waveSum(0.5 * dscale);
@@ -206,16 +207,23 @@ __device__ __forceinline__ void nonbonded(const Dev &d, float alpha, int block, 
       partial[2][wave] = c;
     }
     __syncthreads();
-    if (threadIdx.x == 0) {
+    // One slot per 256 threads, numbered as a 256-thread launch numbers its
+    // workgroups, whatever this launch's workgroup size: the slot sums, and
+    // so the energies, are bit for bit those of the 256-thread kernels.
+    if ((threadIdx.x & 255) == 0) {
+      const int group = threadIdx.x >> 8, slot = block * (int)(blockDim.x >> 8) + group;
+      const int waves = min(4, (int)(blockDim.x >> 6) - 4 * group);
       double sa = 0, sb = 0, sc = 0;
-      for (int w = 0; w < (int)(blockDim.x >> 6); ++w) {
+      for (int w = 4 * group; w < 4 * group + waves; ++w) {
         sa += partial[0][w];
         sb += partial[1][w];
         sc += partial[2][w];
       }
-      d.slots[E_LJ_DIRECT * MAX_SLOTS + block] = sa;
-      d.slots[E_COUL_DIRECT * MAX_SLOTS + block] = sb;
-      d.slots[E_DSCALE_DIRECT * MAX_SLOTS + block] = sc;
+      if (slot < (int)(((long long)d.n * LANES + 255) / 256)) {
+        d.slots[E_LJ_DIRECT * MAX_SLOTS + slot] = sa;
+        d.slots[E_COUL_DIRECT * MAX_SLOTS + slot] = sb;
+        d.slots[E_DSCALE_DIRECT * MAX_SLOTS + slot] = sc;
+      }
     }
   }
 }
@@ -226,19 +234,24 @@ __global__ void __launch_bounds__(256) nonbondedKernel(Dev d, float alpha) {
   nonbonded<LANES, ENERGIES, LDS_TABLE, RECT, RATIONAL>(d, alpha, blockIdx.x, ldsTable);
 }
 
-// The direct space and the bonded terms of one system in one launch:
-// workgroups [0, pairBlocks) run the direct space, the rest the bonded terms,
-// each with the same body as its own kernel. A captured step runs its force
-// branches one after another, so two launches cost the sum of their
-// durations, where one launch overlaps them.
+// One system's direct space, bonded terms and PME forward planes in one launch
+// of PLANE_THREADS-thread workgroups: [0, pairBlocks) run the direct space,
+// [pairBlocks, planesFrom) the bonded terms, and each workgroup after that one
+// x plane of the PME forward stage, each with the same body as its own
+// kernel. A captured step runs its force branches one after another, so
+// separate launches cost the sum of their durations, where one launch
+// overlaps them.
 template <int LANES, bool ENERGIES, bool LDS_TABLE, bool RECT, bool RATIONAL>
-__global__ void __launch_bounds__(256)
-    nonbondedBondedKernel(Dev d, float alpha, double alphaD, int terms, int pairBlocks) {
-  extern __shared__ float2 ldsTable[];
-  if ((int)blockIdx.x < pairBlocks)
-    nonbonded<LANES, ENERGIES, LDS_TABLE, RECT, RATIONAL>(d, alpha, blockIdx.x, ldsTable);
+__global__ void __launch_bounds__(PLANE_THREADS)
+    forcesJoinedKernel(Dev d, PmeDev p, float alpha, double alphaD, int terms, int pairBlocks, int planesFrom) {
+  extern __shared__ float2 lds[];
+  const int b = blockIdx.x;
+  if (b < pairBlocks)
+    nonbonded<LANES, ENERGIES, LDS_TABLE, RECT, RATIONAL>(d, alpha, b, lds);
+  else if (b < planesFrom)
+    bonded(d, alphaD, terms, ENERGIES ? 1 : 0, b - pairBlocks);
   else
-    bonded(d, alphaD, terms, ENERGIES ? 1 : 0, (int)blockIdx.x - pairBlocks);
+    pmePlaneForward(d, p, nullptr, b - planesFrom, lds);
 }
 
 // Batched.
@@ -251,32 +264,36 @@ __global__ void __launch_bounds__(256) nonbondedBatchKernel(const BatchView *__r
   nonbonded<LANES, ENERGIES, LDS_TABLE, RECT, RATIONAL>(v[t.x].d, alpha, t.y, ldsTable);
 }
 
-// The bonded terms joined to a single system's direct-space launch (no
-// workgroups: none joined).
-struct BondedPart {
-  int blocks;
+// The bonded terms and PME forward planes joined to a single system's
+// direct-space launch (p null: none joined).
+struct JoinedPart {
+  const PmeDev *p;
   double alpha;
   int terms;
 };
 
 template <bool ENERGIES, bool LDS_TABLE, bool RECT, bool RATIONAL>
 static void launchLanes(const Dev &d, float alpha, int lanes, int blocks, size_t lds, hipStream_t s,
-                        const BondedPart &b) {
-  if (b.blocks > 0) {
-    const int all = blocks + b.blocks;
-    switch (lanes) {
-      case 16:
-        nonbondedBondedKernel<16, ENERGIES, LDS_TABLE, RECT, RATIONAL><<<all, 256, lds, s>>>(d, alpha, b.alpha, b.terms,
-                                                                                           blocks);
-        return;
-      case 32:
-        nonbondedBondedKernel<32, ENERGIES, LDS_TABLE, RECT, RATIONAL><<<all, 256, lds, s>>>(d, alpha, b.alpha, b.terms,
-                                                                                           blocks);
-        return;
-      case 64:
-        nonbondedBondedKernel<64, ENERGIES, LDS_TABLE, RECT, RATIONAL><<<all, 256, lds, s>>>(d, alpha, b.alpha, b.terms,
-                                                                                           blocks);
-        return;
+                        const JoinedPart &j) {
+  if constexpr (LDS_TABLE) {  // the joined launch needs the LJ table in LDS (forcesJoin)
+    if (j.p) {
+      const int pair = (int)(((long long)d.n * lanes + PLANE_THREADS - 1) / PLANE_THREADS);
+      const int planesFrom = pair + (bondedThreads(d) + PLANE_THREADS - 1) / PLANE_THREADS;
+      const int all = planesFrom + j.p->K[0];
+      switch (lanes) {
+        case 16:
+          forcesJoinedKernel<16, ENERGIES, LDS_TABLE, RECT, RATIONAL><<<all, PLANE_THREADS, lds, s>>>(
+              d, *j.p, alpha, j.alpha, j.terms, pair, planesFrom);
+          return;
+        case 32:
+          forcesJoinedKernel<32, ENERGIES, LDS_TABLE, RECT, RATIONAL><<<all, PLANE_THREADS, lds, s>>>(
+              d, *j.p, alpha, j.alpha, j.terms, pair, planesFrom);
+          return;
+        case 64:
+          forcesJoinedKernel<64, ENERGIES, LDS_TABLE, RECT, RATIONAL><<<all, PLANE_THREADS, lds, s>>>(
+              d, *j.p, alpha, j.alpha, j.terms, pair, planesFrom);
+          return;
+      }
     }
   }
   switch (lanes) {
@@ -317,21 +334,22 @@ struct Variant {
 
 template <bool E, bool L, bool R, bool Q>
 static void launchOne(bool batch, const Dev *d, const BatchView *v, const TaskTable *t, float alpha, int lanes,
-                      int blocks, size_t lds, hipStream_t s, const BondedPart &b) {
+                      int blocks, size_t lds, hipStream_t s, const JoinedPart &j) {
   if (batch)
     launchBatchLanes<E, L, R, Q>(v, *t, lanes, alpha, lds, s);
   else
-    launchLanes<E, L, R, Q>(*d, alpha, lanes, blocks, lds, s, b);
+    launchLanes<E, L, R, Q>(*d, alpha, lanes, blocks, lds, s, j);
 }
 
 static void dispatch(const Variant &k, bool batch, const Dev *d, const BatchView *v, const TaskTable *t, float alpha,
-                     int lanes, int blocks, size_t lds, hipStream_t s, const BondedPart &b = BondedPart{0, 0.0, 0}) {
+                     int lanes, int blocks, size_t lds, hipStream_t s,
+                     const JoinedPart &j = JoinedPart{nullptr, 0.0, 0}) {
   const int code = (k.energies ? 8 : 0) | (k.lds ? 4 : 0) | (k.rect ? 2 : 0) | (k.rational ? 1 : 0);
   switch (code) {
 #define RMD_NB_CASE(C)                                                                                     \
   case C:                                                                                                  \
     launchOne<(C & 8) != 0, (C & 4) != 0, (C & 2) != 0, (C & 1) != 0>(batch, d, v, t, alpha, lanes, blocks, \
-                                                                       lds, s, b);                        \
+                                                                       lds, s, j);                        \
     break;
     RMD_NB_CASE(0) RMD_NB_CASE(1) RMD_NB_CASE(2) RMD_NB_CASE(3) RMD_NB_CASE(4) RMD_NB_CASE(5) RMD_NB_CASE(6)
     RMD_NB_CASE(7) RMD_NB_CASE(8) RMD_NB_CASE(9) RMD_NB_CASE(10) RMD_NB_CASE(11) RMD_NB_CASE(12) RMD_NB_CASE(13)
@@ -348,23 +366,25 @@ int launchNonbonded(const Dev &d, float alpha, int rational, int lanes, int ener
   return blocks;
 }
 
-// The joined kernel holds the bonded body's registers (92 VGPRs against the
-// direct space's 63 at 64 lanes, so five waves per SIMD rather than eight): a
-// system joins when its direct space fits in one pass at four waves per SIMD
-// of the 256 CUs, as small systems' launch-bound steps do.
-bool nonbondedJoinsBonded(int lanes, int atoms) {
+// The joined launch holds the bonded and PME bodies' registers (about 95
+// VGPRs against the direct space's 63 at 64 lanes): a system joins when its
+// direct space fits in one pass at four waves per SIMD of the 256 CUs, as
+// small systems' launch-bound steps do, with the LJ table in LDS, the
+// unbinned plane pipeline and at most 64 KB of LDS per workgroup.
+bool forcesJoin(const Dev &d, const PmeDev &p, int lanes) {
   constexpr long long ONE_PASS = 256LL * 4 * 4 * 64;  // threads
-  return (lanes == 16 || lanes == 32 || lanes == 64) && (long long)atoms * lanes <= ONE_PASS;
+  return (lanes == 16 || lanes == 32 || lanes == 64) && (long long)d.n * lanes <= ONE_PASS && tableInLds(d.types) &&
+         p.planes && !p.binned && std::max(tableLdsBytes(d.types), pmePlaneForwardLds(p)) <= 65536;
 }
 
-int launchNonbondedWithBonded(const Dev &d, float alpha, int rational, int lanes, int energies, double alphaD,
-                              int terms, hipStream_t s) {
-  const long long threads = (long long)d.n * lanes;
-  const int blocks = (int)((threads + 255) / 256);
-  const Variant k{energies != 0, tableInLds(d.types), d.rect != 0, rational != 0};
-  dispatch(k, false, &d, nullptr, nullptr, alpha, lanes, blocks, tableLdsBytes(d.types), s,
-           BondedPart{bondedBlocks(d), alphaD, terms});
-  return blocks;
+void launchForcesJoined(const Dev &d, const PmeDev &p, float alpha, int rational, int lanes, int energies,
+                        double alphaD, int terms, hipStream_t s, int *pairSlots, int *bondedSlots) {
+  const Variant k{energies != 0, true, d.rect != 0, rational != 0};
+  const size_t lds = std::max(tableLdsBytes(d.types), pmePlaneForwardLds(p));
+  dispatch(k, false, &d, nullptr, nullptr, alpha, lanes, 0, lds, s, JoinedPart{&p, alphaD, terms});
+  // energy slots as the 256-thread kernels number them
+  *pairSlots = (int)(((long long)d.n * lanes + 255) / 256);
+  *bondedSlots = (bondedThreads(d) + 255) / 256;
 }
 
 // `types` is the largest type count of the batch's replicas: every workgroup
