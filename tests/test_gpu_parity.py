@@ -154,8 +154,10 @@ def test_energies_and_forces_workloads(workload):
 
 @pytest.mark.parametrize("concurrent, graphs", [(True, True), (False, True), (True, False)])
 def test_steps_are_deterministic(ala_wat, concurrent, graphs):
-    # the same 20 steps twice, and against the default mode (concurrent force
-    # branches in captured graphs): bit-identical states and energies
+    # the same 20 steps twice, and against the default mode (overlapped force
+    # branches in captured graphs: for this small system the direct space,
+    # bonded terms and PME forward planes in one launch; concurrent=False runs
+    # the separate kernels): bit-identical states, forces and energies
     system, _ = ala_wat
     runs = []
     for mode in [(True, True), (concurrent, graphs)]:
