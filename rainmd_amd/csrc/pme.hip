@@ -108,7 +108,7 @@ __global__ void __launch_bounds__(PLANE_THREADS) pmePlaneForwardBatchKernel(cons
   extern __shared__ float2 lds[];
   const int2 t = task[blockIdx.x];
   if (t.x < 0) return;
-  pmePlaneForward(v[t.x].d, v[t.x].p, nullptr, t.y, lds);
+  pmePlaneForward<SCAN_BATCHED>(v[t.x].d, v[t.x].p, nullptr, t.y, lds);
 }
 
 // 2. x lines (j, m0 .. m0 + XLINES - 1): forward transform, then, with
@@ -173,6 +173,7 @@ __global__ void __launch_bounds__(XLINE_THREADS) pmeXLinesBatchKernel(const Batc
 // 3. Inverse y and z transforms of plane i into the real potential, then each
 // atom whose support meets the plane takes that plane's share of its force
 // (webgpu/kernels/pme.js:281-336; cpu/forces/pme.js:222-257).
+template <int SCAN = SCAN_PER_THREAD>
 __device__ __forceinline__ void pmePlaneGather(const Dev &d, const PmeDev &p, int i, float2 *lds) {
   const int K0 = p.K[0], K1 = p.K[1], K2 = p.K[2], K2h = p.K2h;
   const int S = p.planeSpan, P = (K1 + 1) / 2;
@@ -226,10 +227,10 @@ __device__ __forceinline__ void pmePlaneGather(const Dev &d, const PmeDev &p, in
     return;
   }
   const double *recip = d.box->recip64;
-  for (int a0 = 0; a0 < d.n; a0 += PLANE_LIST) {
+  for (int a0 = 0; a0 < d.n; a0 += PLANE_THREADS * SCAN) {
     if (t == 0) *listCount = 0;
     __syncthreads();
-    planeScan(d, p, i, a0, list, listCount);
+    planeScan<SCAN>(d, p, i, a0, list, listCount);
     __syncthreads();
     const int count = *listCount;
     for (int k = t; k < count; k += nt) {
@@ -275,12 +276,14 @@ __global__ void __launch_bounds__(PLANE_THREADS) pmePlaneGatherKernel(Dev d, Pme
   pmePlaneGather(d, p, blockIdx.x, lds);
 }
 
-__global__ void __launch_bounds__(PLANE_THREADS) pmePlaneGatherBatchKernel(const BatchView *__restrict__ v,
-                                                                         const int2 *__restrict__ task) {
+// (Six waves per SIMD: three workgroups per CU, the register count the body
+// needs without spills.)
+__global__ void __launch_bounds__(PLANE_THREADS) __attribute__((amdgpu_waves_per_eu(6)))
+pmePlaneGatherBatchKernel(const BatchView *__restrict__ v, const int2 *__restrict__ task) {
   extern __shared__ float2 lds[];
   const int2 t = task[blockIdx.x];
   if (t.x < 0) return;
-  pmePlaneGather(v[t.x].d, v[t.x].p, t.y, lds);
+  pmePlaneGather<SCAN_BATCHED>(v[t.x].d, v[t.x].p, t.y, lds);
 }
 
 // The gather of a binned system, one thread per charged atom in bin order (a

@@ -303,29 +303,34 @@ __device__ __forceinline__ void planeRanges(const PmeDev &p, int i, int2 range[2
   }
 }
 
-// One round of a plane's atom scan: of atoms [a0, a0 + PLANE_LIST), those
-// whose x support meets `plane` are appended to `list` as 16 bits, (atom - a0)
-// << 3 | weight index, wave-aggregated into the shared `count`. The expensive
-// per-atom work then runs over the compacted list with every lane busy (about
-// 5 in K0 atoms land on a given plane). Six atoms per thread: a 3,072-atom
-// system scans in one round (each round is a global-load latency and three
-// barriers on the kernel's serial path), in less LDS than 2,048 int entries.
-constexpr int SCAN_PER_THREAD = 6;
+// One round of a plane's atom scan: of atoms [a0, a0 + PLANE_THREADS * SCAN),
+// those whose x support meets `plane` are appended to `list` as 16 bits,
+// (atom - a0) << 3 | weight index, wave-aggregated into the shared `count`.
+// The expensive per-atom work then runs over the compacted list with every
+// lane busy (about 5 in K0 atoms land on a given plane). Single systems scan
+// six atoms per thread: a 3,072-atom system in one round (each round is a
+// global-load latency and three barriers on the kernel's serial path), in
+// less LDS than 2,048 int entries. The batched kernels scan four: their
+// members mostly take the binned path, and the wider scan's registers would
+// cost them a workgroup per CU.
+constexpr int SCAN_PER_THREAD = 6;  // the widest; sizes the list in LDS
+constexpr int SCAN_BATCHED = 4;
 constexpr int PLANE_LIST = PLANE_THREADS * SCAN_PER_THREAD;
 static_assert(PLANE_LIST <= (1 << 13), "a list entry holds the atom's offset in its round in 13 bits");
 typedef unsigned short PlaneEntry;
 
+template <int SCAN>
 __device__ void planeScan(const Dev &d, const PmeDev &p, int plane, int a0, PlaneEntry *list, int *count) {
   const double *recip = d.box->recip64;
   const int lane = threadIdx.x & 63;
-  float4 batch[SCAN_PER_THREAD];
+  float4 batch[SCAN];
 #pragma unroll
-  for (int u = 0; u < SCAN_PER_THREAD; ++u) {
+  for (int u = 0; u < SCAN; ++u) {
     const int a = a0 + u * blockDim.x + threadIdx.x;
     batch[u] = a < d.n ? d.pos[a] : make_float4(0.f, 0.f, 0.f, 0.f);
   }
 #pragma unroll
-  for (int u = 0; u < SCAN_PER_THREAD; ++u) {
+  for (int u = 0; u < SCAN; ++u) {
     const int a = a0 + u * blockDim.x + threadIdx.x;
     int r = ORDER;
     if (a < d.n && batch[u].w != 0.f) {
@@ -348,6 +353,7 @@ __device__ void planeScan(const Dev &d, const PmeDev &p, int plane, int a0, Plan
 // 1. Spread onto plane i and forward z, y transforms of the plane. With
 // `real` set, the plane is read from a real K0*K1*K2 grid instead (the FFT
 // verification entry).
+template <int SCAN = SCAN_PER_THREAD>
 __device__ __forceinline__ void pmePlaneForward(const Dev &d, const PmeDev &p, const float *real, int i, float2 *lds) {
   const int K0 = p.K[0], K1 = p.K[1], K2 = p.K[2], K2h = p.K2h;
   const int S = p.planeSpan, P = (K1 + 1) / 2;
@@ -404,10 +410,10 @@ __device__ __forceinline__ void pmePlaneForward(const Dev &d, const PmeDev &p, c
       for (int h = 0; h < 2; ++h)
         for (int k = range[h].x + t; k < range[h].y; k += nt) spread(p.binAtoms[k], -1);
     } else {
-      for (int a0 = 0; a0 < d.n; a0 += PLANE_LIST) {
+      for (int a0 = 0; a0 < d.n; a0 += PLANE_THREADS * SCAN) {
         if (t == 0) *listCount = 0;
         __syncthreads();
-        planeScan(d, p, i, a0, list, listCount);
+        planeScan<SCAN>(d, p, i, a0, list, listCount);
         __syncthreads();
         const int count = *listCount;
         for (int k = t; k < count; k += nt) spread(a0 + (list[k] >> 3), list[k] & 7);
