@@ -222,6 +222,10 @@ typedef struct rmd_options {
   int32_t grid[3];               /* <= 0 derives                               */
   int32_t device;                /* HIP device ordinal                         */
   int32_t lanes_per_atom;        /* nonbonded lanes per atom (1..64), 0 = tuned */
+  int32_t cell_reach;            /* widest stencil reach the cell plan may take
+                                    (1..3; 1 is the reference's plan), 0 = 3    */
+  int32_t search_mode;           /* list search: 0 by the plan's cell occupancy,
+                                    1 one wave per atom, 2 one wave per cell   */
 } rmd_options;
 void rmd_options_default(rmd_options *options);
 
@@ -323,6 +327,11 @@ typedef struct rmd_diagnostics {
   int32_t lanes_per_atom;
   int32_t water_units, cluster_units, free_units;
   int32_t pme_planes; /* 1: plane PME pipeline, 0: line pipeline */
+  int32_t cell_reach;      /* the cell plan's stencil reach (1: the reference's plan) */
+  int32_t search_by_cell;  /* 1: the list search runs one wave per cell */
+  int64_t steps_since_barostat; /* steps run since the last volume move (a
+                                   move is due at a block start once this
+                                   reaches the barostat interval) */
 } rmd_diagnostics;
 int rmd_diagnostics_get(rmd_ctx *ctx, rmd_diagnostics *out);
 
@@ -359,6 +368,14 @@ int rmd_step_once(rmd_ctx *ctx);
  * K0*K1*(K2/2+1) complex values as interleaved floats. */
 int rmd_debug_fft_forward(rmd_ctx *ctx, const float *real, float *out);
 
+/* Verification entry: the thermostat's noise for every atom at `step` with
+ * the context's seed, drawn by the integrator's own device functions.
+ * uniforms (4N floats, may be NULL) receives uniformAt(seed, atom, step, k)
+ * for k = 0..3; normals (3N floats) the Box-Muller triple built from them.
+ * Counterparts: uniformAt and gaussianTriple (shared/random.js:59-87), whose
+ * device arm tools/gpu/check-integrator.mjs:44-66 checks against the host. */
+int rmd_debug_noise(rmd_ctx *ctx, int64_t step, float *uniforms, float *normals);
+
 /* Tuning and plumbing: the nonbonded lanes per atom (a power of two <= 64),
  * hipGraph capture of step blocks on/off, and the context's HIP stream (for
  * timing on that stream). */
@@ -378,6 +395,12 @@ int rmd_get_stream(rmd_ctx *ctx, void **stream);
  * timed with HIP events on the context's stream over `reps` launches:
  * 0 nonbonded, 1 PME chain, 2 integrator, 3 bonded/exceptions, 4 rebuild. */
 int rmd_profile_kernel(rmd_ctx *ctx, int32_t kernel, int32_t reps, double *avg_ms);
+/* Average durations (ms) of the PME chain's three stages over reps chains at
+ * the current state, HIP events on the context's stream between them:
+ * stage_ms[0] the charge spread with the forward transforms it carries,
+ * [1] the remaining transforms and the reciprocal convolution, [2] the
+ * inverse transforms and the force gather (webgpu/kernels/pme.js:137-336). */
+int rmd_profile_pme_stages(rmd_ctx *ctx, int32_t reps, double *stage_ms);
 /* Measurement calibration: `reps` launches of a plain device-to-device copy
  * of `bytes` bytes (16-byte vector loads and stores) on the context's stream;
  * its hardware byte counters scale the HBM traffic read for other kernels. */

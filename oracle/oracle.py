@@ -51,6 +51,17 @@ class OracleTerms(C.Structure):
         "bonded", "total"]]
 
 
+class OracleMinOptions(C.Structure):
+    _fields_ = [("max_iterations", C.c_int32), ("force_tolerance", C.c_double), ("max_step", C.c_double),
+                ("max_displacement", C.c_double), ("tolerance", C.c_double), ("restore_forces", C.c_int32)]
+
+
+class OracleMinResult(C.Structure):
+    _fields_ = [("iterations", C.c_int32), ("rejected", C.c_int32), ("stalled", C.c_int32),
+                ("converged", C.c_int32), ("potential", C.c_double), ("max_force", C.c_double),
+                ("max_step", C.c_double)]
+
+
 class OracleIntegrator(C.Structure):
     _fields_ = [("timestep", C.c_double), ("temperature", C.c_double), ("friction", C.c_double),
                 ("tolerance", C.c_double), ("seed", C.c_uint32), ("step", C.c_int64),
@@ -103,6 +114,15 @@ def load():
         lib.oracle_scale_molecules.argtypes = [C.POINTER(OracleSystem), _dp, C.c_double]
         lib.oracle_metropolis_weight.restype = C.c_double
         lib.oracle_metropolis_weight.argtypes = [C.c_double] * 5 + [C.c_int32]
+        lib.oracle_skin_for.restype = C.c_double
+        lib.oracle_skin_for.argtypes = [C.POINTER(OracleSystem), C.c_double, C.c_double, _dp]
+        lib.oracle_plan_cells.restype = C.c_int32
+        lib.oracle_plan_cells.argtypes = [C.POINTER(OracleSystem), C.c_double, C.c_double, _dp, _ip, _ip, _dp]
+        lib.oracle_minimise.argtypes = [C.POINTER(OracleSystem), _dp, C.POINTER(OracleOptions),
+                                        C.POINTER(OracleMinOptions), C.POINTER(OracleMinResult)]
+        lib.oracle_pairs_f32_binned.restype = C.c_int32
+        lib.oracle_pairs_f32_binned.argtypes = [C.c_int32, _fp, _dp, _fp, _fp, C.c_float, _ip, _ip, C.c_int32,
+                                                _ip, _ip]
         lib.oracle_cells_f32.argtypes = [C.c_int32, _fp, _fp, _ip, _ip]
         lib.oracle_r2_f32.restype = C.c_float
         lib.oracle_r2_f32.argtypes = [_fp, _fp, _fp, _fp]
@@ -179,6 +199,46 @@ class Oracle:
 
     def set_box(self, box):
         self.s.box[:] = list(box)
+
+    @staticmethod
+    def _energy_options(method="PME", alpha=None, grid=None, terms=31, f32_membership=False, use_cells=False):
+        o = OracleOptions()
+        o.method = 0 if method == "PME" else 1
+        o.alpha = alpha or 0.0
+        o.grid[:] = list(grid or (0, 0, 0))
+        o.terms = terms
+        o.f32_membership = 1 if f32_membership else 0
+        o.use_cells = 1 if use_cells else 0
+        return o
+
+    def skin_for(self, timestep=0.002, temperature=298.0, box=None):
+        """skinFor (shared/topology/cells.js:77-94)."""
+        b = np.ascontiguousarray(self.s.box if box is None else box, dtype=np.float64).reshape(9)
+        return self.lib.oracle_skin_for(C.byref(self.s), timestep, temperature, _p(b, C.c_double))
+
+    def plan_cells(self, skin, slack=2.0, box=None):
+        """planCellList (shared/topology/cells.js:111-181): counts, stencil
+        entries, neighbour capacity and the estimate it came from."""
+        b = np.ascontiguousarray(self.s.box if box is None else box, dtype=np.float64).reshape(9)
+        counts = np.zeros(3, dtype=np.int32)
+        stencil = C.c_int32()
+        estimated = C.c_double()
+        cap = self.lib.oracle_plan_cells(C.byref(self.s), skin, slack, _p(b, C.c_double), _p(counts, C.c_int32),
+                                         C.byref(stencil), C.byref(estimated))
+        return dict(counts=counts.tolist(), stencil=stencil.value, max_neighbours=cap, estimated=estimated.value)
+
+    def minimise(self, positions, max_iterations=200, force_tolerance=500.0, max_step=0.002, max_displacement=0.05,
+                 tolerance=1e-10, restore_forces=True, **energy):
+        """GpuSimulation.minimise (webgpu/simulation.js:880-979) in FP64;
+        returns the final positions and the result fields."""
+        x = np.array(positions, dtype=np.float64).reshape(-1)
+        eo = self._energy_options(**energy)
+        mo = OracleMinOptions(max_iterations, force_tolerance, max_step, max_displacement, tolerance,
+                              1 if restore_forces else 0)
+        r = OracleMinResult()
+        if self.lib.oracle_minimise(C.byref(self.s), _p(x, C.c_double), C.byref(eo), C.byref(mo), C.byref(r)) != 0:
+            raise MemoryError("oracle_minimise could not allocate")
+        return x, {name: getattr(r, name) for name, _ in OracleMinResult._fields_}
 
     def potential(self, positions, method="PME", alpha=None, grid=None, terms=31, f32_membership=False,
                   use_cells=False, restraint=None, forces=True):
@@ -366,4 +426,26 @@ def pairs_f32(positions_f32, box, list_cutoff, excl_start, excl_partners, cap=No
     lists = np.zeros(n * cap, dtype=np.int32)
     load().oracle_pairs_f32(n, _p(pos, C.c_float), _p(box9, C.c_float), _p(inv3, C.c_float), rl2,
                             _p(es, C.c_int32), _p(ep, C.c_int32), cap, _p(counts, C.c_int32), _p(lists, C.c_int32))
+    return counts, lists.reshape(n, cap)
+
+
+def pairs_f32_binned(positions_f32, box, list_cutoff, excl_start, excl_partners, cap=None):
+    """pairs_f32's sets over coarse-cell candidates (the same FP32 pair test),
+    for systems too large for the all-pairs loop."""
+    pos = np.ascontiguousarray(positions_f32, dtype=np.float32).reshape(-1)
+    n = len(pos) // 3
+    box64 = np.ascontiguousarray(box, dtype=np.float64).reshape(9)
+    box9, inv3, _ = f32_geometry(box64)
+    rl2 = np.float32(list_cutoff * list_cutoff)
+    es = np.ascontiguousarray(excl_start, dtype=np.int32)
+    ep = np.ascontiguousarray(excl_partners, dtype=np.int32)
+    counts = np.zeros(n, dtype=np.int32)
+    args = (n, _p(pos, C.c_float), _p(box64, C.c_double), _p(box9, C.c_float), _p(inv3, C.c_float), rl2,
+            _p(es, C.c_int32), _p(ep, C.c_int32))
+    widest = load().oracle_pairs_f32_binned(*args, 0, _p(counts, C.c_int32), None)
+    if widest < 0:
+        raise MemoryError("oracle_pairs_f32_binned could not allocate")
+    cap = cap or max(1, widest)
+    lists = np.zeros(n * cap, dtype=np.int32)
+    load().oracle_pairs_f32_binned(*args, cap, _p(counts, C.c_int32), _p(lists, C.c_int32))
     return counts, lists.reshape(n, cap)

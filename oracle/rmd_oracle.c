@@ -1095,3 +1095,230 @@ void oracle_scale_molecules(const oracle_system *s, double *p, double scale) {
 double oracle_metropolis_weight(double dE, double dV, double ratio, double pressure, double kT, int32_t molecules) {
   return dE + pressure * dV - molecules * kT * log(ratio);
 }
+
+/* ---- cell plan (shared/topology/cells.js:25-29,77-181) ------------------ */
+
+/* skinFor (cells.js:77-94) with halfSmallestWidth (:97-101): the measured
+ * 0.40 nm anchor scaled linearly by the timestep and by sqrt(T/298) and
+ * sqrt(1.008/m_lightest), capped at 0.9 of the room the box leaves beside the
+ * cutoff and floored at 0.1 nm. The box widths are its diagonal entries. */
+double oracle_skin_for(const oracle_system *s, double timestep, double temperature, const double box[9]) {
+  double lightest = INFINITY;
+  for (int a = 0; a < s->n; ++a)
+    if (s->masses[a] > 0 && s->masses[a] < lightest) lightest = s->masses[a];
+  if (!isfinite(lightest)) lightest = 1.008;
+  const double scaled = 0.40 * (timestep / 0.002) * sqrt((temperature / 298.0) * (1.008 / lightest));
+  double room = INFINITY;
+  if (box) {
+    double w = fabs(box[0]);
+    if (fabs(box[4]) < w) w = fabs(box[4]);
+    if (fabs(box[8]) < w) w = fabs(box[8]);
+    room = w / 2 - s->cutoff;
+  }
+  double capped = room > 0 ? room * 0.9 : 0.1;
+  double skin = scaled < capped ? scaled : capped;
+  return skin > 0.1 ? skin : 0.1;
+}
+
+/* axisOffsets (cells.js:25-29): the deduplicated one-deep stencil per axis. */
+static int axis_offsets(int cells) { return cells >= 3 ? 3 : cells == 2 ? 2 : 1; }
+
+/* planCellList (cells.js:111-181): floor(width / (cutoff + skin)) cells per
+ * axis (at least one), the deduplicated stencil, and the list capacity
+ * max(32, ceil(slack * density * 4/3 pi (cutoff + skin)^3)). Returns the
+ * capacity; counts, the stencil's entry count and the estimate go out. */
+int32_t oracle_plan_cells(const oracle_system *s, double skin, double slack, const double box[9], int32_t counts[3],
+                          int32_t *stencil, double *estimated) {
+  const double rl = s->cutoff + skin;
+  const double widths[3] = {box[0], box[4], box[8]};
+  int entries = 1;
+  for (int k = 0; k < 3; ++k) {
+    int c = (int)floor(fabs(widths[k]) / rl);
+    counts[k] = c > 1 ? c : 1;
+    entries *= axis_offsets(counts[k]);
+  }
+  if (stencil) *stencil = entries;
+  const double volume = fabs(box[0] * box[4] * box[8]);
+  const double estimate = (s->n / volume) * (4.0 / 3.0) * M_PI * rl * rl * rl;
+  if (estimated) *estimated = estimate;
+  const int cap = (int)ceil(estimate * slack);
+  return cap > 32 ? cap : 32;
+}
+
+/* ---- minimiser (webgpu/simulation.js:880-979) ------------------------------ */
+
+static double max_atom_force(const oracle_system *s, const double *f) {
+  double worst = 0;
+  for (int a = 0; a < s->n; ++a) {
+    double m = sqrt(f[3 * a] * f[3 * a] + f[3 * a + 1] * f[3 * a + 1] + f[3 * a + 2] * f[3 * a + 2]);
+    if (m > worst) worst = m;
+  }
+  return worst;
+}
+
+/* GpuSimulation.minimise (webgpu/simulation.js:880-979): steepest descent in
+ * which each trial moves every atom by (max_step / max(worst, 1e-6)) F_i with
+ * the displacement capped at max_displacement per atom (displaceShader,
+ * webgpu/kernels/integrate.js:653-667), projects the constraints onto the
+ * pre-move positions (encodeConstrainPositions :671-688, SHAKE here) and
+ * keeps the trial only if the potential went down (max_step x 1.3), else
+ * restores the positions and halves max_step, 14 tries per iteration. The
+ * largest per-atom force |F_i| decides convergence (forceMagnitudeShader
+ * :589-603). restore_forces 0 keeps the rejected trial's forces for the next
+ * try, as the reference does; 1 restores them with the positions. */
+int oracle_minimise(const oracle_system *s, double *pos, const oracle_options *eo, const oracle_min_options *mo,
+                    oracle_min_result *out) {
+  const int n3 = 3 * s->n;
+  double *f = (double *)calloc(n3, sizeof(double));
+  double *fs = (double *)calloc(n3, sizeof(double));
+  double *saved = (double *)calloc(n3, sizeof(double));
+  oracle_terms t;
+  if (!f || !fs || !saved) {
+    free(f), free(fs), free(saved);
+    return -1;
+  }
+  double max_step = mo->max_step;
+  oracle_potential(s, pos, eo, &t, f);
+  double potential = t.total;
+  int iterations = 0, rejected = 0, stalled = 0;
+  for (int it = 0; it < mo->max_iterations; ++it) {
+    iterations = it + 1;
+    const double worst = max_atom_force(s, f);
+    if (worst <= mo->force_tolerance) break;
+    memcpy(saved, pos, sizeof(double) * n3);
+    memcpy(fs, f, sizeof(double) * n3);
+    int accepted = 0;
+    for (int attempt = 0; attempt < 14; ++attempt) {
+      const double scale = max_step / (worst > 1e-6 ? worst : 1e-6);
+      for (int a = 0; a < s->n; ++a) {
+        double d[3] = {scale * f[3 * a], scale * f[3 * a + 1], scale * f[3 * a + 2]};
+        double len = sqrt(d[0] * d[0] + d[1] * d[1] + d[2] * d[2]);
+        double k = len > mo->max_displacement ? mo->max_displacement / len : 1.0;
+        for (int c = 0; c < 3; ++c) pos[3 * a + c] = saved[3 * a + c] + k * d[c];
+      }
+      if (s->nconstraint > 0) oracle_shake(s, pos, saved, NULL, mo->tolerance, 100000);
+      oracle_potential(s, pos, eo, &t, f);
+      if (t.total < potential) {
+        potential = t.total;
+        max_step *= 1.3;
+        accepted = 1;
+        break;
+      }
+      rejected += 1;
+      memcpy(pos, saved, sizeof(double) * n3);
+      if (mo->restore_forces) memcpy(f, fs, sizeof(double) * n3);
+      max_step *= 0.5;
+      if (max_step < 1e-9) break;
+    }
+    if (!accepted) {
+      stalled = 1;
+      if (mo->restore_forces) memcpy(f, fs, sizeof(double) * n3);
+      break;
+    }
+  }
+  if (out) {
+    out->iterations = iterations;
+    out->rejected = rejected;
+    out->stalled = stalled;
+    out->max_force = max_atom_force(s, f);
+    out->converged = out->max_force <= mo->force_tolerance;
+    out->potential = potential;
+    out->max_step = max_step;
+  }
+  free(f), free(fs), free(saved);
+  return 0;
+}
+
+/* The brute-force FP32 neighbour sets of oracle_pairs_f32 for large systems:
+ * the same pair test (oracle_r2_f32, exclusions, self) over candidates from
+ * coarse cells in fractional coordinates whose face-to-face thickness is at
+ * least rl + 0.01 nm, so every pair within rl is a candidate (a pair closer
+ * than rl differs by less than one cell along every fractional axis). Lists
+ * are sorted ascending like the brute force's. box64 the FP64 box rows. */
+static int cmp_int(const void *a, const void *b) {
+  int x = *(const int *)a, y = *(const int *)b;
+  return (x > y) - (x < y);
+}
+
+int32_t oracle_pairs_f32_binned(int32_t n, const float *pos, const double box64[9], const float box9[9],
+                                const float inv3[3], float rl2, const int32_t *excl_start,
+                                const int32_t *excl_partners, int32_t cap, int32_t *counts, int32_t *lists) {
+  const double ax = box64[0], bx = box64[3], by = box64[4], cx = box64[6], cy = box64[7], cz = box64[8];
+  const double r[9] = {1 / ax, 0, 0, -bx / (ax * by), 1 / by, 0, (bx * cy - by * cx) / (ax * by * cz), -cy / (by * cz),
+                       1 / cz};
+  const double rl = sqrt((double)rl2) + 0.01;
+  int cells[3];
+  for (int k = 0; k < 3; ++k) {
+    double face = 1.0 / sqrt(r[k] * r[k] + r[3 + k] * r[3 + k] + r[6 + k] * r[6 + k]);
+    cells[k] = (int)floor(face / rl);
+    if (cells[k] < 1) cells[k] = 1;
+  }
+  const int total = cells[0] * cells[1] * cells[2];
+  int *cellOf = (int *)malloc(sizeof(int) * (n > 0 ? n : 1));
+  int *start = (int *)calloc(total + 1, sizeof(int));
+  int *order = (int *)malloc(sizeof(int) * (n > 0 ? n : 1));
+  if (!cellOf || !start || !order) {
+    free(cellOf), free(start), free(order);
+    return -1;
+  }
+  for (int i = 0; i < n; ++i) {
+    int c3[3];
+    for (int k = 0; k < 3; ++k) {
+      double s = pos[3 * i] * r[k] + pos[3 * i + 1] * r[3 + k] + pos[3 * i + 2] * r[6 + k];
+      s -= floor(s);
+      int ci = (int)(s * cells[k]);
+      c3[k] = ci < 0 ? 0 : ci >= cells[k] ? cells[k] - 1 : ci;
+    }
+    cellOf[i] = (c3[0] * cells[1] + c3[1]) * cells[2] + c3[2];
+    start[cellOf[i] + 1] += 1;
+  }
+  for (int c = 0; c < total; ++c) start[c + 1] += start[c];
+  {
+    int *cursor = (int *)malloc(sizeof(int) * total);
+    memcpy(cursor, start, sizeof(int) * total);
+    for (int i = 0; i < n; ++i) order[cursor[cellOf[i]]++] = i;
+    free(cursor);
+  }
+  int32_t widest = 0;
+#pragma omp parallel
+  {
+  int *cand = (int *)malloc(sizeof(int) * (n > 0 ? n : 1));
+  int32_t mine = 0;
+#pragma omp for schedule(dynamic, 256)
+  for (int i = 0; i < n; ++i) {
+    const int c = cellOf[i];
+    const int c3[3] = {c / (cells[1] * cells[2]), (c / cells[2]) % cells[1], c % cells[2]};
+    int off[3][3], noff[3];
+    for (int k = 0; k < 3; ++k) {
+      noff[k] = cells[k] >= 3 ? 3 : cells[k];
+      for (int o = 0; o < noff[k]; ++o) {
+        int d = cells[k] >= 3 ? o - 1 : o;
+        off[k][o] = ((c3[k] + d) % cells[k] + cells[k]) % cells[k];
+      }
+    }
+    int m = 0;
+    for (int ox = 0; ox < noff[0]; ++ox)
+      for (int oy = 0; oy < noff[1]; ++oy)
+        for (int oz = 0; oz < noff[2]; ++oz) {
+          const int nc = (off[0][ox] * cells[1] + off[1][oy]) * cells[2] + off[2][oz];
+          for (int k = start[nc]; k < start[nc + 1]; ++k) {
+            const int j = order[k];
+            if (j == i) continue;
+            if (oracle_r2_f32(pos + 3 * i, pos + 3 * j, box9, inv3) > rl2) continue;
+            if (is_excluded(excl_start, excl_partners, i, j)) continue;
+            cand[m++] = j;
+          }
+        }
+    qsort(cand, m, sizeof(int), cmp_int);
+    if (lists)
+      for (int k = 0; k < m && k < cap; ++k) lists[(int64_t)i * cap + k] = cand[k];
+    counts[i] = m;
+    if (m > mine) mine = m;
+  }
+#pragma omp critical
+  if (mine > widest) widest = mine;
+  free(cand);
+  }
+  free(cellOf), free(start), free(order);
+  return widest;
+}

@@ -133,6 +133,26 @@ void oracle_scale_molecules(const oracle_system *s, double *pos, double length_s
 double oracle_metropolis_weight(double delta_energy, double delta_volume, double volume_ratio,
                                 double pressure_kj, double kT, int32_t molecules);
 
+/* skinFor and planCellList (shared/topology/cells.js:77-181). box may be
+ * NULL (no cap). oracle_plan_cells returns the neighbour capacity. */
+double oracle_skin_for(const oracle_system *s, double timestep, double temperature, const double box[9]);
+int32_t oracle_plan_cells(const oracle_system *s, double skin, double slack, const double box[9], int32_t counts[3],
+                          int32_t *stencil, double *estimated);
+
+/* Steepest-descent minimiser (webgpu/simulation.js:880-979). */
+typedef struct oracle_min_options {
+  int32_t max_iterations;
+  double force_tolerance, max_step, max_displacement;
+  double tolerance;        /* SHAKE tolerance of the constraint projection */
+  int32_t restore_forces;  /* 1: a rejected trial restores the forces too  */
+} oracle_min_options;
+typedef struct oracle_min_result {
+  int32_t iterations, rejected, stalled, converged;
+  double potential, max_force, max_step;
+} oracle_min_result;
+int oracle_minimise(const oracle_system *s, double *pos, const oracle_options *energy, const oracle_min_options *o,
+                    oracle_min_result *out);
+
 /* FP32 emulation of the cell binning (webgpu/kernels/wgsl-common.js:112-123)
  * and of the neighbour test (neighbours.js:116-139), with this build's fixed
  * operation order. pos is 3N float; recip the 9 f32 reciprocal-box entries;
@@ -146,6 +166,13 @@ float oracle_r2_f32(const float *pi, const float *pj, const float box9[9], const
 int32_t oracle_pairs_f32(int32_t n, const float *pos, const float box9[9], const float inv3[3], float rl2,
                          const int32_t *excl_start, const int32_t *excl_partners, int32_t cap,
                          int32_t *counts, int32_t *lists);
+
+/* oracle_pairs_f32's sets for large systems: the same test over candidates
+ * from coarse cells at least rl + 0.01 nm thick. box64 the FP64 box rows.
+ * Returns the maximum count, or -1 when it cannot allocate. */
+int32_t oracle_pairs_f32_binned(int32_t n, const float *pos, const double box64[9], const float box9[9],
+                                const float inv3[3], float rl2, const int32_t *excl_start,
+                                const int32_t *excl_partners, int32_t cap, int32_t *counts, int32_t *lists);
 
 #ifdef __cplusplus
 }

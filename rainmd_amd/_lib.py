@@ -128,6 +128,8 @@ class Options(C.Structure):
         ("grid", C.c_int32 * 3),
         ("device", C.c_int32),
         ("lanes_per_atom", C.c_int32),
+        ("cell_reach", C.c_int32),
+        ("search_mode", C.c_int32),
     ]
 
 
@@ -193,6 +195,8 @@ class Diagnostics(C.Structure):
         ("lanes_per_atom", C.c_int32),
         ("water_units", C.c_int32), ("cluster_units", C.c_int32), ("free_units", C.c_int32),
         ("pme_planes", C.c_int32),
+        ("cell_reach", C.c_int32), ("search_by_cell", C.c_int32),
+        ("steps_since_barostat", C.c_int64),
     ]
 
 
@@ -283,6 +287,8 @@ SIGNATURES = {
     "rmd_read_neighbours": (C.c_int, [C.c_void_p, _ip, _ip]),
     "rmd_step_once": (C.c_int, [C.c_void_p]),
     "rmd_debug_fft_forward": (C.c_int, [C.c_void_p, _fp, _fp]),
+    "rmd_debug_noise": (C.c_int, [C.c_void_p, C.c_int64, _fp, _fp]),
+    "rmd_profile_pme_stages": (C.c_int, [C.c_void_p, C.c_int32, _dp]),
     "rmd_set_lanes": (C.c_int, [C.c_void_p, C.c_int32]),
     "rmd_set_graphs": (C.c_int, [C.c_void_p, C.c_int32]),
     "rmd_set_pme_pipeline": (C.c_int, [C.c_void_p, C.c_int32]),

@@ -167,6 +167,8 @@ struct rmd_ctx {
   int windowAttempts = 0, windowAccepted = 0;
   double volumeScale = -1;
   int64_t stepsSinceBarostat = 0;
+  int reachMax = 0;    // the cell plan's widest reach (0: CELL_REACH_MAX)
+  int searchMode = 0;  // 0 by occupancy, 1 per atom, 2 per cell where the launch covers the cells
   int neighbourGrowths = 0;
   unsigned int worstRequired = 0;
   unsigned int rebuildCount = 0;  // Scalars::rebuilds at the last read-back
@@ -214,7 +216,7 @@ BoxF boxFor(const rmd_ctx *c) {
   for (int k = 0; k < 3; ++k) b.cells[k] = c->plan.counts[k];
   b.cellTotal = c->plan.cellCount;
   b.reach = c->plan.reach;
-  b.cellSearch = cellSearchFits(c->d.n, c->plan.cellCount) ? 1 : 0;
+  b.cellSearch = searchByCell(c->searchMode, c->d.n, c->plan.cellCount) ? 1 : 0;
   return b;
 }
 
@@ -773,9 +775,9 @@ void runBlock(rmd_ctx *c, int steps) {
 bool updateBoxHost(rmd_ctx *c, const double *nb) {
   c->blockEndStep = -1;
   for (int k = 0; k < 9; ++k) c->box[k] = nb[k];
-  CellPlan estimate = planCellList(c->sys, c->plan.skin, c->slack, 0, c->box);
+  CellPlan estimate = planCellList(c->sys, c->plan.skin, c->slack, 0, c->box, c->reachMax);
   const int capacity = std::max(c->d.cap, estimate.maxNeighbours);
-  CellPlan plan = planCellList(c->sys, c->plan.skin, c->slack, capacity, c->box);
+  CellPlan plan = planCellList(c->sys, c->plan.skin, c->slack, capacity, c->box, c->reachMax);
   c->plan = plan;
   allocateCells(c, plan.cellCount);
   const bool replaced = capacity > c->d.cap;
@@ -884,7 +886,13 @@ void create(rmd_ctx *c, const System &sys, const rmd_options &o, const Streams *
   if (c->lanes & (c->lanes - 1) || c->lanes > 64) throw Error(RMD_E_INVALID, "lanes_per_atom must be a power of two <= 64");
   for (int k = 0; k < 9; ++k) c->box[k] = sys.box[k];
   c->skin = o.skin > 0 ? o.skin : skinFor(sys, c->timestep, c->temperature, nullptr);
-  c->plan = planCellList(sys, c->skin, c->slack, o.max_neighbours, c->box);
+  if (o.cell_reach < 0 || o.cell_reach > CELL_REACH_MAX)
+    throw Error(RMD_E_INVALID, "cell_reach must be 0 (the default, 3) or 1..3");
+  if (o.search_mode < 0 || o.search_mode > 2)
+    throw Error(RMD_E_INVALID, "search_mode must be 0 (by occupancy), 1 (per atom) or 2 (per cell)");
+  c->reachMax = o.cell_reach;
+  c->searchMode = o.search_mode;
+  c->plan = planCellList(sys, c->skin, c->slack, o.max_neighbours, c->box, c->reachMax);
   {
     rmd_pme_parameters pp;
     const int32_t forced[3] = {o.grid[0], o.grid[1], o.grid[2]};
@@ -1697,6 +1705,9 @@ int rmd_diagnostics_get(rmd_ctx *c, rmd_diagnostics *o) {
     o->water_units = c->waterUnits;
     o->cluster_units = c->clusterUnits;
     o->free_units = c->freeUnits;
+    o->cell_reach = c->plan.reach;
+    o->search_by_cell = searchByCell(c->searchMode, c->d.n, c->plan.cellCount) ? 1 : 0;
+    o->steps_since_barostat = c->stepsSinceBarostat;
   });
 }
 
@@ -1768,6 +1779,31 @@ int rmd_debug_fft_forward(rmd_ctx *c, const float *real, float *out) {
   });
 }
 
+// Verification entry: the thermostat's counter-hash draws for every atom at
+// one step with the context's seed: 4 uniforms per atom (may be null) and the
+// 3 Box-Muller normals per atom, from the integrator's own device functions.
+int rmd_debug_noise(rmd_ctx *c, int64_t step, float *uniforms, float *normals) {
+  CTX_GUARD(c, {
+    if (!normals) throw Error(RMD_E_INVALID, "normals are required");
+    const int n = c->d.n;
+    std::vector<void *> tmp;
+    try {
+      float *du = uniforms ? devAlloc<float>(4 * (size_t)n, tmp) : nullptr;
+      float *dg = devAlloc<float>(3 * (size_t)n, tmp);
+      launchNoise(n, c->ip.seed, (unsigned int)step, du, dg, c->stream);
+      HIP_CHECK(hipGetLastError());
+      if (uniforms)
+        HIP_CHECK(hipMemcpyAsync(uniforms, du, 4 * (size_t)n * sizeof(float), hipMemcpyDeviceToHost, c->stream));
+      HIP_CHECK(hipMemcpyAsync(normals, dg, 3 * (size_t)n * sizeof(float), hipMemcpyDeviceToHost, c->stream));
+      HIP_CHECK(hipStreamSynchronize(c->stream));
+    } catch (...) {
+      for (void *p : tmp) (void)hipFree(p);
+      throw;
+    }
+    for (void *p : tmp) (void)hipFree(p);
+  });
+}
+
 int rmd_set_lanes(rmd_ctx *c, int32_t lanes) {
   CTX_GUARD(c, {
     if (lanes <= 0 || lanes > 64 || (lanes & (lanes - 1))) throw Error(RMD_E_INVALID, "lanes must be a power of two <= 64");
@@ -1825,6 +1861,33 @@ int rmd_profile_kernel(rmd_ctx *c, int32_t kernel, int32_t reps, double *avg_ms)
     (void)hipEventDestroy(b);
     if (kernel == 2) restore(c);
     (void)slots;
+    evaluate(c, RMD_TERM_ALL, true, nullptr);
+  });
+}
+
+// Average durations of the PME chain's three stages (spread with the forward
+// transforms it carries; the remaining transforms and the convolution; the
+// inverse transforms and the gather), HIP events on the context's stream
+// between the stages, over reps chains at the current state.
+int rmd_profile_pme_stages(rmd_ctx *c, int32_t reps, double *stage_ms) {
+  CTX_GUARD(c, {
+    if (!stage_ms || reps <= 0) throw Error(RMD_E_INVALID, "bad arguments");
+    evaluate(c, RMD_TERM_ALL, true, nullptr);
+    hipEvent_t ev[4];
+    for (auto &e : ev) HIP_CHECK(hipEventCreate(&e));
+    double total[3] = {0, 0, 0};
+    launchPmeStages(c->d, c->p, c->stream, ev);  // warm
+    for (int r = 0; r < reps; ++r) {
+      launchPmeStages(c->d, c->p, c->stream, ev);
+      HIP_CHECK(hipEventSynchronize(ev[3]));
+      for (int k = 0; k < 3; ++k) {
+        float ms = 0;
+        HIP_CHECK(hipEventElapsedTime(&ms, ev[k], ev[k + 1]));
+        total[k] += ms;
+      }
+    }
+    for (auto &e : ev) (void)hipEventDestroy(e);
+    for (int k = 0; k < 3; ++k) stage_ms[k] = total[k] / reps;
     evaluate(c, RMD_TERM_ALL, true, nullptr);
   });
 }

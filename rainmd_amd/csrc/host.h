@@ -142,8 +142,10 @@ struct CellPlan {
 constexpr int CELL_REACH_MAX = 3;
 constexpr int CELL_LIMIT_SMALL = 4096;
 constexpr int CELL_ATOMS_SMALL = 16384;
+// reachMax limits the plan's reach (1 gives the reference's plan; values
+// outside 1..CELL_REACH_MAX mean CELL_REACH_MAX).
 CellPlan planCellList(const System &system, double skin, double slack, int maxNeighbours,
-                      const double *box);
+                      const double *box, int reachMax = 0);
 
 // Dispersion (shared/dispersion.js:37-82) and Ewald self energy
 // (cpu/forces/pme.js:270-286).
@@ -162,6 +164,7 @@ void gaussianTriple(uint32_t seed, uint32_t atom, uint32_t step, double out[3]);
 
 // Geometry helpers (cpu/forces/geometry.js).
 void reciprocalBoxVectors(const double box[9], double out[9]);
+void faceWidths(const double box[9], double out[3]);
 inline double boxVolume(const double box[9]) { return box[0] * box[4] * box[8]; }
 
 }  // namespace rmd

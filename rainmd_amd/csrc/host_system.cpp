@@ -54,6 +54,16 @@ void gaussianTriple(uint32_t seed, uint32_t atom, uint32_t step, double out[3]) 
 
 // ---- geometry (cpu/forces/geometry.js:47-69) ------------------------------
 
+// Distances between the box's opposite faces, 1/|column k of the reciprocal
+// box|: the widths that bound a cell's thickness along each fractional axis.
+// For a rectangular box they are the diagonal; a sheared box's faces are
+// closer than its diagonal entries.
+void faceWidths(const double box[9], double out[3]) {
+  double r[9];
+  reciprocalBoxVectors(box, r);
+  for (int k = 0; k < 3; ++k) out[k] = 1.0 / std::sqrt(r[k] * r[k] + r[3 + k] * r[3 + k] + r[6 + k] * r[6 + k]);
+}
+
 void reciprocalBoxVectors(const double box[9], double out[9]) {
   const double det = box[0] * box[4] * box[8];
   if (!(std::fabs(det) > 0)) throw Error(RMD_E_INVALID, "box vectors are singular; volume is zero");
@@ -566,20 +576,30 @@ double skinFor(const System &s, double timestep, double temperature, const doubl
   const double *b = box ? box : s.box;
   double room = INFINITY;
   if (s.isPeriodic) {
-    double half = std::min({std::fabs(b[0]), std::fabs(b[4]), std::fabs(b[8])}) / 2;
-    room = half - s.cutoff;
+    // half the smallest face-to-face width (the reference takes the diagonal,
+    // which is the same for a rectangular box and too wide for a sheared one)
+    double w[3];
+    faceWidths(b, w);
+    room = std::min({w[0], w[1], w[2]}) / 2 - s.cutoff;
   }
   return std::max(0.1, std::min(scaled, room > 0 ? room * 0.9 : 0.1));
 }
 
-CellPlan planCellList(const System &s, double skin, double slack, int maxNeighbours, const double *box) {
+CellPlan planCellList(const System &s, double skin, double slack, int maxNeighbours, const double *box,
+                      int reachMax) {
   if (!s.isPeriodic) throw Error(RMD_E_INVALID, "a cell list needs a periodic box");
   const double *b = box ? box : s.box;
   CellPlan p;
   p.skin = skin;
   p.cutoff = s.cutoff;
   p.listCutoff = s.cutoff + skin;
-  const double widths[3] = {b[0], b[4], b[8]};
+  // Face-to-face widths: a cell of width w along fractional axis k is
+  // w/counts[k] thick only when w is the distance between the box's faces.
+  // For a rectangular box this is the diagonal the reference uses; for a
+  // sheared box the diagonal overstates it and a stencil over cells that thin
+  // would miss listed pairs.
+  double widths[3];
+  faceWidths(b, widths);
   // The reference's cells are at least one list cutoff wide (reach 1), so in
   // a box under four cutoffs wide the 3-cell stencil covers every cell and
   // the search is all-pairs. Cells of listCutoff / reach with a stencil of
@@ -589,7 +609,8 @@ CellPlan planCellList(const System &s, double skin, double slack, int maxNeighbo
   // systems the single-workgroup build takes.
   const int n = s.topology.atomCount;
   double best = 2.0;
-  for (int reach = 1; reach <= CELL_REACH_MAX; ++reach) {
+  const int widest = reachMax >= 1 && reachMax <= CELL_REACH_MAX ? reachMax : CELL_REACH_MAX;
+  for (int reach = 1; reach <= widest; ++reach) {
     int counts[3];
     double fraction = 1.0;
     long long cells = 1;

@@ -87,10 +87,23 @@ __device__ __forceinline__ float uniformAt(unsigned int seed, unsigned int a, un
   return ((float)(st >> 9) + 0.5f) * (1.0f / 8388608.0f);
 }
 
+// gaussianTriple (shared/random.js:66-87): three normals from the four
+// counter-hash uniforms of (seed, atom, step) by Box-Muller, in FP32.
+__device__ __forceinline__ void gaussianTriple(unsigned int seed, unsigned int atom, unsigned int step, float u[4],
+                                               float g[3]) {
+  for (int k = 0; k < 4; ++k) u[k] = uniformAt(seed, atom, step, k);
+  const float r1 = sqrtf(-2.0f * logf(u[0])), r2 = sqrtf(-2.0f * logf(u[2]));
+  float s1, c1, s2, c2;
+  sincospif(2.0f * u[1], &s1, &c1);
+  sincospif(2.0f * u[3], &s2, &c2);
+  g[0] = r1 * c1;
+  g[1] = r1 * s1;
+  g[2] = r2 * c2;
+}
+
 // O: v <- a v + sqrt(1 - a^2) sqrt(kT/m) xi (integrate.js:406-437). The
-// three normals come from four counter-hash uniforms by Box-Muller
-// (shared/random.js:66-87) in FP32 - the noise is a random draw, and FP64 is
-// kept for the velocity update itself.
+// normals are drawn in FP32 - the noise is a random draw, and FP64 is kept
+// for the velocity update itself.
 __device__ __forceinline__ void thermostat(const Dev &d, const IntegratorParams &ip, int a, Atom &s,
                                            unsigned int step) {
   if (s.im <= 0) return;
@@ -98,15 +111,9 @@ __device__ __forceinline__ void thermostat(const Dev &d, const IntegratorParams 
   if (ip.suppliedNoise) {
     for (int c = 0; c < 3; ++c) g[c] = d.noise[3 * a + c];
   } else {
-    const float u1 = uniformAt(ip.seed, a, step, 0), u2 = uniformAt(ip.seed, a, step, 1);
-    const float u3 = uniformAt(ip.seed, a, step, 2), u4 = uniformAt(ip.seed, a, step, 3);
-    const float r1 = sqrtf(-2.0f * logf(u1)), r2 = sqrtf(-2.0f * logf(u3));
-    float s1, c1, s2, c2;
-    sincospif(2.0f * u2, &s1, &c1);
-    sincospif(2.0f * u4, &s2, &c2);
-    g[0] = r1 * c1;
-    g[1] = r1 * s1;
-    g[2] = r2 * c2;
+    float u[4], gf[3];
+    gaussianTriple(ip.seed, a, step, u, gf);
+    for (int c = 0; c < 3; ++c) g[c] = gf[c];
   }
   const double amp = ip.noiseScale * sqrt(ip.kT * s.im);
   for (int c = 0; c < 3; ++c) s.v[c] = ip.decay * s.v[c] + amp * g[c];
@@ -1164,6 +1171,22 @@ void launchScaleMolecules(const Dev &d, const int *molStart, const int *molAtoms
 
 void launchMaxForce(const Dev &d, unsigned int *out, hipStream_t s) {
   maxForceKernel<<<(d.n + 255) / 256, 256, 0, s>>>(d, out);
+}
+
+// The thermostat's draws for every atom at one step: the four uniforms
+// (4 per atom, may be null) and the three normals (3 per atom).
+__global__ void noiseKernel(int n, unsigned int seed, unsigned int step, float *uniforms, float *normals) {
+  const int a = blockIdx.x * blockDim.x + threadIdx.x;
+  if (a >= n) return;
+  float u[4], g[3];
+  gaussianTriple(seed, a, step, u, g);
+  if (uniforms)
+    for (int k = 0; k < 4; ++k) uniforms[4 * a + k] = u[k];
+  for (int c = 0; c < 3; ++c) normals[3 * a + c] = g[c];
+}
+
+void launchNoise(int n, unsigned int seed, unsigned int step, float *uniforms, float *normals, hipStream_t s) {
+  noiseKernel<<<(n + 255) / 256, 256, 0, s>>>(n, seed, step, uniforms, normals);
 }
 
 void launchDisplace(const Dev &d, float scale, float cap, hipStream_t s) {

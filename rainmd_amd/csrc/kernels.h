@@ -66,6 +66,7 @@ bool forcesJoin(const Dev &d, const PmeDev &p, int lanes);
 void launchForcesJoined(const Dev &d, const PmeDev &p, float alpha, int rational, int lanes, int energies,
                         double alphaD, int terms, hipStream_t s, int *pairSlots, int *bondedSlots);
 int launchPmeAfterForward(const Dev &d, const PmeDev &p, int energies, hipStream_t s);
+void launchPmeStages(const Dev &d, const PmeDev &p, hipStream_t s, hipEvent_t ev[4]);
 // Debug entry: forward R2C 3D FFT of a real grid into the half spectrum.
 void launchFftForward(const Dev &d, const PmeDev &p, const float *real, float2 *out, hipStream_t s);
 
@@ -129,6 +130,7 @@ void launchScaleMolecules(const Dev &d, const int *molStart, const int *molAtoms
 int launchMolecularTerm(const Dev &d, const int *molStart, const int *molAtoms, int nMol, hipStream_t s);
 // Minimiser helpers.
 void launchMaxForce(const Dev &d, unsigned int *out, hipStream_t s);
+void launchNoise(int n, unsigned int seed, unsigned int step, float *uniforms, float *normals, hipStream_t s);
 void launchDisplace(const Dev &d, float scale, float cap, hipStream_t s);
 void launchConstrainPositions(const Dev &d, const float4 *reference, const IntegratorParams &ip, hipStream_t s);
 
@@ -207,6 +209,9 @@ bool fitsSmallCellBuild(int atoms, int cells);
 // atoms at most on average), and a cell count the per-atom launch's blocks
 // cover at four cells each.
 bool cellSearchFits(int atoms, int cells);
+// The search path for a search mode (0 by occupancy, 1 per atom, 2 per cell
+// where the launch covers the cells).
+bool searchByCell(int mode, int atoms, int cells);
 void launchRefreshSlotsBatch(const BatchView *v, const TaskTable &atoms256, hipStream_t s);
 // The cell lists of the replicas due a rebuild, then their neighbour lists;
 // `queue` holds searchQueueInts(replicas) ints of device memory, `grid` is a

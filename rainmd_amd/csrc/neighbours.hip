@@ -766,9 +766,19 @@ int searchQueueInts(int replicas) { return 3 + 2 * replicas; }
 
 bool fitsSmallCellBuild(int atoms, int cells) { return atoms <= SMALL_MAX_ATOMS && cells <= SMALL_MAX_CELLS; }
 
+// The per-cell search runs in the per-atom search's launch (4 waves per
+// workgroup, one workgroup per 4 atoms), so it covers at most that many cells.
+static bool cellLaunchCovers(int atoms, int cells) { return cells > 0 && cells <= 4 * ((atoms + 3) / 4); }
+
 bool cellSearchFits(int atoms, int cells) {
   constexpr int MAX_AVERAGE = 32;  // atoms per cell
-  return cells > 0 && cells <= 4 * ((atoms + 3) / 4) && atoms <= MAX_AVERAGE * cells;
+  return cellLaunchCovers(atoms, cells) && atoms <= MAX_AVERAGE * cells;
+}
+
+bool searchByCell(int mode, int atoms, int cells) {
+  if (mode == 1) return false;
+  if (mode == 2) return cellLaunchCovers(atoms, cells);
+  return cellSearchFits(atoms, cells);
 }
 
 }  // namespace rmd

@@ -702,6 +702,40 @@ int launchPme(const Dev &d, const PmeDev &p, int energies, hipStream_t s, int *s
   return xBlocks;
 }
 
+// The chain of launchPme (no energies) in three stages - the spread with the
+// forward transforms it carries, the remaining transforms with the
+// convolution, the inverse transforms with the gather - with ev[k] recorded
+// before stage k and ev[3] after the last, for per-stage timing.
+void launchPmeStages(const Dev &d, const PmeDev &p, hipStream_t s, hipEvent_t ev[4]) {
+  (void)hipEventRecord(ev[0], s);
+  if (p.planes) {
+    if (p.binned) pmeBinKernel<<<1, BIN_THREADS, 0, s>>>(d, p);
+    pmePlaneForwardKernel<<<p.K[0], PLANE_THREADS, planeForwardLds(p), s>>>(d, p, nullptr);
+    (void)hipEventRecord(ev[1], s);
+    pmeXLinesKernel<<<xLineBlocks(p), XLINE_THREADS, xLinesLds(p), s>>>(d, p, 1, 0);
+    (void)hipEventRecord(ev[2], s);
+    pmePlaneGatherKernel<<<p.K[0], PLANE_THREADS, planeGatherLds(p), s>>>(d, p);
+    if (p.binned) pmeAtomGatherKernel<<<(d.n + 255) / 256, 256, 0, s>>>(d, p);
+    (void)hipEventRecord(ev[3], s);
+    return;
+  }
+  const int atomBlocks = (d.n + 255) / 256;
+  pmeSpreadKernel<<<atomBlocks, 256, 0, s>>>(d, p);
+  const int zPairs = (p.K[0] * p.K[1] + 1) / 2;
+  const int zBlocks = (zPairs + LINES_PER_BLOCK - 1) / LINES_PER_BLOCK;
+  (void)hipEventRecord(ev[1], s);
+  pmeZForwardKernel<<<zBlocks, 256, lineLds(p.K[2]), s>>>(p);
+  const int yBlocks = (p.K[0] * p.K2h + LINES_PER_BLOCK - 1) / LINES_PER_BLOCK;
+  pmeYKernel<<<yBlocks, 256, lineLds(p.K[1]), s>>>(p, +1);
+  const int xBlocks = (p.K[1] * p.K2h + LINES_PER_BLOCK - 1) / LINES_PER_BLOCK;
+  pmeXConvolveKernel<<<xBlocks, 256, lineLds(p.K[0]), s>>>(d, p, 1, 0);
+  pmeYKernel<<<yBlocks, 256, lineLds(p.K[1]), s>>>(p, -1);
+  pmeZInverseKernel<<<zBlocks, 256, lineLds(p.K[2]), s>>>(p);
+  (void)hipEventRecord(ev[2], s);
+  pmeGatherKernel<<<atomBlocks, 256, 0, s>>>(d, p);
+  (void)hipEventRecord(ev[3], s);
+}
+
 // Verification entry: the forward R2C 3D transform of a real grid, through
 // the same kernels as the active pipeline.
 __global__ void realToFixedKernel(PmeDev p, const float *real) {

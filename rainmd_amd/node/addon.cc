@@ -575,6 +575,8 @@ void readOptions(napi_env env, napi_value opt, rmd_options &o, std::vector<int32
     for (int k = 0; k < 3; ++k) o.grid[k] = grid[k];
   o.device = (int32_t)prop(env, opt, "device", o.device);
   o.lanes_per_atom = (int32_t)prop(env, opt, "lanesPerAtom", o.lanes_per_atom);
+  o.cell_reach = (int32_t)prop(env, opt, "cellReach", o.cell_reach);
+  o.search_mode = (int32_t)prop(env, opt, "searchMode", o.search_mode);
 }
 
 napi_value Destroy(napi_env env, napi_callback_info info) {
@@ -744,6 +746,9 @@ napi_value Diagnostics(napi_env env, napi_callback_info info) {
   setNumber(env, o, "clusterUnits", d.cluster_units);
   setNumber(env, o, "freeUnits", d.free_units);
   setBool(env, o, "pmePlanes", d.pme_planes != 0);
+  setNumber(env, o, "cellReach", d.cell_reach);
+  setBool(env, o, "searchByCell", d.search_by_cell != 0);
+  setNumber(env, o, "stepsSinceBarostat", (double)d.steps_since_barostat);
   return o;
 }
 

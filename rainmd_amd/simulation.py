@@ -328,7 +328,7 @@ def _options(options):
     lib.rmd_options_default(C.byref(o))
     simple = ["timestep", "temperature", "friction", "seed", "tolerance", "max_constraint_iterations",
               "rebuild_interval", "barostat_interval", "skin", "neighbour_slack", "max_neighbours",
-              "ewald_tolerance", "alpha", "device", "lanes_per_atom"]
+              "ewald_tolerance", "alpha", "device", "lanes_per_atom", "cell_reach", "search_mode"]
     for key in simple:
         if options.get(key) is not None:
             setattr(o, key, options[key])
@@ -493,6 +493,16 @@ class GpuSimulation:
         c = out.reshape(grid[0], grid[1], half, 2)
         return c[..., 0] + 1j * c[..., 1]
 
+    def debug_noise(self, step):
+        """The thermostat's device draws at `step` (rmd_debug_noise): the
+        uniforms (N x 4) and the normals (N x 3), both float32."""
+        n = self.system.atom_count
+        u = np.zeros(4 * n, dtype=np.float32)
+        g = np.zeros(3 * n, dtype=np.float32)
+        _check(L.load().rmd_debug_noise(self._h, int(step), u.ctypes.data_as(L._fp), g.ctypes.data_as(L._fp)),
+               self._h)
+        return u.reshape(n, 4), g.reshape(n, 3)
+
     def set_lanes(self, lanes):
         _check(L.load().rmd_set_lanes(self._h, int(lanes)), self._h)
 
@@ -546,6 +556,14 @@ class GpuSimulation:
         ms = C.c_double()
         _check(L.load().rmd_profile_kernel(self._h, ids[kernel], int(reps), C.byref(ms)), self._h)
         return ms.value
+
+    def profile_pme_stages(self, reps=50):
+        """Average durations (ms) of the PME chain's stages: spread (with the
+        forward transforms it carries), transforms and convolution, gather
+        (with the inverse transforms)."""
+        out = np.zeros(3)
+        _check(L.load().rmd_profile_pme_stages(self._h, int(reps), out.ctypes.data_as(L._dp)), self._h)
+        return dict(zip(("spread", "transform", "gather"), out.tolist()))
 
     def release(self):
         if getattr(self, "_h", None):
