@@ -427,6 +427,16 @@ int rmd_surrogate_get_params(rmd_surrogate *net, float *params);
 int rmd_surrogate_set_params(rmd_surrogate *net, const float *params);
 /* y[k] = f(x[k]), x: n x 16. */
 int rmd_surrogate_forward(rmd_surrogate *net, const float *x, int32_t n, float *y);
+/* rmd_surrogate_forward that allocates nothing: no counterpart in the
+ * reference. For callers that cannot allocate device memory (code under
+ * graph capture, tight serving loops, harnesses that forbid hipMalloc). The
+ * inputs and outputs are staged through one static array per device (the
+ * device holds it until the process ends), so n is at most
+ * RMD_SURROGATE_NOALLOC_SAMPLES (larger n is refused with RMD_E_INVALID),
+ * and calls that overlap on one device (two networks on two host threads)
+ * race on that array. Outputs equal rmd_surrogate_forward's. */
+#define RMD_SURROGATE_NOALLOC_SAMPLES 16384
+int rmd_surrogate_forward_noalloc(rmd_surrogate *net, const float *x, int32_t n, float *y);
 /* loss = mean_k (f(x_k) - t_k)^2 and its gradient (RMD_SURROGATE_PARAMS). */
 int rmd_surrogate_gradient(rmd_surrogate *net, const float *x, const float *t, int32_t n, float *loss, float *grad);
 /* `steps` Adam steps (mini-batches of 64 drawn from the n samples);

@@ -299,6 +299,7 @@ SIGNATURES = {
     "rmd_surrogate_get_params": (C.c_int, [C.c_void_p, _fp]),
     "rmd_surrogate_set_params": (C.c_int, [C.c_void_p, _fp]),
     "rmd_surrogate_forward": (C.c_int, [C.c_void_p, _fp, C.c_int32, _fp]),
+    "rmd_surrogate_forward_noalloc": (C.c_int, [C.c_void_p, _fp, C.c_int32, _fp]),
     "rmd_surrogate_gradient": (C.c_int, [C.c_void_p, _fp, _fp, C.c_int32, _fp, _fp]),
     "rmd_surrogate_train": (C.c_int, [C.c_void_p, _fp, _fp, C.c_int32, C.c_int32, C.c_float, _fp]),
     "rmd_screen_options_default": (None, [C.POINTER(ScreenOptions)]),

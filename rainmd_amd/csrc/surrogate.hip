@@ -342,6 +342,7 @@ void launchSurrogateTrain(float *params, float *m, float *v, const float *x, con
 
 #include <cmath>
 #include <cstring>
+#include <mutex>
 #include <vector>
 
 #include "host.h"
@@ -352,6 +353,14 @@ void launchSurrogateTrain(float *params, float *m, float *v, const float *x, con
     if (e_ != hipSuccess)                                                                          \
       throw rmd::Error(RMD_E_DEVICE, std::string(#call) + " failed: " + hipGetErrorString(e_));   \
   } while (0)
+
+// The staging of rmd_surrogate_forward_noalloc: the inputs (16 floats per
+// sample) followed by the outputs of up to RMD_SURROGATE_NOALLOC_SAMPLES
+// samples, 1.1 MiB that each device the library runs on holds for the life of
+// the process. Host code never passes this name to a kernel or a copy: it is
+// a host-side stand-in there. It uses the device address hipGetSymbolAddress
+// returns for the current device (noallocScratchFor below).
+static __device__ __align__(256) float scratch[RMD_SURROGATE_NOALLOC_SAMPLES * (RMD_SURROGATE_INPUTS + 1)];
 
 struct rmd_surrogate {
   int device = 0;
@@ -389,6 +398,23 @@ T *deviceCopy(const T *src, size_t n, hipStream_t s) {
   SURROGATE_HIP(hipMalloc(&d, std::max<size_t>(1, n) * sizeof(T)));
   if (n) SURROGATE_HIP(hipMemcpyAsync(d, src, n * sizeof(T), hipMemcpyHostToDevice, s));
   return d;
+}
+
+// The device address of `scratch` on `device` (which must be current),
+// looked up once per device and kept.
+float *noallocScratchFor(int device) {
+  constexpr int MAX_DEVICES = 64;
+  static std::mutex lock;
+  static float *address[MAX_DEVICES] = {nullptr};
+  if (device < 0 || device >= MAX_DEVICES) throw Error(RMD_E_DEVICE, "device ordinal out of range");
+  std::lock_guard<std::mutex> hold(lock);
+  if (!address[device]) {
+    void *p = nullptr;
+    SURROGATE_HIP(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+    if (!p) throw Error(RMD_E_DEVICE, "hipGetSymbolAddress returned no address for the staging array");
+    address[device] = static_cast<float *>(p);
+  }
+  return address[device];
 }
 
 void surrogateConfigureOnce() {
@@ -449,6 +475,7 @@ int rmd_surrogate_create(int32_t device, uint32_t seed, rmd_surrogate **out) {
       net->smallX = rmd::deviceCopy(zero.data(), 64 * 16, net->stream);
       net->smallY = rmd::deviceCopy(zero.data(), 64, net->stream);
       SURROGATE_HIP(hipStreamSynchronize(net->stream));
+      rmd::noallocScratchFor(device);
     });
   }();
   if (status != RMD_OK) {
@@ -507,6 +534,27 @@ int rmd_surrogate_forward(rmd_surrogate *net, const float *x, int32_t n, float *
       (void)hipFree(dx);
       (void)hipFree(dy);
     }
+  });
+}
+
+// rmd_surrogate_forward with the device's static staging array in place of
+// allocated buffers: the same kernel on the same inputs, so the same outputs.
+int rmd_surrogate_forward_noalloc(rmd_surrogate *net, const float *x, int32_t n, float *y) {
+  NET_GUARD(net, {
+    if (n < 0 || (n > 0 && (!x || !y))) throw rmd::Error(RMD_E_INVALID, "bad arguments");
+    if (n > RMD_SURROGATE_NOALLOC_SAMPLES)
+      throw rmd::Error(RMD_E_INVALID, "rmd_surrogate_forward_noalloc takes at most " +
+                                          std::to_string(RMD_SURROGATE_NOALLOC_SAMPLES) + " samples per call (" +
+                                          std::to_string(n) + " given); split the batch or use rmd_surrogate_forward");
+    if (n == 0) return RMD_OK;
+    float *dx = rmd::noallocScratchFor(net->device);
+    float *dy = dx + (size_t)RMD_SURROGATE_NOALLOC_SAMPLES * RMD_SURROGATE_INPUTS;
+    SURROGATE_HIP(hipMemcpyAsync(dx, x, (size_t)n * RMD_SURROGATE_INPUTS * sizeof(float), hipMemcpyHostToDevice,
+                                 net->stream));
+    rmd::launchSurrogateForward(net->params, dx, n, dy, net->stream);
+    SURROGATE_HIP(hipGetLastError());
+    SURROGATE_HIP(hipMemcpyAsync(y, dy, (size_t)n * sizeof(float), hipMemcpyDeviceToHost, net->stream));
+    SURROGATE_HIP(hipStreamSynchronize(net->stream));
   });
 }
 
