@@ -278,6 +278,11 @@ typedef struct rmd_energy_terms {
 } rmd_energy_terms;
 int rmd_energy(rmd_ctx *ctx, rmd_energy_terms *out);
 
+/* The context's atom count N (the reference's GpuSimulation.atomCount,
+ * webgpu/simulation.js:92), which sizes every per-atom array below; a
+ * negative RMD_E_* status for a null context. */
+int32_t rmd_atom_count(const rmd_ctx *ctx);
+
 /* readPositions() / readState() (simulation.js:1239-1261). Positions,
  * velocities and forces are 3N float (f32), like the reference's readback. */
 int rmd_read_positions(rmd_ctx *ctx, float *positions);

@@ -288,6 +288,7 @@ SIGNATURES = {
     "rmd_step_once": (C.c_int, [C.c_void_p]),
     "rmd_debug_fft_forward": (C.c_int, [C.c_void_p, _fp, _fp]),
     "rmd_debug_noise": (C.c_int, [C.c_void_p, C.c_int64, _fp, _fp]),
+    "rmd_atom_count": (C.c_int32, [C.c_void_p]),
     "rmd_profile_pme_stages": (C.c_int, [C.c_void_p, C.c_int32, _dp]),
     "rmd_set_lanes": (C.c_int, [C.c_void_p, C.c_int32]),
     "rmd_set_graphs": (C.c_int, [C.c_void_p, C.c_int32]),

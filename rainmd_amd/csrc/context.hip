@@ -1401,6 +1401,8 @@ int rmd_evaluate(rmd_ctx *c, int32_t terms, rmd_energy_terms *energies, double *
   });
 }
 
+int32_t rmd_atom_count(const rmd_ctx *c) { return c ? c->d.n : RMD_E_INVALID; }
+
 int rmd_read_positions(rmd_ctx *c, float *out) {
   CTX_GUARD(c, {
     std::vector<float4> p(c->d.nPad);

@@ -266,6 +266,19 @@ CtxBox *unwrapCtx(napi_env env, napi_value v) {
   return box;
 }
 
+// A context that no asynchronous call is using: the synchronous entry points
+// enqueue on the context's stream and read its host state, which a run,
+// minimise or move on a worker thread (or its batch's run) owns until its
+// promise settles.
+CtxBox *unwrapIdleCtx(napi_env env, napi_value v) {
+  CtxBox *box = unwrapCtx(env, v);
+  if (box && box->busy) {
+    napi_throw(env, makeError(env, RMD_E_INVALID, "another call on this simulation is still running"));
+    return nullptr;
+  }
+  return box;
+}
+
 napi_value args(napi_env env, napi_callback_info info, size_t want, napi_value *argv) {
   size_t argc = want;
   for (size_t k = 0; k < want; ++k) napi_get_undefined(env, &argv[k]);
@@ -601,7 +614,7 @@ napi_value Destroy(napi_env env, napi_callback_info info) {
 napi_value SetState(napi_env env, napi_callback_info info) {
   napi_value argv[4];
   args(env, info, 4, argv);
-  CtxBox *box = unwrapCtx(env, argv[0]);
+  CtxBox *box = unwrapIdleCtx(env, argv[0]);
   if (!box) return nullptr;
   std::vector<double> pos, vel;
   if (!getDoubles(env, argv[1], pos)) {
@@ -621,7 +634,7 @@ napi_value SetState(napi_env env, napi_callback_info info) {
 napi_value SetNoise(napi_env env, napi_callback_info info) {
   napi_value argv[2];
   args(env, info, 2, argv);
-  CtxBox *box = unwrapCtx(env, argv[0]);
+  CtxBox *box = unwrapIdleCtx(env, argv[0]);
   if (!box) return nullptr;
   std::vector<double> d;
   if (!getDoubles(env, argv[1], d)) {
@@ -675,19 +688,36 @@ napi_value EnergyObject(napi_env env, const rmd_energy_terms &e) {
 napi_value Energy(napi_env env, napi_callback_info info) {
   napi_value argv[1];
   args(env, info, 1, argv);
-  CtxBox *box = unwrapCtx(env, argv[0]);
+  CtxBox *box = unwrapIdleCtx(env, argv[0]);
   if (!box) return nullptr;
   rmd_energy_terms e{};
   if (!check(env, rmd_energy(box->ctx, &e), box->ctx)) return nullptr;
   return EnergyObject(env, e);
 }
 
+// The context's atom count, which sizes every per-atom read; a count the
+// caller passes must agree with it.
+bool atomCount(napi_env env, CtxBox *box, napi_value given, size_t &n) {
+  const int32_t count = rmd_atom_count(box->ctx);
+  if (count < 0) {
+    napi_throw(env, makeError(env, count, "the simulation has no atom count"));
+    return false;
+  }
+  if (!isNullish(env, given) && (size_t)getNumber(env, given, count) != (size_t)count) {
+    napi_throw(env, makeError(env, RMD_E_INVALID, "the atom count given does not match the simulation's"));
+    return false;
+  }
+  n = (size_t)count;
+  return true;
+}
+
 napi_value ReadPositions(napi_env env, napi_callback_info info) {
   napi_value argv[2];
   args(env, info, 2, argv);
-  CtxBox *box = unwrapCtx(env, argv[0]);
+  CtxBox *box = unwrapIdleCtx(env, argv[0]);
   if (!box) return nullptr;
-  const size_t n = (size_t)getNumber(env, argv[1], 0);
+  size_t n = 0;
+  if (!atomCount(env, box, argv[1], n)) return nullptr;
   std::vector<float> p(3 * n);
   if (!check(env, rmd_read_positions(box->ctx, p.data()), box->ctx)) return nullptr;
   return f32(env, p.data(), p.size());
@@ -696,9 +726,10 @@ napi_value ReadPositions(napi_env env, napi_callback_info info) {
 napi_value ReadState(napi_env env, napi_callback_info info) {
   napi_value argv[2];
   args(env, info, 2, argv);
-  CtxBox *box = unwrapCtx(env, argv[0]);
+  CtxBox *box = unwrapIdleCtx(env, argv[0]);
   if (!box) return nullptr;
-  const size_t n = (size_t)getNumber(env, argv[1], 0);
+  size_t n = 0;
+  if (!atomCount(env, box, argv[1], n)) return nullptr;
   std::vector<float> p(3 * n), v(3 * n), f(3 * n);
   double b[9], volume = 0;
   if (!check(env, rmd_read_state(box->ctx, p.data(), v.data(), f.data(), b, &volume), box->ctx)) return nullptr;
@@ -714,7 +745,7 @@ napi_value ReadState(napi_env env, napi_callback_info info) {
 napi_value Diagnostics(napi_env env, napi_callback_info info) {
   napi_value argv[1];
   args(env, info, 1, argv);
-  CtxBox *box = unwrapCtx(env, argv[0]);
+  CtxBox *box = unwrapIdleCtx(env, argv[0]);
   if (!box) return nullptr;
   rmd_diagnostics d{};
   if (!check(env, rmd_diagnostics_get(box->ctx, &d), box->ctx)) return nullptr;

@@ -31,6 +31,20 @@ async function main() {
   } catch (e) {
     out.stopped = e.name;
   }
+  // a synchronous call while a run is in flight is refused, not raced
+  const pending = sim.run(200, { blockSize: 25 });
+  out.busy = {};
+  for (const [name, call] of [['energy', () => sim.energy()], ['readState', () => sim.readState()],
+    ['readPositions', () => sim.readPositions()], ['diagnostics', () => sim.diagnostics()],
+    ['setState', async () => sim.setState(new Float64Array(3 * sim.atomCount), null, { step: 0 })]]) {
+    try {
+      await call();
+      out.busy[name] = null;
+    } catch (e) {
+      out.busy[name] = e.message;
+    }
+  }
+  out.afterBusy = await pending;
   const diag = await sim.diagnostics();
   out.diagnostics = { rebuilds: diag.rebuilds, barostat: diag.barostat, pmePlanes: diag.pmePlanes };
   sim.release();

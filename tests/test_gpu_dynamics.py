@@ -177,3 +177,69 @@ def test_stop_between_blocks_and_nonfinite_input_are_errors():
     bad[7] = np.nan
     with pytest.raises(R.NonFiniteStateError):
         sim.set_state(bad)
+
+
+def _drift_fit(times_ps, totals, atoms):
+    """Least-squares slope of the total energy, kJ/mol/ns per atom."""
+    t = np.asarray(times_ps) / 1000.0
+    e = np.asarray(totals)
+    return float(np.polyfit(t, e, 1)[0] / atoms)
+
+
+def test_conservative_run_drift_is_below_the_bar():
+    # tools/gpu/check-integrator.mjs:130-260, the conservative arm: no
+    # friction, 20 ps from the solvated fixture, the total energy's
+    # least-squares slope under 5 kJ/mol/ns/atom. The direct-space force is
+    # the rational fit, not the exact gradient of the erfc energy, so this
+    # is also what that approximation costs in conservation.
+    system, coords = fixture_system("ala-wat", dict(constraints="HBonds", rigid_water=True, cutoff=1.0))
+    sim = R.GpuSimulation(system, alpha=ALPHA_24, grid=[24, 24, 24], temperature=298.0, friction=0.0, seed=7,
+                          skin=0.2)
+    o = O.Oracle(system)
+    x0 = coords.positions.astype(np.float64)
+    x0, _ = o.shake(x0, x0, 1e-12)
+    v0, _ = o.rattle(x0, R.maxwell_boltzmann(system, 298.0, 3), 1e-12)
+    sim.set_state(x0, v0, step=0)
+    sim.initialise()
+    trace = [sim.energy()]
+    for _ in range(50):
+        sim.run(200, 20)
+        trace.append(sim.energy())
+    times = [e["time"] for e in trace]
+    totals = [e["total"] for e in trace]
+    temps = [e["temperature"] for e in trace]
+    n = system.atom_count
+    windows = {f"{times[k]:.0f} ps": _drift_fit(times[:k + 1], totals[:k + 1], n) for k in (12, 25, 50)}
+    spread = float(np.std(totals))
+    print("drift windows (kJ/mol/ns/atom)", windows, "total spread", spread, "mean T", np.mean(temps))
+    drift = _drift_fit(times, totals, n)
+    assert abs(drift) < 5.0
+    assert constraint_error(system, sim.read_positions()) < 2e-5
+    d = sim.diagnostics()
+    assert not d["shake_hit_iteration_cap"] and not d["rattle_hit_iteration_cap"]
+
+
+def test_thermostat_holds_the_setpoint_over_200_ps():
+    # equipartition: after 10 ps of equilibration, the mean temperature of
+    # 200 ps at friction 1/ps sampled every 100 fs is the bath's within 2 K
+    # (the reference measured a -0.07 K offset at 2 fs, docs/STATUS.md:48).
+    # Measured over 100 ps: 300.27 K with a 1.0 K standard error from 10 ps
+    # blocks, so 200 ps puts the bar near three standard errors.
+    system = R.workload_system("ala-389w")
+    sim = R.GpuSimulation(system, temperature=300.0, friction=1.0, seed=12)
+    sim.minimise(max_iterations=300, force_tolerance=1000.0)
+    sim.set_state(sim.read_positions(), R.maxwell_boltzmann(system, 300.0, 12), step=0)
+    sim.run(5000, 25)
+    temps = []
+    for _ in range(2000):
+        sim.run(50, 25)
+        temps.append(sim.energy()["temperature"])
+    temps = np.array(temps)
+    # the standard error from blocks of 10 ps
+    blocks = temps.reshape(20, -1).mean(1)
+    error = blocks.std(ddof=1) / np.sqrt(len(blocks))
+    print("mean T %.3f K, block standard error %.3f K, sample std %.2f K" % (temps.mean(), error, temps.std()))
+    assert abs(temps.mean() - 300.0) < 2.0
+    # the instantaneous fluctuation is the canonical sqrt(2/DOF) T
+    dof = system.degrees_of_freedom
+    assert abs(temps.std() / (300.0 * np.sqrt(2.0 / dof)) - 1) < 0.15

@@ -138,6 +138,10 @@ def test_node_host_runs_on_the_gpu():
     assert np.isfinite(out["after"]["potential"]) and 200 < out["after"]["temperature"] < 700
     assert out["move"] is not None and "accepted" in out["move"]
     assert out["stopped"] == "SimulationStopped"
+    # every synchronous entry point refuses a simulation whose run is in flight
+    for name, message in out["busy"].items():
+        assert message is not None and "still running" in message, name
+    assert out["afterBusy"] == 300
     assert out["diagnostics"]["barostat"]["attempts"] >= 1
     assert "released" in out["released"]
 
