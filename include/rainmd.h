@@ -393,7 +393,9 @@ int rmd_set_graphs(rmd_ctx *ctx, int32_t enabled);
  * neighbour list and the direct space. Results are identical either way. */
 int rmd_set_concurrent_forces(rmd_ctx *ctx, int32_t enabled);
 /* PME pipeline: 0 automatic (planes when one x plane of the grid fits in
- * LDS), 1 planes (RMD_E_INVALID when it does not fit), 2 lines. */
+ * LDS), 1 planes (RMD_E_INVALID when it does not fit), 2 lines, 3 planes with
+ * every plane scanning every atom (no atom bins, which large single systems
+ * and batch members build by default). Results are identical for 1 and 3. */
 int rmd_set_pme_pipeline(rmd_ctx *ctx, int32_t mode);
 int rmd_get_stream(rmd_ctx *ctx, void **stream);
 /* Average launch duration (ms) of one kernel family at the current state,

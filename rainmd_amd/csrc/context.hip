@@ -167,6 +167,7 @@ struct rmd_ctx {
   int windowAttempts = 0, windowAccepted = 0;
   double volumeScale = -1;
   int64_t stepsSinceBarostat = 0;
+  int pmeBinned = 0;   // whether the plane pipeline bins its atoms (set at creation)
   int reachMax = 0;    // the cell plan's widest reach (0: CELL_REACH_MAX)
   int searchMode = 0;  // 0 by occupancy, 1 per atom, 2 per cell where the launch covers the cells
   int neighbourGrowths = 0;
@@ -1142,16 +1143,24 @@ void create(rmd_ctx *c, const System &sys, const rmd_options &o, const Streams *
   p.influence = devAlloc<float>(half, c->held);
   p.piOverAlpha2 = M_PI * M_PI / (c->pme.alpha * c->pme.alpha);
   pmeConfigure(p, PME_AUTO, c->maxLds);
-  // The plane pipeline's atom bins. A single system scans every atom per
-  // plane (one launch fewer on a short critical path); replica batches bin
-  // their members (the same results: see pme.hip).
+  // The plane pipeline's atom bins. A small single system scans every atom
+  // per plane (one launch fewer on a short critical path); replica batches
+  // bin their members in one workgroup each, and a large single system bins
+  // its atoms in groups over the whole chip (the same results: see pme.hip).
   p.binned = 0;
-  if (shared && pmeBinsFit(p, d.n)) {
+  p.binGroups = 0;
+  const bool wide = !shared && d.n >= BIN_SINGLE_MIN_ATOMS && pmeBinsFitWide(p);
+  if ((shared && pmeBinsFit(p, d.n)) || wide) {
     p.binned = 1;
     p.binStart = devAlloc<int>(p.K[0] + 1, c->held);
     p.atomBin = devAlloc<int2>(d.nPad, c->held);
     p.binAtoms = devAlloc<int>(d.nPad, c->held);
+    if (wide) {
+      p.binGroups = (d.n + BIN_GROUP_ATOMS - 1) / BIN_GROUP_ATOMS;
+      p.binTable = devAlloc<int>((size_t)p.binGroups * p.K[0], c->held);
+    }
   }
+  c->pmeBinned = p.binned;
   writeBox(c);
   setupIntegrator(c, 0);
   // Initial state from the system's own coordinates, if it has them.
@@ -1963,9 +1972,12 @@ int rmd_set_concurrent_forces(rmd_ctx *c, int32_t enabled) {
 
 int rmd_set_pme_pipeline(rmd_ctx *c, int32_t mode) {
   CTX_GUARD(c, {
-    if (mode < PME_AUTO || mode > PME_LINES) throw Error(RMD_E_INVALID, "PME pipeline must be 0, 1 or 2");
-    if (!pmeConfigure(c->p, mode, c->maxLds))
+    if (mode < PME_AUTO || mode > PME_LINES + 1) throw Error(RMD_E_INVALID, "PME pipeline must be 0, 1, 2 or 3");
+    if (!pmeConfigure(c->p, mode == PME_LINES + 1 ? PME_PLANES : mode, c->maxLds))
       throw Error(RMD_E_INVALID, "one x plane of this PME grid does not fit in LDS");
+    // planes without atom bins (every plane scans every atom), or the bins
+    // the context was created with
+    c->p.binned = mode == PME_LINES + 1 ? 0 : c->pmeBinned;
     invalidateGraphs(c);
   });
 }

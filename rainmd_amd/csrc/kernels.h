@@ -44,10 +44,15 @@ struct PmeDev {
   int planes;               // 1: plane pipeline, 0: line pipeline
   // The plane pipeline's atoms by the base index of their x support (K0
   // bins): a plane reads the ORDER bins that reach it instead of every atom.
-  int binned;               // 1: the bins are built each evaluation (replica batches)
+  int binned;               // 1: the bins are built each evaluation (replica batches, large systems)
   int *binStart;            // K0 + 1 offsets into binAtoms
   int2 *atomBin;            // per atom: (bin or -1 for no charge, rank in the bin)
   int *binAtoms;            // atoms grouped by bin
+  // A large single system builds its bins in three launches over groups of
+  // BIN_GROUP_ATOMS atoms (binGroups > 0): per-group counts, their offsets,
+  // the scatter. 0: one workgroup builds them (batch members).
+  int binGroups;
+  int *binTable;            // K0 x binGroups counts, then offsets within each bin
 };
 enum { PME_AUTO = 0, PME_PLANES = 1, PME_LINES = 2 };
 // Chooses the pipeline (PME_AUTO: planes when a plane fits in maxLds bytes of
@@ -56,6 +61,12 @@ bool pmeConfigure(PmeDev &p, int mode, size_t maxLds);
 // Whether a plane-pipeline system can bin its atoms in one workgroup.
 constexpr int BIN_MAX_ATOMS = 16384;
 bool pmeBinsFit(const PmeDev &p, int atoms);
+// Single systems of at least this many atoms bin their plane-pipeline atoms
+// (smaller ones scan every atom per plane: one launch fewer on a short
+// critical path), in groups of BIN_GROUP_ATOMS.
+constexpr int BIN_SINGLE_MIN_ATOMS = 8192;
+constexpr int BIN_GROUP_ATOMS = 1024;
+bool pmeBinsFitWide(const PmeDev &p);
 void pmeComputeInfluence(const PmeDev &p, const BoxF *box, const double *moduli[3], double alpha, hipStream_t s);
 int launchPme(const Dev &d, const PmeDev &p, int energies, hipStream_t s, int *slotsUsed);
 // A small system's direct space, bonded terms and PME forward planes in one

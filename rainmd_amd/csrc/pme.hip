@@ -98,6 +98,99 @@ bool pmeBinsFit(const PmeDev &p, int atoms) {
   return p.planes && p.K[0] >= ORDER && p.K[0] <= BIN_MAX_K && atoms <= BIN_MAX_ATOMS;
 }
 
+bool pmeBinsFitWide(const PmeDev &p) { return p.planes && p.K[0] >= ORDER && p.K[0] <= BIN_MAX_K; }
+
+// The same bins for a large single system, in three launches. 1: each group
+// of BIN_GROUP_ATOMS atoms counts its charged atoms per bin in LDS (an atom's
+// rank in its group's bin comes from the LDS atomic) and writes the counts,
+// bin-major (entry b * binGroups + g).
+constexpr int BIN_WIDE_THREADS = 256;
+
+__global__ void __launch_bounds__(BIN_WIDE_THREADS) pmeBinCountKernel(Dev d, PmeDev p) {
+  __shared__ int count[BIN_MAX_K];
+  const int K0 = p.K[0], G = p.binGroups, t = threadIdx.x;
+  for (int b = t; b < K0; b += BIN_WIDE_THREADS) count[b] = 0;
+  __syncthreads();
+  const double *recip = d.box->recip64;
+  const int first = blockIdx.x * BIN_GROUP_ATOMS, last = min(d.n, first + BIN_GROUP_ATOMS);
+  for (int a = first + t; a < last; a += BIN_WIDE_THREADS) {
+    const float4 pos = d.pos[a];
+    int2 e = make_int2(-1, 0);
+    if (pos.w != 0.f) {
+      int bx;
+      float ox;
+      gridCoordinate(p, recip, 0, pos, bx, ox);
+      e = make_int2(bx, atomicAdd(&count[bx], 1));
+    }
+    p.atomBin[a] = e;
+  }
+  __syncthreads();
+  for (int b = t; b < K0; b += BIN_WIDE_THREADS) p.binTable[(size_t)b * G + blockIdx.x] = count[b];
+}
+
+// 2: one workgroup turns the counts into offsets within each bin (the groups
+// in order; a wave scans a bin's counts 64 at a time) and the bins' starts.
+__global__ void __launch_bounds__(BIN_THREADS) pmeBinOffsetsKernel(PmeDev p) {
+  __shared__ int total[BIN_MAX_K];
+  __shared__ int waveTotal[BIN_THREADS / 64];
+  const int K0 = p.K[0], G = p.binGroups, t = threadIdx.x;
+  const int lane = t & 63, wave = t >> 6, waves = BIN_THREADS / 64;
+  for (int b = wave; b < K0; b += waves) {
+    int *row = p.binTable + (size_t)b * G;
+    int carry = 0;
+    for (int g0 = 0; g0 < G; g0 += 64) {
+      const int g = g0 + lane;
+      const int v = g < G ? row[g] : 0;
+      int inclusive = v;
+      for (int offset = 1; offset < 64; offset <<= 1) {
+        const int u = __shfl_up(inclusive, offset, 64);
+        if (lane >= offset) inclusive += u;
+      }
+      if (g < G) row[g] = carry + inclusive - v;
+      carry += __shfl(inclusive, 63, 64);
+    }
+    if (lane == 0) total[b] = carry;
+  }
+  __syncthreads();
+  // the bins' starts: one bin per thread (K0 <= BIN_MAX_K = BIN_THREADS)
+  const int sum = t < K0 ? total[t] : 0;
+  int inclusive = sum;
+  for (int offset = 1; offset < 64; offset <<= 1) {
+    const int u = __shfl_up(inclusive, offset, 64);
+    if (lane >= offset) inclusive += u;
+  }
+  if (lane == 63) waveTotal[wave] = inclusive;
+  __syncthreads();
+  int before = 0;
+  for (int w = 0; w < wave; ++w) before += waveTotal[w];
+  const int start = before + inclusive - sum;
+  if (t < K0) p.binStart[t] = start;
+  if (t == K0 - 1) p.binStart[K0] = start + sum;
+}
+
+// 3: each atom goes to its bin's start plus its group's offset in the bin
+// plus its rank.
+__global__ void __launch_bounds__(BIN_WIDE_THREADS) pmeBinScatterKernel(Dev d, PmeDev p) {
+  const int G = p.binGroups;
+  const int first = blockIdx.x * BIN_GROUP_ATOMS, last = min(d.n, first + BIN_GROUP_ATOMS);
+  for (int a = first + threadIdx.x; a < last; a += BIN_WIDE_THREADS) {
+    const int2 e = p.atomBin[a];
+    if (e.x >= 0) p.binAtoms[p.binStart[e.x] + p.binTable[(size_t)e.x * G + blockIdx.x] + e.y] = a;
+  }
+}
+
+// The bins of a binned system, by whichever build it was given.
+static void launchPmeBins(const Dev &d, const PmeDev &p, hipStream_t s) {
+  if (!p.binned) return;
+  if (p.binGroups == 0) {
+    pmeBinKernel<<<1, BIN_THREADS, 0, s>>>(d, p);
+    return;
+  }
+  pmeBinCountKernel<<<p.binGroups, BIN_WIDE_THREADS, 0, s>>>(d, p);
+  pmeBinOffsetsKernel<<<1, BIN_THREADS, 0, s>>>(p);
+  pmeBinScatterKernel<<<p.binGroups, BIN_WIDE_THREADS, 0, s>>>(d, p);
+}
+
 __global__ void __launch_bounds__(PLANE_THREADS) pmePlaneForwardKernel(Dev d, PmeDev p, const float *real) {
   extern __shared__ float2 lds[];
   pmePlaneForward(d, p, real, blockIdx.x, lds);
@@ -677,7 +770,7 @@ int launchPmeAfterForward(const Dev &d, const PmeDev &p, int energies, hipStream
 
 int launchPme(const Dev &d, const PmeDev &p, int energies, hipStream_t s, int *slotsUsed) {
   if (p.planes) {
-    if (p.binned) pmeBinKernel<<<1, BIN_THREADS, 0, s>>>(d, p);
+    launchPmeBins(d, p, s);
     pmePlaneForwardKernel<<<p.K[0], PLANE_THREADS, planeForwardLds(p), s>>>(d, p, nullptr);
     const int xBlocks = xLineBlocks(p);
     pmeXLinesKernel<<<xBlocks, XLINE_THREADS, xLinesLds(p), s>>>(d, p, 1, energies);
@@ -709,7 +802,7 @@ int launchPme(const Dev &d, const PmeDev &p, int energies, hipStream_t s, int *s
 void launchPmeStages(const Dev &d, const PmeDev &p, hipStream_t s, hipEvent_t ev[4]) {
   (void)hipEventRecord(ev[0], s);
   if (p.planes) {
-    if (p.binned) pmeBinKernel<<<1, BIN_THREADS, 0, s>>>(d, p);
+    launchPmeBins(d, p, s);
     pmePlaneForwardKernel<<<p.K[0], PLANE_THREADS, planeForwardLds(p), s>>>(d, p, nullptr);
     (void)hipEventRecord(ev[1], s);
     pmeXLinesKernel<<<xLineBlocks(p), XLINE_THREADS, xLinesLds(p), s>>>(d, p, 1, 0);

@@ -540,8 +540,8 @@ class GpuSimulation:
         _check(L.load().rmd_set_concurrent_forces(self._h, 1 if enabled else 0), self._h)
 
     def set_pme_pipeline(self, mode):
-        """'auto', 'planes' or 'lines'."""
-        ids = {"auto": 0, "planes": 1, "lines": 2}
+        """'auto', 'planes', 'lines' or 'planes-scan' (planes without atom bins)."""
+        ids = {"auto": 0, "planes": 1, "lines": 2, "planes-scan": 3}
         _check(L.load().rmd_set_pme_pipeline(self._h, ids[mode]), self._h)
 
     def stream(self):

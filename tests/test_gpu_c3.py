@@ -84,6 +84,21 @@ def test_c3_pipelines_agree(c3):
     assert rel_rms(fp, fl) < 3e-5  # two FP32 transform orders; each is within 1e-5 of FP64
 
 
+def test_c3_binned_planes_equal_scanned_planes(c3):
+    # a large single system bins its atoms by x plane over the whole chip;
+    # each plane then spreads only the atoms that reach it, and the gather
+    # runs per atom from the potential grid: the forces and the energy are
+    # those of planes that scan every atom, bit for bit
+    system, sim = c3
+    binned, fb = sim.evaluate(terms=R._lib.TERM_RECIPROCAL)
+    sim.set_pme_pipeline("planes-scan")
+    scanned, fs = sim.evaluate(terms=R._lib.TERM_RECIPROCAL)
+    sim.set_pme_pipeline("auto")
+    print("C3 reciprocal binned", binned["coulomb_reciprocal"], "scanned", scanned["coulomb_reciprocal"])
+    assert binned["coulomb_reciprocal"] == scanned["coulomb_reciprocal"]
+    assert np.array_equal(fb, fs)
+
+
 def test_c3_steps_hold_the_water_geometry(c3):
     system, sim = c3
     sim.set_state(sim.read_positions().astype(np.float64), R.maxwell_boltzmann(system, 300.0, 13), step=0)
