@@ -234,7 +234,7 @@ def replica_batch(args, count, warmup, steps, single_steps, screen=False):
     atoms = [s.atom_count for s in systems]
     log(f"{count} replicas, {sum(atoms)} atoms ({min(atoms)}-{max(atoms)}), built in {time.perf_counter() - t0:.1f} s")
     t0 = time.perf_counter()
-    extra = dict(skin=args.replica_skin) if args.replica_skin > 0 else {}
+    extra = dict(skin=args.replica_skin) if args.replica_skin != 0 else {}
     if args.lanes > 0:
         extra["lanes_per_atom"] = args.lanes
     batch = R.Batch(systems, seeds=seeds, temperature=300.0, pressure=1.0, barostat_interval=25, **extra)
@@ -488,7 +488,9 @@ def main():
     ap.add_argument("--steps", type=int, default=5000)
     ap.add_argument("--warmup", type=int, default=1000)
     ap.add_argument("--workload", default="ala-944w")
-    ap.add_argument("--skin", type=float, default=0.0, help="nm; 0 derives the reference's skinFor")
+    ap.add_argument("--skin", type=float, default=None,
+                    help="nm; 0 derives the reference's skinFor, < 0 takes the autotuner's skin; default: the "
+                         "workload's configured skin, else the autotuner's")
     ap.add_argument("--lanes", type=int, default=0)
     ap.add_argument("--block", type=int, default=25)
     ap.add_argument("--prep-steps", type=int, default=5000,
@@ -516,7 +518,8 @@ def main():
     ap.add_argument("--replica-minimise", type=int, default=150)
     ap.add_argument("--serial-forces", action="store_true",
                     help="run the force branches one after another (for per-kernel profiles)")
-    ap.add_argument("--replica-skin", type=float, default=0.15, help="replica workloads: nm; 0 derives skinFor")
+    ap.add_argument("--replica-skin", type=float, default=-1.0,
+                    help="replica workloads: nm; 0 derives skinFor, < 0 takes the autotuner's skin")
     ap.add_argument("--screen", action="store_true", help="replica workloads: the shared learned barostat screen")
     ap.add_argument("--screen-training-moves", type=int, default=2000)
     args = ap.parse_args()
@@ -535,8 +538,10 @@ def main():
     system = R.workload_system(args.workload)
     spec = WORKLOADS[args.workload]
     opts = dict(temperature=300.0, seed=202, **spec["options"])
-    if args.skin > 0:
+    if args.skin is not None:
         opts["skin"] = args.skin
+    elif "skin" not in opts:
+        opts["skin"] = -1.0
     if args.lanes > 0:
         opts["lanes_per_atom"] = args.lanes
     sim = R.GpuSimulation(system, **opts)

@@ -214,7 +214,8 @@ typedef struct rmd_options {
   int32_t restraint_count;
   const int32_t *restraint_atoms;       /* restraint_count                     */
   const double *restraint_reference;    /* 3N (only restrained entries read)   */
-  double skin;                   /* nm; <= 0 derives skinFor                   */
+  double skin;                   /* nm; 0 derives skinFor, < 0 takes the
+                                    autotuner's skin for the workload         */
   double neighbour_slack;        /* 2.0                                        */
   int32_t max_neighbours;        /* <= 0 estimates                             */
   double ewald_tolerance;        /* 5e-4                                       */
@@ -223,9 +224,14 @@ typedef struct rmd_options {
   int32_t device;                /* HIP device ordinal                         */
   int32_t lanes_per_atom;        /* nonbonded lanes per atom (1..64), 0 = tuned */
   int32_t cell_reach;            /* widest stencil reach the cell plan may take
-                                    (1..3; 1 is the reference's plan), 0 = 3    */
-  int32_t search_mode;           /* list search: 0 by the plan's cell occupancy,
-                                    1 one wave per atom, 2 one wave per cell   */
+                                    (1..3; 1 is the reference's plan), 0 = tuned */
+  int32_t search_mode;           /* list search: 1 one wave per atom, 2 one wave
+                                    per cell, 3 by the plan's cell occupancy,
+                                    0 = tuned                                  */
+  int32_t pme_bins;              /* PME planes of a single system: 1 bin the
+                                    atoms by x plane, 2 every plane scans every
+                                    atom, 3 by atom count, 0 = tuned (batch
+                                    members always bin)                        */
 } rmd_options;
 void rmd_options_default(rmd_options *options);
 
@@ -384,6 +390,19 @@ int rmd_debug_noise(rmd_ctx *ctx, int64_t step, float *uniforms, float *normals)
 /* Tuning and plumbing: the nonbonded lanes per atom (a power of two <= 64),
  * hipGraph capture of step blocks on/off, and the context's HIP stream (for
  * timing on that stream). */
+/* The autotuner's shapes compiled into this library for a single system
+ * (kind 0) or a replica batch (kind 1) of `atoms` atoms: lanes per atom, the
+ * cell plan's widest reach, the search mode, the tuned skin (nm; 0 means the
+ * reference's skinFor) and the PME bin choice. Options left at 0 (and a
+ * negative skin) take these values. No counterpart in the reference, whose
+ * skin and rebuild interval were swept by hand (tools/gpu/bench-skin.mjs). */
+typedef struct rmd_tuned_shape {
+  int32_t lanes_per_atom, cell_reach, search_mode;
+  double skin;
+  int32_t pme_bins;
+} rmd_tuned_shape;
+int rmd_tuned_shape_get(int32_t kind, int32_t atoms, rmd_tuned_shape *out);
+
 int rmd_set_lanes(rmd_ctx *ctx, int32_t lanes);
 int rmd_set_graphs(rmd_ctx *ctx, int32_t enabled);
 /* Overlapped force branches (1, the default) or each branch launched on its

@@ -8,5 +8,5 @@ from .simulation import (  # noqa: F401
     ConstraintFailure, Coordinates, DeviceError, GpuBudgetError, GpuSimulation, NeighbourOverflowError,
     NonFiniteStateError, ParseError, RainmdError, SimulationStopped, System, SystemError_, Topology,
     WORKLOADS, box_vectors_from_lengths_and_angles, device_info, pme_moduli, pme_parameters, read_data,
-    workload_system, maxwell_boltzmann, uniform_at, Surrogate, Batch, replica_workload, replica_systems,
+    workload_system, maxwell_boltzmann, uniform_at, Surrogate, Batch, replica_workload, replica_systems, tuned_shape,
 )

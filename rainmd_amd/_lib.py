@@ -130,6 +130,7 @@ class Options(C.Structure):
         ("lanes_per_atom", C.c_int32),
         ("cell_reach", C.c_int32),
         ("search_mode", C.c_int32),
+        ("pme_bins", C.c_int32),
     ]
 
 
@@ -198,6 +199,11 @@ class Diagnostics(C.Structure):
         ("cell_reach", C.c_int32), ("search_by_cell", C.c_int32),
         ("steps_since_barostat", C.c_int64),
     ]
+
+
+class TunedShape(C.Structure):
+    _fields_ = [("lanes_per_atom", C.c_int32), ("cell_reach", C.c_int32), ("search_mode", C.c_int32),
+                ("skin", C.c_double), ("pme_bins", C.c_int32)]
 
 
 class ScreenOptions(C.Structure):
@@ -289,6 +295,7 @@ SIGNATURES = {
     "rmd_debug_fft_forward": (C.c_int, [C.c_void_p, _fp, _fp]),
     "rmd_debug_noise": (C.c_int, [C.c_void_p, C.c_int64, _fp, _fp]),
     "rmd_atom_count": (C.c_int32, [C.c_void_p]),
+    "rmd_tuned_shape_get": (C.c_int, [C.c_int32, C.c_int32, C.c_void_p]),
     "rmd_profile_pme_stages": (C.c_int, [C.c_void_p, C.c_int32, _dp]),
     "rmd_set_lanes": (C.c_int, [C.c_void_p, C.c_int32]),
     "rmd_set_graphs": (C.c_int, [C.c_void_p, C.c_int32]),

@@ -14,6 +14,7 @@
 #include <string>
 #include <vector>
 
+#include "tuned.h"
 #include "host.h"
 #include "kernels.h"
 #include "surrogate.h"
@@ -883,16 +884,25 @@ void create(rmd_ctx *c, const System &sys, const rmd_options &o, const Streams *
   c->barostatInterval = o.barostat_interval > 0 ? o.barostat_interval : 25;
   c->dispersionOn = o.dispersion_correction != 0;
   c->slack = o.neighbour_slack > 0 ? o.neighbour_slack : 2.0;
-  c->lanes = o.lanes_per_atom > 0 ? o.lanes_per_atom : 64;
+  // Shapes the caller leaves at 0 (and a negative skin) come from the
+  // autotuner's table for this kind of workload.
+  const TunedShape &tuned = tunedShape(shared ? 1 : 0, t.atomCount);
+  c->lanes = o.lanes_per_atom > 0 ? o.lanes_per_atom : tuned.lanes;
   if (c->lanes & (c->lanes - 1) || c->lanes > 64) throw Error(RMD_E_INVALID, "lanes_per_atom must be a power of two <= 64");
   for (int k = 0; k < 9; ++k) c->box[k] = sys.box[k];
-  c->skin = o.skin > 0 ? o.skin : skinFor(sys, c->timestep, c->temperature, nullptr);
+  if (o.skin > 0)
+    c->skin = o.skin;
+  else if (o.skin < 0 && tuned.skin > 0)
+    c->skin = tuned.skin;
+  else
+    c->skin = skinFor(sys, c->timestep, c->temperature, nullptr);
   if (o.cell_reach < 0 || o.cell_reach > CELL_REACH_MAX)
     throw Error(RMD_E_INVALID, "cell_reach must be 0 (the default, 3) or 1..3");
-  if (o.search_mode < 0 || o.search_mode > 2)
-    throw Error(RMD_E_INVALID, "search_mode must be 0 (by occupancy), 1 (per atom) or 2 (per cell)");
-  c->reachMax = o.cell_reach;
-  c->searchMode = o.search_mode;
+  if (o.search_mode < 0 || o.search_mode > 3)
+    throw Error(RMD_E_INVALID, "search_mode must be 0 (tuned), 1 (per atom), 2 (per cell) or 3 (by occupancy)");
+  c->reachMax = o.cell_reach > 0 ? o.cell_reach : tuned.reach;
+  // searchMode 0 is the rule by the plan's cell occupancy (option 3)
+  c->searchMode = o.search_mode == 0 ? tuned.searchMode : o.search_mode == 3 ? 0 : o.search_mode;
   c->plan = planCellList(sys, c->skin, c->slack, o.max_neighbours, c->box, c->reachMax);
   {
     rmd_pme_parameters pp;
@@ -1149,7 +1159,11 @@ void create(rmd_ctx *c, const System &sys, const rmd_options &o, const Streams *
   // its atoms in groups over the whole chip (the same results: see pme.hip).
   p.binned = 0;
   p.binGroups = 0;
-  const bool wide = !shared && d.n >= BIN_SINGLE_MIN_ATOMS && pmeBinsFitWide(p);
+  if (o.pme_bins < 0 || o.pme_bins > 3)
+    throw Error(RMD_E_INVALID, "pme_bins must be 0 (tuned), 1 (bins), 2 (scan) or 3 (by atom count)");
+  // bins 0 is the rule by atom count (option 3)
+  const int bins = o.pme_bins == 0 ? tuned.pmeBins : o.pme_bins == 3 ? 0 : o.pme_bins;
+  const bool wide = !shared && pmeBinsFitWide(p) && (bins == 1 || (bins == 0 && d.n >= BIN_SINGLE_MIN_ATOMS));
   if ((shared && pmeBinsFit(p, d.n)) || wide) {
     p.binned = 1;
     p.binStart = devAlloc<int>(p.K[0] + 1, c->held);
@@ -2609,7 +2623,7 @@ void createBatch(rmd_batch *b, const rmd_system *const *systems, int count, cons
   // Lanes per atom: with hundreds of thousands of atoms in flight, fewer
   // lanes per atom and longer per-lane loops hide the gather latency better
   // than the single system's 64.
-  b->lanes = o.lanes_per_atom > 0 ? o.lanes_per_atom : 16;
+  b->lanes = o.lanes_per_atom > 0 ? o.lanes_per_atom : tunedShape(1, 0).lanes;
   if (b->lanes != 8 && b->lanes != 16 && b->lanes != 32 && b->lanes != 64)
     throw Error(RMD_E_INVALID, "a batch runs 8, 16, 32 or 64 lanes per atom");
   b->scAll = devAlloc<Scalars>(count, b->held);

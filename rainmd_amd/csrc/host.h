@@ -147,6 +147,12 @@ constexpr int CELL_ATOMS_SMALL = 16384;
 CellPlan planCellList(const System &system, double skin, double slack, int maxNeighbours,
                       const double *box, int reachMax = 0);
 
+// The autotuner's shapes (tuned.h) for a single system (kind 0) or a batch
+// member (kind 1) of `atoms` atoms: the row of that kind with the largest
+// minAtoms not above it.
+struct TunedShape;
+const TunedShape &tunedShape(int kind, int atoms);
+
 // Dispersion (shared/dispersion.js:37-82) and Ewald self energy
 // (cpu/forces/pme.js:270-286).
 struct DispersionCoefficients {

@@ -15,6 +15,7 @@
 #include <set>
 
 #include "host.h"
+#include "tuned.h"
 
 namespace rmd {
 
@@ -635,6 +636,14 @@ CellPlan planCellList(const System &s, double skin, double slack, int maxNeighbo
   return p;
 }
 
+const TunedShape &tunedShape(int kind, int atoms) {
+  const TunedShape *best = nullptr;
+  for (const TunedShape &t : TUNED_SHAPES)
+    if (t.kind == kind && t.minAtoms <= atoms && (!best || t.minAtoms > best->minAtoms)) best = &t;
+  if (!best) throw Error(RMD_E_INVALID, "the tuning table has no row for this workload");
+  return *best;
+}
+
 // ---- dispersion / self ----------------------------------------------------
 
 DispersionCoefficients dispersionCoefficients(const System &s) {
@@ -896,6 +905,25 @@ bool isSoluteResidue(std::string name) {
   return solvent.count(name) == 0;
 }
 }  // namespace
+
+int rmd_tuned_shape_get(int32_t kind, int32_t atoms, rmd_tuned_shape *out) {
+  if (!out || (kind != 0 && kind != 1) || atoms < 0) {
+    rmd::setGlobalError("bad arguments");
+    return RMD_E_INVALID;
+  }
+  try {
+    const rmd::TunedShape &t = rmd::tunedShape(kind, atoms);
+    out->lanes_per_atom = t.lanes;
+    out->cell_reach = t.reach;
+    out->search_mode = t.searchMode;
+    out->skin = t.skin;
+    out->pme_bins = t.pmeBins;
+    return RMD_OK;
+  } catch (const rmd::Error &e) {
+    rmd::setGlobalError(e.what());
+    return e.code;
+  }
+}
 
 int rmd_system_rebox(const rmd_system *sys, const double box[9], rmd_system **out) {
   RMD_GUARD({

@@ -590,6 +590,7 @@ void readOptions(napi_env env, napi_value opt, rmd_options &o, std::vector<int32
   o.lanes_per_atom = (int32_t)prop(env, opt, "lanesPerAtom", o.lanes_per_atom);
   o.cell_reach = (int32_t)prop(env, opt, "cellReach", o.cell_reach);
   o.search_mode = (int32_t)prop(env, opt, "searchMode", o.search_mode);
+  o.pme_bins = (int32_t)prop(env, opt, "pmeBins", o.pme_bins);
 }
 
 napi_value Destroy(napi_env env, napi_callback_info info) {

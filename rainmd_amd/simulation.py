@@ -301,6 +301,14 @@ def pme_moduli(size, order=5):
     return np.array(out[:])
 
 
+def tuned_shape(kind, atoms):
+    """The autotuner's compiled-in shapes for a single system ('single') or a
+    replica batch ('batch') of `atoms` atoms (rmd_tuned_shape_get)."""
+    out = L.TunedShape()
+    _check(L.load().rmd_tuned_shape_get({"single": 0, "batch": 1}[kind], int(atoms), C.byref(out)))
+    return _struct_dict(out)
+
+
 def device_info(device=0):
     info = L.DeviceInfo()
     rc = L.load().rmd_device_info_get(device, C.byref(info))
@@ -328,7 +336,7 @@ def _options(options):
     lib.rmd_options_default(C.byref(o))
     simple = ["timestep", "temperature", "friction", "seed", "tolerance", "max_constraint_iterations",
               "rebuild_interval", "barostat_interval", "skin", "neighbour_slack", "max_neighbours",
-              "ewald_tolerance", "alpha", "device", "lanes_per_atom", "cell_reach", "search_mode"]
+              "ewald_tolerance", "alpha", "device", "lanes_per_atom", "cell_reach", "search_mode", "pme_bins"]
     for key in simple:
         if options.get(key) is not None:
             setattr(o, key, options[key])

@@ -24,7 +24,11 @@ def _systems(picks):
 
 # small and mid-sized replicas of the workload (1.1k - 10.6k atoms)
 PICKS = [1, 2, 5, 9]
-LANES = 16  # a batch's default lanes per atom; the single runs compared use the same
+# the shapes the batch and the single runs compared share: the lanes per atom
+# and the cell plan's reach (the tuned defaults of a batch member and of a
+# single system may differ, and the cell plan orders the direct space's sums)
+LANES = 16
+SHAPE = dict(lanes_per_atom=LANES, cell_reach=3)
 
 
 def _start_states(systems, seeds):
@@ -45,14 +49,14 @@ def _run_pair(options, steps, graphs=True, max_neighbours=None):
     extra = dict(options)
     if max_neighbours:
         extra["max_neighbours"] = max_neighbours
-    batch = R.Batch(systems, seeds=seeds, lanes_per_atom=LANES, **extra)
+    batch = R.Batch(systems, seeds=seeds, **SHAPE, **extra)
     batch.set_graphs(graphs)
     for r, (x, v) in enumerate(starts):
         batch.replica(r).set_state(x, v, step=0)
     batch.run(steps)
     singles = []
     for r, (sys_, seed) in enumerate(zip(systems, seeds)):
-        sim = R.GpuSimulation(sys_, seed=seed, lanes_per_atom=LANES, **extra)
+        sim = R.GpuSimulation(sys_, seed=seed, **SHAPE, **extra)
         sim.set_graphs(graphs)
         sim.set_state(*starts[r], step=0)
         sim.run(steps)

@@ -80,7 +80,7 @@ def test_batch_with_a_triclinic_replica_equals_single_runs():
         sim.minimise(max_iterations=60, force_tolerance=1000.0)
         starts.append((sim.read_positions().astype(np.float64), R.maxwell_boltzmann(sys_, 300.0, seed)))
         sim.release()
-    options = dict(temperature=300.0, pressure=1.0, barostat_interval=10, lanes_per_atom=16)
+    options = dict(temperature=300.0, pressure=1.0, barostat_interval=10, lanes_per_atom=16, cell_reach=3)
     batch = R.Batch(systems, seeds=seeds, **options)
     for r, (x, v) in enumerate(starts):
         batch.replica(r).set_state(x, v, step=0)
