@@ -276,77 +276,125 @@ __device__ void starLoad(const Dev &d, int cluster, int m, Atom *w, int *atom, d
   }
 }
 
-__device__ void starRattle(const Dev &d, const IntegratorParams &ip, Atom *w, int m, const double *invT2) {
-  bool converged = false, stalled = false;
-  double previous = 1e300;
-  for (int it = 0; it < ip.maxIterations; ++it) {
-    double worst = 0;
+// x = A^-1 b for a star's STAR_MAX x STAR_MAX system by Gaussian elimination
+// without pivoting: A is the constraints' coupling matrix (or, for SHAKE, its
+// Newton Jacobian, close to it), im_0 times a Gram matrix of a few bonds at
+// tetrahedral angles plus a positive diagonal, so its pivots stay large.
+// Rows and columns past the cluster's constraints are the identity.
+__device__ __forceinline__ void solveStar(double A[STAR_MAX][STAR_MAX], double b[STAR_MAX], double x[STAR_MAX]) {
 #pragma unroll
-    for (int k = 0; k < STAR_MAX; ++k) {
-      if (k >= m) continue;
-      Atom &a = w[k + 1], &b = w[0];
-      const double r[3] = {a.x[0] - b.x[0], a.x[1] - b.x[1], a.x[2] - b.x[2]};
-      const double vr[3] = {a.v[0] - b.v[0], a.v[1] - b.v[1], a.v[2] - b.v[2]};
-      const double proj = r[0] * vr[0] + r[1] * vr[1] + r[2] * vr[2];
-      const double rel = fabs(proj) * invT2[k];
-      worst = fmax(worst, rel);
-      if (rel <= ip.tolerance) continue;
-      const double den = (a.im + b.im) * (r[0] * r[0] + r[1] * r[1] + r[2] * r[2]);
-      if (den < 1e-20) continue;
-      const double lam = -proj * reciprocal(den);
-      for (int c = 0; c < 3; ++c) {
-        a.v[c] += lam * a.im * r[c];
-        b.v[c] -= lam * b.im * r[c];
-      }
+  for (int p = 0; p < STAR_MAX; ++p) {
+    const double inv = reciprocal(A[p][p]);
+#pragma unroll
+    for (int i = p + 1; i < STAR_MAX; ++i) {
+      const double f = A[i][p] * inv;
+#pragma unroll
+      for (int j = p + 1; j < STAR_MAX; ++j) A[i][j] -= f * A[p][j];
+      b[i] -= f * b[p];
     }
-    if (worst <= ip.tolerance) {
-      converged = true;
-      break;
-    }
-    if (it >= 2 && worst >= previous * 0.99) {
-      stalled = true;
-      break;
-    }
-    previous = worst;
   }
-  if (!converged && !stalled) atomicMax(&d.sc->rattleFailed, 1u);
+#pragma unroll
+  for (int i = STAR_MAX - 1; i >= 0; --i) {
+    double t = b[i];
+#pragma unroll
+    for (int j = i + 1; j < STAR_MAX; ++j) t -= A[i][j] * x[j];
+    x[i] = t * reciprocal(A[i][i]);
+  }
 }
 
-// A with SHAKE: half drift, then the position constraints along the bond
-// vectors before the drift; the velocity takes the constraint displacement.
+// R: the velocity constraints r_k . (v_k - v_0) = 0 of a star are linear in
+// the multipliers, so they are solved at once (the limit of RATTLE's sweeps,
+// cpu/integrate/constraints.js:173-222): v_k += lam_k im_k r_k and
+// v_0 -= im_0 sum_j lam_j r_j with
+// (im_0 r_k . r_j + [k = j] im_k |r_k|^2) lam_j = -r_k . (v_k - v_0).
+__device__ void starRattle(const Dev &d, const IntegratorParams &ip, Atom *w, int m, const double *invT2) {
+  (void)d;
+  (void)ip;
+  (void)invT2;
+  double r[STAR_MAX][3], A[STAR_MAX][STAR_MAX], b[STAR_MAX], lam[STAR_MAX];
+#pragma unroll
+  for (int k = 0; k < STAR_MAX; ++k)
+    for (int c = 0; c < 3; ++c) r[k][c] = w[k + 1].x[c] - w[0].x[c];
+  const double i0 = w[0].im;
+#pragma unroll
+  for (int k = 0; k < STAR_MAX; ++k) {
+    const bool live = k < m;
+    b[k] = 0.0;
+    if (live)
+      for (int c = 0; c < 3; ++c) b[k] -= r[k][c] * (w[k + 1].v[c] - w[0].v[c]);
+#pragma unroll
+    for (int j = 0; j < STAR_MAX; ++j) {
+      double a = 0.0;
+      if (live && j < m) {
+        const double dot = r[k][0] * r[j][0] + r[k][1] * r[j][1] + r[k][2] * r[j][2];
+        a = i0 * dot + (k == j ? w[k + 1].im * dot : 0.0);
+      }
+      A[k][j] = live ? a : (k == j ? 1.0 : 0.0);
+    }
+  }
+  solveStar(A, b, lam);
+#pragma unroll
+  for (int k = 0; k < STAR_MAX; ++k) {
+    if (k >= m) continue;
+    for (int c = 0; c < 3; ++c) {
+      w[k + 1].v[c] += lam[k] * w[k + 1].im * r[k][c];
+      w[0].v[c] -= lam[k] * i0 * r[k][c];
+    }
+  }
+}
+
+// A with SHAKE: half drift, then the position constraints with the
+// displacements along the bond vectors before the drift
+// (cpu/integrate/constraints.js:93-171): x_k += lam_k im_k r_k,
+// x_0 -= im_0 sum_j lam_j r_j, |x_k - x_0|^2 = d_k^2. The sweeps' fixed point
+// is found by Newton's method on the multipliers (each iteration a 3 x 3
+// solve; quadratic convergence where the sweeps converge linearly), to the
+// same relative tolerance on |x_k - x_0|^2. The velocity takes the
+// constraint displacement.
 __device__ void starDrift(const Dev &d, const IntegratorParams &ip, Atom *w, int m, const double *t2,
                           const double *invT2, double h) {
-  double ref[STAR_ATOMS][3], corr[STAR_ATOMS][3];
+  static_assert(STAR_MAX == 4, "the multipliers' initialiser below has STAR_MAX entries");
+  double r[STAR_MAX][3], s[STAR_MAX][3];
+  // the bond vectors before the drift, and after it
+#pragma unroll
+  for (int k = 0; k < STAR_MAX; ++k)
+    for (int c = 0; c < 3; ++c) r[k][c] = w[k + 1].x[c] - w[0].x[c];
 #pragma unroll
   for (int k = 0; k < STAR_ATOMS; ++k)
-    for (int c = 0; c < 3; ++c) {
-      ref[k][c] = w[k].x[c];
-      w[k].x[c] += h * w[k].v[c];
-      corr[k][c] = 0.0;
-    }
+    for (int c = 0; c < 3; ++c) w[k].x[c] += h * w[k].v[c];
+#pragma unroll
+  for (int k = 0; k < STAR_MAX; ++k)
+    for (int c = 0; c < 3; ++c) s[k][c] = w[k + 1].x[c] - w[0].x[c];
+  const double i0 = w[0].im;
+  double lam[STAR_MAX] = {0.0, 0.0, 0.0, 0.0};
   int iterations = 0;
   bool converged = false, stalled = false;
   double previous = 1e300;
   for (int it = 0; it < ip.maxIterations; ++it) {
+    double u[STAR_MAX][3], g[STAR_MAX], A[STAR_MAX][STAR_MAX];
     double worst = 0;
 #pragma unroll
-    for (int k = 0; k < STAR_MAX; ++k) {
-      if (k >= m) continue;
-      Atom &a = w[k + 1], &b = w[0];
-      const double dx = a.x[0] - b.x[0], dy = a.x[1] - b.x[1], dz = a.x[2] - b.x[2];
-      const double err = dx * dx + dy * dy + dz * dz - t2[k];
-      const double rel = fabs(err) * invT2[k];
-      worst = fmax(worst, rel);
-      if (rel <= ip.tolerance) continue;
-      const double r[3] = {ref[k + 1][0] - ref[0][0], ref[k + 1][1] - ref[0][1], ref[k + 1][2] - ref[0][2]};
-      const double den = 2.0 * (a.im + b.im) * (dx * r[0] + dy * r[1] + dz * r[2]);
-      if (fabs(den) < 1e-20) continue;
-      const double lam = -err * reciprocal(den);
+    for (int k = 0; k < STAR_MAX; ++k)
       for (int c = 0; c < 3; ++c) {
-        a.x[c] += lam * a.im * r[c];
-        b.x[c] -= lam * b.im * r[c];
-        corr[k + 1][c] += lam * a.im * r[c];
-        corr[0][c] -= lam * b.im * r[c];
+        double sum = 0.0;
+#pragma unroll
+        for (int j = 0; j < STAR_MAX; ++j) sum += (j < m ? lam[j] : 0.0) * r[j][c];
+        u[k][c] = s[k][c] + lam[k] * w[k + 1].im * r[k][c] + i0 * sum;
+      }
+#pragma unroll
+    for (int k = 0; k < STAR_MAX; ++k) {
+      const bool live = k < m;
+      const double err = live ? u[k][0] * u[k][0] + u[k][1] * u[k][1] + u[k][2] * u[k][2] - t2[k] : 0.0;
+      worst = fmax(worst, fabs(err) * (live ? invT2[k] : 0.0));
+      g[k] = -err;
+#pragma unroll
+      for (int j = 0; j < STAR_MAX; ++j) {
+        double a = 0.0;
+        if (live && j < m) {
+          const double dot = u[k][0] * r[j][0] + u[k][1] * r[j][1] + u[k][2] * r[j][2];
+          a = 2.0 * (i0 * dot + (k == j ? w[k + 1].im * dot : 0.0));
+        }
+        A[k][j] = live ? a : (k == j ? 1.0 : 0.0);
       }
     }
     iterations = it + 1;
@@ -359,11 +407,22 @@ __device__ void starDrift(const Dev &d, const IntegratorParams &ip, Atom *w, int
       break;
     }
     previous = worst;
+    double step[STAR_MAX];
+    solveStar(A, g, step);
+    for (int k = 0; k < STAR_MAX; ++k) lam[k] += step[k];
   }
   const double ih = 1.0 / h;
 #pragma unroll
-  for (int k = 0; k < STAR_ATOMS; ++k)
-    for (int c = 0; c < 3; ++c) w[k].v[c] += corr[k][c] * ih;
+  for (int k = 0; k < STAR_MAX; ++k) {
+    if (k >= m) continue;
+    for (int c = 0; c < 3; ++c) {
+      const double dk = lam[k] * w[k + 1].im * r[k][c], d0 = lam[k] * i0 * r[k][c];
+      w[k + 1].x[c] += dk;
+      w[0].x[c] -= d0;
+      w[k + 1].v[c] += dk * ih;
+      w[0].v[c] -= d0 * ih;
+    }
+  }
   atomicMax(&d.sc->shakeMaxIterations, (unsigned int)iterations);
   if (!converged && !stalled) atomicMax(&d.sc->shakeFailed, 1u);
 }
