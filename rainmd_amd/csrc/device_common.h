@@ -192,19 +192,20 @@ __device__ __forceinline__ BoxF uniformPairBox(const BoxF &g, bool rows64) {
 
 // erfc(x) for x >= 0 as exp(-x^2) * t * P(t), t = 1/(1 + x/2). P is a
 // degree-8 relative-error minimax fit of erfcx(x)/t over x in [0, 4] made for
-// this file (max fit error 2e-10; the f32 evaluation is ~1e-7 rms). x^2 is
-// split into hi + lo with an FMA so the exponential's argument is exact.
+// this file by erfc_fit.py (max relative fit error 1.2e-8; the f32
+// evaluation's mean error -3e-9). x^2 is split into hi + lo with an FMA so
+// the exponential's argument is exact.
 // Also returns exp(-x^2), which the force needs.
 __device__ __forceinline__ float erfcWithGaussian(float x, float &gaussian) {
   const float t = __builtin_amdgcn_rcpf(fmaf(0.5f, x, 1.0f));  // 1 ulp
-  float p = -4.261296242e-02f;
-  p = fmaf(p, t, 1.989050657e-01f);
-  p = fmaf(p, t, -2.959875166e-01f);
-  p = fmaf(p, t, 3.936168179e-02f);
-  p = fmaf(p, t, 1.554754525e-01f);
-  p = fmaf(p, t, 1.158383936e-01f);
-  p = fmaf(p, t, 2.683742046e-01f);
-  p = fmaf(p, t, 2.782751918e-01f);
+  float p = -4.261279106e-02f;
+  p = fmaf(p, t, 1.989041418e-01f);
+  p = fmaf(p, t, -2.959853113e-01f);
+  p = fmaf(p, t, 3.935875744e-02f);
+  p = fmaf(p, t, 1.554778367e-01f);
+  p = fmaf(p, t, 1.158371866e-01f);
+  p = fmaf(p, t, 2.683745921e-01f);
+  p = fmaf(p, t, 2.782751322e-01f);
   p = fmaf(p, t, 2.823704481e-01f);
   const float hi = x * x;
   const float lo = fmaf(x, x, -hi);
@@ -214,14 +215,14 @@ __device__ __forceinline__ float erfcWithGaussian(float x, float &gaussian) {
 }
 
 // erf(x) for x >= 0: an odd polynomial x*Q(x^2) below 0.8 (a degree-5
-// relative-error fit in x^2 made for this file, max fit error 3e-9), and
-// 1 - erfc above.
+// relative-error fit in x^2 made for this file by erfc_fit.py, max fit
+// error 3e-9), and 1 - erfc above.
 __device__ __forceinline__ float erfWithGaussian(float x, float &gaussian) {
   if (x < 0.8f) {
     const float x2 = x * x;
-    float s = -6.535415887e-04f;
-    s = fmaf(s, x2, 5.085162353e-03f);
-    s = fmaf(s, x2, -2.682082728e-02f);
+    float s = -6.535268622e-04f;
+    s = fmaf(s, x2, 5.085136276e-03f);
+    s = fmaf(s, x2, -2.682081051e-02f);
     s = fmaf(s, x2, 1.128311306e-01f);
     s = fmaf(s, x2, -3.761260211e-01f);
     s = fmaf(s, x2, 1.128379107e+00f);
