@@ -94,12 +94,13 @@ def counted_traffic(workload, lanes):
     summary (FETCH_SIZE and WRITE_SIZE scaled by a plain copy's), when it
     covers this workload and lane count; else None."""
     path = os.path.join(ROOT, "profiles", f"nonbonded_{workload.replace('-', '')}_traffic.md")
-    if not os.path.exists(path) or lanes != 64:
+    if not os.path.exists(path):
         return None, None
-    for line in open(path):
-        if line.startswith("traffic_bytes_per_launch:"):
-            return float(line.split(":")[1]), os.path.relpath(path, ROOT)
-    return None, None
+    fields = dict(line.split(":", 1) for line in open(path) if line.startswith(("traffic_bytes_per_launch:",
+                                                                               "lanes_per_atom:")))
+    if int(fields.get("lanes_per_atom", -1)) != lanes or "traffic_bytes_per_launch" not in fields:
+        return None, None
+    return float(fields["traffic_bytes_per_launch"]), os.path.relpath(path, ROOT)
 
 
 def cpu_baseline(system, pme, state, target_seconds, temperature=300.0, seed=202):
