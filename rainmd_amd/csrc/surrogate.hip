@@ -1,8 +1,9 @@
 // The learned part of the barostat: a 16 -> 64 -> 64 -> 1 tanh MLP that
 // predicts the part of a volume move's energy change that the first-order
 // (scaling-derivative) estimate misses. Inference and training run on
-// MI355X matrix cores: every layer product is v_mfma_f32_16x16x16_bf16 over
-// BF16 operands staged in LDS, accumulating in FP32; biases, activations,
+// MI355X matrix cores: every layer product is v_mfma_f32_16x16x32_bf16 (the
+// 16-deep input layer v_mfma_f32_16x16x16_bf16) over BF16 operands staged in
+// LDS, accumulating in FP32; biases, activations,
 // the loss and Adam run in FP32 on FP32 master weights.
 //
 // No counterpart in the reference (its barostat evaluates every move
@@ -30,17 +31,27 @@ __device__ __forceinline__ unsigned short toBf16(float f) {
   return (unsigned short)(u >> 16);
 }
 
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+
 // One 16x16 output tile of A[M][K] * B[K][N] with A row-major (lda) and B
 // given transposed, Bt[N][K] (ldb), both BF16 in LDS; K a multiple of 16.
-// Operand layout of v_mfma_f32_16x16x16_bf16 (wave64): lane l holds
-// A[m0 + l%16][k + 4*(l/16) .. +3] and B[k + 4*(l/16) .. +3][n0 + l%16]; the
-// result D[m0 + 4*(l/16) + i][n0 + l%16] is element i of the accumulator.
+// K in steps of 32 takes gfx950's v_mfma_f32_16x16x32_bf16 (lane l holds
+// A[m0 + l%16][k + 8*(l/16) .. +7] and B[k + 8*(l/16) .. +7][n0 + l%16]); a
+// remaining 16 takes v_mfma_f32_16x16x16_bf16 (4 elements per lane at
+// k + 4*(l/16)). Both leave D[m0 + 4*(l/16) + i][n0 + l%16] in element i of
+// the accumulator.
 __device__ __forceinline__ f32x4 mfmaTile(const unsigned short *A, int lda, const unsigned short *Bt, int ldb, int K,
                                           int m0, int n0) {
   const int lane = threadIdx.x & 63;
   const int r = lane & 15, q = lane >> 4;
   f32x4 acc = {0.f, 0.f, 0.f, 0.f};
-  for (int k = 0; k < K; k += 16) {
+  int k = 0;
+  for (; k + 32 <= K; k += 32) {
+    const bf16x8 a = *reinterpret_cast<const bf16x8 *>(A + (m0 + r) * lda + k + 8 * q);
+    const bf16x8 b = *reinterpret_cast<const bf16x8 *>(Bt + (n0 + r) * ldb + k + 8 * q);
+    acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b, acc, 0, 0, 0);
+  }
+  for (; k < K; k += 16) {
     const bf16x4 a = *reinterpret_cast<const bf16x4 *>(A + (m0 + r) * lda + k + 4 * q);
     const bf16x4 b = *reinterpret_cast<const bf16x4 *>(Bt + (n0 + r) * ldb + k + 4 * q);
     acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b, acc, 0, 0, 0);
@@ -48,24 +59,26 @@ __device__ __forceinline__ f32x4 mfmaTile(const unsigned short *A, int lda, cons
   return acc;
 }
 
+// (Every BF16 operand array starts on a 16-byte boundary: the matrix
+// products read it 8 or 16 bytes per lane.)
 struct Lds {
   // weights (BF16 operand copies) and FP32 biases / last layer
-  unsigned short w1t[64 * 16];    // [out][in]
-  unsigned short w2t[64 * 64];    // [out][in]
-  unsigned short w2n[64 * 64];    // [in][out]
+  alignas(16) unsigned short w1t[64 * 16];    // [out][in]
+  alignas(16) unsigned short w2t[64 * 64];    // [out][in]
+  alignas(16) unsigned short w2n[64 * 64];    // [in][out]
   float b1[64], b2[64], w3[64], b3;
   // activations for one tile
-  unsigned short x[TILE * 16];    // [sample][in]
-  unsigned short xt[16 * TILE];   // [in][sample]
-  unsigned short a1[TILE * 64];   // [sample][unit]
-  unsigned short a1t[64 * TILE];  // [unit][sample]
+  alignas(16) unsigned short x[TILE * 16];    // [sample][in]
+  alignas(16) unsigned short xt[16 * TILE];   // [in][sample]
+  alignas(16) unsigned short a1[TILE * 64];   // [sample][unit]
+  alignas(16) unsigned short a1t[64 * TILE];  // [unit][sample]
   float a1f[TILE * 64];
   float a2f[TILE * 64];
   float y[TILE];
   float dy[TILE];
-  unsigned short dz2[TILE * 64];  // [sample][unit]
-  unsigned short dz2t[64 * TILE]; // [unit][sample]
-  unsigned short dz1t[64 * TILE]; // [unit][sample]
+  alignas(16) unsigned short dz2[TILE * 64];  // [sample][unit]
+  alignas(16) unsigned short dz2t[64 * TILE]; // [unit][sample]
+  alignas(16) unsigned short dz1t[64 * TILE]; // [unit][sample]
   float grad[SURROGATE_PARAMS];
   float lossPart[THREADS / 64];
 };
@@ -221,7 +234,7 @@ __device__ float backward(Lds &s, const float *target, int first, int count, int
 
 // y = f(x) for n samples, one workgroup per tile of 64.
 __global__ void __launch_bounds__(THREADS) surrogateForwardKernel(const float *params, const float *x, int n, float *y) {
-  extern __shared__ unsigned char smem[];
+  extern __shared__ __attribute__((aligned(16))) unsigned char smem[];
   Lds &s = *reinterpret_cast<Lds *>(smem);
   const int first = blockIdx.x * TILE;
   const int count = min(TILE, n - first);
@@ -235,7 +248,7 @@ __global__ void __launch_bounds__(THREADS) surrogateForwardKernel(const float *p
 // Loss and gradient over all n samples (one workgroup walks the tiles).
 __global__ void __launch_bounds__(THREADS) surrogateGradientKernel(const float *params, const float *x, const float *t,
                                                                    int n, float *loss, float *grad) {
-  extern __shared__ unsigned char smem[];
+  extern __shared__ __attribute__((aligned(16))) unsigned char smem[];
   Lds &s = *reinterpret_cast<Lds *>(smem);
   loadWeights(s, params);
   for (int k = threadIdx.x; k < SURROGATE_PARAMS; k += blockDim.x) s.grad[k] = 0.f;
@@ -268,7 +281,7 @@ __device__ __forceinline__ unsigned int hashIndex(unsigned int v) {
 __global__ void __launch_bounds__(THREADS) surrogateTrainKernel(float *params, float *m, float *v, const float *x,
                                                                 const float *t, int n, int steps, float lr,
                                                                 unsigned int seed, int stepOffset, float *loss) {
-  extern __shared__ unsigned char smem[];
+  extern __shared__ __attribute__((aligned(16))) unsigned char smem[];
   Lds &s = *reinterpret_cast<Lds *>(smem);
   __shared__ int pick[TILE];
   __shared__ float batchX[TILE * 16];
