@@ -387,9 +387,6 @@ int rmd_debug_fft_forward(rmd_ctx *ctx, const float *real, float *out);
  * device arm tools/gpu/check-integrator.mjs:44-66 checks against the host. */
 int rmd_debug_noise(rmd_ctx *ctx, int64_t step, float *uniforms, float *normals);
 
-/* Tuning and plumbing: the nonbonded lanes per atom (a power of two <= 64),
- * hipGraph capture of step blocks on/off, and the context's HIP stream (for
- * timing on that stream). */
 /* The autotuner's shapes compiled into this library for a single system
  * (kind 0) or a replica batch (kind 1) of `atoms` atoms: lanes per atom, the
  * cell plan's widest reach, the search mode, the tuned skin (nm; 0 means the
@@ -403,6 +400,9 @@ typedef struct rmd_tuned_shape {
 } rmd_tuned_shape;
 int rmd_tuned_shape_get(int32_t kind, int32_t atoms, rmd_tuned_shape *out);
 
+/* Tuning and plumbing: the nonbonded lanes per atom (a power of two <= 64),
+ * hipGraph capture of step blocks on/off, and the context's HIP stream (for
+ * timing on that stream). */
 int rmd_set_lanes(rmd_ctx *ctx, int32_t lanes);
 int rmd_set_graphs(rmd_ctx *ctx, int32_t enabled);
 /* Overlapped force branches (1, the default) or each branch launched on its

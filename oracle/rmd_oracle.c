@@ -6,13 +6,16 @@
  * cpu_baseline leg load it, and only to check or to time the reference
  * algorithm; the product never does.
  *
- * What pins each function (tests/test_oracle.py):
+ * What pins each function (tests/test_oracle_pins.py, test_oracle_openmm.py,
+ * test_oracle_pins_host.py; no function here is unpinned):
  *   oracle_erfc / oracle_erf      SciPy special.erfc/erf values, incl. the
  *                                 reference's erfc(2.628261) known answer.
+ *   oracle_alpha_from_tolerance   alpha(1 nm, 5e-4) = 2.628261.
  *   oracle_potential (bonded)     OpenMM force-group energies in the fixtures
- *                                 (1e-10); central differences of itself.
- *   oracle_potential (LJ, Coulomb) OpenMM ljOnly / coulombOnly (1e-9 / 1e-7)
- *                                 on ala-vac (NoCutoff) and ala-wat (PME);
+ *                                 (1e-10); central differences of itself;
+ *                                 bonded forces sum to zero.
+ *   oracle_potential (LJ, Coulomb, OpenMM ljOnly / coulombOnly (1e-9 / 1e-7)
+ *     self and net-charge terms)  on ala-vac (NoCutoff) and ala-wat (PME);
  *                                 forces vs OpenMM (1e-6 of rms).
  *   reciprocal (naive DFT PME)    the plain Ewald k-sum (oracle_ewald_reciprocal)
  *                                 at a fine grid, on a rectangular and a sheared
@@ -20,13 +23,31 @@
  *   oracle_bspline / moduli       partition of unity, zero derivative sum,
  *                                 closed-form moduli of [1,11,11,1]/24.
  *   oracle_dispersion             the reference's -41.537158 kJ/mol and 1/V.
- *   oracle_uniform / gaussian     integer-exact hash vectors; Box-Muller moments.
+ *   oracle_mix / uniform /        the reference module's own output vectors
+ *     gaussian                    (tests/golden/random), bit for bit; mix is
+ *                                 lowbias32 (its published inverse undoes it).
  *   oracle_langevin_*             one step against the closed-form B/A/O
- *                                 update with supplied noise; constraint lengths.
+ *                                 update with supplied noise; the thermostat's
+ *                                 decay closed form; constraint lengths.
+ *   oracle_shake / oracle_rattle  constraint lengths and velocity projections
+ *                                 to the tolerance; SETTLE as their limit.
  *   oracle_settle                 converged SHAKE (tolerance 1e-13) on the
  *                                 same input; exact constraint lengths.
+ *   oracle_kinetic                sum of m v^2 / 2 for unit velocities by
+ *                                 hand; T = 2 KE / (dof k_B).
+ *   oracle_velocities             per-type Maxwell-Boltzmann moments
+ *                                 (mean 0, variance kT/m) over many atoms.
+ *   oracle_skin_for /             the reference's rules restated by hand at
+ *     oracle_plan_cells           the workload sizes (cell counts floor(w/rl),
+ *                                 stencil, capacity), the skin's cap and floor.
+ *   oracle_minimise               reaches scipy's L-BFGS minimum of the same
+ *                                 potential; monotone descent, displacement
+ *                                 cap, first trial moves the largest force by
+ *                                 max_step, constraints projected.
  *   oracle_cells_f32 / pairs_f32  brute force FP64 distances away from the
  *                                 boundary; partition and sortedness.
+ *   oracle_pairs_f32_binned       the all-pairs finder's sets (rectangular
+ *                                 and triclinic boxes).
  *   oracle_scale_molecules        molecule geometry preserved, centroids scaled.
  *   oracle_metropolis_weight      the reference's sign known-answers.
  */

@@ -448,7 +448,7 @@ __global__ void decideRebuildKernel(Scalars *sc, double dispHalf, int force) {
   if (rebuild) {
     sc->maxDisp2Bits = 0u;
     sc->slack = 0.f;
-    sc->required = 0u;
+    retireRequirement(sc);
     sc->rebuilds += 1u;
     sc->forceRebuild = 0u;
   }
@@ -544,9 +544,10 @@ void evaluate(rmd_ctx *c, int terms, bool forceRebuild, double terms_out[E_TERM_
     Scalars h;
     checkScalars(c, h);
     c->rebuildCount = h.rebuilds;
-    c->worstRequired = std::max(c->worstRequired, h.requiredMax);
-    if (h.requiredMax > (unsigned int)c->d.cap) {
-      const int grown = std::max(c->d.cap + 32, (int)std::ceil(h.requiredMax * 1.25));
+    const unsigned int need = requirementSince(h);
+    c->worstRequired = std::max(c->worstRequired, need);
+    if (need > (unsigned int)c->d.cap) {
+      const int grown = std::max(c->d.cap + 32, (int)std::ceil(need * 1.25));
       allocateNeighbours(c, grown);
       ++c->neighbourGrowths;
       forceRebuild = true;
@@ -749,12 +750,13 @@ void runBlock(rmd_ctx *c, int steps) {
                                        "; everything written during it should be discarded too.");
     if (h.shakeFailed) throw Error(RMD_E_CONSTRAINT, "constraints did not converge in the iteration limit");
     c->rebuildCount = h.rebuilds;
-    c->worstRequired = std::max(c->worstRequired, h.requiredMax);
-    if (h.requiredMax > (unsigned int)c->d.cap) {
+    const unsigned int need = requirementSince(h);
+    c->worstRequired = std::max(c->worstRequired, need);
+    if (need > (unsigned int)c->d.cap) {
       // The list overflowed somewhere in the block: grow it and run the
       // block again from its start, so no step ever used a truncated list.
       restore(c);
-      const int grown = std::max(c->d.cap + 32, (int)std::ceil(h.requiredMax * 1.25));
+      const int grown = std::max(c->d.cap + 32, (int)std::ceil(need * 1.25));
       allocateNeighbours(c, grown);
       ++c->neighbourGrowths;
       setForceRebuildKernel<<<1, 1, 0, c->stream>>>(c->d.sc);
@@ -2222,9 +2224,10 @@ bool growOverflowed(rmd_batch *b) {
   for (int r = 0; r < b->replicas; ++r) {
     rmd_ctx *c = b->reps[r];
     const Scalars &h = b->hostScalars[r];
-    c->worstRequired = std::max(c->worstRequired, h.requiredMax);
-    if (h.requiredMax > (unsigned int)c->d.cap) {
-      allocateNeighbours(c, std::max(c->d.cap + 32, (int)std::ceil(h.requiredMax * 1.25)));
+    const unsigned int need = requirementSince(h);
+    c->worstRequired = std::max(c->worstRequired, need);
+    if (need > (unsigned int)c->d.cap) {
+      allocateNeighbours(c, std::max(c->d.cap + 32, (int)std::ceil(need * 1.25)));
       ++c->neighbourGrowths;
       b->views[r].rebuild = 1;
       grew = true;

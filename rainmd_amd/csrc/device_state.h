@@ -46,7 +46,7 @@ struct Scalars {
   unsigned int maxDisp2Bits;  // max |x - x_rebuild|^2 since the last rebuild (float bits)
   unsigned int rebuild;       // this step rebuilds the cell and neighbour lists
   unsigned int forceRebuild;  // set by the host: rebuild at the next decision
-  unsigned int required;      // neighbour requirement seen at the last rebuild
+  unsigned int required;      // neighbour requirement of the current list (the longest list found)
   unsigned int ticket;        // last-block ticket of the integrator
   unsigned int rebuilds;      // rebuild count
   unsigned int shakeMaxIterations;
@@ -54,11 +54,25 @@ struct Scalars {
   unsigned int rattleFailed;
   unsigned int cellOverflow;  // a cell held more atoms than the sort can take
   unsigned int nonFinite;     // a non-finite coordinate was seen
-  unsigned int requiredMax;   // largest requirement since the host last cleared it
+  unsigned int requiredMax;   // largest requirement of the lists retired since the host last cleared
+                              // it (with `required`, every list's since then: requirementSince)
   float slack;                // nm of the skin spent by volume moves since the last rebuild
   unsigned int spanBits;      // largest atom-to-centroid distance of any molecule seen (float bits)
   long long step;             // the noise counter (completed steps)
 };
+
+// A rebuild retires the current list's requirement into requiredMax before
+// the search starts counting the next one.
+__device__ __forceinline__ void retireRequirement(Scalars *sc) {
+  sc->requiredMax = sc->required > sc->requiredMax ? sc->required : sc->requiredMax;
+  sc->required = 0u;
+}
+
+// The largest requirement of every list since the host last cleared
+// requiredMax: the retired ones and the current one.
+inline unsigned int requirementSince(const Scalars &h) {
+  return h.required > h.requiredMax ? h.required : h.requiredMax;
+}
 
 // The rebuild test: the largest displacement since the last rebuild against
 // half the skin less half of what volume moves have spent of it (NaN rebuilds).

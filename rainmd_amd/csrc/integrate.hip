@@ -751,7 +751,7 @@ __device__ __forceinline__ void integrateOpen(const Dev &d, const IntegratorPara
     if (rebuild) {
       d.sc->maxDisp2Bits = 0u;
       d.sc->slack = 0.f;
-      d.sc->required = 0u;
+      retireRequirement(d.sc);
       d.sc->rebuilds += 1u;
       d.sc->forceRebuild = 0u;
     }
@@ -913,7 +913,7 @@ __global__ void decideRebuildBatchKernel(const BatchView *__restrict__ v, int re
   if (rebuild) {
     sc->maxDisp2Bits = 0u;
     sc->slack = 0.f;
-    sc->required = 0u;
+    retireRequirement(sc);
     sc->rebuilds += 1u;
     sc->forceRebuild = 0u;
   }

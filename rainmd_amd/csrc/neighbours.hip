@@ -58,7 +58,7 @@ __global__ void setRebuildKernel(Scalars *sc) {
   sc->rebuild = 1u;
   sc->maxDisp2Bits = 0u;
   sc->slack = 0.f;
-  sc->required = 0u;
+  retireRequirement(sc);
   sc->rebuilds += 1u;
 }
 
@@ -159,14 +159,17 @@ __global__ void __launch_bounds__(256) cellsSortKernel(Dev d) {
 }
 
 // The whole cell list of a small system in one workgroup: assignment with
-// LDS-atomic counts, the scan, an LDS scatter, and a rank sort that puts each
-// cell's atoms in index order (rank = atoms of the same cell with a lower
-// index) - the same lists as the four-kernel path, with no global atomics
-// and three launches fewer on every step. Used when the atoms and the cells
-// fit the LDS arrays below.
+// LDS-atomic counts, the scan, then each cell's atoms in index order - by
+// wave ballots where there are at most 64 cells, else an LDS scatter and a
+// sort of each cell's run (insertion by one thread per cell, or where a cell
+// is crowded a rank sort: rank = atoms of the same cell with a lower index) -
+// the same lists as the four-kernel path, with no global atomics and three
+// launches fewer on every step. Used when the atoms and the cells fit the
+// LDS arrays below.
 constexpr int SMALL_THREADS = 1024;
 constexpr int SMALL_MAX_ATOMS = 16384;
 constexpr int SMALL_MAX_CELLS = 4096;
+constexpr int INSERTION_MAX = 32;  // cells up to this many atoms are sorted by one thread each
 
 __device__ __forceinline__ void cellsBuildSmall(const Dev &d) {
   if (!d.sc->rebuild) return;
@@ -202,9 +205,11 @@ __device__ __forceinline__ void cellsBuildSmall(const Dev &d) {
     __syncthreads();
   }
   int at = partial[t] - run;
+  bool crowdedRun = false;
   for (int c = c0; c < c1; ++c) {
     begin[c] = at;
     d.cellStart[c] = at;
+    crowdedRun |= count[c] > INSERTION_MAX;
     at += count[c];
     count[c] = begin[c];  // now the scatter cursor
   }
@@ -212,9 +217,62 @@ __device__ __forceinline__ void cellsBuildSmall(const Dev &d) {
     begin[total] = partial[t];
     d.cellStart[total] = partial[t];
   }
-  __syncthreads();
+  const bool crowded = __syncthreads_or(crowdedRun);
+  if (total <= 64) {
+    // Few (and so crowded) cells: place the atoms in index order directly,
+    // SMALL_THREADS at a time. Each wave splits its 64 atoms by cell with one
+    // ballot per cell (an atom's place among its wave's atoms of the same
+    // cell is the count of lower lanes in its cell's mask); one thread per
+    // cell then turns the waves' counts into each wave's start in the cell,
+    // in wave order, and advances the cell's cursor. No per-cell rank scan.
+    __shared__ int waveStart[SMALL_THREADS / 64][64];
+    const int lane = t & 63, wave = t >> 6;
+    for (int base = 0; base < n; base += SMALL_THREADS) {
+      const int i = base + t;
+      const int c = i < n ? d.cellOfAtom[i] : -1;
+      unsigned long long same = 0ull;
+      for (int k = 0; k < total; ++k) {
+        const unsigned long long m = __ballot(c == k);
+        if (c == k) same = m;
+        if (lane == k) waveStart[wave][k] = __popcll(m);
+      }
+      __syncthreads();
+      if (t < total) {
+        int at = count[t];
+        for (int w = 0; w < SMALL_THREADS / 64; ++w) {
+          const int m = waveStart[w][t];
+          waveStart[w][t] = at;
+          at += m;
+        }
+        count[t] = at;
+      }
+      __syncthreads();
+      if (i < n) writeSlot(d, waveStart[wave][c] + __popcll(same & ((1ull << lane) - 1ull)), i);
+      __syncthreads();
+    }
+    return;
+  }
   for (int i = t; i < n; i += SMALL_THREADS) unsorted[atomicAdd(&count[d.cellOfAtom[i]], 1)] = i;
   __syncthreads();
+  if (!crowded) {
+    // every cell holds at most INSERTION_MAX atoms: one thread sorts each
+    // cell's run in place, then the slots are written in order (coalesced)
+    for (int c = t; c < total; c += SMALL_THREADS) {
+      const int b = begin[c], e = begin[c + 1];
+      for (int k = b + 1; k < e; ++k) {
+        const int v = unsorted[k];
+        int m = k - 1;
+        while (m >= b && unsorted[m] > v) {
+          unsorted[m + 1] = unsorted[m];
+          --m;
+        }
+        unsorted[m + 1] = v;
+      }
+    }
+    __syncthreads();
+    for (int k = t; k < n; k += SMALL_THREADS) writeSlot(d, k, unsorted[k]);
+    return;
+  }
   for (int i = t; i < n; i += SMALL_THREADS) {
     const int c = d.cellOfAtom[i];
     const int b = begin[c], e = begin[c + 1];
@@ -286,8 +344,7 @@ __device__ __forceinline__ bool acceptPair(GlobalPtr<int> excl, const BoxF &box,
 // Scalars::required (the reference's overflow contract,
 // webgpu/kernels/neighbours.js:128-146).
 template <bool RECT>
-__device__ __forceinline__ void neighbours(const Dev &d, int block) {
-  if (!d.sc->rebuild) return;
+__device__ __forceinline__ unsigned int neighbours(const Dev &d, int block) {
   __shared__ int rangeBegin[4][STENCIL_RANGES];
   __shared__ int rangeStart[4][STENCIL_RANGES + 1];
   const BoxF box = uniformPairBox(*d.box, false);
@@ -297,7 +354,7 @@ __device__ __forceinline__ void neighbours(const Dev &d, int block) {
   const GlobalPtr<int> excl = globalPtr(d.exclPartners);
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const int row = (block * blockDim.x + threadIdx.x) >> 6;  // a slot
-  if (row >= d.n) return;
+  if (row >= d.n) return 0u;
   const int atom = d.cellAtoms[row];
   const float4 pi = d.pos[atom];
   if (lane == 0) d.posRebuild[atom] = pi;
@@ -403,11 +460,8 @@ __device__ __forceinline__ void neighbours(const Dev &d, int block) {
       if (base + 64 * u < total) emit(accept[u], slot[u]);
   }
   __builtin_amdgcn_wave_barrier();
-  if (lane == 0) {
-    d.nbrCount[row] = found < cap ? found : cap;
-    atomicMax(&d.sc->required, (unsigned int)found);
-    atomicMax(&d.sc->requiredMax, (unsigned int)found);
-  }
+  if (lane == 0) d.nbrCount[row] = found < cap ? found : cap;
+  return (unsigned int)found;
 }
 
 // One wave per cell. Every atom of a cell has the same stencil, so the wave
@@ -418,7 +472,7 @@ __device__ __forceinline__ void neighbours(const Dev &d, int block) {
 // the lists are entry for entry those of one wave per atom, for a fraction
 // of the loads and of the per-atom set-up.
 template <bool RECT>
-__device__ __forceinline__ void neighboursByCell(const Dev &d, int block) {
+__device__ __forceinline__ unsigned int neighboursByCell(const Dev &d, int block) {
   __shared__ int rangeBegin[4][STENCIL_RANGES];
   __shared__ int rangeStart[4][STENCIL_RANGES + 1];
   const BoxF box = uniformPairBox(*d.box, false);
@@ -428,9 +482,9 @@ __device__ __forceinline__ void neighboursByCell(const Dev &d, int block) {
   const GlobalPtr<int> excl = globalPtr(d.exclPartners);
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const int cell = block * 4 + wave;
-  if (cell >= d.box->cellTotal) return;
+  if (cell >= d.box->cellTotal) return 0u;
   const int cellBegin = cellStart[cell], cellEnd = cellStart[cell + 1];
-  if (cellBegin == cellEnd) return;
+  if (cellBegin == cellEnd) return 0u;
   const int c3[3] = {cell / (cells[1] * cells[2]), (cell / cells[2]) % cells[1], cell % cells[2]};
   const int reach = d.box->reach;
   int bx, nx, by, ny, bz, nz;
@@ -602,22 +656,29 @@ __device__ __forceinline__ void neighboursByCell(const Dev &d, int block) {
       required = max(required, (unsigned int)myFound);
     }
   }
-  // one update of the requirement per wave
   for (int offset = 32; offset > 0; offset >>= 1) required = max(required, (unsigned int)__shfl_xor((int)required, offset, 64));
-  if (lane == 0) {
-    atomicMax(&d.sc->required, required);
-    atomicMax(&d.sc->requiredMax, required);
-  }
+  return required;
 }
 
-// A block is four atoms or four cells, as the box's plan says.
+// A block is four atoms or four cells, as the box's plan says. The
+// requirement (the longest list found, stored or not) reaches
+// Scalars::required once per workgroup, and not at all where the value the
+// workgroup read at its start already covers it: every wave updating the one
+// word serialised its atomics (about 60 us of a 2854-atom rebuild's 150).
+// A workgroup's waves all come here (no early exit), so the barriers hold.
 template <bool RECT>
 __device__ __forceinline__ void searchBlock(const Dev &d, int block) {
-  if (!d.sc->rebuild) return;
-  if (d.box->cellSearch)
-    neighboursByCell<RECT>(d, block);
-  else
-    neighbours<RECT>(d, block);
+  if (!d.sc->rebuild) return;  // the same for every wave of the workgroup
+  const unsigned int seen = __hip_atomic_load(&d.sc->required, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  const unsigned int need = d.box->cellSearch ? neighboursByCell<RECT>(d, block) : neighbours<RECT>(d, block);
+  __shared__ unsigned int part[4];
+  if ((threadIdx.x & 63) == 0) part[threadIdx.x >> 6] = need;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    const unsigned int m = max(max(part[0], part[1]), max(part[2], part[3]));
+    if (m > seen) atomicMax(&d.sc->required, m);
+  }
+  __syncthreads();  // part is reused by a batched workgroup's next block
 }
 
 template <bool RECT>

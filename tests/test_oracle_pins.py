@@ -216,6 +216,29 @@ def test_f32_cells_and_pairs_agree_with_fp64_away_from_boundaries():
         assert sure <= got <= maybe
 
 
+def test_binned_pair_finder_equals_the_all_pairs_one():
+    """oracle_pairs_f32_binned (coarse-cell candidates, used for boxes too
+    large for the all-pairs loop) returns the all-pairs finder's sets, in a
+    rectangular and a triclinic box, at list cutoffs that give 2 to 5 coarse
+    cells per axis."""
+    system, coords = fixture_system("ala-wat", dict(constraints="HBonds", rigid_water=True, cutoff=1.0))
+    pos32 = coords.positions.astype(np.float32)
+    es, ep = system.exclusion_start, system.exclusion_partners
+    boxes = [np.asarray(system.box, dtype=np.float64)]
+    sheared = np.array(boxes[0]).reshape(3, 3).copy()
+    sheared[1, 0] = 0.3 * sheared[0, 0]
+    sheared[2, 0], sheared[2, 1] = -0.2 * sheared[0, 0], 0.25 * sheared[1, 1]
+    boxes.append(sheared.reshape(9))
+    for box in boxes:
+        for rl in (0.7, 1.2):
+            counts, lists = O.pairs_f32(pos32, box, rl, es, ep)
+            bcounts, blists = O.pairs_f32_binned(pos32, box, rl, es, ep)
+            assert np.array_equal(counts, bcounts), rl
+            assert counts.sum() > 0
+            for i in range(system.atom_count):
+                assert np.array_equal(np.sort(lists[i, :counts[i]]), blists[i, :bcounts[i]]), (rl, i)
+
+
 def test_centroid_scaling_keeps_molecules_rigid():
     system, coords = fixture_system("ala-wat", dict(constraints="HBonds", rigid_water=True, cutoff=1.0))
     o = O.Oracle(system)
