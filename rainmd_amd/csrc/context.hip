@@ -158,6 +158,7 @@ struct rmd_ctx {
   unsigned int *dMaxForce = nullptr;
   int nUnits = 0, waterUnits = 0, clusterUnits = 0, freeUnits = 0;
   int clusterLdsSlots = 0;  // largest per-block cluster slot count held in LDS
+  int starMax = 0;          // the most constraints of any star unit
   // graphs per block length
   std::map<int, hipGraphExec_t> graphs;
   bool useGraphs = true;
@@ -344,6 +345,7 @@ void buildUnits(rmd_ctx *c, std::vector<int4> &units, std::vector<double4> &geom
       conStart.push_back((int)cons.size());
       atomStart.push_back((int)atoms.size());
       units.push_back(make_int4(UNIT_STAR, cl, (int)ck.size(), 0));
+      c->starMax = std::max(c->starMax, (int)ck.size());
       geom.push_back(make_double4(0, 0, 0, 0));
       ++c->clusterUnits;
       continue;
@@ -436,6 +438,7 @@ void setupIntegrator(rmd_ctx *c, int suppliedNoise) {
   c->ip.maxIterations = c->maxIterations;
   c->ip.suppliedNoise = suppliedNoise;
   c->ip.clusterLdsBytes = (int)(c->clusterLdsSlots * CLUSTER_SLOT * sizeof(double));
+  c->ip.starMax = c->starMax;
   c->ip.dispHalf = std::max(0.0, 0.5 * c->plan.skin - 1e-4);
 }
 
@@ -2067,6 +2070,7 @@ struct rmd_batch {
   std::vector<double> hostTerms;
   size_t ldsForward = 0, ldsXlines = 0, ldsGather = 0;
   int ldsOpen = 0;
+  int starMax = 0;  // the widest star unit of any member
   double alpha = 0;
   int lanes = 16;  // direct-space lanes per atom of every member
   int maxTypes = 0;  // the largest LJ type count of the members
@@ -2271,11 +2275,11 @@ void enqueueBatchBlock(rmd_batch *b, int steps) {
   for (int k = 0; k < steps; ++k) {
     const bool last = k == steps - 1;
     launchIntegrateOpenBatch(b->dViews, b->tables[BK_WATER], b->tables[BK_OPEN], k > 0 ? 1 : 0, last ? 1 : 0,
-                             b->ldsOpen, s);
+                             b->ldsOpen, b->starMax, s);
     enqueueBatchForces(b, last ? 1 : 0);
   }
   launchReduceEnergiesBatch(b->dViews, b->replicas, 1, s);
-  launchIntegrateCloseBatch(b->dViews, b->tables[BK_UNITS], b->ldsOpen, s);
+  launchIntegrateCloseBatch(b->dViews, b->tables[BK_UNITS], b->ldsOpen, b->starMax, s);
 }
 
 void launchBatchBlock(rmd_batch *b, int steps) {
@@ -2678,6 +2682,7 @@ void createBatch(rmd_batch *b, const rmd_system *const *systems, int count, cons
     b->ldsXlines = std::max(b->ldsXlines, pmeXLinesLds(c->p));
     b->ldsGather = std::max(b->ldsGather, pmePlaneGatherLds(c->p));
     b->ldsOpen = std::max(b->ldsOpen, c->ip.clusterLdsBytes);
+    b->starMax = std::max(b->starMax, c->ip.starMax);
   }
   pmeBatchLdsLimits(b->ldsForward, b->ldsXlines, b->ldsGather);
   for (int k = 0; k < BK_COUNT; ++k) b->tables[k] = buildTasks(b, k);

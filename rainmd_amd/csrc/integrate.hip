@@ -26,13 +26,28 @@ namespace rmd {
 struct Atom {
   double x[3], v[3], f[3];
   double im;  // inverse mass (0 for massless)
+  // What storeAtom needs besides the new state, loaded with the state so
+  // that no load waits at the end of the unit's long dependency chain: the
+  // charge, the atom's slot in the cell order and its rebuild position.
+  float q;
+  int slot;
+  float rx, ry, rz;
 };
+
+__device__ __forceinline__ void loadStoreTail(const Dev &d, int a, Atom &s) {
+  const float4 r = d.posRebuild[a];
+  s.slot = d.slotOf[a];
+  s.rx = r.x;
+  s.ry = r.y;
+  s.rz = r.z;
+}
 
 __device__ __forceinline__ void loadAtom(const Dev &d, int a, Atom &s) {
   const float4 p = d.pos[a], v = d.vel[a];
   s.x[0] = p.x;
   s.x[1] = p.y;
   s.x[2] = p.z;
+  s.q = p.w;
   s.v[0] = v.x;
   s.v[1] = v.y;
   s.v[2] = v.z;
@@ -40,23 +55,22 @@ __device__ __forceinline__ void loadAtom(const Dev &d, int a, Atom &s) {
   s.f[0] = (double)d.force[a] * FORCE_UNSCALE;
   s.f[1] = (double)d.force[d.nPad + a] * FORCE_UNSCALE;
   s.f[2] = (double)d.force[2 * d.nPad + a] * FORCE_UNSCALE;
+  loadStoreTail(d, a, s);
 }
 
 // Stores position and velocity, clears the force accumulators for the coming
 // evaluation and returns |x - x_rebuild|^2 in f32.
 __device__ __forceinline__ float storeAtom(const Dev &d, int a, const Atom &s, bool clearForce) {
-  const float4 old = d.pos[a];
-  const float4 p = make_float4((float)s.x[0], (float)s.x[1], (float)s.x[2], old.w);
+  const float4 p = make_float4((float)s.x[0], (float)s.x[1], (float)s.x[2], s.q);
   d.pos[a] = p;
-  d.slotPos[d.slotOf[a]] = p;
+  d.slotPos[s.slot] = p;
   d.vel[a] = make_float4((float)s.v[0], (float)s.v[1], (float)s.v[2], (float)s.im);
   if (clearForce) {
     d.force[a] = 0;
     d.force[d.nPad + a] = 0;
     d.force[2 * d.nPad + a] = 0;
   }
-  const float4 r = d.posRebuild[a];
-  const float dx = p.x - r.x, dy = p.y - r.y, dz = p.z - r.z;
+  const float dx = p.x - s.rx, dy = p.y - s.ry, dz = p.z - s.rz;
   return dx * dx + dy * dy + dz * dz;
 }
 
@@ -241,7 +255,11 @@ __device__ void driftWater(Atom *w, double h, const double4 geom) {
 // The same SHAKE/RATTLE sweeps as a general cluster, with the whole cluster
 // in registers: slot 0 is the centre, constraint k joins slot k + 1 to it.
 
-constexpr int STAR_ATOMS = STAR_MAX + 1;
+// Every star function takes SM, the widest star the kernel handles (3 or 4
+// constraints; the host picks the smallest that covers the system's stars).
+// Constraint slots past a star's own are the identity with a zero right-hand
+// side, so a star of m <= 3 gets the same multipliers, bit for bit, in
+// either width; the narrower one only does less arithmetic.
 
 // 1/x from the hardware FP64 reciprocal refined by one Newton step (about one
 // ulp): the sweeps below divide on their serial dependency chain, where a
@@ -255,10 +273,11 @@ __device__ __forceinline__ double reciprocal(double x) {
 // unused ones are then given zero inverse mass, velocity and force: no branch
 // around the loads, so the compiler keeps the whole cluster in registers.
 // t2 and invT2: each constraint's squared length and its reciprocal.
+template <int SM>
 __device__ void starLoad(const Dev &d, int cluster, int m, Atom *w, int *atom, double *t2, double *invT2) {
   const int ab = d.clusterAtomStart[cluster], cb = d.clusterConStart[cluster];
 #pragma unroll
-  for (int k = 0; k < STAR_ATOMS; ++k) {
+  for (int k = 0; k < SM + 1; ++k) {
     atom[k] = d.clusterAtoms[ab + (k <= m ? k : m)];
     loadAtom(d, atom[k], w[k]);
     const double use = k <= m ? 1.0 : 0.0;
@@ -269,35 +288,36 @@ __device__ void starLoad(const Dev &d, int cluster, int m, Atom *w, int *atom, d
     }
   }
 #pragma unroll
-  for (int k = 0; k < STAR_MAX; ++k) {
+  for (int k = 0; k < SM; ++k) {
     const double target = d.clusterConD[cb + (k < m ? k : m - 1)];
     t2[k] = target * target;
     invT2[k] = reciprocal(t2[k]);
   }
 }
 
-// x = A^-1 b for a star's STAR_MAX x STAR_MAX system by Gaussian elimination
+// x = A^-1 b for a star's SM x SM system by Gaussian elimination
 // without pivoting: A is the constraints' coupling matrix (or, for SHAKE, its
 // Newton Jacobian, close to it), im_0 times a Gram matrix of a few bonds at
 // tetrahedral angles plus a positive diagonal, so its pivots stay large.
 // Rows and columns past the cluster's constraints are the identity.
-__device__ __forceinline__ void solveStar(double A[STAR_MAX][STAR_MAX], double b[STAR_MAX], double x[STAR_MAX]) {
+template <int SM>
+__device__ __forceinline__ void solveStar(double A[SM][SM], double b[SM], double x[SM]) {
 #pragma unroll
-  for (int p = 0; p < STAR_MAX; ++p) {
+  for (int p = 0; p < SM; ++p) {
     const double inv = reciprocal(A[p][p]);
 #pragma unroll
-    for (int i = p + 1; i < STAR_MAX; ++i) {
+    for (int i = p + 1; i < SM; ++i) {
       const double f = A[i][p] * inv;
 #pragma unroll
-      for (int j = p + 1; j < STAR_MAX; ++j) A[i][j] -= f * A[p][j];
+      for (int j = p + 1; j < SM; ++j) A[i][j] -= f * A[p][j];
       b[i] -= f * b[p];
     }
   }
 #pragma unroll
-  for (int i = STAR_MAX - 1; i >= 0; --i) {
+  for (int i = SM - 1; i >= 0; --i) {
     double t = b[i];
 #pragma unroll
-    for (int j = i + 1; j < STAR_MAX; ++j) t -= A[i][j] * x[j];
+    for (int j = i + 1; j < SM; ++j) t -= A[i][j] * x[j];
     x[i] = t * reciprocal(A[i][i]);
   }
 }
@@ -307,23 +327,21 @@ __device__ __forceinline__ void solveStar(double A[STAR_MAX][STAR_MAX], double b
 // cpu/integrate/constraints.js:173-222): v_k += lam_k im_k r_k and
 // v_0 -= im_0 sum_j lam_j r_j with
 // (im_0 r_k . r_j + [k = j] im_k |r_k|^2) lam_j = -r_k . (v_k - v_0).
-__device__ void starRattle(const Dev &d, const IntegratorParams &ip, Atom *w, int m, const double *invT2) {
-  (void)d;
-  (void)ip;
-  (void)invT2;
-  double r[STAR_MAX][3], A[STAR_MAX][STAR_MAX], b[STAR_MAX], lam[STAR_MAX];
+template <int SM>
+__device__ void starRattle(Atom *w, int m) {
+  double r[SM][3], A[SM][SM], b[SM], lam[SM];
 #pragma unroll
-  for (int k = 0; k < STAR_MAX; ++k)
+  for (int k = 0; k < SM; ++k)
     for (int c = 0; c < 3; ++c) r[k][c] = w[k + 1].x[c] - w[0].x[c];
   const double i0 = w[0].im;
 #pragma unroll
-  for (int k = 0; k < STAR_MAX; ++k) {
+  for (int k = 0; k < SM; ++k) {
     const bool live = k < m;
     b[k] = 0.0;
     if (live)
       for (int c = 0; c < 3; ++c) b[k] -= r[k][c] * (w[k + 1].v[c] - w[0].v[c]);
 #pragma unroll
-    for (int j = 0; j < STAR_MAX; ++j) {
+    for (int j = 0; j < SM; ++j) {
       double a = 0.0;
       if (live && j < m) {
         const double dot = r[k][0] * r[j][0] + r[k][1] * r[j][1] + r[k][2] * r[j][2];
@@ -332,9 +350,9 @@ __device__ void starRattle(const Dev &d, const IntegratorParams &ip, Atom *w, in
       A[k][j] = live ? a : (k == j ? 1.0 : 0.0);
     }
   }
-  solveStar(A, b, lam);
+  solveStar<SM>(A, b, lam);
 #pragma unroll
-  for (int k = 0; k < STAR_MAX; ++k) {
+  for (int k = 0; k < SM; ++k) {
     if (k >= m) continue;
     for (int c = 0; c < 3; ++c) {
       w[k + 1].v[c] += lam[k] * w[k + 1].im * r[k][c];
@@ -351,44 +369,46 @@ __device__ void starRattle(const Dev &d, const IntegratorParams &ip, Atom *w, in
 // solve; quadratic convergence where the sweeps converge linearly), to the
 // same relative tolerance on |x_k - x_0|^2. The velocity takes the
 // constraint displacement.
+template <int SM>
 __device__ void starDrift(const Dev &d, const IntegratorParams &ip, Atom *w, int m, const double *t2,
                           const double *invT2, double h) {
-  static_assert(STAR_MAX == 4, "the multipliers' initialiser below has STAR_MAX entries");
-  double r[STAR_MAX][3], s[STAR_MAX][3];
+  double r[SM][3], s[SM][3];
   // the bond vectors before the drift, and after it
 #pragma unroll
-  for (int k = 0; k < STAR_MAX; ++k)
+  for (int k = 0; k < SM; ++k)
     for (int c = 0; c < 3; ++c) r[k][c] = w[k + 1].x[c] - w[0].x[c];
 #pragma unroll
-  for (int k = 0; k < STAR_ATOMS; ++k)
+  for (int k = 0; k < SM + 1; ++k)
     for (int c = 0; c < 3; ++c) w[k].x[c] += h * w[k].v[c];
 #pragma unroll
-  for (int k = 0; k < STAR_MAX; ++k)
+  for (int k = 0; k < SM; ++k)
     for (int c = 0; c < 3; ++c) s[k][c] = w[k + 1].x[c] - w[0].x[c];
   const double i0 = w[0].im;
-  double lam[STAR_MAX] = {0.0, 0.0, 0.0, 0.0};
+  double lam[SM];
+#pragma unroll
+  for (int k = 0; k < SM; ++k) lam[k] = 0.0;
   int iterations = 0;
   bool converged = false, stalled = false;
   double previous = 1e300;
   for (int it = 0; it < ip.maxIterations; ++it) {
-    double u[STAR_MAX][3], g[STAR_MAX], A[STAR_MAX][STAR_MAX];
+    double u[SM][3], g[SM], A[SM][SM];
     double worst = 0;
 #pragma unroll
-    for (int k = 0; k < STAR_MAX; ++k)
+    for (int k = 0; k < SM; ++k)
       for (int c = 0; c < 3; ++c) {
         double sum = 0.0;
 #pragma unroll
-        for (int j = 0; j < STAR_MAX; ++j) sum += (j < m ? lam[j] : 0.0) * r[j][c];
+        for (int j = 0; j < SM; ++j) sum += (j < m ? lam[j] : 0.0) * r[j][c];
         u[k][c] = s[k][c] + lam[k] * w[k + 1].im * r[k][c] + i0 * sum;
       }
 #pragma unroll
-    for (int k = 0; k < STAR_MAX; ++k) {
+    for (int k = 0; k < SM; ++k) {
       const bool live = k < m;
       const double err = live ? u[k][0] * u[k][0] + u[k][1] * u[k][1] + u[k][2] * u[k][2] - t2[k] : 0.0;
       worst = fmax(worst, fabs(err) * (live ? invT2[k] : 0.0));
       g[k] = -err;
 #pragma unroll
-      for (int j = 0; j < STAR_MAX; ++j) {
+      for (int j = 0; j < SM; ++j) {
         double a = 0.0;
         if (live && j < m) {
           const double dot = u[k][0] * r[j][0] + u[k][1] * r[j][1] + u[k][2] * r[j][2];
@@ -407,13 +427,13 @@ __device__ void starDrift(const Dev &d, const IntegratorParams &ip, Atom *w, int
       break;
     }
     previous = worst;
-    double step[STAR_MAX];
-    solveStar(A, g, step);
-    for (int k = 0; k < STAR_MAX; ++k) lam[k] += step[k];
+    double step[SM];
+    solveStar<SM>(A, g, step);
+    for (int k = 0; k < SM; ++k) lam[k] += step[k];
   }
   const double ih = 1.0 / h;
 #pragma unroll
-  for (int k = 0; k < STAR_MAX; ++k) {
+  for (int k = 0; k < SM; ++k) {
     if (k >= m) continue;
     for (int c = 0; c < 3; ++c) {
       const double dk = lam[k] * w[k + 1].im * r[k][c], d0 = lam[k] * i0 * r[k][c];
@@ -652,6 +672,7 @@ __global__ void __launch_bounds__(INTEGRATE_BLOCK)
 
 // One step's opening half for the units of workgroup `block`, one of the
 // `blocks` general workgroups (whose last one decides the rebuild).
+template <int SM>
 __device__ __forceinline__ void integrateOpen(const Dev &d, const IntegratorParams &ip, int closing, int early,
                                               int block, int blocks, double *clusterLds) {
   const int u = block * blockDim.x + threadIdx.x;
@@ -675,26 +696,26 @@ __device__ __forceinline__ void integrateOpen(const Dev &d, const IntegratorPara
       openWater(d, ip, u, desc, closing, step, h, ke, disp);
     } else if (desc.x == UNIT_STAR) {
       const int m = desc.z;
-      Atom w[STAR_ATOMS];
-      int atom[STAR_ATOMS];
-      double t2[STAR_MAX], invT2[STAR_MAX];
-      starLoad(d, desc.y, m, w, atom, t2, invT2);
+      Atom w[SM + 1];
+      int atom[SM + 1];
+      double t2[SM], invT2[SM];
+      starLoad<SM>(d, desc.y, m, w, atom, t2, invT2);
       if (closing) {
 #pragma unroll
-        for (int k = 0; k < STAR_ATOMS; ++k) kick(w[k], h);
-        starRattle(d, ip, w, m, invT2);
+        for (int k = 0; k < SM + 1; ++k) kick(w[k], h);
+        starRattle<SM>(w, m);
       }
 #pragma unroll
-      for (int k = 0; k < STAR_ATOMS; ++k) kick(w[k], h);
-      starRattle(d, ip, w, m, invT2);
-      starDrift(d, ip, w, m, t2, invT2, h);
+      for (int k = 0; k < SM + 1; ++k) kick(w[k], h);
+      starRattle<SM>(w, m);
+      starDrift<SM>(d, ip, w, m, t2, invT2, h);
 #pragma unroll
-      for (int k = 0; k < STAR_ATOMS; ++k)
+      for (int k = 0; k < SM + 1; ++k)
         if (k <= m) thermostat(d, ip, atom[k], w[k], step);
-      starRattle(d, ip, w, m, invT2);
-      starDrift(d, ip, w, m, t2, invT2, h);
+      starRattle<SM>(w, m);
+      starDrift<SM>(d, ip, w, m, t2, invT2, h);
 #pragma unroll
-      for (int k = 0; k < STAR_ATOMS; ++k)
+      for (int k = 0; k < SM + 1; ++k)
         if (k <= m) {
           ke += kinetic(w[k]);
           disp = fmaxf(disp, storeAtom(d, atom[k], w[k], true));
@@ -729,8 +750,11 @@ __device__ __forceinline__ void integrateOpen(const Dev &d, const IntegratorPara
           t.x[c] = x[c];
           t.v[c] = x[6 + c];
         }
+        const int a = d.clusterAtoms[s];
+        t.q = d.pos[a].w;
+        loadStoreTail(d, a, t);
         ke += kinetic(t);
-        disp = fmaxf(disp, storeAtom(d, d.clusterAtoms[s], t, true));
+        disp = fmaxf(disp, storeAtom(d, a, t, true));
       }
     }
   }
@@ -762,21 +786,24 @@ __device__ __forceinline__ void integrateOpen(const Dev &d, const IntegratorPara
 
 // The units of blocks firstBlock.. (0: every unit; the water blocks' count:
 // the general units after the water kernel).
+template <int SM>
 __global__ void __launch_bounds__(INTEGRATE_BLOCK)
     integrateOpenKernel(Dev d, IntegratorParams ip, int closing, int early, int firstBlock) {
   extern __shared__ double clusterLds[];
-  integrateOpen(d, ip, closing, early, firstBlock + blockIdx.x, gridDim.x, clusterLds);
+  integrateOpen<SM>(d, ip, closing, early, firstBlock + blockIdx.x, gridDim.x, clusterLds);
 }
 
+template <int SM>
 __global__ void __launch_bounds__(INTEGRATE_BLOCK)
     integrateOpenBatchKernel(const BatchView *__restrict__ v, const int2 *__restrict__ task, int closing, int early) {
   extern __shared__ double clusterLds[];
   const int2 t = task[blockIdx.x];
   if (t.x < 0) return;
-  integrateOpen(v[t.x].d, v[t.x].ip, closing, early, v[t.x].d.waterBlocks + t.y, v[t.x].openBlocks, clusterLds);
+  integrateOpen<SM>(v[t.x].d, v[t.x].ip, closing, early, v[t.x].d.waterBlocks + t.y, v[t.x].openBlocks, clusterLds);
 }
 
 // The closing kick + RATTLE of a block's last step.
+template <int SM>
 __device__ __forceinline__ void integrateClose(const Dev &d, const IntegratorParams &ip, int withKinetic, int block,
                                                double *clusterLds) {
   const int u = block * blockDim.x + threadIdx.x;
@@ -802,15 +829,15 @@ __device__ __forceinline__ void integrateClose(const Dev &d, const IntegratorPar
       }
     } else if (desc.x == UNIT_STAR) {
       const int m = desc.z;
-      Atom w[STAR_ATOMS];
-      int atom[STAR_ATOMS];
-      double t2[STAR_MAX], invT2[STAR_MAX];
-      starLoad(d, desc.y, m, w, atom, t2, invT2);
+      Atom w[SM + 1];
+      int atom[SM + 1];
+      double t2[SM], invT2[SM];
+      starLoad<SM>(d, desc.y, m, w, atom, t2, invT2);
 #pragma unroll
-      for (int k = 0; k < STAR_ATOMS; ++k) kick(w[k], h);
-      starRattle(d, ip, w, m, invT2);
+      for (int k = 0; k < SM + 1; ++k) kick(w[k], h);
+      starRattle<SM>(w, m);
 #pragma unroll
-      for (int k = 0; k < STAR_ATOMS; ++k)
+      for (int k = 0; k < SM + 1; ++k)
         if (k <= m) {
           ke += kinetic(w[k]);
           d.vel[atom[k]] = make_float4((float)w[k].v[0], (float)w[k].v[1], (float)w[k].v[2], (float)w[k].im);
@@ -835,17 +862,19 @@ __device__ __forceinline__ void integrateClose(const Dev &d, const IntegratorPar
   if (withKinetic) blockSumToSlot(ke, &d.slots[E_KINETIC * MAX_SLOTS + block]);
 }
 
+template <int SM>
 __global__ void __launch_bounds__(INTEGRATE_BLOCK) integrateCloseKernel(Dev d, IntegratorParams ip, int withKinetic) {
   extern __shared__ double clusterLds[];
-  integrateClose(d, ip, withKinetic, blockIdx.x, clusterLds);
+  integrateClose<SM>(d, ip, withKinetic, blockIdx.x, clusterLds);
 }
 
+template <int SM>
 __global__ void __launch_bounds__(INTEGRATE_BLOCK)
     integrateCloseBatchKernel(const BatchView *__restrict__ v, const int2 *__restrict__ task) {
   extern __shared__ double clusterLds[];
   const int2 t = task[blockIdx.x];
   if (t.x < 0) return;
-  integrateClose(v[t.x].d, v[t.x].ip, 0, t.y, clusterLds);
+  integrateClose<SM>(v[t.x].d, v[t.x].ip, 0, t.y, clusterLds);
 }
 
 __global__ void __launch_bounds__(256) kineticKernel(Dev d, int term) {
@@ -1193,20 +1222,29 @@ __global__ void constrainPositionsKernel(Dev d, const float4 *reference, Integra
 
 // The water blocks, then the general blocks; returns the in-step kinetic
 // slots written.
-int launchIntegrateOpen(const Dev &d, const IntegratorParams &ip, int closing, int early, hipStream_t s) {
+template <int SM>
+int launchIntegrateOpenShaped(const Dev &d, const IntegratorParams &ip, int closing, int early, hipStream_t s) {
   if (!splitWater(d)) {
-    integrateOpenKernel<<<unitBlockCount(d), INTEGRATE_BLOCK, ip.clusterLdsBytes, s>>>(d, ip, closing, early, 0);
+    integrateOpenKernel<SM><<<unitBlockCount(d), INTEGRATE_BLOCK, ip.clusterLdsBytes, s>>>(d, ip, closing, early, 0);
     return unitBlockCount(d);
   }
   integrateWaterKernel<<<d.waterBlocks, INTEGRATE_BLOCK, 0, s>>>(d, ip, closing, early);
-  integrateOpenKernel<<<generalBlockCount(d), INTEGRATE_BLOCK, ip.clusterLdsBytes, s>>>(d, ip, closing, early,
-                                                                                       d.waterBlocks);
+  integrateOpenKernel<SM><<<generalBlockCount(d), INTEGRATE_BLOCK, ip.clusterLdsBytes, s>>>(d, ip, closing, early,
+                                                                                           d.waterBlocks);
   return openSlotCount(d);
+}
+
+int launchIntegrateOpen(const Dev &d, const IntegratorParams &ip, int closing, int early, hipStream_t s) {
+  return ip.starMax <= 3 ? launchIntegrateOpenShaped<3>(d, ip, closing, early, s)
+                         : launchIntegrateOpenShaped<4>(d, ip, closing, early, s);
 }
 
 int launchIntegrateClose(const Dev &d, const IntegratorParams &ip, int withKinetic, hipStream_t s) {
   const int blocks = unitBlockCount(d);
-  integrateCloseKernel<<<blocks, INTEGRATE_BLOCK, ip.clusterLdsBytes, s>>>(d, ip, withKinetic);
+  if (ip.starMax <= 3)
+    integrateCloseKernel<3><<<blocks, INTEGRATE_BLOCK, ip.clusterLdsBytes, s>>>(d, ip, withKinetic);
+  else
+    integrateCloseKernel<4><<<blocks, INTEGRATE_BLOCK, ip.clusterLdsBytes, s>>>(d, ip, withKinetic);
   return blocks;
 }
 
@@ -1259,13 +1297,19 @@ void launchConstrainPositions(const Dev &d, const float4 *reference, const Integ
 // ---- batched launchers ---------------------------------------------------------
 
 void launchIntegrateOpenBatch(const BatchView *v, const TaskTable &water, const TaskTable &general, int closing,
-                              int early, int lds, hipStream_t s) {
+                              int early, int lds, int starMax, hipStream_t s) {
   if (water.blocks > 0) integrateWaterBatchKernel<<<water.blocks, INTEGRATE_BLOCK, 0, s>>>(v, water.task, closing, early);
-  integrateOpenBatchKernel<<<general.blocks, INTEGRATE_BLOCK, lds, s>>>(v, general.task, closing, early);
+  if (starMax <= 3)
+    integrateOpenBatchKernel<3><<<general.blocks, INTEGRATE_BLOCK, lds, s>>>(v, general.task, closing, early);
+  else
+    integrateOpenBatchKernel<4><<<general.blocks, INTEGRATE_BLOCK, lds, s>>>(v, general.task, closing, early);
 }
 
-void launchIntegrateCloseBatch(const BatchView *v, const TaskTable &t, int lds, hipStream_t s) {
-  integrateCloseBatchKernel<<<t.blocks, INTEGRATE_BLOCK, lds, s>>>(v, t.task);
+void launchIntegrateCloseBatch(const BatchView *v, const TaskTable &t, int lds, int starMax, hipStream_t s) {
+  if (starMax <= 3)
+    integrateCloseBatchKernel<3><<<t.blocks, INTEGRATE_BLOCK, lds, s>>>(v, t.task);
+  else
+    integrateCloseBatchKernel<4><<<t.blocks, INTEGRATE_BLOCK, lds, s>>>(v, t.task);
 }
 
 void launchReduceEnergiesBatch(const BatchView *v, int replicas, int withEarly, hipStream_t s) {

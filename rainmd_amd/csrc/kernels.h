@@ -101,6 +101,7 @@ struct IntegratorParams {
   int maxIterations;
   int suppliedNoise;
   int clusterLdsBytes;  // dynamic LDS per integrator block
+  int starMax;          // the most constraints of any star unit (the star solver's width: 3 or 4)
 };
 // closing: apply the previous step's closing kick + RATTLE first.
 // early: record the in-step kinetic energy (E_KINETIC_EARLY).
@@ -197,8 +198,8 @@ enum BatchKernel {
 
 // Batched launchers. `lds` is the largest dynamic LDS any replica needs.
 void launchIntegrateOpenBatch(const BatchView *v, const TaskTable &water, const TaskTable &general, int closing,
-                              int early, int lds, hipStream_t s);
-void launchIntegrateCloseBatch(const BatchView *v, const TaskTable &t, int lds, hipStream_t s);
+                              int early, int lds, int starMax, hipStream_t s);
+void launchIntegrateCloseBatch(const BatchView *v, const TaskTable &t, int lds, int starMax, hipStream_t s);
 // withEarly: a block's last step (the in-step kinetic energy is summed too).
 void launchReduceEnergiesBatch(const BatchView *v, int replicas, int withEarly, hipStream_t s);
 void launchZeroForcesBatch(const BatchView *v, const TaskTable &t, hipStream_t s);
