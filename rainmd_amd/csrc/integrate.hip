@@ -118,19 +118,23 @@ __device__ __forceinline__ void gaussianTriple(unsigned int seed, unsigned int a
 // O: v <- a v + sqrt(1 - a^2) sqrt(kT/m) xi (integrate.js:406-437). The
 // normals are drawn in FP32 - the noise is a random draw, and FP64 is kept
 // for the velocity update itself.
-__device__ __forceinline__ void thermostat(const Dev &d, const IntegratorParams &ip, int a, Atom &s,
-                                           unsigned int step) {
-  if (s.im <= 0) return;
-  double g[3];
+// The draw depends only on (seed, atom, step), so the units take it right
+// after issuing their state loads: its arithmetic runs while those loads are
+// in flight instead of in the middle of the unit's dependency chain.
+__device__ __forceinline__ void drawNoise(const Dev &d, const IntegratorParams &ip, int a, unsigned int step,
+                                          float g[3]) {
   if (ip.suppliedNoise) {
     for (int c = 0; c < 3; ++c) g[c] = d.noise[3 * a + c];
   } else {
-    float u[4], gf[3];
-    gaussianTriple(ip.seed, a, step, u, gf);
-    for (int c = 0; c < 3; ++c) g[c] = gf[c];
+    float u[4];
+    gaussianTriple(ip.seed, a, step, u, g);
   }
+}
+
+__device__ __forceinline__ void thermostat(const IntegratorParams &ip, Atom &s, const float g[3]) {
+  if (s.im <= 0) return;
   const double amp = ip.noiseScale * sqrt(ip.kT * s.im);
-  for (int c = 0; c < 3; ++c) s.v[c] = ip.decay * s.v[c] + amp * g[c];
+  for (int c = 0; c < 3; ++c) s.v[c] = ip.decay * s.v[c] + amp * (double)g[c];
 }
 
 // ---- rigid water ----------------------------------------------------------
@@ -621,6 +625,8 @@ __device__ __forceinline__ void openWater(const Dev &d, const IntegratorParams &
   const int atoms[3] = {desc.y, desc.z, desc.w};
   for (int a = 0; a < 3; ++a) loadAtom(d, atoms[a], w[a]);
   const double4 geom = d.waterGeom[u];
+  float g[3][3];
+  for (int a = 0; a < 3; ++a) drawNoise(d, ip, atoms[a], step, g[a]);
   if (closing) {
     for (int a = 0; a < 3; ++a) kick(w[a], h);
     rattleWater(w);
@@ -628,7 +634,7 @@ __device__ __forceinline__ void openWater(const Dev &d, const IntegratorParams &
   for (int a = 0; a < 3; ++a) kick(w[a], h);
   rattleWater(w);
   driftWater(w, h, geom);
-  for (int a = 0; a < 3; ++a) thermostat(d, ip, atoms[a], w[a], step);
+  for (int a = 0; a < 3; ++a) thermostat(ip, w[a], g[a]);
   rattleWater(w);
   driftWater(w, h, geom);
   for (int a = 0; a < 3; ++a) {
@@ -658,12 +664,13 @@ __device__ __forceinline__ void integrateWater(const Dev &d, const IntegratorPar
   if (early) blockSumToSlot(ke, &d.slots[E_KINETIC_EARLY * MAX_SLOTS + block]);
 }
 
-__global__ void __launch_bounds__(INTEGRATE_BLOCK)
+// (Two waves per SIMD: the water path fits 256 registers.)
+__global__ void __launch_bounds__(INTEGRATE_BLOCK) __attribute__((amdgpu_waves_per_eu(2)))
     integrateWaterKernel(Dev d, IntegratorParams ip, int closing, int early) {
   integrateWater(d, ip, closing, early, blockIdx.x);
 }
 
-__global__ void __launch_bounds__(INTEGRATE_BLOCK)
+__global__ void __launch_bounds__(INTEGRATE_BLOCK) __attribute__((amdgpu_waves_per_eu(2)))
     integrateWaterBatchKernel(const BatchView *__restrict__ v, const int2 *__restrict__ task, int closing, int early) {
   const int2 t = task[blockIdx.x];
   if (t.x < 0) return;
@@ -685,10 +692,12 @@ __device__ __forceinline__ void integrateOpen(const Dev &d, const IntegratorPara
     if (desc.x == UNIT_FREE) {
       Atom s;
       loadAtom(d, desc.y, s);
+      float g[3];
+      drawNoise(d, ip, desc.y, step, g);
       if (closing) kick(s, h);
       kick(s, h);
       for (int c = 0; c < 3; ++c) s.x[c] += h * s.v[c];
-      thermostat(d, ip, desc.y, s, step);
+      thermostat(ip, s, g);
       for (int c = 0; c < 3; ++c) s.x[c] += h * s.v[c];
       ke = kinetic(s);
       disp = storeAtom(d, desc.y, s, true);
@@ -700,6 +709,9 @@ __device__ __forceinline__ void integrateOpen(const Dev &d, const IntegratorPara
       int atom[SM + 1];
       double t2[SM], invT2[SM];
       starLoad<SM>(d, desc.y, m, w, atom, t2, invT2);
+      float g[SM + 1][3];
+#pragma unroll
+      for (int k = 0; k < SM + 1; ++k) drawNoise(d, ip, atom[k], step, g[k]);
       if (closing) {
 #pragma unroll
         for (int k = 0; k < SM + 1; ++k) kick(w[k], h);
@@ -711,7 +723,7 @@ __device__ __forceinline__ void integrateOpen(const Dev &d, const IntegratorPara
       starDrift<SM>(d, ip, w, m, t2, invT2, h);
 #pragma unroll
       for (int k = 0; k < SM + 1; ++k)
-        if (k <= m) thermostat(d, ip, atom[k], w[k], step);
+        if (k <= m) thermostat(ip, w[k], g[k]);
       starRattle<SM>(w, m);
       starDrift<SM>(d, ip, w, m, t2, invT2, h);
 #pragma unroll
@@ -737,7 +749,9 @@ __device__ __forceinline__ void integrateOpen(const Dev &d, const IntegratorPara
         Atom t;
         t.im = x[12];
         for (int c = 0; c < 3; ++c) t.v[c] = x[6 + c];
-        thermostat(d, ip, d.clusterAtoms[s], t, step);
+        float g[3];
+        drawNoise(d, ip, d.clusterAtoms[s], step, g);
+        thermostat(ip, t, g);
         for (int c = 0; c < 3; ++c) x[6 + c] = t.v[c];
       }
       clusterRattle(d, ip, cl, S);
