@@ -11,7 +11,7 @@
 // The binning arithmetic is FP32 with a fixed operation order and no
 // contraction, so the FP32 CPU emulation in the oracle reproduces every cell
 // assignment and every pair decision bit for bit.
-#include "kernels.h"
+#include "search.h"
 
 // Multiply-adds fuse only within one expression, so each body rounds the same
 // way wherever it is inlined (the single-system and the batched kernels).
@@ -20,39 +20,6 @@
 namespace rmd {
 
 constexpr int SORT_CAPACITY = 4096;  // atoms per cell the LDS sort takes
-
-// cellOfPosition (webgpu/kernels/wgsl-common.js:112-123): fractional
-// coordinate from the reciprocal rows, wrapped with f - floor(f), scaled,
-// truncated and clamped.
-__device__ __forceinline__ int cellIndex(float x, float y, float z, const BoxF &b, int c[3]) {
-#pragma clang fp contract(off)
-  const int *cells = b.cells;
-  for (int d = 0; d < 3; ++d) {
-    float a0 = x * b.recip[d];
-    float a1 = y * b.recip[3 + d];
-    float a2 = z * b.recip[6 + d];
-    float f = (a0 + a1) + a2;
-    float w = f - floorf(f);
-    float scaled = w * (float)cells[d];
-    int ci = (int)scaled;
-    ci = ci > cells[d] - 1 ? cells[d] - 1 : ci;
-    ci = ci < 0 ? 0 : ci;
-    c[d] = ci;
-  }
-  return (c[0] * cells[1] + c[1]) * cells[2] + c[2];
-}
-
-// Slot k of the cell order gets atom `atom`: its rebuild-time position with
-// the atom index (for the search), its current position and charge, its type,
-// and the inverse map.
-__device__ __forceinline__ void writeSlot(const Dev &d, int k, int atom) {
-  const float4 p = d.pos[atom];
-  d.cellAtoms[k] = atom;
-  d.cellPos[k] = make_float4(p.x, p.y, p.z, __int_as_float(atom));
-  d.slotPos[k] = p;
-  d.slotType[k] = d.type[atom];
-  d.slotOf[atom] = k;
-}
 
 __global__ void setRebuildKernel(Scalars *sc) {
   sc->rebuild = 1u;
@@ -75,37 +42,35 @@ __global__ void cellsAssignKernel(Dev d) {
   if (!(isfinite(p.x) && isfinite(p.y) && isfinite(p.z))) atomicMax(&d.sc->nonFinite, 1u);
 }
 
-// Exclusive scan of the per-cell counts in one workgroup; leaves the counts
-// zeroed for the next rebuild and the scatter cursors at the cell starts.
+// Exclusive scan of the per-cell counts in one workgroup: each thread sums a
+// contiguous run of cells, the runs' sums are scanned across the workgroup
+// once, then each thread writes its run's starts. Leaves the counts zeroed
+// for the next rebuild and the scatter cursors at the cell starts.
 __global__ void __launch_bounds__(1024) cellsScanKernel(Dev d) {
   if (!d.sc->rebuild) return;
   __shared__ int partial[1024];
-  __shared__ int carry;
-  const int total = d.box->cellTotal;
-  if (threadIdx.x == 0) carry = 0;
+  const int total = d.box->cellTotal, t = threadIdx.x;
+  const int per = (total + 1023) / 1024;
+  const int c0 = min(total, t * per), c1 = min(total, c0 + per);
+  int run = 0;
+  for (int c = c0; c < c1; ++c) run += d.cellCount[c];
+  partial[t] = run;
   __syncthreads();
-  for (int base = 0; base < total; base += 1024) {
-    const int c = base + threadIdx.x;
-    const int v = c < total ? d.cellCount[c] : 0;
-    partial[threadIdx.x] = v;
+  for (int offset = 1; offset < 1024; offset <<= 1) {
+    const int add = t >= offset ? partial[t - offset] : 0;
     __syncthreads();
-    for (int offset = 1; offset < 1024; offset <<= 1) {
-      int add = threadIdx.x >= offset ? partial[threadIdx.x - offset] : 0;
-      __syncthreads();
-      partial[threadIdx.x] += add;
-      __syncthreads();
-    }
-    const int exclusive = carry + partial[threadIdx.x] - v;
-    if (c < total) {
-      d.cellStart[c] = exclusive;
-      d.cellCursor[c] = exclusive;
-      d.cellCount[c] = 0;
-    }
-    __syncthreads();
-    if (threadIdx.x == 1023) carry += partial[1023];
+    partial[t] += add;
     __syncthreads();
   }
-  if (threadIdx.x == 0) d.cellStart[total] = carry;
+  int at = partial[t] - run;
+  for (int c = c0; c < c1; ++c) {
+    const int count = d.cellCount[c];
+    d.cellStart[c] = at;
+    d.cellCursor[c] = at;
+    d.cellCount[c] = 0;
+    at += count;
+  }
+  if (t == 1023) d.cellStart[total] = partial[1023];
 }
 
 __global__ void cellsScatterKernel(Dev d) {
@@ -116,56 +81,81 @@ __global__ void cellsScatterKernel(Dev d) {
   d.cellAtoms[slot] = i;
 }
 
-// The scatter leaves each cell's atoms in arrival order; a bitonic sort in
-// LDS puts them in index order so the list, and every sum over it, is the
-// same on every run. The sorted cell also gets its positions copied in cell
-// order (atom index in .w) for the candidate scan.
-__global__ void __launch_bounds__(256) cellsSortKernel(Dev d) {
+// The scatter leaves each cell's atoms in arrival order; sorting each cell's
+// run puts them in index order so the list, and every sum over it, is the
+// same on every run. Sorted cells also get their slots written (positions in
+// cell order, with the atom index in .w, for the candidate scan). Cells of at
+// most SHORT_CELL atoms - every cell of a plan whose cells are a fraction of
+// the list cutoff - are sorted by one thread each, by insertion in that
+// thread's slice of LDS; longer ones by a bitonic sort over a workgroup
+// (cellsSortLongKernel).
+constexpr int SHORT_CELL = 32;
+constexpr int SHORT_SORT_THREADS = 128;
+
+__global__ void __launch_bounds__(SHORT_SORT_THREADS) cellsSortShortKernel(Dev d) {
+  if (!d.sc->rebuild) return;
+  __shared__ int keys[SHORT_SORT_THREADS][SHORT_CELL + 1];  // +1: the threads' slices start in different banks
+  const int c = blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= d.box->cellTotal) return;
+  const int begin = d.cellStart[c], len = d.cellStart[c + 1] - begin;
+  if (len > SHORT_CELL) return;
+  int *k = keys[threadIdx.x];
+  for (int a = 0; a < len; ++a) {
+    const int v = d.cellAtoms[begin + a];
+    int m = a - 1;
+    while (m >= 0 && k[m] > v) {
+      k[m + 1] = k[m];
+      --m;
+    }
+    k[m + 1] = v;
+  }
+  for (int a = 0; a < len; ++a) writeSlot(d, begin + a, k[a]);
+}
+
+__global__ void __launch_bounds__(256) cellsSortLongKernel(Dev d) {
   if (!d.sc->rebuild) return;
   __shared__ int keys[SORT_CAPACITY];
-  const int c = blockIdx.x;
-  if (c >= d.box->cellTotal) return;
-  const int begin = d.cellStart[c], end = d.cellStart[c + 1];
-  const int len = end - begin;
-  if (len <= 0) return;
-  if (len > SORT_CAPACITY) {
-    if (threadIdx.x == 0) atomicMax(&d.sc->cellOverflow, 1u);
-    return;
-  }
-  int size = 1;
-  while (size < len) size <<= 1;
-  for (int k = threadIdx.x; k < size; k += blockDim.x) keys[k] = k < len ? d.cellAtoms[begin + k] : 0x7fffffff;
-  __syncthreads();
-  for (int k = 2; k <= size; k <<= 1) {
-    for (int j = k >> 1; j > 0; j >>= 1) {
-      for (int t = threadIdx.x; t < size; t += blockDim.x) {
-        const int partner = t ^ j;
-        if (partner > t) {
-          const bool ascending = (t & k) == 0;
-          const int a = keys[t], b = keys[partner];
-          if ((a > b) == ascending) {
-            keys[t] = b;
-            keys[partner] = a;
+  for (int c = blockIdx.x; c < d.box->cellTotal; c += gridDim.x) {
+    const int begin = d.cellStart[c], end = d.cellStart[c + 1];
+    const int len = end - begin;
+    if (len <= SHORT_CELL) continue;
+    if (len > SORT_CAPACITY) {
+      if (threadIdx.x == 0) atomicMax(&d.sc->cellOverflow, 1u);
+      continue;
+    }
+    int size = 1;
+    while (size < len) size <<= 1;
+    for (int k = threadIdx.x; k < size; k += blockDim.x) keys[k] = k < len ? d.cellAtoms[begin + k] : 0x7fffffff;
+    __syncthreads();
+    for (int k = 2; k <= size; k <<= 1) {
+      for (int j = k >> 1; j > 0; j >>= 1) {
+        for (int t = threadIdx.x; t < size; t += blockDim.x) {
+          const int partner = t ^ j;
+          if (partner > t) {
+            const bool ascending = (t & k) == 0;
+            const int a = keys[t], b = keys[partner];
+            if ((a > b) == ascending) {
+              keys[t] = b;
+              keys[partner] = a;
+            }
           }
         }
+        __syncthreads();
       }
-      __syncthreads();
     }
-  }
-  for (int k = threadIdx.x; k < len; k += blockDim.x) {
-    const int atom = keys[k];
-    writeSlot(d, begin + k, atom);
+    for (int k = threadIdx.x; k < len; k += blockDim.x) writeSlot(d, begin + k, keys[k]);
+    __syncthreads();  // keys is reused for the workgroup's next cell
   }
 }
 
 // The whole cell list of a small system in one workgroup: assignment with
 // LDS-atomic counts, the scan, then each cell's atoms in index order - by
-// wave ballots where there are at most 64 cells, else an LDS scatter and a
-// sort of each cell's run (insertion by one thread per cell, or where a cell
-// is crowded a rank sort: rank = atoms of the same cell with a lower index) -
-// the same lists as the four-kernel path, with no global atomics and three
-// launches fewer on every step. Used when the atoms and the cells fit the
-// LDS arrays below.
+// wave ballots where there are at most FEW_MAX_CELLS cells (cellsBuildFew),
+// else an LDS scatter and a sort of each cell's run (insertion by one thread
+// per cell, or where a cell is crowded a rank sort: rank = atoms of the same
+// cell with a lower index) - the same lists as the four-kernel path, with no
+// global atomics and three launches fewer on every step. Used when the atoms
+// and the cells fit the LDS arrays below.
 constexpr int SMALL_THREADS = 1024;
 constexpr int SMALL_MAX_ATOMS = 16384;
 constexpr int SMALL_MAX_CELLS = 4096;
@@ -173,6 +163,10 @@ constexpr int INSERTION_MAX = 32;  // cells up to this many atoms are sorted by 
 
 __device__ __forceinline__ void cellsBuildSmall(const Dev &d) {
   if (!d.sc->rebuild) return;
+  if (d.box->cellTotal <= FEW_MAX_CELLS) {
+    cellsBuildFew<SMALL_THREADS>(d);
+    return;
+  }
   __shared__ int count[SMALL_MAX_CELLS];
   __shared__ int begin[SMALL_MAX_CELLS + 1];
   __shared__ int unsorted[SMALL_MAX_ATOMS];
@@ -218,40 +212,6 @@ __device__ __forceinline__ void cellsBuildSmall(const Dev &d) {
     d.cellStart[total] = partial[t];
   }
   const bool crowded = __syncthreads_or(crowdedRun);
-  if (total <= 64) {
-    // Few (and so crowded) cells: place the atoms in index order directly,
-    // SMALL_THREADS at a time. Each wave splits its 64 atoms by cell with one
-    // ballot per cell (an atom's place among its wave's atoms of the same
-    // cell is the count of lower lanes in its cell's mask); one thread per
-    // cell then turns the waves' counts into each wave's start in the cell,
-    // in wave order, and advances the cell's cursor. No per-cell rank scan.
-    __shared__ int waveStart[SMALL_THREADS / 64][64];
-    const int lane = t & 63, wave = t >> 6;
-    for (int base = 0; base < n; base += SMALL_THREADS) {
-      const int i = base + t;
-      const int c = i < n ? d.cellOfAtom[i] : -1;
-      unsigned long long same = 0ull;
-      for (int k = 0; k < total; ++k) {
-        const unsigned long long m = __ballot(c == k);
-        if (c == k) same = m;
-        if (lane == k) waveStart[wave][k] = __popcll(m);
-      }
-      __syncthreads();
-      if (t < total) {
-        int at = count[t];
-        for (int w = 0; w < SMALL_THREADS / 64; ++w) {
-          const int m = waveStart[w][t];
-          waveStart[w][t] = at;
-          at += m;
-        }
-        count[t] = at;
-      }
-      __syncthreads();
-      if (i < n) writeSlot(d, waveStart[wave][c] + __popcll(same & ((1ull << lane) - 1ull)), i);
-      __syncthreads();
-    }
-    return;
-  }
   for (int i = t; i < n; i += SMALL_THREADS) unsorted[atomicAdd(&count[d.cellOfAtom[i]], 1)] = i;
   __syncthreads();
   if (!crowded) {
@@ -290,395 +250,6 @@ __global__ void __launch_bounds__(SMALL_THREADS) cellsBuildSmallBatchKernel(cons
   const int2 t = task[blockIdx.x];
   if (t.x < 0) return;
   cellsBuildSmall(v[t.x].d);
-}
-
-// Per-axis stencil (shared/topology/cells.js:25-29, generalised to cells of
-// rl / reach): `reach` cells either side when the axis has at least
-// 2 * reach + 1 cells, else every cell of the axis once.
-constexpr int REACH_MAX = 3;
-constexpr int STENCIL_ROWS = (2 * REACH_MAX + 1) * (2 * REACH_MAX + 1);
-constexpr int STENCIL_RANGES = 2 * STENCIL_ROWS;  // a row's z run splits in two where it wraps
-
-__device__ __forceinline__ void axisStencil(int count, int reach, int &base, int &len) {
-  if (count >= 2 * reach + 1) {
-    base = -reach;
-    len = 2 * reach + 1;
-  } else {
-    base = 0;
-    len = count;
-  }
-}
-
-__device__ __forceinline__ int wrapCell(int v, int count) {
-  v %= count;
-  return v < 0 ? v + count : v;
-}
-
-// Whether candidate pj joins atom's list: not itself, within rl (the FP32
-// minimum image of the kernels), not excluded. Exclusions are checked only
-// for candidates inside the range of the atom's (sorted) exclusion partners.
-template <bool RECT>
-__device__ __forceinline__ bool acceptPair(GlobalPtr<int> excl, const BoxF &box, float4 pi, float4 pj, int atom,
-                                           int exBegin, int exEnd, int exMin, int exMax) {
-  // the distance test first and without a branch, so the candidate's loads
-  // are used whole where they are issued
-  const int j = __float_as_int(pj.w);
-  float dx = pi.x - pj.x, dy = pi.y - pj.y, dz = pi.z - pj.z;
-  float s0, s1, s2;
-  const float r2 = imageR2<RECT>(dx, dy, dz, box, s0, s1, s2);
-  bool accept = (j != atom) & (r2 <= box.rl2);
-  if (accept & (j >= exMin) & (j <= exMax))
-    for (int e = exBegin; e < exEnd; ++e)
-      if (excl[e] == j) accept = false;
-  return accept;
-}
-
-// One wave per atom. The stencil is a set of z runs, one per (x, y) stencil
-// row, each a contiguous range of the cell-ordered positions (two where the
-// run wraps). The wave writes the ranges and their prefix sums to LDS and
-// streams the concatenated candidates 256 at a time (four per lane, their
-// loads issued together), each lane keeping a cursor into the ranges; a
-// ballot compacts the accepted ones into the atom's slice in candidate
-// order, so the list is the same on every run. Every pair found is counted;
-// only `cap` are stored, and the true requirement reaches the host through
-// Scalars::required (the reference's overflow contract,
-// webgpu/kernels/neighbours.js:128-146).
-template <bool RECT>
-__device__ __forceinline__ unsigned int neighbours(const Dev &d, int block) {
-  __shared__ int rangeBegin[4][STENCIL_RANGES];
-  __shared__ int rangeStart[4][STENCIL_RANGES + 1];
-  const BoxF box = uniformPairBox(*d.box, false);
-  const int cells[3] = {d.box->cells[0], d.box->cells[1], d.box->cells[2]};
-  const GlobalPtr<float4> cellPos = globalPtr(d.cellPos);
-  const GlobalPtr<int> cellStart = globalPtr(d.cellStart);
-  const GlobalPtr<int> excl = globalPtr(d.exclPartners);
-  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-  const int row = (block * blockDim.x + threadIdx.x) >> 6;  // a slot
-  if (row >= d.n) return 0u;
-  const int atom = d.cellAtoms[row];
-  const float4 pi = d.pos[atom];
-  if (lane == 0) d.posRebuild[atom] = pi;
-  const int own = d.cellOfAtom[atom];
-  const int c3[3] = {own / (cells[1] * cells[2]), (own / cells[2]) % cells[1], own % cells[2]};
-  const int exBegin = d.exclStart[atom], exEnd = d.exclStart[atom + 1];
-  const int exMin = exBegin < exEnd ? excl[exBegin] : 1;
-  const int exMax = exBegin < exEnd ? excl[exEnd - 1] : 0;
-  const int reach = d.box->reach;
-  int bx, nx, by, ny, bz, nz;
-  axisStencil(cells[0], reach, bx, nx);
-  axisStencil(cells[1], reach, by, ny);
-  axisStencil(cells[2], reach, bz, nz);
-  // the z segments, the same for every row: the whole row, or [lo, hi)
-  // split where it wraps
-  int za0, zb0, za1 = 0, zb1 = 0, segs = 1;
-  if (nz >= cells[2]) {
-    za0 = 0;
-    zb0 = cells[2];
-  } else {
-    const int lo = c3[2] + bz, hi = lo + nz;
-    if (lo < 0) {
-      segs = 2;
-      za0 = 0, zb0 = hi;
-      za1 = lo + cells[2], zb1 = cells[2];
-    } else if (hi > cells[2]) {
-      segs = 2;
-      za0 = 0, zb0 = hi - cells[2];
-      za1 = lo, zb1 = cells[2];
-    } else {
-      za0 = lo, zb0 = hi;
-    }
-  }
-  const int ranges = nx * ny * segs;
-  int *rb = rangeBegin[wave], *rs = rangeStart[wave];
-  // ranges r = lane and lane + 64: their slots and lengths, then an exclusive
-  // prefix over all of them
-  int len[2] = {0, 0};
-  for (int h = 0; h < 2; ++h) {
-    const int r = lane + 64 * h;
-    if (r < ranges) {
-      const int row = r / segs, seg = r - row * segs;
-      const int cx = wrapCell(c3[0] + bx + row / ny, cells[0]);
-      const int cy = wrapCell(c3[1] + by + row % ny, cells[1]);
-      const int rowBase = (cx * cells[1] + cy) * cells[2];
-      const int begin = cellStart[rowBase + (seg ? za1 : za0)];
-      const int end = cellStart[rowBase + (seg ? zb1 : zb0)];
-      rb[r] = begin;
-      len[h] = end - begin;
-    }
-  }
-  int carry = 0;
-  for (int h = 0; h < 2; ++h) {
-    int inclusive = len[h];
-    for (int offset = 1; offset < 64; offset <<= 1) {
-      const int v = __shfl_up(inclusive, offset, 64);
-      if (lane >= offset) inclusive += v;
-    }
-    const int r = lane + 64 * h;
-    if (r < ranges) rs[r] = carry + inclusive - len[h];
-    carry += __shfl(inclusive, 63, 64);
-  }
-  if (lane == 0) rs[ranges] = carry;
-  __builtin_amdgcn_wave_barrier();
-  const int total = carry;
-  int found = 0;
-  const GlobalOut<int> out = globalOut(d.nbrList + (size_t)row * d.cap);
-  const int cap = d.cap;
-  auto emit = [&](bool accept, int j) {  // j: the candidate's slot
-    const unsigned long long mask = __ballot(accept);
-    if (accept) {
-      const int where = found + __popcll(mask & ((1ull << lane) - 1ull));
-      if (where < cap) out[where] = j;
-    }
-    found += __popcll(mask);
-  };
-  // Four candidates per lane per pass, base + lane + 64 u. A candidate past
-  // the total loads the row's own slot, which the pair test turns down (the
-  // atom itself), so every load is in bounds and issued with the others.
-  constexpr int CAND = 4;
-  int cur = 0;  // the range of this lane's latest candidate
-  for (int base = 0; base < total; base += 64 * CAND) {
-    int slot[CAND];
-#pragma unroll
-    for (int u = 0; u < CAND; ++u) {
-      const int g = base + lane + 64 * u;
-      slot[u] = row;
-      if (g < total) {
-        while (rs[cur + 1] <= g) ++cur;
-        slot[u] = rb[cur] + (g - rs[cur]);
-      }
-    }
-    float4 p[CAND];
-#pragma unroll
-    for (int u = 0; u < CAND; ++u) p[u] = cellPos[slot[u]];
-    // the tests of all four, then the stores (which share the loads'
-    // completion counter)
-    bool accept[CAND];
-#pragma unroll
-    for (int u = 0; u < CAND; ++u) accept[u] = acceptPair<RECT>(excl, box, pi, p[u], atom, exBegin, exEnd, exMin, exMax);
-#pragma unroll
-    for (int u = 0; u < CAND; ++u)
-      if (base + 64 * u < total) emit(accept[u], slot[u]);
-  }
-  __builtin_amdgcn_wave_barrier();
-  if (lane == 0) d.nbrCount[row] = found < cap ? found : cap;
-  return (unsigned int)found;
-}
-
-// One wave per cell. Every atom of a cell has the same stencil, so the wave
-// streams the stencil's candidates as above (the same ranges, in the same
-// order) and tests each candidate against every atom of the cell, up to 64
-// at a time, one per lane, broadcast from its lane in turn. Each atom's
-// accepted candidates are compacted into its own slice in candidate order:
-// the lists are entry for entry those of one wave per atom, for a fraction
-// of the loads and of the per-atom set-up.
-template <bool RECT>
-__device__ __forceinline__ unsigned int neighboursByCell(const Dev &d, int block) {
-  __shared__ int rangeBegin[4][STENCIL_RANGES];
-  __shared__ int rangeStart[4][STENCIL_RANGES + 1];
-  const BoxF box = uniformPairBox(*d.box, false);
-  const int cells[3] = {d.box->cells[0], d.box->cells[1], d.box->cells[2]};
-  const GlobalPtr<float4> cellPos = globalPtr(d.cellPos);
-  const GlobalPtr<int> cellStart = globalPtr(d.cellStart);
-  const GlobalPtr<int> excl = globalPtr(d.exclPartners);
-  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-  const int cell = block * 4 + wave;
-  if (cell >= d.box->cellTotal) return 0u;
-  const int cellBegin = cellStart[cell], cellEnd = cellStart[cell + 1];
-  if (cellBegin == cellEnd) return 0u;
-  const int c3[3] = {cell / (cells[1] * cells[2]), (cell / cells[2]) % cells[1], cell % cells[2]};
-  const int reach = d.box->reach;
-  int bx, nx, by, ny, bz, nz;
-  axisStencil(cells[0], reach, bx, nx);
-  axisStencil(cells[1], reach, by, ny);
-  axisStencil(cells[2], reach, bz, nz);
-  int za0, zb0, za1 = 0, zb1 = 0, segs = 1;
-  if (nz >= cells[2]) {
-    za0 = 0;
-    zb0 = cells[2];
-  } else {
-    const int lo = c3[2] + bz, hi = lo + nz;
-    if (lo < 0) {
-      segs = 2;
-      za0 = 0, zb0 = hi;
-      za1 = lo + cells[2], zb1 = cells[2];
-    } else if (hi > cells[2]) {
-      segs = 2;
-      za0 = 0, zb0 = hi - cells[2];
-      za1 = lo, zb1 = cells[2];
-    } else {
-      za0 = lo, zb0 = hi;
-    }
-  }
-  // Rectangular boxes: each stencil z run is trimmed at both ends to the
-  // cells within reach of the box around this cell's atoms (wrapped into the
-  // cell frame), by the periodic gap between the box and each cell's extent,
-  // with a margin far above the FP32 rounding of either. A trimmed cell holds
-  // no candidate the pair test could accept, so the lists are unchanged.
-  // per axis: the box's centre and half width (uniform), the cell width
-  float centre[3] = {0.f, 0.f, 0.f}, half[3] = {0.f, 0.f, 0.f};
-  const float limit = sqrtf(box.rl2) + 1e-4f, limit2 = limit * limit;
-  const float edge[3] = {box.a[0], box.b[1], box.c[2]};
-  if (RECT) {
-    for (int dd = 0; dd < 3; ++dd) {
-      float lo = __builtin_inff(), hi = -__builtin_inff();
-      for (int k = cellBegin + lane; k < cellEnd; k += 64) {
-        const float4 q = cellPos[k];
-        const float x = dd == 0 ? q.x : dd == 1 ? q.y : q.z;
-        const float w = x - edge[dd] * floorf(x * box.inv[dd]);
-        lo = fminf(lo, w);
-        hi = fmaxf(hi, w);
-      }
-      for (int offset = 32; offset > 0; offset >>= 1) {
-        lo = fminf(lo, __shfl_xor(lo, offset, 64));
-        hi = fmaxf(hi, __shfl_xor(hi, offset, 64));
-      }
-      centre[dd] = uniformFloat(0.5f * (lo + hi));
-      half[dd] = uniformFloat(0.5f * (hi - lo));
-    }
-  }
-  // the periodic gap along axis dd between the atoms' box and cell index c:
-  // the centres' distance folded into [0, L/2], less both half widths
-  auto gap = [&](int dd, int c) {
-    const float width = edge[dd] / (float)cells[dd];
-    float dist = fabsf((c + 0.5f) * width - centre[dd]);
-    dist = fminf(dist, edge[dd] - dist);
-    return fmaxf(0.f, dist - half[dd] - 0.5f * width);
-  };
-  const int ranges = nx * ny * segs;
-  int *rb = rangeBegin[wave], *rs = rangeStart[wave];
-  int len[2] = {0, 0};
-  for (int h = 0; h < 2; ++h) {
-    const int r = lane + 64 * h;
-    if (r < ranges) {
-      const int row = r / segs, seg = r - row * segs;
-      const int cx = wrapCell(c3[0] + bx + row / ny, cells[0]);
-      const int cy = wrapCell(c3[1] + by + row % ny, cells[1]);
-      const int rowBase = (cx * cells[1] + cy) * cells[2];
-      int zFirst = seg ? za1 : za0, zEnd = seg ? zb1 : zb0;
-      if (RECT) {
-        const float gx = gap(0, cx), gy = gap(1, cy), gxy = gx * gx + gy * gy;
-        if (gxy > limit2) {
-          zEnd = zFirst;
-        } else {
-          while (zFirst < zEnd) {
-            const float gz = gap(2, zFirst);
-            if (gxy + gz * gz <= limit2) break;
-            ++zFirst;
-          }
-          while (zEnd > zFirst) {
-            const float gz = gap(2, zEnd - 1);
-            if (gxy + gz * gz <= limit2) break;
-            --zEnd;
-          }
-        }
-      }
-      const int begin = cellStart[rowBase + zFirst];
-      const int end = cellStart[rowBase + zEnd];
-      rb[r] = begin;
-      len[h] = end - begin;
-    }
-  }
-  int carry = 0;
-  for (int h = 0; h < 2; ++h) {
-    int inclusive = len[h];
-    for (int offset = 1; offset < 64; offset <<= 1) {
-      const int v = __shfl_up(inclusive, offset, 64);
-      if (lane >= offset) inclusive += v;
-    }
-    const int r = lane + 64 * h;
-    if (r < ranges) rs[r] = carry + inclusive - len[h];
-    carry += __shfl(inclusive, 63, 64);
-  }
-  if (lane == 0) rs[ranges] = carry;
-  __builtin_amdgcn_wave_barrier();
-  const int total = carry;
-  const int cap = d.cap;
-  constexpr int CAND = 4;
-  unsigned int required = 0;
-  for (int first = cellBegin; first < cellEnd; first += 64) {
-    // lane l holds the cell's atom first + l (the last one again past the end)
-    const int atoms = min(64, cellEnd - first);
-    const int own = first + min(lane, atoms - 1);
-    const float4 mine = cellPos[own];
-    const int myAtom = __float_as_int(mine.w);
-    const int myExBegin = d.exclStart[myAtom], myExEnd = d.exclStart[myAtom + 1];
-    const int myExMin = myExBegin < myExEnd ? excl[myExBegin] : 1;
-    const int myExMax = myExBegin < myExEnd ? excl[myExEnd - 1] : 0;
-    if (lane < atoms) d.posRebuild[myAtom] = d.pos[myAtom];
-    int myFound = 0;
-    int cur = 0;
-    for (int base = 0; base < total; base += 64 * CAND) {
-      int slot[CAND];
-#pragma unroll
-      for (int u = 0; u < CAND; ++u) {
-        const int g = base + lane + 64 * u;
-        slot[u] = own;  // past the total: an in-bounds load, never accepted
-        if (g < total) {
-          while (rs[cur + 1] <= g) ++cur;
-          slot[u] = rb[cur] + (g - rs[cur]);
-        }
-      }
-      float4 p[CAND];
-#pragma unroll
-      for (int u = 0; u < CAND; ++u) p[u] = cellPos[slot[u]];
-      for (int i = 0; i < atoms; ++i) {
-        const float4 pi = make_float4(__int_as_float(__builtin_amdgcn_readlane(__float_as_int(mine.x), i)),
-                                      __int_as_float(__builtin_amdgcn_readlane(__float_as_int(mine.y), i)),
-                                      __int_as_float(__builtin_amdgcn_readlane(__float_as_int(mine.z), i)), 0.f);
-        const int atom = __builtin_amdgcn_readlane(myAtom, i);
-        const int exBegin = __builtin_amdgcn_readlane(myExBegin, i);
-        const int exEnd = __builtin_amdgcn_readlane(myExEnd, i);
-        const int exMin = __builtin_amdgcn_readlane(myExMin, i);
-        const int exMax = __builtin_amdgcn_readlane(myExMax, i);
-        int found = __builtin_amdgcn_readlane(myFound, i);
-        // a lane past the total holds a real atom of the cell: turned down by
-        // its position in the stream, not by the pair test
-        bool accept[CAND];
-#pragma unroll
-        for (int u = 0; u < CAND; ++u)
-          accept[u] = (base + lane + 64 * u < total) &
-                      acceptPair<RECT>(excl, box, pi, p[u], atom, exBegin, exEnd, exMin, exMax);
-        const GlobalOut<int> out = globalOut(d.nbrList + (size_t)(first + i) * cap);
-#pragma unroll
-        for (int u = 0; u < CAND; ++u) {
-          const unsigned long long mask = __ballot(accept[u]);
-          if (accept[u]) {
-            const int where = found + __popcll(mask & ((1ull << lane) - 1ull));
-            if (where < cap) out[where] = slot[u];
-          }
-          found += __popcll(mask);
-        }
-        if (lane == i) myFound = found;
-      }
-    }
-    if (lane < atoms) {
-      d.nbrCount[own] = myFound < cap ? myFound : cap;
-      required = max(required, (unsigned int)myFound);
-    }
-  }
-  for (int offset = 32; offset > 0; offset >>= 1) required = max(required, (unsigned int)__shfl_xor((int)required, offset, 64));
-  return required;
-}
-
-// A block is four atoms or four cells, as the box's plan says. The
-// requirement (the longest list found, stored or not) reaches
-// Scalars::required once per workgroup, and not at all where the value the
-// workgroup read at its start already covers it: every wave updating the one
-// word serialised its atomics (about 60 us of a 2854-atom rebuild's 150).
-// A workgroup's waves all come here (no early exit), so the barriers hold.
-template <bool RECT>
-__device__ __forceinline__ void searchBlock(const Dev &d, int block) {
-  if (!d.sc->rebuild) return;  // the same for every wave of the workgroup
-  const unsigned int seen = __hip_atomic_load(&d.sc->required, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-  const unsigned int need = d.box->cellSearch ? neighboursByCell<RECT>(d, block) : neighbours<RECT>(d, block);
-  __shared__ unsigned int part[4];
-  if ((threadIdx.x & 63) == 0) part[threadIdx.x >> 6] = need;
-  __syncthreads();
-  if (threadIdx.x == 0) {
-    const unsigned int m = max(max(part[0], part[1]), max(part[2], part[3]));
-    if (m > seen) atomicMax(&d.sc->required, m);
-  }
-  __syncthreads();  // part is reused by a batched workgroup's next block
 }
 
 template <bool RECT>
@@ -801,7 +372,8 @@ void launchRebuild(const Dev &d, hipStream_t s) {
     cellsAssignKernel<<<blocks, 256, 0, s>>>(d);
     cellsScanKernel<<<1, 1024, 0, s>>>(d);
     cellsScatterKernel<<<blocks, 256, 0, s>>>(d);
-    cellsSortKernel<<<d.cellCapacity, 256, 0, s>>>(d);
+    cellsSortShortKernel<<<(d.cellCapacity + SHORT_SORT_THREADS - 1) / SHORT_SORT_THREADS, SHORT_SORT_THREADS, 0, s>>>(d);
+    cellsSortLongKernel<<<std::min(d.cellCapacity, 1024), 256, 0, s>>>(d);
   }
   if (d.rect)
     neighboursKernel<true><<<(d.n + 3) / 4, 256, 0, s>>>(d);

@@ -240,7 +240,11 @@ __global__ void __launch_bounds__(256) nonbondedKernel(Dev d, float alpha) {
 // x plane of the PME forward stage, each with the same body as its own
 // kernel. A captured step runs its force branches one after another, so
 // separate launches cost the sum of their durations, where one launch
-// overlaps them.
+// overlaps them. (The cell build, the list search and the x lines stay
+// launches of their own: joining them behind in-launch waits between
+// workgroups - bounded waits, device-scope release and acquire across the
+// XCDs' caches - made the 2854-atom step 5 to 25 us slower, and its rebuild
+// twice as long, for dispatch gaps of about 1.6 us per dependent launch.)
 template <int LANES, bool ENERGIES, bool LDS_TABLE, bool RECT, bool RATIONAL>
 __global__ void __launch_bounds__(PLANE_THREADS)
     forcesJoinedKernel(Dev d, PmeDev p, float alpha, double alphaD, int terms, int pairBlocks, int planesFrom) {

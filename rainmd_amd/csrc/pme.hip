@@ -204,52 +204,6 @@ __global__ void __launch_bounds__(PLANE_THREADS) pmePlaneForwardBatchKernel(cons
   pmePlaneForward<SCAN_BATCHED>(v[t.x].d, v[t.x].p, nullptr, t.y, lds);
 }
 
-// 2. x lines (j, m0 .. m0 + XLINES - 1): forward transform, then, with
-// `convolve`, the influence function (energy in FP64, each half-spectrum
-// column weighted by its multiplicity in the full spectrum) and the inverse
-// transform.
-__device__ __forceinline__ void pmeXLines(const Dev &d, const PmeDev &p, int convolve, int energies, int block,
-                                          float2 *lds) {
-  const int K0 = p.K[0], K1 = p.K[1], K2h = p.K2h;
-  const int chunks = (K2h + XLINES - 1) / XLINES;
-  const int j = block / chunks, m0 = (block - j * chunks) * XLINES;
-  const int lines = min(XLINES, K2h - m0);
-  const int t = threadIdx.x, nt = blockDim.x;
-  float2 *tw = lds, *A = tw + K0, *B = A + XLINES * K0;
-  loadTwiddles(tw, p.twiddle[0], K0);
-  const size_t stride = (size_t)K1 * K2h;
-  float2 *col = p.cgrid + (size_t)j * K2h + m0;
-  for (int idx = t; idx < lines * K0; idx += nt) {
-    const int x = idx / lines, l = idx - x * lines;
-    A[l * K0 + x] = col[x * stride + l];
-  }
-  __syncthreads();
-  float2 *R = fftBatch(A, B, lines, K0, p.radices[0], tw, +1);
-  double energy = 0.0, dscale = 0.0;
-  if (convolve) {
-    const float *g0 = p.influence + (size_t)j * K2h + m0;
-    const double *recip = d.box->recip64;
-    for (int idx = t; idx < lines * K0; idx += nt) {
-      const int x = idx / lines, l = idx - x * lines;
-      const int m = m0 + l;
-      const double weight = (m == 0 || (p.K[2] % 2 == 0 && m == p.K[2] / 2)) ? 1.0 : 2.0;
-      const float g = g0[x * stride + l];
-      const float2 f = R[l * K0 + x];
-      const double e = weight * (double)g * ((double)f.x * f.x + (double)f.y * f.y);
-      energy += e;
-      if (energies) dscale += e * (2.0 * p.piOverAlpha2 * waveVector2(p, recip, x, j, m) - 1.0);
-      R[l * K0 + x] = make_float2(2.f * g * f.x, 2.f * g * f.y);
-    }
-    __syncthreads();
-    R = fftBatch(R, R == A ? B : A, lines, K0, p.radices[0], tw, -1);
-  }
-  for (int idx = t; idx < lines * K0; idx += nt) {
-    const int x = idx / lines, l = idx - x * lines;
-    col[x * stride + l] = R[l * K0 + x];
-  }
-  if (convolve && energies) storeReciprocal(d, energy, dscale, block);
-}
-
 __global__ void __launch_bounds__(XLINE_THREADS) pmeXLinesKernel(Dev d, PmeDev p, int convolve, int energies) {
   extern __shared__ float2 lds[];
   pmeXLines(d, p, convolve, energies, blockIdx.x, lds);
@@ -261,107 +215,6 @@ __global__ void __launch_bounds__(XLINE_THREADS) pmeXLinesBatchKernel(const Batc
   const int2 t = task[blockIdx.x];
   if (t.x < 0) return;
   pmeXLines(v[t.x].d, v[t.x].p, 1, energies, t.y, lds);
-}
-
-// 3. Inverse y and z transforms of plane i into the real potential, then each
-// atom whose support meets the plane takes that plane's share of its force
-// (webgpu/kernels/pme.js:281-336; cpu/forces/pme.js:222-257).
-template <int SCAN = SCAN_PER_THREAD>
-__device__ __forceinline__ void pmePlaneGather(const Dev &d, const PmeDev &p, int i, float2 *lds) {
-  const int K0 = p.K[0], K1 = p.K[1], K2 = p.K[2], K2h = p.K2h;
-  const int S = p.planeSpan, P = (K1 + 1) / 2;
-  const int t = threadIdx.x, nt = blockDim.x;
-  float2 *tw1 = lds, *tw2 = tw1 + K1;
-  float2 *A = tw2 + K2, *B = A + S;
-  PlaneEntry *list = reinterpret_cast<PlaneEntry *>(B + S);
-  int *listCount = reinterpret_cast<int *>(list + PLANE_LIST);
-  loadTwiddles(tw1, p.twiddle[1], K1);
-  loadTwiddles(tw2, p.twiddle[2], K2);
-  const float2 *in = p.cgrid + (size_t)i * K1 * K2h;
-  for (int idx = t; idx < K1 * K2h; idx += nt) {
-    const int j = idx / K2h, m = idx - j * K2h;
-    A[m * K1 + j] = in[idx];
-  }
-  __syncthreads();
-  float2 *Y = fftBatch(A, B, K2h, K1, p.radices[1], tw1, -1);
-  float2 *O = (Y == A) ? B : A;
-  // Hermitian extension of lines 2p and 2p+1 packed into one complex line.
-  for (int idx = t; idx < P * K2; idx += nt) {
-    const int pr = idx / K2, k = idx - pr * K2;
-    const int j0 = 2 * pr, j1 = 2 * pr + 1;
-    float2 a, b = make_float2(0.f, 0.f);
-    if (k < K2h) {
-      a = Y[k * K1 + j0];
-      if (j1 < K1) b = Y[k * K1 + j1];
-    } else {
-      a = Y[(K2 - k) * K1 + j0];
-      a.y = -a.y;
-      if (j1 < K1) {
-        b = Y[(K2 - k) * K1 + j1];
-        b.y = -b.y;
-      }
-    }
-    O[idx] = make_float2(a.x - b.y, a.y + b.x);
-  }
-  __syncthreads();
-  float2 *Z = fftBatch(O, Y, P, K2, p.radices[2], tw2, -1);
-  float *phi = reinterpret_cast<float *>(Z == A ? B : A);
-  for (int idx = t; idx < P * K2; idx += nt) {
-    const int pr = idx / K2, k = idx - pr * K2;
-    phi[2 * pr * K2 + k] = Z[idx].x;
-    if (2 * pr + 1 < K1) phi[(2 * pr + 1) * K2 + k] = Z[idx].y;
-  }
-  __syncthreads();
-  if (p.binned) {
-    // a binned system gathers per atom from the potential grid
-    // (pmeAtomGather)
-    float *out = p.rgrid + (size_t)i * K1 * K2;
-    for (int idx = t; idx < K1 * K2; idx += nt) out[idx] = phi[idx];
-    return;
-  }
-  const double *recip = d.box->recip64;
-  for (int a0 = 0; a0 < d.n; a0 += PLANE_THREADS * SCAN) {
-    if (t == 0) *listCount = 0;
-    __syncthreads();
-    planeScan<SCAN>(d, p, i, a0, list, listCount);
-    __syncthreads();
-    const int count = *listCount;
-    for (int k = t; k < count; k += nt) {
-      const int a = a0 + (list[k] >> 3), r = list[k] & 7;
-      const float4 pos = d.pos[a];
-      int bx, by, bz;
-      float ox, oy, oz;
-      gridCoordinate(p, recip, 0, pos, bx, ox);
-      float w[ORDER], dw[ORDER], wy[ORDER], dwy[ORDER], wz[ORDER], dwz[ORDER];
-      bspline5(ox, w, dw);
-      float wx = 0.f, dwx = 0.f;
-      for (int q = r; q < ORDER; q += K0) {
-        wx += w[q];
-        dwx += dw[q];
-      }
-      gridCoordinate(p, recip, 1, pos, by, oy);
-      gridCoordinate(p, recip, 2, pos, bz, oz);
-      bspline5(oy, wy, dwy);
-      bspline5(oz, wz, dwz);
-      int iz[ORDER];
-      for (int c = 0; c < ORDER; ++c) iz[c] = gridIndex(bz, c, K2);
-      float s0 = 0.f, s1 = 0.f, s2 = 0.f;
-      for (int b = 0; b < ORDER; ++b) {
-        const float *row = phi + gridIndex(by, b, K1) * K2;
-        float r0 = 0.f, r2 = 0.f;
-        for (int c = 0; c < ORDER; ++c) {
-          const float v = row[iz[c]];
-          r0 += wz[c] * v;
-          r2 += dwz[c] * v;
-        }
-        s0 += wy[b] * r0;
-        s1 += dwy[b] * r0;
-        s2 += wy[b] * r2;
-      }
-      addGridForce(d, p, a, pos.w, dwx * s0, wx * s1, wx * s2);
-    }
-    __syncthreads();
-  }
 }
 
 __global__ void __launch_bounds__(PLANE_THREADS) pmePlaneGatherKernel(Dev d, PmeDev p) {
@@ -761,6 +614,8 @@ void pmeBatchLdsLimits(size_t forwardLds, size_t xlinesLds, size_t gatherLds) {
                             (int)gatherLds);
 }
 
+// The plane pipeline after its forward planes: x lines, then the plane
+// gathers; returns the x lines' energy slots.
 int launchPmeAfterForward(const Dev &d, const PmeDev &p, int energies, hipStream_t s) {
   const int xBlocks = xLineBlocks(p);
   pmeXLinesKernel<<<xBlocks, XLINE_THREADS, xLinesLds(p), s>>>(d, p, 1, energies);
