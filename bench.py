@@ -224,7 +224,8 @@ def replica_batch(args, count, warmup, steps, single_steps, screen=False):
     batched direct space's roofline, the smallest and largest replicas'
     energy and force gates, and the same replicas run one system per launch
     for `single_steps` each (each alone through its own context, graphs and
-    concurrent force branches on, 64 lanes per atom). Returns the result and,
+    concurrent force branches on, at the lanes per atom the autotuner picked
+    for a single system of its size). Returns the result and,
     with the screen, what the plain-barostat comparison needs."""
     import rainmd_amd as R
 
@@ -309,12 +310,13 @@ def replica_batch(args, count, warmup, steps, single_steps, screen=False):
             log(f"energy gate replica {r} ({atoms[r]} atoms): max rel {gates[r][0]:.2e}, forces {gates[r][2]:.2e}")
 
     # one system per launch: every replica alone through its own context, the
-    # same kernels and options, at the single system's 64 lanes per atom
+    # same kernels and options, at the lanes per atom tuned for a single system
+    # of its size
     single_total = 0.0
     for r in range(count if single_steps > 0 else 0):
         sim = batch.replica(r)
         lanes = sim.diagnostics()["lanes_per_atom"]
-        sim.set_lanes(64)
+        sim.set_lanes(R.tuned_shape("single", atoms[r])["lanes_per_atom"])
         sim.run(args.block, args.block)  # captures its block graph
         if single_steps % args.block:
             sim.run(single_steps % args.block, args.block)
@@ -350,7 +352,8 @@ def replica_batch(args, count, warmup, steps, single_steps, screen=False):
             "unit": "ns/day",
             "ms_per_step": single_ms,
             "sample": f"every replica run alone for {single_steps} steps through its own context "
-                      "(the batch's options; graphs, concurrent force branches and 64 lanes per atom)",
+                      "(the batch's options; graphs, concurrent force branches and the single-system tuned lanes "
+                      "per atom for its size)",
         } if single_ms else None,
         "batch_speedup": aggregate / single_aggregate if single_ms else None,
         "barostat_in_window": {"attempts": moves, "accepted": accepted},

@@ -222,6 +222,39 @@ __global__ void __launch_bounds__(PLANE_THREADS) pmePlaneGatherKernel(Dev d, Pme
   pmePlaneGather(d, p, blockIdx.x, lds);
 }
 
+// A large binned system's planes in workgroups of WIDE_PLANE_THREADS: its
+// plane's LDS (153 KB at 112^3) leaves room for one workgroup per CU, and K0
+// planes occupy fewer CUs than the chip has, so each plane's spread,
+// transforms and gather get the CU's four SIMDs at four waves each. (The
+// scan list is not used by a binned system; SCAN 3 keeps a round of it within
+// the list's LDS at this width.) The results are the narrow kernels' bit for
+// bit: the plane is an integer sum and each transform output has one formula.
+constexpr int WIDE_PLANE_THREADS = 1024;
+
+__global__ void __launch_bounds__(WIDE_PLANE_THREADS) pmePlaneForwardWideKernel(Dev d, PmeDev p) {
+  extern __shared__ float2 lds[];
+  pmePlaneForward<PLANE_LIST / WIDE_PLANE_THREADS>(d, p, nullptr, blockIdx.x, lds);
+}
+
+__global__ void __launch_bounds__(WIDE_PLANE_THREADS) pmePlaneGatherWideKernel(Dev d, PmeDev p) {
+  extern __shared__ float2 lds[];
+  pmePlaneGather<PLANE_LIST / WIDE_PLANE_THREADS>(d, p, blockIdx.x, lds);
+}
+
+static void launchPlaneForward(const Dev &d, const PmeDev &p, size_t lds, hipStream_t s) {
+  if (p.binned)
+    pmePlaneForwardWideKernel<<<p.K[0], WIDE_PLANE_THREADS, lds, s>>>(d, p);
+  else
+    pmePlaneForwardKernel<<<p.K[0], PLANE_THREADS, lds, s>>>(d, p, nullptr);
+}
+
+static void launchPlaneGather(const Dev &d, const PmeDev &p, size_t lds, hipStream_t s) {
+  if (p.binned)
+    pmePlaneGatherWideKernel<<<p.K[0], WIDE_PLANE_THREADS, lds, s>>>(d, p);
+  else
+    pmePlaneGatherKernel<<<p.K[0], PLANE_THREADS, lds, s>>>(d, p);
+}
+
 // (Six waves per SIMD: three workgroups per CU, the register count the body
 // needs without spills.)
 __global__ void __launch_bounds__(PLANE_THREADS) __attribute__((amdgpu_waves_per_eu(6)))
@@ -578,6 +611,10 @@ bool pmeConfigure(PmeDev &p, int mode, size_t maxLds) {
                               (int)planeForwardLds(p));
     (void)hipFuncSetAttribute((const void *)pmePlaneGatherKernel, hipFuncAttributeMaxDynamicSharedMemorySize,
                               (int)planeGatherLds(p));
+    (void)hipFuncSetAttribute((const void *)pmePlaneForwardWideKernel, hipFuncAttributeMaxDynamicSharedMemorySize,
+                              (int)planeForwardLds(p));
+    (void)hipFuncSetAttribute((const void *)pmePlaneGatherWideKernel, hipFuncAttributeMaxDynamicSharedMemorySize,
+                              (int)planeGatherLds(p));
   }
   return mode != PME_PLANES || fits;
 }
@@ -619,17 +656,17 @@ void pmeBatchLdsLimits(size_t forwardLds, size_t xlinesLds, size_t gatherLds) {
 int launchPmeAfterForward(const Dev &d, const PmeDev &p, int energies, hipStream_t s) {
   const int xBlocks = xLineBlocks(p);
   pmeXLinesKernel<<<xBlocks, XLINE_THREADS, xLinesLds(p), s>>>(d, p, 1, energies);
-  pmePlaneGatherKernel<<<p.K[0], PLANE_THREADS, planeGatherLds(p), s>>>(d, p);
+  launchPlaneGather(d, p, planeGatherLds(p), s);
   return xBlocks;
 }
 
 int launchPme(const Dev &d, const PmeDev &p, int energies, hipStream_t s, int *slotsUsed) {
   if (p.planes) {
     launchPmeBins(d, p, s);
-    pmePlaneForwardKernel<<<p.K[0], PLANE_THREADS, planeForwardLds(p), s>>>(d, p, nullptr);
+    launchPlaneForward(d, p, planeForwardLds(p), s);
     const int xBlocks = xLineBlocks(p);
     pmeXLinesKernel<<<xBlocks, XLINE_THREADS, xLinesLds(p), s>>>(d, p, 1, energies);
-    pmePlaneGatherKernel<<<p.K[0], PLANE_THREADS, planeGatherLds(p), s>>>(d, p);
+    launchPlaneGather(d, p, planeGatherLds(p), s);
     if (p.binned) pmeAtomGatherKernel<<<(d.n + 255) / 256, 256, 0, s>>>(d, p);
     if (slotsUsed) *slotsUsed = xBlocks;
     return xBlocks;
@@ -658,11 +695,11 @@ void launchPmeStages(const Dev &d, const PmeDev &p, hipStream_t s, hipEvent_t ev
   (void)hipEventRecord(ev[0], s);
   if (p.planes) {
     launchPmeBins(d, p, s);
-    pmePlaneForwardKernel<<<p.K[0], PLANE_THREADS, planeForwardLds(p), s>>>(d, p, nullptr);
+    launchPlaneForward(d, p, planeForwardLds(p), s);
     (void)hipEventRecord(ev[1], s);
     pmeXLinesKernel<<<xLineBlocks(p), XLINE_THREADS, xLinesLds(p), s>>>(d, p, 1, 0);
     (void)hipEventRecord(ev[2], s);
-    pmePlaneGatherKernel<<<p.K[0], PLANE_THREADS, planeGatherLds(p), s>>>(d, p);
+    launchPlaneGather(d, p, planeGatherLds(p), s);
     if (p.binned) pmeAtomGatherKernel<<<(d.n + 255) / 256, 256, 0, s>>>(d, p);
     (void)hipEventRecord(ev[3], s);
     return;

@@ -18,10 +18,10 @@ struct TunedShape {
 };
 
 constexpr TunedShape TUNED_SHAPES[] = {
-    {0, 0, 32, 2, 1, 0.0, 2},
-    {0, 2000, 32, 3, 0, 0.0, 0},
+    {0, 0, 64, 2, 1, 0.0, 2},
+    {0, 2000, 32, 3, 1, 0.4, 0},
     {0, 6000, 16, 3, 2, 0.0, 1},
-    {0, 40000, 16, 3, 2, 0.2, 0},
+    {0, 40000, 16, 3, 2, 0.15, 0},
     {1, 0, 16, 3, 0, 0.12, 0},
 };
 
