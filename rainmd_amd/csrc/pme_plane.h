@@ -319,16 +319,22 @@ constexpr int PLANE_LIST = PLANE_THREADS * SCAN_PER_THREAD;
 static_assert(PLANE_LIST <= (1 << 13), "a list entry holds the atom's offset in its round in 13 bits");
 typedef unsigned short PlaneEntry;
 
+// The round's positions, loaded apart from their use so that a kernel can
+// issue its first round's loads at its very start, beside its other loads.
 template <int SCAN>
-__device__ void planeScan(const Dev &d, const PmeDev &p, int plane, int a0, PlaneEntry *list, int *count) {
-  const double *recip = d.box->recip64;
-  const int lane = threadIdx.x & 63;
-  float4 batch[SCAN];
+__device__ __forceinline__ void planeScanLoad(const Dev &d, int a0, float4 batch[SCAN]) {
 #pragma unroll
   for (int u = 0; u < SCAN; ++u) {
     const int a = a0 + u * blockDim.x + threadIdx.x;
     batch[u] = a < d.n ? d.pos[a] : make_float4(0.f, 0.f, 0.f, 0.f);
   }
+}
+
+template <int SCAN>
+__device__ __forceinline__ void planeScanTake(const Dev &d, const PmeDev &p, int plane, int a0, const float4 batch[SCAN],
+                                              PlaneEntry *list, int *count) {
+  const double *recip = d.box->recip64;
+  const int lane = threadIdx.x & 63;
 #pragma unroll
   for (int u = 0; u < SCAN; ++u) {
     const int a = a0 + u * blockDim.x + threadIdx.x;
@@ -348,6 +354,13 @@ __device__ void planeScan(const Dev &d, const PmeDev &p, int plane, int a0, Plan
   }
 }
 
+template <int SCAN>
+__device__ void planeScan(const Dev &d, const PmeDev &p, int plane, int a0, PlaneEntry *list, int *count) {
+  float4 batch[SCAN];
+  planeScanLoad<SCAN>(d, a0, batch);
+  planeScanTake<SCAN>(d, p, plane, a0, batch, list, count);
+}
+
 
 
 // 1. Spread onto plane i and forward z, y transforms of the plane. With
@@ -363,6 +376,10 @@ __device__ __forceinline__ void pmePlaneForward(const Dev &d, const PmeDev &p, c
   long long *plane = reinterpret_cast<long long *>(B);
   PlaneEntry *list = reinterpret_cast<PlaneEntry *>(B + p.planeRegion);
   int *listCount = reinterpret_cast<int *>(list + PLANE_LIST);
+  // the first scan round's positions, in flight with the twiddles' loads
+  const bool scanning = !real && !p.binned;
+  float4 first[SCAN];
+  if (scanning) planeScanLoad<SCAN>(d, 0, first);
   loadTwiddles(tw1, p.twiddle[1], K1);
   loadTwiddles(tw2, p.twiddle[2], K2);
   if (real) {
@@ -410,10 +427,13 @@ __device__ __forceinline__ void pmePlaneForward(const Dev &d, const PmeDev &p, c
       for (int h = 0; h < 2; ++h)
         for (int k = range[h].x + t; k < range[h].y; k += nt) spread(p.binAtoms[k], -1);
     } else {
-      for (int a0 = 0; a0 < d.n; a0 += PLANE_THREADS * SCAN) {
+      for (int a0 = 0; a0 < d.n; a0 += nt * SCAN) {
         if (t == 0) *listCount = 0;
         __syncthreads();
-        planeScan<SCAN>(d, p, i, a0, list, listCount);
+        if (a0 == 0)
+          planeScanTake<SCAN>(d, p, i, 0, first, list, listCount);
+        else
+          planeScan<SCAN>(d, p, i, a0, list, listCount);
         __syncthreads();
         const int count = *listCount;
         for (int k = t; k < count; k += nt) spread(a0 + (list[k] >> 3), list[k] & 7);
@@ -509,6 +529,11 @@ __device__ __forceinline__ void pmePlaneGather(const Dev &d, const PmeDev &p, in
   float2 *A = tw2 + K2, *B = A + S;
   PlaneEntry *list = reinterpret_cast<PlaneEntry *>(B + S);
   int *listCount = reinterpret_cast<int *>(list + PLANE_LIST);
+  // The first scan round's positions are loaded with the plane's spectrum and
+  // its list taken before the transforms (the list has LDS of its own), so
+  // the positions' load latency overlaps the spectrum's.
+  float4 first[SCAN];
+  if (!p.binned) planeScanLoad<SCAN>(d, 0, first);
   loadTwiddles(tw1, p.twiddle[1], K1);
   loadTwiddles(tw2, p.twiddle[2], K2);
   const float2 *in = p.cgrid + (size_t)i * K1 * K2h;
@@ -516,7 +541,9 @@ __device__ __forceinline__ void pmePlaneGather(const Dev &d, const PmeDev &p, in
     const int j = idx / K2h, m = idx - j * K2h;
     A[m * K1 + j] = in[idx];
   }
+  if (!p.binned && t == 0) *listCount = 0;
   __syncthreads();
+  if (!p.binned) planeScanTake<SCAN>(d, p, i, 0, first, list, listCount);
   float2 *Y = fftBatch(A, B, K2h, K1, p.radices[1], tw1, -1);
   float2 *O = (Y == A) ? B : A;
   // Hermitian extension of lines 2p and 2p+1 packed into one complex line.
@@ -554,10 +581,12 @@ __device__ __forceinline__ void pmePlaneGather(const Dev &d, const PmeDev &p, in
     return;
   }
   const double *recip = d.box->recip64;
-  for (int a0 = 0; a0 < d.n; a0 += PLANE_THREADS * SCAN) {
-    if (t == 0) *listCount = 0;
-    __syncthreads();
-    planeScan<SCAN>(d, p, i, a0, list, listCount);
+  for (int a0 = 0; a0 < d.n; a0 += nt * SCAN) {
+    if (a0 > 0) {  // round 0's list was taken before the transforms
+      if (t == 0) *listCount = 0;
+      __syncthreads();
+      planeScan<SCAN>(d, p, i, a0, list, listCount);
+    }
     __syncthreads();
     const int count = *listCount;
     for (int k = t; k < count; k += nt) {

@@ -1,6 +1,7 @@
 """The kernels' approximations come from the fit scripts committed beside
-them: the erfc and erf polynomials (csrc/erfc_fit.py, device_common.h) and
-the rational screened-Coulomb force (csrc/ewald_force_fit.py, nonbonded.hip)
+them: the erfc and erf polynomials (csrc/erfc_fit.py, device_common.h), the
+rational screened-Coulomb force (csrc/ewald_force_fit.py, nonbonded.hip) and
+the thermostat noise's sine and cosine (csrc/sincos_fit.py, integrate.hip)
 are what the scripts produce, and the fits meet their stated accuracy."""
 import math
 import os
@@ -16,6 +17,7 @@ sys.path.insert(0, CSRC)
 
 import erfc_fit  # noqa: E402
 import ewald_force_fit  # noqa: E402
+import sincos_fit  # noqa: E402
 
 
 def _held(path, name, count):
@@ -46,3 +48,21 @@ def test_rational_force_coefficients_are_the_fit_s():
     ws = np.linspace(0.0, ewald_force_fit.W_MAX, 2001)
     exact = np.array([ewald_force_fit.g_exact(w) for w in ws])
     assert np.abs(np.polyval(p[::-1], ws) / np.polyval(q[::-1], ws) / exact - 1).max() < 1e-7
+
+
+def test_noise_sine_and_cosine_are_the_fit_s():
+    s, c = sincos_fit.fit()
+    hs, hc = sincos_fit.device_coefficients()
+    # the kernel's Horner chains run from the highest power down
+    assert np.allclose(hs, s[::-1], rtol=1e-9, atol=0)
+    assert np.allclose(hc, c[::-1], rtol=1e-9, atol=0)
+    # the kernel's FP32 arithmetic over the counter hash's uniforms, and the
+    # quadrant rotation at its edges (u = 1/8, 3/8, ... and near 0 and 1)
+    u = np.concatenate([sincos_fit.noise_uniforms(500_000, seed=3),
+                        ((np.arange(0, 1 << 23, 4099) + 0.5) * 2.0 ** -23).astype(np.float32),
+                        np.array([0.5 * 2.0 ** -23, 1 - 0.5 * 2.0 ** -23, 0.125, 0.375, 0.625, 0.875],
+                                 dtype=np.float32)])
+    fs, fc = sincos_fit.sincos_f32(s, c, u)
+    exact = 2 * np.pi * u.astype(np.float64)
+    assert np.abs(fs - np.sin(exact)).max() < 1.5e-7
+    assert np.abs(fc - np.cos(exact)).max() < 1.5e-7

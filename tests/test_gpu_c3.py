@@ -116,3 +116,29 @@ def test_c3_steps_hold_the_water_geometry(c3):
     # loose; the point is a finite state with rigid waters
     assert np.isfinite(e["potential"]) and 250 < e["temperature"] < 700
     assert err.max() < 5e-5  # f32 positions at 10 nm carry ~5e-7 nm of rounding
+
+
+def test_c3_crowded_cells_at_the_reference_reach(c3):
+    # cell_reach 1 is the reference's plan: 8^3 cells of ~196 atoms, where the
+    # multi-kernel build sorts each cell with a workgroup's bitonic sort (the
+    # 26^3 plan's cells go through the one-thread-per-cell sort). The cells are
+    # the FP32 emulation's and the pair set is the same, so the forces and
+    # energies equal the 26^3 plan's up to the order of their FP32 lane sums.
+    system, sim = c3
+    pos = sim.read_positions().astype(np.float64)
+    ref_terms, ref_forces = sim.evaluate()
+    wide = R.GpuSimulation(system, temperature=300.0, seed=13, skin=0.15, grid=[112, 112, 112], cell_reach=1)
+    wide.set_state(pos, np.zeros_like(pos), step=0)
+    terms, forces = wide.evaluate()
+    d = wide.diagnostics()
+    cell, start, atoms = wide.read_cells()
+    print("C3 reach 1: cells", d["cell_counts"], "atoms per cell", np.diff(start).min(), np.diff(start).max())
+    assert d["cell_reach"] == 1 and d["cell_counts"] == [8, 8, 8]
+    assert np.diff(start).max() > 32
+    assert np.array_equal(cell, O.cells_f32(wide.read_positions(), system.box, d["cell_counts"]))
+    assert all(np.all(np.diff(atoms[start[c]:start[c + 1]]) > 0) for c in range(len(start) - 1))
+    for key in ("lennard_jones", "coulomb_direct", "coulomb_reciprocal"):
+        print("C3 reach 1", key, terms[key], ref_terms[key])
+        assert rel(terms[key], ref_terms[key]) < 1e-9, key
+    assert rel_rms(forces, ref_forces) < 1e-6
+    wide.release()
