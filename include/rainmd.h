@@ -343,6 +343,8 @@ typedef struct rmd_diagnostics {
   int64_t steps_since_barostat; /* steps run since the last volume move (a
                                    move is due at a block start once this
                                    reaches the barostat interval) */
+  int32_t pme_x_folded; /* 1: force-only evaluations fold the PME x lines into
+                           the plane gathers (rmd_set_pme_pipeline mode 4) */
 } rmd_diagnostics;
 int rmd_diagnostics_get(rmd_ctx *ctx, rmd_diagnostics *out);
 
@@ -355,7 +357,8 @@ enum {
 };
 /* One force evaluation at the current positions with a fresh neighbour list,
  * restricted to `terms`. Fills energies (FP64 sums; may be NULL) and forces
- * (3N doubles converted from the fixed-point accumulators; may be NULL). The
+ * (3N doubles converted from the fixed-point accumulators; may be NULL).
+ * With energies NULL the evaluation runs a step's force-only launches. The
  * reference's counterpart is #evaluatePotential (simulation.js:734-789). */
 int rmd_evaluate(rmd_ctx *ctx, int32_t terms, rmd_energy_terms *energies, double *forces);
 
@@ -412,9 +415,14 @@ int rmd_set_graphs(rmd_ctx *ctx, int32_t enabled);
  * neighbour list and the direct space. Results are identical either way. */
 int rmd_set_concurrent_forces(rmd_ctx *ctx, int32_t enabled);
 /* PME pipeline: 0 automatic (planes when one x plane of the grid fits in
- * LDS), 1 planes (RMD_E_INVALID when it does not fit), 2 lines, 3 planes with
+ * LDS; mode 4 for a single system of at most 32 x planes that does not bin),
+ * 1 planes (RMD_E_INVALID when it does not fit), 2 lines, 3 planes with
  * every plane scanning every atom (no atom bins, which large single systems
- * and batch members build by default). Results are identical for 1 and 3. */
+ * and batch members build by default), 4 as 3 with force-only evaluations
+ * folding the x lines into the plane gathers (one launch fewer per
+ * evaluation; RMD_E_INVALID beyond 32 x planes or for a batch member).
+ * Results are identical for 1 and 3; mode 4's forces differ from theirs by
+ * FP32 rounding (evaluations with energies are those of mode 3). */
 int rmd_set_pme_pipeline(rmd_ctx *ctx, int32_t mode);
 int rmd_get_stream(rmd_ctx *ctx, void **stream);
 /* Average launch duration (ms) of one kernel family at the current state,

@@ -198,6 +198,7 @@ class Diagnostics(C.Structure):
         ("pme_planes", C.c_int32),
         ("cell_reach", C.c_int32), ("search_by_cell", C.c_int32),
         ("steps_since_barostat", C.c_int64),
+        ("pme_x_folded", C.c_int32),
     ]
 
 

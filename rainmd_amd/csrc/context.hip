@@ -170,6 +170,7 @@ struct rmd_ctx {
   double volumeScale = -1;
   int64_t stepsSinceBarostat = 0;
   int pmeBinned = 0;   // whether the plane pipeline bins its atoms (set at creation)
+  int pmeFolded = 0;   // whether force-only evaluations fold the x lines into the gathers (set at creation)
   int reachMax = 0;    // the cell plan's widest reach (0: CELL_REACH_MAX)
   int searchMode = 0;  // 0 by occupancy, 1 per atom, 2 per cell where the launch covers the cells
   int neighbourGrowths = 0;
@@ -235,6 +236,7 @@ void writeBox(rmd_ctx *c) {
     c->d.rect = rect;
     invalidateGraphs(c);
   }
+  c->p.xReal = rect;
   const BoxF b = boxFor(c);
   HIP_CHECK(hipMemcpyAsync(c->boxDev, &b, sizeof b, hipMemcpyHostToDevice, c->stream));
   // the box crosses from pageable host memory: finish before `b` goes
@@ -1180,6 +1182,17 @@ void create(rmd_ctx *c, const System &sys, const rmd_options &o, const Streams *
     }
   }
   c->pmeBinned = p.binned;
+  // A small single system folds its x lines into the plane gathers on
+  // force-only evaluations (PmeDev::xFolded); the kernel X is kept whenever
+  // the grid allows it, so the pipeline can be switched either way.
+  p.xFolded = 0;
+  p.xReal = 1;
+  p.xKernel = nullptr;
+  if (!shared && p.K[0] <= X_FOLD_MAX_K) {
+    p.xKernel = devAlloc<float>(2 * half, c->held);
+    p.xFolded = p.planes && !p.binned ? 1 : 0;
+  }
+  c->pmeFolded = p.xFolded;
   writeBox(c);
   setupIntegrator(c, 0);
   // Initial state from the system's own coordinates, if it has them.
@@ -1423,7 +1436,8 @@ int rmd_energy(rmd_ctx *c, rmd_energy_terms *out) {
 int rmd_evaluate(rmd_ctx *c, int32_t terms, rmd_energy_terms *energies, double *forces) {
   CTX_GUARD(c, {
     double t[E_TERM_COUNT];
-    evaluate(c, terms ? terms : RMD_TERM_ALL, true, t);
+    // without energies, the force-only launches of a step
+    evaluate(c, terms ? terms : RMD_TERM_ALL, true, energies ? t : nullptr);
     if (energies) fillEnergy(c, t, energies, false, 0, 0, false);
     if (forces) toDouble(c, forces);
   });
@@ -1732,6 +1746,7 @@ int rmd_diagnostics_get(rmd_ctx *c, rmd_diagnostics *o) {
     o->pme_alpha = c->pme.alpha;
     o->lanes_per_atom = c->lanes;
     o->pme_planes = c->p.planes;
+    o->pme_x_folded = c->p.planes && c->p.xFolded && !c->p.binned ? 1 : 0;
     o->water_units = c->waterUnits;
     o->cluster_units = c->clusterUnits;
     o->free_units = c->freeUnits;
@@ -1991,12 +2006,17 @@ int rmd_set_concurrent_forces(rmd_ctx *c, int32_t enabled) {
 
 int rmd_set_pme_pipeline(rmd_ctx *c, int32_t mode) {
   CTX_GUARD(c, {
-    if (mode < PME_AUTO || mode > PME_LINES + 1) throw Error(RMD_E_INVALID, "PME pipeline must be 0, 1, 2 or 3");
-    if (!pmeConfigure(c->p, mode == PME_LINES + 1 ? PME_PLANES : mode, c->maxLds))
+    if (mode < PME_AUTO || mode > PME_PLANES_FOLDED)
+      throw Error(RMD_E_INVALID, "PME pipeline must be 0, 1, 2, 3 or 4");
+    const bool scan = mode == PME_PLANES_SCAN || mode == PME_PLANES_FOLDED;
+    if (!pmeConfigure(c->p, scan ? PME_PLANES : mode, c->maxLds))
       throw Error(RMD_E_INVALID, "one x plane of this PME grid does not fit in LDS");
+    if (mode == PME_PLANES_FOLDED && !c->p.xKernel)
+      throw Error(RMD_E_INVALID, "folded x lines need a single system of at most 32 x planes");
     // planes without atom bins (every plane scans every atom), or the bins
-    // the context was created with
-    c->p.binned = mode == PME_LINES + 1 ? 0 : c->pmeBinned;
+    // the context was created with; folded x lines as asked, or as created
+    c->p.binned = scan ? 0 : c->pmeBinned;
+    c->p.xFolded = mode == PME_PLANES_FOLDED ? 1 : mode == PME_AUTO ? c->pmeFolded : 0;
     invalidateGraphs(c);
   });
 }

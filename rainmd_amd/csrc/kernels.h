@@ -53,8 +53,24 @@ struct PmeDev {
   // the scatter. 0: one workgroup builds them (batch members).
   int binGroups;
   int *binTable;            // K0 x binGroups counts, then offsets within each bin
+  // A small single system's force-only evaluations fold the x lines into the
+  // plane gathers: the forward x transform, the influence function and the
+  // inverse x transform of every (y, z) column are one circular convolution
+  // along x with the kernel
+  //   X(l, c) = sum_k 2 g(k, c) exp(-2 pi i k l / K0),  l in [0, K0),
+  // g the influence, so gather workgroup i forms column c of its plane as
+  // sum_x X(i - x, c) F(x, c) from the forward planes F - one launch and
+  // its dependency fewer on the PME chain. Evaluations with energies run the
+  // x lines (the energy and its scaling derivative are sums over the full
+  // spectrum).
+  int xFolded;              // 1: force-only evaluations fold the x lines into the gathers
+  int xReal;                // 1: X is real (a rectangular box, where g is even in k)
+  float *xKernel;           // X as K0 x (K1 * K2h) floats (xReal) or float2s; null when not allocated
 };
-enum { PME_AUTO = 0, PME_PLANES = 1, PME_LINES = 2 };
+// Grids up to this many x planes may fold their x lines into the gathers
+// (each gather workgroup reads all K0 forward planes and K0 kernel rows).
+constexpr int X_FOLD_MAX_K = 32;
+enum { PME_AUTO = 0, PME_PLANES = 1, PME_LINES = 2, PME_PLANES_SCAN = 3, PME_PLANES_FOLDED = 4 };
 // Chooses the pipeline (PME_AUTO: planes when a plane fits in maxLds bytes of
 // LDS). Returns false when PME_PLANES was asked for and does not fit.
 bool pmeConfigure(PmeDev &p, int mode, size_t maxLds);

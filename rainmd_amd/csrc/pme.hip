@@ -217,9 +217,9 @@ __global__ void __launch_bounds__(XLINE_THREADS) pmeXLinesBatchKernel(const Batc
   pmeXLines(v[t.x].d, v[t.x].p, 1, energies, t.y, lds);
 }
 
-__global__ void __launch_bounds__(PLANE_THREADS) pmePlaneGatherKernel(Dev d, PmeDev p) {
+__global__ void __launch_bounds__(PLANE_THREADS) pmePlaneGatherKernel(Dev d, PmeDev p, int folded) {
   extern __shared__ float2 lds[];
-  pmePlaneGather(d, p, blockIdx.x, lds);
+  pmePlaneGather(d, p, blockIdx.x, lds, folded);
 }
 
 // A large binned system's planes in workgroups of WIDE_PLANE_THREADS: its
@@ -248,11 +248,11 @@ static void launchPlaneForward(const Dev &d, const PmeDev &p, size_t lds, hipStr
     pmePlaneForwardKernel<<<p.K[0], PLANE_THREADS, lds, s>>>(d, p, nullptr);
 }
 
-static void launchPlaneGather(const Dev &d, const PmeDev &p, size_t lds, hipStream_t s) {
+static void launchPlaneGather(const Dev &d, const PmeDev &p, size_t lds, hipStream_t s, int folded = 0) {
   if (p.binned)
     pmePlaneGatherWideKernel<<<p.K[0], WIDE_PLANE_THREADS, lds, s>>>(d, p);
   else
-    pmePlaneGatherKernel<<<p.K[0], PLANE_THREADS, lds, s>>>(d, p);
+    pmePlaneGatherKernel<<<p.K[0], PLANE_THREADS, lds, s>>>(d, p, folded);
 }
 
 // (Six waves per SIMD: three workgroups per CU, the register count the body
@@ -578,9 +578,34 @@ __global__ void __launch_bounds__(256) pmeInfluenceBatchKernel(const BatchView *
                (size_t)t.y * blockDim.x + threadIdx.x);
 }
 
+// The folded gathers' kernel X(l, c) = sum_k 2 g(k, c) exp(-2 pi i k l / K0)
+// (PmeDev::xFolded), one thread per (l, c), summed in FP64 over ascending k
+// with the angle's numerator reduced modulo K0; for a rectangular box g is
+// even in k and X real (the sine terms cancel), stored as its real part.
+__global__ void pmeXKernelKernel(PmeDev p) {
+  const int K0 = p.K[0], PK = p.K[1] * p.K2h;
+  const size_t idx = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (idx >= (size_t)K0 * PK) return;
+  const int l = (int)(idx / PK), c = (int)(idx - (size_t)l * PK);
+  double re = 0.0, im = 0.0;
+  for (int k = 0; k < K0; ++k) {
+    const double g = 2.0 * (double)p.influence[(size_t)k * PK + c];
+    const int r = (k * l) % K0;
+    double sn, cs;
+    sincospi(2.0 * r / K0, &sn, &cs);
+    re += g * cs;
+    im -= g * sn;
+  }
+  if (p.xReal)
+    p.xKernel[idx] = (float)re;
+  else
+    reinterpret_cast<float2 *>(p.xKernel)[idx] = make_float2((float)re, (float)im);
+}
+
 void pmeComputeInfluence(const PmeDev &p, const BoxF *box, const double *moduli[3], double alpha, hipStream_t s) {
   const size_t total = (size_t)p.K[0] * p.K[1] * p.K2h;
   pmeInfluenceKernel<<<(unsigned)((total + 255) / 256), 256, 0, s>>>(p, box, moduli[0], moduli[1], moduli[2], alpha);
+  if (p.xKernel) pmeXKernelKernel<<<(unsigned)((total + 255) / 256), 256, 0, s>>>(p);
 }
 
 // ---- configuration and launchers --------------------------------------------
@@ -652,21 +677,25 @@ void pmeBatchLdsLimits(size_t forwardLds, size_t xlinesLds, size_t gatherLds) {
 }
 
 // The plane pipeline after its forward planes: x lines, then the plane
-// gathers; returns the x lines' energy slots.
-int launchPmeAfterForward(const Dev &d, const PmeDev &p, int energies, hipStream_t s) {
+// gathers, or, for a force-only evaluation of a folding system, the gathers
+// alone; returns the x lines' energy slots.
+static int launchAfterForward(const Dev &d, const PmeDev &p, int energies, hipStream_t s) {
   const int xBlocks = xLineBlocks(p);
-  pmeXLinesKernel<<<xBlocks, XLINE_THREADS, xLinesLds(p), s>>>(d, p, 1, energies);
-  launchPlaneGather(d, p, planeGatherLds(p), s);
+  const int folded = p.xFolded && !energies && !p.binned;
+  if (!folded) pmeXLinesKernel<<<xBlocks, XLINE_THREADS, xLinesLds(p), s>>>(d, p, 1, energies);
+  launchPlaneGather(d, p, planeGatherLds(p), s, folded);
   return xBlocks;
+}
+
+int launchPmeAfterForward(const Dev &d, const PmeDev &p, int energies, hipStream_t s) {
+  return launchAfterForward(d, p, energies, s);
 }
 
 int launchPme(const Dev &d, const PmeDev &p, int energies, hipStream_t s, int *slotsUsed) {
   if (p.planes) {
     launchPmeBins(d, p, s);
     launchPlaneForward(d, p, planeForwardLds(p), s);
-    const int xBlocks = xLineBlocks(p);
-    pmeXLinesKernel<<<xBlocks, XLINE_THREADS, xLinesLds(p), s>>>(d, p, 1, energies);
-    launchPlaneGather(d, p, planeGatherLds(p), s);
+    const int xBlocks = launchAfterForward(d, p, energies, s);
     if (p.binned) pmeAtomGatherKernel<<<(d.n + 255) / 256, 256, 0, s>>>(d, p);
     if (slotsUsed) *slotsUsed = xBlocks;
     return xBlocks;
@@ -690,16 +719,18 @@ int launchPme(const Dev &d, const PmeDev &p, int energies, hipStream_t s, int *s
 // The chain of launchPme (no energies) in three stages - the spread with the
 // forward transforms it carries, the remaining transforms with the
 // convolution, the inverse transforms with the gather - with ev[k] recorded
-// before stage k and ev[3] after the last, for per-stage timing.
+// before stage k and ev[3] after the last, for per-stage timing. (Folded x
+// lines leave the middle stage empty and are timed with the gathers.)
 void launchPmeStages(const Dev &d, const PmeDev &p, hipStream_t s, hipEvent_t ev[4]) {
   (void)hipEventRecord(ev[0], s);
   if (p.planes) {
     launchPmeBins(d, p, s);
     launchPlaneForward(d, p, planeForwardLds(p), s);
     (void)hipEventRecord(ev[1], s);
-    pmeXLinesKernel<<<xLineBlocks(p), XLINE_THREADS, xLinesLds(p), s>>>(d, p, 1, 0);
+    const int folded = p.xFolded && !p.binned;
+    if (!folded) pmeXLinesKernel<<<xLineBlocks(p), XLINE_THREADS, xLinesLds(p), s>>>(d, p, 1, 0);
     (void)hipEventRecord(ev[2], s);
-    launchPlaneGather(d, p, planeGatherLds(p), s);
+    launchPlaneGather(d, p, planeGatherLds(p), s, folded);
     if (p.binned) pmeAtomGatherKernel<<<(d.n + 255) / 256, 256, 0, s>>>(d, p);
     (void)hipEventRecord(ev[3], s);
     return;

@@ -517,11 +517,53 @@ __device__ __forceinline__ void pmeXLines(const Dev &d, const PmeDev &p, int con
   if (convolve && energies) storeReciprocal(d, energy, dscale, block);
 }
 
+// 2'. The x lines folded into plane i's gather (PmeDev::xFolded): column c
+// of the plane is sum_x X(i - x, c) F(x, c) over the forward planes F, in
+// ascending x, each thread one column at a time. The loads of a run of x are
+// issued together (the planes were written by other workgroups of the
+// previous launch; the kernel rows are read-only).
+__device__ __forceinline__ void planeXFold(const PmeDev &p, int i, float2 *A) {
+  const int K0 = p.K[0], K1 = p.K[1], K2h = p.K2h, PK = K1 * K2h;
+  constexpr int RUN = 8;
+  for (int c = threadIdx.x; c < PK; c += blockDim.x) {
+    const int j = c / K2h, m = c - j * K2h;
+    float re = 0.f, im = 0.f;
+    for (int x0 = 0; x0 < K0; x0 += RUN) {
+      float2 f[RUN], k[RUN];
+#pragma unroll
+      for (int u = 0; u < RUN; ++u) {
+        // past the last plane: the last plane's loads, weighted zero
+        const bool in = x0 + u < K0;
+        const int x = in ? x0 + u : K0 - 1;
+        int l = i - x;
+        l += l < 0 ? K0 : 0;
+        const float2 fx = p.cgrid[(size_t)x * PK + c];
+        const float2 kx = p.xReal ? make_float2(p.xKernel[(size_t)l * PK + c], 0.f)
+                                  : reinterpret_cast<const float2 *>(p.xKernel)[(size_t)l * PK + c];
+        f[u] = in ? fx : make_float2(0.f, 0.f);
+        k[u] = in ? kx : make_float2(0.f, 0.f);
+      }
+#pragma unroll
+      for (int u = 0; u < RUN; ++u) {
+        re = fmaf(k[u].x, f[u].x, re);
+        im = fmaf(k[u].x, f[u].y, im);
+        if (!p.xReal) {
+          re = fmaf(-k[u].y, f[u].y, re);
+          im = fmaf(k[u].y, f[u].x, im);
+        }
+      }
+    }
+    A[m * K1 + j] = make_float2(re, im);
+  }
+}
+
 // 3. Inverse y and z transforms of plane i into the real potential, then each
 // atom whose support meets the plane takes that plane's share of its force
-// (webgpu/kernels/pme.js:281-336; cpu/forces/pme.js:222-257).
+// (webgpu/kernels/pme.js:281-336; cpu/forces/pme.js:222-257). With `folded`
+// the plane's spectrum is formed from the forward planes (planeXFold) instead
+// of read from the x lines' output.
 template <int SCAN = SCAN_PER_THREAD>
-__device__ __forceinline__ void pmePlaneGather(const Dev &d, const PmeDev &p, int i, float2 *lds) {
+__device__ __forceinline__ void pmePlaneGather(const Dev &d, const PmeDev &p, int i, float2 *lds, int folded = 0) {
   const int K0 = p.K[0], K1 = p.K[1], K2 = p.K[2], K2h = p.K2h;
   const int S = p.planeSpan, P = (K1 + 1) / 2;
   const int t = threadIdx.x, nt = blockDim.x;
@@ -536,10 +578,14 @@ __device__ __forceinline__ void pmePlaneGather(const Dev &d, const PmeDev &p, in
   if (!p.binned) planeScanLoad<SCAN>(d, 0, first);
   loadTwiddles(tw1, p.twiddle[1], K1);
   loadTwiddles(tw2, p.twiddle[2], K2);
-  const float2 *in = p.cgrid + (size_t)i * K1 * K2h;
-  for (int idx = t; idx < K1 * K2h; idx += nt) {
-    const int j = idx / K2h, m = idx - j * K2h;
-    A[m * K1 + j] = in[idx];
+  if (folded) {
+    planeXFold(p, i, A);
+  } else {
+    const float2 *in = p.cgrid + (size_t)i * K1 * K2h;
+    for (int idx = t; idx < K1 * K2h; idx += nt) {
+      const int j = idx / K2h, m = idx - j * K2h;
+      A[m * K1 + j] = in[idx];
+    }
   }
   if (!p.binned && t == 0) *listCount = 0;
   __syncthreads();

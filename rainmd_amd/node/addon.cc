@@ -781,6 +781,7 @@ napi_value Diagnostics(napi_env env, napi_callback_info info) {
   setNumber(env, o, "cellReach", d.cell_reach);
   setBool(env, o, "searchByCell", d.search_by_cell != 0);
   setNumber(env, o, "stepsSinceBarostat", (double)d.steps_since_barostat);
+  setBool(env, o, "pmeXFolded", d.pme_x_folded != 0);
   return o;
 }
 

@@ -424,12 +424,15 @@ class GpuSimulation:
         d["kinetic_sampled_at"] = "before the closing half kick" if e.kinetic_sampled_in_step else "after the step"
         return d
 
-    def evaluate(self, terms=L.TERM_ALL, forces=True):
+    def evaluate(self, terms=L.TERM_ALL, forces=True, energies=True):
+        """(energies, forces) at the current positions with a fresh list;
+        energies=False runs a step's force-only launches and returns None for
+        the energies."""
         e = L.EnergyTerms()
         f = np.zeros(3 * self.atom_count) if forces else None
-        _check(L.load().rmd_evaluate(self._h, terms, C.byref(e), f.ctypes.data_as(L._dp) if forces else None),
-               self._h)
-        return _struct_dict(e), (f.reshape(-1, 3) if forces else None)
+        _check(L.load().rmd_evaluate(self._h, terms, C.byref(e) if energies else None,
+                                     f.ctypes.data_as(L._dp) if forces else None), self._h)
+        return (_struct_dict(e) if energies else None), (f.reshape(-1, 3) if forces else None)
 
     def read_positions(self):
         out = np.zeros(3 * self.atom_count, dtype=np.float32)
@@ -548,8 +551,10 @@ class GpuSimulation:
         _check(L.load().rmd_set_concurrent_forces(self._h, 1 if enabled else 0), self._h)
 
     def set_pme_pipeline(self, mode):
-        """'auto', 'planes', 'lines' or 'planes-scan' (planes without atom bins)."""
-        ids = {"auto": 0, "planes": 1, "lines": 2, "planes-scan": 3}
+        """'auto', 'planes', 'lines', 'planes-scan' (planes without atom bins)
+        or 'planes-folded' (planes-scan with the x lines folded into the plane
+        gathers on force-only evaluations)."""
+        ids = {"auto": 0, "planes": 1, "lines": 2, "planes-scan": 3, "planes-folded": 4}
         _check(L.load().rmd_set_pme_pipeline(self._h, ids[mode]), self._h)
 
     def stream(self):

@@ -57,6 +57,9 @@ def _run_pair(options, steps, graphs=True, max_neighbours=None):
     singles = []
     for r, (sys_, seed) in enumerate(zip(systems, seeds)):
         sim = R.GpuSimulation(sys_, seed=seed, **SHAPE, **extra)
+        # the batch members' PME x lines (a small single system would
+        # otherwise fold them into its plane gathers)
+        sim.set_pme_pipeline("planes")
         sim.set_graphs(graphs)
         sim.set_state(*starts[r], step=0)
         sim.run(steps)

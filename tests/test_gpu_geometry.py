@@ -8,6 +8,11 @@
   single run.
 - An Ewald alpha beyond the rational force fit ((alpha rc)^2 > 12) takes the
   erfc force: energies and forces against the FP64 oracle.
+- A small system's force-only evaluations fold the PME x lines into the
+  plane gathers (a circular convolution along x), in a cube and in a
+  triclinic box (where the convolution kernel is complex): reciprocal and
+  total forces within FP32 rounding of the x lines' and within the force bar
+  of the FP64 oracle; evaluations with energies are the x lines' own.
 """
 import numpy as np
 import pytest
@@ -87,6 +92,7 @@ def test_batch_with_a_triclinic_replica_equals_single_runs():
     batch.run(60)
     for r, (sys_, seed) in enumerate(zip(systems, seeds)):
         sim = R.GpuSimulation(sys_, seed=seed, **options)
+        sim.set_pme_pipeline("planes")  # the batch members' x lines
         sim.set_state(*starts[r], step=0)
         sim.run(60)
         sb, ss = batch.replica(r).read_state(), sim.read_state()
@@ -106,3 +112,37 @@ def test_alpha_beyond_the_rational_force_fit_takes_erfc():
     sim.minimise(max_iterations=60, force_tolerance=1000.0)
     _check("alpha 4.2", system, sim, alpha, grid)
     sim.release()
+
+
+@pytest.mark.parametrize("shape", ["cube", "triclinic"])
+def test_folded_x_lines(shape):
+    system = R.workload_system("ala-944w")
+    if shape == "triclinic":
+        system = _triclinic(system)
+    sim = R.GpuSimulation(system, temperature=300.0, seed=5)
+    diag = sim.diagnostics()
+    assert diag["pme_x_folded"] == 1 and diag["pme_grid"][0] <= 32
+    recip = R._lib.TERM_RECIPROCAL
+    _, folded = sim.evaluate(terms=recip, energies=False)
+    _, folded_all = sim.evaluate(energies=False)
+    e_folded, _ = sim.evaluate(terms=recip)
+    sim.set_pme_pipeline("planes-scan")
+    assert sim.diagnostics()["pme_x_folded"] == 0
+    _, lines = sim.evaluate(terms=recip, energies=False)
+    _, lines_all = sim.evaluate(energies=False)
+    e_lines, _ = sim.evaluate(terms=recip)
+    sim.set_pme_pipeline("auto")
+    assert sim.diagnostics()["pme_x_folded"] == 1
+    assert e_folded["coulomb_reciprocal"] == e_lines["coulomb_reciprocal"]
+    pos = sim.read_positions().astype(np.float64)
+    oracle = O.Oracle(system)
+    oracle.set_box(np.asarray(sim.read_state()["box_vectors"]).reshape(9))
+    _, ref = oracle.potential(pos, alpha=diag["pme_alpha"], grid=list(diag["pme_grid"]), terms=recip,
+                              f32_membership=True)
+    d_recip, d_all = rel_rms(folded, lines), rel_rms(folded_all, lines_all)
+    o_folded, o_lines = rel_rms(folded, ref), rel_rms(lines, ref)
+    print(shape, "folded vs x lines: reciprocal", d_recip, "all terms", d_all,
+          "| reciprocal vs FP64: folded", o_folded, "x lines", o_lines)
+    sim.release()
+    assert d_recip < 2e-6 and d_all < 1e-6
+    assert o_folded < FORCE_BAR and o_folded < 1.5 * o_lines + 1e-7
