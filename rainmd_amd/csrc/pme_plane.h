@@ -519,12 +519,15 @@ __device__ __forceinline__ void pmeXLines(const Dev &d, const PmeDev &p, int con
 
 // 2'. The x lines folded into plane i's gather (PmeDev::xFolded): column c
 // of the plane is sum_x X(i - x, c) F(x, c) over the forward planes F, in
-// ascending x, each thread one column at a time. The loads of a run of x are
+// ascending x, each thread one column at a time. The loads of RUN planes are
 // issued together (the planes were written by other workgroups of the
-// previous launch; the kernel rows are read-only).
+// previous launch; the kernel rows are read-only). REAL: X is real.
+template <bool REAL>
 __device__ __forceinline__ void planeXFold(const PmeDev &p, int i, float2 *A) {
   const int K0 = p.K[0], K1 = p.K[1], K2h = p.K2h, PK = K1 * K2h;
-  constexpr int RUN = 8;
+  constexpr int RUN = 16;
+  const float *Xr = p.xKernel;
+  const float2 *Xc = reinterpret_cast<const float2 *>(p.xKernel);
   for (int c = threadIdx.x; c < PK; c += blockDim.x) {
     const int j = c / K2h, m = c - j * K2h;
     float re = 0.f, im = 0.f;
@@ -538,8 +541,7 @@ __device__ __forceinline__ void planeXFold(const PmeDev &p, int i, float2 *A) {
         int l = i - x;
         l += l < 0 ? K0 : 0;
         const float2 fx = p.cgrid[(size_t)x * PK + c];
-        const float2 kx = p.xReal ? make_float2(p.xKernel[(size_t)l * PK + c], 0.f)
-                                  : reinterpret_cast<const float2 *>(p.xKernel)[(size_t)l * PK + c];
+        const float2 kx = REAL ? make_float2(Xr[(size_t)l * PK + c], 0.f) : Xc[(size_t)l * PK + c];
         f[u] = in ? fx : make_float2(0.f, 0.f);
         k[u] = in ? kx : make_float2(0.f, 0.f);
       }
@@ -547,7 +549,7 @@ __device__ __forceinline__ void planeXFold(const PmeDev &p, int i, float2 *A) {
       for (int u = 0; u < RUN; ++u) {
         re = fmaf(k[u].x, f[u].x, re);
         im = fmaf(k[u].x, f[u].y, im);
-        if (!p.xReal) {
+        if (!REAL) {
           re = fmaf(-k[u].y, f[u].y, re);
           im = fmaf(k[u].y, f[u].x, im);
         }
@@ -579,7 +581,10 @@ __device__ __forceinline__ void pmePlaneGather(const Dev &d, const PmeDev &p, in
   loadTwiddles(tw1, p.twiddle[1], K1);
   loadTwiddles(tw2, p.twiddle[2], K2);
   if (folded) {
-    planeXFold(p, i, A);
+    if (p.xReal)
+      planeXFold<true>(p, i, A);
+    else
+      planeXFold<false>(p, i, A);
   } else {
     const float2 *in = p.cgrid + (size_t)i * K1 * K2h;
     for (int idx = t; idx < K1 * K2h; idx += nt) {
