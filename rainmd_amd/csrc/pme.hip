@@ -217,18 +217,17 @@ __global__ void __launch_bounds__(XLINE_THREADS) pmeXLinesBatchKernel(const Batc
   pmeXLines(v[t.x].d, v[t.x].p, 1, energies, t.y, lds);
 }
 
-__global__ void __launch_bounds__(PLANE_THREADS) pmePlaneGatherKernel(Dev d, PmeDev p, int folded) {
-  extern __shared__ float2 lds[];
-  pmePlaneGather(d, p, blockIdx.x, lds, folded);
-}
-
 // A large binned system's planes in workgroups of WIDE_PLANE_THREADS: its
 // plane's LDS (153 KB at 112^3) leaves room for one workgroup per CU, and K0
 // planes occupy fewer CUs than the chip has, so each plane's spread,
-// transforms and gather get the CU's four SIMDs at four waves each. (The
-// scan list is not used by a binned system; SCAN 3 keeps a round of it within
-// the list's LDS at this width.) The results are the narrow kernels' bit for
-// bit: the plane is an integer sum and each transform output has one formula.
+// transforms and gather get the CU's four SIMDs at four waves each. (SCAN 3
+// keeps a round of the scan list within its LDS at this width.) Every single
+// system gathers in such workgroups - K0 planes leave most CUs idle whatever
+// the system's size, and a plane's atoms are gathered twice as wide (2854
+// atoms, 30^3: 18.3 -> 17.1 us per chain). The results are the 512-thread
+// kernels' bit for bit: the plane is an integer sum, each transform output
+// and each atom's share of a plane has one formula, and forces are integer
+// sums.
 constexpr int WIDE_PLANE_THREADS = 1024;
 
 __global__ void __launch_bounds__(WIDE_PLANE_THREADS) pmePlaneForwardWideKernel(Dev d, PmeDev p) {
@@ -236,9 +235,9 @@ __global__ void __launch_bounds__(WIDE_PLANE_THREADS) pmePlaneForwardWideKernel(
   pmePlaneForward<PLANE_LIST / WIDE_PLANE_THREADS>(d, p, nullptr, blockIdx.x, lds);
 }
 
-__global__ void __launch_bounds__(WIDE_PLANE_THREADS) pmePlaneGatherWideKernel(Dev d, PmeDev p) {
+__global__ void __launch_bounds__(WIDE_PLANE_THREADS) pmePlaneGatherKernel(Dev d, PmeDev p, int folded) {
   extern __shared__ float2 lds[];
-  pmePlaneGather<PLANE_LIST / WIDE_PLANE_THREADS>(d, p, blockIdx.x, lds);
+  pmePlaneGather<PLANE_LIST / WIDE_PLANE_THREADS>(d, p, blockIdx.x, lds, folded);
 }
 
 static void launchPlaneForward(const Dev &d, const PmeDev &p, size_t lds, hipStream_t s) {
@@ -249,10 +248,7 @@ static void launchPlaneForward(const Dev &d, const PmeDev &p, size_t lds, hipStr
 }
 
 static void launchPlaneGather(const Dev &d, const PmeDev &p, size_t lds, hipStream_t s, int folded = 0) {
-  if (p.binned)
-    pmePlaneGatherWideKernel<<<p.K[0], WIDE_PLANE_THREADS, lds, s>>>(d, p);
-  else
-    pmePlaneGatherKernel<<<p.K[0], PLANE_THREADS, lds, s>>>(d, p, folded);
+  pmePlaneGatherKernel<<<p.K[0], WIDE_PLANE_THREADS, lds, s>>>(d, p, folded);
 }
 
 // (Six waves per SIMD: three workgroups per CU, the register count the body
@@ -638,8 +634,6 @@ bool pmeConfigure(PmeDev &p, int mode, size_t maxLds) {
                               (int)planeGatherLds(p));
     (void)hipFuncSetAttribute((const void *)pmePlaneForwardWideKernel, hipFuncAttributeMaxDynamicSharedMemorySize,
                               (int)planeForwardLds(p));
-    (void)hipFuncSetAttribute((const void *)pmePlaneGatherWideKernel, hipFuncAttributeMaxDynamicSharedMemorySize,
-                              (int)planeGatherLds(p));
   }
   return mode != PME_PLANES || fits;
 }
