@@ -575,22 +575,24 @@ __global__ void __launch_bounds__(256) pmeInfluenceBatchKernel(const BatchView *
 }
 
 // The folded gathers' kernel X(l, c) = sum_k 2 g(k, c) exp(-2 pi i k l / K0)
-// (PmeDev::xFolded), one thread per (l, c), summed in FP64 over ascending k
-// with the angle's numerator reduced modulo K0; for a rectangular box g is
-// even in k and X real (the sine terms cancel), stored as its real part.
+// (PmeDev::xFolded), one thread per (l, c), summed in FP64 over ascending k,
+// the roots from the x axis' twiddle table (exp(+2 pi i r / K0) at the
+// numerator r = k l reduced modulo K0); for a rectangular box g is even in k
+// and X real (the sine terms cancel), stored as its real part.
 __global__ void pmeXKernelKernel(PmeDev p) {
   const int K0 = p.K[0], PK = p.K[1] * p.K2h;
   const size_t idx = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (idx >= (size_t)K0 * PK) return;
   const int l = (int)(idx / PK), c = (int)(idx - (size_t)l * PK);
   double re = 0.0, im = 0.0;
+  int r = 0;  // k l mod K0
   for (int k = 0; k < K0; ++k) {
     const double g = 2.0 * (double)p.influence[(size_t)k * PK + c];
-    const int r = (k * l) % K0;
-    double sn, cs;
-    sincospi(2.0 * r / K0, &sn, &cs);
-    re += g * cs;
-    im -= g * sn;
+    const float2 w = p.twiddle[0][r];
+    re += g * (double)w.x;
+    im -= g * (double)w.y;
+    r += l;
+    r -= r >= K0 ? K0 : 0;
   }
   if (p.xReal)
     p.xKernel[idx] = (float)re;
