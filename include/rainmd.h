@@ -429,11 +429,13 @@ int rmd_get_stream(rmd_ctx *ctx, void **stream);
  * timed with HIP events on the context's stream over `reps` launches:
  * 0 nonbonded, 1 PME chain, 2 integrator, 3 bonded/exceptions, 4 rebuild. */
 int rmd_profile_kernel(rmd_ctx *ctx, int32_t kernel, int32_t reps, double *avg_ms);
-/* Average durations (ms) of the PME chain's three stages over reps chains at
- * the current state, HIP events on the context's stream between them:
- * stage_ms[0] the charge spread with the forward transforms it carries,
- * [1] the remaining transforms and the reciprocal convolution, [2] the
- * inverse transforms and the force gather (webgpu/kernels/pme.js:137-336). */
+/* Average durations (ms) of the PME chain's three stages at the current
+ * state, each over reps back-to-back launches of that stage on the context's
+ * stream (HIP events around them): stage_ms[0] the charge spread with the
+ * forward transforms it carries, [1] the remaining transforms and the
+ * reciprocal convolution (0 where force-only evaluations fold them into the
+ * gathers, rmd_set_pme_pipeline mode 4), [2] the inverse transforms and the
+ * force gather (webgpu/kernels/pme.js:137-336). */
 int rmd_profile_pme_stages(rmd_ctx *ctx, int32_t reps, double *stage_ms);
 /* Measurement calibration: `reps` launches of a plain device-to-device copy
  * of `bytes` bytes (16-byte vector loads and stores) on the context's stream;

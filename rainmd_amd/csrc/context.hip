@@ -8,6 +8,7 @@
 #include <cmath>
 #include <cstring>
 #include <functional>
+#include <initializer_list>
 #include <map>
 #include <unordered_map>
 #include <numeric>
@@ -1912,27 +1913,34 @@ int rmd_profile_kernel(rmd_ctx *c, int32_t kernel, int32_t reps, double *avg_ms)
 
 // Average durations of the PME chain's three stages (spread with the forward
 // transforms it carries; the remaining transforms and the convolution; the
-// inverse transforms and the gather), HIP events on the context's stream
-// between the stages, over reps chains at the current state.
+// inverse transforms and the gather), each over reps back-to-back launches of
+// the stage on the context's stream (the middle stage as the first stage with
+// it less the first stage alone: the x lines transform in place), at the
+// current state; the middle stage is 0 where the gathers fold it in.
 int rmd_profile_pme_stages(rmd_ctx *c, int32_t reps, double *stage_ms) {
   CTX_GUARD(c, {
     if (!stage_ms || reps <= 0) throw Error(RMD_E_INVALID, "bad arguments");
     evaluate(c, RMD_TERM_ALL, true, nullptr);
-    hipEvent_t ev[4];
-    for (auto &e : ev) HIP_CHECK(hipEventCreate(&e));
-    double total[3] = {0, 0, 0};
-    launchPmeStages(c->d, c->p, c->stream, ev);  // warm
-    for (int r = 0; r < reps; ++r) {
-      launchPmeStages(c->d, c->p, c->stream, ev);
-      HIP_CHECK(hipEventSynchronize(ev[3]));
-      for (int k = 0; k < 3; ++k) {
-        float ms = 0;
-        HIP_CHECK(hipEventElapsedTime(&ms, ev[k], ev[k + 1]));
-        total[k] += ms;
-      }
-    }
-    for (auto &e : ev) (void)hipEventDestroy(e);
-    for (int k = 0; k < 3; ++k) stage_ms[k] = total[k] / reps;
+    hipEvent_t a, b;
+    HIP_CHECK(hipEventCreate(&a));
+    HIP_CHECK(hipEventCreate(&b));
+    auto timed = [&](std::initializer_list<int> stages) {
+      for (int st : stages) launchPmeStage(c->d, c->p, c->stream, st);  // warm
+      HIP_CHECK(hipEventRecord(a, c->stream));
+      for (int r = 0; r < reps; ++r)
+        for (int st : stages) launchPmeStage(c->d, c->p, c->stream, st);
+      HIP_CHECK(hipEventRecord(b, c->stream));
+      HIP_CHECK(hipEventSynchronize(b));
+      float ms = 0;
+      HIP_CHECK(hipEventElapsedTime(&ms, a, b));
+      return (double)ms / reps;
+    };
+    const double forward = timed({0});
+    stage_ms[0] = forward;
+    stage_ms[1] = pmeTransformsFolded(c->p) ? 0.0 : std::max(0.0, timed({0, 1}) - forward);
+    stage_ms[2] = timed({2});
+    (void)hipEventDestroy(a);
+    (void)hipEventDestroy(b);
     evaluate(c, RMD_TERM_ALL, true, nullptr);
   });
 }

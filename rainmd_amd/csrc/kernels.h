@@ -93,7 +93,8 @@ bool forcesJoin(const Dev &d, const PmeDev &p, int lanes);
 void launchForcesJoined(const Dev &d, const PmeDev &p, float alpha, int rational, int lanes, int energies,
                         double alphaD, int terms, hipStream_t s, int *pairSlots, int *bondedSlots);
 int launchPmeAfterForward(const Dev &d, const PmeDev &p, int energies, hipStream_t s);
-void launchPmeStages(const Dev &d, const PmeDev &p, hipStream_t s, hipEvent_t ev[4]);
+void launchPmeStage(const Dev &d, const PmeDev &p, hipStream_t s, int stage);
+bool pmeTransformsFolded(const PmeDev &p);
 // Debug entry: forward R2C 3D FFT of a real grid into the half spectrum.
 void launchFftForward(const Dev &d, const PmeDev &p, const float *real, float2 *out, hipStream_t s);
 

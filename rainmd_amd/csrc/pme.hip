@@ -712,41 +712,43 @@ int launchPme(const Dev &d, const PmeDev &p, int energies, hipStream_t s, int *s
   return xBlocks;
 }
 
-// The chain of launchPme (no energies) in three stages - the spread with the
-// forward transforms it carries, the remaining transforms with the
-// convolution, the inverse transforms with the gather - with ev[k] recorded
-// before stage k and ev[3] after the last, for per-stage timing. (Folded x
-// lines leave the middle stage empty and are timed with the gathers.)
-void launchPmeStages(const Dev &d, const PmeDev &p, hipStream_t s, hipEvent_t ev[4]) {
-  (void)hipEventRecord(ev[0], s);
+// One stage of launchPme's chain (no energies), for per-stage timing: 0 the
+// spread with the forward transforms it carries, 1 the remaining transforms
+// with the convolution (nothing where a force-only chain folds the x lines
+// into the gathers), 2 the inverse transforms with the gather.
+void launchPmeStage(const Dev &d, const PmeDev &p, hipStream_t s, int stage) {
   if (p.planes) {
-    launchPmeBins(d, p, s);
-    launchPlaneForward(d, p, planeForwardLds(p), s);
-    (void)hipEventRecord(ev[1], s);
     const int folded = p.xFolded && !p.binned;
-    if (!folded) pmeXLinesKernel<<<xLineBlocks(p), XLINE_THREADS, xLinesLds(p), s>>>(d, p, 1, 0);
-    (void)hipEventRecord(ev[2], s);
-    launchPlaneGather(d, p, planeGatherLds(p), s, folded);
-    if (p.binned) pmeAtomGatherKernel<<<(d.n + 255) / 256, 256, 0, s>>>(d, p);
-    (void)hipEventRecord(ev[3], s);
+    if (stage == 0) {
+      launchPmeBins(d, p, s);
+      launchPlaneForward(d, p, planeForwardLds(p), s);
+    } else if (stage == 1) {
+      if (!folded) pmeXLinesKernel<<<xLineBlocks(p), XLINE_THREADS, xLinesLds(p), s>>>(d, p, 1, 0);
+    } else {
+      launchPlaneGather(d, p, planeGatherLds(p), s, folded);
+      if (p.binned) pmeAtomGatherKernel<<<(d.n + 255) / 256, 256, 0, s>>>(d, p);
+    }
     return;
   }
   const int atomBlocks = (d.n + 255) / 256;
-  pmeSpreadKernel<<<atomBlocks, 256, 0, s>>>(d, p);
-  const int zPairs = (p.K[0] * p.K[1] + 1) / 2;
-  const int zBlocks = (zPairs + LINES_PER_BLOCK - 1) / LINES_PER_BLOCK;
-  (void)hipEventRecord(ev[1], s);
-  pmeZForwardKernel<<<zBlocks, 256, lineLds(p.K[2]), s>>>(p);
-  const int yBlocks = (p.K[0] * p.K2h + LINES_PER_BLOCK - 1) / LINES_PER_BLOCK;
-  pmeYKernel<<<yBlocks, 256, lineLds(p.K[1]), s>>>(p, +1);
-  const int xBlocks = (p.K[1] * p.K2h + LINES_PER_BLOCK - 1) / LINES_PER_BLOCK;
-  pmeXConvolveKernel<<<xBlocks, 256, lineLds(p.K[0]), s>>>(d, p, 1, 0);
-  pmeYKernel<<<yBlocks, 256, lineLds(p.K[1]), s>>>(p, -1);
-  pmeZInverseKernel<<<zBlocks, 256, lineLds(p.K[2]), s>>>(p);
-  (void)hipEventRecord(ev[2], s);
-  pmeGatherKernel<<<atomBlocks, 256, 0, s>>>(d, p);
-  (void)hipEventRecord(ev[3], s);
+  if (stage == 0) {
+    pmeSpreadKernel<<<atomBlocks, 256, 0, s>>>(d, p);
+  } else if (stage == 1) {
+    const int zPairs = (p.K[0] * p.K[1] + 1) / 2;
+    const int zBlocks = (zPairs + LINES_PER_BLOCK - 1) / LINES_PER_BLOCK;
+    pmeZForwardKernel<<<zBlocks, 256, lineLds(p.K[2]), s>>>(p);
+    const int yBlocks = (p.K[0] * p.K2h + LINES_PER_BLOCK - 1) / LINES_PER_BLOCK;
+    pmeYKernel<<<yBlocks, 256, lineLds(p.K[1]), s>>>(p, +1);
+    const int xBlocks = (p.K[1] * p.K2h + LINES_PER_BLOCK - 1) / LINES_PER_BLOCK;
+    pmeXConvolveKernel<<<xBlocks, 256, lineLds(p.K[0]), s>>>(d, p, 1, 0);
+    pmeYKernel<<<yBlocks, 256, lineLds(p.K[1]), s>>>(p, -1);
+    pmeZInverseKernel<<<zBlocks, 256, lineLds(p.K[2]), s>>>(p);
+  } else {
+    pmeGatherKernel<<<atomBlocks, 256, 0, s>>>(d, p);
+  }
 }
+
+bool pmeTransformsFolded(const PmeDev &p) { return p.planes && p.xFolded && !p.binned; }
 
 // Verification entry: the forward R2C 3D transform of a real grid, through
 // the same kernels as the active pipeline.
