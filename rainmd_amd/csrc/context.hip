@@ -238,10 +238,8 @@ void writeBox(rmd_ctx *c) {
     invalidateGraphs(c);
   }
   c->p.xReal = rect;
-  const BoxF b = boxFor(c);
-  HIP_CHECK(hipMemcpyAsync(c->boxDev, &b, sizeof b, hipMemcpyHostToDevice, c->stream));
-  // the box crosses from pageable host memory: finish before `b` goes
-  HIP_CHECK(hipStreamSynchronize(c->stream));
+  // the box travels as a kernel argument: no staging buffer, no wait
+  launchSetBox(c->boxDev, boxFor(c), c->stream);
   ewaldSelfEnergy(c->sys, c->pme.alpha, volumeOf(c->box), &c->selfEnergy, &c->netEnergy);
   const double *mods[3] = {c->dModuli[0], c->dModuli[1], c->dModuli[2]};
   pmeComputeInfluence(c->p, c->boxDev, mods, c->pme.alpha, c->stream);
@@ -448,6 +446,7 @@ void setupIntegrator(rmd_ctx *c, int suppliedNoise) {
 // ---- force evaluation ---------------------------------------------------
 
 __global__ void decideRebuildKernel(Scalars *sc, double dispHalf, int force) {
+  sc->requiredMax = 0u;  // this evaluation's requirement starts here
   const float maxDisp2 = __uint_as_float(sc->maxDisp2Bits);
   const bool rebuild = force || sc->forceRebuild || displacementExceeded(sc, maxDisp2, dispHalf);
   sc->rebuild = rebuild ? 1u : 0u;
@@ -540,12 +539,20 @@ void checkScalars(rmd_ctx *c, Scalars &h) {
 // reference's #evaluatePotential, which always rebuilds.
 void evaluate(rmd_ctx *c, int terms, bool forceRebuild, double terms_out[E_TERM_COUNT]) {
   for (int attempt = 0; attempt < 4; ++attempt) {
-    HIP_CHECK(hipMemsetAsync(&c->d.sc->requiredMax, 0, sizeof(unsigned int), c->stream));
     decideRebuildKernel<<<1, 1, 0, c->stream>>>(c->d.sc, c->ip.dispHalf, forceRebuild ? 1 : 0);
     launchZeroForces(c->d, c->stream);
     launchRefreshSlots(c->d, c->stream);
     int slots[E_TERM_COUNT];
     enqueueForces(c, terms, terms_out ? 1 : 0, slots);
+    if (terms_out) {
+      // reduced and copied before the one wait below (discarded if the list
+      // overflowed and the evaluation runs again)
+      SlotCounts counts;
+      std::memcpy(counts.c, slots, sizeof slots);
+      launchReduceEnergies(c->d, counts, c->dTerms, c->stream);
+      HIP_CHECK(hipMemcpyAsync(terms_out, c->dTerms, sizeof(double) * E_TERM_COUNT, hipMemcpyDeviceToHost,
+                               c->stream));
+    }
     HIP_CHECK(hipGetLastError());
     Scalars h;
     checkScalars(c, h);
@@ -559,15 +566,7 @@ void evaluate(rmd_ctx *c, int terms, bool forceRebuild, double terms_out[E_TERM_
       forceRebuild = true;
       continue;
     }
-    if (terms_out) {
-      std::memcpy(c->slotCounts, slots, sizeof slots);
-      SlotCounts counts;
-      std::memcpy(counts.c, slots, sizeof slots);
-      launchReduceEnergies(c->d, counts, c->dTerms, c->stream);
-      HIP_CHECK(hipMemcpyAsync(terms_out, c->dTerms, sizeof(double) * E_TERM_COUNT, hipMemcpyDeviceToHost,
-                               c->stream));
-      HIP_CHECK(hipStreamSynchronize(c->stream));
-    }
+    if (terms_out) std::memcpy(c->slotCounts, slots, sizeof slots);
     return;
   }
   throw Error(RMD_E_NEIGHBOUR_OVERFLOW, "the neighbour list kept overflowing after growth");
@@ -725,25 +724,23 @@ void launchBlock(rmd_ctx *c, int steps) {
   HIP_CHECK(hipGraphLaunch(it->second, c->stream));
 }
 
+SnapshotBuffers savedBuffers(const rmd_ctx *c) {
+  return SnapshotBuffers{c->posSaved, c->velSaved, c->rebuildSaved, c->forceSaved, c->scSaved};
+}
+
+// The state a block starts from (positions, velocities, forces, Scalars, with
+// the block's list requirement cleared first), to run the block again if the
+// list overflows in it; one launch each way.
 void snapshot(rmd_ctx *c) {
-  const size_t n = c->d.nPad;
-  HIP_CHECK(hipMemcpyAsync(c->posSaved, c->d.pos, n * sizeof(float4), hipMemcpyDeviceToDevice, c->stream));
-  HIP_CHECK(hipMemcpyAsync(c->velSaved, c->d.vel, n * sizeof(float4), hipMemcpyDeviceToDevice, c->stream));
-  HIP_CHECK(hipMemcpyAsync(c->forceSaved, c->d.force, 3 * n * sizeof(long long), hipMemcpyDeviceToDevice, c->stream));
-  HIP_CHECK(hipMemcpyAsync(c->scSaved, c->d.sc, sizeof(Scalars), hipMemcpyDeviceToDevice, c->stream));
+  launchSnapshot(c->d, savedBuffers(c), SNAP_VELOCITIES | SNAP_SCALARS, c->stream);
 }
 
 void restore(rmd_ctx *c) {
-  const size_t n = c->d.nPad;
-  HIP_CHECK(hipMemcpyAsync(c->d.pos, c->posSaved, n * sizeof(float4), hipMemcpyDeviceToDevice, c->stream));
-  HIP_CHECK(hipMemcpyAsync(c->d.vel, c->velSaved, n * sizeof(float4), hipMemcpyDeviceToDevice, c->stream));
-  HIP_CHECK(hipMemcpyAsync(c->d.force, c->forceSaved, 3 * n * sizeof(long long), hipMemcpyDeviceToDevice, c->stream));
-  HIP_CHECK(hipMemcpyAsync(c->d.sc, c->scSaved, sizeof(Scalars), hipMemcpyDeviceToDevice, c->stream));
+  launchSnapshot(c->d, savedBuffers(c), SNAP_RESTORE | SNAP_VELOCITIES | SNAP_SCALARS, c->stream);
 }
 
 void runBlock(rmd_ctx *c, int steps) {
   for (int attempt = 0; attempt < 4; ++attempt) {
-    HIP_CHECK(hipMemsetAsync(&c->d.sc->requiredMax, 0, sizeof(unsigned int), c->stream));
     snapshot(c);
     launchBlock(c, steps);
     HIP_CHECK(hipMemcpyAsync(c->blockEndTerms, c->dTermsBlock, sizeof c->blockEndTerms, hipMemcpyDeviceToHost,
@@ -1513,13 +1510,9 @@ int rmd_attempt_volume_change(rmd_ctx *c, rmd_move *m) {
       evaluate(c, RMD_TERM_ALL, true, oldTerms);
     }
     const double oldPotential = potentialFromTerms(c, oldTerms);
-    const size_t np = c->d.nPad;
     double oldBox[9];
     std::memcpy(oldBox, c->box, sizeof oldBox);
-    HIP_CHECK(hipMemcpyAsync(c->posSaved, c->d.pos, np * sizeof(float4), hipMemcpyDeviceToDevice, c->stream));
-    HIP_CHECK(hipMemcpyAsync(c->rebuildSaved, c->d.posRebuild, np * sizeof(float4), hipMemcpyDeviceToDevice,
-                             c->stream));
-    HIP_CHECK(hipMemcpyAsync(c->forceSaved, c->d.force, 3 * np * sizeof(long long), hipMemcpyDeviceToDevice, c->stream));
+    launchSnapshot(c->d, savedBuffers(c), SNAP_REBUILD, c->stream);  // positions, rebuild positions, forces
     const unsigned int rebuildsBefore = c->rebuildCount;
     c->barostatAttempts += 1;
     c->windowAttempts += 1;
@@ -1604,17 +1597,11 @@ int rmd_attempt_volume_change(rmd_ctx *c, rmd_move *m) {
         // back), forces; the old energies describe it again. The list stays
         // with its rebuild references restored, unless the trial rebuilt it
         // (then it describes the trial state) or replaced it.
-        HIP_CHECK(hipMemcpyAsync(c->d.pos, c->posSaved, np * sizeof(float4), hipMemcpyDeviceToDevice, c->stream));
         const bool replaced = updateBoxHost(c, oldBox);
+        const bool relist = replaced || c->rebuildCount != rebuildsBefore;
+        launchSnapshot(c->d, savedBuffers(c), SNAP_RESTORE | (relist ? 0 : SNAP_REBUILD), c->stream);
         writeBox(c);
-        HIP_CHECK(hipMemcpyAsync(c->d.force, c->forceSaved, 3 * np * sizeof(long long), hipMemcpyDeviceToDevice,
-                                 c->stream));
-        if (replaced || c->rebuildCount != rebuildsBefore)
-          setForceRebuildKernel<<<1, 1, 0, c->stream>>>(c->d.sc);
-        else
-          HIP_CHECK(hipMemcpyAsync(c->d.posRebuild, c->rebuildSaved, np * sizeof(float4), hipMemcpyDeviceToDevice,
-                                   c->stream));
-        HIP_CHECK(hipStreamSynchronize(c->stream));
+        if (relist) setForceRebuildKernel<<<1, 1, 0, c->stream>>>(c->d.sc);
         std::memcpy(c->blockEndTerms, oldTerms, sizeof oldTerms);
       }
     }

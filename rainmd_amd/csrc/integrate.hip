@@ -994,41 +994,69 @@ __global__ void decideRebuildBatchKernel(const BatchView *__restrict__ v, int re
 
 // Barostat (webgpu/kernels/barostat.js:25-107): mass-weighted molecule
 // centroids (geometric when massless), then x += (s - 1) * centroid.
-__device__ __forceinline__ void centroid(const Dev &d, const int *molStart, const int *molAtoms, int nMol,
-                                         float4 *centroids, int m) {
-  if (m >= nMol) return;
-  double w[3] = {0, 0, 0}, total = 0;
-  for (int k = molStart[m]; k < molStart[m + 1]; ++k) {
-    const int a = molAtoms[k];
-    const float4 p = d.pos[a];
-    const double im = d.vel[a].w;
-    const double mass = im > 0 ? 1.0 / im : 0.0;
-    w[0] += mass * p.x;
-    w[1] += mass * p.y;
-    w[2] += mass * p.z;
-    total += mass;
+//
+// One thread per molecule; a large molecule's atoms are loaded MOL_RUN at a
+// time (their loads in flight together, not one dependent round trip per
+// atom) and summed in atom order.
+constexpr int MOL_RUN = 4;
+
+// Molecule [begin, end)'s mass-weighted centroid in FP64, or its atoms'
+// mean where every mass is 0.
+__device__ __forceinline__ void moleculeCentroid(const Dev &d, const int *molAtoms, int begin, int end, double w[3]) {
+  double total = 0, u[3] = {0, 0, 0};
+  w[0] = w[1] = w[2] = 0.0;
+  for (int k0 = begin; k0 < end; k0 += MOL_RUN) {
+    float4 p[MOL_RUN];
+    float im[MOL_RUN];
+#pragma unroll
+    for (int q = 0; q < MOL_RUN; ++q) {
+      const int a = molAtoms[min(k0 + q, end - 1)];
+      p[q] = d.pos[a];
+      im[q] = d.vel[a].w;
+    }
+#pragma unroll
+    for (int q = 0; q < MOL_RUN; ++q) {
+      if (k0 + q >= end) break;
+      const double mass = im[q] > 0 ? 1.0 / (double)im[q] : 0.0;
+      w[0] += mass * p[q].x;
+      w[1] += mass * p[q].y;
+      w[2] += mass * p[q].z;
+      total += mass;
+      u[0] += p[q].x;
+      u[1] += p[q].y;
+      u[2] += p[q].z;
+    }
   }
   if (total > 0) {
     for (int c = 0; c < 3; ++c) w[c] /= total;
   } else {
-    const int count = molStart[m + 1] - molStart[m];
-    for (int k = molStart[m]; k < molStart[m + 1]; ++k) {
-      const float4 p = d.pos[molAtoms[k]];
-      w[0] += p.x;
-      w[1] += p.y;
-      w[2] += p.z;
-    }
-    for (int c = 0; c < 3; ++c) w[c] /= (count > 0 ? count : 1);
+    const int count = end - begin;
+    for (int c = 0; c < 3; ++c) w[c] = u[c] / (count > 0 ? count : 1);
   }
+}
+
+// Molecule m's centroid into centroids[m]; returns the molecule's span (its
+// atoms' largest distance from the centroid; 0 past the last molecule), which
+// the caller's workgroup reduces into Scalars::spanBits with one atomic.
+__device__ __forceinline__ float centroid(const Dev &d, const int *molStart, const int *molAtoms, int nMol,
+                                          float4 *centroids, int m) {
+  if (m >= nMol) return 0.f;
+  const int begin = molStart[m], end = molStart[m + 1];
+  double w[3];
+  moleculeCentroid(d, molAtoms, begin, end, w);
   centroids[m] = make_float4((float)w[0], (float)w[1], (float)w[2], 0.f);
-  // the molecule's span: its atoms' largest distance from the centroid
   double span2 = 0.0;
-  for (int k = molStart[m]; k < molStart[m + 1]; ++k) {
-    const float4 p = d.pos[molAtoms[k]];
-    const double ex = p.x - w[0], ey = p.y - w[1], ez = p.z - w[2];
-    span2 = fmax(span2, ex * ex + ey * ey + ez * ez);
+  for (int k0 = begin; k0 < end; k0 += MOL_RUN) {
+    float4 p[MOL_RUN];
+#pragma unroll
+    for (int q = 0; q < MOL_RUN; ++q) p[q] = d.pos[molAtoms[min(k0 + q, end - 1)]];
+#pragma unroll
+    for (int q = 0; q < MOL_RUN; ++q) {
+      const double ex = p[q].x - w[0], ey = p[q].y - w[1], ez = p[q].z - w[2];
+      span2 = fmax(span2, ex * ex + ey * ey + ez * ez);
+    }
   }
-  atomicMax(&d.sc->spanBits, __float_as_uint((float)sqrt(span2) * 1.0001f));
+  return (float)sqrt(span2) * 1.0001f;
 }
 
 // sum_i F_i . (r_i - c_mol(i)) over molecule m's atoms with the total forces
@@ -1037,36 +1065,25 @@ __device__ __forceinline__ void centroid(const Dev &d, const int *molStart, cons
 __device__ __forceinline__ double molecularTerm(const Dev &d, const int *molStart, const int *molAtoms, int nMol,
                                                 int m) {
   if (m >= nMol) return 0.0;
-  double w[3] = {0, 0, 0}, total = 0;
-  for (int k = molStart[m]; k < molStart[m + 1]; ++k) {
-    const int a = molAtoms[k];
-    const float4 p = d.pos[a];
-    const double im = d.vel[a].w;
-    const double mass = im > 0 ? 1.0 / im : 0.0;
-    w[0] += mass * p.x;
-    w[1] += mass * p.y;
-    w[2] += mass * p.z;
-    total += mass;
-  }
-  const int count = molStart[m + 1] - molStart[m];
-  if (total > 0) {
-    for (int c = 0; c < 3; ++c) w[c] /= total;
-  } else {
-    w[0] = w[1] = w[2] = 0.0;
-    for (int k = molStart[m]; k < molStart[m + 1]; ++k) {
-      const float4 p = d.pos[molAtoms[k]];
-      w[0] += p.x;
-      w[1] += p.y;
-      w[2] += p.z;
-    }
-    for (int c = 0; c < 3; ++c) w[c] /= (count > 0 ? count : 1);
-  }
+  const int begin = molStart[m], end = molStart[m + 1];
+  double w[3];
+  moleculeCentroid(d, molAtoms, begin, end, w);
   double s = 0.0;
-  for (int k = molStart[m]; k < molStart[m + 1]; ++k) {
-    const int a = molAtoms[k];
-    const float4 p = d.pos[a];
-    s += (double)d.force[a] * FORCE_UNSCALE * (p.x - w[0]) + (double)d.force[d.nPad + a] * FORCE_UNSCALE * (p.y - w[1]) +
-         (double)d.force[2 * d.nPad + a] * FORCE_UNSCALE * (p.z - w[2]);
+  for (int k0 = begin; k0 < end; k0 += MOL_RUN) {
+    float4 p[MOL_RUN];
+    long long f[MOL_RUN][3];
+#pragma unroll
+    for (int q = 0; q < MOL_RUN; ++q) {
+      const int a = molAtoms[min(k0 + q, end - 1)];
+      p[q] = d.pos[a];
+      for (int c = 0; c < 3; ++c) f[q][c] = d.force[(size_t)c * d.nPad + a];
+    }
+#pragma unroll
+    for (int q = 0; q < MOL_RUN; ++q) {
+      if (k0 + q >= end) break;
+      s += (double)f[q][0] * FORCE_UNSCALE * (p[q].x - w[0]) + (double)f[q][1] * FORCE_UNSCALE * (p[q].y - w[1]) +
+           (double)f[q][2] * FORCE_UNSCALE * (p[q].z - w[2]);
+    }
   }
   return s;
 }
@@ -1085,8 +1102,10 @@ __global__ void __launch_bounds__(256) molecularTermBatchKernel(const BatchView 
   blockSumToSlot(s, &b.d.slots[E_DSCALE_MOLECULAR * MAX_SLOTS + t.y]);
 }
 
-__global__ void centroidKernel(Dev d, const int *molStart, const int *molAtoms, int nMol, float4 *centroids) {
-  centroid(d, molStart, molAtoms, nMol, centroids, blockIdx.x * blockDim.x + threadIdx.x);
+__global__ void __launch_bounds__(256) centroidKernel(Dev d, const int *molStart, const int *molAtoms, int nMol,
+                                                      float4 *centroids) {
+  const float span = centroid(d, molStart, molAtoms, nMol, centroids, blockIdx.x * blockDim.x + threadIdx.x);
+  blockMaxToWord(span, &d.sc->spanBits);
 }
 
 __device__ __forceinline__ void scaleAtom(const Dev &d, const float4 *centroids, const int *molOfAtom, float factor,
@@ -1124,7 +1143,8 @@ __global__ void __launch_bounds__(256) centroidBatchKernel(const BatchView *__re
   const int2 t = task[blockIdx.x];
   if (t.x < 0 || v[t.x].scale == 1.0f) return;
   const BatchView &b = v[t.x];
-  centroid(b.d, b.molStart, b.molAtoms, b.nMol, b.centroids, t.y * blockDim.x + threadIdx.x);
+  const float span = centroid(b.d, b.molStart, b.molAtoms, b.nMol, b.centroids, t.y * blockDim.x + threadIdx.x);
+  blockMaxToWord(span, &b.d.sc->spanBits);
 }
 
 __global__ void __launch_bounds__(256) scaleBatchKernel(const BatchView *__restrict__ v,
@@ -1158,6 +1178,41 @@ __global__ void __launch_bounds__(256) snapshotBatchKernel(const BatchView *__re
     for (int c = 0; c < 3; ++c) b.forceSaved[c * np + a] = b.d.force[c * np + a];
   }
 }
+
+__global__ void __launch_bounds__(256) snapshotKernel(Dev d, SnapshotBuffers saved, int mode) {
+  const int a = blockIdx.x * blockDim.x + threadIdx.x;
+  const int np = d.nPad;
+  if ((mode & SNAP_SCALARS) && a == 0) {
+    if (mode & SNAP_RESTORE) {
+      *d.sc = *saved.sc;
+    } else {
+      d.sc->requiredMax = 0u;
+      *saved.sc = *d.sc;
+    }
+  }
+  if (a >= np) return;
+  if (mode & SNAP_RESTORE) {
+    d.pos[a] = saved.pos[a];
+    if (mode & SNAP_VELOCITIES) d.vel[a] = saved.vel[a];
+    if (mode & SNAP_REBUILD) d.posRebuild[a] = saved.rebuild[a];
+    for (int c = 0; c < 3; ++c) d.force[c * np + a] = saved.force[c * np + a];
+  } else {
+    saved.pos[a] = d.pos[a];
+    if (mode & SNAP_VELOCITIES) saved.vel[a] = d.vel[a];
+    if (mode & SNAP_REBUILD) saved.rebuild[a] = d.posRebuild[a];
+    for (int c = 0; c < 3; ++c) saved.force[c * np + a] = d.force[c * np + a];
+  }
+}
+
+void launchSnapshot(const Dev &d, const SnapshotBuffers &saved, int mode, hipStream_t s) {
+  snapshotKernel<<<(d.nPad + 255) / 256, 256, 0, s>>>(d, saved, mode);
+}
+
+__global__ void setBoxKernel(BoxF *dst, BoxF box) {
+  if (threadIdx.x == 0) *dst = box;
+}
+
+void launchSetBox(BoxF *dst, const BoxF &box, hipStream_t s) { setBoxKernel<<<1, 64, 0, s>>>(dst, box); }
 
 __global__ void forceRebuildFlaggedBatchKernel(const BatchView *__restrict__ v, int replicas) {
   const int r = blockIdx.x * blockDim.x + threadIdx.x;

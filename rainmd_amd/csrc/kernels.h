@@ -226,8 +226,20 @@ void launchMolecularTermBatch(const BatchView *v, const TaskTable &molecules, hi
 // Saves (or restores) positions and forces, with SNAP_VELOCITIES the
 // velocities too, of every replica (with SNAP_FLAGGED, of those whose
 // `restore` is set).
-enum { SNAP_RESTORE = 1, SNAP_VELOCITIES = 2, SNAP_FLAGGED = 4, SNAP_REBUILD = 8 };
+enum { SNAP_RESTORE = 1, SNAP_VELOCITIES = 2, SNAP_FLAGGED = 4, SNAP_REBUILD = 8, SNAP_SCALARS = 16 };
 void launchSnapshotBatch(const BatchView *v, const TaskTable &atoms, int mode, hipStream_t s);
+// A single system's save (or restore, SNAP_RESTORE) of positions and forces,
+// with SNAP_VELOCITIES the velocities, SNAP_REBUILD the rebuild positions and
+// SNAP_SCALARS the Scalars too (a save with SNAP_SCALARS clears
+// Scalars::requiredMax first), in one launch.
+struct SnapshotBuffers {
+  float4 *pos, *vel, *rebuild;
+  long long *force;
+  Scalars *sc;
+};
+void launchSnapshot(const Dev &d, const SnapshotBuffers &saved, int mode, hipStream_t s);
+// The device box from its value (a kernel argument: no host staging).
+void launchSetBox(BoxF *dst, const BoxF &box, hipStream_t s);
 // Sets Scalars::forceRebuild of the replicas whose `restore` and `rebuild` are set.
 void launchForceRebuildFlaggedBatch(const BatchView *v, int replicas, hipStream_t s);
 // Clears Scalars::requiredMax of `replicas` consecutive scalar blocks.
