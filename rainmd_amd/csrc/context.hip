@@ -238,11 +238,11 @@ void writeBox(rmd_ctx *c) {
     invalidateGraphs(c);
   }
   c->p.xReal = rect;
-  // the box travels as a kernel argument: no staging buffer, no wait
-  launchSetBox(c->boxDev, boxFor(c), c->stream);
+  // the box travels as a kernel argument of the influence function's launch,
+  // which also stores it: no staging buffer, no wait
   ewaldSelfEnergy(c->sys, c->pme.alpha, volumeOf(c->box), &c->selfEnergy, &c->netEnergy);
   const double *mods[3] = {c->dModuli[0], c->dModuli[1], c->dModuli[2]};
-  pmeComputeInfluence(c->p, c->boxDev, mods, c->pme.alpha, c->stream);
+  pmeSetBoxAndInfluence(c->p, boxFor(c), c->boxDev, mods, c->pme.alpha, c->stream);
   HIP_CHECK(hipGetLastError());
 }
 
@@ -445,18 +445,31 @@ void setupIntegrator(rmd_ctx *c, int suppliedNoise) {
 
 // ---- force evaluation ---------------------------------------------------
 
-__global__ void decideRebuildKernel(Scalars *sc, double dispHalf, int force) {
-  sc->requiredMax = 0u;  // this evaluation's requirement starts here
-  const float maxDisp2 = __uint_as_float(sc->maxDisp2Bits);
-  const bool rebuild = force || sc->forceRebuild || displacementExceeded(sc, maxDisp2, dispHalf);
-  sc->rebuild = rebuild ? 1u : 0u;
-  if (rebuild) {
-    sc->maxDisp2Bits = 0u;
-    sc->slack = 0.f;
-    retireRequirement(sc);
-    sc->rebuilds += 1u;
-    sc->forceRebuild = 0u;
+// The start of an evaluation outside the step loop, in one launch: the
+// rebuild decision (thread 0; this evaluation's list requirement starts
+// here), the forces zeroed, the slots' positions refreshed from the atoms'.
+__global__ void __launch_bounds__(256) beginEvaluationKernel(Dev d, double dispHalf, int force) {
+  const int a = blockIdx.x * blockDim.x + threadIdx.x;
+  if (a == 0) {
+    Scalars *sc = d.sc;
+    sc->requiredMax = 0u;
+    const float maxDisp2 = __uint_as_float(sc->maxDisp2Bits);
+    const bool rebuild = force || sc->forceRebuild || displacementExceeded(sc, maxDisp2, dispHalf);
+    sc->rebuild = rebuild ? 1u : 0u;
+    if (rebuild) {
+      sc->maxDisp2Bits = 0u;
+      sc->slack = 0.f;
+      retireRequirement(sc);
+      sc->rebuilds += 1u;
+      sc->forceRebuild = 0u;
+    }
   }
+  if (a < d.nPad) {
+    d.force[a] = 0;
+    d.force[d.nPad + a] = 0;
+    d.force[2 * d.nPad + a] = 0;
+  }
+  if (a < d.n) d.slotPos[a] = d.pos[d.cellAtoms[a]];
 }
 
 __global__ void setForceRebuildKernel(Scalars *sc) { sc->forceRebuild = 1u; }
@@ -539,9 +552,8 @@ void checkScalars(rmd_ctx *c, Scalars &h) {
 // reference's #evaluatePotential, which always rebuilds.
 void evaluate(rmd_ctx *c, int terms, bool forceRebuild, double terms_out[E_TERM_COUNT]) {
   for (int attempt = 0; attempt < 4; ++attempt) {
-    decideRebuildKernel<<<1, 1, 0, c->stream>>>(c->d.sc, c->ip.dispHalf, forceRebuild ? 1 : 0);
-    launchZeroForces(c->d, c->stream);
-    launchRefreshSlots(c->d, c->stream);
+    beginEvaluationKernel<<<(c->d.nPad + 255) / 256, 256, 0, c->stream>>>(c->d, c->ip.dispHalf,
+                                                                          forceRebuild ? 1 : 0);
     int slots[E_TERM_COUNT];
     enqueueForces(c, terms, terms_out ? 1 : 0, slots);
     if (terms_out) {

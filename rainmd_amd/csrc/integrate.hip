@@ -965,8 +965,6 @@ __device__ __forceinline__ void zeroForces(const Dev &d, int a) {
   d.force[2 * d.nPad + a] = 0;
 }
 
-__global__ void zeroForcesKernel(Dev d) { zeroForces(d, blockIdx.x * blockDim.x + threadIdx.x); }
-
 __global__ void __launch_bounds__(256) zeroForcesBatchKernel(const BatchView *__restrict__ v,
                                                              const int2 *__restrict__ task) {
   const int2 t = task[blockIdx.x];
@@ -1208,12 +1206,6 @@ void launchSnapshot(const Dev &d, const SnapshotBuffers &saved, int mode, hipStr
   snapshotKernel<<<(d.nPad + 255) / 256, 256, 0, s>>>(d, saved, mode);
 }
 
-__global__ void setBoxKernel(BoxF *dst, BoxF box) {
-  if (threadIdx.x == 0) *dst = box;
-}
-
-void launchSetBox(BoxF *dst, const BoxF &box, hipStream_t s) { setBoxKernel<<<1, 64, 0, s>>>(dst, box); }
-
 __global__ void forceRebuildFlaggedBatchKernel(const BatchView *__restrict__ v, int replicas) {
   const int r = blockIdx.x * blockDim.x + threadIdx.x;
   if (r < replicas && v[r].restore && v[r].rebuild) v[r].d.sc->forceRebuild = 1u;
@@ -1357,7 +1349,6 @@ void launchReduceEnergies(const Dev &d, const SlotCounts &counts, double *out, h
   reduceEnergiesKernel<<<E_TERM_COUNT, 256, 0, s>>>(d.slots, counts, out);
 }
 
-void launchZeroForces(const Dev &d, hipStream_t s) { zeroForcesKernel<<<(d.nPad + 255) / 256, 256, 0, s>>>(d); }
 
 void launchScaleMolecules(const Dev &d, const int *molStart, const int *molAtoms, int nMol, float4 *centroids,
                           const int *molOfAtom, float scale, hipStream_t s) {

@@ -13,9 +13,6 @@ namespace rmd {
 // the integrator's last block (or the host) asked for a rebuild.
 void launchRebuild(const Dev &d, hipStream_t s);
 void launchForceRebuildFlag(const Dev &d, hipStream_t s);
-// Refresh the cell-ordered position copy from the positions (evaluations
-// outside the step loop, where positions may have been set or restored).
-void launchRefreshSlots(const Dev &d, hipStream_t s);
 
 // ---- nonbonded direct space (nonbonded.hip) ----
 // lanes: lanes per atom (1, 2, 4, ..., 64). energies != 0 fills the
@@ -83,7 +80,10 @@ bool pmeBinsFit(const PmeDev &p, int atoms);
 constexpr int BIN_SINGLE_MIN_ATOMS = 8192;
 constexpr int BIN_GROUP_ATOMS = 1024;
 bool pmeBinsFitWide(const PmeDev &p);
-void pmeComputeInfluence(const PmeDev &p, const BoxF *box, const double *moduli[3], double alpha, hipStream_t s);
+// The device box set from its value and the influence function computed from
+// it, in one launch (then the folded x lines' kernel where it is kept).
+void pmeSetBoxAndInfluence(const PmeDev &p, const BoxF &box, BoxF *dst, const double *moduli[3], double alpha,
+                           hipStream_t s);
 int launchPme(const Dev &d, const PmeDev &p, int energies, hipStream_t s, int *slotsUsed);
 // A small system's direct space, bonded terms and PME forward planes in one
 // launch (where forcesJoin allows), with the energy slots each part used;
@@ -147,7 +147,6 @@ struct SlotCounts {
   int c[E_TERM_COUNT];
 };
 void launchReduceEnergies(const Dev &d, const SlotCounts &counts, double *out, hipStream_t s);
-void launchZeroForces(const Dev &d, hipStream_t s);
 // Barostat: mass-weighted molecule centroids and the scaling about them. The
 // rebuild references are scaled with the positions, and the move's share of
 // the skin is added to Scalars::slack, so the neighbour list stays valid
@@ -238,8 +237,6 @@ struct SnapshotBuffers {
   Scalars *sc;
 };
 void launchSnapshot(const Dev &d, const SnapshotBuffers &saved, int mode, hipStream_t s);
-// The device box from its value (a kernel argument: no host staging).
-void launchSetBox(BoxF *dst, const BoxF &box, hipStream_t s);
 // Sets Scalars::forceRebuild of the replicas whose `restore` and `rebuild` are set.
 void launchForceRebuildFlaggedBatch(const BatchView *v, int replicas, hipStream_t s);
 // Clears Scalars::requiredMax of `replicas` consecutive scalar blocks.
@@ -253,6 +250,9 @@ bool cellSearchFits(int atoms, int cells);
 // The search path for a search mode (0 by occupancy, 1 per atom, 2 per cell
 // where the launch covers the cells).
 bool searchByCell(int mode, int atoms, int cells);
+// Refresh every replica's cell-ordered position copy from its positions
+// (evaluations outside the step loop, where positions may have been set or
+// restored).
 void launchRefreshSlotsBatch(const BatchView *v, const TaskTable &atoms256, hipStream_t s);
 // The cell lists of the replicas due a rebuild, then their neighbour lists;
 // `queue` holds searchQueueInts(replicas) ints of device memory, `grid` is a

@@ -344,11 +344,6 @@ void launchForceRebuildFlag(const Dev &d, hipStream_t s) { setRebuildKernel<<<1,
 
 // The slot copy from the positions through the current cell order, for an
 // evaluation that did not follow an integrator step (which keeps it current).
-__global__ void refreshSlotsKernel(Dev d) {
-  const int k = blockIdx.x * blockDim.x + threadIdx.x;
-  if (k < d.n) d.slotPos[k] = d.pos[d.cellAtoms[k]];
-}
-
 __global__ void __launch_bounds__(256) refreshSlotsBatchKernel(const BatchView *__restrict__ v,
                                                                const int2 *__restrict__ task) {
   const int2 t = task[blockIdx.x];
@@ -357,8 +352,6 @@ __global__ void __launch_bounds__(256) refreshSlotsBatchKernel(const BatchView *
   const int k = t.y * blockDim.x + threadIdx.x;
   if (k < d.n) d.slotPos[k] = d.pos[d.cellAtoms[k]];
 }
-
-void launchRefreshSlots(const Dev &d, hipStream_t s) { refreshSlotsKernel<<<(d.n + 255) / 256, 256, 0, s>>>(d); }
 
 void launchRefreshSlotsBatch(const BatchView *v, const TaskTable &atoms256, hipStream_t s) {
   refreshSlotsBatchKernel<<<atoms256.blocks, 256, 0, s>>>(v, atoms256.task);

@@ -559,11 +559,6 @@ __device__ __forceinline__ void pmeInfluence(const PmeDev &p, const BoxF *box, c
   p.influence[idx] = (float)(pref * exp(-factor * m2) / (m2 * modulus));
 }
 
-__global__ void pmeInfluenceKernel(PmeDev p, const BoxF *box, const double *mod0, const double *mod1,
-                                   const double *mod2, double alpha) {
-  pmeInfluence(p, box, mod0, mod1, mod2, alpha, (size_t)blockIdx.x * blockDim.x + threadIdx.x);
-}
-
 // Batched: the replicas whose box changed.
 __global__ void __launch_bounds__(256) pmeInfluenceBatchKernel(const BatchView *__restrict__ v,
                                                                const int2 *__restrict__ task) {
@@ -600,9 +595,20 @@ __global__ void pmeXKernelKernel(PmeDev p) {
     reinterpret_cast<float2 *>(p.xKernel)[idx] = make_float2((float)re, (float)im);
 }
 
-void pmeComputeInfluence(const PmeDev &p, const BoxF *box, const double *moduli[3], double alpha, hipStream_t s) {
+// A single system's box change in one launch: the box by value, stored to
+// the device box by the first thread, and the influence function from it.
+__global__ void pmeSetBoxAndInfluenceKernel(PmeDev p, BoxF box, BoxF *dst, const double *mod0, const double *mod1,
+                                            const double *mod2, double alpha) {
+  const size_t idx = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (idx == 0) *dst = box;
+  pmeInfluence(p, &box, mod0, mod1, mod2, alpha, idx);
+}
+
+void pmeSetBoxAndInfluence(const PmeDev &p, const BoxF &box, BoxF *dst, const double *moduli[3], double alpha,
+                           hipStream_t s) {
   const size_t total = (size_t)p.K[0] * p.K[1] * p.K2h;
-  pmeInfluenceKernel<<<(unsigned)((total + 255) / 256), 256, 0, s>>>(p, box, moduli[0], moduli[1], moduli[2], alpha);
+  pmeSetBoxAndInfluenceKernel<<<(unsigned)((total + 255) / 256), 256, 0, s>>>(p, box, dst, moduli[0], moduli[1],
+                                                                              moduli[2], alpha);
   if (p.xKernel) pmeXKernelKernel<<<(unsigned)((total + 255) / 256), 256, 0, s>>>(p);
 }
 
