@@ -155,6 +155,14 @@ struct rmd_ctx {
   // evaluation.
   double *dTermsBlock = nullptr;
   double blockEndTerms[E_TERM_COUNT] = {0};
+  // A volume move's trial appended to the block before it (rmd_run): the
+  // proposal on the device and its pinned staging copy, the state the trial
+  // starts from, and the trial's energy terms (allocated at the first such
+  // move).
+  TrialParams *trialDev = nullptr, *trialHost = nullptr;
+  float4 *trialPos = nullptr;
+  long long *trialForce = nullptr;
+  double *dTermsTrial = nullptr;
   int64_t blockEndStep = -1;
   unsigned int *dMaxForce = nullptr;
   int nUnits = 0, waterUnits = 0, clusterUnits = 0, freeUnits = 0;
@@ -197,33 +205,35 @@ void invalidateGraphs(rmd_ctx *c) {
 
 // Write the device box: f32 rows and inverse widths for the pair kernels,
 // FP64 rows for PME and restraints, the cell grid shape.
-BoxF boxFor(const rmd_ctx *c) {
+BoxF boxFor(const rmd_ctx *c, const double *box, const CellPlan &plan) {
   BoxF b{};
   double recip[9];
-  reciprocalBoxVectors(c->box, recip);
+  reciprocalBoxVectors(box, recip);
   for (int k = 0; k < 3; ++k) {
-    b.a[k] = (float)c->box[k];
-    b.b[k] = (float)c->box[3 + k];
-    b.c[k] = (float)c->box[6 + k];
+    b.a[k] = (float)box[k];
+    b.b[k] = (float)box[3 + k];
+    b.c[k] = (float)box[6 + k];
   }
-  b.inv[0] = (float)(1.0 / c->box[0]);
-  b.inv[1] = (float)(1.0 / c->box[4]);
-  b.inv[2] = (float)(1.0 / c->box[8]);
+  b.inv[0] = (float)(1.0 / box[0]);
+  b.inv[1] = (float)(1.0 / box[4]);
+  b.inv[2] = (float)(1.0 / box[8]);
   for (int k = 0; k < 9; ++k) {
     b.recip[k] = (float)recip[k];
     b.recip64[k] = recip[k];
-    b.box64[k] = c->box[k];
+    b.box64[k] = box[k];
   }
   b.cutoff = (float)c->sys.cutoff;
   b.rc2 = (float)(c->sys.cutoff * c->sys.cutoff);
-  b.rl2 = (float)(c->plan.listCutoff * c->plan.listCutoff);
+  b.rl2 = (float)(plan.listCutoff * plan.listCutoff);
   b.periodic = 1;
-  for (int k = 0; k < 3; ++k) b.cells[k] = c->plan.counts[k];
-  b.cellTotal = c->plan.cellCount;
-  b.reach = c->plan.reach;
-  b.cellSearch = searchByCell(c->searchMode, c->d.n, c->plan.cellCount) ? 1 : 0;
+  for (int k = 0; k < 3; ++k) b.cells[k] = plan.counts[k];
+  b.cellTotal = plan.cellCount;
+  b.reach = plan.reach;
+  b.cellSearch = searchByCell(c->searchMode, c->d.n, plan.cellCount) ? 1 : 0;
   return b;
 }
+
+BoxF boxFor(const rmd_ctx *c) { return boxFor(c, c->box, c->plan); }
 
 bool rectangular(const double *box) {
   return box[1] == 0.0 && box[2] == 0.0 && box[3] == 0.0 && box[5] == 0.0 && box[6] == 0.0 && box[7] == 0.0;
@@ -447,12 +457,14 @@ void setupIntegrator(rmd_ctx *c, int suppliedNoise) {
 
 // The start of an evaluation outside the step loop, in one launch: the
 // rebuild decision (thread 0; this evaluation's list requirement starts
-// here), the forces zeroed, the slots' positions refreshed from the atoms'.
-__global__ void __launch_bounds__(256) beginEvaluationKernel(Dev d, double dispHalf, int force) {
+// here, unless `keepRequirement`: a trial appended to a block is checked with
+// the block), the forces zeroed, the slots' positions refreshed from the
+// atoms'.
+__global__ void __launch_bounds__(256) beginEvaluationKernel(Dev d, double dispHalf, int force, int keepRequirement) {
   const int a = blockIdx.x * blockDim.x + threadIdx.x;
   if (a == 0) {
     Scalars *sc = d.sc;
-    sc->requiredMax = 0u;
+    if (!keepRequirement) sc->requiredMax = 0u;
     const float maxDisp2 = __uint_as_float(sc->maxDisp2Bits);
     const bool rebuild = force || sc->forceRebuild || displacementExceeded(sc, maxDisp2, dispHalf);
     sc->rebuild = rebuild ? 1u : 0u;
@@ -553,7 +565,7 @@ void checkScalars(rmd_ctx *c, Scalars &h) {
 void evaluate(rmd_ctx *c, int terms, bool forceRebuild, double terms_out[E_TERM_COUNT]) {
   for (int attempt = 0; attempt < 4; ++attempt) {
     beginEvaluationKernel<<<(c->d.nPad + 255) / 256, 256, 0, c->stream>>>(c->d, c->ip.dispHalf,
-                                                                          forceRebuild ? 1 : 0);
+                                                                          forceRebuild ? 1 : 0, 0);
     int slots[E_TERM_COUNT];
     enqueueForces(c, terms, terms_out ? 1 : 0, slots);
     if (terms_out) {
@@ -698,7 +710,28 @@ double evaluatePotential(rmd_ctx *c) {
 
 // ---- the block loop ---------------------------------------------------------
 
-void enqueueBlock(rmd_ctx *c, int steps) {
+// A volume move's trial after a block, as the barostat's own trial runs it
+// (rmd_attempt_volume_change: the state saved, molecules scaled about their
+// centroids, the box and the influence function, one evaluation with
+// energies), from the proposal in trialDev; its energies go to dTermsTrial
+// and its list requirement is checked with the block's.
+void enqueueTrial(rmd_ctx *c) {
+  const SnapshotBuffers from{c->trialPos, nullptr, c->rebuildSaved, c->trialForce, nullptr,
+                             &c->trialDev->rebuildsBefore};
+  launchSnapshot(c->d, from, SNAP_REBUILD, c->stream);
+  launchScaleMoleculesFrom(c->d, c->molStart, c->molAtoms, c->sys.topology.moleculeCount, c->centroids,
+                           c->molOfAtom, &c->trialDev->scale, c->stream);
+  const double *mods[3] = {c->dModuli[0], c->dModuli[1], c->dModuli[2]};
+  pmeSetBoxAndInfluenceFrom(c->p, &c->trialDev->box, c->boxDev, mods, c->pme.alpha, c->stream);
+  beginEvaluationKernel<<<(c->d.nPad + 255) / 256, 256, 0, c->stream>>>(c->d, c->ip.dispHalf, 0, 1);
+  int slots[E_TERM_COUNT];
+  enqueueForces(c, RMD_TERM_ALL, 1, slots);
+  SlotCounts counts;
+  std::memcpy(counts.c, slots, sizeof slots);
+  launchReduceEnergies(c->d, counts, c->dTermsTrial, c->stream);
+}
+
+void enqueueBlock(rmd_ctx *c, int steps, bool trial) {
   SlotCounts counts{};
   for (int s = 0; s < steps; ++s) {
     const bool last = s == steps - 1;
@@ -714,24 +747,30 @@ void enqueueBlock(rmd_ctx *c, int steps) {
   // The last step's energies, reduced in the same launch sequence.
   launchReduceEnergies(c->d, counts, c->dTermsBlock, c->stream);
   launchIntegrateClose(c->d, c->ip, 0, c->stream);
+  if (trial) enqueueTrial(c);
 }
 
-void launchBlock(rmd_ctx *c, int steps) {
+// Block lengths up to this many steps key the captured graphs; a block with
+// a trial appended is another graph of the same length.
+constexpr int GRAPH_TRIAL_KEY = 1 << 24;
+
+void launchBlock(rmd_ctx *c, int steps, bool trial) {
   if (!c->useGraphs) {
-    enqueueBlock(c, steps);
+    enqueueBlock(c, steps, trial);
     HIP_CHECK(hipGetLastError());
     return;
   }
-  auto it = c->graphs.find(steps);
+  const int key = steps + (trial ? GRAPH_TRIAL_KEY : 0);
+  auto it = c->graphs.find(key);
   if (it == c->graphs.end()) {
     hipGraph_t graph;
     HIP_CHECK(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
-    enqueueBlock(c, steps);
+    enqueueBlock(c, steps, trial);
     HIP_CHECK(hipStreamEndCapture(c->stream, &graph));
     hipGraphExec_t exec;
     HIP_CHECK(hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0));
     HIP_CHECK(hipGraphDestroy(graph));
-    it = c->graphs.emplace(steps, exec).first;
+    it = c->graphs.emplace(key, exec).first;
   }
   HIP_CHECK(hipGraphLaunch(it->second, c->stream));
 }
@@ -751,12 +790,25 @@ void restore(rmd_ctx *c) {
   launchSnapshot(c->d, savedBuffers(c), SNAP_RESTORE | SNAP_VELOCITIES | SNAP_SCALARS, c->stream);
 }
 
-void runBlock(rmd_ctx *c, int steps) {
+// With `trial`, the proposal staged in trialHost goes to the device first and
+// the block runs with the trial appended: trialTerms and rebuildsBefore (the
+// list's rebuild count when the trial started) come back with the block's
+// scalars.
+void runBlock(rmd_ctx *c, int steps, bool trial = false, double *trialTerms = nullptr,
+              unsigned int *rebuildsBefore = nullptr) {
+  if (trial)
+    HIP_CHECK(hipMemcpyAsync(c->trialDev, c->trialHost, sizeof(TrialParams), hipMemcpyHostToDevice, c->stream));
   for (int attempt = 0; attempt < 4; ++attempt) {
     snapshot(c);
-    launchBlock(c, steps);
+    launchBlock(c, steps, trial);
     HIP_CHECK(hipMemcpyAsync(c->blockEndTerms, c->dTermsBlock, sizeof c->blockEndTerms, hipMemcpyDeviceToHost,
                              c->stream));
+    if (trial) {
+      HIP_CHECK(hipMemcpyAsync(trialTerms, c->dTermsTrial, sizeof(double) * E_TERM_COUNT, hipMemcpyDeviceToHost,
+                               c->stream));
+      HIP_CHECK(hipMemcpyAsync(rebuildsBefore, &c->trialDev->rebuildsBefore, sizeof(unsigned int),
+                               hipMemcpyDeviceToHost, c->stream));
+    }
     Scalars h;
     checkScalars(c, h);
     if (h.nonFinite || std::isnan(__builtin_bit_cast(float, h.maxDisp2Bits)))
@@ -771,6 +823,7 @@ void runBlock(rmd_ctx *c, int steps) {
       // The list overflowed somewhere in the block: grow it and run the
       // block again from its start, so no step ever used a truncated list.
       restore(c);
+      if (trial) writeBox(c);  // the trial's box back to the block's
       const int grown = std::max(c->d.cap + 32, (int)std::ceil(need * 1.25));
       allocateNeighbours(c, grown);
       ++c->neighbourGrowths;
@@ -1326,6 +1379,7 @@ void destroyContext(rmd_ctx *c) {
   if (c->screen.net) rmd_surrogate_free(c->screen.net);
   (void)hipSetDevice(c->device);
   for (void *p : c->held) (void)hipFree(p);
+  if (c->trialHost) (void)hipHostFree(c->trialHost);
   if (c->ownsStreams) {
     for (int k = 0; k < 2; ++k) {
       if (c->side[k]) (void)hipStreamDestroy(c->side[k]);
@@ -1376,6 +1430,114 @@ int rmd_request_stop(rmd_ctx *c) {
 
 int rmd_attempt_volume_change(rmd_ctx *c, rmd_move *m);
 
+// ---- volume moves appended to the block before them -------------------------
+//
+// rmd_run's volume move comes right after a block, and everything about its
+// proposal is known before that block runs: the draw (a counter hash of the
+// attempt number), the window, and so the scale factor and the trial box. So
+// the trial (enqueueTrial) runs in the block's own launch sequence and its
+// energies come back with the block's, one wait instead of two and none of
+// the trial's launches from the host. The host then takes the plain
+// barostat's decision with rmd_attempt_volume_change's arithmetic, and
+// restores the state if the move is turned down; positions, box, energies and
+// counters are those of that function bit for bit. Moves the screen decides,
+// and proposals that are refused or would grow the cell or list buffers, take
+// the host path.
+struct FusedMove {
+  double volume = 0, deltaVolume = 0, newVolume = 0;
+  double box[9] = {0};
+};
+
+SnapshotBuffers trialBuffers(const rmd_ctx *c) {
+  return SnapshotBuffers{c->trialPos, nullptr, c->rebuildSaved, c->trialForce, nullptr, nullptr};
+}
+
+bool proposeFusedMove(rmd_ctx *c, FusedMove &mv) {
+  if (!c->hasPressure || c->screen.options.enabled) return false;
+  const double volume = volumeOf(c->box);
+  const double window = c->volumeScale < 0 ? 0.01 * volume : c->volumeScale;
+  const uint32_t bseed = c->seed ^ 0x5a17u;
+  const double draw = uniformAt(bseed, (uint32_t)(c->barostatAttempts + 1), 0, 0);
+  const double deltaVolume = window * 2 * (draw - 0.5);
+  const double newVolume = volume + deltaVolume;
+  if (!(newVolume > 0)) return false;
+  const double s = std::cbrt(newVolume / volume);
+  const double smallest = s * std::min({std::fabs(c->box[0]), std::fabs(c->box[4]), std::fabs(c->box[8])});
+  if (c->sys.cutoff > 0.5 * smallest) return false;
+  double nb[9];
+  for (int k = 0; k < 9; ++k) nb[k] = c->box[k] * s;
+  // the trial box's plan as updateBoxHost will make it, without buffer growth
+  const CellPlan estimate = planCellList(c->sys, c->plan.skin, c->slack, 0, nb, c->reachMax);
+  const int capacity = std::max(c->d.cap, estimate.maxNeighbours);
+  const CellPlan plan = planCellList(c->sys, c->plan.skin, c->slack, capacity, nb, c->reachMax);
+  if (capacity > c->d.cap || plan.cellCount > c->d.cellCapacity) return false;
+  if (!c->trialDev) {
+    c->trialDev = devAlloc<TrialParams>(1, c->held);
+    c->trialPos = devAlloc<float4>(c->d.nPad, c->held);
+    c->trialForce = devAlloc<long long>(3 * (size_t)c->d.nPad, c->held);
+    c->dTermsTrial = devAlloc<double>(E_TERM_COUNT, c->held);
+    void *host = nullptr;
+    HIP_CHECK(hipHostMalloc(&host, sizeof(TrialParams), hipHostMallocDefault));
+    c->trialHost = static_cast<TrialParams *>(host);
+  }
+  c->trialHost->box = boxFor(c, nb, plan);
+  c->trialHost->scale = (float)s;
+  c->trialHost->rebuildsBefore = 0u;
+  mv.volume = volume;
+  mv.deltaVolume = deltaVolume;
+  mv.newVolume = newVolume;
+  std::memcpy(mv.box, nb, sizeof nb);
+  return true;
+}
+
+// Back to the block's end state (a move that is not taken after all).
+void undoFusedTrial(rmd_ctx *c, unsigned int rebuildsBefore) {
+  const bool relist = c->rebuildCount != rebuildsBefore;
+  launchSnapshot(c->d, trialBuffers(c), SNAP_RESTORE | (relist ? 0 : SNAP_REBUILD), c->stream);
+  writeBox(c);
+  if (relist) setForceRebuildKernel<<<1, 1, 0, c->stream>>>(c->d.sc);
+}
+
+void finishFusedMove(rmd_ctx *c, const FusedMove &mv, const double *trialTerms, unsigned int rebuildsBefore) {
+  const double volume = mv.volume;
+  if (c->volumeScale < 0) c->volumeScale = 0.01 * volume;
+  const double kT = BOLTZMANN * c->temperature;
+  const double pressure = c->pressure * BAR_TO_KJ_PER_MOL_NM3;
+  double oldTerms[E_TERM_COUNT];
+  std::memcpy(oldTerms, c->blockEndTerms, sizeof oldTerms);
+  const double oldPotential = potentialFromTerms(c, oldTerms);
+  double oldBox[9];
+  std::memcpy(oldBox, c->box, sizeof oldBox);
+  c->barostatAttempts += 1;
+  c->windowAttempts += 1;
+  const uint32_t bseed = c->seed ^ 0x5a17u;
+  const double lnRatio = std::log(mv.newVolume / volume);
+  const double work = pressure * mv.deltaVolume - c->sys.topology.moleculeCount * kT * lnRatio;
+  (void)updateBoxHost(c, mv.box);  // the trial box's plan and self energy, as the trial itself had them
+  const double newPotential = potentialFromTerms(c, trialTerms);
+  const double weight = (newPotential - oldPotential) + work;
+  const bool accept =
+      weight <= 0 || uniformAt(bseed, (uint32_t)c->barostatAttempts, 1, 0) < std::exp(-weight / kT);
+  if (accept) {
+    std::memcpy(c->blockEndTerms, trialTerms, sizeof c->blockEndTerms);
+    c->barostatAccepted += 1;
+    c->windowAccepted += 1;
+  } else {
+    (void)updateBoxHost(c, oldBox);
+    undoFusedTrial(c, rebuildsBefore);
+    std::memcpy(c->blockEndTerms, oldTerms, sizeof oldTerms);
+  }
+  c->blockEndStep = c->step;
+  if (c->windowAttempts >= 10) {
+    const double rate = (double)c->windowAccepted / c->windowAttempts;
+    if (rate < 0.25)
+      c->volumeScale /= 1.1;
+    else if (rate > 0.75)
+      c->volumeScale = std::min(c->volumeScale * 1.1, 0.3 * volume);
+    c->windowAttempts = c->windowAccepted = 0;
+  }
+}
+
 int rmd_run(rmd_ctx *c, int64_t steps, int32_t block, int64_t *stepOut) {
   CTX_GUARD(c, {
     if (!c->initialised) {
@@ -1393,9 +1555,24 @@ int rmd_run(rmd_ctx *c, int64_t steps, int32_t block, int64_t *stepOut) {
         c->stepsSinceBarostat = 0;
       }
       const int n = (int)std::min<int64_t>(blockSize, remaining);
-      runBlock(c, n);
+      // the move due after this block (there are steps left to run) rides
+      // in the block's launch sequence where it can
+      FusedMove mv;
+      const bool fused = remaining > n && c->hasPressure && c->stepsSinceBarostat + n >= c->barostatInterval &&
+                         proposeFusedMove(c, mv);
+      double trialTerms[E_TERM_COUNT];
+      unsigned int rebuildsBefore = 0;
+      runBlock(c, n, fused, trialTerms, &rebuildsBefore);
       remaining -= n;
       c->stepsSinceBarostat += n;
+      if (fused) {
+        if (c->stop.load()) {
+          undoFusedTrial(c, rebuildsBefore);  // stopped before the move, as without the trial
+        } else {
+          finishFusedMove(c, mv, trialTerms, rebuildsBefore);
+          c->stepsSinceBarostat = 0;
+        }
+      }
     }
     if (stepOut) *stepOut = c->step;
   });

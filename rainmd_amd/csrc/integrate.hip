@@ -1180,6 +1180,7 @@ __global__ void __launch_bounds__(256) snapshotBatchKernel(const BatchView *__re
 __global__ void __launch_bounds__(256) snapshotKernel(Dev d, SnapshotBuffers saved, int mode) {
   const int a = blockIdx.x * blockDim.x + threadIdx.x;
   const int np = d.nPad;
+  if (saved.rebuilds && !(mode & SNAP_RESTORE) && a == 0) *saved.rebuilds = d.sc->rebuilds;
   if ((mode & SNAP_SCALARS) && a == 0) {
     if (mode & SNAP_RESTORE) {
       *d.sc = *saved.sc;
@@ -1354,6 +1355,16 @@ void launchScaleMolecules(const Dev &d, const int *molStart, const int *molAtoms
                           const int *molOfAtom, float scale, hipStream_t s) {
   centroidKernel<<<(nMol + 255) / 256, 256, 0, s>>>(d, molStart, molAtoms, nMol, centroids);
   scaleKernel<<<(d.n + 255) / 256, 256, 0, s>>>(d, centroids, molOfAtom, scale);
+}
+
+__global__ void scaleFromKernel(Dev d, const float4 *centroids, const int *molOfAtom, const float *scale) {
+  scaleAtom(d, centroids, molOfAtom, *scale, blockIdx.x * blockDim.x + threadIdx.x);
+}
+
+void launchScaleMoleculesFrom(const Dev &d, const int *molStart, const int *molAtoms, int nMol, float4 *centroids,
+                              const int *molOfAtom, const float *scale, hipStream_t s) {
+  centroidKernel<<<(nMol + 255) / 256, 256, 0, s>>>(d, molStart, molAtoms, nMol, centroids);
+  scaleFromKernel<<<(d.n + 255) / 256, 256, 0, s>>>(d, centroids, molOfAtom, scale);
 }
 
 void launchMaxForce(const Dev &d, unsigned int *out, hipStream_t s) {

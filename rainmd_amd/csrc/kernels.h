@@ -84,6 +84,9 @@ bool pmeBinsFitWide(const PmeDev &p);
 // it, in one launch (then the folded x lines' kernel where it is kept).
 void pmeSetBoxAndInfluence(const PmeDev &p, const BoxF &box, BoxF *dst, const double *moduli[3], double alpha,
                            hipStream_t s);
+// The same from a box in device memory.
+void pmeSetBoxAndInfluenceFrom(const PmeDev &p, const BoxF *box, BoxF *dst, const double *moduli[3], double alpha,
+                               hipStream_t s);
 int launchPme(const Dev &d, const PmeDev &p, int energies, hipStream_t s, int *slotsUsed);
 // A small system's direct space, bonded terms and PME forward planes in one
 // launch (where forcesJoin allows), with the energy slots each part used;
@@ -235,8 +238,20 @@ struct SnapshotBuffers {
   float4 *pos, *vel, *rebuild;
   long long *force;
   Scalars *sc;
+  unsigned int *rebuilds;  // saving, if set: Scalars::rebuilds stored here
 };
 void launchSnapshot(const Dev &d, const SnapshotBuffers &saved, int mode, hipStream_t s);
+// A volume move's trial appended to a block: what the host proposes before
+// the block runs (the trial box and the scale factor), and the list's rebuild
+// count when the trial starts.
+struct TrialParams {
+  BoxF box;
+  float scale;
+  unsigned int rebuildsBefore;
+};
+// The barostat's scaling with the factor read from device memory.
+void launchScaleMoleculesFrom(const Dev &d, const int *molStart, const int *molAtoms, int nMol, float4 *centroids,
+                              const int *molOfAtom, const float *scale, hipStream_t s);
 // Sets Scalars::forceRebuild of the replicas whose `restore` and `rebuild` are set.
 void launchForceRebuildFlaggedBatch(const BatchView *v, int replicas, hipStream_t s);
 // Clears Scalars::requiredMax of `replicas` consecutive scalar blocks.

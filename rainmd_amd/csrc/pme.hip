@@ -604,6 +604,21 @@ __global__ void pmeSetBoxAndInfluenceKernel(PmeDev p, BoxF box, BoxF *dst, const
   pmeInfluence(p, &box, mod0, mod1, mod2, alpha, idx);
 }
 
+__global__ void pmeSetBoxAndInfluenceFromKernel(PmeDev p, const BoxF *box, BoxF *dst, const double *mod0,
+                                                const double *mod1, const double *mod2, double alpha) {
+  const size_t idx = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (idx == 0) *dst = *box;
+  pmeInfluence(p, box, mod0, mod1, mod2, alpha, idx);
+}
+
+void pmeSetBoxAndInfluenceFrom(const PmeDev &p, const BoxF *box, BoxF *dst, const double *moduli[3], double alpha,
+                               hipStream_t s) {
+  const size_t total = (size_t)p.K[0] * p.K[1] * p.K2h;
+  pmeSetBoxAndInfluenceFromKernel<<<(unsigned)((total + 255) / 256), 256, 0, s>>>(p, box, dst, moduli[0], moduli[1],
+                                                                                  moduli[2], alpha);
+  if (p.xKernel) pmeXKernelKernel<<<(unsigned)((total + 255) / 256), 256, 0, s>>>(p);
+}
+
 void pmeSetBoxAndInfluence(const PmeDev &p, const BoxF &box, BoxF *dst, const double *moduli[3], double alpha,
                            hipStream_t s) {
   const size_t total = (size_t)p.K[0] * p.K[1] * p.K2h;
