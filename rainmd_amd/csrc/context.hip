@@ -104,6 +104,17 @@ struct ScreenState {
   double trainingLoss = 0, residual2 = 0, estimate2 = 0;
 };
 
+// What a block (and a trial appended to it) hands back to the host: the
+// Scalars, the last step's energy terms, the trial's terms and the list's
+// rebuild count when the trial started, gathered on the device at the end of
+// the block's launch sequence so that one copy brings them over.
+struct ReadBack {
+  Scalars sc;
+  double block[E_TERM_COUNT];
+  double trial[E_TERM_COUNT];
+  unsigned int rebuildsBefore;
+};
+
 struct rmd_ctx {
   System sys;
   // A member of a replica batch: the batch owns it (rmd_destroy leaves it
@@ -163,6 +174,8 @@ struct rmd_ctx {
   float4 *trialPos = nullptr;
   long long *trialForce = nullptr;
   double *dTermsTrial = nullptr;
+  // what a block hands back, gathered on the device for one copy
+  ReadBack *readBack = nullptr;
   int64_t blockEndStep = -1;
   unsigned int *dMaxForce = nullptr;
   int nUnits = 0, waterUnits = 0, clusterUnits = 0, freeUnits = 0;
@@ -197,6 +210,17 @@ struct rmd_ctx {
 namespace {
 
 double volumeOf(const double *b) { return std::fabs(b[0] * b[4] * b[8]); }
+
+__global__ void gatherReadBackKernel(const Scalars *sc, const double *block, const double *trial,
+                                     const unsigned int *rebuildsBefore, ReadBack *out) {
+  const int t = threadIdx.x;
+  if (t == 0) out->sc = *sc;
+  if (t < E_TERM_COUNT) {
+    out->block[t] = block[t];
+    if (trial) out->trial[t] = trial[t];
+  }
+  if (t == 0 && rebuildsBefore) out->rebuildsBefore = *rebuildsBefore;
+}
 
 void invalidateGraphs(rmd_ctx *c) {
   for (auto &g : c->graphs) (void)hipGraphExecDestroy(g.second);
@@ -552,12 +576,15 @@ void enqueueForces(rmd_ctx *c, int terms, int energies, int slots[E_TERM_COUNT])
         launchMolecularTerm(d, c->molStart, c->molAtoms, c->sys.topology.moleculeCount, c->stream);
 }
 
+[[noreturn]] void cellOverflowError() {
+  throw Error(RMD_E_INVALID, "a cell held more atoms than the cell sort supports (4096); the box is too dense "
+                             "or too small for this cutoff");
+}
+
 void checkScalars(rmd_ctx *c, Scalars &h) {
   HIP_CHECK(hipMemcpyAsync(&h, c->d.sc, sizeof h, hipMemcpyDeviceToHost, c->stream));
   HIP_CHECK(hipStreamSynchronize(c->stream));
-  if (h.cellOverflow)
-    throw Error(RMD_E_INVALID, "a cell held more atoms than the cell sort supports (4096); the box is too dense "
-                               "or too small for this cutoff");
+  if (h.cellOverflow) cellOverflowError();
 }
 
 // One evaluation at the current positions. force_rebuild mirrors the
@@ -748,6 +775,9 @@ void enqueueBlock(rmd_ctx *c, int steps, bool trial) {
   launchReduceEnergies(c->d, counts, c->dTermsBlock, c->stream);
   launchIntegrateClose(c->d, c->ip, 0, c->stream);
   if (trial) enqueueTrial(c);
+  static_assert(E_TERM_COUNT <= 64, "one thread per energy term");
+  gatherReadBackKernel<<<1, 64, 0, c->stream>>>(c->d.sc, c->dTermsBlock, trial ? c->dTermsTrial : nullptr,
+                                                trial ? &c->trialDev->rebuildsBefore : nullptr, c->readBack);
 }
 
 // Block lengths up to this many steps key the captured graphs; a block with
@@ -801,16 +831,16 @@ void runBlock(rmd_ctx *c, int steps, bool trial = false, double *trialTerms = nu
   for (int attempt = 0; attempt < 4; ++attempt) {
     snapshot(c);
     launchBlock(c, steps, trial);
-    HIP_CHECK(hipMemcpyAsync(c->blockEndTerms, c->dTermsBlock, sizeof c->blockEndTerms, hipMemcpyDeviceToHost,
-                             c->stream));
+    ReadBack back;
+    HIP_CHECK(hipMemcpyAsync(&back, c->readBack, sizeof back, hipMemcpyDeviceToHost, c->stream));
+    HIP_CHECK(hipStreamSynchronize(c->stream));
+    const Scalars &h = back.sc;
+    if (h.cellOverflow) cellOverflowError();
+    std::memcpy(c->blockEndTerms, back.block, sizeof c->blockEndTerms);
     if (trial) {
-      HIP_CHECK(hipMemcpyAsync(trialTerms, c->dTermsTrial, sizeof(double) * E_TERM_COUNT, hipMemcpyDeviceToHost,
-                               c->stream));
-      HIP_CHECK(hipMemcpyAsync(rebuildsBefore, &c->trialDev->rebuildsBefore, sizeof(unsigned int),
-                               hipMemcpyDeviceToHost, c->stream));
+      std::memcpy(trialTerms, back.trial, sizeof(double) * E_TERM_COUNT);
+      *rebuildsBefore = back.rebuildsBefore;
     }
-    Scalars h;
-    checkScalars(c, h);
     if (h.nonFinite || std::isnan(__builtin_bit_cast(float, h.maxDisp2Bits)))
       throw Error(RMD_E_NONFINITE, "a coordinate became non-finite in the block of " + std::to_string(steps) +
                                        " steps that ended at step " + std::to_string(c->step + steps) +
@@ -1187,6 +1217,7 @@ void create(rmd_ctx *c, const System &sys, const rmd_options &o, const Streams *
   c->scSaved = devAlloc<Scalars>(1, c->held);
   c->dTerms = devAlloc<double>(E_TERM_COUNT, c->held);
   c->dTermsBlock = devAlloc<double>(E_TERM_COUNT, c->held);
+  c->readBack = devAlloc<ReadBack>(1, c->held);
   c->dMaxForce = devAlloc<unsigned int>(1, c->held);
   // Molecules for the barostat.
   c->molStart = devAlloc<int>(sys.moleculeStart.size(), c->held);

@@ -579,13 +579,19 @@ __global__ void pmeXKernelKernel(PmeDev p) {
   const size_t idx = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (idx >= (size_t)K0 * PK) return;
   const int l = (int)(idx / PK), c = (int)(idx - (size_t)l * PK);
+  // the column's influence values loaded together (K0 <= X_FOLD_MAX_K), not
+  // one dependent round trip per term
+  float g[X_FOLD_MAX_K];
+#pragma unroll
+  for (int k = 0; k < X_FOLD_MAX_K; ++k) g[k] = k < K0 ? p.influence[(size_t)k * PK + c] : 0.f;
   double re = 0.0, im = 0.0;
   int r = 0;  // k l mod K0
-  for (int k = 0; k < K0; ++k) {
-    const double g = 2.0 * (double)p.influence[(size_t)k * PK + c];
+#pragma unroll
+  for (int k = 0; k < X_FOLD_MAX_K; ++k) {
+    if (k >= K0) break;
     const float2 w = p.twiddle[0][r];
-    re += g * (double)w.x;
-    im -= g * (double)w.y;
+    re += 2.0 * (double)g[k] * (double)w.x;
+    im -= 2.0 * (double)g[k] * (double)w.y;
     r += l;
     r -= r >= K0 ? K0 : 0;
   }
