@@ -521,7 +521,7 @@ __global__ void setStepKernel(Scalars *sc, long long step) { sc->step = step; }
 // streams forked after the integrator and joined before anything that reads
 // the forces. Forces are int64 atomics and every energy has its own slot row,
 // so the results do not depend on the overlap.
-void enqueueForces(rmd_ctx *c, int terms, int energies, int slots[E_TERM_COUNT]) {
+void enqueueForces(rmd_ctx *c, int terms, int energies, int slots[E_TERM_COUNT], bool molecular = true) {
   Dev &d = c->d;
   for (int k = 0; k < E_TERM_COUNT; ++k) slots[k] = 0;
   const int bondedMask = RMD_TERM_BONDED | RMD_TERM_EXCEPTIONS | RMD_TERM_RESTRAINT;
@@ -571,7 +571,7 @@ void enqueueForces(rmd_ctx *c, int terms, int energies, int slots[E_TERM_COUNT])
       }
     }
   }
-  if (energies)
+  if (energies && molecular)
     slots[E_DSCALE_MOLECULAR] =
         launchMolecularTerm(d, c->molStart, c->molAtoms, c->sys.topology.moleculeCount, c->stream);
 }
@@ -737,6 +737,12 @@ double evaluatePotential(rmd_ctx *c) {
 
 // ---- the block loop ---------------------------------------------------------
 
+// Whether the energies a block ends with (and its appended trial's) include
+// the molecular scaling term: only the barostat's screen reads it from them
+// (its features; a batch's screen may read a member's), while rmd_energy and
+// rmd_evaluate evaluate afresh.
+bool blockEndMolecular(const rmd_ctx *c) { return c->screen.options.enabled || c->batchMember; }
+
 // A volume move's trial after a block, as the barostat's own trial runs it
 // (rmd_attempt_volume_change: the state saved, molecules scaled about their
 // centroids, the box and the influence function, one evaluation with
@@ -752,19 +758,20 @@ void enqueueTrial(rmd_ctx *c) {
   pmeSetBoxAndInfluenceFrom(c->p, &c->trialDev->box, c->boxDev, mods, c->pme.alpha, c->stream);
   beginEvaluationKernel<<<(c->d.nPad + 255) / 256, 256, 0, c->stream>>>(c->d, c->ip.dispHalf, 0, 1);
   int slots[E_TERM_COUNT];
-  enqueueForces(c, RMD_TERM_ALL, 1, slots);
+  enqueueForces(c, RMD_TERM_ALL, 1, slots, blockEndMolecular(c));
   SlotCounts counts;
   std::memcpy(counts.c, slots, sizeof slots);
   launchReduceEnergies(c->d, counts, c->dTermsTrial, c->stream);
 }
 
 void enqueueBlock(rmd_ctx *c, int steps, bool trial) {
+  const bool molecular = blockEndMolecular(c);
   SlotCounts counts{};
   for (int s = 0; s < steps; ++s) {
     const bool last = s == steps - 1;
     const int openBlocks = launchIntegrateOpen(c->d, c->ip, s > 0 ? 1 : 0, last ? 1 : 0, c->stream);
     int slots[E_TERM_COUNT];
-    enqueueForces(c, RMD_TERM_ALL, last ? 1 : 0, slots);
+    enqueueForces(c, RMD_TERM_ALL, last ? 1 : 0, slots, molecular);
     if (last) {
       std::memcpy(counts.c, slots, sizeof slots);
       counts.c[E_KINETIC] = 0;
@@ -2172,6 +2179,7 @@ int rmd_screen_configure(rmd_ctx *c, const rmd_screen_options *options) {
     if (sc.net) rmd_surrogate_free(sc.net);
     sc = ScreenState();
     sc.options = o;
+    invalidateGraphs(c);  // the blocks' end energies change with the screen (blockEndMolecular)
     if (o.enabled) {
       rmd_surrogate *net = nullptr;
       const int st = rmd_surrogate_create(c->device, o.seed, &net);
