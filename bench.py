@@ -317,7 +317,9 @@ def replica_batch(args, count, warmup, steps, single_steps, screen=False):
         sim = batch.replica(r)
         lanes = sim.diagnostics()["lanes_per_atom"]
         sim.set_lanes(R.tuned_shape("single", atoms[r])["lanes_per_atom"])
-        sim.run(args.block, args.block)  # captures its block graph
+        # captures its block graphs: with a volume move's trial appended (the
+        # first block, a move being due after it) and without (the last)
+        sim.run(2 * args.block, args.block)
         if single_steps % args.block:
             sim.run(single_steps % args.block, args.block)
         t0 = time.perf_counter()
