@@ -85,6 +85,37 @@ def test_npt_run_moves_the_volume():
     assert np.isfinite(e["potential"]) and 240 < e["temperature"] < 400
 
 
+def test_a_move_appended_to_its_block_equals_the_stand_alone_move():
+    # rmd_run appends the trial of the move due after a block to that block's
+    # launch sequence; a run that stops at the block boundary makes the same
+    # move on its own at the start of the next run: every move of the two
+    # runs, and so the final state, must agree bit for bit
+    system = R.workload_system("ala-389w")
+    x0 = v0 = None
+    finals = []
+    for split in (False, True):
+        sim = R.GpuSimulation(system, temperature=300.0, pressure=1.0, barostat_interval=25, seed=8)
+        if x0 is None:
+            sim.minimise(max_iterations=30, force_tolerance=1000.0)
+            x0, v0 = sim.read_positions().astype(np.float64), R.maxwell_boltzmann(system, 300.0, 8)
+        sim.set_state(x0, v0, step=0)
+        if split:
+            for _ in range(8):
+                sim.run(25, 25)  # each call's move (from the second on) runs on its own
+        else:
+            sim.run(200, 25)  # moves appended to the blocks before them
+        d = sim.diagnostics()
+        finals.append((sim.read_state(), d["barostat_attempts"], d["barostat_accepted"], sim.energy()["potential"]))
+        sim.release()
+    (a, na, ka, ua), (b, nb, kb, ub) = finals
+    print("moves", na, ka, "/", nb, kb, "potential", ua, ub)
+    assert na == nb >= 7 and ka == kb and 0 < ka < na  # both outcomes taken
+    assert np.array_equal(a["positions"], b["positions"])
+    assert np.array_equal(a["velocities"], b["velocities"])
+    assert np.array_equal(a["box_vectors"], b["box_vectors"])
+    assert ua == ub
+
+
 def test_volume_move_reuses_the_block_energy_and_a_rejection_restores_the_state():
     system = R.workload_system("ala-389w")
     sim = R.GpuSimulation(system, temperature=300.0, pressure=1.0, barostat_interval=1000, seed=5)
