@@ -462,6 +462,16 @@ void buildUnits(rmd_ctx *c, std::vector<int4> &units, std::vector<double4> &geom
   }
 }
 
+// Whether the step's integrator builds the cell list (IntegratorParams::
+// cellsInIntegrator), from the current plan; the captured steps follow it.
+void refreshCellBuilder(rmd_ctx *c) {
+  const int builds = !c->batchMember && integratorBuildsCells(c->d, c->plan.cellCount) ? 1 : 0;
+  if (builds != c->ip.cellsInIntegrator) {
+    c->ip.cellsInIntegrator = builds;
+    invalidateGraphs(c);
+  }
+}
+
 void setupIntegrator(rmd_ctx *c, int suppliedNoise) {
   const double decay = std::exp(-c->friction * c->timestep);
   c->ip.halfStep = 0.5 * c->timestep;
@@ -475,6 +485,7 @@ void setupIntegrator(rmd_ctx *c, int suppliedNoise) {
   c->ip.clusterLdsBytes = (int)(c->clusterLdsSlots * CLUSTER_SLOT * sizeof(double));
   c->ip.starMax = c->starMax;
   c->ip.dispHalf = std::max(0.0, 0.5 * c->plan.skin - 1e-4);
+  refreshCellBuilder(c);
 }
 
 // ---- force evaluation ---------------------------------------------------
@@ -521,7 +532,10 @@ __global__ void setStepKernel(Scalars *sc, long long step) { sc->step = step; }
 // streams forked after the integrator and joined before anything that reads
 // the forces. Forces are int64 atomics and every energy has its own slot row,
 // so the results do not depend on the overlap.
-void enqueueForces(rmd_ctx *c, int terms, int energies, int slots[E_TERM_COUNT], bool molecular = true) {
+// `molecular`: energies with the molecular scaling term; `cellsBuilt`: the
+// step's integrator has built the cell list.
+void enqueueForces(rmd_ctx *c, int terms, int energies, int slots[E_TERM_COUNT], bool molecular = true,
+                   bool cellsBuilt = false) {
   Dev &d = c->d;
   for (int k = 0; k < E_TERM_COUNT; ++k) slots[k] = 0;
   const int bondedMask = RMD_TERM_BONDED | RMD_TERM_EXCEPTIONS | RMD_TERM_RESTRAINT;
@@ -533,7 +547,7 @@ void enqueueForces(rmd_ctx *c, int terms, int energies, int slots[E_TERM_COUNT],
   const bool joined = c->concurrent && (terms & RMD_TERM_DIRECT) && (terms & RMD_TERM_RECIPROCAL) &&
                       (terms & bondedMask) && forcesJoin(d, c->p, c->lanes);
   if (joined) {
-    launchRebuild(d, c->stream);
+    launchRebuild(d, c->stream, cellsBuilt);
     int pair = 0, bonded = 0;
     launchForcesJoined(d, c->p, (float)c->pme.alpha, rational, c->lanes, energies, c->pme.alpha, terms, c->stream,
                        &pair, &bonded);
@@ -559,7 +573,7 @@ void enqueueForces(rmd_ctx *c, int terms, int energies, int slots[E_TERM_COUNT],
       launchBonded(d, c->pme.alpha, terms, energies, bondedStream, &b);
       setBondedSlots(b);
     }
-    launchRebuild(d, listStream);
+    launchRebuild(d, listStream, cellsBuilt);
     if (terms & RMD_TERM_DIRECT) {
       const int b = launchNonbonded(d, (float)c->pme.alpha, rational, c->lanes, energies, listStream);
       slots[E_LJ_DIRECT] = slots[E_COUL_DIRECT] = slots[E_DSCALE_DIRECT] = b;
@@ -771,7 +785,7 @@ void enqueueBlock(rmd_ctx *c, int steps, bool trial) {
     const bool last = s == steps - 1;
     const int openBlocks = launchIntegrateOpen(c->d, c->ip, s > 0 ? 1 : 0, last ? 1 : 0, c->stream);
     int slots[E_TERM_COUNT];
-    enqueueForces(c, RMD_TERM_ALL, last ? 1 : 0, slots, molecular);
+    enqueueForces(c, RMD_TERM_ALL, last ? 1 : 0, slots, molecular, c->ip.cellsInIntegrator != 0);
     if (last) {
       std::memcpy(counts.c, slots, sizeof slots);
       counts.c[E_KINETIC] = 0;
@@ -895,6 +909,7 @@ bool updateBoxHost(rmd_ctx *c, const double *nb) {
     ++c->neighbourGrowths;
   }
   ewaldSelfEnergy(c->sys, c->pme.alpha, volumeOf(c->box), &c->selfEnergy, &c->netEnergy);
+  refreshCellBuilder(c);
   return replaced;
 }
 
@@ -2098,7 +2113,14 @@ int rmd_profile_kernel(rmd_ctx *c, int32_t kernel, int32_t reps, double *avg_ms)
                           0, c->stream);
           break;
         case 1: launchPme(c->d, c->p, 0, c->stream, nullptr); break;
-        case 2: launchIntegrateOpen(c->d, c->ip, 1, 0, c->stream); break;
+        case 2: {
+          // the integrator alone: repeated launches drift into rebuilds,
+          // whose cell list a step's integrator may build (cellsInIntegrator)
+          IntegratorParams ip = c->ip;
+          ip.cellsInIntegrator = 0;
+          launchIntegrateOpen(c->d, ip, 1, 0, c->stream);
+          break;
+        }
         case 3: launchBonded(c->d, c->pme.alpha, RMD_TERM_ALL, 0, c->stream, nullptr); break;
         default:
           launchForceRebuildFlag(c->d, c->stream);

@@ -16,6 +16,7 @@
 // kind. The closing B R of step n is folded into the start of step n+1's
 // launch, so a step is this kernel plus the force kernels.
 #include "kernels.h"
+#include "search.h"
 
 // Multiply-adds fuse only within one expression, so each body rounds the same
 // way wherever it is inlined (the single-system and the batched kernels).
@@ -806,7 +807,7 @@ __device__ __forceinline__ void integrateOpen(const Dev &d, const IntegratorPara
   if (early) blockSumToSlot(ke, &d.slots[E_KINETIC_EARLY * MAX_SLOTS + block]);
   // Last block: the displacement maximum is final; decide the rebuild and
   // advance the noise counter (integrate.js:456-464).
-  __shared__ bool last;
+  __shared__ bool last, build;
   __threadfence();
   if (threadIdx.x == 0) last = atomicAdd(&d.sc->ticket, 1u) == (unsigned int)blocks - 1;
   __syncthreads();
@@ -825,6 +826,16 @@ __device__ __forceinline__ void integrateOpen(const Dev &d, const IntegratorPara
     }
     d.sc->ticket = 0u;
     d.sc->step += 1;
+    build = rebuild && ip.cellsInIntegrator;
+  }
+  // The cell list of a rebuild, by the last workgroup (every workgroup's
+  // positions were released before its ticket; each thread acquires them).
+  if (ip.cellsInIntegrator && last) {
+    __syncthreads();
+    if (build) {
+      __threadfence();
+      cellsBuildFew<INTEGRATE_BLOCK>(d);
+    }
   }
 }
 
@@ -843,7 +854,9 @@ __global__ void __launch_bounds__(INTEGRATE_BLOCK)
   extern __shared__ double clusterLds[];
   const int2 t = task[blockIdx.x];
   if (t.x < 0) return;
-  integrateOpen<SM>(v[t.x].d, v[t.x].ip, closing, early, v[t.x].d.waterBlocks + t.y, v[t.x].openBlocks, clusterLds);
+  IntegratorParams ip = v[t.x].ip;
+  ip.cellsInIntegrator = 0;  // a batch builds its replicas' cell lists in a launch of its own
+  integrateOpen<SM>(v[t.x].d, ip, closing, early, v[t.x].d.waterBlocks + t.y, v[t.x].openBlocks, clusterLds);
 }
 
 // The closing kick + RATTLE of a block's last step.

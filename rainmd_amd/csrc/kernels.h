@@ -10,8 +10,10 @@ namespace rmd {
 
 // ---- cell list and neighbour list (neighbours.hip) ----
 // The five rebuild stages are launched every step and return at once unless
-// the integrator's last block (or the host) asked for a rebuild.
-void launchRebuild(const Dev &d, hipStream_t s);
+// the integrator's last block (or the host) asked for a rebuild; with
+// `cellsBuilt` a small system's cell build is left out (the step's integrator
+// builds its cell list: IntegratorParams::cellsInIntegrator).
+void launchRebuild(const Dev &d, hipStream_t s, bool cellsBuilt = false);
 void launchForceRebuildFlag(const Dev &d, hipStream_t s);
 
 // ---- nonbonded direct space (nonbonded.hip) ----
@@ -122,6 +124,11 @@ struct IntegratorParams {
   int suppliedNoise;
   int clusterLdsBytes;  // dynamic LDS per integrator block
   int starMax;          // the most constraints of any star unit (the star solver's width: 3 or 4)
+  // 1: the last workgroup of the step's integrator, which decides the
+  // rebuild, also builds the cell list when one is due (a small single system
+  // in one integrator launch whose plan has at most FEW_MAX_CELLS cells), and
+  // the step launches no cell build of its own (integratorBuildsCells)
+  int cellsInIntegrator;
 };
 // closing: apply the previous step's closing kick + RATTLE first.
 // early: record the in-step kinetic energy (E_KINETIC_EARLY).
@@ -142,6 +149,9 @@ constexpr int SPLIT_WATER_BLOCKS = 32;
 inline bool splitWater(const Dev &d) { return d.waterBlocks >= SPLIT_WATER_BLOCKS; }
 inline int singleOpenSlotCount(const Dev &d) { return splitWater(d) ? openSlotCount(d) : unitBlockCount(d); }
 int launchIntegrateOpen(const Dev &d, const IntegratorParams &ip, int closing, int early, hipStream_t s);
+// Whether a single system's integrator can build its cell list (one
+// integrator launch, the one-workgroup build, a plan of few cells).
+bool integratorBuildsCells(const Dev &d, int cellCount);
 int launchIntegrateClose(const Dev &d, const IntegratorParams &ip, int kinetic, hipStream_t s);
 int launchKinetic(const Dev &d, int term, hipStream_t s);
 // Sums each term's slot row in a fixed order; the counts travel by value so a

@@ -357,9 +357,13 @@ void launchRefreshSlotsBatch(const BatchView *v, const TaskTable &atoms256, hipS
   refreshSlotsBatchKernel<<<atoms256.blocks, 256, 0, s>>>(v, atoms256.task);
 }
 
-void launchRebuild(const Dev &d, hipStream_t s) {
+bool integratorBuildsCells(const Dev &d, int cellCount) {
+  return d.n <= SMALL_MAX_ATOMS && d.cellCapacity <= SMALL_MAX_CELLS && cellCount <= FEW_MAX_CELLS && !splitWater(d);
+}
+
+void launchRebuild(const Dev &d, hipStream_t s, bool cellsBuilt) {
   if (d.n <= SMALL_MAX_ATOMS && d.cellCapacity <= SMALL_MAX_CELLS) {
-    cellsBuildSmallKernel<<<1, SMALL_THREADS, 0, s>>>(d);
+    if (!cellsBuilt) cellsBuildSmallKernel<<<1, SMALL_THREADS, 0, s>>>(d);
   } else {
     const int blocks = (d.n + 255) / 256;
     cellsAssignKernel<<<blocks, 256, 0, s>>>(d);
