@@ -195,6 +195,7 @@ struct rmd_ctx {
   int pmeFolded = 0;   // whether force-only evaluations fold the x lines into the gathers (set at creation)
   int reachMax = 0;    // the cell plan's widest reach (0: CELL_REACH_MAX)
   int searchMode = 0;  // 0 by occupancy, 1 per atom, 2 per cell where the launch covers the cells
+  int searchByCellCaptured = -1;  // the search form the captured steps were built for
   int neighbourGrowths = 0;
   unsigned int worstRequired = 0;
   unsigned int rebuildCount = 0;  // Scalars::rebuilds at the last read-back
@@ -466,8 +467,12 @@ void buildUnits(rmd_ctx *c, std::vector<int4> &units, std::vector<double4> &geom
 // cellsInIntegrator), from the current plan; the captured steps follow it.
 void refreshCellBuilder(rmd_ctx *c) {
   const int builds = !c->batchMember && integratorBuildsCells(c->d, c->plan.cellCount) ? 1 : 0;
-  if (builds != c->ip.cellsInIntegrator) {
+  // (the search's form, one wave per atom or per cell, decides whether the
+  // joined force launch runs it: captured steps follow that too)
+  const int byCell = searchByCell(c->searchMode, c->d.n, c->plan.cellCount) ? 1 : 0;
+  if (builds != c->ip.cellsInIntegrator || byCell != c->searchByCellCaptured) {
     c->ip.cellsInIntegrator = builds;
+    c->searchByCellCaptured = byCell;
     invalidateGraphs(c);
   }
 }
@@ -547,10 +552,12 @@ void enqueueForces(rmd_ctx *c, int terms, int energies, int slots[E_TERM_COUNT],
   const bool joined = c->concurrent && (terms & RMD_TERM_DIRECT) && (terms & RMD_TERM_RECIPROCAL) &&
                       (terms & bondedMask) && forcesJoin(d, c->p, c->lanes);
   if (joined) {
-    launchRebuild(d, c->stream, cellsBuilt);
+    // the one-wave-per-atom search runs in the joined launch
+    const bool searchJoined = !searchByCell(c->searchMode, d.n, c->plan.cellCount);
+    launchRebuild(d, c->stream, cellsBuilt, searchJoined);
     int pair = 0, bonded = 0;
     launchForcesJoined(d, c->p, (float)c->pme.alpha, rational, c->lanes, energies, c->pme.alpha, terms, c->stream,
-                       &pair, &bonded);
+                       &pair, &bonded, searchJoined ? 1 : 0);
     slots[E_LJ_DIRECT] = slots[E_COUL_DIRECT] = slots[E_DSCALE_DIRECT] = pair;
     setBondedSlots(bonded);
     slots[E_RECIPROCAL] = slots[E_DSCALE_RECIPROCAL] = launchPmeAfterForward(d, c->p, energies, c->stream);

@@ -12,8 +12,9 @@ namespace rmd {
 // The five rebuild stages are launched every step and return at once unless
 // the integrator's last block (or the host) asked for a rebuild; with
 // `cellsBuilt` a small system's cell build is left out (the step's integrator
-// builds its cell list: IntegratorParams::cellsInIntegrator).
-void launchRebuild(const Dev &d, hipStream_t s, bool cellsBuilt = false);
+// builds its cell list: IntegratorParams::cellsInIntegrator), with `searched`
+// the list search (the joined force launch runs it).
+void launchRebuild(const Dev &d, hipStream_t s, bool cellsBuilt = false, bool searched = false);
 void launchForceRebuildFlag(const Dev &d, hipStream_t s);
 
 // ---- nonbonded direct space (nonbonded.hip) ----
@@ -93,10 +94,12 @@ int launchPme(const Dev &d, const PmeDev &p, int energies, hipStream_t s, int *s
 // A small system's direct space, bonded terms and PME forward planes in one
 // launch (where forcesJoin allows), with the energy slots each part used;
 // launchPmeAfterForward then runs the rest of the plane pipeline and returns
-// its slots.
+// its slots. With `search` (the one-wave-per-atom list search) a rebuild's
+// list search runs there too: each direct-space workgroup searches its own
+// rows before it computes them, and launchRebuild leaves its search out.
 bool forcesJoin(const Dev &d, const PmeDev &p, int lanes);
 void launchForcesJoined(const Dev &d, const PmeDev &p, float alpha, int rational, int lanes, int energies,
-                        double alphaD, int terms, hipStream_t s, int *pairSlots, int *bondedSlots);
+                        double alphaD, int terms, hipStream_t s, int *pairSlots, int *bondedSlots, int search = 0);
 int launchPmeAfterForward(const Dev &d, const PmeDev &p, int energies, hipStream_t s);
 void launchPmeStage(const Dev &d, const PmeDev &p, hipStream_t s, int stage);
 bool pmeTransformsFolded(const PmeDev &p);

@@ -361,7 +361,7 @@ bool integratorBuildsCells(const Dev &d, int cellCount) {
   return d.n <= SMALL_MAX_ATOMS && d.cellCapacity <= SMALL_MAX_CELLS && cellCount <= FEW_MAX_CELLS && !splitWater(d);
 }
 
-void launchRebuild(const Dev &d, hipStream_t s, bool cellsBuilt) {
+void launchRebuild(const Dev &d, hipStream_t s, bool cellsBuilt, bool searched) {
   if (d.n <= SMALL_MAX_ATOMS && d.cellCapacity <= SMALL_MAX_CELLS) {
     if (!cellsBuilt) cellsBuildSmallKernel<<<1, SMALL_THREADS, 0, s>>>(d);
   } else {
@@ -372,6 +372,7 @@ void launchRebuild(const Dev &d, hipStream_t s, bool cellsBuilt) {
     cellsSortShortKernel<<<(d.cellCapacity + SHORT_SORT_THREADS - 1) / SHORT_SORT_THREADS, SHORT_SORT_THREADS, 0, s>>>(d);
     cellsSortLongKernel<<<std::min(d.cellCapacity, 1024), 256, 0, s>>>(d);
   }
+  if (searched) return;
   if (d.rect)
     neighboursKernel<true><<<(d.n + 3) / 4, 256, 0, s>>>(d);
   else

@@ -14,6 +14,7 @@
 #include "bonded_terms.h"
 #include "kernels.h"
 #include "pme_plane.h"
+#include "search.h"
 
 // Multiply-adds fuse only within one expression, so each body rounds the same
 // way wherever it is inlined (the single-system and the batched kernels).
@@ -240,19 +241,30 @@ __global__ void __launch_bounds__(256) nonbondedKernel(Dev d, float alpha) {
 // x plane of the PME forward stage, each with the same body as its own
 // kernel. A captured step runs its force branches one after another, so
 // separate launches cost the sum of their durations, where one launch
-// overlaps them. (The cell build, the list search and the x lines stay
-// launches of their own: joining them behind in-launch waits between
-// workgroups - bounded waits, device-scope release and acquire across the
-// XCDs' caches - made the 2854-atom step 5 to 25 us slower, and its rebuild
-// twice as long, for dispatch gaps of about 1.6 us per dependent launch.)
+// overlaps them. With `search`, a rebuild's one-wave-per-atom list search
+// runs here as well, each direct-space workgroup searching the rows it then
+// computes (no workgroup reads another's rows). (The cell build and the x
+// lines stay out: joining them behind in-launch waits between workgroups -
+// bounded waits, device-scope release and acquire across the XCDs' caches -
+// made the 2854-atom step 5 to 25 us slower, and its rebuild twice as long,
+// for dispatch gaps of about 1.6 us per dependent launch.)
 template <int LANES, bool ENERGIES, bool LDS_TABLE, bool RECT, bool RATIONAL>
-__global__ void __launch_bounds__(PLANE_THREADS)
-    forcesJoinedKernel(Dev d, PmeDev p, float alpha, double alphaD, int terms, int pairBlocks, int planesFrom) {
+__global__ void __launch_bounds__(PLANE_THREADS) forcesJoinedKernel(Dev d, PmeDev p, float alpha, double alphaD,
+                                                                   int terms, int pairBlocks, int planesFrom,
+                                                                   int search) {
   extern __shared__ float2 lds[];
   const int b = blockIdx.x;
-  if (b < pairBlocks)
+  if (b < pairBlocks) {
+    if (search && d.sc->rebuild) {
+      // rows [b R, (b + 1) R), R = PLANE_THREADS / LANES, eight per pass:
+      // searchBlock's block k holds rows 8 k .. 8 k + 7, one per wave
+      constexpr int WAVES = PLANE_THREADS / 64, ROWS = PLANE_THREADS / LANES;
+      static_assert(ROWS % WAVES == 0, "whole passes of rows");
+      for (int pass = 0; pass < ROWS / WAVES; ++pass) searchBlock<RECT, WAVES>(d, b * (ROWS / WAVES) + pass);
+      __syncthreads();  // the workgroup's lists before its direct space
+    }
     nonbonded<LANES, ENERGIES, LDS_TABLE, RECT, RATIONAL>(d, alpha, b, lds);
-  else if (b < planesFrom)
+  } else if (b < planesFrom)
     bonded(d, alphaD, terms, ENERGIES ? 1 : 0, b - pairBlocks);
   else
     pmePlaneForward(d, p, nullptr, b - planesFrom, lds);
@@ -274,6 +286,7 @@ struct JoinedPart {
   const PmeDev *p;
   double alpha;
   int terms;
+  int search;
 };
 
 template <bool ENERGIES, bool LDS_TABLE, bool RECT, bool RATIONAL>
@@ -287,15 +300,15 @@ static void launchLanes(const Dev &d, float alpha, int lanes, int blocks, size_t
       switch (lanes) {
         case 16:
           forcesJoinedKernel<16, ENERGIES, LDS_TABLE, RECT, RATIONAL><<<all, PLANE_THREADS, lds, s>>>(
-              d, *j.p, alpha, j.alpha, j.terms, pair, planesFrom);
+              d, *j.p, alpha, j.alpha, j.terms, pair, planesFrom, j.search);
           return;
         case 32:
           forcesJoinedKernel<32, ENERGIES, LDS_TABLE, RECT, RATIONAL><<<all, PLANE_THREADS, lds, s>>>(
-              d, *j.p, alpha, j.alpha, j.terms, pair, planesFrom);
+              d, *j.p, alpha, j.alpha, j.terms, pair, planesFrom, j.search);
           return;
         case 64:
           forcesJoinedKernel<64, ENERGIES, LDS_TABLE, RECT, RATIONAL><<<all, PLANE_THREADS, lds, s>>>(
-              d, *j.p, alpha, j.alpha, j.terms, pair, planesFrom);
+              d, *j.p, alpha, j.alpha, j.terms, pair, planesFrom, j.search);
           return;
       }
     }
@@ -347,7 +360,7 @@ static void launchOne(bool batch, const Dev *d, const BatchView *v, const TaskTa
 
 static void dispatch(const Variant &k, bool batch, const Dev *d, const BatchView *v, const TaskTable *t, float alpha,
                      int lanes, int blocks, size_t lds, hipStream_t s,
-                     const JoinedPart &j = JoinedPart{nullptr, 0.0, 0}) {
+                     const JoinedPart &j = JoinedPart{nullptr, 0.0, 0, 0}) {
   const int code = (k.energies ? 8 : 0) | (k.lds ? 4 : 0) | (k.rect ? 2 : 0) | (k.rational ? 1 : 0);
   switch (code) {
 #define RMD_NB_CASE(C)                                                                                     \
@@ -382,10 +395,10 @@ bool forcesJoin(const Dev &d, const PmeDev &p, int lanes) {
 }
 
 void launchForcesJoined(const Dev &d, const PmeDev &p, float alpha, int rational, int lanes, int energies,
-                        double alphaD, int terms, hipStream_t s, int *pairSlots, int *bondedSlots) {
+                        double alphaD, int terms, hipStream_t s, int *pairSlots, int *bondedSlots, int search) {
   const Variant k{energies != 0, true, d.rect != 0, rational != 0};
   const size_t lds = std::max(tableLdsBytes(d.types), pmePlaneForwardLds(p));
-  dispatch(k, false, &d, nullptr, nullptr, alpha, lanes, 0, lds, s, JoinedPart{&p, alphaD, terms});
+  dispatch(k, false, &d, nullptr, nullptr, alpha, lanes, 0, lds, s, JoinedPart{&p, alphaD, terms, search});
   // energy slots as the 256-thread kernels number them
   *pairSlots = (int)(((long long)d.n * lanes + 255) / 256);
   *bondedSlots = (bondedThreads(d) + 255) / 256;
