@@ -154,20 +154,22 @@ static void reciprocal_box(const double *box, double *r) {
 }
 
 /* ---- FP32 distance test, this build's fixed operation order ---------------
- * d = p_i - p_j; image along c, then b, then a with s = floor(d*inv + 0.5);
+ * d = p_i - p_j; image along c, then b, then a with s = rint(d*inv), the
+ * nearest integer (the reference's floor(d/L + 0.5) differs only at
+ * half-box separations, beyond the list cutoff);
  * r2 = (dx*dx + dy*dy) + dz*dz. No fused multiply-add anywhere (the file is
  * compiled with -ffp-contract=off). Emulates the device's list test and the
  * reference's minimumImage order (webgpu/kernels/wgsl-common.js:81-88).     */
 float oracle_r2_f32(const float *pi, const float *pj, const float box9[9], const float inv3[3]) {
   float dx = pi[0] - pj[0], dy = pi[1] - pj[1], dz = pi[2] - pj[2];
-  float s2 = floorf(dz * inv3[2] + 0.5f);
+  float s2 = rintf(dz * inv3[2]);
   dx = dx - s2 * box9[6];
   dy = dy - s2 * box9[7];
   dz = dz - s2 * box9[8];
-  float s1 = floorf(dy * inv3[1] + 0.5f);
+  float s1 = rintf(dy * inv3[1]);
   dx = dx - s1 * box9[3];
   dy = dy - s1 * box9[4];
-  float s0 = floorf(dx * inv3[0] + 0.5f);
+  float s0 = rintf(dx * inv3[0]);
   dx = dx - s0 * box9[0];
   float xx = dx * dx, yy = dy * dy, zz = dz * dz;
   float sum = xx + yy;

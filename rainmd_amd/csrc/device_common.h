@@ -48,20 +48,22 @@ __device__ __forceinline__ void atomicAddFixed(long long *address, long long val
 
 // Minimum image and squared distance. The order of operations is fixed and
 // contraction is off, so oracle_r2_f32 reproduces every bit: image along c,
-// then b, then a, with s = floor(d * inv + 0.5) (the reference's kernel
-// order, webgpu/kernels/wgsl-common.js:81-88, with the division replaced by
-// a multiply by the f32 inverse width).
+// then b, then a (the reference's kernel order,
+// webgpu/kernels/wgsl-common.js:81-88), with s the nearest integer to
+// d * inv (one rounding instruction; the reference's floor(d / L + 0.5)
+// differs only where d / L is within rounding of a half-integer, where both
+// images lie half a box away, beyond the list cutoff).
 __device__ __forceinline__ float minimumImageR2(float &dx, float &dy, float &dz, const BoxF &b, float &s0, float &s1,
                                                 float &s2) {
 #pragma clang fp contract(off)
-  s2 = floorf(dz * b.inv[2] + 0.5f);
+  s2 = rintf(dz * b.inv[2]);
   dx = dx - s2 * b.c[0];
   dy = dy - s2 * b.c[1];
   dz = dz - s2 * b.c[2];
-  s1 = floorf(dy * b.inv[1] + 0.5f);
+  s1 = rintf(dy * b.inv[1]);
   dx = dx - s1 * b.b[0];
   dy = dy - s1 * b.b[1];
-  s0 = floorf(dx * b.inv[0] + 0.5f);
+  s0 = rintf(dx * b.inv[0]);
   dx = dx - s0 * b.a[0];
   float xx = dx * dx;
   float yy = dy * dy;
@@ -76,11 +78,11 @@ __device__ __forceinline__ float minimumImageR2(float &dx, float &dy, float &dz,
 __device__ __forceinline__ float minimumImageR2Rect(float &dx, float &dy, float &dz, const BoxF &b, float &s0,
                                                     float &s1, float &s2) {
 #pragma clang fp contract(off)
-  s2 = floorf(dz * b.inv[2] + 0.5f);
+  s2 = rintf(dz * b.inv[2]);
   dz = dz - s2 * b.c[2];
-  s1 = floorf(dy * b.inv[1] + 0.5f);
+  s1 = rintf(dy * b.inv[1]);
   dy = dy - s1 * b.b[1];
-  s0 = floorf(dx * b.inv[0] + 0.5f);
+  s0 = rintf(dx * b.inv[0]);
   dx = dx - s0 * b.a[0];
   float xx = dx * dx;
   float yy = dy * dy;
