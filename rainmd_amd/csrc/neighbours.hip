@@ -42,35 +42,72 @@ __global__ void cellsAssignKernel(Dev d) {
   if (!(isfinite(p.x) && isfinite(p.y) && isfinite(p.z))) atomicMax(&d.sc->nonFinite, 1u);
 }
 
-// Exclusive scan of the per-cell counts in one workgroup: each thread sums a
-// contiguous run of cells, the runs' sums are scanned across the workgroup
-// once, then each thread writes its run's starts. Leaves the counts zeroed
-// for the next rebuild and the scatter cursors at the cell starts.
-__global__ void __launch_bounds__(1024) cellsScanKernel(Dev d) {
+// Exclusive scan of the per-cell counts in one workgroup, SCAN_TILE cells at
+// a time. A tile's counts are loaded coalesced into LDS, every load of a
+// thread in flight together (a thread reading its own contiguous run from
+// global memory waited one full latency per cell). Each thread then sums its
+// run of SCAN_RUN cells in LDS; the runs' sums are scanned across the
+// workgroup; each thread writes its run's starts back to LDS; the tile's
+// starts go out coalesced. Leaves the counts zeroed for the next rebuild and
+// the scatter cursors at the cell starts.
+constexpr int SCAN_THREADS = 1024, SCAN_RUN = 8, SCAN_TILE = SCAN_THREADS * SCAN_RUN;
+
+// LDS index of tile entry k, one pad word per 32 so the threads' runs start
+// in different banks
+__device__ __forceinline__ int scanPad(int k) { return k + (k >> 5); }
+
+__global__ void __launch_bounds__(SCAN_THREADS) cellsScanKernel(Dev d) {
   if (!d.sc->rebuild) return;
-  __shared__ int partial[1024];
+  __shared__ int tile[SCAN_TILE + SCAN_TILE / 32];
+  __shared__ int partial[SCAN_THREADS];
   const int total = d.box->cellTotal, t = threadIdx.x;
-  const int per = (total + 1023) / 1024;
-  const int c0 = min(total, t * per), c1 = min(total, c0 + per);
-  int run = 0;
-  for (int c = c0; c < c1; ++c) run += d.cellCount[c];
-  partial[t] = run;
-  __syncthreads();
-  for (int offset = 1; offset < 1024; offset <<= 1) {
-    const int add = t >= offset ? partial[t - offset] : 0;
+  int carry = 0;
+  for (int base = 0; base < total; base += SCAN_TILE) {
+    const int here = min(SCAN_TILE, total - base);
+    int v[SCAN_RUN];
+#pragma unroll
+    for (int u = 0; u < SCAN_RUN; ++u) {
+      const int k = t + SCAN_THREADS * u;
+      v[u] = k < here ? d.cellCount[base + k] : 0;
+    }
+#pragma unroll
+    for (int u = 0; u < SCAN_RUN; ++u) tile[scanPad(t + SCAN_THREADS * u)] = v[u];
     __syncthreads();
-    partial[t] += add;
+    int run = 0;
+#pragma unroll
+    for (int u = 0; u < SCAN_RUN; ++u) run += tile[scanPad(t * SCAN_RUN + u)];
+    partial[t] = run;
     __syncthreads();
+    for (int offset = 1; offset < SCAN_THREADS; offset <<= 1) {
+      const int add = t >= offset ? partial[t - offset] : 0;
+      __syncthreads();
+      partial[t] += add;
+      __syncthreads();
+    }
+    const int tileTotal = partial[SCAN_THREADS - 1];
+    int at = carry + partial[t] - run;
+#pragma unroll
+    for (int u = 0; u < SCAN_RUN; ++u) {
+      const int k = scanPad(t * SCAN_RUN + u);
+      const int count = tile[k];
+      tile[k] = at;
+      at += count;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int u = 0; u < SCAN_RUN; ++u) {
+      const int k = t + SCAN_THREADS * u;
+      if (k < here) {
+        const int start = tile[scanPad(k)];
+        d.cellStart[base + k] = start;
+        d.cellCursor[base + k] = start;
+        d.cellCount[base + k] = 0;
+      }
+    }
+    carry += tileTotal;
+    __syncthreads();  // the next tile reuses tile and partial
   }
-  int at = partial[t] - run;
-  for (int c = c0; c < c1; ++c) {
-    const int count = d.cellCount[c];
-    d.cellStart[c] = at;
-    d.cellCursor[c] = at;
-    d.cellCount[c] = 0;
-    at += count;
-  }
-  if (t == 1023) d.cellStart[total] = partial[1023];
+  if (t == 0) d.cellStart[total] = carry;
 }
 
 __global__ void cellsScatterKernel(Dev d) {
@@ -367,7 +404,7 @@ void launchRebuild(const Dev &d, hipStream_t s, bool cellsBuilt, bool searched) 
   } else {
     const int blocks = (d.n + 255) / 256;
     cellsAssignKernel<<<blocks, 256, 0, s>>>(d);
-    cellsScanKernel<<<1, 1024, 0, s>>>(d);
+    cellsScanKernel<<<1, SCAN_THREADS, 0, s>>>(d);
     cellsScatterKernel<<<blocks, 256, 0, s>>>(d);
     cellsSortShortKernel<<<(d.cellCapacity + SHORT_SORT_THREADS - 1) / SHORT_SORT_THREADS, SHORT_SORT_THREADS, 0, s>>>(d);
     cellsSortLongKernel<<<std::min(d.cellCapacity, 1024), 256, 0, s>>>(d);
