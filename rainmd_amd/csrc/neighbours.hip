@@ -137,8 +137,10 @@ __global__ void __launch_bounds__(SHORT_SORT_THREADS) cellsSortShortKernel(Dev d
   const int begin = d.cellStart[c], len = d.cellStart[c + 1] - begin;
   if (len > SHORT_CELL) return;
   int *k = keys[threadIdx.x];
-  for (int a = 0; a < len; ++a) {
-    const int v = d.cellAtoms[begin + a];
+  // the cell's atoms loaded together, then sorted in LDS
+  for (int a = 0; a < len; ++a) k[a] = d.cellAtoms[begin + a];
+  for (int a = 1; a < len; ++a) {
+    const int v = k[a];
     int m = a - 1;
     while (m >= 0 && k[m] > v) {
       k[m + 1] = k[m];
@@ -146,7 +148,23 @@ __global__ void __launch_bounds__(SHORT_SORT_THREADS) cellsSortShortKernel(Dev d
     }
     k[m + 1] = v;
   }
-  for (int a = 0; a < len; ++a) writeSlot(d, begin + a, k[a]);
+  // positions and types SLOT_BATCH at a time, loaded before any slot is
+  // written (a slot's stores would otherwise wait on its own loads, one
+  // round trip per atom)
+  constexpr int SLOT_BATCH = 4;
+  for (int a = 0; a < len; a += SLOT_BATCH) {
+    float4 p[SLOT_BATCH];
+    int type[SLOT_BATCH];
+#pragma unroll
+    for (int u = 0; u < SLOT_BATCH; ++u) {
+      const int atom = k[min(a + u, len - 1)];
+      p[u] = d.pos[atom];
+      type[u] = d.type[atom];
+    }
+#pragma unroll
+    for (int u = 0; u < SLOT_BATCH; ++u)
+      if (a + u < len) writeSlotLoaded(d, begin + a + u, k[a + u], p[u], type[u]);
+  }
 }
 
 __global__ void __launch_bounds__(256) cellsSortLongKernel(Dev d) {

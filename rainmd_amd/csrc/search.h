@@ -33,13 +33,16 @@ __device__ __forceinline__ int cellIndex(float x, float y, float z, const BoxF &
 // Slot k of the cell order gets atom `atom`: its rebuild-time position with
 // the atom index (for the search), its current position and charge, its type,
 // and the inverse map.
-__device__ __forceinline__ void writeSlot(const Dev &d, int k, int atom) {
-  const float4 p = d.pos[atom];
+__device__ __forceinline__ void writeSlotLoaded(const Dev &d, int k, int atom, float4 p, int type) {
   d.cellAtoms[k] = atom;
   d.cellPos[k] = make_float4(p.x, p.y, p.z, __int_as_float(atom));
   d.slotPos[k] = p;
-  d.slotType[k] = d.type[atom];
+  d.slotType[k] = type;
   d.slotOf[atom] = k;
+}
+
+__device__ __forceinline__ void writeSlot(const Dev &d, int k, int atom) {
+  writeSlotLoaded(d, k, atom, d.pos[atom], d.type[atom]);
 }
 
 // The cell list of a system whose plan has at most FEW_MAX_CELLS cells, by
